@@ -1,0 +1,157 @@
+/*
+ * cans_gpu.h -- C-ABI of libcans_gpu.so, the MI355X (gfx950) implementation
+ * of the CANS MHD time step.
+ *
+ * Every entry point takes plain pointers, sizes and scalars.  Arrays are FP64,
+ * Fortran column-major with the ghost layers included, exactly as the CANS
+ * routines receive them: element (i,j,k) (1-based) is at
+ * (i-1) + ix*((j-1) + jx*(k-1)).  `stream` is a hipStream_t (NULL = the
+ * default stream).  Return value: 0 on success, CANS_ERR_* otherwise, with a
+ * message from cans_gpu_last_error().
+ *
+ * Citations are to the reference routine each entry point replaces
+ * (paths relative to cans_beta_vary/).  The Fortran drop-in libraries
+ * libcans{1,2,3}d_gpu.so bind these entry points with ISO_C_BINDING and
+ * export the reference's own subroutine names (mlw_m, cfl_m, bdperx, ...).
+ */
+#ifndef CANS_GPU_H
+#define CANS_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CANS_ERR_ARG 10001 /* bad argument (abort class, >= 10000 as merr) */
+#define CANS_ERR_HIP 10002 /* HIP runtime error */
+
+/* boundary kinds: the CANS bc/ routines */
+#define CANS_BD_FRE 0  /* bdfre{x,y,z}: cans1d/bc/bdfrex.f:30-40  zero gradient        */
+#define CANS_BD_SPP 1  /* bdspp{x,y,z}: cans1d/bc/bdsppx.f:32-42  mirror at face, +    */
+#define CANS_BD_SPN 2  /* bdspn{x,y,z}: cans1d/bc/bdspnx.f        mirror at face, -    */
+#define CANS_BD_SMP 3  /* bdsmp{x,y,z}: cans1d/bc/bdsmpx.f        mirror at centre, +  */
+#define CANS_BD_SMN 4  /* bdsmn{x,y,z}: cans1d/bc/bdsmnx.f:33-45  mirror at centre, -, wall 0 */
+#define CANS_BD_CNS 5  /* bdcns{x,y,z}: cans1d/bc/bdcnsx.f:31-41  constant q0          */
+#define CANS_BD_FRD 6  /* bdfrd{x,y,z}: cans1d/bc/bdfrdx.f:31-43  linear extrapolation */
+#define CANS_BD_PER 7  /* bdper{x,y,z}: cans1d/bc/bdperx.f:31-40  periodic, both sides */
+#define CANS_BD_NONE (-1)
+
+const char *cans_gpu_last_error(void);
+int cans_gpu_init(int device);
+int cans_gpu_sync(void *stream);
+
+/* ---------------------------------------------------------------- 1D
+ * cans1d/hdmlw/mlw_m.f:2 mlw_m(ro,pr,vx,vy,by,bx,bxm,dt,qav,gm,dx,dxm,ix)
+ * (fuses mlwhalf.f, mlwfull.f, mlwartv.f); device arrays, updated in place
+ * on i=2..ix-1. */
+int cans1d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *by,
+                 const double *bx, const double *bxm, double dt, double qav, double gm,
+                 const double *dx, const double *dxm, int ix, void *stream);
+/* cans1d/hdroe/roe_m.f:2 roe_m(ro,pr,vx,vy,by,bx,bxm,dt,gm,dx,ix)
+ * (fuses roeflux_m.f, tvdminmod.f); updated in place on i=3..ix-2. */
+int cans1d_roe_m(double *ro, double *pr, double *vx, double *vy, double *by,
+                 const double *bx, const double *bxm, double dt, double gm,
+                 const double *dx, int ix, void *stream);
+/* cans1d/common/cfl_m.f:2 cfl_m(dt,safety,dtmin,merr,gm,bx,ro,pr,vx,vy,by,dx,ix);
+ * dt and merr are host pointers (the call synchronises the stream). */
+int cans1d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                 const double *bx, const double *ro, const double *pr, const double *vx,
+                 const double *vy, const double *by, const double *dx, int ix, void *stream);
+/* cans1d/bc/bd*x.f: kind = CANS_BD_*; mbnd = side (0 low, 1 high), or for
+ * CANS_BD_PER margin0 with margin = margin1; q0 for bdcnsx, dxm for bdfrdx. */
+int cans1d_bnd(int kind, int mbnd, int margin, double *qq, double q0, const double *dxm,
+               int ix, void *stream);
+
+/* ---------------------------------------------------------------- 2D
+ * cans2d/hdmlw/mlw_m.f:2-3 mlw_m(ro,pr,vx,vy,bx,by,az,dt,qav,gm,dx,dxm,ix,dy,dym,jx)
+ * (fuses mlwhalf.f, mlwfull.f, mlwartv.f, mlwsrch.f, mlwsrcf.f); updated in
+ * place on i=2..ix-1, j=2..jx-1. */
+int cans2d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *bx, double *by,
+                 double *az, double dt, double qav, double gm, const double *dx,
+                 const double *dxm, int ix, const double *dy, const double *dym, int jx,
+                 void *stream);
+/* cans2d/hdroe/roe_m.f:2 roe_m(ro,pr,vx,vy,bx,by,az,dt,gm,dx,ix,dy,jx);
+ * updated in place on i=3..ix-2, j=3..jx-2. */
+int cans2d_roe_m(double *ro, double *pr, double *vx, double *vy, double *bx, double *by,
+                 double *az, double dt, double gm, const double *dx, int ix,
+                 const double *dy, int jx, void *stream);
+/* cans2d/common/cfl_m.f:2-3 cfl_m(dt,safety,dtmin,merr,gm,ro,pr,vx,vy,bx,by,dx,ix,dy,jx) */
+int cans2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                 const double *ro, const double *pr, const double *vx, const double *vy,
+                 const double *bx, const double *by, const double *dx, int ix,
+                 const double *dy, int jx, void *stream);
+/* cans2d/bc/bd*{x,y}.f; dir 0 = x, 1 = y */
+int cans2d_bnd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
+               const double *dm, int ix, int jx, void *stream);
+
+/* ---------------------------------------------------------------- 3D
+ * cans3d/hdmlw/mlw_m.f:2-3
+ * mlw_m(ro,pr,vx,vy,vz,bx,by,bz,dt,qav,gm,dx,dxm,ix,dy,dym,jx,dz,dzm,kx) */
+int cans3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx,
+                 double *by, double *bz, double dt, double qav, double gm,
+                 const double *dx, const double *dxm, int ix, const double *dy,
+                 const double *dym, int jx, const double *dz, const double *dzm, int kx,
+                 void *stream);
+/* cans3d/hdroe/roe_m.f:2 roe_m(ro,pr,vx,vy,vz,bx,by,bz,dt,gm,dx,ix,dy,jx,dz,kx) */
+int cans3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx,
+                 double *by, double *bz, double dt, double gm, const double *dx, int ix,
+                 const double *dy, int jx, const double *dz, int kx, void *stream);
+/* cans3d/common/cfl_m.f:2-3
+ * cfl_m(dt,safety,dtmin,merr,gm,ro,pr,vx,vy,vz,bx,by,bz,dx,ix,dy,jx,dz,kx) */
+int cans3d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                 const double *ro, const double *pr, const double *vx, const double *vy,
+                 const double *vz, const double *bx, const double *by, const double *bz,
+                 const double *dx, int ix, const double *dy, int jx, const double *dz, int kx,
+                 void *stream);
+/* cans3d/bc/bd*{x,y,z}.f; dir 0 = x, 1 = y, 2 = z */
+int cans3d_bnd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
+               const double *dm, int ix, int jx, int kx, void *stream);
+/* HLLD MUSCL-Hancock engine, cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f:2
+ * engine(ro,pr,vx,vy,vz,bx,by,bz,gm,mu,dt,dx,dxm,dy,dym,dz,dzm,ix,jx,kx,mfdim)
+ * (fuses hlld_mh.f, hlld_flux.f90, inp_const.f, inp_tvdminmod.f, adddfdx.f);
+ * dxm/dym/dzm are unused by the engine and not taken. */
+int cans3d_hlld_engine(double *ro, double *pr, double *vx, double *vy, double *vz,
+                       double *bx, double *by, double *bz, double gm, double mu, double dt,
+                       const double *dx, const double *dy, const double *dz, int ix, int jx,
+                       int kx, const int *mfdim, void *stream);
+
+/* ---------------------------------------------------------------- resident loop
+ * A device clock holds t, ns, merr and dt of the loop in
+ * cans2d/md_mhdshktb/hdmlw/main.f:150-206 so that steps run back to back with
+ * no host round trip; read it back with cans_clock_read().  Step n of a run
+ * reads its state from buffer set a (n even) or b (n odd) and writes the
+ * other; after N issued steps the current state is in a if nexec is even.
+ * No counterpart in the reference (which keeps these scalars on the host). */
+int cans_clock_create(void **clk);
+int cans_clock_destroy(void *clk);
+int cans_clock_reset(void *clk, double t0, double tend, int nstop, double dtmin, void *stream);
+int cans_clock_read(const void *clk, int nsteps_issued, double *t, int *ns, int *nexec,
+                    int *merr, double *dt_last, void *stream);
+/* nsteps of {cfl_m; t=t+dt; mlw_m; bnd} on 2D state (7 device pointers per
+ * set: ro,pr,vx,vy,bx,by,az).  bc == NULL: periodic box of width `margin`
+ * (ghost fill and next-step cfl fused into the engine kernel); otherwise
+ * bc[(dir*2+side)*7+field] gives a CANS_BD_* kind per face and field.
+ * metric_ws: device scratch of cans2d_metric_ws_size(ix,jx) doubles. */
+int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, int nsteps,
+                   double qav, double gm, double safety, const double *dx, const double *dxm,
+                   int ix, const double *dy, const double *dym, int jx, const int *bc,
+                   int margin, double *metric_ws, void *stream);
+int cans2d_metric_ws_size(int ix, int jx);
+
+/* ---------------------------------------------------------------- host arrays
+ * Device mirrors of host arrays, used by the Fortran drop-in wrappers.  In the
+ * default (strict) mode every wrapper call copies its arrays in and out; in
+ * resident mode (cans_gpu_set_resident(1)) arrays stay on the device and
+ * cans_gpu_sync_host() copies them all back (call it before output). */
+void *cans_gpu_map(const void *host, size_t bytes, int copy_in);
+int cans_gpu_unmap(void *host, int copy_out);
+int cans_gpu_set_resident(int on);
+int cans_gpu_sync_host(void);
+int cans_gpu_invalidate(const void *host);
+int cans_gpu_release_all(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,95 @@
+/*
+ * cans_oracle.h -- CPU restatement of the CANS MHD time step (test
+ * infrastructure only).
+ *
+ * This is the checker the GPU kernels are compared against.  It restates the
+ * reference Fortran routines in plain C with the reference's arithmetic order,
+ * its literal kinds (REAL*4 constants such as 1.0e-4 and 1.e20 promoted to
+ * double) and its MAX/MIN/SIGN semantics, and it keeps the reference's
+ * array-at-a-time structure (one full sweep per reference loop) so that it is
+ * a different program from the fused GPU kernels.  Each function cites the
+ * reference file:line it follows (paths relative to cans_beta_vary/).
+ *
+ * Conventions (same as the reference): arrays are Fortran column-major,
+ * dimension (ix[,jx[,kx]]) including `margin` ghost layers; index i of the
+ * reference is element i-1 here.  All arithmetic is IEEE binary64 with no
+ * FMA contraction (build with -ffp-contract=off).
+ *
+ * Only tests/, the smoke check and the CPU-baseline leg of bench.py may load
+ * this library.  It is never part of the product path.
+ */
+#ifndef CANS_ORACLE_H
+#define CANS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- 1D (cans1d) ---------------- */
+void or1d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *by,
+                const double *bx, const double *bxm, double dt, double qav,
+                double gm, const double *dx, const double *dxm, int ix);
+void or1d_roe_m(double *ro, double *pr, double *vx, double *vy, double *by,
+                const double *bx, const double *bxm, double dt, double gm,
+                const double *dx, int ix);
+void or1d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                const double *bx, const double *ro, const double *pr,
+                const double *vx, const double *vy, const double *by,
+                const double *dx, int ix, int *imin);
+/* kind: 0 fre, 1 spp, 2 spn, 3 smp, 4 smn, 5 cns (q0), 6 frd (dxm) */
+void or1d_bd(int kind, int mbnd, int margin, double *qq, double q0,
+             const double *dxm, int ix);
+void or1d_bdperx(int margin0, int margin1, double *qq, int ix);
+void or1d_grdrdy(double *dx, double *xm, double *dxm, const double *x, int ix);
+
+/* ---------------- 2D (cans2d) ---------------- */
+void or2d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *bx,
+                double *by, double *az, double dt, double qav, double gm,
+                const double *dx, const double *dxm, int ix,
+                const double *dy, const double *dym, int jx);
+void or2d_roe_m(double *ro, double *pr, double *vx, double *vy, double *bx,
+                double *by, double *az, double dt, double gm,
+                const double *dx, int ix, const double *dy, int jx);
+void or2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                const double *ro, const double *pr, const double *vx,
+                const double *vy, const double *bx, const double *by,
+                const double *dx, int ix, const double *dy, int jx,
+                int *imin, int *jmin);
+/* dir: 0 = x, 1 = y; kind as in or1d_bd, 7 = periodic (mbnd = margin0,
+ * margin = margin1) */
+void or2d_bd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
+             const double *dm, int ix, int jx);
+void or2d_bbtoaa(double *az, const double *bx, const double *by,
+                 const double *dxm, const double *dym, int margin, int ix,
+                 int jx);
+
+/* ---------------- 3D (cans3d) ---------------- */
+void or3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz,
+                double *bx, double *by, double *bz, double dt, double qav,
+                double gm, const double *dx, const double *dxm, int ix,
+                const double *dy, const double *dym, int jx,
+                const double *dz, const double *dzm, int kx);
+void or3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz,
+                double *bx, double *by, double *bz, double dt, double gm,
+                const double *dx, int ix, const double *dy, int jx,
+                const double *dz, int kx);
+void or3d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                const double *ro, const double *pr, const double *vx,
+                const double *vy, const double *vz, const double *bx,
+                const double *by, const double *bz, const double *dx, int ix,
+                const double *dy, int jx, const double *dz, int kx,
+                int *imin, int *jmin, int *kmin);
+/* dir: 0 x, 1 y, 2 z; kind as or2d_bd */
+void or3d_bd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
+             const double *dm, int ix, int jx, int kx);
+
+/* ------------- HLLD MUSCL engine (cansxx-nosupport, mu units) ------------- */
+void orh_engine(double *ro, double *pr, double *vx, double *vy, double *vz,
+                double *bx, double *by, double *bz, double gm, double mu,
+                double dt, const double *dx, const double *dy,
+                const double *dz, int ix, int jx, int kx, const int *mfdim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

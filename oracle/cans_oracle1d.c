@@ -1,0 +1,232 @@
+/*
+ * cans_oracle1d.c -- C restatement of the cans1d hot path (test
+ * infrastructure only; see cans_oracle.h).  Citations are to
+ * cans_beta_vary/cans1d/.
+ */
+#include "cans_oracle.h"
+#include "cans_oracle_util.h"
+
+/* hdmlw/mlwhalf.f:33-46 */
+static void mlwhalf1(const double *u, double *un, double *du, double dt,
+                     const double *f, const double *dxi, const double *dxim,
+                     int ix) {
+    for (int i = 2; i <= ix - 1; i++)
+        du[I1(i)] = du[I1(i)] - 0.5 * dt * (0.5 * dxi[I1(i)] * (f[I1(i + 1)] - f[I1(i - 1)]));
+    for (int i = 1; i <= ix - 1; i++) {
+        double uh = 0.5 * (u[I1(i + 1)] + u[I1(i)]);
+        double dfdx = dxim[I1(i)] * (f[I1(i + 1)] - f[I1(i)]);
+        un[I1(i)] = uh - dt * dfdx;
+    }
+}
+
+/* hdmlw/mlwfull.f:30-33 */
+static void mlwfull1(double *du, double dt, const double *f, const double *dxi,
+                     int ix) {
+    for (int i = 2; i <= ix - 1; i++) {
+        double dfdx = dxi[I1(i)] * (f[I1(i)] - f[I1(i - 1)]);
+        du[I1(i)] = du[I1(i)] - 0.5 * dt * dfdx;
+    }
+}
+
+/* hdmlw/mlwartv.f:31-34 */
+static void mlwartv1(const double *u, double *du, double dt, const double *qx,
+                     const double *dxi, const double *dxim, int ix) {
+    for (int i = 2; i <= ix - 1; i++)
+        du[I1(i)] = du[I1(i)] + dt * (dxi[I1(i)] * (qx[I1(i)] * dxim[I1(i)] * (u[I1(i + 1)] - u[I1(i)])
+                                                    - qx[I1(i - 1)] * dxim[I1(i - 1)] * (u[I1(i)] - u[I1(i - 1)])));
+}
+
+/* hdmlw/mlw_m.f:2-215 (modified Lax-Wendroff, 1D MHD, Gaussian units) */
+void or1d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *by,
+                const double *bx, const double *bxm, double dt, double qav,
+                double gm, const double *dx, const double *dxm, int ix) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0; /* mlw_m.f:55 */
+    const double pi4i = 1.0 / pi / 4.0; /* mlw_m.f:56 */
+    double *dxi = dalloc(ix), *dxim = dalloc(ix);
+    double *ee = dalloc(ix), *rx = dalloc(ix), *ry = dalloc(ix), *ez = dalloc(ix);
+    double *roh = dalloc(ix), *eeh = dalloc(ix), *rxh = dalloc(ix), *ryh = dalloc(ix), *byh = dalloc(ix);
+    double *prh = dalloc(ix), *vxh = dalloc(ix), *vyh = dalloc(ix), *ezh = dalloc(ix);
+    double *dro = dalloc(ix), *dee = dalloc(ix), *drx = dalloc(ix), *dry = dalloc(ix), *dby = dalloc(ix);
+    double *fx = dalloc(ix), *qx = dalloc(ix);
+
+    for (int i = 1; i <= ix; i++) { /* :60-63 */
+        dxim[I1(i)] = 1.0 / dxm[I1(i)];
+        dxi[I1(i)] = 1.0 / dx[I1(i)];
+    }
+    for (int i = 1; i <= ix; i++) { /* :68-74 */
+        dro[I1(i)] = 0.0; dee[I1(i)] = 0.0; drx[I1(i)] = 0.0; dry[I1(i)] = 0.0; dby[I1(i)] = 0.0;
+    }
+    for (int i = 1; i <= ix; i++) { /* :79-86, Bx not in the energy */
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]);
+        double bb = sq(by[I1(i)]);
+        ee[I1(i)] = pr[I1(i)] / (gm - 1.0) + 0.5 * ro[I1(i)] * vv + pi8i * bb;
+        rx[I1(i)] = ro[I1(i)] * vx[I1(i)];
+        ry[I1(i)] = ro[I1(i)] * vy[I1(i)];
+    }
+    for (int i = 1; i <= ix; i++) /* :87-89 */
+        ez[I1(i)] = -vx[I1(i)] * by[I1(i)] + vy[I1(i)] * bx[I1(i)];
+
+    /* half step, :94-125 */
+    for (int i = 1; i <= ix; i++) fx[I1(i)] = ro[I1(i)] * vx[I1(i)];
+    mlwhalf1(ro, roh, dro, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) {
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]);
+        double ep = pr[I1(i)] * gm / (gm - 1.0) + 0.5 * ro[I1(i)] * vv;
+        fx[I1(i)] = ep * vx[I1(i)] - by[I1(i)] * ez[I1(i)] * pi4i;
+    }
+    mlwhalf1(ee, eeh, dee, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) {
+        double bbm = sq(by[I1(i)]);
+        fx[I1(i)] = ro[I1(i)] * sq(vx[I1(i)]) + pr[I1(i)] + bbm * pi8i;
+    }
+    mlwhalf1(rx, rxh, drx, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++)
+        fx[I1(i)] = ro[I1(i)] * vx[I1(i)] * vy[I1(i)] - bx[I1(i)] * by[I1(i)] * pi4i;
+    mlwhalf1(ry, ryh, dry, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) fx[I1(i)] = -ez[I1(i)];
+    mlwhalf1(by, byh, dby, dt, fx, dxi, dxim, ix);
+
+    /* half-step primitives, :129-139 */
+    for (int i = 1; i <= ix - 1; i++) {
+        vxh[I1(i)] = rxh[I1(i)] / roh[I1(i)];
+        vyh[I1(i)] = ryh[I1(i)] / roh[I1(i)];
+        double vv = sq(vxh[I1(i)]) + sq(vyh[I1(i)]);
+        double bb = sq(byh[I1(i)]);
+        prh[I1(i)] = (gm - 1.0) * (eeh[I1(i)] - 0.5 * roh[I1(i)] * vv - pi8i * bb);
+    }
+    for (int i = 1; i <= ix; i++)
+        ezh[I1(i)] = -vxh[I1(i)] * byh[I1(i)] + vyh[I1(i)] * bxm[I1(i)];
+
+    /* full step, :144-175 */
+    for (int i = 1; i <= ix - 1; i++) fx[I1(i)] = roh[I1(i)] * vxh[I1(i)];
+    mlwfull1(dro, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) {
+        double vv = sq(vxh[I1(i)]) + sq(vyh[I1(i)]);
+        double eph = prh[I1(i)] * gm / (gm - 1.0) + 0.5 * roh[I1(i)] * vv;
+        fx[I1(i)] = eph * vxh[I1(i)] - byh[I1(i)] * ezh[I1(i)] * pi4i;
+    }
+    mlwfull1(dee, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) {
+        double bbm = sq(byh[I1(i)]);
+        fx[I1(i)] = roh[I1(i)] * sq(vxh[I1(i)]) + prh[I1(i)] + bbm * pi8i;
+    }
+    mlwfull1(drx, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++)
+        fx[I1(i)] = roh[I1(i)] * vxh[I1(i)] * vyh[I1(i)] - bxm[I1(i)] * byh[I1(i)] * pi4i;
+    mlwfull1(dry, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) fx[I1(i)] = -ezh[I1(i)];
+    mlwfull1(dby, dt, fx, dxi, ix);
+
+    /* artificial viscosity, :180-191 */
+    const double zero = 0.0;
+    for (int i = 1; i <= ix - 1; i++)
+        qx[I1(i)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I1(i + 1)] - vx[I1(i)]) - R4_1EM4);
+    mlwartv1(ro, dro, dt, qx, dxi, dxim, ix);
+    mlwartv1(ee, dee, dt, qx, dxi, dxim, ix);
+    mlwartv1(rx, drx, dt, qx, dxi, dxim, ix);
+    mlwartv1(ry, dry, dt, qx, dxi, dxim, ix);
+    mlwartv1(by, dby, dt, qx, dxi, dxim, ix);
+
+    /* update and primitive recovery, :195-212 */
+    for (int i = 2; i <= ix - 1; i++) {
+        ro[I1(i)] = ro[I1(i)] + dro[I1(i)];
+        rx[I1(i)] = rx[I1(i)] + drx[I1(i)];
+        ry[I1(i)] = ry[I1(i)] + dry[I1(i)];
+        by[I1(i)] = by[I1(i)] + dby[I1(i)];
+        ee[I1(i)] = ee[I1(i)] + dee[I1(i)];
+    }
+    for (int i = 2; i <= ix - 1; i++) {
+        vx[I1(i)] = rx[I1(i)] / ro[I1(i)];
+        vy[I1(i)] = ry[I1(i)] / ro[I1(i)];
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]);
+        double bb = sq(by[I1(i)]);
+        pr[I1(i)] = (gm - 1.0) * (ee[I1(i)] - 0.5 * ro[I1(i)] * vv - pi8i * bb);
+    }
+    free(dxi); free(dxim); free(ee); free(rx); free(ry); free(ez);
+    free(roh); free(eeh); free(rxh); free(ryh); free(byh);
+    free(prh); free(vxh); free(vyh); free(ezh);
+    free(dro); free(dee); free(drx); free(dry); free(dby); free(fx); free(qx);
+}
+
+/* common/cfl_m.f:42-71 (interior only, |vx| + fast speed) */
+void or1d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                const double *bx, const double *ro, const double *pr,
+                const double *vx, const double *vy, const double *by,
+                const double *dx, int ix, int *imin) {
+    (void)vy;
+    const double pi = acos(-1.0);
+    const double pi4i = 0.25 / pi;
+    double *dtq = dalloc(ix);
+    double d = R4_1E20;
+    int im = 0;
+    for (int i = 2; i <= ix - 1; i++) {
+        double onero = 1.0 / ro[I1(i)];
+        double ca2 = (bx[I1(i)] * bx[I1(i)] + by[I1(i)] * by[I1(i)]) * pi4i * onero;
+        double cs2 = gm * pr[I1(i)] * onero;
+        double dtcfl = dx[I1(i)] / (fabs(vx[I1(i)]) + sqrt(cs2 + ca2));
+        dtq[I1(i)] = safety * dtcfl;
+    }
+    for (int i = 2; i <= ix - 1; i++)
+        if (dtq[I1(i)] < d) { im = i; d = dtq[I1(i)]; }
+    *merr = (d < dtmin) ? 9001 : 0;
+    *dt = d;
+    if (imin) *imin = im;
+    free(dtq);
+}
+
+/* bc/bd{fre,spp,spn,smp,smn,cns,frd}x.f (side mbnd: 0 low, 1 high) */
+void or1d_bd(int kind, int mbnd, int margin, double *qq, double q0,
+             const double *dxm, int ix) {
+    if (mbnd == 0) {
+        int ibnd = 1 + margin;
+        double dqq = 0.0;
+        if (kind == 6) dqq = (qq[I1(ibnd + 1)] - qq[I1(ibnd)]) / dxm[I1(ibnd)]; /* bdfrdx.f */
+        for (int i = 1; i <= margin; i++) {
+            switch (kind) {
+            case 0: qq[I1(ibnd - i)] = qq[I1(ibnd)]; break;          /* bdfrex.f:30-33 */
+            case 1: qq[I1(ibnd - i)] = qq[I1(ibnd - 1 + i)]; break;  /* bdsppx.f */
+            case 2: qq[I1(ibnd - i)] = -qq[I1(ibnd - 1 + i)]; break; /* bdspnx.f */
+            case 3: qq[I1(ibnd - i)] = qq[I1(ibnd + i)]; break;      /* bdsmpx.f */
+            case 4: qq[I1(ibnd - i)] = -qq[I1(ibnd + i)]; break;     /* bdsmnx.f:33-38 */
+            case 5: qq[I1(ibnd - i)] = q0; break;                    /* bdcnsx.f */
+            case 6: qq[I1(ibnd - i)] = qq[I1(ibnd - i + 1)] - dqq * dxm[I1(ibnd - i)]; break;
+            }
+        }
+        if (kind == 4) qq[I1(ibnd)] = 0.0; /* bdsmnx.f: wall value */
+    } else {
+        int ibnd = ix - margin;
+        double dqq = 0.0;
+        if (kind == 6) dqq = (qq[I1(ibnd)] - qq[I1(ibnd - 1)]) / dxm[I1(ibnd - 1)];
+        for (int i = 1; i <= margin; i++) {
+            switch (kind) {
+            case 0: qq[I1(ibnd + i)] = qq[I1(ibnd)]; break;
+            case 1: qq[I1(ibnd + i)] = qq[I1(ibnd + 1 - i)]; break;
+            case 2: qq[I1(ibnd + i)] = -qq[I1(ibnd + 1 - i)]; break;
+            case 3: qq[I1(ibnd + i)] = qq[I1(ibnd - i)]; break;
+            case 4: qq[I1(ibnd + i)] = -qq[I1(ibnd - i)]; break;
+            case 5: qq[I1(ibnd + i)] = q0; break;
+            case 6: qq[I1(ibnd + i)] = qq[I1(ibnd + i - 1)] + dqq * dxm[I1(ibnd + i - 1)]; break;
+            }
+        }
+        if (kind == 4) qq[I1(ibnd)] = 0.0;
+    }
+}
+
+/* bc/bdperx.f:31-40 */
+void or1d_bdperx(int margin0, int margin1, double *qq, int ix) {
+    int ibnd0 = 1 + margin0, ibnd1 = ix - margin1;
+    for (int i = 1; i <= margin0; i++) qq[I1(ibnd0 - i)] = qq[I1(ibnd1 + 1 - i)];
+    for (int i = 1; i <= margin1; i++) qq[I1(ibnd1 + i)] = qq[I1(ibnd0 - 1 + i)];
+}
+
+/* common/grdrdy.f:28-42 */
+void or1d_grdrdy(double *dx, double *xm, double *dxm, const double *x, int ix) {
+    for (int i = 1; i <= ix - 1; i++) dxm[I1(i)] = x[I1(i + 1)] - x[I1(i)];
+    dxm[I1(ix)] = dxm[I1(ix - 1)];
+    for (int i = 2; i <= ix - 1; i++) dx[I1(i)] = (dxm[I1(i - 1)] + dxm[I1(i)]) / 2.0;
+    dx[I1(1)] = dx[I1(2)];
+    dx[I1(ix)] = dx[I1(ix - 1)];
+    for (int i = 1; i <= ix - 1; i++) xm[I1(i)] = (x[I1(i)] + x[I1(i + 1)]) / 2.0;
+    xm[I1(ix)] = xm[I1(ix - 1)] + dx[I1(ix - 1)];
+}

@@ -1,0 +1,297 @@
+/*
+ * cans_oracle2d.c -- C restatement of the cans2d hot path (test
+ * infrastructure only; see cans_oracle.h).  Citations are to
+ * cans_beta_vary/cans2d/.
+ */
+#include "cans_oracle.h"
+#include "cans_oracle_util.h"
+
+/* hdmlw/mlwhalf.f:36-52 (corner-staggered half step) */
+static void mlwhalf2(const double *u, double *un, double *du, double dt,
+                     const double *fx, const double *dxi, const double *dxim, int ix,
+                     const double *fy, const double *dyi, const double *dyim, int jx) {
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++)
+            du[I2(i, j)] = du[I2(i, j)] - 0.5 * dt * (0.5 * dxi[I1(i)] * (fx[I2(i + 1, j)] - fx[I2(i - 1, j)]))
+                                        - 0.5 * dt * (0.5 * dyi[I1(j)] * (fy[I2(i, j + 1)] - fy[I2(i, j - 1)]));
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++) {
+            double uh = 0.25 * (u[I2(i + 1, j)] + u[I2(i, j)] + u[I2(i + 1, j + 1)] + u[I2(i, j + 1)]);
+            double dfdx = dxim[I1(i)] * (fx[I2(i + 1, j)] - fx[I2(i, j)] + fx[I2(i + 1, j + 1)] - fx[I2(i, j + 1)]) / 2.0;
+            double dfdy = dyim[I1(j)] * (fy[I2(i, j + 1)] - fy[I2(i, j)] + fy[I2(i + 1, j + 1)] - fy[I2(i + 1, j)]) / 2.0;
+            un[I2(i, j)] = uh - dt * dfdx - dt * dfdy;
+        }
+}
+
+/* hdmlw/mlwfull.f:31-41 */
+static void mlwfull2(double *du, double dt, const double *fx, const double *dxi,
+                     const double *ux0, const double *ux1, int ix,
+                     const double *fy, const double *dyi, const double *uy0,
+                     const double *uy1, int jx) {
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            double dfdx = dxi[I1(i)] * (uy0[I1(j)] * (fx[I2(i, j - 1)] - fx[I2(i - 1, j - 1)])
+                                        + uy1[I1(j)] * (fx[I2(i, j)] - fx[I2(i - 1, j)]));
+            double dfdy = dyi[I1(j)] * (ux0[I1(i)] * (fy[I2(i - 1, j)] - fy[I2(i - 1, j - 1)])
+                                        + ux1[I1(i)] * (fy[I2(i, j)] - fy[I2(i, j - 1)]));
+            du[I2(i, j)] = du[I2(i, j)] - 0.5 * dt * dfdx - 0.5 * dt * dfdy;
+        }
+}
+
+/* hdmlw/mlwartv.f:33-41 */
+static void mlwartv2(const double *u, double *du, double dt, const double *qx,
+                     const double *dxi, const double *dxim, int ix,
+                     const double *qy, const double *dyi, const double *dyim, int jx) {
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++)
+            du[I2(i, j)] = du[I2(i, j)]
+                + dt * (dxi[I1(i)] * (qx[I2(i, j)] * dxim[I1(i)] * (u[I2(i + 1, j)] - u[I2(i, j)])
+                                      - qx[I2(i - 1, j)] * dxim[I1(i - 1)] * (u[I2(i, j)] - u[I2(i - 1, j)])))
+                + dt * (dyi[I1(j)] * (qy[I2(i, j)] * dyim[I1(j)] * (u[I2(i, j + 1)] - u[I2(i, j)])
+                                      - qy[I2(i, j - 1)] * dyim[I1(j - 1)] * (u[I2(i, j)] - u[I2(i, j - 1)])));
+}
+
+/* hdmlw/mlwsrch.f:32-47 */
+static void mlwsrch2(double *un, double *du, double dt, const double *s, int ix, int jx) {
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++)
+            du[I2(i, j)] = du[I2(i, j)] + 0.5 * dt * s[I2(i, j)];
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++) {
+            double sh = (s[I2(i + 1, j)] + s[I2(i, j)] + s[I2(i + 1, j + 1)] + s[I2(i, j + 1)]) / 4.0;
+            un[I2(i, j)] = un[I2(i, j)] + dt * sh;
+        }
+}
+
+/* hdmlw/mlwsrcf.f:33-40 */
+static void mlwsrcf2(double *du, double dt, const double *s, const double *ux0,
+                     const double *ux1, int ix, const double *uy0,
+                     const double *uy1, int jx) {
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            double sh = uy0[I1(j)] * (ux0[I1(i)] * s[I2(i - 1, j - 1)] + ux1[I1(i)] * s[I2(i, j - 1)])
+                      + uy1[I1(j)] * (ux0[I1(i)] * s[I2(i - 1, j)] + ux1[I1(i)] * s[I2(i, j)]);
+            du[I2(i, j)] = du[I2(i, j)] + 0.5 * dt * sh;
+        }
+}
+
+/* hdmlw/mlw_m.f:2-324 (2D MHD with vector potential Az) */
+void or2d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *bx,
+                double *by, double *az, double dt, double qav, double gm,
+                const double *dx, const double *dxm, int ix,
+                const double *dy, const double *dym, int jx) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0;
+    const double pi4i = 1.0 / pi / 4.0;
+    size_t n = (size_t)ix * jx;
+    double *dxi = dalloc(ix), *dxim = dalloc(ix), *ux0 = dalloc(ix), *ux1 = dalloc(ix);
+    double *dyi = dalloc(jx), *dyim = dalloc(jx), *uy0 = dalloc(jx), *uy1 = dalloc(jx);
+    double *ee = dalloc(n), *rx = dalloc(n), *ry = dalloc(n), *ez = dalloc(n);
+    double *roh = dalloc(n), *eeh = dalloc(n), *rxh = dalloc(n), *ryh = dalloc(n), *bxh = dalloc(n), *byh = dalloc(n);
+    double *prh = dalloc(n), *vxh = dalloc(n), *vyh = dalloc(n), *ezh = dalloc(n), *azh = dalloc(n);
+    double *dro = dalloc(n), *dee = dalloc(n), *drx = dalloc(n), *dry = dalloc(n), *dbx = dalloc(n), *dby = dalloc(n), *daz = dalloc(n);
+    double *fx = dalloc(n), *fy = dalloc(n), *qx = dalloc(n), *qy = dalloc(n), *ss = dalloc(n);
+
+    for (int i = 1; i <= ix; i++) { dxi[I1(i)] = 1.0 / dx[I1(i)]; dxim[I1(i)] = 1.0 / dxm[I1(i)]; } /* :68-71 */
+    for (int i = 2; i <= ix - 1; i++) {                                                             /* :72-75 */
+        ux1[I1(i)] = 0.5 * dxm[I1(i - 1)] / dx[I1(i)];
+        ux0[I1(i)] = 0.5 * dxm[I1(i)] / dx[I1(i)];
+    }
+    for (int j = 1; j <= jx; j++) { dyi[I1(j)] = 1.0 / dy[I1(j)]; dyim[I1(j)] = 1.0 / dym[I1(j)]; }
+    for (int j = 2; j <= jx - 1; j++) {
+        uy1[I1(j)] = 0.5 * dym[I1(j - 1)] / dy[I1(j)];
+        uy0[I1(j)] = 0.5 * dym[I1(j)] / dy[I1(j)];
+    }
+    for (size_t p = 0; p < n; p++) { dro[p] = 0.0; dee[p] = 0.0; drx[p] = 0.0; dry[p] = 0.0; dbx[p] = 0.0; dby[p] = 0.0; daz[p] = 0.0; }
+    for (int j = 1; j <= jx; j++)
+        for (int i = 1; i <= ix; i++) { /* :101-110 */
+            size_t p = I2(i, j);
+            double vv = sq(vx[p]) + sq(vy[p]);
+            double bb = sq(bx[p]) + sq(by[p]);
+            ee[p] = pr[p] / (gm - 1.0) + 0.5 * ro[p] * vv + pi8i * bb;
+            rx[p] = ro[p] * vx[p];
+            ry[p] = ro[p] * vy[p];
+        }
+    for (size_t p = 0; p < n; p++) ez[p] = -vx[p] * by[p] + vy[p] * bx[p];
+
+    /* half step, :121-184 */
+    for (size_t p = 0; p < n; p++) { fx[p] = ro[p] * vx[p]; fy[p] = ro[p] * vy[p]; }
+    mlwhalf2(ro, roh, dro, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) {
+        double vv = sq(vx[p]) + sq(vy[p]);
+        double ep = pr[p] * gm / (gm - 1.0) + 0.5 * ro[p] * vv;
+        fx[p] = ep * vx[p] - by[p] * ez[p] * pi4i;
+        fy[p] = ep * vy[p] + bx[p] * ez[p] * pi4i;
+    }
+    mlwhalf2(ee, eeh, dee, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) {
+        fx[p] = ro[p] * sq(vx[p]) + pr[p] + pi8i * (sq(by[p]) - sq(bx[p]));
+        fy[p] = ro[p] * vx[p] * vy[p] - pi4i * bx[p] * by[p];
+    }
+    mlwhalf2(rx, rxh, drx, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) {
+        fx[p] = ro[p] * vy[p] * vx[p] - pi4i * by[p] * bx[p];
+        fy[p] = ro[p] * sq(vy[p]) + pr[p] + pi8i * (sq(bx[p]) - sq(by[p]));
+    }
+    mlwhalf2(ry, ryh, dry, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) { fx[p] = 0.0; fy[p] = ez[p]; }
+    mlwhalf2(bx, bxh, dbx, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) { fx[p] = -ez[p]; fy[p] = 0.0; }
+    mlwhalf2(by, byh, dby, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) ss[p] = -ez[p];
+    mlwsrch2(azh, daz, dt, ss, ix, jx);
+
+    /* half-step primitives, :189-202 */
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            vxh[p] = rxh[p] / roh[p];
+            vyh[p] = ryh[p] / roh[p];
+            double vv = sq(vxh[p]) + sq(vyh[p]);
+            double bb = sq(bxh[p]) + sq(byh[p]);
+            prh[p] = (gm - 1.0) * (eeh[p] - 0.5 * roh[p] * vv - pi8i * bb);
+        }
+    for (size_t p = 0; p < n; p++) ezh[p] = -vxh[p] * byh[p] + vyh[p] * bxh[p];
+
+    /* full step, :207-270 */
+#define HLOOP for (int j = 1; j <= jx - 1; j++) for (int i = 1; i <= ix - 1; i++)
+    HLOOP { size_t p = I2(i, j); fx[p] = roh[p] * vxh[p]; fy[p] = roh[p] * vyh[p]; }
+    mlwfull2(dro, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP {
+        size_t p = I2(i, j);
+        double vv = sq(vxh[p]) + sq(vyh[p]);
+        double ep = prh[p] * gm / (gm - 1.0) + 0.5 * roh[p] * vv;
+        fx[p] = ep * vxh[p] - byh[p] * ezh[p] * pi4i;
+        fy[p] = ep * vyh[p] + bxh[p] * ezh[p] * pi4i;
+    }
+    mlwfull2(dee, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP {
+        size_t p = I2(i, j);
+        fx[p] = roh[p] * sq(vxh[p]) + prh[p] + pi8i * (sq(byh[p]) - sq(bxh[p]));
+        fy[p] = roh[p] * vxh[p] * vyh[p] - pi4i * bxh[p] * byh[p];
+    }
+    mlwfull2(drx, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP {
+        size_t p = I2(i, j);
+        fx[p] = roh[p] * vyh[p] * vxh[p] - pi4i * byh[p] * bxh[p];
+        fy[p] = roh[p] * sq(vyh[p]) + prh[p] + pi8i * (sq(bxh[p]) - sq(byh[p]));
+    }
+    mlwfull2(dry, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP { size_t p = I2(i, j); fx[p] = 0.0; fy[p] = ezh[p]; }
+    mlwfull2(dbx, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP { size_t p = I2(i, j); fx[p] = -ezh[p]; fy[p] = 0.0; }
+    mlwfull2(dby, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+#undef HLOOP
+    for (size_t p = 0; p < n; p++) ss[p] = -ezh[p];
+    mlwsrcf2(daz, dt, ss, ux0, ux1, ix, uy0, uy1, jx);
+
+    /* artificial viscosity, :276-295 */
+    const double zero = 0.0;
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++)
+            qx[I2(i, j)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I2(i + 1, j)] - vx[I2(i, j)]) - R4_1EM4);
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix; i++)
+            qy[I2(i, j)] = qav * dym[I1(j)] * fmaxF(zero, fabs(vy[I2(i, j + 1)] - vy[I2(i, j)]) - R4_1EM4);
+    mlwartv2(ro, dro, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(ee, dee, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(rx, drx, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(ry, dry, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(bx, dbx, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(by, dby, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+
+    /* update and primitive recovery, :299-321 */
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            ro[p] = ro[p] + dro[p];
+            ee[p] = ee[p] + dee[p];
+            rx[p] = rx[p] + drx[p];
+            ry[p] = ry[p] + dry[p];
+            bx[p] = bx[p] + dbx[p];
+            by[p] = by[p] + dby[p];
+            az[p] = az[p] + daz[p];
+        }
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            vx[p] = rx[p] / ro[p];
+            vy[p] = ry[p] / ro[p];
+            double vv = sq(vx[p]) + sq(vy[p]);
+            double bb = sq(bx[p]) + sq(by[p]);
+            pr[p] = (gm - 1.0) * (ee[p] - 0.5 * ro[p] * vv - pi8i * bb);
+        }
+    double *tofree[] = {dxi, dxim, ux0, ux1, dyi, dyim, uy0, uy1, ee, rx, ry, ez, roh, eeh, rxh, ryh, bxh, byh,
+                        prh, vxh, vyh, ezh, azh, dro, dee, drx, dry, dbx, dby, daz, fx, fy, qx, qy, ss};
+    for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
+}
+
+/* common/cfl_m.f:38-71 (all cells, ghosts included) */
+void or2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                const double *ro, const double *pr, const double *vx,
+                const double *vy, const double *bx, const double *by,
+                const double *dx, int ix, const double *dy, int jx,
+                int *imin, int *jmin) {
+    const double pi = acos(-1.0);
+    const double pi4i = 1.0 / pi / 4.0;
+    size_t n = (size_t)ix * jx;
+    double *dtq = dalloc(n);
+    double d = R4_1E20;
+    int im = 0, jm = 0;
+    for (int j = 1; j <= jx; j++)
+        for (int i = 1; i <= ix; i++) {
+            size_t p = I2(i, j);
+            double b2 = bx[p] * bx[p] + by[p] * by[p];
+            double ca2 = b2 * pi4i / ro[p];
+            double cs2 = gm * pr[p] / ro[p];
+            double v2 = vx[p] * vx[p] + vy[p] * vy[p];
+            double dtcfl = fminF(dx[I1(i)], dy[I1(j)]) / sqrt(v2 + cs2 + ca2);
+            dtq[p] = safety * dtcfl;
+        }
+    for (int j = 1; j <= jx; j++)
+        for (int i = 1; i <= ix; i++)
+            if (dtq[I2(i, j)] < d) { im = i; jm = j; d = dtq[I2(i, j)]; }
+    *merr = (d < dtmin) ? 9001 : 0;
+    *dt = d;
+    if (imin) *imin = im;
+    if (jmin) *jmin = jm;
+    free(dtq);
+}
+
+/* bc/bd*{x,y}.f (2D): the 1D index maps applied along x for every j, or
+ * along y for every i.  kind 7 = bdper{x,y} with (mbnd, margin) read as
+ * (margin0, margin1). */
+void or2d_bd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
+             const double *dm, int ix, int jx) {
+    int n = dir == 0 ? ix : jx;
+    int lines = dir == 0 ? jx : ix;
+    double *line = dalloc(n);
+    for (int l = 1; l <= lines; l++) {
+        for (int a = 1; a <= n; a++) line[I1(a)] = dir == 0 ? qq[I2(a, l)] : qq[I2(l, a)];
+        if (kind == 7) or1d_bdperx(mbnd, margin, line, n);
+        else or1d_bd(kind, mbnd, margin, line, q0, dm, n);
+        for (int a = 1; a <= n; a++) {
+            if (dir == 0) qq[I2(a, l)] = line[I1(a)];
+            else qq[I2(l, a)] = line[I1(a)];
+        }
+    }
+    free(line);
+}
+
+/* common/bbtoaa.f:28-49 (vector potential from B, trapezoid rule) */
+void or2d_bbtoaa(double *az, const double *bx, const double *by,
+                 const double *dxm, const double *dym, int margin, int ix,
+                 int jx) {
+    az[I2(margin + 1, margin + 1)] = 0.0;
+    int i = margin + 1;
+    for (int j = margin + 2; j <= jx; j++)
+        az[I2(i, j)] = az[I2(i, j - 1)] + 0.5 * (bx[I2(i, j - 1)] + bx[I2(i, j)]) * dym[I1(j - 1)];
+    for (int j = margin; j >= 1; j--)
+        az[I2(i, j)] = az[I2(i, j + 1)] - 0.5 * (bx[I2(i, j)] + bx[I2(i, j + 1)]) * dym[I1(j)];
+    for (int j = 1; j <= jx; j++) {
+        for (int ii = margin + 2; ii <= ix; ii++)
+            az[I2(ii, j)] = az[I2(ii - 1, j)] - 0.5 * (by[I2(ii - 1, j)] + by[I2(ii, j)]) * dxm[I1(ii - 1)];
+        for (int ii = margin; ii >= 1; ii--)
+            az[I2(ii, j)] = az[I2(ii + 1, j)] + 0.5 * (by[I2(ii, j)] + by[I2(ii + 1, j)]) * dxm[I1(ii)];
+    }
+}

@@ -1,0 +1,102 @@
+"""Build the native parts of utrip_amd in-tree.
+
+  libcans_gpu.so         HIP kernels + the C-ABI (include/cans_gpu.h), gfx950
+  libcans{1,2,3}d_gpu.so Fortran drop-in wrappers exporting the CANS routine
+                         names over the C-ABI (ISO_C_BINDING), amdflang
+
+Run ``python -m utrip_amd.build`` (or ``__graft_entry__.build()``).  hipcc
+cross-compiles for gfx950 without a GPU present.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+FSRC = PKG / "fortran"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+HIPCC = str(ROCM / "bin" / "hipcc")
+FLANG = str(ROCM / "bin" / "amdflang")
+
+HIP_SOURCES = sorted(CSRC.glob("*.hip"))
+# No FMA contraction anywhere (bitwise parity with the reference Fortran);
+# packed-FP32 instructions are disabled for the whole library.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    f"-I{ROOT / 'include'}",
+]
+
+LIB_GPU = PKG / "libcans_gpu.so"
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True, cwd=cwd)
+
+
+def build_gpu(jobs: int = 8) -> Path:
+    """Compile every .hip source to an object in parallel, then link."""
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    procs = []
+    objs = []
+    for src in HIP_SOURCES:
+        obj = objdir / (src.stem + ".o")
+        objs.append(obj)
+        cmd = [HIPCC, *[f for f in HIP_FLAGS if f != "-shared"], "-c", str(src), "-o", str(obj)]
+        print("+", " ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd)))
+        if len(procs) >= jobs:
+            s, p = procs.pop(0)
+            if p.wait() != 0:
+                raise RuntimeError(f"hipcc failed on {s}")
+    for s, p in procs:
+        if p.wait() != 0:
+            raise RuntimeError(f"hipcc failed on {s}")
+    tmp = LIB_GPU.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, LIB_GPU)
+    return LIB_GPU
+
+
+def build_fortran() -> list[Path]:
+    """Fortran drop-in layer: one library per dimensionality, like the
+    reference's libcans1d.a / libcans2d.a / libcans3d.a."""
+    out = []
+    if not FSRC.exists():
+        return out
+    objdir = PKG / "_build" / "fortran"
+    objdir.mkdir(parents=True, exist_ok=True)
+    common = FSRC / "cans_gpu_iface.f90"
+    for dim in ("1d", "2d", "3d"):
+        src = FSRC / f"cans{dim}_dropin.f90"
+        if not src.exists():
+            continue
+        lib = PKG / f"libcans{dim}_gpu.so"
+        _run([FLANG, "-O2", "-fPIC", "-shared", "-J", str(objdir), "-o", str(lib),
+              str(common), str(src), f"-L{PKG}", "-lcans_gpu", f"-Wl,-rpath,$ORIGIN"], cwd=objdir)
+        out.append(lib)
+    return out
+
+
+def build_all() -> None:
+    build_gpu()
+    build_fortran()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what == "gpu":
+        build_gpu()
+    elif what == "fortran":
+        build_fortran()
+    else:
+        build_all()

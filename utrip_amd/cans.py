@@ -1,0 +1,292 @@
+"""Python mirror of the CANS hot-path routines over libcans_gpu.so.
+
+Every function takes torch float64 device tensors laid out as the reference
+holds its arrays: a Fortran array ``a(ix,jx,kx)`` is a contiguous tensor of
+shape ``(kx, jx, ix)`` (1D: ``(ix,)``, 2D: ``(jx, ix)``), so element
+``(i,j,k)`` (1-based) is ``t[k-1, j-1, i-1]``.  Names and argument order follow
+the reference routines (``cans1d/hdmlw/mlw_m.f:2`` etc.); arrays are updated
+in place, scalars are Python floats.  Errors raise :class:`CansError`; the
+library must be built (``python -m utrip_amd.build``) -- there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import torch
+
+from . import LIB_GPU
+
+BD_FRE, BD_SPP, BD_SPN, BD_SMP, BD_SMN, BD_CNS, BD_FRD, BD_PER, BD_NONE = 0, 1, 2, 3, 4, 5, 6, 7, -1
+BND_NAMES = {"fre": BD_FRE, "spp": BD_SPP, "spn": BD_SPN, "smp": BD_SMP, "smn": BD_SMN,
+             "cns": BD_CNS, "frd": BD_FRD, "per": BD_PER}
+
+
+class CansError(RuntimeError):
+    pass
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load libcans_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_GPU.exists():
+            raise CansError(f"{LIB_GPU} is not built; run `python -m utrip_amd.build`")
+        _lib = C.CDLL(str(LIB_GPU))
+        _lib.cans_gpu_last_error.restype = C.c_char_p
+        _lib.cans_gpu_map.restype = C.c_void_p
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise CansError(f"libcans_gpu error {rc}: {lib().cans_gpu_last_error().decode()}")
+
+
+def _stream() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _d(x: float) -> C.c_double:
+    return C.c_double(float(x))
+
+
+def _ptr(t: Optional[torch.Tensor], shape=None, name="array") -> C.c_void_p:
+    if t is None:
+        return C.c_void_p(0)
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name}: dtype {t.dtype}, expected float64")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: tensor is not on a GPU")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor is not contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    return C.c_void_p(t.data_ptr())
+
+
+def _dims(t: torch.Tensor):
+    """(ix, jx, kx) of a reference-layout tensor."""
+    s = tuple(t.shape)
+    if len(s) == 1:
+        return s[0], 1, 1
+    if len(s) == 2:
+        return s[1], s[0], 1
+    return s[2], s[1], s[0]
+
+
+# ----------------------------------------------------------------------- 1D
+def mlw_m_1d(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm):
+    """cans1d/hdmlw/mlw_m.f:2  (in place on i=2..ix-1)."""
+    ix = ro.shape[0]
+    sh = (ix,)
+    _check(lib().cans1d_mlw_m(*[_ptr(t, sh, n) for t, n in
+                                zip((ro, pr, vx, vy, by, bx, bxm), ("ro", "pr", "vx", "vy", "by", "bx", "bxm"))],
+                              _d(dt), _d(qav), _d(gm), _ptr(dx, sh, "dx"), _ptr(dxm, sh, "dxm"), C.c_int(ix),
+                              _stream()))
+
+
+def roe_m_1d(ro, pr, vx, vy, by, bx, bxm, dt, gm, dx):
+    """cans1d/hdroe/roe_m.f:2  (in place on i=3..ix-2)."""
+    ix = ro.shape[0]
+    sh = (ix,)
+    _check(lib().cans1d_roe_m(*[_ptr(t, sh) for t in (ro, pr, vx, vy, by, bx, bxm)],
+                              _d(dt), _d(gm), _ptr(dx, sh), C.c_int(ix), _stream()))
+
+
+def cfl_m_1d(safety, dtmin, gm, bx, ro, pr, vx, vy, by, dx):
+    """cans1d/common/cfl_m.f:2  -> (dt, merr)."""
+    ix = ro.shape[0]
+    sh = (ix,)
+    dt = C.c_double()
+    merr = C.c_int()
+    _check(lib().cans1d_cfl_m(C.byref(dt), _d(safety), _d(dtmin), C.byref(merr), _d(gm),
+                              *[_ptr(t, sh) for t in (bx, ro, pr, vx, vy, by, dx)], C.c_int(ix), _stream()))
+    return dt.value, merr.value
+
+
+def bnd_1d(kind, mbnd, margin, qq, q0=0.0, dxm=None):
+    """cans1d/bc/bd*x.f  (kind: CANS_BD_* or a name such as 'fre', 'per')."""
+    kind = BND_NAMES.get(kind, kind)
+    ix = qq.shape[0]
+    _check(lib().cans1d_bnd(C.c_int(kind), C.c_int(mbnd), C.c_int(margin), _ptr(qq, (ix,)), _d(q0),
+                            _ptr(dxm, (ix,)) if dxm is not None else C.c_void_p(0), C.c_int(ix), _stream()))
+
+
+# ----------------------------------------------------------------------- 2D
+def mlw_m_2d(ro, pr, vx, vy, bx, by, az, dt, qav, gm, dx, dxm, dy, dym):
+    """cans2d/hdmlw/mlw_m.f:2-3  (in place on i=2..ix-1, j=2..jx-1)."""
+    jx, ix = ro.shape
+    sh = (jx, ix)
+    _check(lib().cans2d_mlw_m(*[_ptr(t, sh) for t in (ro, pr, vx, vy, bx, by, az)],
+                              _d(dt), _d(qav), _d(gm), _ptr(dx, (ix,)), _ptr(dxm, (ix,)), C.c_int(ix),
+                              _ptr(dy, (jx,)), _ptr(dym, (jx,)), C.c_int(jx), _stream()))
+
+
+def roe_m_2d(ro, pr, vx, vy, bx, by, az, dt, gm, dx, dy):
+    """cans2d/hdroe/roe_m.f:2  (in place on i=3..ix-2, j=3..jx-2)."""
+    jx, ix = ro.shape
+    sh = (jx, ix)
+    _check(lib().cans2d_roe_m(*[_ptr(t, sh) for t in (ro, pr, vx, vy, bx, by, az)],
+                              _d(dt), _d(gm), _ptr(dx, (ix,)), C.c_int(ix), _ptr(dy, (jx,)), C.c_int(jx),
+                              _stream()))
+
+
+def cfl_m_2d(safety, dtmin, gm, ro, pr, vx, vy, bx, by, dx, dy):
+    """cans2d/common/cfl_m.f:2-3  -> (dt, merr)."""
+    jx, ix = ro.shape
+    sh = (jx, ix)
+    dt = C.c_double()
+    merr = C.c_int()
+    _check(lib().cans2d_cfl_m(C.byref(dt), _d(safety), _d(dtmin), C.byref(merr), _d(gm),
+                              *[_ptr(t, sh) for t in (ro, pr, vx, vy, bx, by)],
+                              _ptr(dx, (ix,)), C.c_int(ix), _ptr(dy, (jx,)), C.c_int(jx), _stream()))
+    return dt.value, merr.value
+
+
+def bnd_2d(dir, kind, mbnd, margin, qq, q0=0.0, dm=None):
+    """cans2d/bc/bd*{x,y}.f  (dir 0 = x, 1 = y)."""
+    kind = BND_NAMES.get(kind, kind)
+    jx, ix = qq.shape
+    n = ix if dir == 0 else jx
+    _check(lib().cans2d_bnd(C.c_int(dir), C.c_int(kind), C.c_int(mbnd), C.c_int(margin), _ptr(qq, (jx, ix)),
+                            _d(q0), _ptr(dm, (n,)) if dm is not None else C.c_void_p(0),
+                            C.c_int(ix), C.c_int(jx), _stream()))
+
+
+# ----------------------------------------------------------------------- 3D
+def mlw_m_3d(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, dx, dxm, dy, dym, dz, dzm):
+    """cans3d/hdmlw/mlw_m.f:2-3  (in place on the interior)."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    _check(lib().cans3d_mlw_m(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                              _d(dt), _d(qav), _d(gm), _ptr(dx, (ix,)), _ptr(dxm, (ix,)), C.c_int(ix),
+                              _ptr(dy, (jx,)), _ptr(dym, (jx,)), C.c_int(jx),
+                              _ptr(dz, (kx,)), _ptr(dzm, (kx,)), C.c_int(kx), _stream()))
+
+
+def roe_m_3d(ro, pr, vx, vy, vz, bx, by, bz, dt, gm, dx, dy, dz):
+    """cans3d/hdroe/roe_m.f:2."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    _check(lib().cans3d_roe_m(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                              _d(dt), _d(gm), _ptr(dx, (ix,)), C.c_int(ix), _ptr(dy, (jx,)), C.c_int(jx),
+                              _ptr(dz, (kx,)), C.c_int(kx), _stream()))
+
+
+def cfl_m_3d(safety, dtmin, gm, ro, pr, vx, vy, vz, bx, by, bz, dx, dy, dz):
+    """cans3d/common/cfl_m.f:2-3  -> (dt, merr)."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    dt = C.c_double()
+    merr = C.c_int()
+    _check(lib().cans3d_cfl_m(C.byref(dt), _d(safety), _d(dtmin), C.byref(merr), _d(gm),
+                              *[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                              _ptr(dx, (ix,)), C.c_int(ix), _ptr(dy, (jx,)), C.c_int(jx),
+                              _ptr(dz, (kx,)), C.c_int(kx), _stream()))
+    return dt.value, merr.value
+
+
+def bnd_3d(dir, kind, mbnd, margin, qq, q0=0.0, dm=None):
+    """cans3d/bc/bd*{x,y,z}.f  (dir 0 = x, 1 = y, 2 = z)."""
+    kind = BND_NAMES.get(kind, kind)
+    kx, jx, ix = qq.shape
+    n = (ix, jx, kx)[dir]
+    _check(lib().cans3d_bnd(C.c_int(dir), C.c_int(kind), C.c_int(mbnd), C.c_int(margin),
+                            _ptr(qq, (kx, jx, ix)), _d(q0),
+                            _ptr(dm, (n,)) if dm is not None else C.c_void_p(0),
+                            C.c_int(ix), C.c_int(jx), C.c_int(kx), _stream()))
+
+
+def hlld_engine_3d(ro, pr, vx, vy, vz, bx, by, bz, gm, mu, dt, dx, dy, dz, mfdim=(1, 1, 1)):
+    """cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f:2  (HLLD MUSCL-Hancock)."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    mf = (C.c_int * 3)(*[int(m) for m in mfdim])
+    _check(lib().cans3d_hlld_engine(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                                    _d(gm), _d(mu), _d(dt), _ptr(dx, (ix,)), _ptr(dy, (jx,)), _ptr(dz, (kx,)),
+                                    C.c_int(ix), C.c_int(jx), C.c_int(kx), mf, _stream()))
+
+
+# ----------------------------------------------------------------------- resident loop
+class Clock:
+    """Device clock of a resident time loop (t, ns, merr, dt on the GPU)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().cans_clock_create(C.byref(h)))
+        self.handle = h
+        self.issued = 0
+
+    def reset(self, t0=0.0, tend=1e300, nstop=2**31 - 1, dtmin=1e-10):
+        _check(lib().cans_clock_reset(self.handle, _d(t0), _d(tend), C.c_int(nstop), _d(dtmin), _stream()))
+        self.issued = 0
+
+    def read(self):
+        t, dtl = C.c_double(), C.c_double()
+        ns, nexec, merr = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().cans_clock_read(self.handle, C.c_int(self.issued), C.byref(t), C.byref(ns), C.byref(nexec),
+                                     C.byref(merr), C.byref(dtl), _stream()))
+        return {"t": t.value, "ns": ns.value, "nexec": nexec.value, "merr": merr.value, "dt": dtl.value}
+
+    def __del__(self):
+        try:
+            if self.handle and _lib is not None:
+                _lib.cans_clock_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _ptr_array(ts: Sequence[torch.Tensor], shape):
+    arr = (C.c_void_p * len(ts))()
+    for n, t in enumerate(ts):
+        arr[n] = _ptr(t, shape).value
+    return arr
+
+
+class MlwRun2D:
+    """Resident 2D MLW time loop: {cfl_m; t=t+dt; mlw_m; bnd} back to back.
+
+    ``state`` is a list of 7 tensors (ro,pr,vx,vy,bx,by,az) with ghosts
+    filled; a second buffer set is allocated.  ``bc=None`` means a periodic
+    box (ghost fill and the next step's cfl fused into the engine kernel).
+    """
+
+    def __init__(self, state, dx, dxm, dy, dym, qav, gm, safety, margin=1, bc=None, clock=None):
+        self.a = list(state)
+        self.b = [torch.empty_like(t) for t in state]
+        self.jx, self.ix = state[0].shape
+        self.grid = (dx, dxm, dy, dym)
+        self.qav, self.gm, self.safety, self.margin = qav, gm, safety, margin
+        self.clock = clock or Clock()
+        n = lib().cans2d_metric_ws_size(self.ix, self.jx)
+        self.metric = torch.empty(n, dtype=torch.float64, device=state[0].device)
+        if bc is None:
+            self.bc = None
+        else:
+            self.bc = (C.c_int * 28)(*[int(k) for k in bc])
+        self._pa = _ptr_array(self.a, (self.jx, self.ix))
+        self._pb = _ptr_array(self.b, (self.jx, self.ix))
+
+    def reset(self, t0=0.0, tend=1e300, nstop=2**31 - 1, dtmin=1e-10):
+        self.clock.reset(t0, tend, nstop, dtmin)
+
+    def steps(self, n: int):
+        dx, dxm, dy, dym = self.grid
+        _check(lib().cans2d_mlw_run(self._pa, self._pb, self.clock.handle, C.c_int(self.clock.issued), C.c_int(n),
+                                    _d(self.qav), _d(self.gm), _d(self.safety),
+                                    _ptr(dx, (self.ix,)), _ptr(dxm, (self.ix,)), C.c_int(self.ix),
+                                    _ptr(dy, (self.jx,)), _ptr(dym, (self.jx,)), C.c_int(self.jx),
+                                    self.bc if self.bc is not None else None, C.c_int(self.margin),
+                                    _ptr(self.metric), _stream()))
+        self.clock.issued += n
+
+    def state(self):
+        """Current state (the buffer set the last executed step wrote)."""
+        info = self.clock.read()
+        return (self.a if info["nexec"] % 2 == 0 else self.b), info

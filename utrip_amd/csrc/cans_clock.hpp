@@ -1,0 +1,40 @@
+// cans_clock.hpp -- device side of the resident time-loop clock.
+//
+// The reference loop body (e.g. cans2d/md_mhdshktb/hdmlw/main.f:150-206) is
+//     ns = ns + 1; cfl_m -> dt; if (merr != 0) exit; t = t + dt;
+//     engine(dt); bnd; if (.not.(ns < nstop .and. t < tend)) exit
+// clock_enter() evaluates that control flow for step n on the device.
+#pragma once
+#include "cans_common.hpp"
+#include "cans_kernels.hpp"
+
+namespace cans {
+
+// Every block calls this; only the block leader (leader == true) writes the
+// next clock slot.  Returns whether the update of step n runs and its dt.
+CANS_DEV StepCtl clock_enter(Clock *c, int n, bool leader) {
+    const int s = n & 1, d = s ^ 1;
+    const bool live = !c->stopped[s] && c->ns[s] < c->nstop && c->t[s] < c->tend;
+    const double dt = dunkey(c->dtkey[n % 3]);
+    const bool bad = live && (dt < c->dtmin);
+    const bool run = live && !bad;
+    if (leader) {
+        c->ns[d] = live ? c->ns[s] + 1 : c->ns[s];
+        c->nexec[d] = run ? c->nexec[s] + 1 : c->nexec[s];
+        c->t[d] = run ? c->t[s] + dt : c->t[s];
+        c->dt_last[d] = run ? dt : c->dt_last[s];
+        c->merr[d] = bad ? 9001 : c->merr[s];
+        c->stopped[d] = run ? 0 : 1;
+        c->dtkey[(n + 2) % 3] = dkey(kR4_1e20);
+    }
+    return StepCtl{run, dt};
+}
+
+// Read-only view of the same decision, for kernels of step n other than the
+// one that carries the clock (the separate cfl pass, boundary passes).
+CANS_DEV bool clock_live(const Clock *c, int n) {
+    const int s = n & 1;
+    return !c->stopped[s] && c->ns[s] < c->nstop && c->t[s] < c->tend;
+}
+
+}  // namespace cans

@@ -1,0 +1,121 @@
+// cans_kernels.hpp -- launch interface between the C-ABI layer and the HIP
+// kernels.  Internal to the library; the public surface is include/cans_gpu.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cans {
+
+// Device-side clock of a resident time loop.  It holds the scalars a CANS
+// main program keeps on the host (t, ns, merr) so that a run of steps needs
+// no host round trip.  Step n (an index the host passes to every kernel of
+// that step) reads the slot n&1 of every scalar and its leader writes slot
+// (n+1)&1, so no block ever reads a value another block of the same launch
+// writes.  dt travels as order-preserving keys in three rotating slots:
+// step n uses dtkey[n%3], min-reduces the next dt into dtkey[(n+1)%3] and
+// re-arms dtkey[(n+2)%3] to 1.e20 (the reference's REAL*4 initial value).
+struct Clock {
+    unsigned long long dtkey[3];
+    double t[2];
+    double dt_last[2];
+    double tend;
+    double dtmin;
+    int ns[2];       // loop entries, as the reference's ns
+    int nexec[2];    // steps whose update ran
+    int stopped[2];
+    int merr[2];
+    int nstop;
+    int pad;
+};
+
+struct StepCtl {
+    bool run;   // this step updates the state
+    double dt;
+};
+
+// Metric coefficients precomputed once per grid axis; bitwise the same as
+// the reference computes them at the top of every engine call.
+struct Metric {
+    double *di;   // 1.0/d(i)
+    double *dim;  // 1.0/dm(i)
+    double *u0;   // 0.5*dm(i)/d(i)     (i = 2..n-1)
+    double *u1;   // 0.5*dm(i-1)/d(i)   (i = 2..n-1)
+};
+
+struct Mlw2dArgs {
+    const double *ro, *pr, *vx, *vy, *bx, *by, *az;
+    double *ro_o, *pr_o, *vx_o, *vy_o, *bx_o, *by_o, *az_o;
+    const double *dx, *dxm, *dy, *dym;
+    Metric mx, my;
+    double qav, gm, dt;
+    Clock *clk;        // non-null: dt and the run flag come from the clock
+    int step;          // step index n for the clock
+    int ix, jx;
+    int per_margin;    // > 0: also write the periodic ghost images (width)
+    double safety;     // > 0 with clk: fused cfl_m of the new state
+};
+
+struct Mlw1dArgs {
+    const double *ro, *pr, *vx, *vy, *by;
+    double *ro_o, *pr_o, *vx_o, *vy_o, *by_o;
+    const double *bx, *bxm, *dx, *dxm;
+    Metric mx;
+    double qav, gm, dt;
+    Clock *clk;
+    int step;
+    int ix;
+};
+
+// Boundary kinds (the CANS bc/ routines)
+enum BndKind : int {
+    BD_FRE = 0,  // bdfre*  zero gradient
+    BD_SPP = 1,  // bdspp*  mirror about the cell face, sign kept
+    BD_SPN = 2,  // bdspn*  mirror about the cell face, sign flipped
+    BD_SMP = 3,  // bdsmp*  mirror about the cell centre, sign kept
+    BD_SMN = 4,  // bdsmn*  mirror about the cell centre, sign flipped, wall 0
+    BD_CNS = 5,  // bdcns*  constant q0
+    BD_FRD = 6,  // bdfrd*  linear extrapolation with dxm
+    BD_PER = 7,  // bdper*  periodic, both sides (mbnd = margin0)
+    BD_NONE = -1,
+};
+
+constexpr int kMaxBndFields = 16;
+struct BndArgs {
+    double *q[kMaxBndFields];
+    double q0[kMaxBndFields];
+    int kind[kMaxBndFields];
+    int nf;
+    int dir;          // 0 x, 1 y, 2 z
+    int mbnd;         // side (0 low, 1 high); for BD_PER: margin0
+    int margin;       // for BD_PER: margin1
+    const double *dm; // dxm/dym/dzm, for BD_FRD
+    int ix, jx, kx;
+};
+
+struct CflArgs {
+    const double *ro, *pr, *vx, *vy, *vz, *bx, *by, *bz;
+    const double *dx, *dy, *dz;
+    double safety, gm;
+    int ix, jx, kx;
+    int dim;                    // 1, 2 or 3 (selects the reference's formula)
+    unsigned long long *key;    // order-preserving key of the min (pre-armed)
+    Clock *clk;                 // non-null: reduce into clk->dtkey[step%3] if the step runs
+    int step;
+};
+
+hipError_t launch_metric(const double *d, const double *dm, int n, Metric m, hipStream_t s);
+hipError_t launch_mlw2d(const Mlw2dArgs &a, hipStream_t s);
+hipError_t launch_mlw1d(const Mlw1dArgs &a, hipStream_t s);
+hipError_t launch_bnd(const BndArgs &a, hipStream_t s);
+hipError_t launch_copy_interior(double *const *dst, const double *const *src, int nf,
+                                int ix, int jx, int kx, int lo, hipStream_t s);
+hipError_t launch_cfl(const CflArgs &a, hipStream_t s);
+// first cell (reference loop order) whose dtq equals dt; loc[0..2] 1-based
+hipError_t launch_cfl_locate(const CflArgs &a, double dt, int *loc_dev, hipStream_t s);
+hipError_t launch_arm_key(unsigned long long *key, hipStream_t s);  // *key = key(1.e20)
+hipError_t launch_clock_reset(Clock *clk, double t0, double tend, double dtmin, int nstop,
+                              hipStream_t s);
+// a clock-only step (for paths whose update kernel does not carry the clock)
+hipError_t launch_clock_step(Clock *clk, int step, hipStream_t s);
+
+}  // namespace cans
