@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the CANS MHD time step on one MI355X.
+
+Workload (default): 2D Orszag-Tang vortex, 1024^2 interior cells, periodic
+box [0,2pi)^2, gamma = 5/3, CFL safety 0.4, artificial viscosity 3.0, FP64
+modified Lax-Wendroff (the cans2d mlw_m path).  One step is one pass of the
+CANS main loop over the whole grid: cfl_m -> t = t + dt -> mlw_m -> bnd
+(periodic ghost fill and the next step's cfl_m are fused into the engine
+kernel; dt, t and the stop test stay on the device).
+
+Printed: one JSON line with zone-updates/s (interior cells x steps / s), the
+roofline of the engine kernel (algorithmic bytes per launch / measured
+average launch duration, HIP events on the launch stream) and a CPU baseline:
+the reference Fortran routines compiled from source (oracle/_ref) run on the
+host for a bounded number of steps of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "zone-updates/s on 1 MI355X (FP64 plain vs FP32-mixed/batched), %HBM peak, adj/fwd"
+HBM_PEAK_GBS = 8000.0
+# compulsory HBM traffic of one 2D zone-update: 7 FP64 fields read + 7 written
+B_ALG_2D = 112
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2000)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--n", type=int, default=1024, help="interior cells per side")
+    p.add_argument("--cpu-steps", type=int, default=40, help="reference steps timed on the host")
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def traffic_from_profile(name: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
+            try:
+                return float(line.split("=")[1])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def cpu_baseline(pb, nsteps: int):
+    """The reference Fortran (oracle/_ref/libcans2d_ref.so, built from the
+    reference sources with amdflang -O2 -ffp-contract=off) on one host core:
+    cfl_m + mlw_m + bdperx/bdpery per step, 1 warm-up step, median step time."""
+    import ctypes as C
+
+    from oracle.oracle import REF_LIBS, Ref
+    if not REF_LIBS["2d"].exists():
+        return None
+    R = Ref("2d")
+    f = {k: v.copy(order="F") for k, v in pb.fields.items()}
+    g, pp = pb.grid, pb.params
+    ix, jx = f["ro"].shape
+    names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+
+    def step():
+        dt = C.c_double()
+        merr = C.c_int()
+        R.call("cfl_m", dt, pp["safety"], pp["dtmin"], merr, pp["gm"], *[f[k] for k in names[:6]], g["dx"], ix,
+               g["dy"], jx)
+        R.call("mlw_m", *[f[k] for k in names], dt.value, pp["qav"], pp["gm"], g["dx"], g["dxm"], ix, g["dy"],
+               g["dym"], jx)
+        for k in names:
+            R.call("bdperx", 1, 1, f[k], ix, jx)
+        for k in names:
+            R.call("bdpery", 1, 1, f[k], ix, jx)
+
+    step()  # warm-up
+    times = []
+    for _ in range(nsteps):
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    zones = (ix - 2) * (jx - 2)
+    return {"value": zones / med, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"{nsteps} timed steps (after 1 warm-up) of the same {ix - 2}^2 Orszag-Tang MLW loop, "
+                      f"median step {med:.3f} s; reference cfl_m/mlw_m/bdperx/bdpery compiled from source "
+                      f"with amdflang -O2 -ffp-contract=off, single thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from utrip_amd import cans, problems
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    pb = problems.orszag_tang_2d(n=n, margin=1)
+    names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+    g, pp = pb.grid, pb.params
+
+    def td(a):
+        return torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+
+    st = [td(pb.fields[k]) for k in names]
+    gd = {k: td(v) for k, v in g.items()}
+    run = cans.MlwRun2D(st, gd["dx"], gd["dxm"], gd["dy"], gd["dym"], pp["qav"], pp["gm"], pp["safety"])
+    run.reset(dtmin=pp["dtmin"])
+    mass0 = float(st[0][1:-1, 1:-1].sum())
+
+    run.steps(args.warmup)
+    torch.cuda.synchronize()
+
+    K = args.steps
+    ev = cans.EventSet(2 * K + 2)
+    torch.cuda.synchronize()
+    ev.record(2 * K)               # start of the timed region
+    run.steps(K, events=ev)
+    ev.record(2 * K + 1)           # end of the timed region
+    torch.cuda.synchronize()
+    total_ms = ev.elapsed_ms(2 * K, 2 * K + 1)
+    kern_ms = [ev.elapsed_ms(2 * s, 2 * s + 1) for s in range(K)]
+    avg_kern_s = statistics.fmean(kern_ms) / 1e3
+
+    cur, info = run.state()
+    assert info["merr"] == 0 and info["nexec"] == args.warmup + K, info
+    mass1 = float(cur[0][1:-1, 1:-1].sum())
+
+    zones = n * n
+    value = zones * K / (total_ms / 1e3)
+    achieved = B_ALG_2D * zones / avg_kern_s / 1e9
+    traffic = traffic_from_profile("mlw2d_kernel_ot1024_pmc.txt")
+
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        cpu = cpu_baseline(pb, args.cpu_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D Orszag-Tang vortex, {n}x{n} periodic, CANS mlw_m FP64 "
+                               f"(cfl_m + mlw_m + bdper per step)",
+                   "cells": zones, "gamma": pp["gm"], "safety": pp["safety"], "qav": pp["qav"],
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "mlw2d_kernel", "bytes_per_zone_update": B_ALG_2D,
+                     "avg_launch_us": avg_kern_s * 1e6},
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "check": {"t": info["t"], "dt_last": info["dt"], "mass_rel_drift": (mass1 - mass0) / mass0},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -136,8 +136,17 @@ int cans_clock_read(const void *clk, int nsteps_issued, double *t, int *ns, int 
 int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, int nsteps,
                    double qav, double gm, double safety, const double *dx, const double *dxm,
                    int ix, const double *dy, const double *dym, int jx, const int *bc,
-                   int margin, double *metric_ws, void *stream);
+                   int margin, double *metric_ws, void *const *events, void *stream);
 int cans2d_metric_ws_size(int ix, int jx);
+
+/* HIP events for timing the engine kernels of a run: `events` of the run
+ * entry points may be NULL or hold 2*nsteps events; the engine kernel of the
+ * s-th step of the call is bracketed by events[2s] and events[2s+1], recorded
+ * on the run's stream. */
+int cans_event_create(void **ev);
+int cans_event_destroy(void *ev);
+int cans_event_record(void *ev, void *stream);
+int cans_event_elapsed_ms(void *start, void *stop, float *ms);
 
 /* ---------------------------------------------------------------- host arrays
  * Device mirrors of host arrays, used by the Fortran drop-in wrappers.  In the

@@ -276,17 +276,47 @@ class MlwRun2D:
     def reset(self, t0=0.0, tend=1e300, nstop=2**31 - 1, dtmin=1e-10):
         self.clock.reset(t0, tend, nstop, dtmin)
 
-    def steps(self, n: int):
+    def steps(self, n: int, events=None):
+        """Issue n steps; `events` (an EventSet of >= 2n events) brackets the
+        engine kernel of every step."""
         dx, dxm, dy, dym = self.grid
         _check(lib().cans2d_mlw_run(self._pa, self._pb, self.clock.handle, C.c_int(self.clock.issued), C.c_int(n),
                                     _d(self.qav), _d(self.gm), _d(self.safety),
                                     _ptr(dx, (self.ix,)), _ptr(dxm, (self.ix,)), C.c_int(self.ix),
                                     _ptr(dy, (self.jx,)), _ptr(dym, (self.jx,)), C.c_int(self.jx),
                                     self.bc if self.bc is not None else None, C.c_int(self.margin),
-                                    _ptr(self.metric), _stream()))
+                                    _ptr(self.metric), events.array if events is not None else None, _stream()))
         self.clock.issued += n
 
     def state(self):
         """Current state (the buffer set the last executed step wrote)."""
         info = self.clock.read()
         return (self.a if info["nexec"] % 2 == 0 else self.b), info
+
+
+class EventSet:
+    """n HIP events (for bracketing kernels inside a run)."""
+
+    def __init__(self, n: int):
+        self.array = (C.c_void_p * n)()
+        for k in range(n):
+            h = C.c_void_p()
+            _check(lib().cans_event_create(C.byref(h)))
+            self.array[k] = h.value
+        self.n = n
+
+    def record(self, k: int):
+        _check(lib().cans_event_record(C.c_void_p(self.array[k]), _stream()))
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        _check(lib().cans_event_elapsed_ms(C.c_void_p(self.array[a]), C.c_void_p(self.array[b]), C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if _lib is not None:
+                for k in range(self.n):
+                    _lib.cans_event_destroy(C.c_void_p(self.array[k]))
+        except Exception:
+            pass

@@ -280,7 +280,7 @@ int cans_clock_read(const void *clk, int nsteps_issued, double *t, int *ns, int 
 int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, int nsteps,
                    double qav, double gm, double safety, const double *dx, const double *dxm,
                    int ix, const double *dy, const double *dym, int jx, const int *bc,
-                   int margin, double *metric_ws, void *stream) {
+                   int margin, double *metric_ws, void *const *events, void *stream) {
     if (!clk || !a7 || !b7 || !metric_ws) return fail(CANS_ERR_ARG, "cans2d_mlw_run: NULL argument");
     if (ix < 3 || jx < 3 || margin < 1) return fail(CANS_ERR_ARG, "cans2d_mlw_run: ix=%d jx=%d margin=%d", ix, jx, margin);
     hipStream_t s = (hipStream_t)stream;
@@ -314,7 +314,10 @@ int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, i
         a.qav = qav; a.gm = gm; a.dt = 0.0; a.clk = c; a.step = n; a.ix = ix; a.jx = jx;
         a.per_margin = periodic ? margin : 0;
         a.safety = periodic ? safety : 0.0;
+        const int e = n - step0;
+        if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e], s));
         HIPCHK(launch_mlw2d(a, s));
+        if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e + 1], s));
         if (!periodic) {
             // ghost cells of the new state: bnd (x faces, then y faces)
             for (int dir = 0; dir < 2; dir++) {
@@ -348,6 +351,25 @@ int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, i
 }
 
 int cans2d_metric_ws_size(int ix, int jx) { return (int)(4 * al(ix) + 4 * al(jx)); }
+
+// ====================================================================== events
+int cans_event_create(void **ev) {
+    HIPCHK(hipEventCreate((hipEvent_t *)ev));
+    return 0;
+}
+int cans_event_destroy(void *ev) {
+    if (ev) HIPCHK(hipEventDestroy((hipEvent_t)ev));
+    return 0;
+}
+int cans_event_record(void *ev, void *stream) {
+    HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+    return 0;
+}
+int cans_event_elapsed_ms(void *start, void *stop, float *ms) {
+    HIPCHK(hipEventSynchronize((hipEvent_t)stop));
+    HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return 0;
+}
 
 // ====================================================================== host mapping
 // Device mirrors of host arrays for the Fortran drop-in wrappers.
