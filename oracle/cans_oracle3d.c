@@ -1,0 +1,318 @@
+/*
+ * cans_oracle3d.c -- C restatement of the cans3d MLW engine, cfl_m and bc
+ * (test infrastructure only; see cans_oracle.h).  Citations are to
+ * cans_beta_vary/cans3d/.
+ */
+#include "cans_oracle.h"
+#include "cans_oracle_util.h"
+
+/* hdmlw/mlwhalf.f:38-68 */
+static void mlwhalf3(const double *u, double *un, double *du, double dt,
+                     const double *fx, const double *dxi, const double *dxim, int ix,
+                     const double *fy, const double *dyi, const double *dyim, int jx,
+                     const double *fz, const double *dzi, const double *dzim, int kx) {
+    for (int k = 2; k <= kx - 1; k++)
+        for (int j = 2; j <= jx - 1; j++)
+            for (int i = 2; i <= ix - 1; i++)
+                du[I3(i, j, k)] = du[I3(i, j, k)]
+                    - 0.5 * dt * (0.5 * dxi[I1(i)] * (fx[I3(i + 1, j, k)] - fx[I3(i - 1, j, k)]))
+                    - 0.5 * dt * (0.5 * dyi[I1(j)] * (fy[I3(i, j + 1, k)] - fy[I3(i, j - 1, k)]))
+                    - 0.5 * dt * (0.5 * dzi[I1(k)] * (fz[I3(i, j, k + 1)] - fz[I3(i, j, k - 1)]));
+    for (int k = 1; k <= kx - 1; k++)
+        for (int j = 1; j <= jx - 1; j++)
+            for (int i = 1; i <= ix - 1; i++) {
+                double uh = (u[I3(i + 1, j, k)] + u[I3(i, j, k)] + u[I3(i + 1, j + 1, k)] + u[I3(i, j + 1, k)]
+                             + u[I3(i + 1, j, k + 1)] + u[I3(i, j, k + 1)] + u[I3(i + 1, j + 1, k + 1)] + u[I3(i, j + 1, k + 1)]) / 8.0;
+                double dfdx = (fx[I3(i + 1, j, k)] - fx[I3(i, j, k)] + fx[I3(i + 1, j + 1, k)] - fx[I3(i, j + 1, k)]
+                               + fx[I3(i + 1, j, k + 1)] - fx[I3(i, j, k + 1)] + fx[I3(i + 1, j + 1, k + 1)] - fx[I3(i, j + 1, k + 1)])
+                              / 4.0 * dxim[I1(i)];
+                double dfdy = (fy[I3(i, j + 1, k)] - fy[I3(i, j, k)] + fy[I3(i + 1, j + 1, k)] - fy[I3(i + 1, j, k)]
+                               + fy[I3(i, j + 1, k + 1)] - fy[I3(i, j, k + 1)] + fy[I3(i + 1, j + 1, k + 1)] - fy[I3(i + 1, j, k + 1)])
+                              / 4.0 * dyim[I1(j)];
+                double dfdz = (fz[I3(i, j, k + 1)] - fz[I3(i, j, k)] + fz[I3(i + 1, j, k + 1)] - fz[I3(i + 1, j, k)]
+                               + fz[I3(i, j + 1, k + 1)] - fz[I3(i, j + 1, k)] + fz[I3(i + 1, j + 1, k + 1)] - fz[I3(i + 1, j + 1, k)])
+                              / 4.0 * dzim[I1(k)];
+                un[I3(i, j, k)] = uh - dt * dfdx - dt * dfdy - dt * dfdz;
+            }
+}
+
+/* hdmlw/mlwfull.f:34-55 */
+static void mlwfull3(double *du, double dt, const double *fx, const double *dxi, const double *ux0,
+                     const double *ux1, int ix, const double *fy, const double *dyi, const double *uy0,
+                     const double *uy1, int jx, const double *fz, const double *dzi, const double *uz0,
+                     const double *uz1, int kx) {
+    for (int k = 2; k <= kx - 1; k++)
+        for (int j = 2; j <= jx - 1; j++)
+            for (int i = 2; i <= ix - 1; i++) {
+                double dfdx = dxi[I1(i)] *
+                    (uz0[I1(k)] * (uy0[I1(j)] * (fx[I3(i, j - 1, k - 1)] - fx[I3(i - 1, j - 1, k - 1)])
+                                   + uy1[I1(j)] * (fx[I3(i, j, k - 1)] - fx[I3(i - 1, j, k - 1)]))
+                     + uz1[I1(k)] * (uy0[I1(j)] * (fx[I3(i, j - 1, k)] - fx[I3(i - 1, j - 1, k)])
+                                     + uy1[I1(j)] * (fx[I3(i, j, k)] - fx[I3(i - 1, j, k)])));
+                double dfdy = dyi[I1(j)] *
+                    (ux0[I1(i)] * (uz0[I1(k)] * (fy[I3(i - 1, j, k - 1)] - fy[I3(i - 1, j - 1, k - 1)])
+                                   + uz1[I1(k)] * (fy[I3(i - 1, j, k)] - fy[I3(i - 1, j - 1, k)]))
+                     + ux1[I1(i)] * (uz0[I1(k)] * (fy[I3(i, j, k - 1)] - fy[I3(i, j - 1, k - 1)])
+                                     + uz1[I1(k)] * (fy[I3(i, j, k)] - fy[I3(i, j - 1, k)])));
+                double dfdz = dzi[I1(k)] *
+                    (uy0[I1(j)] * (ux0[I1(i)] * (fz[I3(i - 1, j - 1, k)] - fz[I3(i - 1, j - 1, k - 1)])
+                                   + ux1[I1(i)] * (fz[I3(i, j - 1, k)] - fz[I3(i, j - 1, k - 1)]))
+                     + uy1[I1(j)] * (ux0[I1(i)] * (fz[I3(i - 1, j, k)] - fz[I3(i - 1, j, k - 1)])
+                                     + ux1[I1(i)] * (fz[I3(i, j, k)] - fz[I3(i, j, k - 1)])));
+                du[I3(i, j, k)] = du[I3(i, j, k)] - 0.5 * dt * dfdx - 0.5 * dt * dfdy - 0.5 * dt * dfdz;
+            }
+}
+
+/* hdmlw/mlwartv.f:36-48 */
+static void mlwartv3(const double *u, double *du, double dt, const double *qx, const double *dxi,
+                     const double *dxim, int ix, const double *qy, const double *dyi, const double *dyim,
+                     int jx, const double *qz, const double *dzi, const double *dzim, int kx) {
+    for (int k = 2; k <= kx - 1; k++)
+        for (int j = 2; j <= jx - 1; j++)
+            for (int i = 2; i <= ix - 1; i++) {
+                size_t p = I3(i, j, k);
+                du[p] = du[p]
+                    + dt * (dxi[I1(i)] * (qx[p] * dxim[I1(i)] * (u[I3(i + 1, j, k)] - u[p])
+                                          - qx[I3(i - 1, j, k)] * dxim[I1(i - 1)] * (u[p] - u[I3(i - 1, j, k)])))
+                    + dt * (dyi[I1(j)] * (qy[p] * dyim[I1(j)] * (u[I3(i, j + 1, k)] - u[p])
+                                          - qy[I3(i, j - 1, k)] * dyim[I1(j - 1)] * (u[p] - u[I3(i, j - 1, k)])))
+                    + dt * (dzi[I1(k)] * (qz[p] * dzim[I1(k)] * (u[I3(i, j, k + 1)] - u[p])
+                                          - qz[I3(i, j, k - 1)] * dzim[I1(k - 1)] * (u[p] - u[I3(i, j, k - 1)])));
+            }
+}
+
+/* hdmlw/mlw_m.f:2-453 (3D MHD, 8 fields) */
+void or3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz,
+                double *bx, double *by, double *bz, double dt, double qav,
+                double gm, const double *dx, const double *dxm, int ix,
+                const double *dy, const double *dym, int jx,
+                const double *dz, const double *dzm, int kx) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0;
+    const double pi4i = 1.0 / pi / 4.0;
+    size_t n = (size_t)ix * jx * kx;
+    double *dxi = dalloc(ix), *dxim = dalloc(ix), *ux0 = dalloc(ix), *ux1 = dalloc(ix);
+    double *dyi = dalloc(jx), *dyim = dalloc(jx), *uy0 = dalloc(jx), *uy1 = dalloc(jx);
+    double *dzi = dalloc(kx), *dzim = dalloc(kx), *uz0 = dalloc(kx), *uz1 = dalloc(kx);
+    double *ex = dalloc(n), *ey = dalloc(n), *ez = dalloc(n);
+    double *ee = dalloc(n), *rx = dalloc(n), *ry = dalloc(n), *rz = dalloc(n);
+    double *roh = dalloc(n), *eeh = dalloc(n), *rxh = dalloc(n), *ryh = dalloc(n), *rzh = dalloc(n);
+    double *bxh = dalloc(n), *byh = dalloc(n), *bzh = dalloc(n);
+    double *exh = dalloc(n), *eyh = dalloc(n), *ezh = dalloc(n);
+    double *prh = dalloc(n), *vxh = dalloc(n), *vyh = dalloc(n), *vzh = dalloc(n);
+    double *dro = dalloc(n), *dee = dalloc(n), *drx = dalloc(n), *dry = dalloc(n), *drz = dalloc(n);
+    double *dbx = dalloc(n), *dby = dalloc(n), *dbz = dalloc(n);
+    double *fx = dalloc(n), *fy = dalloc(n), *fz = dalloc(n), *qx = dalloc(n), *qy = dalloc(n), *qz = dalloc(n);
+
+    for (int i = 1; i <= ix; i++) { dxi[I1(i)] = 1.0 / dx[I1(i)]; dxim[I1(i)] = 1.0 / dxm[I1(i)]; }
+    for (int i = 2; i <= ix - 1; i++) { ux1[I1(i)] = 0.5 * dxm[I1(i - 1)] / dx[I1(i)]; ux0[I1(i)] = 0.5 * dxm[I1(i)] / dx[I1(i)]; }
+    for (int j = 1; j <= jx; j++) { dyi[I1(j)] = 1.0 / dy[I1(j)]; dyim[I1(j)] = 1.0 / dym[I1(j)]; }
+    for (int j = 2; j <= jx - 1; j++) { uy1[I1(j)] = 0.5 * dym[I1(j - 1)] / dy[I1(j)]; uy0[I1(j)] = 0.5 * dym[I1(j)] / dy[I1(j)]; }
+    for (int k = 1; k <= kx; k++) { dzi[I1(k)] = 1.0 / dz[I1(k)]; dzim[I1(k)] = 1.0 / dzm[I1(k)]; }
+    for (int k = 2; k <= kx - 1; k++) { uz1[I1(k)] = 0.5 * dzm[I1(k - 1)] / dz[I1(k)]; uz0[I1(k)] = 0.5 * dzm[I1(k)] / dz[I1(k)]; }
+    for (size_t p = 0; p < n; p++) {
+        dro[p] = dee[p] = drx[p] = dry[p] = drz[p] = dbx[p] = dby[p] = dbz[p] = 0.0;
+    }
+    for (size_t p = 0; p < n; p++) { /* :70-81 */
+        double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+        double bb = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+        ee[p] = pr[p] / (gm - 1.0) + 0.5 * ro[p] * vv + pi8i * bb;
+        rx[p] = ro[p] * vx[p];
+        ry[p] = ro[p] * vy[p];
+        rz[p] = ro[p] * vz[p];
+    }
+    for (size_t p = 0; p < n; p++) { /* :82-90 */
+        ex[p] = -vy[p] * bz[p] + vz[p] * by[p];
+        ey[p] = -vz[p] * bx[p] + vx[p] * bz[p];
+        ez[p] = -vx[p] * by[p] + vy[p] * bx[p];
+    }
+#define HALF(u, un, du) mlwhalf3(u, un, du, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx, fz, dzi, dzim, kx)
+    for (size_t p = 0; p < n; p++) { fx[p] = ro[p] * vx[p]; fy[p] = ro[p] * vy[p]; fz[p] = ro[p] * vz[p]; }
+    HALF(ro, roh, dro);
+    for (size_t p = 0; p < n; p++) {
+        double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+        double ep = pr[p] * gm / (gm - 1.0) + 0.5 * ro[p] * vv;
+        fx[p] = ep * vx[p] + (bz[p] * ey[p] - by[p] * ez[p]) * pi4i;
+        fy[p] = ep * vy[p] + (bx[p] * ez[p] - bz[p] * ex[p]) * pi4i;
+        fz[p] = ep * vz[p] + (by[p] * ex[p] - bx[p] * ey[p]) * pi4i;
+    }
+    HALF(ee, eeh, dee);
+    for (size_t p = 0; p < n; p++) {
+        fx[p] = ro[p] * sq(vx[p]) + pr[p] + pi8i * (sq(by[p]) + sq(bz[p]) - sq(bx[p]));
+        fy[p] = ro[p] * vx[p] * vy[p] - pi4i * bx[p] * by[p];
+        fz[p] = ro[p] * vx[p] * vz[p] - pi4i * bx[p] * bz[p];
+    }
+    HALF(rx, rxh, drx);
+    for (size_t p = 0; p < n; p++) {
+        fx[p] = ro[p] * vy[p] * vx[p] - pi4i * by[p] * bx[p];
+        fy[p] = ro[p] * sq(vy[p]) + pr[p] + pi8i * (sq(bx[p]) + sq(bz[p]) - sq(by[p]));
+        fz[p] = ro[p] * vy[p] * vz[p] - pi4i * by[p] * bz[p];
+    }
+    HALF(ry, ryh, dry);
+    for (size_t p = 0; p < n; p++) {
+        fx[p] = ro[p] * vz[p] * vx[p] - pi4i * bz[p] * bx[p];
+        fy[p] = ro[p] * vz[p] * vy[p] - pi4i * bz[p] * by[p];
+        fz[p] = ro[p] * sq(vz[p]) + pr[p] + pi8i * (sq(bx[p]) + sq(by[p]) - sq(bz[p]));
+    }
+    HALF(rz, rzh, drz);
+    for (size_t p = 0; p < n; p++) { fx[p] = 0.0; fy[p] = ez[p]; fz[p] = -ey[p]; }
+    HALF(bx, bxh, dbx);
+    for (size_t p = 0; p < n; p++) { fx[p] = -ez[p]; fy[p] = 0.0; fz[p] = ex[p]; }
+    HALF(by, byh, dby);
+    for (size_t p = 0; p < n; p++) { fx[p] = ey[p]; fy[p] = -ex[p]; fz[p] = 0.0; }
+    HALF(bz, bzh, dbz);
+#undef HALF
+    for (int k = 1; k <= kx - 1; k++) /* :193-204 */
+        for (int j = 1; j <= jx - 1; j++)
+            for (int i = 1; i <= ix - 1; i++) {
+                size_t p = I3(i, j, k);
+                vxh[p] = rxh[p] / roh[p];
+                vyh[p] = ryh[p] / roh[p];
+                vzh[p] = rzh[p] / roh[p];
+                double vv = sq(vxh[p]) + sq(vyh[p]) + sq(vzh[p]);
+                double bb = sq(bxh[p]) + sq(byh[p]) + sq(bzh[p]);
+                prh[p] = (gm - 1.0) * (eeh[p] - 0.5 * roh[p] * vv - pi8i * bb);
+            }
+    for (size_t p = 0; p < n; p++) {
+        exh[p] = -vyh[p] * bzh[p] + vzh[p] * byh[p];
+        eyh[p] = -vzh[p] * bxh[p] + vxh[p] * bzh[p];
+        ezh[p] = -vxh[p] * byh[p] + vyh[p] * bxh[p];
+    }
+#define FULL(du) mlwfull3(du, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx, fz, dzi, uz0, uz1, kx)
+#define HL for (int k = 1; k <= kx - 1; k++) for (int j = 1; j <= jx - 1; j++) for (int i = 1; i <= ix - 1; i++)
+    HL { size_t p = I3(i, j, k); fx[p] = roh[p] * vxh[p]; fy[p] = roh[p] * vyh[p]; fz[p] = roh[p] * vzh[p]; }
+    FULL(dro);
+    HL {
+        size_t p = I3(i, j, k);
+        double vv = sq(vxh[p]) + sq(vyh[p]) + sq(vzh[p]);
+        double ep = prh[p] * gm / (gm - 1.0) + 0.5 * roh[p] * vv;
+        fx[p] = ep * vxh[p] + (bzh[p] * eyh[p] - byh[p] * ezh[p]) * pi4i;
+        fy[p] = ep * vyh[p] + (bxh[p] * ezh[p] - bzh[p] * exh[p]) * pi4i;
+        fz[p] = ep * vzh[p] + (byh[p] * exh[p] - bxh[p] * eyh[p]) * pi4i;
+    }
+    FULL(dee);
+    HL {
+        size_t p = I3(i, j, k);
+        fx[p] = roh[p] * sq(vxh[p]) + prh[p] + pi8i * (sq(byh[p]) + sq(bzh[p]) - sq(bxh[p]));
+        fy[p] = roh[p] * vxh[p] * vyh[p] - pi4i * bxh[p] * byh[p];
+        fz[p] = roh[p] * vxh[p] * vzh[p] - pi4i * bxh[p] * bzh[p];
+    }
+    FULL(drx);
+    HL {
+        size_t p = I3(i, j, k);
+        fx[p] = roh[p] * vyh[p] * vxh[p] - pi4i * byh[p] * bxh[p];
+        fy[p] = roh[p] * sq(vyh[p]) + prh[p] + pi8i * (sq(bxh[p]) + sq(bzh[p]) - sq(byh[p]));
+        fz[p] = roh[p] * vyh[p] * vzh[p] - pi4i * byh[p] * bzh[p];
+    }
+    FULL(dry);
+    for (int k = 1; k <= kx - 1; k++) /* :269-280 runs j=1..jx, i=1..ix */
+        for (int j = 1; j <= jx; j++)
+            for (int i = 1; i <= ix; i++) {
+                size_t p = I3(i, j, k);
+                fx[p] = roh[p] * vzh[p] * vxh[p] - pi4i * bzh[p] * bxh[p];
+                fy[p] = roh[p] * vzh[p] * vyh[p] - pi4i * bzh[p] * byh[p];
+                fz[p] = roh[p] * sq(vzh[p]) + prh[p] + pi8i * (sq(bxh[p]) + sq(byh[p]) - sq(bzh[p]));
+            }
+    FULL(drz);
+    HL { size_t p = I3(i, j, k); fx[p] = 0.0; fy[p] = ezh[p]; fz[p] = -eyh[p]; }
+    FULL(dbx);
+    HL { size_t p = I3(i, j, k); fx[p] = -ezh[p]; fy[p] = 0.0; fz[p] = exh[p]; }
+    FULL(dby);
+    HL { size_t p = I3(i, j, k); fx[p] = eyh[p]; fy[p] = -exh[p]; fz[p] = 0.0; }
+    FULL(dbz);
+#undef HL
+#undef FULL
+    const double zero = 0.0;
+    for (int k = 1; k <= kx; k++)
+        for (int j = 1; j <= jx; j++)
+            for (int i = 1; i <= ix - 1; i++)
+                qx[I3(i, j, k)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I3(i + 1, j, k)] - vx[I3(i, j, k)]) - R4_1EM4);
+    for (int k = 1; k <= kx; k++)
+        for (int j = 1; j <= jx - 1; j++)
+            for (int i = 1; i <= ix; i++)
+                qy[I3(i, j, k)] = qav * dym[I1(j)] * fmaxF(zero, fabs(vy[I3(i, j + 1, k)] - vy[I3(i, j, k)]) - R4_1EM4);
+    for (int k = 1; k <= kx - 1; k++)
+        for (int j = 1; j <= jx; j++)
+            for (int i = 1; i <= ix; i++)
+                qz[I3(i, j, k)] = qav * dzm[I1(k)] * fmaxF(zero, fabs(vz[I3(i, j, k + 1)] - vz[I3(i, j, k)]) - R4_1EM4);
+#define ARTV(u, du) mlwartv3(u, du, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx, qz, dzi, dzim, kx)
+    ARTV(ro, dro); ARTV(ee, dee); ARTV(rx, drx); ARTV(ry, dry); ARTV(rz, drz);
+    ARTV(bx, dbx); ARTV(by, dby); ARTV(bz, dbz);
+#undef ARTV
+    for (int k = 2; k <= kx - 1; k++)
+        for (int j = 2; j <= jx - 1; j++)
+            for (int i = 2; i <= ix - 1; i++) {
+                size_t p = I3(i, j, k);
+                ro[p] = ro[p] + dro[p]; ee[p] = ee[p] + dee[p]; rx[p] = rx[p] + drx[p]; ry[p] = ry[p] + dry[p];
+                rz[p] = rz[p] + drz[p]; bx[p] = bx[p] + dbx[p]; by[p] = by[p] + dby[p]; bz[p] = bz[p] + dbz[p];
+            }
+    for (int k = 2; k <= kx - 1; k++)
+        for (int j = 2; j <= jx - 1; j++)
+            for (int i = 2; i <= ix - 1; i++) {
+                size_t p = I3(i, j, k);
+                vx[p] = rx[p] / ro[p];
+                vy[p] = ry[p] / ro[p];
+                vz[p] = rz[p] / ro[p];
+                double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+                double bb = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+                pr[p] = (gm - 1.0) * (ee[p] - 0.5 * ro[p] * vv - pi8i * bb);
+            }
+    double *tofree[] = {dxi, dxim, ux0, ux1, dyi, dyim, uy0, uy1, dzi, dzim, uz0, uz1, ex, ey, ez, ee, rx, ry, rz,
+                        roh, eeh, rxh, ryh, rzh, bxh, byh, bzh, exh, eyh, ezh, prh, vxh, vyh, vzh,
+                        dro, dee, drx, dry, drz, dbx, dby, dbz, fx, fy, fz, qx, qy, qz};
+    for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
+}
+
+/* common/cfl_m.f:39-80 (all cells; REAL*4 pi) */
+void or3d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
+                const double *ro, const double *pr, const double *vx,
+                const double *vy, const double *vz, const double *bx,
+                const double *by, const double *bz, const double *dx, int ix,
+                const double *dy, int jx, const double *dz, int kx,
+                int *imin, int *jmin, int *kmin) {
+    const double pai = (double)acosf(-1.0f);
+    const double onep4 = 0.25 / pai;
+    size_t n = (size_t)ix * jx * kx;
+    double *dtq = dalloc(n);
+    double d = R4_1E20;
+    int im = 0, jm = 0, km = 0;
+    for (int k = 1; k <= kx; k++)
+        for (int j = 1; j <= jx; j++)
+            for (int i = 1; i <= ix; i++) {
+                size_t p = I3(i, j, k);
+                double b2 = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+                double ca2 = b2 * onep4 / ro[p];
+                double cs2 = gm * pr[p] / ro[p];
+                double v2 = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+                double dtcfl = fminF(fminF(dx[I1(i)], dy[I1(j)]), dz[I1(k)]) / sqrt(v2 + cs2 + ca2);
+                dtq[p] = safety * dtcfl;
+            }
+    for (int k = 1; k <= kx; k++)
+        for (int j = 1; j <= jx; j++)
+            for (int i = 1; i <= ix; i++)
+                if (dtq[I3(i, j, k)] < d) { im = i; jm = j; km = k; d = dtq[I3(i, j, k)]; }
+    *merr = (d < dtmin) ? 9001 : 0;
+    *dt = d;
+    if (imin) *imin = im;
+    if (jmin) *jmin = jm;
+    if (kmin) *kmin = km;
+    free(dtq);
+}
+
+/* bc/bd*{x,y,z}.f (3D): the 1D index map along `dir` for every line */
+void or3d_bd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
+             const double *dm, int ix, int jx, int kx) {
+    int n = dir == 0 ? ix : (dir == 1 ? jx : kx);
+    size_t stride = dir == 0 ? 1 : (dir == 1 ? (size_t)ix : (size_t)ix * jx);
+    double *line = dalloc(n);
+    for (int k = 1; k <= (dir == 2 ? 1 : kx); k++)
+        for (int j = 1; j <= (dir == 1 ? 1 : jx); j++)
+            for (int i = 1; i <= (dir == 0 ? 1 : ix); i++) {
+                size_t base = I3(i, j, k);
+                for (int a = 0; a < n; a++) line[a] = qq[base + (size_t)a * stride];
+                if (kind == 7) or1d_bdperx(mbnd, margin, line, n);
+                else or1d_bd(kind, mbnd, margin, line, q0, dm, n);
+                for (int a = 0; a < n; a++) qq[base + (size_t)a * stride] = line[a];
+            }
+    free(line);
+}
