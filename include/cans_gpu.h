@@ -116,6 +116,19 @@ int cans3d_hlld_engine(double *ro, double *pr, double *vx, double *vy, double *v
                        const double *dx, const double *dy, const double *dz, int ix, int jx,
                        int kx, const int *mfdim, void *stream);
 
+/* Mixed precision: the same engines with the face-flux arithmetic in FP32
+ * (roeflux / hlld_flux evaluated in float) and everything else -- the
+ * conserved-variable accumulation, the MUSCL reconstruction, the pressure
+ * recovery and cfl_m -- in FP64.  No counterpart in the reference; for
+ * callers that trade ~1e-6 relative error for FP32 VALU throughput. */
+int cans3d_roe_m_mixed(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx,
+                       double *by, double *bz, double dt, double gm, const double *dx, int ix,
+                       const double *dy, int jx, const double *dz, int kx, void *stream);
+int cans3d_hlld_engine_mixed(double *ro, double *pr, double *vx, double *vy, double *vz,
+                             double *bx, double *by, double *bz, double gm, double mu, double dt,
+                             const double *dx, const double *dy, const double *dz, int ix, int jx,
+                             int kx, const int *mfdim, void *stream);
+
 /* ---------------------------------------------------------------- resident loop
  * A device clock holds t, ns, merr and dt of the loop in
  * cans2d/md_mhdshktb/hdmlw/main.f:150-206 so that steps run back to back with
@@ -138,6 +151,24 @@ int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, i
                    int ix, const double *dy, const double *dym, int jx, const int *bc,
                    int margin, double *metric_ws, void *const *events, void *stream);
 int cans2d_metric_ws_size(int ix, int jx);
+
+/* Engines of the resident run */
+#define CANS_ENGINE_MLW 0   /* cans{1,2,3}d/hdmlw/mlw_m.f */
+#define CANS_ENGINE_ROE 1   /* cans{1,2,3}d/hdroe/roe_m.f */
+#define CANS_ENGINE_HLLD 2  /* cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f (3D only) */
+
+/* Device workspace (bytes) a cans_run of this shape needs. */
+size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx);
+/* nsteps of the CANS loop {cfl_m; t=t+dt; engine; bnd} for any dimension and
+ * engine (mixed != 0: FP32 face fluxes for Roe/HLLD).  Field sets a/b:
+ * 1D ro,pr,vx,vy,by (bx1d, bxm1d separate); 2D ro,pr,vx,vy,bx,by,az;
+ * 3D ro,pr,vx,vy,vz,bx,by,bz.  bc[(dir*2+side)*nf+field] = CANS_BD_* kind
+ * (NULL only for the fused periodic 2D MLW path).  mu is used by HLLD. */
+int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
+             const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
+             double safety, const double *dx, const double *dxm, int ix, const double *dy,
+             const double *dym, int jx, const double *dz, const double *dzm, int kx, const int *bc,
+             int margin, void *ws, void *const *events, void *stream);
 
 /* HIP events for timing the engine kernels of a run: `events` of the run
  * entry points may be NULL or hold 2*nsteps events; the engine kernel of the

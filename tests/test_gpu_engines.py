@@ -1,0 +1,273 @@
+"""GPU parity of the Roe-MUSCL (1D/2D/3D), 3D MLW and HLLD engines, the 3D
+cfl_m/bnd and the resident runs of every engine, against the C restatement
+of the reference.  FP64 results must be bit-identical.
+
+The Roe and HLLD engines read half-step values of the outermost cell that the
+reference never sets, so their first and last interior cells are only
+defined for margin >= 3 (the reference drivers use 3); one-call comparisons
+therefore cover cells [4, n-3] (the oracle marks the others NaN), and the
+resident runs use margin 3, after which the boundary fill defines every cell.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import assert_bitwise, rel_l1, smooth_state_1d, smooth_state_2d, stretched_grid, to_dev, to_host
+from utrip_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+F1 = ("ro", "pr", "vx", "vy", "by")
+F2 = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+F3 = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")
+
+
+def case(*a):
+    print("CASE", *a, flush=True)
+
+
+def smooth_state_3d(ix, jx, kx, seed=0, bscale=1.0):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = np.meshgrid(np.linspace(0, 1, ix), np.linspace(0, 1, jx), np.linspace(0, 1, kx), indexing="ij")
+
+    def w(a, o):
+        kk = rng.uniform(1, 2, 4)
+        return np.asfortranarray(o + a * np.sin(2 * np.pi * (kk[0] * X + kk[1] * Y + kk[2] * Z) + kk[3]))
+    return dict(ro=w(.3, 1), pr=w(.2, .8), vx=w(.3, 0), vy=w(.3, 0), vz=w(.3, 0), bx=w(bscale, .2 * bscale),
+                by=w(bscale, -.1 * bscale), bz=w(bscale, .3 * bscale))
+
+
+def grid3(ix, jx, kx):
+    _, dx, dxm = stretched_grid(max(ix, 3), 1)
+    _, dy, dym = stretched_grid(max(jx, 3), 4)
+    _, dz, dzm = stretched_grid(max(kx, 3), 9)
+    return dx[:ix].copy(), dxm[:ix].copy(), dy[:jx].copy(), dym[:jx].copy(), dz[:kx].copy(), dzm[:kx].copy()
+
+
+def inner(a, lo, dims):
+    sl = tuple(slice(lo, n - lo) if n > 1 else slice(None) for n in a.shape[:dims]) + (slice(None),) * (a.ndim - dims)
+    return a[sl]
+
+
+# ------------------------------------------------------------------ Roe one call
+@pytest.mark.parametrize("ix", [9, 80, 300])
+def test_roe1d_one_call(dev, oracle, ix):
+    from utrip_amd import cans
+    case("roe1d", ix)
+    s = smooth_state_1d(ix, seed=ix)
+    _, dx, _ = stretched_grid(ix, seed=ix)
+    ref = {k: v.copy() for k, v in s.items()}
+    oracle.roe_m_1d(ref["ro"], ref["pr"], ref["vx"], ref["vy"], ref["by"], ref["bx"], ref["bxm"], 1.1e-3, 5 / 3, dx)
+    t = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.roe_m_1d(t["ro"], t["pr"], t["vx"], t["vy"], t["by"], t["bx"], t["bxm"], 1.1e-3, 5 / 3, to_dev(dx, dev))
+    for k in F1:
+        assert_bitwise(to_host(t[k])[3:-3], ref[k][3:-3], k)
+
+
+@pytest.mark.parametrize("ix,jx", [(9, 9), (40, 33), (70, 45)])
+def test_roe2d_one_call(dev, oracle, ix, jx):
+    from utrip_amd import cans
+    case("roe2d", ix, jx)
+    s = smooth_state_2d(ix, jx, seed=ix + jx)
+    _, dx, _ = stretched_grid(ix, 2)
+    _, dy, _ = stretched_grid(jx, 3)
+    ref = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.roe_m_2d(*[ref[k] for k in F2], 9e-4, 5 / 3, dx, dy)
+    t = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.roe_m_2d(*[t[k] for k in F2], 9e-4, 5 / 3, to_dev(dx, dev), to_dev(dy, dev))
+    for k in F2:
+        assert_bitwise(inner(to_host(t[k]), 3, 2), inner(ref[k], 3, 2), k)
+
+
+@pytest.mark.parametrize("shape", [(9, 9, 9), (14, 12, 11), (33, 10, 12)])
+def test_roe3d_one_call(dev, oracle, shape):
+    from utrip_amd import cans
+    case("roe3d", shape)
+    s = smooth_state_3d(*shape, seed=sum(shape))
+    dx, _, dy, _, dz, _ = grid3(*shape)
+    ref = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.roe_m_3d(*[ref[k] for k in F3], 8e-4, 5 / 3, dx, dy, dz)
+    t = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.roe_m_3d(*[t[k] for k in F3], 8e-4, 5 / 3, *[to_dev(v, dev) for v in (dx, dy, dz)])
+    for k in F3:
+        assert_bitwise(inner(to_host(t[k]), 3, 3), inner(ref[k], 3, 3), k)
+
+
+# ------------------------------------------------------------------ 3D MLW one call
+@pytest.mark.parametrize("shape", [(3, 3, 3), (13, 11, 10), (40, 7, 9)])
+def test_mlw3d_one_call(dev, oracle, shape):
+    from utrip_amd import cans
+    case("mlw3d", shape)
+    s = smooth_state_3d(*shape, seed=7 + sum(shape))
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    ref = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_3d(*[ref[k] for k in F3], 1.2e-3, 3.0, 5 / 3, dx, dxm, dy, dym, dz, dzm)
+    t = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_3d(*[t[k] for k in F3], 1.2e-3, 3.0, 5 / 3, *[to_dev(v, dev) for v in (dx, dxm, dy, dym, dz, dzm)])
+    for k in F3:
+        assert_bitwise(to_host(t[k]), ref[k], k)
+
+
+# ------------------------------------------------------------------ HLLD one call
+@pytest.mark.parametrize("shape,mf", [((14, 12, 11), (1, 1, 1)), ((60, 1, 1), (1, 0, 0)), ((20, 17, 1), (1, 1, 0)),
+                                      ((9, 9, 9), (1, 1, 1))])
+def test_hlld_one_call(dev, oracle, shape, mf):
+    from utrip_amd import cans
+    case("hlld", shape, mf)
+    s = smooth_state_3d(*shape, seed=11 + sum(shape))
+    dx, _, dy, _, dz, _ = grid3(*shape)
+    ref = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.hlld_engine(*[ref[k] for k in F3], 5 / 3, 1.0, 7e-4, dx, dy, dz, mf)
+    t = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.hlld_engine_3d(*[t[k] for k in F3], 5 / 3, 1.0, 7e-4, *[to_dev(v, dev) for v in (dx, dy, dz)], mfdim=mf)
+    for k in F3:
+        assert_bitwise(inner(to_host(t[k]), 3, 3), inner(ref[k], 3, 3), k)
+
+
+# ------------------------------------------------------------------ 3D cfl and bnd
+def test_cfl3d(dev, oracle):
+    from utrip_amd import cans
+    s = smooth_state_3d(15, 12, 9, seed=3)
+    dx, _, dy, _, dz, _ = grid3(15, 12, 9)
+    s["pr"][4, 5, 6] = np.nan
+    dto, merro, _ = oracle.cfl_m_3d(0.4, 1e-10, 5 / 3, *[s[k] for k in F3], dx, dy, dz)
+    t = {k: to_dev(v, dev) for k, v in s.items()}
+    dtg, merrg = cans.cfl_m_3d(0.4, 1e-10, 5 / 3, *[t[k] for k in F3], *[to_dev(v, dev) for v in (dx, dy, dz)])
+    assert dtg == dto and merrg == merro == 0
+    dtg, merrg = cans.cfl_m_3d(0.4, 1e3, 5 / 3, *[t[k] for k in F3], *[to_dev(v, dev) for v in (dx, dy, dz)])
+    assert merrg == 9001
+
+
+@pytest.mark.parametrize("kind", range(8))
+def test_bnd3d_kinds(dev, oracle, kind):
+    from utrip_amd import cans
+    rng = np.random.default_rng(kind)
+    shape = (9, 8, 7)
+    for d in range(3):
+        _, _, dm = stretched_grid(shape[d], seed=d)
+        for margin in (1, 2):
+            sides = [(margin, margin)] if kind == 7 else [(0, margin), (1, margin)]
+            for mbnd, mg in sides:
+                q = np.asfortranarray(rng.standard_normal(shape))
+                r = q.copy(order="F")
+                oracle.bnd_3d(d, kind, mbnd, mg, r, 0.37, dm)
+                t = to_dev(q, dev)
+                cans.bnd_3d(d, kind, mbnd, mg, t, 0.37, to_dev(dm, dev))
+                assert_bitwise(to_host(t), r, f"3d dir {d} kind {kind} side {mbnd} margin {margin}")
+
+
+# ------------------------------------------------------------------ resident runs
+def oracle_loop(oracle, dim, engine, fields, grid, pp, nsteps, bc_kind, margin, bx=None, bxm=None, mu=1.0):
+    """The reference main loop on the CPU: cfl_m, engine, bnd (all fields,
+    x then y then z faces, one boundary kind everywhere)."""
+    f = {k: v.copy(order="F") for k, v in fields.items()}
+    names = {1: F1, 2: F2, 3: F3}[dim]
+    dts = []
+    for _ in range(nsteps):
+        if dim == 1:
+            dt, merr, _ = oracle.cfl_m_1d(pp["safety"], pp["dtmin"], pp["gm"], bx, f["ro"], f["pr"], f["vx"], f["vy"],
+                                          f["by"], grid["dx"])
+        elif dim == 2:
+            dt, merr, _ = oracle.cfl_m_2d(pp["safety"], pp["dtmin"], pp["gm"], *[f[k] for k in F2[:6]], grid["dx"],
+                                          grid["dy"])
+        else:
+            dt, merr, _ = oracle.cfl_m_3d(pp["safety"], pp["dtmin"], pp["gm"], *[f[k] for k in F3], grid["dx"],
+                                          grid["dy"], grid["dz"])
+        assert merr == 0
+        if engine == "mlw":
+            if dim == 1:
+                oracle.mlw_m_1d(*[f[k] for k in F1], bx, bxm, dt, pp["qav"], pp["gm"], grid["dx"], grid["dxm"])
+            elif dim == 2:
+                oracle.mlw_m_2d(*[f[k] for k in F2], dt, pp["qav"], pp["gm"], grid["dx"], grid["dxm"], grid["dy"],
+                                grid["dym"])
+            else:
+                oracle.mlw_m_3d(*[f[k] for k in F3], dt, pp["qav"], pp["gm"], grid["dx"], grid["dxm"], grid["dy"],
+                                grid["dym"], grid["dz"], grid["dzm"])
+        elif engine == "roe":
+            if dim == 1:
+                oracle.roe_m_1d(*[f[k] for k in F1], bx, bxm, dt, pp["gm"], grid["dx"])
+            elif dim == 2:
+                oracle.roe_m_2d(*[f[k] for k in F2], dt, pp["gm"], grid["dx"], grid["dy"])
+            else:
+                oracle.roe_m_3d(*[f[k] for k in F3], dt, pp["gm"], grid["dx"], grid["dy"], grid["dz"])
+        else:
+            oracle.hlld_engine(*[f[k] for k in F3], pp["gm"], mu, dt, grid["dx"], grid["dy"], grid["dz"], (1, 1, 1))
+        for d in range(dim):
+            dm = grid[("dxm", "dym", "dzm")[d]]
+            for k in names:
+                if bc_kind == 7:
+                    (oracle.bnd_1d(7, margin, margin, f[k]) if dim == 1 else
+                     oracle.bnd_2d(d, 7, margin, margin, f[k]) if dim == 2 else
+                     oracle.bnd_3d(d, 7, margin, margin, f[k]))
+                else:
+                    for side in (0, 1):
+                        (oracle.bnd_1d(bc_kind, side, margin, f[k], 0.0, dm) if dim == 1 else
+                         oracle.bnd_2d(d, bc_kind, side, margin, f[k], 0.0, dm) if dim == 2 else
+                         oracle.bnd_3d(d, bc_kind, side, margin, f[k], 0.0, dm))
+        dts.append(dt)
+    return f, dts
+
+
+def run_gpu(dev, dim, engine, fields, grid, pp, nsteps, bc_kind, margin, bx=None, bxm=None, mixed=False, mu=1.0):
+    from utrip_amd import cans
+    names = {1: F1, 2: F2, 3: F3}[dim]
+    st = [to_dev(fields[k], dev) for k in names]
+    gd = {k: to_dev(v, dev) for k, v in grid.items() if k in ("dx", "dxm", "dy", "dym", "dz", "dzm")}
+    eng = {"mlw": cans.ENGINE_MLW, "roe": cans.ENGINE_ROE, "hlld": cans.ENGINE_HLLD}[engine]
+    run = cans.Run(dim, eng, st, gd, pp["gm"], pp["safety"], qav=pp.get("qav", 3.0), mu=mu, margin=margin,
+                   bc=bc_kind, mixed=mixed, bx1d=to_dev(bx, dev) if bx is not None else None,
+                   bxm1d=to_dev(bxm, dev) if bxm is not None else None)
+    run.reset(dtmin=pp["dtmin"])
+    run.steps(nsteps // 2)
+    run.steps(nsteps - nsteps // 2)
+    cur, info = run.state()
+    return {k: to_host(t) for k, t in zip(names, cur)}, info
+
+
+@pytest.mark.parametrize("engine", ["mlw", "roe"])
+def test_run_1d_shock_tube(dev, oracle, engine):
+    case("run 1d", engine)
+    margin = 3
+    pb = P.shock_tube_1d(nx=256, margin=margin)
+    f = {k: pb.fields[k] for k in F1}
+    ref, dts = oracle_loop(oracle, 1, engine, f, pb.grid, pb.params, 60, 0, margin, pb.fields["bx"], pb.fields["bxm"])
+    got, info = run_gpu(dev, 1, engine, f, pb.grid, pb.params, 60, 0, margin, pb.fields["bx"], pb.fields["bxm"])
+    assert info["nexec"] == 60 and info["dt"] == dts[-1]
+    for k in F1:
+        assert_bitwise(got[k], ref[k], k)
+
+
+@pytest.mark.parametrize("engine,margin", [("mlw", 1), ("roe", 3)])
+def test_run_2d_ot(dev, oracle, engine, margin):
+    case("run 2d OT", engine)
+    pb = P.orszag_tang_2d(n=48, margin=margin)
+    ref, dts = oracle_loop(oracle, 2, engine, pb.fields, pb.grid, pb.params, 12, 7, margin)
+    got, info = run_gpu(dev, 2, engine, pb.fields, pb.grid, pb.params, 12, 7, margin)
+    assert info["nexec"] == 12 and info["dt"] == dts[-1]
+    for k in F2:
+        assert_bitwise(got[k], ref[k], k)
+
+
+@pytest.mark.parametrize("engine,margin", [("mlw", 2), ("roe", 3), ("hlld", 3)])
+def test_run_3d_blast(dev, oracle, engine, margin):
+    case("run 3d blast", engine)
+    pb = P.blast_3d(n=14, margin=margin, gaussian_b=(engine != "hlld"), r0=0.25)
+    ref, dts = oracle_loop(oracle, 3, engine, pb.fields, pb.grid, pb.params, 6, 7, margin)
+    got, info = run_gpu(dev, 3, engine, pb.fields, pb.grid, pb.params, 6, 7, margin)
+    assert info["nexec"] == 6 and info["dt"] == dts[-1]
+    for k in F3:
+        assert_bitwise(got[k], ref[k], k)
+
+
+@pytest.mark.parametrize("engine", ["roe", "hlld"])
+def test_mixed_precision_3d(dev, oracle, engine):
+    """FP32 face fluxes + FP64 state: density within 1e-5 relative L1 of the
+    FP64 oracle after a few steps of the blast wave."""
+    case("mixed 3d", engine)
+    margin = 3
+    pb = P.blast_3d(n=16, margin=margin, gaussian_b=(engine != "hlld"), r0=0.25)
+    ref, _ = oracle_loop(oracle, 3, engine, pb.fields, pb.grid, pb.params, 5, 7, margin)
+    got, info = run_gpu(dev, 3, engine, pb.fields, pb.grid, pb.params, 5, 7, margin, mixed=True)
+    err = {k: rel_l1(got[k], ref[k]) for k in F3}
+    print("rel L1 mixed vs fp64:", err)
+    assert err["ro"] < 1e-5, err
+    assert max(err[k] for k in ("ro", "pr", "bx", "by")) < 1e-4, err

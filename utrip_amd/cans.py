@@ -320,3 +320,91 @@ class EventSet:
                     _lib.cans_event_destroy(C.c_void_p(self.array[k]))
         except Exception:
             pass
+
+
+def roe_m_3d_mixed(ro, pr, vx, vy, vz, bx, by, bz, dt, gm, dx, dy, dz):
+    """roe_m (3D) with FP32 face-flux arithmetic, FP64 state (no reference counterpart)."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    _check(lib().cans3d_roe_m_mixed(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                                    _d(dt), _d(gm), _ptr(dx, (ix,)), C.c_int(ix), _ptr(dy, (jx,)), C.c_int(jx),
+                                    _ptr(dz, (kx,)), C.c_int(kx), _stream()))
+
+
+def hlld_engine_3d_mixed(ro, pr, vx, vy, vz, bx, by, bz, gm, mu, dt, dx, dy, dz, mfdim=(1, 1, 1)):
+    """HLLD engine with FP32 face-flux arithmetic, FP64 state (no reference counterpart)."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    mf = (C.c_int * 3)(*[int(m) for m in mfdim])
+    _check(lib().cans3d_hlld_engine_mixed(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                                          _d(gm), _d(mu), _d(dt), _ptr(dx, (ix,)), _ptr(dy, (jx,)), _ptr(dz, (kx,)),
+                                          C.c_int(ix), C.c_int(jx), C.c_int(kx), mf, _stream()))
+
+
+ENGINE_MLW, ENGINE_ROE, ENGINE_HLLD = 0, 1, 2
+FIELDS = {1: ("ro", "pr", "vx", "vy", "by"), 2: ("ro", "pr", "vx", "vy", "bx", "by", "az"),
+          3: ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")}
+
+
+class Run:
+    """Resident CANS time loop for any dimension and engine (cans_run).
+
+    ``state``: list of the dimension's fields (see FIELDS) with ghosts filled.
+    ``grid``: dict with dx, dxm (and dy, dym, dz, dzm as the dimension needs).
+    ``bc``: per (dir, side, field) CANS_BD_* kinds, a flat list of
+    2*dim*nf ints, or a single kind for every face and field; None = the
+    fused periodic 2D MLW path.
+    """
+
+    def __init__(self, dim, engine, state, grid, gm, safety, qav=3.0, mu=1.0, margin=1, bc=None, mixed=False,
+                 bx1d=None, bxm1d=None, clock=None):
+        self.dim, self.engine, self.mixed = dim, engine, mixed
+        self.nf = len(FIELDS[dim])
+        assert len(state) == self.nf, f"{dim}D needs {self.nf} fields"
+        self.a = list(state)
+        self.b = [torch.empty_like(t) for t in state]
+        shape = tuple(state[0].shape)
+        self.ix, self.jx, self.kx = _dims(state[0])
+        self.shape = shape
+        self.grid = grid
+        self.gm, self.safety, self.qav, self.mu, self.margin = gm, safety, qav, mu, margin
+        self.bx1d, self.bxm1d = bx1d, bxm1d
+        self.clock = clock or Clock()
+        lib().cans_run_ws_bytes.restype = C.c_size_t
+        nbytes = lib().cans_run_ws_bytes(C.c_int(dim), C.c_int(engine), C.c_int(self.ix), C.c_int(self.jx),
+                                         C.c_int(self.kx))
+        self.ws = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=state[0].device)
+        if bc is None:
+            self.bc = None
+        else:
+            if isinstance(bc, int):
+                bc = [bc] * (2 * dim * self.nf)
+            assert len(bc) == 2 * dim * self.nf
+            self.bc = (C.c_int * len(bc))(*[int(k) for k in bc])
+        self._pa = _ptr_array(self.a, shape)
+        self._pb = _ptr_array(self.b, shape)
+
+    def reset(self, t0=0.0, tend=1e300, nstop=2**31 - 1, dtmin=1e-10):
+        self.clock.reset(t0, tend, nstop, dtmin)
+
+    def steps(self, n: int, events=None):
+        g = self.grid
+
+        def gp(name, ln):
+            return _ptr(g[name], (ln,)) if name in g and g[name] is not None else C.c_void_p(0)
+        _check(lib().cans_run(C.c_int(self.dim), C.c_int(self.engine), C.c_int(1 if self.mixed else 0),
+                              self._pa, self._pb,
+                              _ptr(self.bx1d, (self.ix,)) if self.bx1d is not None else C.c_void_p(0),
+                              _ptr(self.bxm1d, (self.ix,)) if self.bxm1d is not None else C.c_void_p(0),
+                              self.clock.handle, C.c_int(self.clock.issued), C.c_int(n),
+                              _d(self.qav), _d(self.gm), _d(self.mu), _d(self.safety),
+                              gp("dx", self.ix), gp("dxm", self.ix), C.c_int(self.ix),
+                              gp("dy", self.jx), gp("dym", self.jx), C.c_int(self.jx),
+                              gp("dz", self.kx), gp("dzm", self.kx), C.c_int(self.kx),
+                              self.bc, C.c_int(self.margin), _ptr(self.ws),
+                              events.array if events is not None else None, _stream()))
+        self.clock.issued += n
+
+    def state(self):
+        info = self.clock.read()
+        return (self.a if info["nexec"] % 2 == 0 else self.b), info

@@ -20,18 +20,22 @@
 
 using namespace cans;
 
-namespace {
-
+namespace cans {
 thread_local char g_err[512] = "";
-std::mutex g_mu;
-
-int fail(int code, const char *fmt, ...) {
+int abi_fail(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
     return code;
 }
+int abi_scratch(size_t bytes, void **out);
+}  // namespace cans
+
+namespace {
+
+std::mutex g_mu;
+#define fail cans::abi_fail
 #define HIPCHK(expr)                                                                    \
     do {                                                                                \
         hipError_t e_ = (expr);                                                         \
@@ -72,9 +76,11 @@ inline size_t al(size_t n) { return (n + 63) & ~size_t(63); }  // doubles, 512-b
 
 }  // namespace
 
+int cans::abi_scratch(size_t bytes, void **out) { return scratch(bytes, out); }
+
 extern "C" {
 
-const char *cans_gpu_last_error(void) { return g_err; }
+const char *cans_gpu_last_error(void) { return cans::g_err; }
 
 int cans_gpu_init(int device) {
     int n = 0;

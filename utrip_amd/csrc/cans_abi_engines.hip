@@ -1,0 +1,359 @@
+// cans_abi_engines.hip -- extern "C" entry points of the Roe-MUSCL, 3D MLW
+// and HLLD engines, and the resident multi-step runs of every engine
+// (declared in include/cans_gpu.h).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "cans_common.hpp"
+#include "cans_kernels.hpp"
+#include "cans_roe.hpp"
+#include "../../include/cans_gpu.h"
+
+namespace cans {
+int abi_fail(int code, const char *fmt, ...);
+int abi_scratch(size_t bytes, void **out);
+}  // namespace cans
+
+using namespace cans;
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return abi_fail(CANS_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                             \
+    } while (0)
+
+namespace {
+
+inline size_t al(size_t n) { return (n + 63) & ~size_t(63); }
+
+// carve `count` arrays of n doubles out of a workspace cursor
+struct Carver {
+    double *p;
+    double *take(size_t n) {
+        double *r = p;
+        p += al(n);
+        return r;
+    }
+};
+
+size_t metric_doubles(int ix, int jx, int kx) { return 4 * (al(ix) + al(jx) + al(kx)); }
+
+void carve_metric(Carver &c, int ix, int jx, int kx, Metric *m) {
+    const int n[3] = {ix, jx, kx};
+    for (int d = 0; d < 3; d++) {
+        m[d].di = c.take(n[d]);
+        m[d].dim = c.take(n[d]);
+        m[d].u0 = c.take(n[d]);
+        m[d].u1 = c.take(n[d]);
+    }
+}
+
+// engine scratch (doubles) beyond the metric
+size_t engine_doubles(int dim, int engine, size_t N) {
+    const size_t a = al(N);
+    switch (engine) {
+    case CANS_ENGINE_MLW: return dim == 3 ? (size_t)(kMlw3dCellFields + kMlw3dCornerFields) * a : 0;
+    case CANS_ENGINE_ROE: return (8 + 3 * 8) * a;
+    case CANS_ENGINE_HLLD: return 4 * 8 * a;
+    default: return 0;
+    }
+}
+
+void roe_scratch(RoeArgs &r, Carver &c, size_t N, int dim) {
+    const bool used[8] = {true, true, true, true, dim == 3, dim >= 2, true, dim == 3};
+    for (int q = 0; q < 8; q++) r.half[q] = used[q] ? c.take(N) : (c.take(N), nullptr);
+    for (int d = 0; d < 3; d++)
+        for (int q = 0; q < 8; q++) r.flux[d][q] = c.take(N);
+}
+
+void hlld_scratch(HlldArgs &h, Carver &c, size_t N) {
+    for (int q = 0; q < 8; q++) h.half[q] = c.take(N);
+    for (int q = 0; q < 8; q++) h.u1[q] = c.take(N);
+    for (int q = 0; q < 8; q++) h.u2[q] = c.take(N);
+    for (int q = 0; q < 8; q++) h.flux[q] = c.take(N);
+}
+
+int check_margin(const char *who, int n, int lo) {
+    if (n < lo) return abi_fail(CANS_ERR_ARG, "%s: dimension %d < %d", who, n, lo);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ====================================================================== Roe drop-in
+static int roe_dropin(int dim, bool mixed, double *const *prim, double *az, const double *bx, const double *bxm,
+                      double dt, double gm, const double *dx, int ix, const double *dy, int jx, const double *dz,
+                      int kx, void *stream) {
+    const int n[3] = {ix, jx, kx};
+    for (int d = 0; d < dim; d++)
+        if (int rc = check_margin("roe_m", n[d], 5)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t N = (size_t)ix * jx * kx;
+    void *w = nullptr;
+    if (int rc = abi_scratch((engine_doubles(dim, CANS_ENGINE_ROE, N) + 9 * al(N)) * sizeof(double), &w)) return rc;
+    Carver c{(double *)w};
+    RoeArgs r{};
+    for (int q = 0; q < 8; q++) r.prim[q] = prim[q];
+    roe_scratch(r, c, N, dim);
+    double *outbuf[8];
+    for (int q = 0; q < 8; q++) { outbuf[q] = c.take(N); r.out[q] = prim[q] ? outbuf[q] : nullptr; }
+    double *azo = c.take(N);
+    r.az = az; r.out_az = dim == 2 ? azo : nullptr;
+    r.bx = bx; r.bxm = bxm; r.dx = dx; r.dy = dy; r.dz = dz;
+    r.gm = gm; r.dt = dt; r.clk = nullptr; r.ix = ix; r.jx = jx; r.kx = kx;
+    HIPCHK(launch_roe(r, dim, mixed, s));
+    double *dst[9];
+    const double *src[9];
+    int nf = 0;
+    for (int q = 0; q < 8; q++)
+        if (prim[q]) { dst[nf] = prim[q]; src[nf] = outbuf[q]; nf++; }
+    if (dim == 2) { dst[nf] = az; src[nf] = azo; nf++; }
+    HIPCHK(launch_copy_interior(dst, src, nf, ix, jx, kx, 3, s));
+    return 0;
+}
+
+int cans1d_roe_m(double *ro, double *pr, double *vx, double *vy, double *by, const double *bx,
+                 const double *bxm, double dt, double gm, const double *dx, int ix, void *stream) {
+    double *prim[8] = {ro, pr, vx, vy, nullptr, nullptr, by, nullptr};
+    return roe_dropin(1, false, prim, nullptr, bx, bxm, dt, gm, dx, ix, nullptr, 1, nullptr, 1, stream);
+}
+
+int cans2d_roe_m(double *ro, double *pr, double *vx, double *vy, double *bx, double *by, double *az,
+                 double dt, double gm, const double *dx, int ix, const double *dy, int jx, void *stream) {
+    double *prim[8] = {ro, pr, vx, vy, nullptr, bx, by, nullptr};
+    return roe_dropin(2, false, prim, az, nullptr, nullptr, dt, gm, dx, ix, dy, jx, nullptr, 1, stream);
+}
+
+int cans3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
+                 double *bz, double dt, double gm, const double *dx, int ix, const double *dy, int jx,
+                 const double *dz, int kx, void *stream) {
+    double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
+    return roe_dropin(3, false, prim, nullptr, nullptr, nullptr, dt, gm, dx, ix, dy, jx, dz, kx, stream);
+}
+
+int cans3d_roe_m_mixed(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
+                       double *bz, double dt, double gm, const double *dx, int ix, const double *dy, int jx,
+                       const double *dz, int kx, void *stream) {
+    double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
+    return roe_dropin(3, true, prim, nullptr, nullptr, nullptr, dt, gm, dx, ix, dy, jx, dz, kx, stream);
+}
+
+// ====================================================================== 3D MLW drop-in
+int cans3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
+                 double *bz, double dt, double qav, double gm, const double *dx, const double *dxm, int ix,
+                 const double *dy, const double *dym, int jx, const double *dz, const double *dzm, int kx,
+                 void *stream) {
+    if (ix < 3 || jx < 3 || kx < 3) return abi_fail(CANS_ERR_ARG, "cans3d_mlw_m: %d x %d x %d", ix, jx, kx);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t N = (size_t)ix * jx * kx;
+    void *w = nullptr;
+    const size_t need = metric_doubles(ix, jx, kx) + engine_doubles(3, CANS_ENGINE_MLW, N) + 8 * al(N);
+    if (int rc = abi_scratch(need * sizeof(double), &w)) return rc;
+    Carver c{(double *)w};
+    Metric m[3];
+    carve_metric(c, ix, jx, kx, m);
+    HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
+    HIPCHK(launch_metric(dy, dym, jx, m[1], s));
+    HIPCHK(launch_metric(dz, dzm, kx, m[2], s));
+    Mlw3dArgs a{};
+    double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
+    for (int q = 0; q < 8; q++) a.prim[q] = prim[q];
+    a.cell = c.take((size_t)kMlw3dCellFields * al(N));
+    a.corner = c.take((size_t)kMlw3dCornerFields * al(N));
+    for (int q = 0; q < 8; q++) a.out[q] = c.take(N);
+    a.dxm = dxm; a.dym = dym; a.dzm = dzm; a.mx = m[0]; a.my = m[1]; a.mz = m[2];
+    a.qav = qav; a.gm = gm; a.dt = dt; a.clk = nullptr; a.ix = ix; a.jx = jx; a.kx = kx;
+    HIPCHK(launch_mlw3d(a, s));
+    const double *src[8];
+    for (int q = 0; q < 8; q++) src[q] = a.out[q];
+    HIPCHK(launch_copy_interior(prim, src, 8, ix, jx, kx, 2, s));
+    return 0;
+}
+
+// ====================================================================== HLLD drop-in
+static int hlld_dropin(bool mixed, double *const *prim, double gm, double mu, double dt, const double *dx,
+                       const double *dy, const double *dz, int ix, int jx, int kx, const int *mfdim, void *stream) {
+    if (!mfdim) return abi_fail(CANS_ERR_ARG, "hlld: mfdim is NULL");
+    const int n[3] = {ix, jx, kx};
+    for (int d = 0; d < 3; d++)
+        if (mfdim[d] == 1 && n[d] < 5) return abi_fail(CANS_ERR_ARG, "hlld: direction %d has %d cells", d, n[d]);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t N = (size_t)ix * jx * kx;
+    void *w = nullptr;
+    if (int rc = abi_scratch(engine_doubles(3, CANS_ENGINE_HLLD, N) * sizeof(double), &w)) return rc;
+    Carver c{(double *)w};
+    HlldArgs h{};
+    for (int q = 0; q < 8; q++) { h.prim[q] = prim[q]; h.out[q] = prim[q]; }
+    hlld_scratch(h, c, N);
+    h.dx = dx; h.dy = dy; h.dz = dz; h.gm = gm; h.mu = mu; h.dt = dt;
+    for (int d = 0; d < 3; d++) h.mfdim[d] = mfdim[d];
+    h.clk = nullptr; h.ix = ix; h.jx = jx; h.kx = kx;
+    HIPCHK(launch_hlld(h, mixed, s));
+    return 0;
+}
+
+int cans3d_hlld_engine(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
+                       double *bz, double gm, double mu, double dt, const double *dx, const double *dy,
+                       const double *dz, int ix, int jx, int kx, const int *mfdim, void *stream) {
+    double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
+    return hlld_dropin(false, prim, gm, mu, dt, dx, dy, dz, ix, jx, kx, mfdim, stream);
+}
+
+int cans3d_hlld_engine_mixed(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
+                             double *bz, double gm, double mu, double dt, const double *dx, const double *dy,
+                             const double *dz, int ix, int jx, int kx, const int *mfdim, void *stream) {
+    double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
+    return hlld_dropin(true, prim, gm, mu, dt, dx, dy, dz, ix, jx, kx, mfdim, stream);
+}
+
+// ====================================================================== resident runs
+size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx) {
+    const size_t N = (size_t)ix * jx * kx;
+    return (metric_doubles(ix, jx, kx) + engine_doubles(dim, engine, N) + 64) * sizeof(double);
+}
+
+// One resident run of `nsteps` CANS loop iterations {cfl_m; t=t+dt; engine;
+// bnd} for any dimension/engine with generic boundaries.  State set a holds
+// step n (n even), b step n+1, alternating.  nf fields per set:
+// 1D 5 (ro pr vx vy by), 2D 7 (ro pr vx vy bx by az), 3D 8 (ro pr vx vy vz bx by bz).
+int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
+             const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
+             double safety, const double *dx, const double *dxm, int ix, const double *dy, const double *dym,
+             int jx, const double *dz, const double *dzm, int kx, const int *bc, int margin, void *ws,
+             void *const *events, void *stream) {
+    if (dim < 1 || dim > 3) return abi_fail(CANS_ERR_ARG, "cans_run: dim %d", dim);
+    if (!a || !b || !clk || !ws) return abi_fail(CANS_ERR_ARG, "cans_run: NULL argument");
+    if (engine == CANS_ENGINE_HLLD && dim != 3) return abi_fail(CANS_ERR_ARG, "cans_run: HLLD is 3D");
+    if (engine == CANS_ENGINE_MLW && dim == 2 && !bc)
+        return cans2d_mlw_run(a, b, clk, step0, nsteps, qav, gm, safety, dx, dxm, ix, dy, dym, jx, nullptr, margin,
+                              (double *)ws, events, stream);
+    if (!bc) return abi_fail(CANS_ERR_ARG, "cans_run: bc is NULL (only 2D MLW has the fused periodic path)");
+    if (dim == 1) { jx = 1; kx = 1; }
+    if (dim == 2) kx = 1;
+    const int nf = dim == 1 ? 5 : (dim == 2 ? 7 : 8);
+    const int lo = engine == CANS_ENGINE_MLW ? 3 : 5;
+    const int n[3] = {ix, jx, kx};
+    for (int d = 0; d < dim; d++)
+        if (n[d] < lo || n[d] < 2 * margin + 1) return abi_fail(CANS_ERR_ARG, "cans_run: dimension %d too small", n[d]);
+    hipStream_t s = (hipStream_t)stream;
+    Clock *c = (Clock *)clk;
+    const size_t N = (size_t)ix * jx * kx;
+    Carver cv{(double *)ws};
+    Metric m[3];
+    carve_metric(cv, ix, jx, kx, m);
+    if (engine == CANS_ENGINE_MLW) {
+        HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
+        if (dim >= 2) HIPCHK(launch_metric(dy, dym, jx, m[1], s));
+        if (dim == 3) HIPCHK(launch_metric(dz, dzm, kx, m[2], s));
+    }
+    double *eng = cv.p;
+    for (int st = step0; st < step0 + nsteps; st++) {
+        double *const *in = (st & 1) ? b : a;
+        double *const *out = (st & 1) ? a : b;
+        // map the field list onto the engine slots (ro pr vx vy vz bx by bz)
+        double *pin[8] = {nullptr}, *pout[8] = {nullptr};
+        if (dim == 1) {
+            const int map[5] = {0, 1, 2, 3, 6};
+            for (int f = 0; f < 5; f++) { pin[map[f]] = in[f]; pout[map[f]] = out[f]; }
+        } else if (dim == 2) {
+            const int map[6] = {0, 1, 2, 3, 5, 6};
+            for (int f = 0; f < 6; f++) { pin[map[f]] = in[f]; pout[map[f]] = out[f]; }
+        } else {
+            for (int f = 0; f < 8; f++) { pin[f] = in[f]; pout[f] = out[f]; }
+        }
+        // cfl_m of the current state (cfl_m.f for the dimension; the HLLD
+        // driver also calls the Gaussian cans3d cfl_m)
+        CflArgs ca{};
+        ca.ro = pin[0]; ca.pr = pin[1]; ca.vx = pin[2]; ca.vy = pin[3]; ca.vz = pin[4];
+        ca.bx = dim == 1 ? bx1d : pin[5]; ca.by = pin[6]; ca.bz = pin[7];
+        ca.dx = dx; ca.dy = dy; ca.dz = dz; ca.safety = safety; ca.gm = gm;
+        ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st;
+        HIPCHK(launch_cfl(ca, s));
+        HIPCHK(launch_clock_step(c, st, s));
+        const int e = st - step0;
+        if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e], s));
+        if (engine == CANS_ENGINE_MLW) {
+            if (dim == 1) {
+                Mlw1dArgs g{};
+                g.ro = pin[0]; g.pr = pin[1]; g.vx = pin[2]; g.vy = pin[3]; g.by = pin[6];
+                g.ro_o = pout[0]; g.pr_o = pout[1]; g.vx_o = pout[2]; g.vy_o = pout[3]; g.by_o = pout[6];
+                g.bx = bx1d; g.bxm = bxm1d; g.dx = dx; g.dxm = dxm; g.mx = m[0];
+                g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix;
+                HIPCHK(launch_mlw1d(g, s));
+            } else if (dim == 2) {
+                Mlw2dArgs g{};
+                g.ro = in[0]; g.pr = in[1]; g.vx = in[2]; g.vy = in[3]; g.bx = in[4]; g.by = in[5]; g.az = in[6];
+                g.ro_o = out[0]; g.pr_o = out[1]; g.vx_o = out[2]; g.vy_o = out[3]; g.bx_o = out[4];
+                g.by_o = out[5]; g.az_o = out[6];
+                g.dx = dx; g.dxm = dxm; g.dy = dy; g.dym = dym; g.mx = m[0]; g.my = m[1];
+                g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx;
+                g.per_margin = 0; g.safety = 0.0;
+                g.peek = 1;
+                HIPCHK(launch_mlw2d(g, s));
+            } else {
+                Mlw3dArgs g{};
+                for (int q = 0; q < 8; q++) { g.prim[q] = pin[q]; g.out[q] = pout[q]; }
+                Carver ec{eng};
+                g.cell = ec.take((size_t)kMlw3dCellFields * N);
+                g.corner = ec.take((size_t)kMlw3dCornerFields * N);
+                g.dxm = dxm; g.dym = dym; g.dzm = dzm; g.mx = m[0]; g.my = m[1]; g.mz = m[2];
+                g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx; g.kx = kx;
+                HIPCHK(launch_mlw3d(g, s));
+            }
+        } else if (engine == CANS_ENGINE_ROE) {
+            RoeArgs r{};
+            for (int q = 0; q < 8; q++) { r.prim[q] = pin[q]; r.out[q] = pout[q]; }
+            Carver ec{eng};
+            roe_scratch(r, ec, N, dim);
+            r.az = dim == 2 ? in[6] : nullptr;
+            r.out_az = dim == 2 ? out[6] : nullptr;
+            r.bx = bx1d; r.bxm = bxm1d; r.dx = dx; r.dy = dy; r.dz = dz;
+            r.gm = gm; r.clk = c; r.step = st; r.ix = ix; r.jx = jx; r.kx = kx;
+            HIPCHK(launch_roe(r, dim, mixed != 0, s));
+        } else {
+            HlldArgs h{};
+            for (int q = 0; q < 8; q++) { h.prim[q] = pin[q]; h.out[q] = pout[q]; }
+            Carver ec{eng};
+            hlld_scratch(h, ec, N);
+            h.dx = dx; h.dy = dy; h.dz = dz; h.gm = gm; h.mu = mu;
+            h.mfdim[0] = h.mfdim[1] = h.mfdim[2] = 1;
+            h.clk = c; h.step = st; h.ix = ix; h.jx = jx; h.kx = kx;
+            HIPCHK(launch_hlld(h, mixed != 0, s));
+        }
+        if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e + 1], s));
+        // bnd: x faces, then y, then z; each face: periodic fields both sides
+        for (int dir = 0; dir < dim; dir++) {
+            for (int side = 0; side < 2; side++) {
+                BndArgs bp{}, bo{};
+                for (int f = 0; f < nf; f++) {
+                    const int kind = bc[(dir * 2 + side) * nf + f];
+                    if (kind == BD_NONE) continue;
+                    if (kind == BD_PER) {
+                        if (side == 1) continue;
+                        bp.q[bp.nf] = out[f]; bp.kind[bp.nf] = kind; bp.nf++;
+                    } else {
+                        bo.q[bo.nf] = out[f]; bo.kind[bo.nf] = kind; bo.nf++;
+                    }
+                }
+                const double *dm = dir == 0 ? dxm : (dir == 1 ? dym : dzm);
+                if (bp.nf) {
+                    bp.dir = dir; bp.mbnd = margin; bp.margin = margin; bp.dm = dm;
+                    bp.ix = ix; bp.jx = jx; bp.kx = kx;
+                    HIPCHK(launch_bnd(bp, s));
+                }
+                if (bo.nf) {
+                    bo.dir = dir; bo.mbnd = side; bo.margin = margin; bo.dm = dm;
+                    bo.ix = ix; bo.jx = jx; bo.kx = kx;
+                    HIPCHK(launch_bnd(bo, s));
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -30,6 +30,18 @@ CANS_DEV StepCtl clock_enter(Clock *c, int n, bool leader) {
     return StepCtl{run, dt};
 }
 
+// For kernels that run after the clock kernel of step n: its decision.
+CANS_DEV StepCtl clock_peek(const Clock *c, int n) {
+    const int d = (n + 1) & 1;
+    return StepCtl{c->stopped[d] == 0, c->dt_last[d]};
+}
+
+// Step n's run decision and dt, for the engine kernel of the step: advance
+// the clock (leader block) unless a clock kernel of the step already did.
+CANS_DEV StepCtl clock_use(Clock *c, int n, bool peek, bool leader) {
+    return peek ? clock_peek(c, n) : clock_enter(c, n, leader);
+}
+
 // Read-only view of the same decision, for kernels of step n other than the
 // one that carries the clock (the separate cfl pass, boundary passes).
 CANS_DEV bool clock_live(const Clock *c, int n) {

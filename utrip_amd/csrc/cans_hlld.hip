@@ -1,0 +1,309 @@
+// cans_hlld.hip -- HLLD MUSCL-Hancock MHD engine (cansxx-nosupport/hdhlld,
+// Miyoshi & Kusano 2005; mu units) on gfx950, FP64 (bit-exact) and FP32-flux
+// mixed precision.
+//
+// engine.f:2-73 as kernels: conserved state, then per stage and direction a
+// face-flux kernel (inp_const / inp_tvdminmod + hlld_flux.f90) and an
+// accumulation kernel (adddfdx.f), then the primitive recovery.  Each face
+// flux is computed once; the state accumulates in FP64.
+#include "cans_common.hpp"
+#include "cans_clock.hpp"
+#include "cans_roe.hpp"
+#include "cans_roeflux.hpp"
+
+namespace cans {
+namespace {
+
+enum { P_RO, P_PR, P_VX, P_VY, P_VZ, P_BX, P_BY, P_BZ };
+enum { U_RO, U_EE, U_RX, U_RY, U_RZ, U_BX, U_BY, U_BZ };
+
+CANS_DEV bool hstep(const HlldArgs &a, double &dt) {
+    if (!a.clk) { dt = a.dt; return true; }
+    const int d = (a.step + 1) & 1;
+    if (a.clk->stopped[d]) return false;
+    dt = a.clk->dt_last[d];
+    return true;
+}
+
+// hdhlld/hlld_flux.f90:51-280 for one face; o = fro, fee, frx, fry, frz, fby, fbz
+template <class T>
+CANS_DEV void hlld_face(T gm, T zfloor, T bxs, T rol, T prl, T vxl, T vyl, T vzl, T byl, T bzl, T ror, T prr,
+                        T vxr, T vyr, T vzr, T byr, T bzr, T *o) {
+    const T one = T(1), half = T(0.5), zero = T(0), two = T(2);
+    const T bxsq = bxs * bxs;
+    const T pbl = half * (bxs * bxs + byl * byl + bzl * bzl);
+    const T ptl = prl + pbl;
+    const T eel = prl / (gm - one) + half * rol * (vxl * vxl + vyl * vyl + vzl * vzl) + pbl;
+    const T rxl = rol * vxl, ryl = rol * vyl, rzl = rol * vzl;
+    const T pbr = half * (bxs * bxs + byr * byr + bzr * bzr);
+    const T ptr = prr + pbr;
+    const T eer = prr / (gm - one) + half * (ror * (vxr * vxr + vyr * vyr + vzr * vzr)) + pbr;
+    const T rxr = ror * vxr, ryr = ror * vyr, rzr = ror * vzr;
+    const T gmpl = gm * prl, gmpr = gm * prr;
+    const T gpbl = gmpl + two * pbl, gpbr = gmpr + two * pbr;
+    const T cfl = tsqrt((gpbl + tsqrt(gpbl * gpbl - T(4) * gmpl * bxsq)) / (two * rol));
+    const T cfr = tsqrt((gpbr + tsqrt(gpbr * gpbr - T(4) * gmpr * bxsq)) / (two * ror));
+    const T sl = fminF(vxl, vxr) - fmaxF(cfl, cfr);
+    const T sr = fmaxF(vxl, vxr) + fmaxF(cfl, cfr);
+    const T frol = rxl, frxl = rxl * vxl + ptl - bxsq, fryl = rol * vxl * vyl - bxs * byl;
+    const T frzl = rol * vxl * vzl - bxs * bzl;
+    const T feel = vxl * (eel + ptl - bxsq) - bxs * (vyl * byl + vzl * bzl);
+    const T fbyl = byl * vxl - bxs * vyl, fbzl = bzl * vxl - bxs * vzl;
+    const T fror = rxr, frxr = rxr * vxr + ptr - bxsq, fryr = ror * vxr * vyr - bxs * byr;
+    const T frzr = ror * vxr * vzr - bxs * bzr;
+    const T feer = vxr * (eer + ptr - bxsq) - bxs * (vyr * byr + vzr * bzr);
+    const T fbyr = byr * vxr - bxs * vyr, fbzr = bzr * vxr - bxs * vzr;
+    if (sl >= zero) {
+        o[0] = frol; o[1] = feel; o[2] = frxl; o[3] = fryl; o[4] = frzl; o[5] = fbyl; o[6] = fbzl;
+        return;
+    }
+    if (sr <= zero) {
+        o[0] = fror; o[1] = feer; o[2] = frxr; o[3] = fryr; o[4] = frzr; o[5] = fbyr; o[6] = fbzr;
+        return;
+    }
+    const T sdl = sl - vxl, sdr = sr - vxr;
+    const T den = sdr * ror - sdl * rol;
+    const T sm = (sdr * ror * vxr - sdl * rol * vxl - ptr + ptl) / den;
+    const T sdml = sl - sm, sdmr = sr - sm;
+    const T ptst = (sdr * ror * ptl - sdl * rol * ptr + rol * ror * sdr * sdl * (vxr - vxl)) / den;
+    // left star state
+    T tf = (rol * sdl * sdml - bxsq);
+    T s3 = signF(one, tf + zfloor) * signF(one, tf - zfloor);
+    T mx = fmaxF(zero, s3), mn = fminF(zero, s3);
+    T tmp = bxs * (sdl - sdml) / (tf + mn);
+    const T rolst = mx * (rol * sdl / sdml) - mn * rol;
+    const T vxlst = mx * (sm) - mn * vxl;
+    const T rxlst = rolst * vxlst;
+    const T vylst = mx * (vyl - byl * tmp) - mn * vyl;
+    const T rylst = rolst * vylst;
+    const T vzlst = mx * (vzl - bzl * tmp) - mn * vzl;
+    const T rzlst = rolst * vzlst;
+    tmp = (rol * (sdl * sdl) - bxsq) / (tf + mn);
+    const T bylst = mx * (byl * tmp) - mn * byl;
+    const T bzlst = mx * (bzl * tmp) - mn * bzl;
+    const T vdbstl = (vxlst * bxs + vylst * bylst + vzlst * bzlst);
+    const T eelst = mx * ((sdl * eel - ptl * vxl + ptst * sm + bxs * (vxl * bxs + vyl * byl + vzl * bzl - vdbstl)) / sdml)
+                  - mn * eel;
+    // right star state
+    tf = (ror * sdr * sdmr - bxsq);
+    s3 = signF(one, tf + zfloor) * signF(one, tf - zfloor);
+    mx = fmaxF(zero, s3);
+    mn = fminF(zero, s3);
+    tmp = bxs * (sdr - sdmr) / (tf + mn);
+    const T rorst = mx * (ror * sdr / sdmr) - mn * ror;
+    const T vxrst = mx * (sm) - mn * vxr;
+    const T rxrst = rorst * vxrst;
+    const T vyrst = mx * (vyr - byr * tmp) - mn * vyr;
+    const T ryrst = rorst * vyrst;
+    const T vzrst = mx * (vzr - bzr * tmp) - mn * vzr;
+    const T rzrst = rorst * vzrst;
+    tmp = (ror * (sdr * sdr) - bxsq) / (tf + mn);
+    const T byrst = mx * (byr * tmp) - mn * byr;
+    const T bzrst = mx * (bzr * tmp) - mn * bzr;
+    const T vdbstr = (vxrst * bxs + vyrst * byrst + vzrst * bzrst);
+    const T eerst = mx * ((sdr * eer - ptr * vxr + ptst * sm + bxs * (vxr * bxs + vyr * byr + vzr * bzr - vdbstr)) / sdmr)
+                  - mn * eer;
+    const T sqrtrol = tsqrt(rolst), sqrtror = tsqrt(rorst);
+    const T slst = sm - tabs(bxs) / sqrtrol;
+    const T srst = sm + tabs(bxs) / sqrtror;
+    T roldst, rxldst, ryldst, rzldst, eeldst, byldst, bzldst;
+    T rordst, rxrdst, ryrdst, rzrdst, eerdst, byrdst, bzrdst;
+    if (bxs == zero) {
+        roldst = rolst; rxldst = rxlst; ryldst = rylst; rzldst = rzlst; eeldst = eelst; byldst = bylst; bzldst = bzlst;
+        rordst = rorst; rxrdst = rxrst; ryrdst = ryrst; rzrdst = rzrst; eerdst = eerst; byrdst = byrst; bzrdst = bzrst;
+    } else {
+        const T invsumro = one / (sqrtrol + sqrtror);
+        const T signbx = signF(one, bxs);
+        roldst = rolst; rordst = rorst; rxldst = rxlst; rxrdst = rxrst;
+        T temp = invsumro * (sqrtrol * vylst + sqrtror * vyrst + signbx * (byrst - bylst));
+        const T vyldst = temp;
+        ryldst = roldst * temp; ryrdst = rordst * temp;
+        temp = invsumro * (sqrtrol * vzlst + sqrtror * vzrst + signbx * (bzrst - bzlst));
+        const T vzldst = temp;
+        rzldst = roldst * temp; rzrdst = rordst * temp;
+        temp = invsumro * (sqrtrol * byrst + sqrtror * bylst + signbx * sqrtrol * sqrtror * (vyrst - vylst));
+        byldst = temp; byrdst = temp;
+        temp = invsumro * (sqrtrol * bzrst + sqrtror * bzlst + signbx * sqrtrol * sqrtror * (vzrst - vzlst));
+        bzldst = temp; bzrdst = temp;
+        temp = sm * bxs + vyldst * byldst + vzldst * bzldst;
+        eeldst = eelst - sqrtrol * signbx * (vdbstl - temp);
+        eerdst = eerst + sqrtror * signbx * (vdbstr - temp);
+    }
+    if (slst >= zero) {
+        o[0] = frol + sl * (rolst - rol); o[2] = frxl + sl * (rxlst - rxl); o[3] = fryl + sl * (rylst - ryl);
+        o[4] = frzl + sl * (rzlst - rzl); o[1] = feel + sl * (eelst - eel); o[5] = fbyl + sl * (bylst - byl);
+        o[6] = fbzl + sl * (bzlst - bzl);
+    } else if (sm >= zero) {
+        const T temp = slst - sl;
+        o[0] = frol - sl * rol - temp * rolst + slst * roldst;
+        o[2] = frxl - sl * rxl - temp * rxlst + slst * rxldst;
+        o[3] = fryl - sl * ryl - temp * rylst + slst * ryldst;
+        o[4] = frzl - sl * rzl - temp * rzlst + slst * rzldst;
+        o[1] = feel - sl * eel - temp * eelst + slst * eeldst;
+        o[5] = fbyl - sl * byl - temp * bylst + slst * byldst;
+        o[6] = fbzl - sl * bzl - temp * bzlst + slst * bzldst;
+    } else if (srst > zero) {
+        const T temp = srst - sr;
+        o[0] = fror - sr * ror - temp * rorst + srst * rordst;
+        o[2] = frxr - sr * rxr - temp * rxrst + srst * rxrdst;
+        o[3] = fryr - sr * ryr - temp * ryrst + srst * ryrdst;
+        o[4] = frzr - sr * rzr - temp * rzrst + srst * rzrdst;
+        o[1] = feer - sr * eer - temp * eerst + srst * eerdst;
+        o[5] = fbyr - sr * byr - temp * byrst + srst * byrdst;
+        o[6] = fbzr - sr * bzr - temp * bzrst + srst * bzrdst;
+    } else {
+        o[0] = fror + sr * (rorst - ror); o[2] = frxr + sr * (rxrst - rxr); o[3] = fryr + sr * (ryrst - ryr);
+        o[4] = frzr + sr * (rzrst - rzr); o[1] = feer + sr * (eerst - eer); o[5] = fbyr + sr * (byrst - byr);
+        o[6] = fbzr + sr * (bzrst - bzr);
+    }
+}
+
+CANS_DEV double flmtH(double a, double b) {
+    return fmaxF(0.0, fminF(b * signF(1.0, a), __builtin_fabs(a))) * signF(1.0, a);
+}
+
+// conserved state of every cell: vvtorv.f, pr2eh.f, eh2ee.f; into u1 and u2
+__global__ __launch_bounds__(256) void hlld_cons(HlldArgs a) {
+    double dt;
+    if (!hstep(a, dt)) return;
+    const size_t N = (size_t)a.ix * a.jx * a.kx;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const double ro = a.prim[P_RO][p], pr = a.prim[P_PR][p], vx = a.prim[P_VX][p], vy = a.prim[P_VY][p];
+    const double vz = a.prim[P_VZ][p], bx = a.prim[P_BX][p], by = a.prim[P_BY][p], bz = a.prim[P_BZ][p];
+    const double eh = pr / (a.gm - 1.0);
+    const double ek = (vx * vx + vy * vy + vz * vz) * ro / 2.0;
+    const double em = (bx * bx + by * by + bz * bz) / (2.0 * a.mu);
+    const double U[8] = {ro, eh + ek + em, vx * ro, vy * ro, vz * ro, bx, by, bz};
+    for (int q = 0; q < 8; q++) { a.u1[q][p] = U[q]; a.u2[q][p] = U[q]; }
+}
+
+// faces along `dir` with index [1, n_dir-1], every transverse index
+template <class T>
+__global__ __launch_bounds__(256) void hlld_flux_kernel(HlldArgs a, int dir, int stage) {
+    double dt;
+    if (!hstep(a, dt)) return;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int n[3] = {ix, jx, kx};
+    int cnt[3] = {ix, jx, kx};
+    cnt[dir] = n[dir] - 1;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)cnt[0] * cnt[1] * cnt[2]) return;
+    const int i = 1 + (int)(t % cnt[0]);
+    const int j = 1 + (int)((t / cnt[0]) % cnt[1]);
+    const int k = 1 + (int)(t / ((size_t)cnt[0] * cnt[1]));
+    const int idx[3] = {i, j, k};
+    const size_t p = (size_t)(i - 1) + (size_t)ix * ((size_t)(j - 1) + (size_t)jx * (size_t)(k - 1));
+    const size_t s = dir == 0 ? 1 : (dir == 1 ? (size_t)ix : (size_t)ix * jx);
+    double *const *P = stage == 1 ? a.prim : a.half;
+    const int aidx = idx[dir];
+    const bool muscl = stage == 2 && aidx >= 2 && aidx <= n[dir] - 2;   // inp_tvdminmod.f
+    T L[8], R[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const double *da = P[q];
+        if (!muscl) {   // inp_const.f (and the end faces of inp_tvdminmod.f)
+            L[q] = T(da[p]);
+            R[q] = T(da[p + s]);
+        } else {
+            const double d0 = da[p - s], d1 = da[p], d2 = da[p + s], d3 = da[p + 2 * s];
+            L[q] = T(d1 + 0.5 * flmtH(d1 - d0, d2 - d1));
+            R[q] = T(d2 - 0.5 * flmtH(d3 - d2, d2 - d1));
+        }
+    }
+    const int bn = P_BX + dir;
+    const T bb = T((P[bn][p] + P[bn][p + s]) / 2.0);   // hlld_mh.f:47/65/82
+    const int vn = P_VX + dir, v1 = P_VX + (dir + 1) % 3, v2 = P_VX + (dir + 2) % 3;
+    const int b1 = P_BX + (dir + 1) % 3, b2 = P_BX + (dir + 2) % 3;
+    T o[7];
+    hlld_face<T>(T(a.gm), T(1.0e-10), bb, L[P_RO], L[P_PR], L[vn], L[v1], L[v2], L[b1], L[b2], R[P_RO], R[P_PR],
+                 R[vn], R[v1], R[v2], R[b1], R[b2], o);
+    double *const *F = a.flux;
+    F[U_RO][p] = double(o[0]);
+    F[U_EE][p] = double(o[1]);
+    F[U_RX + dir][p] = double(o[2]);
+    F[U_RX + (dir + 1) % 3][p] = double(o[3]);
+    F[U_RX + (dir + 2) % 3][p] = double(o[4]);
+    F[U_BX + (dir + 1) % 3][p] = double(o[5]);
+    F[U_BX + (dir + 2) % 3][p] = double(o[6]);
+    F[U_BX + dir][p] = 0.0;   // normal-field flux, hlld_mh.f:56-94
+}
+
+// common/adddfdx.f: cells [2, n_dir-1] along dir, every transverse index
+__global__ __launch_bounds__(256) void hlld_add_kernel(HlldArgs a, int dir, int stage) {
+    double dt;
+    if (!hstep(a, dt)) return;
+    const double dts = stage == 1 ? 0.5 * dt : dt;   // engine.f: dth = 0.5d0*dt
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int n[3] = {ix, jx, kx};
+    int cnt[3] = {ix, jx, kx};
+    cnt[dir] = n[dir] - 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)cnt[0] * cnt[1] * cnt[2]) return;
+    int idx[3];
+    idx[0] = 1 + (int)(t % cnt[0]);
+    idx[1] = 1 + (int)((t / cnt[0]) % cnt[1]);
+    idx[2] = 1 + (int)(t / ((size_t)cnt[0] * cnt[1]));
+    idx[dir] += 1;
+    const size_t p = (size_t)(idx[0] - 1) + (size_t)ix * ((size_t)(idx[1] - 1) + (size_t)jx * (size_t)(idx[2] - 1));
+    const size_t s = dir == 0 ? 1 : (dir == 1 ? (size_t)ix : (size_t)ix * jx);
+    const double dd = (dir == 0 ? a.dx : (dir == 1 ? a.dy : a.dz))[idx[dir] - 1];
+    double *const *u = stage == 1 ? a.u1 : a.u2;
+#pragma unroll
+    for (int q = 0; q < 8; q++) u[q][p] = u[q][p] - dts * ((a.flux[q][p] - a.flux[q][p - s]) / dd);
+}
+
+// rvtovv.f, ee2eh.f, eh2pr.f: primitives of every cell from u (stage 1 -> half,
+// stage 2 -> prim)
+__global__ __launch_bounds__(256) void hlld_prim(HlldArgs a, int stage) {
+    double dt;
+    if (!hstep(a, dt)) return;
+    const size_t N = (size_t)a.ix * a.jx * a.kx;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    double *const *u = stage == 1 ? a.u1 : a.u2;
+    double *const *P = stage == 1 ? a.half : a.out;
+    const double ro = u[U_RO][p];
+    const double vx = u[U_RX][p] / ro;
+    const double vy = u[U_RY][p] / ro;
+    const double vz = u[U_RZ][p] / ro;
+    const double bx = u[U_BX][p], by = u[U_BY][p], bz = u[U_BZ][p];
+    const double ek = (vx * vx + vy * vy + vz * vz) * ro / 2.0;
+    const double em = (bx * bx + by * by + bz * bz) / (2.0 * a.mu);
+    const double eh = u[U_EE][p] - ek - em;
+    P[P_RO][p] = ro;
+    P[P_PR][p] = (a.gm - 1.0) * eh;
+    P[P_VX][p] = vx;
+    P[P_VY][p] = vy;
+    P[P_VZ][p] = vz;
+    P[P_BX][p] = bx;
+    P[P_BY][p] = by;
+    P[P_BZ][p] = bz;
+}
+
+template <class T>
+hipError_t launch_hlld_t(const HlldArgs &a, hipStream_t s) {
+    const size_t N = (size_t)a.ix * a.jx * a.kx;
+    const int n[3] = {a.ix, a.jx, a.kx};
+    const unsigned gN = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(hlld_cons, dim3(gN), dim3(256), 0, s, a);
+    for (int stage = 1; stage <= 2; stage++) {
+        for (int dir = 0; dir < 3; dir++) {
+            if (a.mfdim[dir] != 1) continue;
+            const size_t nf = N / n[dir] * (size_t)(n[dir] - 1);
+            const size_t na = N / n[dir] * (size_t)(n[dir] - 2);
+            if (nf) hipLaunchKernelGGL(hlld_flux_kernel<T>, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, a, dir, stage);
+            if (na) hipLaunchKernelGGL(hlld_add_kernel, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, a, dir, stage);
+        }
+        hipLaunchKernelGGL(hlld_prim, dim3(gN), dim3(256), 0, s, a, stage);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_hlld(const HlldArgs &a, bool mixed, hipStream_t s) {
+    return mixed ? launch_hlld_t<float>(a, s) : launch_hlld_t<double>(a, s);
+}
+
+}  // namespace cans

@@ -53,6 +53,7 @@ struct Mlw2dArgs {
     int ix, jx;
     int per_margin;    // > 0: also write the periodic ghost images (width)
     double safety;     // > 0 with clk: fused cfl_m of the new state
+    int peek;          // 1: a clock kernel of this step already ran (read, do not advance)
 };
 
 struct Mlw1dArgs {
@@ -64,6 +65,7 @@ struct Mlw1dArgs {
     Clock *clk;
     int step;
     int ix;
+    int peek;          // 1: a clock kernel of this step already ran (read, do not advance)
 };
 
 // Boundary kinds (the CANS bc/ routines)

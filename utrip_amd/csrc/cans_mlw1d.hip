@@ -30,7 +30,7 @@ __global__ __launch_bounds__(NT) void mlw1d_kernel(Mlw1dArgs a) {
 
     double dt = a.dt;
     if (a.clk) {
-        StepCtl c = clock_enter(a.clk, a.step, blockIdx.x == 0 && tid == 0);
+        StepCtl c = clock_use(a.clk, a.step, a.peek != 0, blockIdx.x == 0 && tid == 0);
         if (!c.run) return;
         dt = c.dt;
     }

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(NT) void mlw2d_kernel(Mlw2dArgs a) {
     double dt = a.dt;
     bool run = true;
     if (a.clk) {
-        StepCtl c = clock_enter(a.clk, a.step, blockIdx.x == 0 && tid == 0);
+        StepCtl c = clock_use(a.clk, a.step, a.peek != 0, blockIdx.x == 0 && tid == 0);
         run = c.run;
         dt = c.dt;
     }

@@ -1,0 +1,283 @@
+// cans_mlw3d.hip -- 3D modified Lax-Wendroff MHD step (cans3d/hdmlw/mlw_m.f)
+// on gfx950, bit-exact FP64.
+//
+// Three kernels over SoA scratch in HBM (the 3D stencil's working set -- 8
+// fields, 15 distinct cell fluxes, 18 corner fluxes -- does not fit a useful
+// LDS tile at FP64):
+//   cells    conserved variables, E = -v x B and the cell-centred fluxes
+//            (mlw_m.f:70-192 flux loops), 23 values per cell;
+//   corners  8-point half step (mlwhalf.f:38-68), half-step primitives and
+//            the corner fluxes (mlw_m.f:193-315), 18 values per corner;
+//   update   centre term of mlwhalf, mlwfull.f:34-55, mlwartv.f:36-48,
+//            update and primitive recovery (mlw_m.f:316-453).
+#include "cans_common.hpp"
+#include "cans_clock.hpp"
+#include "cans_roe.hpp"
+
+namespace cans {
+namespace {
+
+constexpr double pi8i = 1.0 / kPi / 8.0;
+constexpr double pi4i = 1.0 / kPi / 4.0;
+
+// cell scratch slots
+enum { C_RO, C_EE, C_RX, C_RY, C_RZ, C_BX, C_BY, C_BZ,
+       C_EEX, C_EEY, C_EEZ, C_RXX, C_RXY, C_RXZ, C_RYX, C_RYY, C_RYZ, C_RZX, C_RZY, C_RZZ,
+       C_EX, C_EY, C_EZ };
+// corner scratch slots
+enum { H_ROX, H_ROY, H_ROZ, H_EEX, H_EEY, H_EEZ, H_RXX, H_RXY, H_RXZ, H_RYX, H_RYY, H_RYZ,
+       H_RZX, H_RZY, H_RZZ, H_EX, H_EY, H_EZ };
+
+// flux of equation q (ro ee rx ry rz bx by bz) in direction d: scratch slot
+// and sign; slot -1 is an identically zero flux.
+__constant__ int kCellFlux[8][3][2] = {
+    {{C_RX, 1}, {C_RY, 1}, {C_RZ, 1}},
+    {{C_EEX, 1}, {C_EEY, 1}, {C_EEZ, 1}},
+    {{C_RXX, 1}, {C_RXY, 1}, {C_RXZ, 1}},
+    {{C_RYX, 1}, {C_RYY, 1}, {C_RYZ, 1}},
+    {{C_RZX, 1}, {C_RZY, 1}, {C_RZZ, 1}},
+    {{-1, 1}, {C_EZ, 1}, {C_EY, -1}},     // bx: 0, ez, -ey
+    {{C_EZ, -1}, {-1, 1}, {C_EX, 1}},     // by: -ez, 0, ex
+    {{C_EY, 1}, {C_EX, -1}, {-1, 1}},     // bz: ey, -ex, 0
+};
+__constant__ int kCornFlux[8][3][2] = {
+    {{H_ROX, 1}, {H_ROY, 1}, {H_ROZ, 1}},
+    {{H_EEX, 1}, {H_EEY, 1}, {H_EEZ, 1}},
+    {{H_RXX, 1}, {H_RXY, 1}, {H_RXZ, 1}},
+    {{H_RYX, 1}, {H_RYY, 1}, {H_RYZ, 1}},
+    {{H_RZX, 1}, {H_RZY, 1}, {H_RZZ, 1}},
+    {{-1, 1}, {H_EZ, 1}, {H_EY, -1}},
+    {{H_EZ, -1}, {-1, 1}, {H_EX, 1}},
+    {{H_EY, 1}, {H_EX, -1}, {-1, 1}},
+};
+
+CANS_DEV bool step_dt3(const Mlw3dArgs &a, double &dt) {
+    if (!a.clk) { dt = a.dt; return true; }
+    const int d = (a.step + 1) & 1;
+    if (a.clk->stopped[d]) return false;
+    dt = a.clk->dt_last[d];
+    return true;
+}
+
+__global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
+    double dt;
+    if (!step_dt3(a, dt)) return;
+    const size_t N = (size_t)a.ix * a.jx * a.kx;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const double gm = a.gm;
+    const double ro = a.prim[0][p], pr = a.prim[1][p], vx = a.prim[2][p], vy = a.prim[3][p], vz = a.prim[4][p];
+    const double bx = a.prim[5][p], by = a.prim[6][p], bz = a.prim[7][p];
+    double *c = a.cell;
+#define W(slot, v) c[(size_t)(slot) * N + p] = (v)
+    const double vv = vx * vx + vy * vy + vz * vz;
+    const double bb = bx * bx + by * by + bz * bz;
+    W(C_RO, ro);
+    W(C_EE, pr / (gm - 1.0) + 0.5 * ro * vv + pi8i * bb);
+    W(C_RX, ro * vx);
+    W(C_RY, ro * vy);
+    W(C_RZ, ro * vz);
+    W(C_BX, bx);
+    W(C_BY, by);
+    W(C_BZ, bz);
+    const double ex = -vy * bz + vz * by;
+    const double ey = -vz * bx + vx * bz;
+    const double ez = -vx * by + vy * bx;
+    const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
+    W(C_EEX, ep * vx + (bz * ey - by * ez) * pi4i);
+    W(C_EEY, ep * vy + (bx * ez - bz * ex) * pi4i);
+    W(C_EEZ, ep * vz + (by * ex - bx * ey) * pi4i);
+    W(C_RXX, ro * (vx * vx) + pr + pi8i * (by * by + bz * bz - bx * bx));
+    W(C_RXY, ro * vx * vy - pi4i * bx * by);
+    W(C_RXZ, ro * vx * vz - pi4i * bx * bz);
+    W(C_RYX, ro * vy * vx - pi4i * by * bx);
+    W(C_RYY, ro * (vy * vy) + pr + pi8i * (bx * bx + bz * bz - by * by));
+    W(C_RYZ, ro * vy * vz - pi4i * by * bz);
+    W(C_RZX, ro * vz * vx - pi4i * bz * bx);
+    W(C_RZY, ro * vz * vy - pi4i * bz * by);
+    W(C_RZZ, ro * (vz * vz) + pr + pi8i * (bx * bx + by * by - bz * bz));
+    W(C_EX, ex);
+    W(C_EY, ey);
+    W(C_EZ, ez);
+#undef W
+}
+
+CANS_DEV double cflux(const double *c, size_t N, int q, int d, size_t p) {
+    const int slot = kCellFlux[q][d][0];
+    if (slot < 0) return 0.0;
+    const double v = c[(size_t)slot * N + p];
+    return kCellFlux[q][d][1] < 0 ? -v : v;
+}
+CANS_DEV double hflux(const double *h, size_t N, int q, int d, size_t p) {
+    const int slot = kCornFlux[q][d][0];
+    if (slot < 0) return 0.0;
+    const double v = h[(size_t)slot * N + p];
+    return kCornFlux[q][d][1] < 0 ? -v : v;
+}
+
+// corner (i,j,k) between cells i..i+1, j..j+1, k..k+1, i,j,k in [1,n-1]
+__global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
+    double dt;
+    if (!step_dt3(a, dt)) return;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int ci = ix - 1, cj = jx - 1, ck = kx - 1;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ci * cj * ck) return;
+    const int i = 1 + (int)(t % ci);
+    const int j = 1 + (int)((t / ci) % cj);
+    const int k = 1 + (int)(t / ((size_t)ci * cj));
+    const size_t N = (size_t)ix * jx * kx;
+    const size_t sy = ix, sz = (size_t)ix * jx;
+    const size_t p000 = (size_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    const size_t p100 = p000 + 1, p010 = p000 + sy, p110 = p010 + 1;
+    const size_t p001 = p000 + sz, p101 = p001 + 1, p011 = p001 + sy, p111 = p011 + 1;
+    const double *c = a.cell;
+    const double dxim = a.mx.dim[i - 1], dyim = a.my.dim[j - 1], dzim = a.mz.dim[k - 1];
+    double un[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const double *u = c + (size_t)q * N;    // C_RO..C_BZ are slots 0..7
+        const double uh = (u[p100] + u[p000] + u[p110] + u[p010] + u[p101] + u[p001] + u[p111] + u[p011]) / 8.0;
+        const double dfdx = (cflux(c, N, q, 0, p100) - cflux(c, N, q, 0, p000) + cflux(c, N, q, 0, p110)
+                             - cflux(c, N, q, 0, p010) + cflux(c, N, q, 0, p101) - cflux(c, N, q, 0, p001)
+                             + cflux(c, N, q, 0, p111) - cflux(c, N, q, 0, p011)) / 4.0 * dxim;
+        const double dfdy = (cflux(c, N, q, 1, p010) - cflux(c, N, q, 1, p000) + cflux(c, N, q, 1, p110)
+                             - cflux(c, N, q, 1, p100) + cflux(c, N, q, 1, p011) - cflux(c, N, q, 1, p001)
+                             + cflux(c, N, q, 1, p111) - cflux(c, N, q, 1, p101)) / 4.0 * dyim;
+        const double dfdz = (cflux(c, N, q, 2, p001) - cflux(c, N, q, 2, p000) + cflux(c, N, q, 2, p101)
+                             - cflux(c, N, q, 2, p100) + cflux(c, N, q, 2, p011) - cflux(c, N, q, 2, p010)
+                             + cflux(c, N, q, 2, p111) - cflux(c, N, q, 2, p110)) / 4.0 * dzim;
+        un[q] = uh - dt * dfdx - dt * dfdy - dt * dfdz;
+    }
+    const double gm = a.gm;
+    const double roh = un[0], eeh = un[1], rxh = un[2], ryh = un[3], rzh = un[4];
+    const double bxh = un[5], byh = un[6], bzh = un[7];
+    const double vxh = rxh / roh;
+    const double vyh = ryh / roh;
+    const double vzh = rzh / roh;
+    const double vv = vxh * vxh + vyh * vyh + vzh * vzh;
+    const double bb = bxh * bxh + byh * byh + bzh * bzh;
+    const double prh = (gm - 1.0) * (eeh - 0.5 * roh * vv - pi8i * bb);
+    const double exh = -vyh * bzh + vzh * byh;
+    const double eyh = -vzh * bxh + vxh * bzh;
+    const double ezh = -vxh * byh + vyh * bxh;
+    const double ep = prh * gm / (gm - 1.0) + 0.5 * roh * vv;
+    double *h = a.corner;
+    const size_t pc = p000;   // corner (i,j,k) stored at the index of cell (i,j,k)
+#define W(slot, v) h[(size_t)(slot) * N + pc] = (v)
+    W(H_ROX, roh * vxh);
+    W(H_ROY, roh * vyh);
+    W(H_ROZ, roh * vzh);
+    W(H_EEX, ep * vxh + (bzh * eyh - byh * ezh) * pi4i);
+    W(H_EEY, ep * vyh + (bxh * ezh - bzh * exh) * pi4i);
+    W(H_EEZ, ep * vzh + (byh * exh - bxh * eyh) * pi4i);
+    W(H_RXX, roh * (vxh * vxh) + prh + pi8i * (byh * byh + bzh * bzh - bxh * bxh));
+    W(H_RXY, roh * vxh * vyh - pi4i * bxh * byh);
+    W(H_RXZ, roh * vxh * vzh - pi4i * bxh * bzh);
+    W(H_RYX, roh * vyh * vxh - pi4i * byh * bxh);
+    W(H_RYY, roh * (vyh * vyh) + prh + pi8i * (bxh * bxh + bzh * bzh - byh * byh));
+    W(H_RYZ, roh * vyh * vzh - pi4i * byh * bzh);
+    W(H_RZX, roh * vzh * vxh - pi4i * bzh * bxh);
+    W(H_RZY, roh * vzh * vyh - pi4i * bzh * byh);
+    W(H_RZZ, roh * (vzh * vzh) + prh + pi8i * (bxh * bxh + byh * byh - bzh * bzh));
+    W(H_EX, exh);
+    W(H_EY, eyh);
+    W(H_EZ, ezh);
+#undef W
+}
+
+// interior cell (i,j,k) in [2,n-1]^3
+__global__ __launch_bounds__(256) void mlw3d_update(Mlw3dArgs a) {
+    double dt;
+    if (!step_dt3(a, dt)) return;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int ni = ix - 2, nj = jx - 2, nk = kx - 2;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ni * nj * nk) return;
+    const int i = 2 + (int)(t % ni);
+    const int j = 2 + (int)((t / ni) % nj);
+    const int k = 2 + (int)(t / ((size_t)ni * nj));
+    const size_t N = (size_t)ix * jx * kx;
+    const size_t sx = 1, sy = ix, sz = (size_t)ix * jx;
+    const size_t p = (size_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    const double *c = a.cell, *h = a.corner;
+    const double dxi = a.mx.di[i - 1], dyi = a.my.di[j - 1], dzi = a.mz.di[k - 1];
+    const double dxim = a.mx.dim[i - 1], dyim = a.my.dim[j - 1], dzim = a.mz.dim[k - 1];
+    const double dxim1 = a.mx.dim[i - 2], dyim1 = a.my.dim[j - 2], dzim1 = a.mz.dim[k - 2];
+    const double ux0 = a.mx.u0[i - 1], ux1 = a.mx.u1[i - 1];
+    const double uy0 = a.my.u0[j - 1], uy1 = a.my.u1[j - 1];
+    const double uz0 = a.mz.u0[k - 1], uz1 = a.mz.u1[k - 1];
+    // corners around the cell: (i-1|i, j-1|j, k-1|k)
+    const size_t c000 = p - sx - sy - sz;
+    const size_t c100 = c000 + sx, c010 = c000 + sy, c110 = c010 + sx;
+    const size_t c001 = c000 + sz, c101 = c001 + sx, c011 = c001 + sy, c111 = c011 + sx;
+    // artificial viscosity coefficients (mlw_m.f:316-337)
+    const double zero = 0.0;
+    const double *vxp = a.prim[2], *vyp = a.prim[3], *vzp = a.prim[4];
+    const double qxc = a.qav * a.dxm[i - 1] * fmaxF(zero, __builtin_fabs(vxp[p + sx] - vxp[p]) - kR4_1em4);
+    const double qxw = a.qav * a.dxm[i - 2] * fmaxF(zero, __builtin_fabs(vxp[p] - vxp[p - sx]) - kR4_1em4);
+    const double qyc = a.qav * a.dym[j - 1] * fmaxF(zero, __builtin_fabs(vyp[p + sy] - vyp[p]) - kR4_1em4);
+    const double qys = a.qav * a.dym[j - 2] * fmaxF(zero, __builtin_fabs(vyp[p] - vyp[p - sy]) - kR4_1em4);
+    const double qzc = a.qav * a.dzm[k - 1] * fmaxF(zero, __builtin_fabs(vzp[p + sz] - vzp[p]) - kR4_1em4);
+    const double qzb = a.qav * a.dzm[k - 2] * fmaxF(zero, __builtin_fabs(vzp[p] - vzp[p - sz]) - kR4_1em4);
+    double U[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        double du = 0.0;
+        // mlwhalf centre term
+        du = du - 0.5 * dt * (0.5 * dxi * (cflux(c, N, q, 0, p + sx) - cflux(c, N, q, 0, p - sx)))
+                - 0.5 * dt * (0.5 * dyi * (cflux(c, N, q, 1, p + sy) - cflux(c, N, q, 1, p - sy)))
+                - 0.5 * dt * (0.5 * dzi * (cflux(c, N, q, 2, p + sz) - cflux(c, N, q, 2, p - sz)));
+        // mlwfull
+        const double dfdx = dxi * (uz0 * (uy0 * (hflux(h, N, q, 0, c100) - hflux(h, N, q, 0, c000))
+                                          + uy1 * (hflux(h, N, q, 0, c110) - hflux(h, N, q, 0, c010)))
+                                   + uz1 * (uy0 * (hflux(h, N, q, 0, c101) - hflux(h, N, q, 0, c001))
+                                            + uy1 * (hflux(h, N, q, 0, c111) - hflux(h, N, q, 0, c011))));
+        const double dfdy = dyi * (ux0 * (uz0 * (hflux(h, N, q, 1, c010) - hflux(h, N, q, 1, c000))
+                                          + uz1 * (hflux(h, N, q, 1, c011) - hflux(h, N, q, 1, c001)))
+                                   + ux1 * (uz0 * (hflux(h, N, q, 1, c110) - hflux(h, N, q, 1, c100))
+                                            + uz1 * (hflux(h, N, q, 1, c111) - hflux(h, N, q, 1, c101))));
+        const double dfdz = dzi * (uy0 * (ux0 * (hflux(h, N, q, 2, c001) - hflux(h, N, q, 2, c000))
+                                          + ux1 * (hflux(h, N, q, 2, c101) - hflux(h, N, q, 2, c100)))
+                                   + uy1 * (ux0 * (hflux(h, N, q, 2, c011) - hflux(h, N, q, 2, c010))
+                                            + ux1 * (hflux(h, N, q, 2, c111) - hflux(h, N, q, 2, c110))));
+        du = du - 0.5 * dt * dfdx - 0.5 * dt * dfdy - 0.5 * dt * dfdz;
+        // mlwartv
+        const double *u = c + (size_t)q * N;
+        const double uc = u[p];
+        du = du + dt * (dxi * (qxc * dxim * (u[p + sx] - uc) - qxw * dxim1 * (uc - u[p - sx])))
+                + dt * (dyi * (qyc * dyim * (u[p + sy] - uc) - qys * dyim1 * (uc - u[p - sy])))
+                + dt * (dzi * (qzc * dzim * (u[p + sz] - uc) - qzb * dzim1 * (uc - u[p - sz])));
+        U[q] = uc + du;
+    }
+    const double gm = a.gm;
+    const double ro = U[0];
+    const double vx = U[2] / ro;
+    const double vy = U[3] / ro;
+    const double vz = U[4] / ro;
+    const double vv = vx * vx + vy * vy + vz * vz;
+    const double bb = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+    a.out[0][p] = ro;
+    a.out[1][p] = (gm - 1.0) * (U[1] - 0.5 * ro * vv - pi8i * bb);
+    a.out[2][p] = vx;
+    a.out[3][p] = vy;
+    a.out[4][p] = vz;
+    a.out[5][p] = U[5];
+    a.out[6][p] = U[6];
+    a.out[7][p] = U[7];
+}
+
+}  // namespace
+
+hipError_t launch_mlw3d(const Mlw3dArgs &a, hipStream_t s) {
+    if (a.ix < 3 || a.jx < 3 || a.kx < 3) return hipSuccess;
+    const size_t N = (size_t)a.ix * a.jx * a.kx;
+    const size_t nc = (size_t)(a.ix - 1) * (a.jx - 1) * (a.kx - 1);
+    const size_t nu = (size_t)(a.ix - 2) * (a.jx - 2) * (a.kx - 2);
+    hipLaunchKernelGGL(mlw3d_cells, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(mlw3d_corners, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(mlw3d_update, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace cans

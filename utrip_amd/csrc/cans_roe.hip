@@ -1,0 +1,322 @@
+// cans_roe.hip -- Roe + MUSCL-minmod MHD engines (cans{1,2,3}d/hdroe/roe_m.f)
+// on gfx950, FP64 (bit-exact) and FP32-flux mixed precision.
+//
+// One step = first-order Roe fluxes on every face of each direction, the
+// half-step predictor U^{n+1/2} = U + 1/2 dt sum_d (F_{i-1}-F_i)/dx, MUSCL-
+// minmod reconstruction of the half-step primitives, Roe fluxes again, and
+// the corrector U^{n+1} = U + dt sum_d (F_{i-1}-F_i)/dx.  Roe is FP64-VALU
+// bound (~500-700 flops per face), so each face flux is computed exactly
+// once (a flux kernel per direction writing a face array), and the per-cell
+// kernels only combine fluxes.  Arrays are SoA FP64, column-major.
+//
+// Fields: U  = ro, ee, rx, ry, rz, bx, by, bz (8; 2D: no rz/bz; 1D: ro ee rx ry by)
+//         P  = ro, pr, vx, vy, vz, bx, by, bz (primitives, same slots)
+#include "cans_common.hpp"
+#include "cans_clock.hpp"
+#include "cans_kernels.hpp"
+#include "cans_roe.hpp"
+#include "cans_roeflux.hpp"
+
+namespace cans {
+namespace {
+
+// slot layout shared by U and P
+enum { S_RO = 0, S_E = 1, S_X = 2, S_Y = 3, S_Z = 4, S_BX = 5, S_BY = 6, S_BZ = 7 };
+
+CANS_DEV size_t lin(int i, int j, int k, int ix, int jx) {
+    return (size_t)(i - 1) + (size_t)ix * ((size_t)(j - 1) + (size_t)jx * (size_t)(k - 1));
+}
+
+template <int D> struct RoePi2 {};
+
+CANS_DEV double flmt(double a, double b) {   // tvdminmod.f statement function
+    return fmaxF(0.0, fminF(b * signF(1.0, a), __builtin_fabs(a))) * signF(1.0, a);
+}
+CANS_DEV float flmt(float a, float b) {
+    return fmaxF(0.0f, fminF(b * signF(1.0f, a), __builtin_fabsf(a))) * signF(1.0f, a);
+}
+
+// Run flag / dt of the step: from the clock (after the step's clock kernel
+// ran), or the by-value dt of a drop-in call.
+CANS_DEV bool step_dt(const RoeArgs &a, double &dt) {
+    if (!a.clk) { dt = a.dt; return true; }
+    const int d = (a.step + 1) & 1;
+    if (a.clk->stopped[d]) return false;
+    dt = a.clk->dt_last[d];
+    return true;
+}
+
+// ------------------------------------------------------------------ fluxes
+// Face (i,j,k) in direction `dir` lies between cell (i,j,k) and the next
+// cell along `dir`.  stage 1: first-order states from P; stage 2: MUSCL-
+// minmod states from the half-step primitives Ph.  The face range is what
+// the following cell kernel reads: along dir [lo_n, n-1 or n-2], across
+// [lo_t, n+1-lo_t].
+template <int D, class T>
+__global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int stage) {
+    double dtv;
+    if (!step_dt(a, dtv)) return;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int n[3] = {ix, jx, kx};
+    const int lo_n = stage == 1 ? 1 : 2, hi_n = stage == 1 ? n[dir] - 1 : n[dir] - 2;
+    const int lo_t = stage == 1 ? 2 : 3;
+    int lo[3], cnt[3];
+    for (int d = 0; d < 3; d++) {
+        if (d >= D) { lo[d] = 1; cnt[d] = 1; continue; }
+        if (d == dir) { lo[d] = lo_n; cnt[d] = hi_n - lo_n + 1; }
+        else { lo[d] = lo_t; cnt[d] = n[d] + 1 - 2 * lo_t + 1; }
+        if (cnt[d] < 0) cnt[d] = 0;
+    }
+    const size_t total = (size_t)cnt[0] * cnt[1] * cnt[2];
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int i = lo[0] + (int)(t % cnt[0]);
+    const int j = lo[1] + (int)((t / cnt[0]) % cnt[1]);
+    const int k = lo[2] + (int)(t / ((size_t)cnt[0] * cnt[1]));
+    const size_t p = lin(i, j, k, ix, jx);
+    const size_t s = dir == 0 ? 1 : (dir == 1 ? (size_t)ix : (size_t)ix * jx);
+    const double *const *P = stage == 1 ? a.prim : a.half;
+    // left / right states of the needed slots
+    constexpr int NS = (D == 3) ? 8 : (D == 2 ? 6 : 5);
+    // slots used: 1D: ro pr vx vy by ; 2D: + bx ; 3D: all
+    const int slots3[8] = {S_RO, S_E, S_X, S_Y, S_Z, S_BX, S_BY, S_BZ};
+    const int slots2[6] = {S_RO, S_E, S_X, S_Y, S_BX, S_BY};
+    const int slots1[5] = {S_RO, S_E, S_X, S_Y, S_BY};
+    T L[8], R[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) { L[q] = T(0); R[q] = T(0); }
+#pragma unroll
+    for (int m = 0; m < NS; m++) {
+        const int q = D == 3 ? slots3[m] : (D == 2 ? slots2[m] : slots1[m]);
+        const double *da = P[q];
+        if (stage == 1) {
+            L[q] = T(da[p]);
+            R[q] = T(da[p + s]);
+        } else if (D == 3) {   // cans3d/hdroe/tvdminmod.f:15-55
+            const double d0 = da[p - s], d1 = da[p], d2 = da[p + s], d3 = da[p + 2 * s];
+            L[q] = T(d1 + 0.5 * flmt(d1 - d0, d2 - d1));
+            R[q] = T(d2 - 0.5 * flmt(d3 - d2, d2 - d1));
+        } else {               // cans{1,2}d/hdroe/tvdminmod.f
+            const double d0 = da[p - s], d1 = da[p], d2 = da[p + s], d3 = da[p + 2 * s];
+            L[q] = T(d1 + 0.5 * flmt(d2 - d1, d1 - d0));
+            R[q] = T(d2 - 0.5 * flmt(d2 - d1, d3 - d2));
+        }
+    }
+    if (D == 1) {   // the face field is bxm(i) on both sides in both stages
+        L[S_BX] = T(a.bxm[i - 1]);
+        R[S_BX] = T(a.bxm[i - 1]);
+    }
+    const T gm = T(a.gm);
+    double *const *F = a.flux[dir];
+    if (D == 3) {
+        // sweep frame: (n, t1, t2) = (x,y,z), (y,z,x), (z,x,y)
+        const int vn = S_X + dir, v1 = S_X + (dir + 1) % 3, v2 = S_X + (dir + 2) % 3;
+        const int bn = S_BX + dir, b1 = S_BX + (dir + 1) % 3, b2 = S_BX + (dir + 2) % 3;
+        T o[7];
+        roeflux3<T>(gm, L[S_RO], L[S_E], L[vn], L[v1], L[v2], L[bn], L[b1], L[b2], R[S_RO], R[S_E], R[vn], R[v1],
+                    R[v2], R[bn], R[b1], R[b2], o);
+        F[S_RO][p] = double(o[0]); F[S_E][p] = double(o[1]);
+        F[vn][p] = double(o[2]); F[v1][p] = double(o[3]); F[v2][p] = double(o[4]);
+        F[b1][p] = double(o[5]); F[b2][p] = double(o[6]);
+    } else {
+        // 2D x: (vx,vy,bx,by) -> ro ee rx ry by ; 2D y: (vy,vx,by,bx) -> ro ee ry rx bx ; 1D as 2D x
+        const int vn = dir == 0 ? S_X : S_Y, vt = dir == 0 ? S_Y : S_X;
+        const int bn = dir == 0 ? S_BX : S_BY, bt = dir == 0 ? S_BY : S_BX;
+        T o[5];
+        roeflux12<T>(gm, L[S_RO], L[S_E], L[vn], L[vt], L[bn], L[bt], R[S_RO], R[S_E], R[vn], R[vt], R[bn], R[bt], o);
+        F[S_RO][p] = double(o[0]); F[S_E][p] = double(o[1]);
+        F[vn][p] = double(o[2]); F[vt][p] = double(o[3]); F[bt][p] = double(o[4]);
+    }
+}
+
+// conserved variables of a cell from the primitives, in the reference's
+// expressions (roe_m.f: rx=ro*vx, ee=pr/(gm-1)+0.5*ro*v2+pi8i*b2)
+template <int D>
+CANS_DEV void prim_to_cons(const RoeArgs &a, size_t p, int i, double *U) {
+    const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
+    const double gm = a.gm;
+    const double ro = a.prim[S_RO][p], pr = a.prim[S_E][p], vx = a.prim[S_X][p], vy = a.prim[S_Y][p];
+    U[S_RO] = ro;
+    U[S_X] = ro * vx;
+    U[S_Y] = ro * vy;
+    if (D == 3) {
+        const double vz = a.prim[S_Z][p], bx = a.prim[S_BX][p], by = a.prim[S_BY][p], bz = a.prim[S_BZ][p];
+        U[S_Z] = ro * vz;
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        const double b2 = bx * bx + by * by + bz * bz;
+        U[S_E] = pr / (gm - 1.0) + 0.5 * ro * v2 + pi8i * b2;
+        U[S_BX] = bx; U[S_BY] = by; U[S_BZ] = bz;
+    } else {
+        const double bx = D == 1 ? a.bx[i - 1] : a.prim[S_BX][p], by = a.prim[S_BY][p];
+        const double v2 = vx * vx + vy * vy;
+        const double b2 = bx * bx + by * by;
+        U[S_E] = pr / (gm - 1.0) + 0.5 * ro * v2 + pi8i * b2;
+        U[S_BX] = bx; U[S_BY] = by;
+    }
+}
+
+// ------------------------------------------------------------------ predictor
+// cells [2,n-1]^D: U + 1/2 dt (F(i-1)-F(i))/dx, sweeps in x, y, z order
+// (each sweep a separate rounding, as roe_m.f); skip the face-normal field.
+// Cells on the outer layer keep their primitives (the reference leaves the
+// half-step there unset; it only reaches ghost cells when margin >= 3).
+template <int D>
+__global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
+    double dt;
+    if (!step_dt(a, dt)) return;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const size_t N = (size_t)ix * jx * kx;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    const int i = (int)(p % ix) + 1;
+    const int j = (int)((p / ix) % jx) + 1;
+    const int k = (int)(p / ((size_t)ix * jx)) + 1;
+    const int idx[3] = {i, j, k};
+    const int n[3] = {ix, jx, kx};
+    bool inner = true;
+    for (int d = 0; d < D; d++) inner = inner && idx[d] >= 2 && idx[d] <= n[d] - 1;
+    constexpr int NS = 8;
+    if (!inner) {
+        for (int q = 0; q < NS; q++)
+            if (a.prim[q]) a.half[q][p] = a.prim[q][p];
+        return;
+    }
+    double U[8];
+    prim_to_cons<D>(a, p, i, U);
+    const double *dd[3] = {a.dx, a.dy, a.dz};
+    for (int d = 0; d < D; d++) {
+        const size_t s = d == 0 ? 1 : (d == 1 ? (size_t)ix : (size_t)ix * jx);
+        const double dl = dd[d][idx[d] - 1];
+        double *const *F = a.flux[d];
+        const int skipB = D == 1 ? S_BX : S_BX + d;    // no normal-field update
+        for (int q = 0; q < 8; q++) {
+            if (q == skipB || !a.prim[q] || (D == 1 && q == S_BX)) continue;
+            if (D < 3 && (q == S_Z || q == S_BZ)) continue;
+            U[q] = U[q] + 0.5 * dt * ((F[q][p - s] - F[q][p]) / dl);
+        }
+    }
+    // half-step primitives (roe_m.f; 1D: b2 with the cell bx, roe_m.f:94-100)
+    const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
+    const double gm = a.gm;
+    const double ro = U[S_RO];
+    const double vx = U[S_X] / ro;
+    const double vy = U[S_Y] / ro;
+    double v2, b2, vz = 0.0;
+    if (D == 3) {
+        vz = U[S_Z] / ro;
+        v2 = vx * vx + vy * vy + vz * vz;
+        b2 = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY] + U[S_BZ] * U[S_BZ];
+    } else {
+        v2 = vx * vx + vy * vy;
+        b2 = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY];
+    }
+    a.half[S_RO][p] = ro;
+    a.half[S_E][p] = (gm - 1.0) * (U[S_E] - 0.5 * ro * v2 - pi8i * b2);
+    a.half[S_X][p] = vx;
+    a.half[S_Y][p] = vy;
+    a.half[S_BY][p] = U[S_BY];
+    if (D >= 2) a.half[S_BX][p] = U[S_BX];
+    if (D == 3) { a.half[S_Z][p] = vz; a.half[S_BZ][p] = U[S_BZ]; }
+}
+
+// ------------------------------------------------------------------ corrector
+// cells [3,n-2]^D: U + dt (F(i-1)-F(i))/dx per sweep, then (2D) the Az
+// source from the half-step state, then primitives -> out arrays.
+template <int D>
+__global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
+    double dt;
+    if (!step_dt(a, dt)) return;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int n[3] = {ix, jx, kx};
+    int cnt[3];
+    for (int d = 0; d < 3; d++) cnt[d] = d < D ? n[d] - 4 : 1;
+    if (cnt[0] <= 0 || cnt[1] <= 0 || cnt[2] <= 0) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)cnt[0] * cnt[1] * cnt[2]) return;
+    const int i = 3 + (int)(t % cnt[0]);
+    const int j = D >= 2 ? 3 + (int)((t / cnt[0]) % cnt[1]) : 1;
+    const int k = D == 3 ? 3 + (int)(t / ((size_t)cnt[0] * cnt[1])) : 1;
+    const int idx[3] = {i, j, k};
+    const size_t p = lin(i, j, k, ix, jx);
+    double U[8];
+    prim_to_cons<D>(a, p, i, U);
+    const double *dd[3] = {a.dx, a.dy, a.dz};
+    for (int d = 0; d < D; d++) {
+        const size_t s = d == 0 ? 1 : (d == 1 ? (size_t)ix : (size_t)ix * jx);
+        const double dl = dd[d][idx[d] - 1];
+        double *const *F = a.flux[d];
+        const int skipB = S_BX + d;
+        for (int q = 0; q < 8; q++) {
+            if (q == skipB || (D == 1 && q == S_BX)) continue;
+            if (D < 3 && (q == S_Z || q == S_BZ)) continue;
+            if (D == 1 && q == S_BX) continue;
+            U[q] = U[q] + dt * ((F[q][p - s] - F[q][p]) / dl);
+        }
+    }
+    if (D == 2) {   // roe_m.f:187-193
+        const double vxh = a.half[S_X][p], vyh = a.half[S_Y][p], bxh = a.half[S_BX][p], byh = a.half[S_BY][p];
+        const double ez = -vxh * byh + vyh * bxh;
+        const double saz = -ez;
+        a.out_az[p] = a.az[p] + dt * saz;
+    }
+    const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
+    const double gm = a.gm;
+    const double ro = U[S_RO];
+    const double vx = U[S_X] / ro;
+    const double vy = U[S_Y] / ro;
+    double v2, b2;
+    if (D == 3) {
+        const double vz = U[S_Z] / ro;
+        v2 = vx * vx + vy * vy + vz * vz;
+        b2 = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY] + U[S_BZ] * U[S_BZ];
+        a.out[S_Z][p] = vz;
+        a.out[S_BZ][p] = U[S_BZ];
+    } else {
+        const double bx = D == 1 ? a.bx[i - 1] : U[S_BX];
+        v2 = vx * vx + vy * vy;
+        b2 = bx * bx + U[S_BY] * U[S_BY];
+    }
+    a.out[S_RO][p] = ro;
+    a.out[S_E][p] = (gm - 1.0) * (U[S_E] - 0.5 * ro * v2 - pi8i * b2);
+    a.out[S_X][p] = vx;
+    a.out[S_Y][p] = vy;
+    a.out[S_BY][p] = U[S_BY];
+    if (D >= 2) a.out[S_BX][p] = U[S_BX];
+}
+
+template <int D, class T>
+hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
+    const int n[3] = {a.ix, a.jx, a.kx};
+    const size_t N = (size_t)a.ix * a.jx * a.kx;
+    for (int stage = 1; stage <= 2; stage++) {
+        for (int dir = 0; dir < D; dir++) {
+            size_t total = 1;
+            for (int d = 0; d < D; d++) {
+                long c;
+                if (d == dir) c = stage == 1 ? n[d] - 1 : n[d] - 3;
+                else c = stage == 1 ? n[d] - 2 : n[d] - 4;
+                total *= (size_t)(c > 0 ? c : 0);
+            }
+            if (total == 0) continue;
+            hipLaunchKernelGGL((roe_flux_kernel<D, T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a,
+                               dir, stage);
+        }
+        if (stage == 1)
+            hipLaunchKernelGGL(roe_half_kernel<D>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
+    }
+    size_t cells = 1;
+    for (int d = 0; d < D; d++) cells *= (size_t)(n[d] - 4 > 0 ? n[d] - 4 : 0);
+    if (cells)
+        hipLaunchKernelGGL(roe_update_kernel<D>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s) {
+    if (dim == 1) return mixed ? launch_roe_t<1, float>(a, s) : launch_roe_t<1, double>(a, s);
+    if (dim == 2) return mixed ? launch_roe_t<2, float>(a, s) : launch_roe_t<2, double>(a, s);
+    return mixed ? launch_roe_t<3, float>(a, s) : launch_roe_t<3, double>(a, s);
+}
+
+}  // namespace cans

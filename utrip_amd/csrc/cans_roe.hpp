@@ -1,0 +1,63 @@
+// cans_roe.hpp -- argument blocks of the multi-kernel engines (Roe-MUSCL,
+// 3D MLW, HLLD).  Internal to the library.
+#pragma once
+#include "cans_kernels.hpp"
+
+namespace cans {
+
+// Roe + MUSCL (cans{1,2,3}d/hdroe/roe_m.f).  Slots: 0 ro, 1 pr(P)/ee(U),
+// 2 vx/rx, 3 vy/ry, 4 vz/rz, 5 bx, 6 by, 7 bz; unused slots are null
+// (2D: 4 and 7; 1D: 4, 5 and 7 -- the 1D field bx is `bx`, the face
+// field `bxm`).
+struct RoeArgs {
+    const double *prim[8];   // state at t^n (primitives)
+    double *half[8];         // scratch: half-step primitives
+    double *flux[3][8];      // scratch: face fluxes per direction (by U slot)
+    double *out[8];          // new primitives, cells [3,n-2]^D
+    const double *az;        // 2D vector potential (in)
+    double *out_az;          // 2D vector potential (out)
+    const double *bx, *bxm;  // 1D only
+    const double *dx, *dy, *dz;
+    double gm, dt;
+    Clock *clk;              // non-null: dt and run flag of step `step`
+    int step;
+    int ix, jx, kx;
+};
+hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s);
+
+// 3D modified Lax-Wendroff (cans3d/hdmlw/mlw_m.f), three kernels with
+// global scratch: cell quantities, corner half-step fluxes, update.
+struct Mlw3dArgs {
+    const double *prim[8];   // ro pr vx vy vz bx by bz (in)
+    double *out[8];          // same (out), interior cells
+    double *cell;            // scratch: 23 doubles per cell (SoA)
+    double *corner;          // scratch: 18 doubles per corner (SoA)
+    const double *dxm, *dym, *dzm;
+    Metric mx, my, mz;
+    double qav, gm, dt;
+    Clock *clk;
+    int step;
+    int ix, jx, kx;
+};
+hipError_t launch_mlw3d(const Mlw3dArgs &a, hipStream_t s);
+constexpr int kMlw3dCellFields = 23;
+constexpr int kMlw3dCornerFields = 18;
+
+// HLLD MUSCL-Hancock (cansxx-nosupport/hdhlld, mu units)
+struct HlldArgs {
+    double *prim[8];         // ro pr vx vy vz bx by bz (in, all cells)
+    double *out[8];          // new primitives, all cells (may alias prim)
+    double *half[8];         // scratch: half-step primitives
+    double *u1[8];           // scratch: conserved state of stage 1
+    double *u2[8];           // scratch: conserved state of stage 2
+    double *flux[8];         // scratch: face fluxes of the current direction
+    const double *dx, *dy, *dz;
+    double gm, mu, dt;
+    int mfdim[3];
+    Clock *clk;
+    int step;
+    int ix, jx, kx;
+};
+hipError_t launch_hlld(const HlldArgs &a, bool mixed, hipStream_t s);
+
+}  // namespace cans
