@@ -70,3 +70,21 @@ def smooth_state_2d(ix, jx, seed=0):
         return np.asfortranarray(off + amp * np.sin(2 * np.pi * (k[n, 0] * X + k[n, 1] * Y) + ph[n]))
     return dict(ro=w(0, 0.3, 1.0), pr=w(1, 0.2, 0.8), vx=w(2, 0.3, 0.0), vy=w(3, 0.3, 0.0),
                 bx=w(4, 1.0, 0.2), by=w(5, 1.0, -0.1), az=w(6, 0.5, 0.0))
+
+
+def smooth_state_3d(ix, jx, kx, seed=0, bscale=1.0):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = np.meshgrid(np.linspace(0, 1, ix), np.linspace(0, 1, jx), np.linspace(0, 1, kx), indexing="ij")
+
+    def w(a, o):
+        kk = rng.uniform(1, 2, 4)
+        return np.asfortranarray(o + a * np.sin(2 * np.pi * (kk[0] * X + kk[1] * Y + kk[2] * Z) + kk[3]))
+    return dict(ro=w(.3, 1), pr=w(.2, .8), vx=w(.3, 0), vy=w(.3, 0), vz=w(.3, 0), bx=w(bscale, .2 * bscale),
+                by=w(bscale, -.1 * bscale), bz=w(bscale, .3 * bscale))
+
+
+def grid3(ix, jx, kx):
+    _, dx, dxm = stretched_grid(max(ix, 3), 1)
+    _, dy, dym = stretched_grid(max(jx, 3), 4)
+    _, dz, dzm = stretched_grid(max(kx, 3), 9)
+    return dx[:ix].copy(), dxm[:ix].copy(), dy[:jx].copy(), dym[:jx].copy(), dz[:kx].copy(), dzm[:kx].copy()

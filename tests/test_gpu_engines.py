@@ -11,7 +11,8 @@ resident runs use margin 3, after which the boundary fill defines every cell.
 import numpy as np
 import pytest
 
-from tests.helpers import assert_bitwise, rel_l1, smooth_state_1d, smooth_state_2d, stretched_grid, to_dev, to_host
+from tests.helpers import (assert_bitwise, grid3, rel_l1, smooth_state_1d, smooth_state_2d, smooth_state_3d,
+                           stretched_grid, to_dev, to_host)
 from utrip_amd import problems as P
 
 pytestmark = pytest.mark.gpu
@@ -23,24 +24,6 @@ F3 = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")
 
 def case(*a):
     print("CASE", *a, flush=True)
-
-
-def smooth_state_3d(ix, jx, kx, seed=0, bscale=1.0):
-    rng = np.random.default_rng(seed)
-    X, Y, Z = np.meshgrid(np.linspace(0, 1, ix), np.linspace(0, 1, jx), np.linspace(0, 1, kx), indexing="ij")
-
-    def w(a, o):
-        kk = rng.uniform(1, 2, 4)
-        return np.asfortranarray(o + a * np.sin(2 * np.pi * (kk[0] * X + kk[1] * Y + kk[2] * Z) + kk[3]))
-    return dict(ro=w(.3, 1), pr=w(.2, .8), vx=w(.3, 0), vy=w(.3, 0), vz=w(.3, 0), bx=w(bscale, .2 * bscale),
-                by=w(bscale, -.1 * bscale), bz=w(bscale, .3 * bscale))
-
-
-def grid3(ix, jx, kx):
-    _, dx, dxm = stretched_grid(max(ix, 3), 1)
-    _, dy, dym = stretched_grid(max(jx, 3), 4)
-    _, dz, dzm = stretched_grid(max(kx, 3), 9)
-    return dx[:ix].copy(), dxm[:ix].copy(), dy[:jx].copy(), dym[:jx].copy(), dz[:kx].copy(), dzm[:kx].copy()
 
 
 def inner(a, lo, dims):
