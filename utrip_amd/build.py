@@ -82,8 +82,16 @@ def build_fortran() -> list[Path]:
             continue
         lib = PKG / f"libcans{dim}_gpu.so"
         _run([FLANG, "-O2", "-fPIC", "-shared", "-J", str(objdir), "-o", str(lib),
-              str(common), str(src), f"-L{PKG}", "-lcans_gpu", f"-Wl,-rpath,$ORIGIN"], cwd=objdir)
+              str(common), str(src), f"-L{PKG}", "-lcans_gpu", "-Wl,-rpath,$ORIGIN"], cwd=objdir)
         out.append(lib)
+        main = FSRC / f"cans{dim}_main.f90"
+        if main.exists():
+            bindir = PKG / "bin"
+            bindir.mkdir(exist_ok=True)
+            exe = bindir / f"cans{dim}_main"
+            _run([FLANG, "-O2", "-J", str(objdir), "-o", str(exe), str(FSRC / "dac_io.f90"), str(main),
+                  f"-L{PKG}", f"-lcans{dim}_gpu", "-lcans_gpu", "-Wl,-rpath,$ORIGIN/.."], cwd=objdir)
+            out.append(exe)
     return out
 
 

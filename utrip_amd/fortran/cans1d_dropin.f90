@@ -1,0 +1,111 @@
+!=======================================================================
+! libcans1d_gpu -- the cans1d hot-path routines with the reference's names
+! and argument lists (cans1d/hdmlw/mlw_m.f:2, cans1d/hdroe/roe_m.f:2,
+! cans1d/common/cfl_m.f:2, cans1d/bc/bd*x.f), computed on the GPU by
+! libcans_gpu.so.  A cans1d main program links this library instead of
+! libcans1d.a and runs unchanged.
+!=======================================================================
+subroutine mlw_m(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: ro(ix), pr(ix), vx(ix), vy(ix), by(ix)
+  double precision, target, intent(in) :: bx(ix), bxm(ix), dx(ix), dxm(ix)
+  double precision, intent(in) :: dt, qav, gm
+  call check(c_mlw1d(dmap(ro, ix), dmap(pr, ix), dmap(vx, ix), dmap(vy, ix), dmap(by, ix), dmap(bx, ix), &
+       dmap(bxm, ix), dt, qav, gm, dmap(dx, ix), dmap(dxm, ix), int(ix, c_int), c_null_ptr), 'mlw_m')
+  call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vy, 1); call dunmap(by, 1)
+  call dunmap(bx, 0); call dunmap(bxm, 0); call dunmap(dx, 0); call dunmap(dxm, 0)
+end subroutine mlw_m
+
+subroutine roe_m(ro, pr, vx, vy, by, bx, bxm, dt, gm, dx, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: ro(ix), pr(ix), vx(ix), vy(ix), by(ix)
+  double precision, target, intent(in) :: bx(ix), bxm(ix), dx(ix)
+  double precision, intent(in) :: dt, gm
+  call check(c_roe1d(dmap(ro, ix), dmap(pr, ix), dmap(vx, ix), dmap(vy, ix), dmap(by, ix), dmap(bx, ix), &
+       dmap(bxm, ix), dt, gm, dmap(dx, ix), int(ix, c_int), c_null_ptr), 'roe_m')
+  call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vy, 1); call dunmap(by, 1)
+  call dunmap(bx, 0); call dunmap(bxm, 0); call dunmap(dx, 0)
+end subroutine roe_m
+
+subroutine cfl_m(dt, safety, dtmin, merr, gm, bx, ro, pr, vx, vy, by, dx, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  integer, intent(out) :: merr
+  double precision, intent(out) :: dt
+  double precision, intent(in) :: safety, dtmin, gm
+  double precision, target, intent(in) :: bx(ix), ro(ix), pr(ix), vx(ix), vy(ix), by(ix), dx(ix)
+  real(c_double) :: d
+  integer(c_int) :: m
+  call check(c_cfl1d(d, safety, dtmin, m, gm, dmap(bx, ix), dmap(ro, ix), dmap(pr, ix), dmap(vx, ix), &
+       dmap(vy, ix), dmap(by, ix), dmap(dx, ix), int(ix, c_int), c_null_ptr), 'cfl_m')
+  call dunmap(bx, 0); call dunmap(ro, 0); call dunmap(pr, 0); call dunmap(vx, 0)
+  call dunmap(vy, 0); call dunmap(by, 0); call dunmap(dx, 0)
+  dt = d
+  merr = m
+end subroutine cfl_m
+
+
+subroutine bdfrex(mbnd, margin, qq, ix)
+  use cans_gpu_iface
+  integer mbnd, margin, ix
+  double precision qq(ix)
+  call bnd1(0, mbnd, margin, qq, 0.0d0, ix)
+end subroutine bdfrex
+
+subroutine bdsppx(mbnd, margin, qq, ix)
+  use cans_gpu_iface
+  integer mbnd, margin, ix
+  double precision qq(ix)
+  call bnd1(1, mbnd, margin, qq, 0.0d0, ix)
+end subroutine bdsppx
+
+subroutine bdspnx(mbnd, margin, qq, ix)
+  use cans_gpu_iface
+  integer mbnd, margin, ix
+  double precision qq(ix)
+  call bnd1(2, mbnd, margin, qq, 0.0d0, ix)
+end subroutine bdspnx
+
+subroutine bdsmpx(mbnd, margin, qq, ix)
+  use cans_gpu_iface
+  integer mbnd, margin, ix
+  double precision qq(ix)
+  call bnd1(3, mbnd, margin, qq, 0.0d0, ix)
+end subroutine bdsmpx
+
+subroutine bdsmnx(mbnd, margin, qq, ix)
+  use cans_gpu_iface
+  integer mbnd, margin, ix
+  double precision qq(ix)
+  call bnd1(4, mbnd, margin, qq, 0.0d0, ix)
+end subroutine bdsmnx
+
+subroutine bdcnsx(mbnd, margin, qq, q0, ix)
+  use cans_gpu_iface
+  integer mbnd, margin, ix
+  double precision qq(ix), q0
+  call bnd1(5, mbnd, margin, qq, q0, ix)
+end subroutine bdcnsx
+
+subroutine bdperx(margin0, margin1, qq, ix)
+  use cans_gpu_iface
+  integer margin0, margin1, ix
+  double precision qq(ix)
+  call bnd1(7, margin0, margin1, qq, 0.0d0, ix)
+end subroutine bdperx
+
+subroutine bdfrdx(mbnd, margin, qq, dxm, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: mbnd, margin, ix
+  double precision, target :: qq(ix)
+  double precision, target, intent(in) :: dxm(ix)
+  call check(c_bnd1d(BD_FRD, int(mbnd, c_int), int(margin, c_int), dmap(qq, ix), 0.0d0, dmap(dxm, ix), &
+       int(ix, c_int), c_null_ptr), 'bdfrdx')
+  call dunmap(qq, 1); call dunmap(dxm, 0)
+end subroutine bdfrdx
