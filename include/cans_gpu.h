@@ -145,7 +145,8 @@ int cans_clock_read(const void *clk, int nsteps_issued, double *t, int *ns, int 
  * set: ro,pr,vx,vy,bx,by,az).  bc == NULL: periodic box of width `margin`
  * (ghost fill and next-step cfl fused into the engine kernel); otherwise
  * bc[(dir*2+side)*7+field] gives a CANS_BD_* kind per face and field.
- * metric_ws: device scratch of cans2d_metric_ws_size(ix,jx) doubles. */
+ * metric_ws: device workspace of cans2d_metric_ws_size(ix,jx) doubles, filled
+ * by the call with step0 == 0 and read by the later calls of the same run. */
 int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, int nsteps,
                    double qav, double gm, double safety, const double *dx, const double *dxm,
                    int ix, const double *dy, const double *dym, int jx, const int *bc,
@@ -163,7 +164,9 @@ size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx);
  * engine (mixed != 0: FP32 face fluxes for Roe/HLLD).  Field sets a/b:
  * 1D ro,pr,vx,vy,by (bx1d, bxm1d separate); 2D ro,pr,vx,vy,bx,by,az;
  * 3D ro,pr,vx,vy,vz,bx,by,bz.  bc[(dir*2+side)*nf+field] = CANS_BD_* kind
- * (NULL only for the fused periodic 2D MLW path).  mu is used by HLLD. */
+ * (NULL only for the fused periodic 2D MLW path).  mu is used by HLLD.
+ * ws (cans_run_ws_bytes) holds the grid metric, filled by the call with
+ * step0 == 0 and read by the later calls of the same run. */
 int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
              const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
              double safety, const double *dx, const double *dxm, int ix, const double *dy,

@@ -285,13 +285,20 @@ class MlwRun2D:
                                     _ptr(dx, (self.ix,)), _ptr(dxm, (self.ix,)), C.c_int(self.ix),
                                     _ptr(dy, (self.jx,)), _ptr(dym, (self.jx,)), C.c_int(self.jx),
                                     self.bc if self.bc is not None else None, C.c_int(self.margin),
-                                    _ptr(self.metric), events.array if events is not None else None, _stream()))
+                                    _ptr(self.metric), _events(events), _stream()))
         self.clock.issued += n
 
     def state(self):
         """Current state (the buffer set the last executed step wrote)."""
         info = self.clock.read()
         return (self.a if info["nexec"] % 2 == 0 else self.b), info
+
+
+def _events(ev):
+    """EventSet, EventSet.at(k) pointer, or None -> the C array argument."""
+    if ev is None:
+        return None
+    return ev.array if isinstance(ev, EventSet) else ev
 
 
 class EventSet:
@@ -304,6 +311,13 @@ class EventSet:
             _check(lib().cans_event_create(C.byref(h)))
             self.array[k] = h.value
         self.n = n
+
+    def at(self, k: int):
+        """The array from event k on (a run given it brackets its steps with
+        events k, k+1, ...)."""
+        if not 0 <= k < self.n:
+            raise IndexError(k)
+        return C.cast(C.byref(self.array, k * C.sizeof(C.c_void_p)), C.POINTER(C.c_void_p))
 
     def record(self, k: int):
         _check(lib().cans_event_record(C.c_void_p(self.array[k]), _stream()))
@@ -402,7 +416,7 @@ class Run:
                               gp("dy", self.jx), gp("dym", self.jx), C.c_int(self.jx),
                               gp("dz", self.kx), gp("dzm", self.kx), C.c_int(self.kx),
                               self.bc, C.c_int(self.margin), _ptr(self.ws),
-                              events.array if events is not None else None, _stream()))
+                              _events(events), _stream()))
         self.clock.issued += n
 
     def state(self):

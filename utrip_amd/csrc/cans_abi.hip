@@ -296,8 +296,10 @@ int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, i
     Metric mx{m, m + mxn, m + 2 * mxn, m + 3 * mxn};
     m += 4 * mxn;
     Metric my{m, m + myn, m + 2 * myn, m + 3 * myn};
-    HIPCHK(launch_metric(dx, dxm, ix, mx, s));
-    HIPCHK(launch_metric(dy, dym, jx, my, s));
+    if (step0 == 0) {   // the first call of a run fills the metric workspace
+        HIPCHK(launch_metric(dx, dxm, ix, mx, s));
+        HIPCHK(launch_metric(dy, dym, jx, my, s));
+    }
     const bool periodic = (bc == nullptr);
     if (periodic && (ix - 2 * margin < margin || jx - 2 * margin < margin))
         return fail(CANS_ERR_ARG, "cans2d_mlw_run: periodic box needs ix-2*margin >= margin (ix=%d jx=%d margin=%d)", ix, jx, margin);

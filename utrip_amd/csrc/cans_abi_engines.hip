@@ -245,7 +245,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     Carver cv{(double *)ws};
     Metric m[3];
     carve_metric(cv, ix, jx, kx, m);
-    if (engine == CANS_ENGINE_MLW) {
+    if (engine == CANS_ENGINE_MLW && step0 == 0) {   // the first call of a run fills the metric
         HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
         if (dim >= 2) HIPCHK(launch_metric(dy, dym, jx, m[1], s));
         if (dim == 3) HIPCHK(launch_metric(dz, dzm, kx, m[2], s));

@@ -1,23 +1,29 @@
 // cans_mlw2d.hip -- fused 2D modified Lax-Wendroff MHD step (gfx950).
 //
 // One launch does the whole of cans2d/hdmlw/mlw_m.f:2-324 (mlwhalf, mlwsrch,
-// mlwfull, mlwsrcf, mlwartv, update, primitive recovery) for a tile of
-// TX x TY interior cells, and optionally the periodic ghost fill of
-// cans2d/bc/bdper{x,y}.f and the CFL reduction of cans2d/common/cfl_m.f for
-// the next step.  State is structure-of-arrays FP64, column-major (ix,jx)
-// exactly as the reference holds it, so the kernel reads each field once
-// with coalesced 8-byte lanes along x.
+// mlwfull, mlwsrcf, mlwartv, update, primitive recovery), and optionally the
+// periodic ghost fill of cans2d/bc/bdper{x,y}.f and the CFL reduction of
+// cans2d/common/cfl_m.f for the next step.  State is structure-of-arrays
+// FP64, column-major (ix,jx) as the reference holds it.
 //
-// Per block:
-//   stage 1  load the (TX+2)x(TY+2) halo cells' primitives, form the
-//            conserved variables and the cell-centred fluxes into LDS;
-//   stage 2  the (TX+1)x(TY+1) corner half-step states (mlwhalf.f:45-52)
-//            and their fluxes (mlw_m.f:189-262) into registers, while the
-//            centre term of mlwhalf (mlwhalf.f:36-41) goes into du;
-//   stage 3  corner fluxes to LDS (aliasing the cell fluxes), full step,
-//            sources, artificial viscosity, update and primitives.
-// Every value is formed by the reference's own expression in its own order,
-// so a value computed by two blocks is the same bits in both.
+// Mapping: one wavefront owns a strip of 64 columns (62 output columns and
+// one halo column each side) and marches along y over H output rows.  The
+// stencil rows live in registers; x neighbours come from the adjacent lanes
+// through DPP wave shifts (v_mov_b32_dpp wave_shl:1 / wave_shr:1), so there is
+// no LDS traffic and no barrier, and each field is read once per strip and
+// row with coalesced 512-byte wave loads.
+//
+// Per output row j a lane forms the cell state of row j+1 (mlw_m.f:101-176),
+// the corner half step (i,j) between rows j and j+1 (mlwhalf.f:45-52,
+// mlw_m.f:189-262), and the update of cell (i,j).  Differences that the
+// reference forms for two neighbouring cells are the same IEEE operation on
+// the same operands -- e.g. (u(i+1)-u(i)) is uE-uc of cell i and uc-uW of
+// cell i+1 (mlwartv.f:33-41), the corner-flux difference of corner row j is
+// the north difference of row j and the south difference of row j+1
+// (mlwfull.f:31-41) -- so each is computed once and handed to the neighbour
+// by a lane shift or carried to the next row.  Every value is formed by the
+// reference's expression in the reference's order, so the results are
+// bit-identical to it and independent of the strip decomposition.
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_kernels.hpp"
@@ -25,297 +31,521 @@
 namespace cans {
 namespace {
 
-constexpr int TX = 32, TY = 8;
-constexpr int NT = TX * TY;                  // 256 threads, one interior cell each
-constexpr int HX = TX + 2, HY = TY + 2;      // halo cells
-constexpr int NH = HX * HY;                  // 340
-constexpr int CXN = TX + 1, CYN = TY + 1;    // corners
-constexpr int NC = CXN * CYN;                // 297
-
+constexpr int WAVE = 64;
+constexpr int COLS = WAVE - 2;               // output columns per wave
+constexpr int WPB = 4;                       // waves per block
 constexpr double pi8i = 1.0 / kPi / 8.0;     // mlw_m.f:27  pi8i=1./pi/8.
 constexpr double pi4i = 1.0 / kPi / 4.0;     // mlw_m.f:28  pi4i=1./pi/4.
 
-// cell arrays in LDS
-enum { C_RO, C_EE, C_RX, C_RY, C_BX, C_BY, C_VX, C_VY, NCELL };
-// cell fluxes (aliased by the corner fluxes after stage 2)
-enum { F_EEX, F_EEY, F_RXX, F_RXY, F_RYX, F_RYY, F_EZ, NFLUX };
-// corner fluxes
-enum { H_ROX, H_ROY, H_EEX, H_EEY, H_RXX, H_RXY, H_RYX, H_RYY, H_EZ, NCORN };
+// Lane l receives lane l+1 (shl) / lane l-1 (shr).  The edge lanes receive
+// 0; their results are never used by an output lane.
+CANS_DEV double shl1(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+CANS_DEV double shr1(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
 
-constexpr int kFluxRegion = (NFLUX * NH > NCORN * NC) ? NFLUX * NH : NCORN * NC;
+// u: ro ee rx ry bx by (the six conserved variables, mlw_m.f:101-112)
+enum { U_RO, U_EE, U_RX, U_RY, U_BX, U_BY, NU };
 
-struct CornerOut {
-    double v[NCORN];
+struct Prim {
+    double ro, pr, vx, vy, bx, by;
 };
 
-__global__ __launch_bounds__(NT) void mlw2d_kernel(Mlw2dArgs a) {
-    __shared__ double cell[NCELL * NH];
-    __shared__ double flx[kFluxRegion];
-    __shared__ double red[NT / 64];
+// Conserved variables and cell-centred fluxes of one cell (mlw_m.f:101-176).
+// x fluxes of the six equations: rx, eex, rxx, ryx, 0, -ez
+// y fluxes:                      ry, eey, rxy, ryy, ez, 0
+struct Cell {
+    double u[NU];
+    double eex, eey, rxx, rxy, ryx, ryy, ez;
+    double vx, vy;
+};
 
-    const int ix = a.ix, jx = a.jx;
-    const int ntx = (ix - 2 + TX - 1) / TX;
-    const int nty = (jx - 2 + TY - 1) / TY;
-    const int tile = xcd_remap(blockIdx.x, ntx * nty);
-    const int i0 = 2 + (tile % ntx) * TX;    // first interior cell of the tile (1-based)
-    const int j0 = 2 + (tile / ntx) * TY;
-    const int tid = threadIdx.x;
+// The part of the east neighbour a cell needs (no vy).
+struct CellE {
+    double u[NU];
+    double eex, eey, rxx, rxy, ryx, ryy, ez;
+    double vx;
+};
 
-    double dt = a.dt;
-    bool run = true;
-    if (a.clk) {
-        StepCtl c = clock_use(a.clk, a.step, a.peek != 0, blockIdx.x == 0 && tid == 0);
-        run = c.run;
-        dt = c.dt;
-    }
-    if (!run) return;
+struct Corner {
+    double rox, roy, eex, eey, rxx, rxy, ryx, ryy, ez;
+};
+
+// Row state carried from output row j-1 to output row j.
+struct RowState {
+    Cell c;          // cell (i, j)
+    CellE e;         // cell (i+1, j)
+    double sy[5];    // y fluxes ry eey rxy ryy ez of cell (i, j-1)
+    double dS[NU];   // u(i,j) - u(i,j-1)               (= uc - uS of row j)
+    double qys;      // AV coefficient of face j-1/2
+    double ky[5];    // y fluxes roy eey rxy ryy ez of corner (i, j-1)
+    double dXS[5];   // corner x-flux differences of corner row j-1
+    double PS;       // ux0*(-ezh(i-1,j-1)) + ux1*(-ezh(i,j-1))
+    Prim nxt;        // primitives of cell (i, j+1), loaded
+    double az;       // az of cell (i, j), loaded
+    // updated conserved variables and az of cell (i, j-1): its primitive
+    // recovery, CFL term and stores run during row j, off the critical path
+    double pro, pee, prx, pry, pbx, pby, paz;
+};
+
+// Per-column constants of a strip.
+struct ColCtx {
+    int i;
+    bool out;                     // output column
+    unsigned boff;                // byte offset of column min(i,ix) within a row
+    double dxi, dxim, dxim1;      // 1/dx(i), 1/dxm(i), 1/dxm(i-1)
+    double ux0, ux1;              // 0.5*dxm(i)/dx(i), 0.5*dxm(i-1)/dx(i)
+    double qxs;                   // qav*dxm(i)
+    double dxc;                   // dx(i)
+};
+
+// Row j (clamped to the grid, so every lane reads a real cell) of a field:
+// a wave-uniform base address plus the lane's 32-bit byte offset.
+CANS_DEV const double *row_of(const double *f, int ix, int jx, int j) {
+    const int jc = j < 1 ? 1 : (j > jx ? jx : j);
+    return f + (size_t)(jc - 1) * (size_t)ix;
+}
+CANS_DEV double ld(const double *row, unsigned boff) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(row) + boff);
+}
+CANS_DEV void st(double *row, unsigned boff, double v) {
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(row) + boff) = v;
+}
+
+CANS_DEV Prim load_prim(const Mlw2dArgs &a, const ColCtx &x, int j) {
+    const size_t r = row_of(a.ro, a.ix, a.jx, j) - a.ro;
+    Prim p;
+    p.ro = ld(a.ro + r, x.boff); p.pr = ld(a.pr + r, x.boff); p.vx = ld(a.vx + r, x.boff);
+    p.vy = ld(a.vy + r, x.boff); p.bx = ld(a.bx + r, x.boff); p.by = ld(a.by + r, x.boff);
+    return p;
+}
+
+CANS_DEV double load_az(const Mlw2dArgs &a, const ColCtx &x, int j) {
+    return ld(row_of(a.az, a.ix, a.jx, j), x.boff);
+}
+
+// mlw_m.f:101-176 for one cell
+CANS_DEV Cell make_cell(const Prim &p, double gm) {
+    Cell c;
+    const double ro = p.ro, pr = p.pr, vx = p.vx, vy = p.vy, bx = p.bx, by = p.by;
+    const double vv = vx * vx + vy * vy;
+    const double bb = bx * bx + by * by;
+    const double ez = -vx * by + vy * bx;
+    const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
+    c.u[U_RO] = ro;
+    c.u[U_EE] = pr / (gm - 1.0) + 0.5 * ro * vv + pi8i * bb;
+    c.u[U_RX] = ro * vx;
+    c.u[U_RY] = ro * vy;
+    c.u[U_BX] = bx;
+    c.u[U_BY] = by;
+    c.eex = ep * vx - by * ez * pi4i;
+    c.eey = ep * vy + bx * ez * pi4i;
+    c.rxx = ro * (vx * vx) + pr + pi8i * (by * by - bx * bx);
+    c.rxy = ro * vx * vy - pi4i * bx * by;
+    c.ryx = ro * vy * vx - pi4i * by * bx;
+    c.ryy = ro * (vy * vy) + pr + pi8i * (bx * bx - by * by);
+    c.ez = ez;
+    c.vx = vx;
+    c.vy = vy;
+    return c;
+}
+
+CANS_DEV CellE shl_cell(const Cell &c) {
+    CellE r;
+#pragma unroll
+    for (int q = 0; q < NU; q++) r.u[q] = shl1(c.u[q]);
+    r.eex = shl1(c.eex); r.eey = shl1(c.eey); r.rxx = shl1(c.rxx); r.rxy = shl1(c.rxy);
+    r.ryx = shl1(c.ryx); r.ryy = shl1(c.ryy); r.ez = shl1(c.ez); r.vx = shl1(c.vx);
+    return r;
+}
+
+// one conserved component of the corner half step (mlwhalf.f:47-50):
+// A=(i,j) B=(i+1,j) C=(i,j+1) D=(i+1,j+1)
+CANS_DEV double half1(double uA, double uB, double uC, double uD, double fA, double fB, double fC, double fD,
+                      double gA, double gB, double gC, double gD, double dxim, double dyim, double dt) {
+    const double uh = 0.25 * (uB + uA + uD + uC);
+    const double dfdx = dxim * (fB - fA + fD - fC) / 2.0;
+    const double dfdy = dyim * (gC - gA + gD - gB) / 2.0;
+    return uh - dt * dfdx - dt * dfdy;
+}
+
+// bx has no x flux and by no y flux.  The reference still forms the zero
+// term, dt*(dm*(+0)/2) = +0 for finite dt >= 0 and positive metric, and
+// subtracting +0 leaves every value (-0 included) unchanged: these variants
+// omit it and give the same bits.
+CANS_DEV double half1_y(double uA, double uB, double uC, double uD, double gA, double gB, double gC, double gD,
+                        double dyim, double dt) {
+    const double uh = 0.25 * (uB + uA + uD + uC);
+    const double dfdy = dyim * (gC - gA + gD - gB) / 2.0;
+    return uh - dt * dfdy;
+}
+CANS_DEV double half1_x(double uA, double uB, double uC, double uD, double fA, double fB, double fC, double fD,
+                        double dxim, double dt) {
+    const double uh = 0.25 * (uB + uA + uD + uC);
+    const double dfdx = dxim * (fB - fA + fD - fC) / 2.0;
+    return uh - dt * dfdx;
+}
+
+// corner between columns i,i+1 and rows j,j+1 (mlwhalf.f:45-52, mlw_m.f:189-262)
+template <class CA, class CB>
+CANS_DEV Corner make_corner(const CA &A, const CB &B, const CA &C, const CB &D, double dxim, double dyim, double dt,
+                            double gm) {
+    const double roh = half1(A.u[U_RO], B.u[U_RO], C.u[U_RO], D.u[U_RO], A.u[U_RX], B.u[U_RX], C.u[U_RX],
+                             D.u[U_RX], A.u[U_RY], B.u[U_RY], C.u[U_RY], D.u[U_RY], dxim, dyim, dt);
+    const double eeh = half1(A.u[U_EE], B.u[U_EE], C.u[U_EE], D.u[U_EE], A.eex, B.eex, C.eex, D.eex, A.eey, B.eey,
+                             C.eey, D.eey, dxim, dyim, dt);
+    const double rxh = half1(A.u[U_RX], B.u[U_RX], C.u[U_RX], D.u[U_RX], A.rxx, B.rxx, C.rxx, D.rxx, A.rxy, B.rxy,
+                             C.rxy, D.rxy, dxim, dyim, dt);
+    const double ryh = half1(A.u[U_RY], B.u[U_RY], C.u[U_RY], D.u[U_RY], A.ryx, B.ryx, C.ryx, D.ryx, A.ryy, B.ryy,
+                             C.ryy, D.ryy, dxim, dyim, dt);
+    const double bxh = half1_y(A.u[U_BX], B.u[U_BX], C.u[U_BX], D.u[U_BX], A.ez, B.ez, C.ez, D.ez, dyim, dt);
+    const double byh = half1_x(A.u[U_BY], B.u[U_BY], C.u[U_BY], D.u[U_BY], -A.ez, -B.ez, -C.ez, -D.ez, dxim, dt);
+    const double vxh = rxh / roh;
+    const double vyh = ryh / roh;
+    const double vv = vxh * vxh + vyh * vyh;
+    const double bb = bxh * bxh + byh * byh;
+    const double prh = (gm - 1.0) * (eeh - 0.5 * roh * vv - pi8i * bb);
+    const double ezh = -vxh * byh + vyh * bxh;
+    const double ep = prh * gm / (gm - 1.0) + 0.5 * roh * vv;
+    Corner k;
+    k.rox = roh * vxh;
+    k.roy = roh * vyh;
+    k.eex = ep * vxh - byh * ezh * pi4i;
+    k.eey = ep * vyh + bxh * ezh * pi4i;
+    k.rxx = roh * (vxh * vxh) + prh + pi8i * (byh * byh - bxh * bxh);
+    k.rxy = roh * vxh * vyh - pi4i * bxh * byh;
+    k.ryx = roh * vyh * vxh - pi4i * byh * bxh;
+    k.ryy = roh * (vyh * vyh) + prh + pi8i * (bxh * bxh - byh * byh);
+    k.ez = ezh;
+    return k;
+}
+
+CANS_DEV double av_coef(double qs, double vhi, double vlo) {   // mlw_m.f:276-288
+    const double zero = 0.0;
+    return qs * fmaxF(zero, __builtin_fabs(vhi - vlo) - kR4_1em4);
+}
+
+// du of one equation: centre term of mlwhalf (mlwhalf.f:36-41), mlwfull
+// (mlwfull.f:31-41) and mlwartv (mlwartv.f:33-41), in the reference's order.
+//   dfx = fE-fW   dgy = gN-gS   hS/hN = corner x differences south/north
+//   kW/kE = corner y differences west/east   dE = uE-uc  dW = uc-uW
+//   dN = uN-uc   dS = uc-uS
+struct RowCoef {
+    double hdt, dt;               // 0.5*dt, dt
+    double hdxi, dxi, hdyi, dyi;  // 0.5/dx(i), 1/dx(i), 0.5/dy(j), 1/dy(j)
+    double ux0, ux1, uy0, uy1;
+    double qxe, qxw, qyn, qys;    // qxc*dxim, qxw*dxim1, qyc*dyim, qys*dyim1
+};
+
+CANS_DEV double du1(const RowCoef &r, double dfx, double dgy, double hS, double hN, double kW, double kE, double dE,
+                    double dW, double dN, double dS) {
+    double du = 0.0;
+    du = du - r.hdt * (r.hdxi * dfx) - r.hdt * (r.hdyi * dgy);
+    const double dfdx = r.dxi * (r.uy0 * hS + r.uy1 * hN);
+    const double dfdy = r.dyi * (r.ux0 * kW + r.ux1 * kE);
+    du = du - r.hdt * dfdx - r.hdt * dfdy;
+    du = du + r.dt * (r.dxi * (r.qxe * dE - r.qxw * dW)) + r.dt * (r.dyi * (r.qyn * dN - r.qys * dS));
+    return du;
+}
+
+// bx (no x flux) and by (no y flux): the zero-flux terms of du1 are +0 for
+// finite dt >= 0 and positive metric and are omitted (same bits).
+CANS_DEV double du1_y(const RowCoef &r, double dgy, double kW, double kE, double dE, double dW, double dN,
+                      double dS) {
+    double du = 0.0;
+    du = du - r.hdt * (r.hdyi * dgy);
+    const double dfdy = r.dyi * (r.ux0 * kW + r.ux1 * kE);
+    du = du - r.hdt * dfdy;
+    du = du + r.dt * (r.dxi * (r.qxe * dE - r.qxw * dW)) + r.dt * (r.dyi * (r.qyn * dN - r.qys * dS));
+    return du;
+}
+CANS_DEV double du1_x(const RowCoef &r, double dfx, double hS, double hN, double dE, double dW, double dN,
+                      double dS) {
+    double du = 0.0;
+    du = du - r.hdt * (r.hdxi * dfx);
+    const double dfdx = r.dxi * (r.uy0 * hS + r.uy1 * hN);
+    du = du - r.hdt * dfdx;
+    du = du + r.dt * (r.dxi * (r.qxe * dE - r.qxw * dW)) + r.dt * (r.dyi * (r.qyn * dN - r.qys * dS));
+    return du;
+}
+
+// One equation with both an x and a y flux (ro ee rx ry).  f*: cell x flux,
+// g*: cell y flux, h*: corner x flux, k*: corner y flux.  The differences
+// handed on: hN (south difference of the next row), kE (shifted to the east
+// neighbour as its kW), dN (the next row's dS).
+CANS_DEV double eq_xy(const RowCoef &r, double fE, double fC, double gN, double gS, double hC, double hS, double kC,
+                      double kS, double uE, double uc, double uN, double dS, double &hN, double &dN) {
+    const double dfx = fE - shr1(fC);
+    const double dgy = gN - gS;
+    hN = hC - shr1(hC);
+    const double kE = kC - kS;
+    const double kW = shr1(kE);
+    const double dE = uE - uc;
+    const double dW = shr1(dE);
+    dN = uN - uc;
+    return du1(r, dfx, dgy, hS, hN, kW, kE, dE, dW, dN, dS);
+}
+
+// Primitive recovery (mlw_m.f:299-321), the CFL term of the new state
+// (cans2d/common/cfl_m.f:46-55) and the stores of the updated cell (i, j),
+// with its periodic ghost images.
+CANS_DEV void finish_row(const Mlw2dArgs &a, const ColCtx &x, int j, const RowState &s, bool do_cfl,
+                         double &dtmin_local) {
     const double gm = a.gm;
-
-    // ---- stage 1: halo cells -> conserved variables and fluxes (mlw_m.f:101-176)
-    for (int c = tid; c < NH; c += NT) {
-        const int hx = c % HX, hy = c / HX;
-        const int i = i0 - 1 + hx, j = j0 - 1 + hy;
-        if (i <= ix && j <= jx) {
-            const size_t p = (size_t)(i - 1) + (size_t)ix * (size_t)(j - 1);
-            const double ro = a.ro[p], pr = a.pr[p], vx = a.vx[p], vy = a.vy[p];
-            const double bx = a.bx[p], by = a.by[p];
-            const double vv = vx * vx + vy * vy;
-            const double bb = bx * bx + by * by;
-            const double ee = pr / (gm - 1.0) + 0.5 * ro * vv + pi8i * bb;
-            const double ez = -vx * by + vy * bx;
-            const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
-            cell[C_RO * NH + c] = ro;
-            cell[C_EE * NH + c] = ee;
-            cell[C_RX * NH + c] = ro * vx;
-            cell[C_RY * NH + c] = ro * vy;
-            cell[C_BX * NH + c] = bx;
-            cell[C_BY * NH + c] = by;
-            cell[C_VX * NH + c] = vx;
-            cell[C_VY * NH + c] = vy;
-            flx[F_EEX * NH + c] = ep * vx - by * ez * pi4i;
-            flx[F_EEY * NH + c] = ep * vy + bx * ez * pi4i;
-            flx[F_RXX * NH + c] = ro * (vx * vx) + pr + pi8i * (by * by - bx * bx);
-            flx[F_RXY * NH + c] = ro * vx * vy - pi4i * bx * by;
-            flx[F_RYX * NH + c] = ro * vy * vx - pi4i * by * bx;
-            flx[F_RYY * NH + c] = ro * (vy * vy) + pr + pi8i * (bx * bx - by * by);
-            flx[F_EZ * NH + c] = ez;
-        }
+    const double ro = s.pro, ee = s.pee, rx = s.prx, ry = s.pry, bx = s.pbx, by = s.pby, az = s.paz;
+    const double vx = rx / ro;
+    const double vy = ry / ro;
+    const double vv = vx * vx + vy * vy;
+    const double bb = bx * bx + by * by;
+    const double pr = (gm - 1.0) * (ee - 0.5 * ro * vv - pi8i * bb);
+    double sq = 0.0;
+    if (do_cfl) {
+        const double b2 = bx * bx + by * by;
+        const double ca2 = b2 * pi4i / ro;
+        const double cs2 = gm * pr / ro;
+        const double v2 = vx * vx + vy * vy;
+        sq = __builtin_sqrt(v2 + cs2 + ca2);
+        const double dtq = a.safety * (fminF(x.dxc, a.dy[j - 1]) / sq);
+        dtmin_local = (x.out && dtq < dtmin_local) ? dtq : dtmin_local;
     }
-    __syncthreads();
-
-    // ---- stage 2a: centre term of mlwhalf for the own cell (mlwhalf.f:36-41)
-    const int tx = tid % TX, ty = tid / TX;
-    const int i = i0 + tx, j = j0 + ty;
-    const bool own = (i <= ix - 1) && (j <= jx - 1);
-    const int hc = (tx + 1) + HX * (ty + 1);          // own cell in the halo
-    const int hE = hc + 1, hW = hc - 1, hN = hc + HX, hS = hc - HX;
-    double du[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};    // ro ee rx ry bx by
-    double daz = 0.0;
-    double dxi = 0, dxim = 0, dxim1 = 0, dyi = 0, dyim = 0, dyim1 = 0, ux0 = 0, ux1 = 0, uy0 = 0, uy1 = 0;
-    if (own) {
-        dxi = a.mx.di[i - 1]; dxim = a.mx.dim[i - 1]; dxim1 = a.mx.dim[i - 2];
-        ux0 = a.mx.u0[i - 1]; ux1 = a.mx.u1[i - 1];
-        dyi = a.my.di[j - 1]; dyim = a.my.dim[j - 1]; dyim1 = a.my.dim[j - 2];
-        uy0 = a.my.u0[j - 1]; uy1 = a.my.u1[j - 1];
-        // fluxes of each equation at E/W (x) and N/S (y)
-        const double fxE[6] = {cell[C_RX * NH + hE], flx[F_EEX * NH + hE], flx[F_RXX * NH + hE],
-                               flx[F_RYX * NH + hE], 0.0, -flx[F_EZ * NH + hE]};
-        const double fxW[6] = {cell[C_RX * NH + hW], flx[F_EEX * NH + hW], flx[F_RXX * NH + hW],
-                               flx[F_RYX * NH + hW], 0.0, -flx[F_EZ * NH + hW]};
-        const double fyN[6] = {cell[C_RY * NH + hN], flx[F_EEY * NH + hN], flx[F_RXY * NH + hN],
-                               flx[F_RYY * NH + hN], flx[F_EZ * NH + hN], 0.0};
-        const double fyS[6] = {cell[C_RY * NH + hS], flx[F_EEY * NH + hS], flx[F_RXY * NH + hS],
-                               flx[F_RYY * NH + hS], flx[F_EZ * NH + hS], 0.0};
-#pragma unroll
-        for (int q = 0; q < 6; q++)
-            du[q] = du[q] - 0.5 * dt * (0.5 * dxi * (fxE[q] - fxW[q]))
-                          - 0.5 * dt * (0.5 * dyi * (fyN[q] - fyS[q]));
-        // mlwsrch.f:32-36, s = -ez
-        daz = daz + 0.5 * dt * (-flx[F_EZ * NH + hc]);
-    }
-
-    // ---- stage 2b: corner half-step states and fluxes (mlwhalf.f:45-52, mlw_m.f:189-262)
-    CornerOut co[(NC + NT - 1) / NT];
-#pragma unroll
-    for (int r = 0; r < (NC + NT - 1) / NT; r++) {
-        const int c = tid + r * NT;
-        if (c >= NC) break;
-        const int cx = c % CXN, cy = c / CXN;
-        const int ci = i0 - 1 + cx, cj = j0 - 1 + cy;   // corner between cells ci,ci+1 / cj,cj+1
-        if (ci > ix - 1 || cj > jx - 1) continue;
-        const int h00 = cx + HX * cy, h10 = h00 + 1, h01 = h00 + HX, h11 = h01 + 1;
-        const double cdxim = a.mx.dim[ci - 1], cdyim = a.my.dim[cj - 1];
-        double un[6];
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            const double *u = cell + q * NH;   // C_RO..C_BY are 0..5 in conserved order
-            double fx00, fx10, fx01, fx11, fy00, fy10, fy01, fy11;
-            switch (q) {
-            case 0:
-                fx00 = cell[C_RX * NH + h00]; fx10 = cell[C_RX * NH + h10]; fx01 = cell[C_RX * NH + h01]; fx11 = cell[C_RX * NH + h11];
-                fy00 = cell[C_RY * NH + h00]; fy10 = cell[C_RY * NH + h10]; fy01 = cell[C_RY * NH + h01]; fy11 = cell[C_RY * NH + h11];
-                break;
-            case 1:
-                fx00 = flx[F_EEX * NH + h00]; fx10 = flx[F_EEX * NH + h10]; fx01 = flx[F_EEX * NH + h01]; fx11 = flx[F_EEX * NH + h11];
-                fy00 = flx[F_EEY * NH + h00]; fy10 = flx[F_EEY * NH + h10]; fy01 = flx[F_EEY * NH + h01]; fy11 = flx[F_EEY * NH + h11];
-                break;
-            case 2:
-                fx00 = flx[F_RXX * NH + h00]; fx10 = flx[F_RXX * NH + h10]; fx01 = flx[F_RXX * NH + h01]; fx11 = flx[F_RXX * NH + h11];
-                fy00 = flx[F_RXY * NH + h00]; fy10 = flx[F_RXY * NH + h10]; fy01 = flx[F_RXY * NH + h01]; fy11 = flx[F_RXY * NH + h11];
-                break;
-            case 3:
-                fx00 = flx[F_RYX * NH + h00]; fx10 = flx[F_RYX * NH + h10]; fx01 = flx[F_RYX * NH + h01]; fx11 = flx[F_RYX * NH + h11];
-                fy00 = flx[F_RYY * NH + h00]; fy10 = flx[F_RYY * NH + h10]; fy01 = flx[F_RYY * NH + h01]; fy11 = flx[F_RYY * NH + h11];
-                break;
-            case 4:  // bx: fx = 0, fy = ez
-                fx00 = 0.0; fx10 = 0.0; fx01 = 0.0; fx11 = 0.0;
-                fy00 = flx[F_EZ * NH + h00]; fy10 = flx[F_EZ * NH + h10]; fy01 = flx[F_EZ * NH + h01]; fy11 = flx[F_EZ * NH + h11];
-                break;
-            default: // by: fx = -ez, fy = 0
-                fx00 = -flx[F_EZ * NH + h00]; fx10 = -flx[F_EZ * NH + h10]; fx01 = -flx[F_EZ * NH + h01]; fx11 = -flx[F_EZ * NH + h11];
-                fy00 = 0.0; fy10 = 0.0; fy01 = 0.0; fy11 = 0.0;
-                break;
-            }
-            const double uh = 0.25 * (u[h10] + u[h00] + u[h11] + u[h01]);
-            const double dfdx = cdxim * (fx10 - fx00 + fx11 - fx01) / 2.0;
-            const double dfdy = cdyim * (fy01 - fy00 + fy11 - fy10) / 2.0;
-            un[q] = uh - dt * dfdx - dt * dfdy;
-        }
-        // half-step primitives and fluxes (mlw_m.f:189-262)
-        const double roh = un[0], eeh = un[1], rxh = un[2], ryh = un[3], bxh = un[4], byh = un[5];
-        const double vxh = rxh / roh;
-        const double vyh = ryh / roh;
-        const double vv = vxh * vxh + vyh * vyh;
-        const double bb = bxh * bxh + byh * byh;
-        const double prh = (gm - 1.0) * (eeh - 0.5 * roh * vv - pi8i * bb);
-        const double ezh = -vxh * byh + vyh * bxh;
-        const double ep = prh * gm / (gm - 1.0) + 0.5 * roh * vv;
-        co[r].v[H_ROX] = roh * vxh;
-        co[r].v[H_ROY] = roh * vyh;
-        co[r].v[H_EEX] = ep * vxh - byh * ezh * pi4i;
-        co[r].v[H_EEY] = ep * vyh + bxh * ezh * pi4i;
-        co[r].v[H_RXX] = roh * (vxh * vxh) + prh + pi8i * (byh * byh - bxh * bxh);
-        co[r].v[H_RXY] = roh * vxh * vyh - pi4i * bxh * byh;
-        co[r].v[H_RYX] = roh * vyh * vxh - pi4i * byh * bxh;
-        co[r].v[H_RYY] = roh * (vyh * vyh) + prh + pi8i * (bxh * bxh - byh * byh);
-        co[r].v[H_EZ] = ezh;
-    }
-    __syncthreads();   // all reads of the cell fluxes are done
-#pragma unroll
-    for (int r = 0; r < (NC + NT - 1) / NT; r++) {
-        const int c = tid + r * NT;
-        if (c >= NC) break;
-#pragma unroll
-        for (int v = 0; v < NCORN; v++) flx[v * NC + c] = co[r].v[v];
-    }
-    __syncthreads();
-
-    double dtmin_local = kR4_1e20;
-    const bool do_cfl = a.clk && a.safety > 0.0;
-    if (own) {
-        // ---- stage 3: full step (mlwfull.f:31-41), mlwsrcf.f:33-40, AV, update
-        const int cSW = tx + CXN * ty;        // corner (i-1, j-1)
-        const int cSE = cSW + 1;              // corner (i,   j-1)
-        const int cNW = cSW + CXN;            // corner (i-1, j)
-        const int cNE = cNW + 1;              // corner (i,   j)
-        const int fxid[6] = {H_ROX, H_EEX, H_RXX, H_RYX, -1, H_EZ};
-        const int fyid[6] = {H_ROY, H_EEY, H_RXY, H_RYY, H_EZ, -1};
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            double fSW, fSE, fNW, fNE, gSW, gSE, gNW, gNE;
-            if (fxid[q] < 0) {
-                fSW = fSE = fNW = fNE = 0.0;
-            } else {
-                fSW = flx[fxid[q] * NC + cSW]; fSE = flx[fxid[q] * NC + cSE];
-                fNW = flx[fxid[q] * NC + cNW]; fNE = flx[fxid[q] * NC + cNE];
-                if (q == 5) { fSW = -fSW; fSE = -fSE; fNW = -fNW; fNE = -fNE; }  // by: fx = -ezh
-            }
-            if (fyid[q] < 0) {
-                gSW = gSE = gNW = gNE = 0.0;
-            } else {
-                gSW = flx[fyid[q] * NC + cSW]; gSE = flx[fyid[q] * NC + cSE];
-                gNW = flx[fyid[q] * NC + cNW]; gNE = flx[fyid[q] * NC + cNE];
-            }
-            const double dfdx = dxi * (uy0 * (fSE - fSW) + uy1 * (fNE - fNW));
-            const double dfdy = dyi * (ux0 * (gNW - gSW) + ux1 * (gNE - gSE));
-            du[q] = du[q] - 0.5 * dt * dfdx - 0.5 * dt * dfdy;
-        }
-        {
-            const double sSW = -flx[H_EZ * NC + cSW], sSE = -flx[H_EZ * NC + cSE];
-            const double sNW = -flx[H_EZ * NC + cNW], sNE = -flx[H_EZ * NC + cNE];
-            const double sh = uy0 * (ux0 * sSW + ux1 * sSE) + uy1 * (ux0 * sNW + ux1 * sNE);
-            daz = daz + 0.5 * dt * sh;
-        }
-        // artificial viscosity (mlw_m.f:276-295, mlwartv.f:33-41)
-        {
-            const double zero = 0.0;
-            const double vxc = cell[C_VX * NH + hc], vyc = cell[C_VY * NH + hc];
-            const double qxc = a.qav * a.dxm[i - 1] * fmaxF(zero, __builtin_fabs(cell[C_VX * NH + hE] - vxc) - kR4_1em4);
-            const double qxw = a.qav * a.dxm[i - 2] * fmaxF(zero, __builtin_fabs(vxc - cell[C_VX * NH + hW]) - kR4_1em4);
-            const double qyc = a.qav * a.dym[j - 1] * fmaxF(zero, __builtin_fabs(cell[C_VY * NH + hN] - vyc) - kR4_1em4);
-            const double qys = a.qav * a.dym[j - 2] * fmaxF(zero, __builtin_fabs(vyc - cell[C_VY * NH + hS]) - kR4_1em4);
-#pragma unroll
-            for (int q = 0; q < 6; q++) {
-                const double *u = cell + q * NH;
-                const double uc = u[hc];
-                du[q] = du[q]
-                      + dt * (dxi * (qxc * dxim * (u[hE] - uc) - qxw * dxim1 * (uc - u[hW])))
-                      + dt * (dyi * (qyc * dyim * (u[hN] - uc) - qys * dyim1 * (uc - u[hS])));
-            }
-        }
-        // update and primitive recovery (mlw_m.f:299-321)
-        const size_t p = (size_t)(i - 1) + (size_t)ix * (size_t)(j - 1);
-        const double ro = cell[C_RO * NH + hc] + du[0];
-        const double ee = cell[C_EE * NH + hc] + du[1];
-        const double rx = cell[C_RX * NH + hc] + du[2];
-        const double ry = cell[C_RY * NH + hc] + du[3];
-        const double bx = cell[C_BX * NH + hc] + du[4];
-        const double by = cell[C_BY * NH + hc] + du[5];
-        const double az = a.az[p] + daz;
-        const double vx = rx / ro;
-        const double vy = ry / ro;
-        const double vv = vx * vx + vy * vy;
-        const double bb = bx * bx + by * by;
-        const double pr = (gm - 1.0) * (ee - 0.5 * ro * vv - pi8i * bb);
-
-        // the cell and, in a periodic box, its ghost images: after bdperx
-        // then bdpery, ghost (gi,gj) holds interior (src_x(gi), src_y(gj))
-        const int m = a.per_margin;   // the host checks ix-2m >= m and jx-2m >= m
-        int xi[3] = {i, 0, 0}, yj[3] = {j, 0, 0};
-        int nxi = 1, nyj = 1;
-        if (m > 0) {
-            if (i >= ix - 2 * m + 1) xi[nxi++] = i - (ix - 2 * m);   // low ghost  <- high interior
-            if (i <= 2 * m) xi[nxi++] = i + (ix - 2 * m);            // high ghost <- low interior
-            if (j >= jx - 2 * m + 1) yj[nyj++] = j - (jx - 2 * m);
-            if (j <= 2 * m) yj[nyj++] = j + (jx - 2 * m);
-        }
-        for (int b = 0; b < nyj; b++) {
-            for (int c = 0; c < nxi; c++) {
-                const size_t q = (size_t)(xi[c] - 1) + (size_t)ix * (size_t)(yj[b] - 1);
-                a.ro_o[q] = ro; a.pr_o[q] = pr; a.vx_o[q] = vx; a.vy_o[q] = vy;
-                a.bx_o[q] = bx; a.by_o[q] = by; a.az_o[q] = az;
-                if (do_cfl) {   // cans2d/common/cfl_m.f:46-55 for this cell
-                    const double b2 = bx * bx + by * by;
-                    const double ca2 = b2 * pi4i / ro;
-                    const double cs2 = gm * pr / ro;
-                    const double v2 = vx * vx + vy * vy;
-                    const double dtcfl = fminF(a.dx[xi[c] - 1], a.dy[yj[b] - 1]) / __builtin_sqrt(v2 + cs2 + ca2);
-                    const double dtq = a.safety * dtcfl;
-                    if (dtq < dtmin_local) dtmin_local = dtq;
+    if (x.out) {
+        const int ix = a.ix, jx = a.jx;
+        // Stores as scalar row address + 32-bit lane offset.  The empty asm
+        // makes the offset opaque so that the compiler does not hoist seven
+        // 64-bit per-lane field pointers out of the loop into registers.
+        const size_t rw = (size_t)(__builtin_amdgcn_readfirstlane(j) - 1) * (size_t)ix;
+        unsigned bo = x.boff;
+        asm volatile("" : "+v"(bo));
+        st(a.ro_o + rw, bo, ro); st(a.pr_o + rw, bo, pr); st(a.vx_o + rw, bo, vx);
+        st(a.vy_o + rw, bo, vy); st(a.bx_o + rw, bo, bx); st(a.by_o + rw, bo, by);
+        st(a.az_o + rw, bo, az);
+        // In a periodic box also the ghost images: after bdperx then
+        // bdpery, ghost (gi,gj) holds interior (src_x(gi), src_y(gj)).
+        const int mg = a.per_margin;   // the host checks ix-2m >= m and jx-2m >= m
+        const int i = x.i;
+        if (mg > 0 && (i >= ix - 2 * mg + 1 || i <= 2 * mg || j >= jx - 2 * mg + 1 || j <= 2 * mg)) {
+            int xi[3] = {i, 0, 0}, yj[3] = {j, 0, 0};
+            int nxi = 1, nyj = 1;
+            if (i >= ix - 2 * mg + 1) xi[nxi++] = i - (ix - 2 * mg);   // low ghost  <- high interior
+            if (i <= 2 * mg) xi[nxi++] = i + (ix - 2 * mg);            // high ghost <- low interior
+            if (j >= jx - 2 * mg + 1) yj[nyj++] = j - (jx - 2 * mg);
+            if (j <= 2 * mg) yj[nyj++] = j + (jx - 2 * mg);
+            for (int b = 0; b < nyj; b++) {
+                for (int c = 0; c < nxi; c++) {
+                    if (b == 0 && c == 0) continue;
+                    const size_t q = (size_t)(xi[c] - 1) + (size_t)ix * (size_t)(yj[b] - 1);
+                    a.ro_o[q] = ro; a.pr_o[q] = pr; a.vx_o[q] = vx; a.vy_o[q] = vy;
+                    a.bx_o[q] = bx; a.by_o[q] = by; a.az_o[q] = az;
+                    if (do_cfl) {
+                        const double dtq = a.safety * (fminF(a.dx[xi[c] - 1], a.dy[yj[b] - 1]) / sq);
+                        if (dtq < dtmin_local) dtmin_local = dtq;
+                    }
                 }
             }
         }
     }
+}
+
+// One output row j: s holds the state entering row j, o receives the state
+// entering row j+1.  Everything but the stores runs on all 64 lanes (the
+// lane shifts need every lane's operands).  With PREV the row also finishes
+// row j-1 (primitives, CFL, stores), so that those dependent chains overlap
+// this row's independent work.
+template <bool PREV>
+CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bool do_cfl, const RowState &s,
+                       RowState &o, double &dtmin_local) {
+    const double gm = a.gm;
+    // prefetch the row after next and the az of the next row
+    o.nxt = load_prim(a, x, j + 2);
+    o.az = load_az(a, x, j + 1);
+
+    const double dyim = a.my.dim[j - 1];
+    const Cell R2 = make_cell(s.nxt, gm);          // cell (i, j+1)
+    const CellE R2p = shl_cell(R2);                // cell (i+1, j+1)
+    const Corner C = make_corner(s.c, s.e, R2, R2p, x.dxim, dyim, dt, gm);
+
+    const double qxc = av_coef(x.qxs, s.e.vx, s.c.vx);
+    const double qyc = av_coef(a.qav * a.dym[j - 1], R2.vy, s.c.vy);
+    RowCoef r;
+    r.hdt = 0.5 * dt;
+    r.dt = dt;
+    r.dxi = x.dxi;
+    r.hdxi = 0.5 * x.dxi;
+    r.dyi = a.my.di[j - 1];
+    r.hdyi = 0.5 * r.dyi;
+    r.ux0 = x.ux0; r.ux1 = x.ux1;
+    r.uy0 = a.my.u0[j - 1]; r.uy1 = a.my.u1[j - 1];
+    r.qxe = qxc * x.dxim;
+    r.qxw = shr1(qxc) * x.dxim1;
+    r.qyn = qyc * dyim;
+    r.qys = s.qys * a.my.dim[j - 2];
+
+    double du[NU];
+    du[U_RO] = eq_xy(r, s.e.u[U_RX], s.c.u[U_RX], R2.u[U_RY], s.sy[0], C.rox, s.dXS[0], C.roy, s.ky[0],
+                     s.e.u[U_RO], s.c.u[U_RO], R2.u[U_RO], s.dS[U_RO], o.dXS[0], o.dS[U_RO]);
+    du[U_EE] = eq_xy(r, s.e.eex, s.c.eex, R2.eey, s.sy[1], C.eex, s.dXS[1], C.eey, s.ky[1], s.e.u[U_EE],
+                     s.c.u[U_EE], R2.u[U_EE], s.dS[U_EE], o.dXS[1], o.dS[U_EE]);
+    du[U_RX] = eq_xy(r, s.e.rxx, s.c.rxx, R2.rxy, s.sy[2], C.rxx, s.dXS[2], C.rxy, s.ky[2], s.e.u[U_RX],
+                     s.c.u[U_RX], R2.u[U_RX], s.dS[U_RX], o.dXS[2], o.dS[U_RX]);
+    du[U_RY] = eq_xy(r, s.e.ryx, s.c.ryx, R2.ryy, s.sy[3], C.ryx, s.dXS[3], C.ryy, s.ky[3], s.e.u[U_RY],
+                     s.c.u[U_RY], R2.u[U_RY], s.dS[U_RY], o.dXS[3], o.dS[U_RY]);
+    const double ezW = shr1(C.ez);                 // ezh(i-1, j)
+    {   // bx: x flux 0, y flux ez
+        const double kE = C.ez - s.ky[4];
+        const double dE = s.e.u[U_BX] - s.c.u[U_BX];
+        const double dN = R2.u[U_BX] - s.c.u[U_BX];
+        du[U_BX] = du1_y(r, R2.ez - s.sy[4], shr1(kE), kE, dE, shr1(dE), dN, s.dS[U_BX]);
+        o.dS[U_BX] = dN;
+    }
+    {   // by: x flux -ez, y flux 0.  (-a)-(-b) is b-a bit for bit (signed
+        // zeros included), so the x differences are taken as west - east.
+        const double hN = ezW - C.ez;
+        const double dE = s.e.u[U_BY] - s.c.u[U_BY];
+        const double dN = R2.u[U_BY] - s.c.u[U_BY];
+        du[U_BY] = du1_x(r, shr1(s.c.ez) - s.e.ez, s.dXS[4], hN, dE, shr1(dE), dN, s.dS[U_BY]);
+        o.dXS[4] = hN;
+        o.dS[U_BY] = dN;
+    }
+    // Az: mlwsrch.f:32-36 then mlwsrcf.f:33-40, source s = -ez
+    const double PN = x.ux0 * (-ezW) + x.ux1 * (-C.ez);
+    double daz = 0.0;
+    daz = daz + r.hdt * (-s.c.ez);
+    const double sh = r.uy0 * s.PS + r.uy1 * PN;
+    daz = daz + r.hdt * sh;
+    // update (mlw_m.f:299-304); the rest of the cell's work runs next row
+    o.pro = s.c.u[U_RO] + du[U_RO];
+    o.pee = s.c.u[U_EE] + du[U_EE];
+    o.prx = s.c.u[U_RX] + du[U_RX];
+    o.pry = s.c.u[U_RY] + du[U_RY];
+    o.pbx = s.c.u[U_BX] + du[U_BX];
+    o.pby = s.c.u[U_BY] + du[U_BY];
+    o.paz = s.az + daz;
+
+    // state entering row j+1
+    o.c = R2;
+    o.e = R2p;
+    o.sy[0] = s.c.u[U_RY]; o.sy[1] = s.c.eey; o.sy[2] = s.c.rxy; o.sy[3] = s.c.ryy; o.sy[4] = s.c.ez;
+    o.qys = qyc;
+    o.ky[0] = C.roy; o.ky[1] = C.eey; o.ky[2] = C.rxy; o.ky[3] = C.ryy; o.ky[4] = C.ez;
+    o.PS = PN;
+
+    if (PREV) finish_row(a, x, j - 1, s, do_cfl, dtmin_local);
+}
+
+template <int OCC>
+__global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC))) void mlw2d_kernel(Mlw2dArgs a,
+                                                                                                        int H) {
+    __shared__ double red[WPB];
+    const int ix = a.ix, jx = a.jx;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nstrip = (ix - 2 + COLS - 1) / COLS;
+    const int nrowb = (jx - 2 + H - 1) / H;
+    const int nblocks = (nstrip * nrowb + WPB - 1) / WPB;
+    const int unit = xcd_remap(blockIdx.x, nblocks) * WPB + wv;
+
+    double dt = a.dt;
+    bool run = true;
+    if (a.clk) {
+        StepCtl c = clock_use(a.clk, a.step, a.peek != 0, blockIdx.x == 0 && threadIdx.x == 0);
+        run = c.run;
+        dt = c.dt;
+    }
+    if (!run) return;                          // uniform across the launch
+    const bool do_cfl = a.clk && a.safety > 0.0;
+    double dtmin_local = kR4_1e20;
+
+    if (unit < nstrip * nrowb) {
+        const int strip = unit % nstrip, rowb = unit / nstrip;
+        const int jb = 2 + rowb * H;
+        const int je = min(jb + H - 1, jx - 1);
+        ColCtx x;
+        x.i = 1 + strip * COLS + lane;         // lane 0 is the west halo column
+        x.out = lane >= 1 && lane <= COLS && x.i <= ix - 1;
+        const int ic = min(x.i, ix);           // lanes past the grid read column ix
+        x.boff = (unsigned)(ic - 1) * 8u;
+        x.dxi = a.mx.di[ic - 1];
+        x.dxim = a.mx.dim[ic - 1];
+        x.dxim1 = a.mx.dim[max(ic - 2, 0)];
+        x.ux0 = a.mx.u0[ic - 1];
+        x.ux1 = a.mx.u1[ic - 1];
+        x.qxs = a.qav * a.dxm[ic - 1];
+        x.dxc = a.dx[ic - 1];
+
+        // prologue: cells of rows jb-1 and jb, corner row jb-1
+        const Cell R0 = make_cell(load_prim(a, x, jb - 1), a.gm);
+        RowState s0, s1;
+        s0.c = make_cell(load_prim(a, x, jb), a.gm);
+        s0.e = shl_cell(s0.c);
+        s0.nxt = load_prim(a, x, jb + 1);
+        s0.az = load_az(a, x, jb);
+        const CellE R0p = shl_cell(R0);
+        const Corner Cm = make_corner(R0, R0p, s0.c, s0.e, x.dxim, a.my.dim[jb - 2], dt, a.gm);
+        const double ezW = shr1(Cm.ez);
+        s0.dXS[0] = Cm.rox - shr1(Cm.rox);
+        s0.dXS[1] = Cm.eex - shr1(Cm.eex);
+        s0.dXS[2] = Cm.rxx - shr1(Cm.rxx);
+        s0.dXS[3] = Cm.ryx - shr1(Cm.ryx);
+        s0.dXS[4] = ezW - Cm.ez;
+        s0.PS = x.ux0 * (-ezW) + x.ux1 * (-Cm.ez);
+        s0.ky[0] = Cm.roy; s0.ky[1] = Cm.eey; s0.ky[2] = Cm.rxy; s0.ky[3] = Cm.ryy; s0.ky[4] = Cm.ez;
+        s0.sy[0] = R0.u[U_RY]; s0.sy[1] = R0.eey; s0.sy[2] = R0.rxy; s0.sy[3] = R0.ryy; s0.sy[4] = R0.ez;
+#pragma unroll
+        for (int q = 0; q < NU; q++) s0.dS[q] = s0.c.u[q] - R0.u[q];
+        s0.qys = av_coef(a.qav * a.dym[jb - 2], s0.c.vy, R0.vy);
+
+        // Row jb, then two rows per trip so that the carried state
+        // ping-pongs between s1 and s0 instead of being copied; every row
+        // finishes the one before it, and the last row is finished after.
+        row_step<false>(a, x, jb, dt, do_cfl, s0, s1, dtmin_local);
+        int j = jb + 1;
+        for (; j + 1 <= je; j += 2) {
+            row_step<true>(a, x, j, dt, do_cfl, s1, s0, dtmin_local);
+            row_step<true>(a, x, j + 1, dt, do_cfl, s0, s1, dtmin_local);
+        }
+        if (j <= je) {
+            row_step<true>(a, x, j, dt, do_cfl, s1, s0, dtmin_local);
+            finish_row(a, x, je, s0, do_cfl, dtmin_local);
+        } else {
+            finish_row(a, x, je, s1, do_cfl, dtmin_local);
+        }
+    }
     if (do_cfl) block_min_to_global(dtmin_local, red, &a.clk->dtkey[(a.step + 1) % 3]);
+}
+
+constexpr int kOcc = 1;             // waves per SIMD the register budget allows
+constexpr int kSlots = 256 * 4 * kOcc; // wave slots of the 256 CUs
+
+// Rows per wave: each wave pays about 1.5 rows of prologue, and the launch
+// runs in ceil(waves / slots) rounds, so pick H minimising rounds * (H + 1.5).
+int pick_rows(int ix, int jx) {
+    const int nstrip = (ix - 2 + COLS - 1) / COLS;
+    int best = 16;
+    double cost = 1e300;
+    for (int h = 8; h <= 96; h++) {
+        const long waves = (long)nstrip * ((jx - 2 + h - 1) / h);
+        const long rounds = (waves + kSlots - 1) / kSlots;
+        const double c = (double)rounds * (h + 1.5);
+        if (c < cost) { cost = c; best = h; }
+    }
+    return best;
 }
 
 }  // namespace
 
 hipError_t launch_mlw2d(const Mlw2dArgs &a, hipStream_t s) {
     if (a.ix < 3 || a.jx < 3) return hipSuccess;
-    const int ntx = (a.ix - 2 + TX - 1) / TX;
-    const int nty = (a.jx - 2 + TY - 1) / TY;
-    hipLaunchKernelGGL(mlw2d_kernel, dim3(ntx * nty), dim3(NT), 0, s, a);
+    const int h = pick_rows(a.ix, a.jx);
+    const int nstrip = (a.ix - 2 + COLS - 1) / COLS;
+    const int nrowb = (a.jx - 2 + h - 1) / h;
+    const int nblocks = (nstrip * nrowb + WPB - 1) / WPB;
+    hipLaunchKernelGGL(mlw2d_kernel<kOcc>, dim3(nblocks), dim3(WAVE * WPB), 0, s, a, h);
     return hipGetLastError();
 }
 
