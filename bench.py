@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--n", type=int, default=1024, help="interior cells per side")
     p.add_argument("--cpu-steps", type=int, default=40, help="reference steps timed on the host")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-extra", action="store_true", help="only the main workload (no 4096^2 line)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run 5 device-to-device copies of this many MiB (the byte-counter calibration "
                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -130,48 +131,60 @@ def main():
     from utrip_amd import cans, problems
 
     dev = torch.device("cuda:0")
+
+    def ot_run(n: int, K: int, warmup: int):
+        """K timed steps of the OT n^2 periodic MLW loop after `warmup`
+        steps.  Every EVERY-th step of the timed region has its engine launch
+        bracketed by a pair of HIP events on the launch stream (the average
+        launch duration of the roofline); the other steps run back to back."""
+        pb = problems.orszag_tang_2d(n=n, margin=1)
+        names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+        g, pp = pb.grid, pb.params
+
+        def td(a):
+            return torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+
+        st = [td(pb.fields[k]) for k in names]
+        gd = {k: td(v) for k, v in g.items()}
+        run = cans.MlwRun2D(st, gd["dx"], gd["dxm"], gd["dy"], gd["dym"], pp["qav"], pp["gm"], pp["safety"])
+        run.reset(dtmin=pp["dtmin"])
+        mass0 = float(st[0][1:-1, 1:-1].sum())
+        run.steps(warmup)
+        torch.cuda.synchronize()
+        EVERY = 8
+        sampled = list(range(0, K, EVERY))
+        M = len(sampled)
+        ev = cans.EventSet(2 * M + 2)
+        torch.cuda.synchronize()
+        ev.record(2 * M)               # start of the timed region
+        for m, s0 in enumerate(sampled):
+            run.steps(1, events=ev.at(2 * m))
+            rest = min(EVERY, K - s0) - 1
+            if rest > 0:
+                run.steps(rest)
+        ev.record(2 * M + 1)           # end of the timed region
+        torch.cuda.synchronize()
+        total_ms = ev.elapsed_ms(2 * M, 2 * M + 1)
+        kern_ms = [ev.elapsed_ms(2 * m, 2 * m + 1) for m in range(M)]
+        cur, info = run.state()
+        assert info["merr"] == 0 and info["nexec"] == warmup + K, info
+        mass1 = float(cur[0][1:-1, 1:-1].sum())
+        return dict(pb=pb, total_ms=total_ms, avg_kern_s=statistics.fmean(kern_ms) / 1e3, info=info,
+                    drift=(mass1 - mass0) / mass0)
+
     n = args.n
-    pb = problems.orszag_tang_2d(n=n, margin=1)
-    names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
-    g, pp = pb.grid, pb.params
-
-    def td(a):
-        return torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
-
-    st = [td(pb.fields[k]) for k in names]
-    gd = {k: td(v) for k, v in g.items()}
-    run = cans.MlwRun2D(st, gd["dx"], gd["dxm"], gd["dy"], gd["dym"], pp["qav"], pp["gm"], pp["safety"])
-    run.reset(dtmin=pp["dtmin"])
-    mass0 = float(st[0][1:-1, 1:-1].sum())
-
-    run.steps(args.warmup)
-    torch.cuda.synchronize()
-
-    # Every EVERY-th step of the timed region has its engine launch bracketed
-    # by a pair of HIP events on the launch stream (the average launch
-    # duration of the roofline); the other steps run back to back.
     K = args.steps
-    EVERY = 8
-    sampled = list(range(0, K, EVERY))
-    M = len(sampled)
-    ev = cans.EventSet(2 * M + 2)
-    torch.cuda.synchronize()
-    ev.record(2 * M)               # start of the timed region
-    for m, s0 in enumerate(sampled):
-        run.steps(1, events=ev.at(2 * m))
-        rest = min(EVERY, K - s0) - 1
-        if rest > 0:
-            run.steps(rest)
-    ev.record(2 * M + 1)           # end of the timed region
-    torch.cuda.synchronize()
-    total_ms = ev.elapsed_ms(2 * M, 2 * M + 1)
-    kern_ms = [ev.elapsed_ms(2 * m, 2 * m + 1) for m in range(M)]
-    avg_kern_s = statistics.fmean(kern_ms) / 1e3
-
-    cur, info = run.state()
-    assert info["merr"] == 0 and info["nexec"] == args.warmup + K, info
-    mass1 = float(cur[0][1:-1, 1:-1].sum())
-
+    r = ot_run(n, K, args.warmup)
+    pb, pp, total_ms, avg_kern_s, info = r["pb"], r["pb"].params, r["total_ms"], r["avg_kern_s"], r["info"]
+    big = None
+    if not args.no_extra:
+        # the same loop at 4096^2: 0.94 GB of state per buffer set, beyond
+        # the 256 MB Infinity Cache that holds the 1024^2 state
+        rb = ot_run(4096, 100, 5)
+        zb = 4096 * 4096
+        big = {"cells": zb, "steps": 100, "zone_updates_per_s": zb * 100 / (rb["total_ms"] / 1e3),
+               "ms_per_step": rb["total_ms"] / 100, "avg_launch_us": rb["avg_kern_s"] * 1e6,
+               "hbm_frac": B_ALG_2D * zb / rb["avg_kern_s"] / 1e9 / HBM_PEAK_GBS}
     zones = n * n
     value = zones * K / (total_ms / 1e3)
     achieved = B_ALG_2D * zones / avg_kern_s / 1e9
@@ -204,7 +217,8 @@ def main():
                      "avg_launch_us": avg_kern_s * 1e6},
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "check": {"t": info["t"], "dt_last": info["dt"], "mass_rel_drift": (mass1 - mass0) / mass0},
+        "check": {"t": info["t"], "dt_last": info["dt"], "mass_rel_drift": r["drift"]},
+        "ot4096_fp64": big,
     }
     print(json.dumps(out), flush=True)
 

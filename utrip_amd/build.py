@@ -34,6 +34,13 @@ HIP_FLAGS = [
     f"-I{ROOT / 'include'}",
 ]
 
+# Per-source extra flags.  The 2D MLW kernel runs one wave per SIMD and is
+# bound by dependent FP64 latency; the iterative ILP scheduler orders it for
+# latency (measured 16% faster at 4096^2, same bits).
+FILE_FLAGS = {
+    "cans_mlw2d.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+}
+
 LIB_GPU = PKG / "libcans_gpu.so"
 
 
@@ -51,7 +58,8 @@ def build_gpu(jobs: int = 8) -> Path:
     for src in HIP_SOURCES:
         obj = objdir / (src.stem + ".o")
         objs.append(obj)
-        cmd = [HIPCC, *[f for f in HIP_FLAGS if f != "-shared"], "-c", str(src), "-o", str(obj)]
+        cmd = [HIPCC, *[f for f in HIP_FLAGS if f != "-shared"], *FILE_FLAGS.get(src.name, []), "-c", str(src),
+               "-o", str(obj)]
         print("+", " ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd)))
         if len(procs) >= jobs:

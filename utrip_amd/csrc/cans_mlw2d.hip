@@ -519,12 +519,21 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
     if (do_cfl) block_min_to_global(dtmin_local, red, &a.clk->dtkey[(a.step + 1) % 3]);
 }
 
-constexpr int kOcc = 1;             // waves per SIMD the register budget allows
+// Build-time tuning: waves per SIMD the register budget is compiled for,
+// and a fixed rows-per-wave (0: chosen per grid below).
+#ifndef CANS_MLW2D_OCC
+#define CANS_MLW2D_OCC 1
+#endif
+#ifndef CANS_MLW2D_ROWS
+#define CANS_MLW2D_ROWS 0
+#endif
+constexpr int kOcc = CANS_MLW2D_OCC;
 constexpr int kSlots = 256 * 4 * kOcc; // wave slots of the 256 CUs
 
 // Rows per wave: each wave pays about 1.5 rows of prologue, and the launch
 // runs in ceil(waves / slots) rounds, so pick H minimising rounds * (H + 1.5).
 int pick_rows(int ix, int jx) {
+    if (CANS_MLW2D_ROWS > 0) return CANS_MLW2D_ROWS;
     const int nstrip = (ix - 2 + COLS - 1) / COLS;
     int best = 16;
     double cost = 1e300;
