@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--cpu-steps", type=int, default=40, help="reference steps timed on the host")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true", help="only the main workload (no 4096^2 line)")
+    p.add_argument("--c2-steps", type=int, default=10, help="timed steps of the 3D blast figures (0: skip)")
+    p.add_argument("--c2-n", type=int, default=384, help="interior cells per side of the 3D blast")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run 5 device-to-device copies of this many MiB (the byte-counter calibration "
                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -116,6 +118,58 @@ def cpu_baseline(pb, nsteps: int):
                       f"with amdflang -O2 -ffp-contract=off, single thread"}
 
 
+def blast3d_figures(n: int, steps: int):
+    """The 384^3 periodic MHD blast wave (rho=1, p=0.1, p=10 inside r<0.1,
+    B_HL=(1,1,0)/sqrt(2)) on the resident run: 3D MLW (FP64), Roe-MUSCL and
+    HLLD in FP64 and with FP32 face fluxes; zone-updates/s of each and the
+    relative L1 difference per field between the FP32-flux and FP64 runs
+    after the same number of steps."""
+    import gc
+
+    import numpy as np
+    import torch
+
+    from utrip_amd import cans, problems
+    dev = torch.device("cuda:0")
+    F3 = cans.FIELDS[3]
+    out = {"cells": n ** 3, "steps": steps, "warmup": 2}
+    finals = {}
+    for engine, mixed in (("mlw", False), ("roe", False), ("roe", True), ("hlld", False), ("hlld", True)):
+        margin = 2 if engine == "mlw" else 3
+        pb = problems.blast_3d(n=n, margin=margin, gaussian_b=(engine != "hlld"))
+        st = [torch.from_numpy(np.ascontiguousarray(pb.fields[k].T)).to(dev) for k in F3]
+        gd = {k: torch.from_numpy(pb.grid[k]).to(dev) for k in ("dx", "dxm", "dy", "dym", "dz", "dzm")}
+        del pb
+        eng = {"mlw": cans.ENGINE_MLW, "roe": cans.ENGINE_ROE, "hlld": cans.ENGINE_HLLD}[engine]
+        run = cans.Run(3, eng, st, gd, 5.0 / 3.0, 0.4, qav=3.0, mu=1.0, margin=margin, bc=cans.BD_PER,
+                       mixed=mixed)
+        run.reset(dtmin=1e-10)
+        run.steps(2)
+        ev = cans.EventSet(2)
+        torch.cuda.synchronize()
+        ev.record(0)
+        run.steps(steps)
+        ev.record(1)
+        torch.cuda.synchronize()
+        ms = ev.elapsed_ms(0, 1)
+        cur, info = run.state()
+        assert info["merr"] == 0 and info["nexec"] == steps + 2, info
+        name = engine + ("_fp32flux" if mixed else "_fp64")
+        out[name] = {"zone_updates_per_s": n ** 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
+                     "hbm_frac_alg": 128 * n ** 3 * steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        if engine != "mlw":
+            finals[name] = [t.clone() for t in cur]   # kept on the GPU (3.7 GB per run at 384^3)
+        del run, st, gd, cur
+        gc.collect()
+        torch.cuda.empty_cache()
+    for engine in ("roe", "hlld"):
+        a, b = finals[engine + "_fp64"], finals[engine + "_fp32flux"]
+        out[engine + "_rel_l1_fp32flux_vs_fp64"] = {
+            k: float((x - y).abs().sum() / y.abs().sum()) if float(y.abs().sum()) > 0 else float((x - y).abs().sum())
+            for k, x, y in zip(F3, b, a)}
+    return out
+
+
 def main():
     args = parse()
     if args.copy_mib > 0:
@@ -176,6 +230,9 @@ def main():
     K = args.steps
     r = ot_run(n, K, args.warmup)
     pb, pp, total_ms, avg_kern_s, info = r["pb"], r["pb"].params, r["total_ms"], r["avg_kern_s"], r["info"]
+    c2 = None
+    if args.c2_steps > 0:
+        c2 = blast3d_figures(args.c2_n, args.c2_steps)
     big = None
     if not args.no_extra:
         # the same loop at 4096^2: 0.94 GB of state per buffer set, beyond
@@ -219,6 +276,7 @@ def main():
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "check": {"t": info["t"], "dt_last": info["dt"], "mass_rel_drift": r["drift"]},
         "ot4096_fp64": big,
+        "blast384": c2,
     }
     print(json.dumps(out), flush=True)
 

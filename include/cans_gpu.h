@@ -173,6 +173,48 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
              const double *dym, int jx, const double *dz, const double *dzm, int kx, const int *bc,
              int margin, void *ws, void *const *events, void *stream);
 
+/* ---------------------------------------------------------------- batched 1D tubes
+ * ntube independent 1D problems, each running the cans1d main loop
+ * (cans1d/md_mhdshktb/hdmlw/main.f:132-186) to its own end:
+ *     ns=ns+1; cfl_m (common/cfl_m.f:2); stop if merr; t=t+dt;
+ *     mlw_m (hdmlw/mlw_m.f:2); bdfrex low and high on ro,pr,vx,vy,by
+ *     (bc/bdfrex.f:30-40); repeat while ns < nstop and t < tend
+ * with t starting at 0.  Packed layout: tube k is cells off[k] .. off[k]+ix[k]-1
+ * of every array (ix[k] = nx_k + 2*margin, ghosts included); off, ix, order
+ * and the outputs t_out, ns_out, merr_out are device arrays of ntube entries.
+ * order (NULL: 0..ntube-1) is the queue order of the persistent kernel --
+ * longest tubes first balances the GPU.  The state ro..by is updated in
+ * place.  mixed = 0 is bit-identical to the reference loop per tube;
+ * mixed = 1 evaluates cell fluxes, half step and face fluxes in FP32 and
+ * accumulates in FP64.  block_threads in {64,128,256,512}, blocks_per_cu
+ * 1..32 (block_threads*blocks_per_cu <= 2048).  ws: cans1d_batch_ws_bytes
+ * (tune = 1 adds room for cans1d_batch_tune's copy of the state).
+ * No single counterpart in the reference, whose 1D driver runs one tube. */
+size_t cans1d_batch_ws_bytes(long long ncells, int tune);
+int cans1d_batch_run(int ntube, const long long *off, const int *ix, const int *order, double *ro, double *pr,
+                     double *vx, double *vy, double *by, const double *bx, const double *bxm, const double *dx,
+                     const double *dxm, long long ncells, double gm, double qav, double safety, double dtmin,
+                     double tend, int nstop, int margin, int mixed, double *t_out, int *ns_out, int *merr_out,
+                     int block_threads, int blocks_per_cu, void *ws, void *stream);
+/* Autotuner of the persistent kernel's launch shape: times every
+ * (block_threads, blocks_per_cu) on a copy of the state for probe_steps
+ * steps per tube and returns the fastest (cached per batch shape for the
+ * process).  The state is not changed; t_out/ns_out/merr_out are. */
+int cans1d_batch_tune(int ntube, const long long *off, const int *ix, const int *order, double *ro, double *pr,
+                      double *vx, double *vy, double *by, const double *bx, const double *bxm, const double *dx,
+                      const double *dxm, long long ncells, double gm, double qav, double safety, double dtmin,
+                      int margin, int mixed, int probe_steps, double *t_out, int *ns_out, int *merr_out, void *ws,
+                      int *block_threads, int *blocks_per_cu, float *best_ms, void *stream);
+/* The whole batch run as an instantiated hipGraph (optionally preceded by a
+ * copy of init5[0..4] into ro..by, so every replay is the same run). */
+int cans1d_batch_graph(int ntube, const long long *off, const int *ix, const int *order, double *ro, double *pr,
+                       double *vx, double *vy, double *by, const double *bx, const double *bxm, const double *dx,
+                       const double *dxm, long long ncells, double gm, double qav, double safety, double dtmin,
+                       double tend, int nstop, int margin, int mixed, double *t_out, int *ns_out, int *merr_out,
+                       int block_threads, int blocks_per_cu, void *ws, const double *const *init5, void **graph);
+int cans_graph_launch(void *graph, void *stream);
+int cans_graph_destroy(void *graph);
+
 /* HIP events for timing the engine kernels of a run: `events` of the run
  * entry points may be NULL or hold 2*nsteps events; the engine kernel of the
  * s-th step of the call is bracketed by events[2s] and events[2s+1], recorded

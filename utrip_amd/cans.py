@@ -422,3 +422,128 @@ class Run:
     def state(self):
         info = self.clock.read()
         return (self.a if info["nexec"] % 2 == 0 else self.b), info
+
+
+def _iptr(t: torch.Tensor, dtype, n: int, name: str) -> C.c_void_p:
+    if t.dtype != dtype or not t.is_cuda or not t.is_contiguous() or t.numel() != n:
+        raise ValueError(f"{name}: expected a contiguous {dtype} GPU tensor of {n} entries")
+    return C.c_void_p(t.data_ptr())
+
+
+class Batch1D:
+    """Many independent 1D tubes, each running the cans1d main loop
+    {cfl_m; t=t+dt; mlw_m; bdfrex both ends} to its own end
+    (cans1d/md_mhdshktb/hdmlw/main.f:132-186), in one persistent kernel
+    (cans1d_batch_run).
+
+    ``tubes``: a list of dicts of 1D numpy arrays ro, pr, vx, vy, by, bx,
+    bxm, dx, dxm (ghosts included, margin filled).  The arrays are packed
+    into one device array per field; tube k occupies ``slice(off[k],
+    off[k]+ix[k])``.  ``mixed``: FP32 fluxes / half step, FP64 accumulation.
+    The queue order is longest-first by the estimated step count
+    ``ix / dt0`` (dt0 from the initial cfl_m)."""
+
+    FIELDS = ("ro", "pr", "vx", "vy", "by")
+    COEF = ("bx", "bxm", "dx", "dxm")
+
+    def __init__(self, tubes, gm, qav, safety, margin, dtmin=1e-10, mixed=False, device="cuda:0", tune_room=True):
+        import numpy as np
+        self.n = len(tubes)
+        self.gm, self.qav, self.safety, self.margin, self.dtmin, self.mixed = gm, qav, safety, margin, dtmin, mixed
+        ix = np.array([len(t["ro"]) for t in tubes], dtype=np.int32)
+        off = np.zeros(self.n, dtype=np.int64)
+        if self.n:
+            off[1:] = np.cumsum(ix[:-1], dtype=np.int64)
+        self.ncells = int(ix.sum())
+        self.ix_host, self.off_host = ix, off
+        dev = torch.device(device)
+
+        def pack(k):
+            a = np.concatenate([np.asarray(t[k], dtype=np.float64) for t in tubes]) if self.n else np.zeros(0)
+            return torch.from_numpy(a).to(dev)
+
+        self.f = [pack(k) for k in self.FIELDS]
+        self.c = [pack(k) for k in self.COEF]
+        # longest first: steps ~ 1/dt0, work per step ~ ix
+        cost = np.zeros(self.n)
+        for k, t in enumerate(tubes):
+            ro, pr, vx, by, bx, dx = (np.asarray(t[q], dtype=np.float64) for q in ("ro", "pr", "vx", "by", "bx", "dx"))
+            s = slice(1, len(ro) - 1)
+            cf = np.abs(vx[s]) + np.sqrt(gm * pr[s] / ro[s] + (bx[s] ** 2 + by[s] ** 2) * 0.25 / np.pi / ro[s])
+            cost[k] = len(ro) * float(np.max(cf / dx[s]))
+        self.order_host = np.argsort(-cost, kind="stable").astype(np.int32)
+        self.off = torch.from_numpy(off).to(dev)
+        self.ix = torch.from_numpy(ix).to(dev)
+        self.order = torch.from_numpy(self.order_host).to(dev)
+        self.t = torch.zeros(self.n, dtype=torch.float64, device=dev)
+        self.ns = torch.zeros(self.n, dtype=torch.int32, device=dev)
+        self.merr = torch.zeros(self.n, dtype=torch.int32, device=dev)
+        lib().cans1d_batch_ws_bytes.restype = C.c_size_t
+        nb = lib().cans1d_batch_ws_bytes(C.c_longlong(self.ncells), C.c_int(1 if tune_room else 0))
+        self.ws = torch.empty((nb + 7) // 8, dtype=torch.float64, device=dev)
+        self.shape = (256, 2)   # (block_threads, blocks_per_cu) until tuned
+        self._graphs = []
+        torch.cuda.synchronize()   # off/ix/order are read on the host by the calls
+
+    def _common(self):
+        n, nc = self.n, self.ncells
+        return [C.c_int(n), _iptr(self.off, torch.int64, n, "off"), _iptr(self.ix, torch.int32, n, "ix"),
+                _iptr(self.order, torch.int32, n, "order"), *[_ptr(t, (nc,)) for t in self.f],
+                *[_ptr(t, (nc,)) for t in self.c], C.c_longlong(nc), _d(self.gm), _d(self.qav), _d(self.safety),
+                _d(self.dtmin)]
+
+    def _outs(self):
+        n = self.n
+        return [_ptr(self.t, (n,)), _iptr(self.ns, torch.int32, n, "ns"), _iptr(self.merr, torch.int32, n, "merr")]
+
+    def tune(self, probe_steps: int = 20):
+        """Pick (block_threads, blocks_per_cu) by timing every candidate on
+        a copy of the state (the state is unchanged)."""
+        bt, bpc, ms = C.c_int(), C.c_int(), C.c_float()
+        _check(lib().cans1d_batch_tune(*self._common(), C.c_int(self.margin), C.c_int(1 if self.mixed else 0),
+                                       C.c_int(probe_steps), *self._outs(), _ptr(self.ws), C.byref(bt), C.byref(bpc),
+                                       C.byref(ms), _stream()))
+        self.shape = (bt.value, bpc.value)
+        return bt.value, bpc.value, ms.value
+
+    def run(self, tend: float, nstop: int = 2**31 - 1):
+        bt, bpc = self.shape
+        _check(lib().cans1d_batch_run(*self._common(), _d(tend), C.c_int(nstop), C.c_int(self.margin),
+                                      C.c_int(1 if self.mixed else 0), *self._outs(), C.c_int(bt), C.c_int(bpc),
+                                      _ptr(self.ws), _stream()))
+
+    def graph(self, tend: float, nstop: int = 2**31 - 1, init=None):
+        """An instantiated hipGraph of the whole run; ``init`` (5 packed
+        device tensors ro..by) is copied into the state at the start of
+        every replay."""
+        bt, bpc = self.shape
+        arr = None
+        if init is not None:
+            arr = (C.c_void_p * 5)(*[_ptr(t, (self.ncells,)).value for t in init])
+        h = C.c_void_p()
+        _check(lib().cans1d_batch_graph(*self._common(), _d(tend), C.c_int(nstop), C.c_int(self.margin),
+                                        C.c_int(1 if self.mixed else 0), *self._outs(), C.c_int(bt), C.c_int(bpc),
+                                        _ptr(self.ws), arr, C.byref(h)))
+        g = BatchGraph(h)
+        self._graphs.append(g)
+        return g
+
+    def tube(self, k: int):
+        """Host copy of tube k's fields."""
+        s = slice(int(self.off_host[k]), int(self.off_host[k] + self.ix_host[k]))
+        return {q: t[s].cpu().numpy() for q, t in zip(self.FIELDS, self.f)}
+
+
+class BatchGraph:
+    def __init__(self, handle):
+        self.h = handle
+
+    def launch(self):
+        _check(lib().cans_graph_launch(self.h, _stream()))
+
+    def __del__(self):
+        try:
+            if _lib is not None and self.h:
+                _lib.cans_graph_destroy(self.h)
+        except Exception:
+            pass

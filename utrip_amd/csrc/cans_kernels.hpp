@@ -105,6 +105,26 @@ struct CflArgs {
     int step;
 };
 
+// Batched 1D tubes (packed: tube k occupies cells off[k] .. off[k]+ix[k]-1
+// of every array).
+struct Batch1dArgs {
+    int ntube;
+    long long ncells;            // total packed cells
+    const long long *off;        // device, per tube
+    const int *ix;               // device, per tube (nx + 2*margin)
+    const int *order;            // device queue order (NULL: 0..ntube-1)
+    double *f[5];                // ro pr vx vy by (state, in place)
+    double *g[5];                // scratch set of the same size
+    const double *bx, *bxm, *dx, *dxm;
+    double *di, *dim;            // 1/dx, 1/dxm (filled by the launch)
+    double gm, qav, safety, dtmin, tend, t0;
+    int nstop, margin;
+    double *t_out;               // device, per tube
+    int *ns_out, *merr_out;      // device, per tube
+    unsigned *next;              // queue counter (zeroed by the launch)
+};
+hipError_t launch_batch1d(const Batch1dArgs &a, int mixed, int block_threads, int nblocks, hipStream_t s);
+
 hipError_t launch_metric(const double *d, const double *dm, int n, Metric m, hipStream_t s);
 hipError_t launch_mlw2d(const Mlw2dArgs &a, hipStream_t s);
 hipError_t launch_mlw1d(const Mlw1dArgs &a, hipStream_t s);

@@ -166,3 +166,32 @@ def blast_3d(n: int = 384, margin: int = 2, gm: float = 5.0 / 3.0, gaussian_b: b
     return Problem("blast_3d", f, dict(x=x, dx=dx, dxm=dxm, y=x.copy(), dy=dx.copy(), dym=dxm.copy(),
                                        z=x.copy(), dz=dx.copy(), dzm=dxm.copy()),
                    dict(gm=gm, margin=margin, safety=0.4, qav=3.0, dtmin=1e-10))
+
+
+C3_SEED = 20160809
+
+
+def tube_set(ntube: int = 4096, seed: int = C3_SEED, nx_lo: int = 256, nx_hi: int = 4096):
+    """The batched-tube workload: ``ntube`` Brio-Wu shock tubes (gamma 2,
+    Bx_HL 0.75, By_HL +-1, rho 1 / 0.125) with nx = rng.integers(nx_lo,
+    nx_hi+1) and plasma beta of the left state log-uniform in [1e-2, 1e2]
+    (beta = 2 p_L / |B_HL|^2 = 2 p_L / 1.5625), reached by scaling both
+    pressures: p_L = s, p_R = 0.1 s with s = beta * 1.5625 / 2.
+    rng = numpy.random.default_rng(seed).  Returns (nx, beta, s)."""
+    rng = np.random.default_rng(seed)
+    nx = rng.integers(nx_lo, nx_hi + 1, ntube)
+    beta = 10.0 ** rng.uniform(-2.0, 2.0, ntube)
+    s = beta * 1.5625 / 2.0
+    return nx.astype(np.int64), beta, s
+
+
+def tubes(nx, s, margin: int = 2):
+    """Shock-tube problems (shock_tube_1d) for the (nx, pressure scale) pairs
+    of tube_set, as the dicts Batch1D packs."""
+    out = []
+    for n, sc in zip(nx, s):
+        pb = shock_tube_1d(nx=int(n), margin=margin, p_l=float(sc), p_r=0.1 * float(sc))
+        d = {k: pb.fields[k] for k in ("ro", "pr", "vx", "vy", "by", "bx", "bxm")}
+        d["dx"], d["dxm"] = pb.grid["dx"], pb.grid["dxm"]
+        out.append(d)
+    return out
