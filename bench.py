@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--no-extra", action="store_true", help="only the main workload (no 4096^2 line)")
     p.add_argument("--c2-steps", type=int, default=10, help="timed steps of the 3D blast figures (0: skip)")
     p.add_argument("--c2-n", type=int, default=384, help="interior cells per side of the 3D blast")
+    p.add_argument("--tubes", type=int, default=4096, help="batched 1D tubes to t=0.1 (0: skip)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run 5 device-to-device copies of this many MiB (the byte-counter calibration "
                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -170,6 +171,53 @@ def blast3d_figures(n: int, steps: int):
     return out
 
 
+def tubes_figures(ntube: int = 4096, tend: float = 0.1):
+    """The batched-tube workload: ntube seeded Brio-Wu tubes (nx 256..4096,
+    beta 1e-2..1e2, tests/golden/tubes_4096.npz), each run to t >= tend
+    with its own cfl dt in one persistent kernel; FP64 (bit-exact path) and
+    FP32 fluxes, after the launch-shape autotuner.  zone-updates = sum over
+    tubes of nx x steps."""
+    import numpy as np
+    import torch
+
+    from utrip_amd import cans, problems
+    nx, beta, s = problems.tube_set(ntube)
+    tubes = problems.tubes(nx, s, margin=2)
+    out = {"tubes": ntube, "cells": int(nx.sum()), "tend": tend, "seed": problems.C3_SEED}
+    finals = {}
+    for mixed in (False, True):
+        b = cans.Batch1D(tubes, 2.0, 3.0, 0.4, 2, mixed=mixed)
+        init = [t.clone() for t in b.f]
+        bt, bpc, tune_ms = b.tune(probe_steps=20)
+        g = b.graph(tend=tend, init=init)
+        g.launch()                      # warm-up replay
+        torch.cuda.synchronize()
+        ev = cans.EventSet(2)
+        ev.record(0)
+        g.launch()                      # timed replay: the whole batch from the initial state
+        ev.record(1)
+        torch.cuda.synchronize()
+        ms = ev.elapsed_ms(0, 1)
+        ns = b.ns.cpu().numpy().astype(np.int64)
+        merr = b.merr.cpu().numpy()
+        assert (merr == 0).all() and (b.t.cpu().numpy() >= tend).all()
+        zu = int((nx * ns).sum())
+        name = "fp32flux" if mixed else "fp64"
+        out[name] = {"zone_updates": zu, "ms": ms, "zone_updates_per_s": zu / (ms / 1e3),
+                     "steps_min": int(ns.min()), "steps_max": int(ns.max()), "launch_shape": [bt, bpc],
+                     "hbm_frac_alg": 112 * zu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        finals[name] = ([t.clone() for t in b.f], b)
+    a, _ = finals["fp64"]
+    c, bb = finals["fp32flux"]
+    rel = []
+    for k in range(ntube):
+        sl = slice(int(bb.off_host[k]), int(bb.off_host[k] + bb.ix_host[k]))
+        rel.append(float((c[0][sl] - a[0][sl]).abs().sum() / a[0][sl].abs().sum()))
+    rel = np.array(rel)
+    out["rho_rel_l1_fp32flux_vs_fp64"] = {"max": float(rel.max()), "median": float(np.median(rel))}
+    return out
+
+
 def main():
     args = parse()
     if args.copy_mib > 0:
@@ -230,6 +278,9 @@ def main():
     K = args.steps
     r = ot_run(n, K, args.warmup)
     pb, pp, total_ms, avg_kern_s, info = r["pb"], r["pb"].params, r["total_ms"], r["avg_kern_s"], r["info"]
+    tb = None
+    if args.tubes > 0:
+        tb = tubes_figures(args.tubes)
     c2 = None
     if args.c2_steps > 0:
         c2 = blast3d_figures(args.c2_n, args.c2_steps)
@@ -277,6 +328,7 @@ def main():
         "check": {"t": info["t"], "dt_last": info["dt"], "mass_rel_drift": r["drift"]},
         "ot4096_fp64": big,
         "blast384": c2,
+        "tubes": tb,
     }
     print(json.dumps(out), flush=True)
 
