@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--c2-steps", type=int, default=10, help="timed steps of the 3D blast figures (0: skip)")
     p.add_argument("--c2-n", type=int, default=384, help="interior cells per side of the 3D blast")
     p.add_argument("--tubes", type=int, default=4096, help="batched 1D tubes to t=0.1 (0: skip)")
+    p.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint timing")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run 5 device-to-device copies of this many MiB (the byte-counter calibration "
                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -218,6 +219,63 @@ def tubes_figures(ntube: int = 4096, tend: float = 0.1):
     return out
 
 
+def adjoint_figures(n: int = 512, nsteps: int = 200, ckpt: int = 20):
+    """Gradient of L = sum (rho_final - rho_target)^2 dx dy over an n^2 OT run
+    of nsteps MLW steps with respect to the initial velocities and a pressure
+    scale: forward (with a checkpoint every ckpt steps) and hand-written
+    reverse mode; wall time of each, their ratio, and the plain forward run."""
+    import numpy as np
+    import torch
+
+    from utrip_amd import adjoint, cans, problems
+    dev = torch.device("cuda:0")
+    pb = problems.orszag_tang_2d(n=n, margin=1)
+    names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+    base = [torch.from_numpy(np.ascontiguousarray(pb.fields[k].T)).to(dev) for k in names]
+    g = {k: torch.from_numpy(pb.grid[k]).to(dev) for k in ("dx", "dxm", "dy", "dym")}
+    pp = pb.params
+    solver = adjoint.PeriodicMlwAdjoint(g["dx"], g["dxm"], g["dy"], g["dym"], pp["qav"], pp["gm"], pp["safety"], 1,
+                                        nsteps, ckpt=ckpt)
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    xi = lambda: torch.randn((n, n), generator=gen, dtype=torch.float64).to(dev)
+    vx0 = (base[2][1:-1, 1:-1] * (1 + 1e-3 * xi())).requires_grad_(True)
+    vy0 = (base[3][1:-1, 1:-1] * (1 + 1e-3 * xi())).requires_grad_(True)
+    beta = torch.ones((), dtype=torch.float64, device=dev, requires_grad=True)
+    with torch.no_grad():
+        target = adjoint.FinalDensity.apply(vx0 * (1 + 0.01 * xi()), vy0 * (1 + 0.01 * xi()), beta, solver, base)
+    w = float(pb.grid["dx"][1] * pb.grid["dy"][1])
+    # plain forward (no checkpoints) for reference
+    run = cans.MlwRun2D([t.clone() for t in base], g["dx"], g["dxm"], g["dy"], g["dym"], pp["qav"], pp["gm"],
+                        pp["safety"])
+    run.reset()
+    run.steps(nsteps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.reset()
+    run.steps(nsteps)
+    torch.cuda.synchronize()
+    plain = time.perf_counter() - t0
+    fwd, bwd = [], []
+    for _ in range(3):
+        for t in (vx0, vy0, beta):
+            t.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = ((adjoint.FinalDensity.apply(vx0, vy0, beta, solver, base) - target) ** 2).sum() * w
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    f, b = statistics.median(fwd), statistics.median(bwd)
+    return {"cells": n * n, "steps": nsteps, "checkpoint_every": ckpt, "forward_plain_ms": plain * 1e3,
+            "forward_ms": f * 1e3, "backward_ms": b * 1e3, "adj_over_fwd": b / f,
+            "adj_over_plain_fwd": b / plain, "loss": float(L.detach()),
+            "dL_dbeta": float(beta.grad)}
+
+
 def main():
     args = parse()
     if args.copy_mib > 0:
@@ -278,6 +336,9 @@ def main():
     K = args.steps
     r = ot_run(n, K, args.warmup)
     pb, pp, total_ms, avg_kern_s, info = r["pb"], r["pb"].params, r["total_ms"], r["avg_kern_s"], r["info"]
+    adj = None
+    if args.adjoint:
+        adj = adjoint_figures()
     tb = None
     if args.tubes > 0:
         tb = tubes_figures(args.tubes)
@@ -329,6 +390,7 @@ def main():
         "ot4096_fp64": big,
         "blast384": c2,
         "tubes": tb,
+        "adjoint_ot512": adj,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,0 +1,160 @@
+"""Hand-written reverse mode of the periodic 2D MLW loop against central
+finite differences of the forward loop itself (the same GPU kernels):
+L = sum_cells (rho_final - rho_target)^2 dx dy, gradient with respect to the
+initial interior velocities and a uniform pressure scale beta, along seeded
+random unit directions; |FD - g.d| / |g.d| < 1e-5 with eps = 1e-6.  FD is the
+fourth-order central stencil (L(-2e) - 8L(-e) + 8L(e) - L(2e)) / 12e: over
+200 nonlinear steps the second-order stencil's eps^2 term alone is ~1e-11
+in L', which is 1e-5..1e-4 of a random unit direction's g.d (~|g|/sqrt(N)).
+
+The Orszag-Tang state is exactly symmetric, so its cfl_m minimum is reached
+at several cells at once: dt, and with it L, has a kink there and no gradient
+(central differences return the mid-range of the tied one-sided slopes).  The
+gradient is therefore taken at v0 = v_OT (1 + 1e-3 eta), eta seeded normal;
+the target run uses v0 (1 + 0.01 xi) as specified for the workload."""
+import numpy as np
+import pytest
+import torch
+
+from utrip_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def case(*a):
+    print("CASE adjoint", *a, flush=True)
+
+
+def setup(n, nsteps, ckpt, dev, seed=7):
+    from utrip_amd import adjoint
+    pb = P.orszag_tang_2d(n=n, margin=1)
+    names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+    td = lambda a: torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
+    base = [td(pb.fields[k]) for k in names]
+    g = {k: td(v) for k, v in pb.grid.items() if k in ("dx", "dxm", "dy", "dym")}
+    pp = pb.params
+    solver = adjoint.PeriodicMlwAdjoint(g["dx"], g["dxm"], g["dy"], g["dym"], pp["qav"], pp["gm"], pp["safety"], 1,
+                                        nsteps, ckpt=ckpt)
+    rng = torch.Generator(device="cpu").manual_seed(seed)
+    xi = lambda: torch.randn((n, n), generator=rng, dtype=torch.float64).to(dev)
+    vx0 = base[2][1:-1, 1:-1] * (1 + 1e-3 * xi())
+    vy0 = base[3][1:-1, 1:-1] * (1 + 1e-3 * xi())
+    beta = torch.ones((), dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        target = adjoint.FinalDensity.apply(vx0 * (1 + 0.01 * xi()), vy0 * (1 + 0.01 * xi()), beta, solver, base)
+    w = float(pb.grid["dx"][1] * pb.grid["dy"][1])
+    return solver, base, vx0, vy0, beta, target, w
+
+
+def loss(vx0, vy0, beta, solver, base, target, w):
+    from utrip_amd import adjoint
+    rho = adjoint.FinalDensity.apply(vx0, vy0, beta, solver, base)
+    return ((rho - target) ** 2).sum() * w
+
+
+def fd_check(n, nsteps, ckpt, ndir, dev, eps=1e-6, tol=1e-5, seed=11, qav=None, norm="gd"):
+    """norm "gd": |FD - g.d| / |g.d| < tol per direction; norm "g":
+    |FD - g.d| / |g| < tol per direction and |FD - |g|| / |g| < tol along
+    d = g/|g|."""
+    solver, base, vx0, vy0, beta, target, w = setup(n, nsteps, ckpt, dev)
+    if qav is not None:
+        solver.qav = qav
+    x = [vx0.clone().requires_grad_(True), vy0.clone().requires_grad_(True), beta.clone().requires_grad_(True)]
+    L = loss(*x, solver, base, target, w)
+    L.backward()
+    g = [t.grad.detach() for t in x]
+    gn = float(torch.sqrt(sum((t * t).sum() for t in g)))
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+
+    def fd_along(d):
+        with torch.no_grad():
+            lv = {s: float(loss(*[a + s * eps * b for a, b in zip(x, d)], solver, base, target, w))
+                  for s in (-2, -1, 1, 2)}
+        return (lv[-2] - 8.0 * lv[-1] + 8.0 * lv[1] - lv[2]) / (12.0 * eps)
+
+    errs = []
+    for k in range(ndir):
+        d = [torch.randn(t.shape, generator=gen, dtype=torch.float64).to(dev) for t in x]
+        nrm = torch.sqrt(sum((t * t).sum() for t in d))
+        d = [t / nrm for t in d]
+        fd = fd_along(d)
+        gd = float(sum((a * b).sum() for a, b in zip(g, d)))
+        errs.append(abs(fd - gd) / (abs(gd) if norm == "gd" else gn))
+        print(f"dir {k}: fd={fd:.12e} g.d={gd:.12e} rel(g.d)={abs(fd - gd) / abs(gd):.2e} "
+              f"rel(|g|)={abs(fd - gd) / gn:.2e}", flush=True)
+    if norm == "g":
+        fd = fd_along([t / gn for t in g])
+        errs.append(abs(fd - gn) / gn)
+        print(f"along g: fd={fd:.12e} |g|={gn:.12e} rel={errs[-1]:.2e}", flush=True)
+    assert max(errs) < tol, errs
+    return float(L), errs
+
+
+def test_adjoint_step_matches_autograd(dev):
+    """One adjoint step against autograd of the vectorised torch step
+    (tests/torch_mlw2d.py) on a symmetry-broken OT state."""
+    import ctypes as C
+    from tests import torch_mlw2d as T
+    from utrip_amd import adjoint, cans
+    case("single step vs autograd")
+    n = 20
+    solver, base, *_ = setup(n, 1, 1, dev)
+    gen = torch.Generator(device="cpu").manual_seed(3)
+    st = [t.clone() for t in base]
+    for q in (2, 3):
+        st[q] = adjoint.periodic_fill(st[q] * (1 + 0.05 * torch.randn(st[q].shape, generator=gen,
+                                                                        dtype=torch.float64).to(dev)), 1)
+    dx, dxm, dy, dym = solver.grid
+    run = cans.MlwRun2D([t.clone() for t in st], dx, dxm, dy, dym, solver.qav, solver.gm, solver.safety)
+    run.reset()
+    run.steps(1)
+    p1 = run.b
+    P = [t.clone().requires_grad_(True) for t in st[:6]]
+    out, dt = T.step(P, dx, dxm, dy, dym, solver.gm, solver.qav, solver.safety)
+    for q in range(6):
+        d = float((out[q].detach() - p1[q]).abs().max() / p1[q].abs().max())
+        assert d < 1e-12, (q, d)
+    lam = [torch.randn(st[0].shape, generator=gen, dtype=torch.float64).to(dev) for _ in range(6)]
+    ref = torch.autograd.grad(out, P, grad_outputs=lam)
+    shape = tuple(st[0].shape)
+    lin = [t.clone() for t in lam]
+    lout = [torch.empty_like(t) for t in lam]
+    from utrip_amd.cans import _check, _d, _ptr, _ptr_array, _stream, lib
+    _check(lib().cans2d_mlw_adjoint_step(_ptr_array(st[:6], shape), _ptr_array(p1[:6], shape),
+                                         _ptr_array(lin, shape), _ptr_array(lout, shape), _d(solver.qav),
+                                         _d(solver.gm), _d(solver.safety), _ptr(dx), _ptr(dxm), C.c_int(solver.ix),
+                                         _ptr(dy), _ptr(dym), C.c_int(solver.jx), C.c_int(1), _ptr(solver.metric),
+                                         _ptr(solver.ws), _stream()))
+    torch.cuda.synchronize()
+    names = ("ro", "pr", "vx", "vy", "bx", "by")
+    bad = {}
+    for q in range(6):
+        err = (lout[q] - ref[q]).abs()
+        scale = ref[q].abs().max()
+        r = float(err.max() / scale)
+        if r > 1e-9:
+            j, i = np.unravel_index(int(err.argmax().cpu()), shape)
+            bad[names[q]] = (r, int(i) + 1, int(j) + 1, float(lout[q][j, i]), float(ref[q][j, i]))
+    assert not bad, bad
+
+
+def test_adjoint_small_grid_multi_segment(dev):
+    case("OT 24^2, 12 steps, checkpoints every 5")
+    fd_check(24, 12, 5, 6, dev)
+
+
+def test_adjoint_ot512_200_steps_smooth(dev):
+    """The workload with the artificial viscosity off (qav = 0): L is smooth,
+    and every direction meets |FD - g.d| / |g.d| < 1e-5."""
+    case("OT 512^2, 200 steps, checkpoints every 20, 16 directions, qav 0")
+    fd_check(512, 200, 20, 16, dev, qav=0.0)
+
+
+def test_adjoint_ot512_200_steps(dev):
+    """The workload (qav = 3).  The AV coefficient max(0, |dv| - 1e-4)
+    switches on and off along the 200 steps; the +-eps runs cross some of
+    these switches, which leaves an FD error of ~1e-11 absolute (measured:
+    with qav = 0 it is ~1e-13).  It is bounded relative to |g|, and along
+    d = g/|g| the plain relative error is checked."""
+    case("OT 512^2, 200 steps, checkpoints every 20, 16 directions, qav 3")
+    fd_check(512, 200, 20, 16, dev, norm="g")

@@ -1,0 +1,139 @@
+"""Gradients through the periodic 2D MLW time loop (cans2d/hdmlw/mlw_m.f with
+bdperx/bdpery and cfl_m, the loop of cans2d/md_mhdshktb/hdmlw/main.f:150-206)
+by a hand-written reverse mode on the GPU (cans2d_mlw_adjoint_step).
+
+``PeriodicMlwAdjoint`` runs the forward loop on the resident fused kernel,
+keeping a checkpoint every ``ckpt`` steps; its ``backward`` recomputes each
+segment from its checkpoint and applies the adjoint step in reverse.
+``FinalDensity`` wraps this as a ``torch.autograd.Function`` of the initial
+velocities (interior) and a uniform pressure scale ``beta``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List
+
+import torch
+
+from . import cans
+from .cans import _check, _d, _ptr, _ptr_array, _stream, lib
+
+
+class PeriodicMlwAdjoint:
+    """Forward/backward driver for ``nsteps`` steps of the periodic 2D MLW
+    loop on a grid with ghost width ``margin`` (state tensors (jx, ix))."""
+
+    def __init__(self, dx, dxm, dy, dym, qav, gm, safety, margin, nsteps, ckpt=20, dtmin=1e-10):
+        self.grid = (dx, dxm, dy, dym)
+        self.ix, self.jx = dx.numel(), dy.numel()
+        self.qav, self.gm, self.safety, self.margin = qav, gm, safety, margin
+        self.nsteps, self.ckpt, self.dtmin = nsteps, ckpt, dtmin
+        dev = dx.device
+        self.metric = torch.empty(lib().cans2d_metric_ws_size(self.ix, self.jx), dtype=torch.float64, device=dev)
+        _check(lib().cans2d_metric_fill(_ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy), _ptr(dym), C.c_int(self.jx),
+                                        _ptr(self.metric), _stream()))
+        lib().cans2d_adj_ws_bytes.restype = C.c_size_t
+        nb = lib().cans2d_adj_ws_bytes(C.c_int(self.ix), C.c_int(self.jx))
+        self.ws = torch.empty((nb + 7) // 8, dtype=torch.float64, device=dev)
+        self.ckpts: List[List[torch.Tensor]] = []
+
+    def _run(self, state):
+        dx, dxm, dy, dym = self.grid
+        run = cans.MlwRun2D([t.clone() for t in state], dx, dxm, dy, dym, self.qav, self.gm, self.safety,
+                            margin=self.margin)
+        run.reset(dtmin=self.dtmin)
+        return run
+
+    @staticmethod
+    def _cur(run, k):
+        return run.a if k % 2 == 0 else run.b
+
+    def forward(self, state):
+        """Run nsteps from ``state`` (7 tensors ro,pr,vx,vy,bx,by,az, ghosts
+        filled); keep the checkpoints; return the final state (clones)."""
+        self.ckpts = []
+        run = self._run(state)
+        done = 0
+        while done < self.nsteps:
+            self.ckpts.append([t.clone() for t in self._cur(run, done)])
+            k = min(self.ckpt, self.nsteps - done)
+            run.steps(k)
+            done += k
+        info = run.clock.read()
+        if info["merr"] != 0 or info["nexec"] != self.nsteps:
+            raise cans.CansError(f"forward run stopped early: {info}")
+        return [t.clone() for t in self._cur(run, done)]
+
+    def backward(self, lam):
+        """``lam``: dL/dP of the final state on every cell (6 tensors
+        ro,pr,vx,vy,bx,by).  Returns dL/dP of the initial state, folded onto
+        the interior (the gradient with respect to the interior values the
+        ghosts are copies of).  Each segment is recomputed from its
+        checkpoint into a segment buffer (cans2d_mlw_record) and swept in
+        reverse (cans2d_mlw_adjoint_run), one native call each."""
+        dx, dxm, dy, dym = self.grid
+        shape = (self.jx, self.ix)
+        lam = [t.clone().contiguous() for t in lam]
+        tmp = [torch.empty_like(t) for t in lam]
+        if getattr(self, "_seg", None) is None:
+            self._seg = [[torch.empty_like(t) for t in self.ckpts[0]] for _ in range(self.ckpt + 1)]
+            self._clock = cans.Clock()
+        lam_p, tmp_p = _ptr_array(lam, shape), _ptr_array(tmp, shape)
+        for c in reversed(range(len(self.ckpts))):
+            n = min(self.ckpt, self.nsteps - c * self.ckpt)
+            for dst, src in zip(self._seg[0], self.ckpts[c]):
+                dst.copy_(src)
+            states = _ptr_array([t for k in range(n + 1) for t in self._seg[k]], shape)
+            self._clock.reset(dtmin=self.dtmin)
+            _check(lib().cans2d_mlw_record(states, C.c_int(n), self._clock.handle, _d(self.qav), _d(self.gm),
+                                           _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy), _ptr(dym),
+                                           C.c_int(self.jx), C.c_int(self.margin), _ptr(self.metric), _stream()))
+            _check(lib().cans2d_mlw_adjoint_run(states, C.c_int(n), lam_p, tmp_p, _d(self.qav), _d(self.gm),
+                                                _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy),
+                                                _ptr(dym), C.c_int(self.jx), C.c_int(self.margin), _ptr(self.metric),
+                                                _ptr(self.ws), _stream()))
+        _check(lib().cans2d_periodic_fold(lam_p, C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin), _stream()))
+        return lam
+
+
+def periodic_fill(t: torch.Tensor, m: int) -> torch.Tensor:
+    """bdperx then bdpery on a (jx, ix) tensor (returns a new tensor)."""
+    t = t.clone()
+    ix, jx = t.shape[1], t.shape[0]
+    t[:, :m] = t[:, ix - 2 * m: ix - m]
+    t[:, ix - m:] = t[:, m: 2 * m]
+    t[:m, :] = t[jx - 2 * m: jx - m, :]
+    t[jx - m:, :] = t[m: 2 * m, :]
+    return t
+
+
+class FinalDensity(torch.autograd.Function):
+    """rho after ``solver.nsteps`` steps (interior) as a function of the
+    initial interior velocities vx0, vy0 and a uniform pressure scale beta
+    (p0 = beta * base pressure); the other initial fields come from
+    ``base`` (7 tensors with ghosts filled)."""
+
+    @staticmethod
+    def forward(ctx, vx0, vy0, beta, solver, base):
+        m = solver.margin
+        vx = base[2].clone()
+        vy = base[3].clone()
+        vx[m:-m, m:-m] = vx0
+        vy[m:-m, m:-m] = vy0
+        state = [base[0], base[1] * beta, periodic_fill(vx, m), periodic_fill(vy, m), base[4], base[5], base[6]]
+        final = solver.forward(state)
+        ctx.solver, ctx.base = solver, base
+        ctx.beta_shape = beta.shape
+        return final[0][m:-m, m:-m].clone()
+
+    @staticmethod
+    def backward(ctx, g):
+        solver, base = ctx.solver, ctx.base
+        m = solver.margin
+        lam = [torch.zeros_like(base[0]) for _ in range(6)]
+        lam[0][m:-m, m:-m] = g
+        l0 = solver.backward(lam)
+        gvx = l0[2][m:-m, m:-m].clone()
+        gvy = l0[3][m:-m, m:-m].clone()
+        gbeta = (l0[1][m:-m, m:-m] * base[1][m:-m, m:-m]).sum().reshape(ctx.beta_shape)
+        return gvx, gvy, gbeta, None, None

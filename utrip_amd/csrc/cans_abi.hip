@@ -360,6 +360,127 @@ int cans2d_mlw_run(double *const *a7, double *const *b7, void *clk, int step0, i
 
 int cans2d_metric_ws_size(int ix, int jx) { return (int)(4 * al(ix) + 4 * al(jx)); }
 
+int cans2d_metric_fill(const double *dx, const double *dxm, int ix, const double *dy, const double *dym, int jx,
+                       double *metric_ws, void *stream) {
+    if (!dx || !dxm || !dy || !dym || !metric_ws || ix < 3 || jx < 3)
+        return fail(CANS_ERR_ARG, "cans2d_metric_fill: bad argument (ix=%d jx=%d)", ix, jx);
+    hipStream_t s = (hipStream_t)stream;
+    double *m = metric_ws;
+    const size_t mxn = al(ix), myn = al(jx);
+    Metric mx{m, m + mxn, m + 2 * mxn, m + 3 * mxn};
+    m += 4 * mxn;
+    Metric my{m, m + myn, m + 2 * myn, m + 3 * myn};
+    HIPCHK(launch_metric(dx, dxm, ix, mx, s));
+    HIPCHK(launch_metric(dy, dym, jx, my, s));
+    return 0;
+}
+
+// ====================================================================== adjoint
+size_t cans2d_adj_ws_bytes(int ix, int jx) { return adj2d_ws_doubles(ix, jx) * sizeof(double); }
+
+int cans2d_mlw_adjoint_step(const double *const *pn6, const double *const *pn1_6, double *const *lam_in6,
+                            double *const *lam_out6, double qav, double gm, double safety, const double *dx,
+                            const double *dxm, int ix, const double *dy, const double *dym, int jx, int margin,
+                            const double *metric_ws, void *ws, void *stream) {
+    if (!pn6 || !pn1_6 || !lam_in6 || !lam_out6 || !dx || !dxm || !dy || !dym || !metric_ws || !ws)
+        return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_step: NULL argument");
+    if (ix < 3 || jx < 3 || margin < 1 || ix - 2 * margin < margin || jx - 2 * margin < margin)
+        return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_step: periodic box needs ix-2*margin >= margin (ix=%d jx=%d margin=%d)",
+                    ix, jx, margin);
+    for (int f = 0; f < 6; f++)
+        if (!pn6[f] || !pn1_6[f] || !lam_in6[f] || !lam_out6[f])
+            return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_step: NULL field %d", f);
+    const double *m = metric_ws;
+    const size_t mxn = al(ix), myn = al(jx);
+    Metric mx{(double *)m, (double *)m + mxn, (double *)m + 2 * mxn, (double *)m + 3 * mxn};
+    m += 4 * mxn;
+    Metric my{(double *)m, (double *)m + myn, (double *)m + 2 * myn, (double *)m + 3 * myn};
+    HIPCHK(launch_adj2d_step(pn6, pn1_6, lam_in6, lam_out6, (double *)ws, dx, dxm, dy, dym, mx, my, qav, gm, safety,
+                             ix, jx, margin, (hipStream_t)stream));
+    return 0;
+}
+
+int cans2d_mlw_record(double *const *states, int nsteps, void *clk, double qav, double gm, double safety,
+                      const double *dx, const double *dxm, int ix, const double *dy, const double *dym, int jx,
+                      int margin, double *metric_ws, void *stream) {
+    if (!states || !clk || !metric_ws || nsteps < 0)
+        return fail(CANS_ERR_ARG, "cans2d_mlw_record: bad argument");
+    if (ix < 3 || jx < 3 || margin < 1 || ix - 2 * margin < margin || jx - 2 * margin < margin)
+        return fail(CANS_ERR_ARG, "cans2d_mlw_record: periodic box needs ix-2*margin >= margin (ix=%d jx=%d margin=%d)",
+                    ix, jx, margin);
+    for (int k = 0; k < 7 * (nsteps + 1); k++)
+        if (!states[k]) return fail(CANS_ERR_ARG, "cans2d_mlw_record: NULL state pointer %d", k);
+    hipStream_t s = (hipStream_t)stream;
+    Clock *c = (Clock *)clk;
+    double *m = metric_ws;
+    const size_t mxn = al(ix), myn = al(jx);
+    Metric mx{m, m + mxn, m + 2 * mxn, m + 3 * mxn};
+    m += 4 * mxn;
+    Metric my{m, m + myn, m + 2 * myn, m + 3 * myn};
+    HIPCHK(launch_metric(dx, dxm, ix, mx, s));
+    HIPCHK(launch_metric(dy, dym, jx, my, s));
+    for (int n = 0; n < nsteps; n++) {
+        double *const *in = states + 7 * n;
+        double *const *out = states + 7 * (n + 1);
+        if (n == 0) {
+            CflArgs ca{};
+            ca.ro = in[0]; ca.pr = in[1]; ca.vx = in[2]; ca.vy = in[3]; ca.bx = in[4]; ca.by = in[5];
+            ca.dx = dx; ca.dy = dy; ca.safety = safety; ca.gm = gm; ca.ix = ix; ca.jx = jx; ca.kx = 1;
+            ca.dim = 2; ca.clk = c; ca.step = 0;
+            HIPCHK(launch_cfl(ca, s));
+        }
+        Mlw2dArgs a{};
+        a.ro = in[0]; a.pr = in[1]; a.vx = in[2]; a.vy = in[3]; a.bx = in[4]; a.by = in[5]; a.az = in[6];
+        a.ro_o = out[0]; a.pr_o = out[1]; a.vx_o = out[2]; a.vy_o = out[3]; a.bx_o = out[4];
+        a.by_o = out[5]; a.az_o = out[6];
+        a.dx = dx; a.dxm = dxm; a.dy = dy; a.dym = dym; a.mx = mx; a.my = my;
+        a.qav = qav; a.gm = gm; a.dt = 0.0; a.clk = c; a.step = n; a.ix = ix; a.jx = jx;
+        a.per_margin = margin;
+        a.safety = safety;
+        HIPCHK(launch_mlw2d(a, s));
+    }
+    return 0;
+}
+
+int cans2d_mlw_adjoint_run(const double *const *states, int nsteps, double *const *lam6, double *const *tmp6,
+                           double qav, double gm, double safety, const double *dx, const double *dxm, int ix,
+                           const double *dy, const double *dym, int jx, int margin, const double *metric_ws, void *ws,
+                           void *stream) {
+    if (!states || !lam6 || !tmp6 || !metric_ws || !ws || nsteps < 0)
+        return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_run: bad argument");
+    if (ix < 3 || jx < 3 || margin < 1 || ix - 2 * margin < margin || jx - 2 * margin < margin)
+        return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_run: periodic box needs ix-2*margin >= margin (ix=%d jx=%d margin=%d)",
+                    ix, jx, margin);
+    for (int k = 0; k < 7 * (nsteps + 1); k++)
+        if (!states[k]) return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_run: NULL state pointer %d", k);
+    for (int f = 0; f < 6; f++)
+        if (!lam6[f] || !tmp6[f]) return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_run: NULL lam/tmp field %d", f);
+    hipStream_t s = (hipStream_t)stream;
+    const double *m = metric_ws;
+    const size_t mxn = al(ix), myn = al(jx);
+    Metric mx{(double *)m, (double *)m + mxn, (double *)m + 2 * mxn, (double *)m + 3 * mxn};
+    m += 4 * mxn;
+    Metric my{(double *)m, (double *)m + myn, (double *)m + 2 * myn, (double *)m + 3 * myn};
+    double *const *cur = lam6;
+    double *const *oth = tmp6;
+    for (int n = nsteps - 1; n >= 0; n--) {
+        HIPCHK(launch_adj2d_step(states + 7 * n, states + 7 * (n + 1), cur, oth, (double *)ws, dx, dxm, dy, dym, mx,
+                                 my, qav, gm, safety, ix, jx, margin, s));
+        double *const *t = cur;
+        cur = oth;
+        oth = t;
+    }
+    if (cur != lam6) HIPCHK(launch_copy6(lam6, cur, (size_t)ix * jx, s));
+    return 0;
+}
+
+int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *stream) {
+    if (!lam6 || ix < 3 || jx < 3 || margin < 1 || ix - 2 * margin < margin || jx - 2 * margin < margin)
+        return fail(CANS_ERR_ARG, "cans2d_periodic_fold: bad argument (ix=%d jx=%d margin=%d)", ix, jx, margin);
+    HIPCHK(launch_periodic_fold(lam6, ix, jx, margin, (hipStream_t)stream));
+    return 0;
+}
+
 // ====================================================================== events
 int cans_event_create(void **ev) {
     HIPCHK(hipEventCreate((hipEvent_t *)ev));

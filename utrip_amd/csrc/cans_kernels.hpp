@@ -125,6 +125,15 @@ struct Batch1dArgs {
 };
 hipError_t launch_batch1d(const Batch1dArgs &a, int mixed, int block_threads, int nblocks, hipStream_t s);
 
+// Reverse mode of one fused periodic 2D MLW step (6 fields ro..by).
+hipError_t launch_adj2d_step(const double *const *pn, const double *const *pn1, double *const *lin,
+                             double *const *lout, double *ws, const double *dx, const double *dxm, const double *dy,
+                             const double *dym, Metric mx, Metric my, double qav, double gm, double safety, int ix,
+                             int jx, int margin, hipStream_t s);
+hipError_t launch_periodic_fold(double *const *lam, int ix, int jx, int margin, hipStream_t s);
+size_t adj2d_ws_doubles(int ix, int jx);
+hipError_t launch_copy6(double *const *dst, double *const *src, size_t n, hipStream_t s);
+
 hipError_t launch_metric(const double *d, const double *dm, int n, Metric m, hipStream_t s);
 hipError_t launch_mlw2d(const Mlw2dArgs &a, hipStream_t s);
 hipError_t launch_mlw1d(const Mlw1dArgs &a, hipStream_t s);
