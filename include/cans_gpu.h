@@ -160,7 +160,7 @@ int cans2d_metric_fill(const double *dx, const double *dxm, int ix, const double
  * P^{n+1} = bnd(mlw_m(P^n, dt_n)), dt_n = cfl_m(P^n)  (cans2d/hdmlw/mlw_m.f:2-324,
  * cans2d/bc/bdper{x,y}.f, cans2d/common/cfl_m.f).  pn6 / pn1_6: ro,pr,vx,vy,bx,by
  * of P^n and P^{n+1} (device, all cells); lam_in6: dL/dP^{n+1} on all cells,
- * folded onto the interior in place (adjoint of the periodic fill);
+ * unchanged (the adjoint of the periodic fill is applied on the fly);
  * lam_out6 receives dL/dP^n on all cells (call cans2d_periodic_fold on the
  * last one).  The dependence of dt on P^n is included (split evenly over
  * cells whose cfl term ties at the minimum).  az carries no adjoint (no
@@ -174,18 +174,18 @@ int cans2d_mlw_adjoint_step(const double *const *pn6, const double *const *pn1_6
                             const double *metric_ws, void *ws, void *stream);
 /* nsteps of the fused periodic 2D MLW loop from states[0..6] (ro..az), writing
  * P^{k} to states[7k .. 7k+6] (a host array of 7*(nsteps+1) device pointers),
- * on a clock reset by the caller (steps 0..nsteps-1).  For the adjoint's
- * segment recompute. */
+ * on a clock reset by the caller (steps 0..nsteps-1); dt_log[k] (device)
+ * receives the dt of step k.  For the adjoint's segment recompute. */
 int cans2d_mlw_record(double *const *states, int nsteps, void *clk, double qav, double gm, double safety,
                       const double *dx, const double *dxm, int ix, const double *dy, const double *dym, int jx,
-                      int margin, double *metric_ws, void *stream);
-/* cans2d_mlw_adjoint_step for n = nsteps-1 .. 0 over states recorded by
- * cans2d_mlw_record: lam6 holds dL/dP^{nsteps} on entry and dL/dP^0 (all
+                      int margin, double *metric_ws, double *dt_log, void *stream);
+/* cans2d_mlw_adjoint_step for n = nsteps-1 .. 0 over states and dt_log
+ * recorded by cans2d_mlw_record: lam6 holds dL/dP^{nsteps} on entry and dL/dP^0 (all
  * cells, not folded) on return; tmp6 is scratch of the same shape. */
-int cans2d_mlw_adjoint_run(const double *const *states, int nsteps, double *const *lam6, double *const *tmp6,
-                           double qav, double gm, double safety, const double *dx, const double *dxm, int ix,
-                           const double *dy, const double *dym, int jx, int margin, const double *metric_ws, void *ws,
-                           void *stream);
+int cans2d_mlw_adjoint_run(const double *const *states, const double *dt_log, int nsteps, double *const *lam6,
+                           double *const *tmp6, double qav, double gm, double safety, const double *dx,
+                           const double *dxm, int ix, const double *dy, const double *dym, int jx, int margin,
+                           const double *metric_ws, void *ws, void *stream);
 /* adjoint of bdperx then bdpery on 6 fields: ghost lam added to its source, cleared */
 int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *stream);
 

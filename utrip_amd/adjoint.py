@@ -68,31 +68,66 @@ class PeriodicMlwAdjoint:
         """``lam``: dL/dP of the final state on every cell (6 tensors
         ro,pr,vx,vy,bx,by).  Returns dL/dP of the initial state, folded onto
         the interior (the gradient with respect to the interior values the
-        ghosts are copies of).  Each segment is recomputed from its
-        checkpoint into a segment buffer (cans2d_mlw_record) and swept in
-        reverse (cans2d_mlw_adjoint_run), one native call each."""
+        ghosts are copies of).
+
+        Segments are processed last to first.  Each is recomputed from its
+        checkpoint into one of two segment buffers (cans2d_mlw_record, on a
+        second stream) while the reverse sweep of the previous segment runs
+        on the caller's stream (cans2d_mlw_adjoint_run); events order a
+        buffer's record before its sweep and its sweep before its reuse."""
         dx, dxm, dy, dym = self.grid
         shape = (self.jx, self.ix)
         lam = [t.clone().contiguous() for t in lam]
         tmp = [torch.empty_like(t) for t in lam]
+        dev = self.ws.device
         if getattr(self, "_seg", None) is None:
-            self._seg = [[torch.empty_like(t) for t in self.ckpts[0]] for _ in range(self.ckpt + 1)]
-            self._clock = cans.Clock()
+            self._seg = [[[torch.empty_like(t) for t in self.ckpts[0]] for _ in range(self.ckpt + 1)]
+                         for _ in range(2)]
+            self._clocks = [cans.Clock(), cans.Clock()]
+            self._dts = [torch.empty(self.ckpt, dtype=torch.float64, device=dev) for _ in range(2)]
+            self._mws = [torch.empty_like(self.metric) for _ in range(2)]
+            self._rec_stream = torch.cuda.Stream(device=dev)
+        main = torch.cuda.current_stream(dev)
+        rec = self._rec_stream
+        ev_rec = [torch.cuda.Event(), torch.cuda.Event()]
+        ev_sweep = [torch.cuda.Event(), torch.cuda.Event()]
+        used = [False, False]
         lam_p, tmp_p = _ptr_array(lam, shape), _ptr_array(tmp, shape)
-        for c in reversed(range(len(self.ckpts))):
+        order = list(reversed(range(len(self.ckpts))))
+
+        def record(c, b):
             n = min(self.ckpt, self.nsteps - c * self.ckpt)
-            for dst, src in zip(self._seg[0], self.ckpts[c]):
-                dst.copy_(src)
-            states = _ptr_array([t for k in range(n + 1) for t in self._seg[k]], shape)
-            self._clock.reset(dtmin=self.dtmin)
-            _check(lib().cans2d_mlw_record(states, C.c_int(n), self._clock.handle, _d(self.qav), _d(self.gm),
-                                           _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy), _ptr(dym),
-                                           C.c_int(self.jx), C.c_int(self.margin), _ptr(self.metric), _stream()))
-            _check(lib().cans2d_mlw_adjoint_run(states, C.c_int(n), lam_p, tmp_p, _d(self.qav), _d(self.gm),
-                                                _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy),
-                                                _ptr(dym), C.c_int(self.jx), C.c_int(self.margin), _ptr(self.metric),
-                                                _ptr(self.ws), _stream()))
+            if used[b]:
+                rec.wait_event(ev_sweep[b])
+            else:
+                rec.wait_stream(main)       # checkpoints and buffers were written on the caller's stream
+            with torch.cuda.stream(rec):
+                for dst, src in zip(self._seg[b][0], self.ckpts[c]):
+                    dst.copy_(src)
+                states = _ptr_array([t for k in range(n + 1) for t in self._seg[b][k]], shape)
+                self._clocks[b].reset(dtmin=self.dtmin)
+                _check(lib().cans2d_mlw_record(states, C.c_int(n), self._clocks[b].handle, _d(self.qav), _d(self.gm),
+                                               _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy),
+                                               _ptr(dym), C.c_int(self.jx), C.c_int(self.margin),
+                                               _ptr(self._mws[b]), _ptr(self._dts[b]), _stream()))
+                ev_rec[b].record(rec)
+            return n, states
+
+        pending = record(order[0], 0)
+        for idx, c in enumerate(order):
+            b = idx % 2
+            n, states = pending
+            if idx + 1 < len(order):
+                pending = record(order[idx + 1], 1 - b)
+            main.wait_event(ev_rec[b])
+            _check(lib().cans2d_mlw_adjoint_run(states, _ptr(self._dts[b]), C.c_int(n), lam_p, tmp_p, _d(self.qav),
+                                                _d(self.gm), _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix),
+                                                _ptr(dy), _ptr(dym), C.c_int(self.jx), C.c_int(self.margin),
+                                                _ptr(self.metric), _ptr(self.ws), _stream()))
+            ev_sweep[b].record(main)
+            used[b] = True
         _check(lib().cans2d_periodic_fold(lam_p, C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin), _stream()))
+        rec.wait_stream(main)   # nothing on the record stream outlives the call
         return lam
 
 

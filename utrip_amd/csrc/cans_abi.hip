@@ -390,20 +390,32 @@ int cans2d_mlw_adjoint_step(const double *const *pn6, const double *const *pn1_6
     for (int f = 0; f < 6; f++)
         if (!pn6[f] || !pn1_6[f] || !lam_in6[f] || !lam_out6[f])
             return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_step: NULL field %d", f);
+    hipStream_t s = (hipStream_t)stream;
     const double *m = metric_ws;
     const size_t mxn = al(ix), myn = al(jx);
     Metric mx{(double *)m, (double *)m + mxn, (double *)m + 2 * mxn, (double *)m + 3 * mxn};
     m += 4 * mxn;
     Metric my{(double *)m, (double *)m + myn, (double *)m + 2 * myn, (double *)m + 3 * myn};
-    HIPCHK(launch_adj2d_step(pn6, pn1_6, lam_in6, lam_out6, (double *)ws, dx, dxm, dy, dym, mx, my, qav, gm, safety,
-                             ix, jx, margin, (hipStream_t)stream));
+    // the step's dt: cfl_m over every cell of P^n, as the forward computed it
+    double *tail = (double *)ws + 12 * al((size_t)ix * jx);
+    unsigned long long *key = reinterpret_cast<unsigned long long *>(tail);
+    HIPCHK(launch_arm_key(key, s));
+    CflArgs c{};
+    c.ro = pn6[0]; c.pr = pn6[1]; c.vx = pn6[2]; c.vy = pn6[3]; c.bx = pn6[4]; c.by = pn6[5];
+    c.dx = dx; c.dy = dy; c.safety = safety; c.gm = gm; c.ix = ix; c.jx = jx; c.kx = 1; c.dim = 2; c.key = key;
+    HIPCHK(launch_cfl(c, s));
+    HIPCHK(launch_key_to_dt(key, tail + 1, s));
+    HIPCHK(launch_zero(tail + 2, 2, s));
+    HIPCHK(launch_adj2d_step(pn6, pn1_6, lam_in6, lam_out6, (double *)ws, tail + 1, tail + 2, nullptr, nullptr,
+                             nullptr, dx, dxm, dy, dym, mx, my, qav, gm, safety, ix, jx, margin, s));
+    HIPCHK(launch_adj2d_dt(pn6, lam_out6, (double *)ws, tail + 1, tail + 2, dx, dy, gm, safety, ix, jx, s));
     return 0;
 }
 
 int cans2d_mlw_record(double *const *states, int nsteps, void *clk, double qav, double gm, double safety,
                       const double *dx, const double *dxm, int ix, const double *dy, const double *dym, int jx,
-                      int margin, double *metric_ws, void *stream) {
-    if (!states || !clk || !metric_ws || nsteps < 0)
+                      int margin, double *metric_ws, double *dt_log, void *stream) {
+    if (!states || !clk || !metric_ws || !dt_log || nsteps < 0)
         return fail(CANS_ERR_ARG, "cans2d_mlw_record: bad argument");
     if (ix < 3 || jx < 3 || margin < 1 || ix - 2 * margin < margin || jx - 2 * margin < margin)
         return fail(CANS_ERR_ARG, "cans2d_mlw_record: periodic box needs ix-2*margin >= margin (ix=%d jx=%d margin=%d)",
@@ -437,16 +449,17 @@ int cans2d_mlw_record(double *const *states, int nsteps, void *clk, double qav, 
         a.qav = qav; a.gm = gm; a.dt = 0.0; a.clk = c; a.step = n; a.ix = ix; a.jx = jx;
         a.per_margin = margin;
         a.safety = safety;
+        a.dt_log = dt_log;
         HIPCHK(launch_mlw2d(a, s));
     }
     return 0;
 }
 
-int cans2d_mlw_adjoint_run(const double *const *states, int nsteps, double *const *lam6, double *const *tmp6,
-                           double qav, double gm, double safety, const double *dx, const double *dxm, int ix,
-                           const double *dy, const double *dym, int jx, int margin, const double *metric_ws, void *ws,
-                           void *stream) {
-    if (!states || !lam6 || !tmp6 || !metric_ws || !ws || nsteps < 0)
+int cans2d_mlw_adjoint_run(const double *const *states, const double *dt_log, int nsteps, double *const *lam6,
+                           double *const *tmp6, double qav, double gm, double safety, const double *dx,
+                           const double *dxm, int ix, const double *dy, const double *dym, int jx, int margin,
+                           const double *metric_ws, void *ws, void *stream) {
+    if (!states || !dt_log || !lam6 || !tmp6 || !metric_ws || !ws || nsteps < 0)
         return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_run: bad argument");
     if (ix < 3 || jx < 3 || margin < 1 || ix - 2 * margin < margin || jx - 2 * margin < margin)
         return fail(CANS_ERR_ARG, "cans2d_mlw_adjoint_run: periodic box needs ix-2*margin >= margin (ix=%d jx=%d margin=%d)",
@@ -463,13 +476,22 @@ int cans2d_mlw_adjoint_run(const double *const *states, int nsteps, double *cons
     Metric my{(double *)m, (double *)m + myn, (double *)m + 2 * myn, (double *)m + 3 * myn};
     double *const *cur = lam6;
     double *const *oth = tmp6;
+    double *acc = (double *)ws + 12 * al((size_t)ix * jx) + 4;   // two slots of 2 accumulators
+    HIPCHK(launch_zero(acc, 4, s));
+    // step n counts into acc slot n&1; its B1 applies the dt gradient of step
+    // n+1 (slot (n+1)&1), which its B2 then clears for step n-1
     for (int n = nsteps - 1; n >= 0; n--) {
-        HIPCHK(launch_adj2d_step(states + 7 * n, states + 7 * (n + 1), cur, oth, (double *)ws, dx, dxm, dy, dym, mx,
-                                 my, qav, gm, safety, ix, jx, margin, s));
+        const bool pend = n < nsteps - 1;
+        HIPCHK(launch_adj2d_step(states + 7 * n, states + 7 * (n + 1), cur, oth, (double *)ws, dt_log + n,
+                                 acc + 2 * (n & 1), pend ? dt_log + n + 1 : nullptr,
+                                 pend ? acc + 2 * ((n + 1) & 1) : nullptr, acc + 2 * ((n + 1) & 1), dx, dxm, dy,
+                                 dym, mx, my, qav, gm, safety, ix, jx, margin, s));
         double *const *t = cur;
         cur = oth;
         oth = t;
     }
+    if (nsteps > 0)   // the dt gradient of the segment's first step
+        HIPCHK(launch_adj2d_dt(states, cur, (double *)ws, dt_log, acc, dx, dy, gm, safety, ix, jx, s));
     if (cur != lam6) HIPCHK(launch_copy6(lam6, cur, (size_t)ix * jx, s));
     return 0;
 }

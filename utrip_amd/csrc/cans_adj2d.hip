@@ -6,9 +6,10 @@
 // One step forward is  P' = bnd(prim(U(P) + du(P, dt))),  dt = cfl(P).  Its
 // transpose, given lam' = dL/dP' on every cell, is built as gathers (each
 // output reads what it needs; no atomics on fields):
-//   fold   adjoint of bdperx then bdpery: ghost lam is added to its source
-//          cell (y first, then x) and cleared;
-//   B1     per interior cell: lam_du = (dP'/dU')^T lam'   (prim recovery);
+//   B1     per interior cell: the adjoint of the periodic fill (the cell's
+//          lam' plus that of its ghost images, the images bdperx then
+//          bdpery made of it), then lam_du = (dP'/dU')^T lam' (primitive
+//          recovery); every cell also counts whether its cfl term ties dt;
 //   B2     per corner: recompute the half step (mlwhalf.f:45-52) and corner
 //          fluxes, gather lam_du of its 4 cells through the full-step
 //          weights (mlwfull.f:31-41), back through the corner flux and
@@ -19,7 +20,8 @@
 //          4 corners; back through U(P), the cell fluxes and E = -v x B;
 //   dt     dL/ddt from every explicit dt and the half step, reduced on the
 //          device, added to the cells whose cfl term equals dt (split evenly
-//          over exact ties: the mean of the one-sided derivatives).
+//          over exact ties: the mean of the one-sided derivatives).  The dt
+//          of each step is the one the forward kernel logged.
 // az is write-only in the step (nothing reads it but its own update), so it
 // carries no adjoint.  Everything is FP64.
 #include "cans_common.hpp"
@@ -42,9 +44,12 @@ struct AdjArgs {
     Metric mx, my;
     double qav, gm, safety;
     int ix, jx, margin;
-    const unsigned long long *key;   // dt of the step (order-preserving key)
-    double *ldt;                     // dL/ddt accumulator
-    unsigned *nties;
+    const double *dtp;               // the dt of the step
+    double *ldt;                     // dL/ddt accumulator (zeroed)
+    unsigned *nties;                 // cells whose cfl term equals dt (zeroed)
+    const double *pend_dt;           // the dt of step n+1, whose dt gradient B1 applies (or NULL)
+    const double *pend_acc;          // its accumulators (ldt, nties)
+    double *acc_zero;                // accumulators B2 clears for step n-1 (or NULL)
 };
 
 struct P6 {
@@ -145,7 +150,7 @@ CANS_DEV void flux_adj(const P6 &p, double gm, double lrx, double leex, double l
     l.by += -leex * ez * pi4i + lrxx * 2.0 * pi8i * by - lxy * pi4i * bx - lryy * 2.0 * pi8i * by - lez_t * vx;
 }
 
-CANS_DEV double step_dt(const AdjArgs &a) { return dunkey(*a.key); }
+CANS_DEV double step_dt(const AdjArgs &a) { return *a.dtp; }
 
 CANS_DEV bool interior(const AdjArgs &a, int i, int j) { return i >= 2 && i <= a.ix - 1 && j >= 2 && j <= a.jx - 1; }
 
@@ -194,25 +199,72 @@ __global__ void fold_x(AdjArgs a) {
     }
 }
 
-__global__ void adj_zero(double *ldt, unsigned *nties) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *ldt = 0.0;
-        *nties = 0u;
-    }
+
+// ------------------------------------------------------------------ dt = cfl(P)
+CANS_DEV double cfl_term(const AdjArgs &a, int i, int j, const P6 &p, double *S) {
+    const double b2 = p.bx * p.bx + p.by * p.by;
+    const double ca2 = b2 * pi4i / p.ro;
+    const double cs2 = a.gm * p.pr / p.ro;
+    const double v2 = p.vx * p.vx + p.vy * p.vy;
+    *S = v2 + cs2 + ca2;
+    const double dtcfl = fminF(a.dx[i - 1], a.dy[j - 1]) / __builtin_sqrt(*S);
+    return a.safety * dtcfl;
+}
+
+// dL/dP at a cell whose cfl term ties dt: (dL/ddt / nties) d(dtq)/dP, with
+// d(dtq)/dS = -dtq/(2S), S = v2 + gm pr/ro + b2 pi4i/ro
+CANS_DEV void dt_grad(const P6 &p, double q, double S, double ldt, unsigned nties, double gm, P6 &l) {
+    const double w = (ldt / (double)nties) * (-0.5 * q / S);
+    const double b2 = p.bx * p.bx + p.by * p.by;
+    l.ro += w * (-(gm * p.pr + b2 * pi4i) / (p.ro * p.ro));
+    l.pr += w * (gm / p.ro);
+    l.vx += w * (2.0 * p.vx);
+    l.vy += w * (2.0 * p.vy);
+    l.bx += w * (2.0 * p.bx * pi4i / p.ro);
+    l.by += w * (2.0 * p.by * pi4i / p.ro);
 }
 
 // ------------------------------------------------------------------ B1
 __global__ __launch_bounds__(256) void adj_b1(AdjArgs a) {
-    const size_t n = (size_t)a.ix * a.jx;
+    const int ix = a.ix, jx = a.jx;
+    const size_t n = (size_t)ix * jx;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const int i = (int)(k % a.ix) + 1, j = (int)(k / a.ix) + 1;
+    const int i = (int)(k % ix) + 1, j = (int)(k / ix) + 1;
+    double S;
+    if (cfl_term(a, i, j, load6(a.p, k), &S) == step_dt(a)) atomicAdd(a.nties, 1u);
     if (!interior(a, i, j)) {
         for (int f = 0; f < 6; f++) a.ldu[f][k] = 0.0;
         return;
     }
+    // adjoint of the periodic fill: after bdperx then bdpery ghost (gi,gj)
+    // holds the interior (src_x(gi), src_y(gj)), so this cell collects the
+    // lam' of every (image column, image row) pair it was copied to
+    const int m = a.margin;
+    int xi[3] = {i, 0, 0}, yj[3] = {j, 0, 0};
+    int nxi = 1, nyj = 1;
+    if (i >= ix - 2 * m + 1) xi[nxi++] = i - (ix - 2 * m);
+    if (i <= 2 * m) xi[nxi++] = i + (ix - 2 * m);
+    if (j >= jx - 2 * m + 1) yj[nyj++] = j - (jx - 2 * m);
+    if (j <= 2 * m) yj[nyj++] = j + (jx - 2 * m);
     const P6 pn = load6(a.q, k);
-    const P6 l = load6(a.lin, k);
+    // the pending dt gradient of step n+1 (dt_{n+1} = cfl_m(P^{n+1}) over every
+    // cell, ghost images included): added at the images that tie
+    const bool pend = a.pend_dt != nullptr;
+    const double pdt = pend ? *a.pend_dt : 0.0;
+    const double pldt = pend ? a.pend_acc[0] : 0.0;
+    const unsigned pnt = pend ? *reinterpret_cast<const unsigned *>(a.pend_acc + 1) : 0u;
+    P6 l{0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nyj; b++)
+        for (int c = 0; c < nxi; c++) {
+            const P6 g = load6(a.lin, at(xi[c], yj[b], ix));
+            l.ro += g.ro; l.pr += g.pr; l.vx += g.vx; l.vy += g.vy; l.bx += g.bx; l.by += g.by;
+            if (pnt) {
+                double Sg;
+                const double qg = cfl_term(a, xi[c], yj[b], pn, &Sg);
+                if (qg == pdt) dt_grad(pn, qg, Sg, pldt, pnt, a.gm, l);
+            }
+        }
     double lu[6];
     prim_adj(pn, l, a.gm, lu);
     for (int f = 0; f < 6; f++) a.ldu[f][k] = lu[f];
@@ -227,6 +279,10 @@ __global__ __launch_bounds__(256) void adj_b2(AdjArgs a) {
     if (t < n) {
         const int ci = (int)(t % (ix - 1)) + 1, cj = (int)(t / (ix - 1)) + 1;
         const double gm = a.gm, dt = step_dt(a), hdt = 0.5 * dt;
+        if (t == 0 && a.acc_zero) {   // B1 of this step has read them; B1 of step n-1 counts into them
+            a.acc_zero[0] = 0.0;
+            a.acc_zero[1] = 0.0;
+        }
         const size_t kA = at(ci, cj, ix), kB = at(ci + 1, cj, ix), kC = at(ci, cj + 1, ix), kD = at(ci + 1, cj + 1, ix);
         const CellV A = cellv(load6(a.p, kA), gm), B = cellv(load6(a.p, kB), gm);
         const CellV Cc = cellv(load6(a.p, kC), gm), D = cellv(load6(a.p, kD), gm);
@@ -285,6 +341,13 @@ CANS_DEV double av_q(double qav, double dm, double vhi, double vlo, double &dq) 
     return qav * dm * fmaxF(0.0, s);
 }
 
+// One neighbour's share of this cell's gather.  nb: its primitives; lnb:
+// its lam_du (zero unless it is interior); dir 0 E, 1 W, 2 N, 3 S.
+struct Gather {
+    double lu[6], lf[6], lg[6];
+    double lvx, lvy, part;
+};
+
 __global__ __launch_bounds__(256) void adj_b3(AdjArgs a) {
     const int ix = a.ix, jx = a.jx;
     const size_t n = (size_t)ix * jx;
@@ -293,49 +356,71 @@ __global__ __launch_bounds__(256) void adj_b3(AdjArgs a) {
     if (k < n) {
         const int i = (int)(k % ix) + 1, j = (int)(k / ix) + 1;
         const double gm = a.gm, dt = step_dt(a), hdt = 0.5 * dt;
-        const bool hasE = i < ix, hasW = i > 1, hasN = j < jx, hasS = j > 1;
         const bool ic = interior(a, i, j);
-        const bool iE = hasE && interior(a, i + 1, j), iW = hasW && interior(a, i - 1, j);
-        const bool iN = hasN && interior(a, i, j + 1), iS = hasS && interior(a, i, j - 1);
         const P6 pc = load6(a.p, k);
         const CellV C = cellv(pc, gm);
-        const size_t kE = hasE ? k + 1 : k, kW = hasW ? k - 1 : k, kN = hasN ? k + ix : k, kS = hasS ? k - ix : k;
-        const P6 pE = load6(a.p, kE), pW = load6(a.p, kW), pN = load6(a.p, kN), pS = load6(a.p, kS);
-        const CellV E = cellv(pE, gm), W = cellv(pW, gm), N = cellv(pN, gm), S = cellv(pS, gm);
-        double lc[6], le[6], lw[6], ln[6], ls[6];
-        for (int q = 0; q < 6; q++) {
-            lc[q] = ic ? a.ldu[q][k] : 0.0;
-            le[q] = iE ? a.ldu[q][kE] : 0.0;
-            lw[q] = iW ? a.ldu[q][kW] : 0.0;
-            ln[q] = iN ? a.ldu[q][kN] : 0.0;
-            ls[q] = iS ? a.ldu[q][kS] : 0.0;
-        }
-        // metric of this cell and its neighbours (u0/u1 are 0 on ghosts)
+        double lc[6];
+        for (int q = 0; q < 6; q++) lc[q] = ic ? a.ldu[q][k] : 0.0;
         const double dxi_c = a.mx.di[i - 1], dyi_c = a.my.di[j - 1];
         const double dxim_c = a.mx.dim[i - 1], dyim_c = a.my.dim[j - 1];
-        const double dxim_w = hasW ? a.mx.dim[i - 2] : 0.0, dyim_s = hasS ? a.my.dim[j - 2] : 0.0;
-        const double dxi_e = hasE ? a.mx.di[i] : 0.0, dxi_w = hasW ? a.mx.di[i - 2] : 0.0;
-        const double dyi_n = hasN ? a.my.di[j] : 0.0, dyi_s = hasS ? a.my.di[j - 2] : 0.0;
-        // AV coefficients of the four faces of this cell (mlw_m.f:276-288)
-        double dqE = 0, dqW = 0, dqN = 0, dqS = 0;
-        const double qE = hasE ? av_q(a.qav, a.dxm[i - 1], pE.vx, pc.vx, dqE) : 0.0;   // face (c,E)
-        const double qW = hasW ? av_q(a.qav, a.dxm[i - 2], pc.vx, pW.vx, dqW) : 0.0;   // face (W,c)
-        const double qN = hasN ? av_q(a.qav, a.dym[j - 1], pN.vy, pc.vy, dqN) : 0.0;   // face (c,N)
-        const double qS = hasS ? av_q(a.qav, a.dym[j - 2], pc.vy, pS.vy, dqS) : 0.0;   // face (S,c)
-
-        double lu[6], lf[6], lg[6];
+        Gather G;
         for (int q = 0; q < 6; q++) {
-            // own update: identity, and the AV centre coefficient
-            double u = lc[q] - lc[q] * dt * (dxi_c * (qE * dxim_c + qW * dxim_w) + dyi_c * (qN * dyim_c + qS * dyim_s));
-            // neighbours' updates: their centre term reads this cell's fluxes, their AV this cell's u
-            u += le[q] * dt * dxi_e * qE * dxim_c + lw[q] * dt * dxi_w * qW * dxim_w;
-            u += ln[q] * dt * dyi_n * qN * dyim_c + ls[q] * dt * dyi_s * qS * dyim_s;
-            double f = le[q] * hdt * (0.5 * dxi_e) - lw[q] * hdt * (0.5 * dxi_w);
-            double g = ln[q] * hdt * (0.5 * dyi_n) - ls[q] * hdt * (0.5 * dyi_s);
-            lu[q] = u;
-            lf[q] = f;
-            lg[q] = g;
+            G.lu[q] = lc[q];          // own update: U' = U + du
+            G.lf[q] = 0.0;
+            G.lg[q] = 0.0;
         }
+        G.lvx = 0.0;
+        G.lvy = 0.0;
+        double avc = 0.0;             // own AV centre coefficient (sum over the 4 faces)
+        // the four neighbours, one at a time
+        for (int d = 0; d < 4; d++) {
+            const bool x = d < 2;
+            const bool hi = (d == 0 || d == 2);           // E or N
+            const bool has = d == 0 ? i < ix : d == 1 ? i > 1 : d == 2 ? j < jx : j > 1;
+            if (!has) continue;
+            const int in_ = d == 0 ? i + 1 : d == 1 ? i - 1 : i, jn = d == 2 ? j + 1 : d == 3 ? j - 1 : j;
+            const size_t kn = at(in_, jn, ix);
+            const bool inb = interior(a, in_, jn);
+            const P6 pn = load6(a.p, kn);
+            const CellV Nb = cellv(pn, gm);
+            double ln[6];
+            for (int q = 0; q < 6; q++) ln[q] = inb ? a.ldu[q][kn] : 0.0;
+            // the face between this cell and the neighbour: its AV coefficient
+            // (the face's low-side cell carries dxm; mlw_m.f:276-288)
+            const int lo_i = (x && !hi) ? in_ : i, lo_j = (!x && !hi) ? jn : j;
+            const double dm = x ? a.dxm[lo_i - 1] : a.dym[lo_j - 1];
+            const double dmi = x ? a.mx.dim[lo_i - 1] : a.my.dim[lo_j - 1];
+            const double vhi = x ? (hi ? pn.vx : pc.vx) : (hi ? pn.vy : pc.vy);
+            const double vlo = x ? (hi ? pc.vx : pn.vx) : (hi ? pc.vy : pn.vy);
+            double dq;
+            const double qf = av_q(a.qav, dm, vhi, vlo, dq);
+            const double di_n = x ? a.mx.di[in_ - 1] : a.my.di[jn - 1];
+            const double di_c = x ? dxi_c : dyi_c;
+            avc += di_c * qf * dmi;
+            double lq = 0.0;
+            for (int q = 0; q < 6; q++) {
+                // neighbour's AV reads this cell's u; its centre term this cell's flux
+                G.lu[q] += ln[q] * dt * di_n * qf * dmi;
+                const double w = ln[q] * hdt * (0.5 * di_n);
+                if (x) G.lf[q] += hi ? w : -w;
+                else G.lg[q] += hi ? w : -w;
+                // the face coefficient's weight: +dt*di_lo*q*dmi*(u_hi-u_lo) in du(lo), minus in du(hi)
+                const double uhi = hi ? Nb.u[q] : C.u[q], ulo = hi ? C.u[q] : Nb.u[q];
+                const double llo = hi ? lc[q] : ln[q], lhi = hi ? ln[q] : lc[q];
+                const double dlo = hi ? di_c : di_n, dhi = hi ? di_n : di_c;
+                lq += (uhi - ulo) * dt * dmi * (dlo * llo - dhi * lhi);
+                // explicit dt of this cell's update: centre term and AV
+                if (ic) {
+                    const double fl = x ? Nb.fx[q] : Nb.gy[q];
+                    part += lc[q] * ((hi ? -0.25 : 0.25) * di_c * fl
+                                     + (hi ? 1.0 : -1.0) * di_c * qf * dmi * (hi ? (Nb.u[q] - C.u[q]) : (C.u[q] - Nb.u[q])));
+                }
+            }
+            // dq is d q / d v_hi; v_hi is this cell's velocity when the neighbour is the low side
+            if (x) G.lvx += hi ? -lq * dq : lq * dq;
+            else G.lvy += hi ? -lq * dq : lq * dq;
+        }
+        for (int q = 0; q < 6; q++) G.lu[q] -= lc[q] * dt * avc;
         // corners around this cell: (i,j) A, (i-1,j) B, (i,j-1) C, (i-1,j-1) D
         for (int r = 0; r < 4; r++) {
             const int ci = (r & 1) ? i - 1 : i, cj = (r & 2) ? j - 1 : j;
@@ -345,70 +430,19 @@ __global__ __launch_bounds__(256) void adj_b3(AdjArgs a) {
             const double wx = sx * dt * a.mx.dim[ci - 1] / 2.0, wy = sy * dt * a.my.dim[cj - 1] / 2.0;
             for (int q = 0; q < 6; q++) {
                 const double l = a.luh[q][kc];
-                lu[q] += 0.25 * l;
-                lf[q] += wx * l;
-                lg[q] += wy * l;
+                G.lu[q] += 0.25 * l;
+                G.lf[q] += wx * l;
+                G.lg[q] += wy * l;
             }
         }
-        // AV coefficient derivatives (velocities enter through max(0,|dv|-1e-4))
-        double lvx = 0.0, lvy = 0.0;
-        if (hasE) {   // face (c,E): in du(c) with +dt*dxi_c*q*dxim_c*(uE-uc), in du(E) with -dt*dxi_e*q*dxim_c*(uE-uc)
-            double lq = 0.0;
-            for (int q = 0; q < 6; q++) lq += (E.u[q] - C.u[q]) * dt * dxim_c * (dxi_c * lc[q] - dxi_e * le[q]);
-            lvx -= lq * dqE;
-        }
-        if (hasW) {
-            double lq = 0.0;
-            for (int q = 0; q < 6; q++) lq += (C.u[q] - W.u[q]) * dt * dxim_w * (dxi_w * lw[q] - dxi_c * lc[q]);
-            lvx += lq * dqW;
-        }
-        if (hasN) {
-            double lq = 0.0;
-            for (int q = 0; q < 6; q++) lq += (N.u[q] - C.u[q]) * dt * dyim_c * (dyi_c * lc[q] - dyi_n * ln[q]);
-            lvy -= lq * dqN;
-        }
-        if (hasS) {
-            double lq = 0.0;
-            for (int q = 0; q < 6; q++) lq += (C.u[q] - S.u[q]) * dt * dyim_s * (dyi_s * ls[q] - dyi_c * lc[q]);
-            lvy += lq * dqS;
-        }
-        P6 l{0.0, 0.0, lvx, lvy, 0.0, 0.0};
-        cons_adj(pc, lu, gm, l);
+        P6 l{0.0, 0.0, G.lvx, G.lvy, 0.0, 0.0};
+        cons_adj(pc, G.lu, gm, l);
         // x fluxes rx eex rxx ryx (bx: none, by: -ez); y fluxes ry eey rxy ryy (bx: ez, by: none)
-        flux_adj(pc, gm, lf[0], lf[1], lf[2], lf[3], lg[0], lg[1], lg[2], lg[3], -lf[5] + lg[4], l);
+        flux_adj(pc, gm, G.lf[0], G.lf[1], G.lf[2], G.lf[3], G.lg[0], G.lg[1], G.lg[2], G.lg[3], -G.lf[5] + G.lg[4], l);
         a.lout[0][k] = l.ro; a.lout[1][k] = l.pr; a.lout[2][k] = l.vx;
         a.lout[3][k] = l.vy; a.lout[4][k] = l.bx; a.lout[5][k] = l.by;
-        // explicit dt dependence of this cell's update (centre term and AV)
-        if (ic) {
-            for (int q = 0; q < 6; q++) {
-                const double d = -0.25 * dxi_c * (E.fx[q] - W.fx[q]) - 0.25 * dyi_c * (N.gy[q] - S.gy[q])
-                               + dxi_c * (qE * dxim_c * (E.u[q] - C.u[q]) - qW * dxim_w * (C.u[q] - W.u[q]))
-                               + dyi_c * (qN * dyim_c * (N.u[q] - C.u[q]) - qS * dyim_s * (C.u[q] - S.u[q]));
-                part += lc[q] * d;
-            }
-        }
     }
     block_sum_add<256>(part, a.ldt);
-}
-
-// ------------------------------------------------------------------ dt = cfl(P)
-CANS_DEV double cfl_term(const AdjArgs &a, int i, int j, const P6 &p, double *S) {
-    const double b2 = p.bx * p.bx + p.by * p.by;
-    const double ca2 = b2 * pi4i / p.ro;
-    const double cs2 = a.gm * p.pr / p.ro;
-    const double v2 = p.vx * p.vx + p.vy * p.vy;
-    *S = v2 + cs2 + ca2;
-    const double dtcfl = fminF(a.dx[i - 1], a.dy[j - 1]) / __builtin_sqrt(*S);
-    return a.safety * dtcfl;
-}
-
-__global__ __launch_bounds__(256) void adj_ties(AdjArgs a) {
-    const size_t n = (size_t)a.ix * a.jx;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int i = (int)(k % a.ix) + 1, j = (int)(k / a.ix) + 1;
-    double S;
-    if (cfl_term(a, i, j, load6(a.p, k), &S) == step_dt(a)) atomicAdd(a.nties, 1u);
 }
 
 __global__ __launch_bounds__(256) void adj_dt(AdjArgs a) {
@@ -420,60 +454,83 @@ __global__ __launch_bounds__(256) void adj_dt(AdjArgs a) {
     double S;
     const double q = cfl_term(a, i, j, p, &S);
     if (q != step_dt(a) || *a.nties == 0u) return;
-    // d(dtq)/dS = -dtq/(2S); S = v2 + gm pr/ro + b2 pi4i/ro
-    const double w = (*a.ldt / (double)(*a.nties)) * (-0.5 * q / S);
-    const double b2 = p.bx * p.bx + p.by * p.by;
-    a.lout[0][k] += w * (-(a.gm * p.pr + b2 * pi4i) / (p.ro * p.ro));
-    a.lout[1][k] += w * (a.gm / p.ro);
-    a.lout[2][k] += w * (2.0 * p.vx);
-    a.lout[3][k] += w * (2.0 * p.vy);
-    a.lout[4][k] += w * (2.0 * p.bx * pi4i / p.ro);
-    a.lout[5][k] += w * (2.0 * p.by * pi4i / p.ro);
+    P6 l{0, 0, 0, 0, 0, 0};
+    dt_grad(p, q, S, *a.ldt, *a.nties, a.gm, l);
+    a.lout[0][k] += l.ro; a.lout[1][k] += l.pr; a.lout[2][k] += l.vx;
+    a.lout[3][k] += l.vy; a.lout[4][k] += l.bx; a.lout[5][k] += l.by;
 }
 
-}  // namespace
-
-hipError_t launch_adj2d_step(const double *const *pn, const double *const *pn1, double *const *lin,
-                             double *const *lout, double *ws, const double *dx, const double *dxm, const double *dy,
-                             const double *dym, Metric mx, Metric my, double qav, double gm, double safety, int ix,
-                             int jx, int margin, hipStream_t s) {
+AdjArgs make_adj(const double *const *pn, const double *const *pn1, const double *const *lin, double *const *lout,
+                 double *ws, const double *dtp, double *acc, const double *dx, const double *dxm, const double *dy,
+                 const double *dym, Metric mx, Metric my, double qav, double gm, double safety, int ix, int jx,
+                 int margin) {
     const size_t n = (size_t)ix * jx;
     const size_t na = (n + 63) & ~size_t(63);
     AdjArgs a{};
     for (int f = 0; f < 6; f++) {
         a.p[f] = pn[f];
-        a.q[f] = pn1[f];
-        a.lin[f] = lin[f];
+        a.q[f] = pn1 ? pn1[f] : nullptr;
+        a.lin[f] = lin ? const_cast<double *>(lin[f]) : nullptr;
         a.lout[f] = lout[f];
         a.ldu[f] = ws + f * na;
         a.luh[f] = ws + (6 + f) * na;
     }
-    unsigned long long *key = reinterpret_cast<unsigned long long *>(ws + 12 * na);
-    a.ldt = ws + 12 * na + 8;
-    a.nties = reinterpret_cast<unsigned *>(ws + 12 * na + 16);
-    a.key = key;
+    a.dtp = dtp;
+    a.ldt = acc;
+    a.nties = reinterpret_cast<unsigned *>(acc + 1);
     a.dx = dx; a.dxm = dxm; a.dy = dy; a.dym = dym;
     a.mx = mx; a.my = my;
     a.qav = qav; a.gm = gm; a.safety = safety;
     a.ix = ix; a.jx = jx; a.margin = margin;
+    return a;
+}
+
+}  // namespace
+
+hipError_t launch_adj2d_step(const double *const *pn, const double *const *pn1, const double *const *lin,
+                             double *const *lout, double *ws, const double *dtp, double *acc, const double *pend_dt,
+                             const double *pend_acc, double *acc_zero, const double *dx, const double *dxm,
+                             const double *dy, const double *dym, Metric mx, Metric my, double qav, double gm,
+                             double safety, int ix, int jx, int margin, hipStream_t s) {
+    AdjArgs a = make_adj(pn, pn1, lin, lout, ws, dtp, acc, dx, dxm, dy, dym, mx, my, qav, gm, safety, ix, jx, margin);
+    a.pend_dt = pend_dt;
+    a.pend_acc = pend_acc;
+    a.acc_zero = acc_zero;
+    const size_t n = (size_t)ix * jx;
     const int nb = (int)((n + 255) / 256);
-    hipError_t e;
-    hipLaunchKernelGGL(fold_y, dim3(64), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(fold_x, dim3(64), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(adj_zero, dim3(1), dim3(64), 0, s, a.ldt, a.nties);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    // dt of the step: cfl_m over every cell of P^n (the forward's fused reduction)
-    if ((e = launch_arm_key(key, s)) != hipSuccess) return e;
-    CflArgs c{};
-    c.ro = pn[0]; c.pr = pn[1]; c.vx = pn[2]; c.vy = pn[3]; c.bx = pn[4]; c.by = pn[5];
-    c.dx = dx; c.dy = dy; c.safety = safety; c.gm = gm; c.ix = ix; c.jx = jx; c.kx = 1; c.dim = 2;
-    c.key = key;
-    if ((e = launch_cfl(c, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(adj_b1, dim3(nb), dim3(256), 0, s, a);
     hipLaunchKernelGGL(adj_b2, dim3((int)(((size_t)(ix - 1) * (jx - 1) + 255) / 256)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(adj_b3, dim3(nb), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(adj_ties, dim3(nb), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(adj_dt, dim3(nb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj2d_dt(const double *const *pn, double *const *lout, double *ws, const double *dtp, double *acc,
+                           const double *dx, const double *dy, double gm, double safety, int ix, int jx,
+                           hipStream_t s) {
+    Metric z{};
+    AdjArgs a = make_adj(pn, nullptr, nullptr, lout, ws, dtp, acc, dx, nullptr, dy, nullptr, z, z, 0.0, gm, safety,
+                         ix, jx, 1);
+    const size_t n = (size_t)ix * jx;
+    hipLaunchKernelGGL(adj_dt, dim3((int)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void key_to_dt(const unsigned long long *key, double *dt) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dt = dunkey(*key);
+}
+
+hipError_t launch_key_to_dt(const unsigned long long *key, double *dt, hipStream_t s) {
+    hipLaunchKernelGGL(key_to_dt, dim3(1), dim3(64), 0, s, key, dt);
+    return hipGetLastError();
+}
+
+__global__ void zero_kernel(double *p, size_t n) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        p[k] = 0.0;
+}
+
+hipError_t launch_zero(double *p, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(zero_kernel, dim3(64), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

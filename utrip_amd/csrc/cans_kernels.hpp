@@ -54,6 +54,7 @@ struct Mlw2dArgs {
     int per_margin;    // > 0: also write the periodic ghost images (width)
     double safety;     // > 0 with clk: fused cfl_m of the new state
     int peek;          // 1: a clock kernel of this step already ran (read, do not advance)
+    double *dt_log;    // non-null: dt_log[step] = the dt this step used
 };
 
 struct Mlw1dArgs {
@@ -126,10 +127,21 @@ struct Batch1dArgs {
 hipError_t launch_batch1d(const Batch1dArgs &a, int mixed, int block_threads, int nblocks, hipStream_t s);
 
 // Reverse mode of one fused periodic 2D MLW step (6 fields ro..by).
-hipError_t launch_adj2d_step(const double *const *pn, const double *const *pn1, double *const *lin,
-                             double *const *lout, double *ws, const double *dx, const double *dxm, const double *dy,
-                             const double *dym, Metric mx, Metric my, double qav, double gm, double safety, int ix,
-                             int jx, int margin, hipStream_t s);
+// B1-B3 of the adjoint of step n.  dtp: the dt step n used; acc: 2 doubles
+// of zeroed accumulators (dL/ddt, ties) for the step; pend_dt/pend_acc: dt and
+// accumulators of step n+1, whose dt gradient B1 adds (NULL: none);
+// acc_zero: accumulators cleared for step n-1 (or NULL).
+hipError_t launch_adj2d_step(const double *const *pn, const double *const *pn1, const double *const *lin,
+                             double *const *lout, double *ws, const double *dtp, double *acc, const double *pend_dt,
+                             const double *pend_acc, double *acc_zero, const double *dx, const double *dxm,
+                             const double *dy, const double *dym, Metric mx, Metric my, double qav, double gm,
+                             double safety, int ix, int jx, int margin, hipStream_t s);
+// the dt gradient of step n added to lout (dL/dP^n) at the cells of P^n that tie
+hipError_t launch_adj2d_dt(const double *const *pn, double *const *lout, double *ws, const double *dtp, double *acc,
+                           const double *dx, const double *dy, double gm, double safety, int ix, int jx,
+                           hipStream_t s);
+hipError_t launch_key_to_dt(const unsigned long long *key, double *dt, hipStream_t s);
+hipError_t launch_zero(double *p, size_t n, hipStream_t s);
 hipError_t launch_periodic_fold(double *const *lam, int ix, int jx, int margin, hipStream_t s);
 size_t adj2d_ws_doubles(int ix, int jx);
 hipError_t launch_copy6(double *const *dst, double *const *src, size_t n, hipStream_t s);

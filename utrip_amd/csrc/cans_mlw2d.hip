@@ -458,6 +458,7 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
         dt = c.dt;
     }
     if (!run) return;                          // uniform across the launch
+    if (a.dt_log && blockIdx.x == 0 && threadIdx.x == 0) a.dt_log[a.step] = dt;
     const bool do_cfl = a.clk && a.safety > 0.0;
     double dtmin_local = kR4_1e20;
 
