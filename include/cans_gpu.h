@@ -251,6 +251,52 @@ int cans1d_batch_graph(int ntube, const long long *off, const int *ix, const int
 int cans_graph_launch(void *graph, void *stream);
 int cans_graph_destroy(void *graph);
 
+/* ---------------------------------------------------------------- UTRIP spicule model
+ * cans1d/hdmlw/mlw_m_bg.f:2-3 mlw_m_bg(ro,pr,vx,vy,by,bx,bxm,dt,qav,gm,gx,gxm,sc,dsc,
+ * scm,dscm,rr,rrm,drr,drrm,dx,dxm,ix): MLW for 1D MHD in a flux tube of
+ * cross-section sc (face values scm), curvature radius rr, gravity gx (fuses
+ * mlwhalf.f, mlwfull.f, mlwartv.f, mlwsrch.f, mlwsrcf.f); device arrays,
+ * updated in place on i=2..ix-1. */
+int cans1d_mlw_m_bg(double *ro, double *pr, double *vx, double *vy, double *by, const double *bx,
+                    const double *bxm, double dt, double qav, double gm, const double *gx, const double *gxm,
+                    const double *sc, const double *dsc, const double *scm, const double *dscm, const double *rr,
+                    const double *rrm, const double *drr, const double *drrm, const double *dx, const double *dxm,
+                    int ix, void *stream);
+/* cans1d/md_spicule/hdmlw/trqalfven.f:2 trqalfven(vy,dt,time,x,ix,rr):
+ * vy(i) += dt*30*q(i)*trq(i), q(i) = 2 rr(i) (anum_i - 0.5), q(ix) = q(ix-2).
+ * trq is the time-independent profile tanh((x-0.75)/0.075)-1 with
+ * trq(ix) = trq(ix-2), computed by the caller (host libm tanh) once.  The
+ * noise anum_i is the i-th number of the Park-Miller stream of
+ * cans1d/common/rangen.f from *mseed (host, 1..2^31-2), in place of the
+ * compiler's RANDOM_NUMBER; *mseed is advanced by ix draws. */
+int cans1d_trqalfven(double *vy, double dt, const double *trq, const double *rr, int ix, int *mseed,
+                     void *stream);
+/* cans1d/common/chkdav.f:2 chkdav(n_events,da,vx,floor,ix): da < floor ->
+ * da = floor, vx = 0.  n_events is a host pointer (the call synchronises). */
+int cans1d_chkdav(int *n_events, double *da, double *vx, double floor_, int ix, void *stream);
+/* The time loop of cans1d/md_spicule/hdmlw/main.f:169-227 for nrun
+ * independent realizations, all on the device in one persistent kernel:
+ *     ns=ns+1; cfl_m (stop with merr=9001 if dt<dtmin); t=t+dt; mlw_m_bg;
+ *     trqalfven; bnd (bdsppx/bdspnx low, bdfrex high: md_spicule/hdmlw/bnd.f);
+ *     chkdav(ro); chkdav(pr); repeat while ns < nstop and t < tend
+ * Realization r: fields state5[0..4] (ro,pr,vx,vy,by) at r*ix, updated in
+ * place; geometry geo14 = bx,bxm,gx,gxm,sc,dsc,scm,dscm,rr,rrm,drr,drrm,dx,dxm
+ * and the torque profile trq at r*ix if per_run_geometry, else shared;
+ * t[r], ns[r] (loop entries so far), seed[r] (Park-Miller, 1..2^31-2) are
+ * read and written back; merr[r] is written.  Calling again continues the
+ * runs bit for bit.  snap (NULL: none) receives the state whenever
+ * int(t/dtout) changes, as main.f writes its DAC output: snap[(r*nsnap_max+k)*5*ix
+ * + q*ix + i], snap_t[r*nsnap_max+k], nsnap[r] of them (at most nsnap_max).
+ * block_threads in {64,128,256,512,1024}; blocks <= 0: one per realization.
+ * ws: cans1d_spicule_ws_bytes.  FP64, bit-identical to the reference loop
+ * with the rangen noise.  The reference runs one realization per program. */
+size_t cans1d_spicule_ws_bytes(int ix, int nrun, int per_run_geometry);
+int cans1d_spicule_run(int nrun, int ix, int margin, double *const *state5, const double *const *geo14,
+                       int per_run_geometry, const double *trq, double gm, double qav, double safety, double dtmin,
+                       double floor_, double tend, int nstop, double *t, int *ns, int *merr, int *seed,
+                       double dtout, double *snap, double *snap_t, int nsnap_max, int *nsnap, int block_threads,
+                       int blocks, void *ws, void *stream);
+
 /* HIP events for timing the engine kernels of a run: `events` of the run
  * entry points may be NULL or hold 2*nsteps events; the engine kernel of the
  * s-th step of the call is bracketed by events[2s] and events[2s+1], recorded

@@ -41,6 +41,30 @@ void or1d_bd(int kind, int mbnd, int margin, double *qq, double q0,
              const double *dxm, int ix);
 void or1d_bdperx(int margin0, int margin1, double *qq, int ix);
 void or1d_grdrdy(double *dx, double *xm, double *dxm, const double *x, int ix);
+void or1d_mlw_m_bg(double *ro, double *pr, double *vx, double *vy, double *by,
+                   const double *bx, const double *bxm, double dt, double qav,
+                   double gm, const double *gx, const double *gxm,
+                   const double *sc, const double *dsc, const double *scm,
+                   const double *dscm, const double *rr, const double *rrm,
+                   const double *drr, const double *drrm, const double *dx,
+                   const double *dxm, int ix);
+void or1d_scrdy(double *dsc, double *dscm, const double *sc, const double *scm,
+                const double *dx, const double *dxm, int ix);
+void or1d_chkdav(int *n_events, double *da, double *vx, double floor_, int ix);
+double or_rangen(int *mseed);
+
+/* ---------------- UTRIP spicule loop (cans1d/md_spicule/hdmlw) ---------------- */
+void orsp_trqalfven(double *vy, double dt, double time, const double *x, int ix,
+                    const double *rr, int *mseed);
+void orsp_bnd(int margin, double *ro, double *pr, double *vx, double *vy,
+              double *by, int ix);
+void orsp_bndsc(int margin, double *sc, double *dsc, double *scm, double *dscm,
+                int ix);
+void orsp_run(double *ro, double *pr, double *vx, double *vy, double *by,
+              const double *const *g, int ix, int margin, double gm,
+              double qav, double safety, double dtmin, double floor_,
+              double tend, int nstop, double *t, int *ns, int *merr,
+              int *mseed);
 
 /* ---------------- 2D (cans2d) ---------------- */
 void or2d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *bx,

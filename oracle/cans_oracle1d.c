@@ -230,3 +230,196 @@ void or1d_grdrdy(double *dx, double *xm, double *dxm, const double *x, int ix) {
     for (int i = 1; i <= ix - 1; i++) xm[I1(i)] = (x[I1(i)] + x[I1(i + 1)]) / 2.0;
     xm[I1(ix)] = xm[I1(ix - 1)] + dx[I1(ix - 1)];
 }
+
+/* hdmlw/mlwsrch.f:32-41: source at the cell (du) and at the half step (un) */
+static void mlwsrch1(double *un, double *du, double dt, const double *s, int ix) {
+    for (int i = 2; i <= ix - 1; i++) du[I1(i)] = du[I1(i)] + 0.5 * dt * s[I1(i)];
+    for (int i = 1; i <= ix - 1; i++) {
+        double sh = 0.5 * (s[I1(i + 1)] + s[I1(i)]);
+        un[I1(i)] = un[I1(i)] + dt * sh;
+    }
+}
+
+/* hdmlw/mlwsrcf.f:31-34: half-step source interpolated to the cell */
+static void mlwsrcf1(double *du, double dt, const double *s, const double *ux0,
+                     const double *ux1, int ix) {
+    for (int i = 2; i <= ix - 1; i++) {
+        double sh = ux0[I1(i)] * s[I1(i - 1)] + ux1[I1(i)] * s[I1(i)];
+        du[I1(i)] = du[I1(i)] + 0.5 * dt * sh;
+    }
+}
+
+/* hdmlw/mlw_m_bg.f:2-271: mlw_m with gravity, flux-tube cross-section sc and
+ * rotation radius rr (conserved variables carried as u*sc, ry*sc*rr,
+ * by*sc/rr); Bx in the energy. */
+void or1d_mlw_m_bg(double *ro, double *pr, double *vx, double *vy, double *by,
+                   const double *bx, const double *bxm, double dt, double qav,
+                   double gm, const double *gx, const double *gxm,
+                   const double *sc, const double *dsc, const double *scm,
+                   const double *dscm, const double *rr, const double *rrm,
+                   const double *drr, const double *drrm, const double *dx,
+                   const double *dxm, int ix) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0; /* :70 */
+    const double pi4i = 1.0 / pi / 4.0; /* :71 */
+    double *dxi = dalloc(ix), *dxim = dalloc(ix), *ux0 = dalloc(ix), *ux1 = dalloc(ix);
+    double *ee = dalloc(ix), *rx = dalloc(ix), *ry = dalloc(ix), *ez = dalloc(ix);
+    double *roh = dalloc(ix), *prh = dalloc(ix), *vxh = dalloc(ix), *vyh = dalloc(ix), *byh = dalloc(ix);
+    double *eeh = dalloc(ix), *ezh = dalloc(ix);
+    double *rosc = dalloc(ix), *eesc = dalloc(ix), *rxsc = dalloc(ix), *rysc = dalloc(ix), *bysc = dalloc(ix);
+    double *rosch = dalloc(ix), *eesch = dalloc(ix), *rxsch = dalloc(ix), *rysch = dalloc(ix), *bysch = dalloc(ix);
+    double *drosc = dalloc(ix), *deesc = dalloc(ix), *drxsc = dalloc(ix), *drysc = dalloc(ix), *dbysc = dalloc(ix);
+    double *fx = dalloc(ix), *qx = dalloc(ix), *ss = dalloc(ix);
+
+    for (int i = 1; i <= ix; i++) { /* :75-78 */
+        dxim[I1(i)] = 1.0 / dxm[I1(i)];
+        dxi[I1(i)] = 1.0 / dx[I1(i)];
+    }
+    for (int i = 2; i <= ix - 1; i++) { /* :79-82 */
+        ux1[I1(i)] = 0.5 * dxm[I1(i - 1)] / dx[I1(i)];
+        ux0[I1(i)] = 0.5 * dxm[I1(i)] / dx[I1(i)];
+    }
+    for (int i = 1; i <= ix; i++) { /* :87-93 */
+        drosc[I1(i)] = 0.0; deesc[I1(i)] = 0.0; drxsc[I1(i)] = 0.0; drysc[I1(i)] = 0.0; dbysc[I1(i)] = 0.0;
+    }
+    for (int i = 1; i <= ix; i++) { /* :97-104, Bx in the energy */
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]);
+        double bb = sq(by[I1(i)]) + sq(bx[I1(i)]);
+        ee[I1(i)] = pr[I1(i)] / (gm - 1.0) + 0.5 * ro[I1(i)] * vv + pi8i * bb;
+        rx[I1(i)] = ro[I1(i)] * vx[I1(i)];
+        ry[I1(i)] = ro[I1(i)] * vy[I1(i)];
+    }
+    for (int i = 1; i <= ix; i++) ez[I1(i)] = -vx[I1(i)] * by[I1(i)] + vy[I1(i)] * bx[I1(i)]; /* :105-107 */
+    for (int i = 1; i <= ix; i++) { /* :108-114 */
+        rosc[I1(i)] = ro[I1(i)] * sc[I1(i)];
+        eesc[I1(i)] = ee[I1(i)] * sc[I1(i)];
+        rxsc[I1(i)] = rx[I1(i)] * sc[I1(i)];
+        rysc[I1(i)] = ry[I1(i)] * sc[I1(i)] * rr[I1(i)];
+        bysc[I1(i)] = by[I1(i)] * sc[I1(i)] / rr[I1(i)];
+    }
+
+    /* half step, :120-167 */
+    for (int i = 1; i <= ix; i++) {
+        fx[I1(i)] = ro[I1(i)] * vx[I1(i)];
+        fx[I1(i)] = fx[I1(i)] * sc[I1(i)];
+    }
+    mlwhalf1(rosc, rosch, drosc, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) {
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]);
+        double ep = pr[I1(i)] * gm / (gm - 1.0) + 0.5 * ro[I1(i)] * vv;
+        fx[I1(i)] = ep * vx[I1(i)] - by[I1(i)] * ez[I1(i)] * pi4i;
+        fx[I1(i)] = fx[I1(i)] * sc[I1(i)];
+        ss[I1(i)] = ro[I1(i)] * vx[I1(i)] * gx[I1(i)];
+        ss[I1(i)] = ss[I1(i)] * sc[I1(i)];
+    }
+    mlwhalf1(eesc, eesch, deesc, dt, fx, dxi, dxim, ix);
+    mlwsrch1(eesch, deesc, dt, ss, ix);
+    for (int i = 1; i <= ix; i++) {
+        double bbm = sq(by[I1(i)]) - sq(bx[I1(i)]);
+        double bb = sq(by[I1(i)]) + sq(bx[I1(i)]);
+        fx[I1(i)] = ro[I1(i)] * sq(vx[I1(i)]) + pr[I1(i)] + bbm * pi8i;
+        ss[I1(i)] = ro[I1(i)] * gx[I1(i)]
+                  + (ro[I1(i)] * sq(vy[I1(i)]) - sq(by[I1(i)]) * pi4i) / rr[I1(i)] * drr[I1(i)];
+        fx[I1(i)] = fx[I1(i)] * sc[I1(i)];
+        ss[I1(i)] = ss[I1(i)] * sc[I1(i)] + (pr[I1(i)] + bb * pi8i) * dsc[I1(i)];
+    }
+    mlwhalf1(rxsc, rxsch, drxsc, dt, fx, dxi, dxim, ix);
+    mlwsrch1(rxsch, drxsc, dt, ss, ix);
+    for (int i = 1; i <= ix; i++) {
+        fx[I1(i)] = ro[I1(i)] * vx[I1(i)] * vy[I1(i)] - bx[I1(i)] * by[I1(i)] * pi4i;
+        fx[I1(i)] = fx[I1(i)] * sc[I1(i)] * rr[I1(i)];
+    }
+    mlwhalf1(rysc, rysch, drysc, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) {
+        fx[I1(i)] = -ez[I1(i)];
+        fx[I1(i)] = fx[I1(i)] * sc[I1(i)];
+        fx[I1(i)] = fx[I1(i)] / rr[I1(i)];
+    }
+    mlwhalf1(bysc, bysch, dbysc, dt, fx, dxi, dxim, ix);
+
+    /* half-step primitives, :170-183 */
+    for (int i = 1; i <= ix - 1; i++) {
+        roh[I1(i)] = rosch[I1(i)] / scm[I1(i)];
+        eeh[I1(i)] = eesch[I1(i)] / scm[I1(i)];
+        vxh[I1(i)] = rxsch[I1(i)] / rosch[I1(i)];
+        vyh[I1(i)] = rysch[I1(i)] / rosch[I1(i)] / rrm[I1(i)];
+        byh[I1(i)] = bysch[I1(i)] / scm[I1(i)] * rrm[I1(i)];
+        double vv = sq(vxh[I1(i)]) + sq(vyh[I1(i)]);
+        double bb = sq(bxm[I1(i)]) + sq(byh[I1(i)]);
+        prh[I1(i)] = (gm - 1.0) * (eeh[I1(i)] - 0.5 * roh[I1(i)] * vv - pi8i * bb);
+    }
+    /* :181-183 loops to ix; element ix reads the never-written vxh(ix) etc.
+     * and is never used, so the restatement stops at ix-1 */
+    for (int i = 1; i <= ix - 1; i++) ezh[I1(i)] = -vxh[I1(i)] * byh[I1(i)] + vyh[I1(i)] * bxm[I1(i)];
+
+    /* full step, :188-234 */
+    for (int i = 1; i <= ix - 1; i++) {
+        fx[I1(i)] = roh[I1(i)] * vxh[I1(i)];
+        fx[I1(i)] = fx[I1(i)] * scm[I1(i)];
+    }
+    mlwfull1(drosc, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) {
+        double vv = sq(vxh[I1(i)]) + sq(vyh[I1(i)]);
+        double eph = prh[I1(i)] * gm / (gm - 1.0) + 0.5 * roh[I1(i)] * vv;
+        fx[I1(i)] = eph * vxh[I1(i)] - byh[I1(i)] * ezh[I1(i)] * pi4i;
+        ss[I1(i)] = roh[I1(i)] * vxh[I1(i)] * gxm[I1(i)];
+        fx[I1(i)] = fx[I1(i)] * scm[I1(i)];
+        ss[I1(i)] = ss[I1(i)] * scm[I1(i)];
+    }
+    mlwfull1(deesc, dt, fx, dxi, ix);
+    mlwsrcf1(deesc, dt, ss, ux0, ux1, ix);
+    for (int i = 1; i <= ix - 1; i++) {
+        double bbm = sq(byh[I1(i)]) - sq(bxm[I1(i)]);
+        double bb = sq(byh[I1(i)]) + sq(bxm[I1(i)]);
+        fx[I1(i)] = roh[I1(i)] * sq(vxh[I1(i)]) + prh[I1(i)] + bbm * pi8i;
+        ss[I1(i)] = roh[I1(i)] * gxm[I1(i)]
+                  + (roh[I1(i)] * sq(vyh[I1(i)]) - sq(byh[I1(i)]) * pi4i) / rrm[I1(i)] * drrm[I1(i)];
+        fx[I1(i)] = fx[I1(i)] * scm[I1(i)];
+        ss[I1(i)] = ss[I1(i)] * scm[I1(i)] + (prh[I1(i)] + bb * pi8i) * dscm[I1(i)];
+    }
+    mlwfull1(drxsc, dt, fx, dxi, ix);
+    mlwsrcf1(drxsc, dt, ss, ux0, ux1, ix);
+    for (int i = 1; i <= ix - 1; i++) {
+        fx[I1(i)] = roh[I1(i)] * vxh[I1(i)] * vyh[I1(i)] - bxm[I1(i)] * byh[I1(i)] * pi4i;
+        fx[I1(i)] = fx[I1(i)] * scm[I1(i)] * rrm[I1(i)];
+    }
+    mlwfull1(drysc, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) {
+        fx[I1(i)] = -ezh[I1(i)];
+        fx[I1(i)] = fx[I1(i)] * scm[I1(i)];
+        fx[I1(i)] = fx[I1(i)] / rrm[I1(i)];
+    }
+    mlwfull1(dbysc, dt, fx, dxi, ix);
+
+    /* artificial viscosity on the sc-weighted variables, :240-247 */
+    const double zero = 0.0;
+    for (int i = 1; i <= ix - 1; i++)
+        qx[I1(i)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I1(i + 1)] - vx[I1(i)]) - R4_1EM4);
+    mlwartv1(rosc, drosc, dt, qx, dxi, dxim, ix);
+    mlwartv1(eesc, deesc, dt, qx, dxi, dxim, ix);
+    mlwartv1(rxsc, drxsc, dt, qx, dxi, dxim, ix);
+    mlwartv1(rysc, drysc, dt, qx, dxi, dxim, ix);
+    mlwartv1(bysc, dbysc, dt, qx, dxi, dxim, ix);
+
+    /* update and primitive recovery, :251-268 */
+    for (int i = 2; i <= ix - 1; i++) {
+        ro[I1(i)] = (rosc[I1(i)] + drosc[I1(i)]) / sc[I1(i)];
+        ee[I1(i)] = (eesc[I1(i)] + deesc[I1(i)]) / sc[I1(i)];
+        rx[I1(i)] = (rxsc[I1(i)] + drxsc[I1(i)]) / sc[I1(i)];
+        ry[I1(i)] = (rysc[I1(i)] + drysc[I1(i)]) / sc[I1(i)] / rr[I1(i)];
+        by[I1(i)] = (bysc[I1(i)] + dbysc[I1(i)]) / sc[I1(i)] * rr[I1(i)];
+    }
+    for (int i = 2; i <= ix - 1; i++) {
+        vx[I1(i)] = rx[I1(i)] / ro[I1(i)];
+        vy[I1(i)] = ry[I1(i)] / ro[I1(i)];
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]);
+        double bb = sq(bx[I1(i)]) + sq(by[I1(i)]);
+        pr[I1(i)] = (gm - 1.0) * (ee[I1(i)] - 0.5 * ro[I1(i)] * vv - pi8i * bb);
+    }
+    free(dxi); free(dxim); free(ux0); free(ux1); free(ee); free(rx); free(ry); free(ez);
+    free(roh); free(prh); free(vxh); free(vyh); free(byh); free(eeh); free(ezh);
+    free(rosc); free(eesc); free(rxsc); free(rysc); free(bysc);
+    free(rosch); free(eesch); free(rxsch); free(rysch); free(bysch);
+    free(drosc); free(deesc); free(drxsc); free(drysc); free(dbysc);
+    free(fx); free(qx); free(ss);
+}

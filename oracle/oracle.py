@@ -2,7 +2,7 @@
 
 ``Oracle``  -- liboracle.so, this repository's C restatement of the reference
               routines (cans_oracle*.c).
-``Ref``     -- _ref/libcans{1d,2d,3d,hlld}_ref.so, the reference Fortran
+``Ref``     -- _ref/libcans{1d,2d,3d,hlld,spicule}_ref.so, the reference Fortran
               routines compiled from their own sources (see Makefile), called
               with Fortran conventions (every argument by reference).
 
@@ -24,7 +24,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_ORACLE = HERE / "liboracle.so"
 REF_DIR = HERE / "_ref"
-REF_LIBS = {d: REF_DIR / f"libcans{d}_ref.so" for d in ("1d", "2d", "3d", "hlld")}
+REF_LIBS = {d: REF_DIR / f"libcans{d}_ref.so" for d in ("1d", "2d", "3d", "hlld", "spicule")}
 REFDIR_SRC = Path("/root/reference/cans_beta_vary")
 
 _P = C.POINTER(C.c_double)
@@ -106,6 +106,57 @@ class Oracle:
         dx, xm, dxm = (np.zeros(ix, order="F") for _ in range(3))
         self.lib.or1d_grdrdy(_arr(dx), _arr(xm), _arr(dxm), _arr(x), C.c_int(ix))
         return dx, xm, dxm
+
+    # ---- UTRIP spicule model (cans1d/hdmlw/mlw_m_bg.f, md_spicule/hdmlw)
+    GEOM = ("bx", "bxm", "gx", "gxm", "sc", "dsc", "scm", "dscm", "rr", "rrm", "drr", "drrm", "dx", "dxm")
+
+    def mlw_m_bg_1d(self, ro, pr, vx, vy, by, g, dt, qav, gm):
+        """g: dict of the geometry arrays bx, bxm, gx, gxm, sc, dsc, scm,
+        dscm, rr, rrm, drr, drrm, dx, dxm."""
+        self.lib.or1d_mlw_m_bg(*map(_arr, (ro, pr, vx, vy, by, g["bx"], g["bxm"])), C.c_double(dt),
+                               C.c_double(qav), C.c_double(gm),
+                               *[_arr(g[k]) for k in ("gx", "gxm", "sc", "dsc", "scm", "dscm", "rr", "rrm", "drr",
+                                                      "drrm", "dx", "dxm")], C.c_int(ro.shape[0]))
+
+    def scrdy_1d(self, dsc, dscm, sc, scm, dx, dxm):
+        self.lib.or1d_scrdy(*map(_arr, (dsc, dscm, sc, scm, dx, dxm)), C.c_int(sc.shape[0]))
+
+    def chkdav_1d(self, da, vx, floor):
+        n = C.c_int()
+        self.lib.or1d_chkdav(C.byref(n), _arr(da), _arr(vx), C.c_double(floor), C.c_int(da.shape[0]))
+        return n.value
+
+    def rangen(self, seed):
+        """(value, new seed)"""
+        s = C.c_int(int(seed))
+        self.lib.or_rangen.restype = C.c_double
+        v = self.lib.or_rangen(C.byref(s))
+        return v, s.value
+
+    def trqalfven(self, vy, dt, t, x, rr, seed):
+        """returns the advanced seed"""
+        s = C.c_int(int(seed))
+        self.lib.orsp_trqalfven(_arr(vy), C.c_double(dt), C.c_double(t), _arr(x), C.c_int(vy.shape[0]), _arr(rr),
+                                C.byref(s))
+        return s.value
+
+    def spicule_bnd(self, margin, ro, pr, vx, vy, by):
+        self.lib.orsp_bnd(C.c_int(margin), *map(_arr, (ro, pr, vx, vy, by)), C.c_int(ro.shape[0]))
+
+    def spicule_run(self, st, g, x, margin, gm, tend, nstop, t, ns, seed, qav=1.0, safety=0.8, dtmin=1e-10,
+                    floor=1e-9):
+        """main.f's loop from (t, ns, seed) on the state dict st (in place);
+        returns (t, ns, merr, seed)."""
+        geo = (C.c_void_p * 15)(*[C.c_void_p(g[k].ctypes.data) for k in self.GEOM] + [C.c_void_p(x.ctypes.data)])
+        for k in self.GEOM:
+            _arr(g[k])
+        _arr(x)
+        tc, nc, mc, sc = C.c_double(t), C.c_int(ns), C.c_int(0), C.c_int(int(seed))
+        self.lib.orsp_run(*[_arr(st[k]) for k in ("ro", "pr", "vx", "vy", "by")], geo, C.c_int(st["ro"].shape[0]),
+                          C.c_int(margin), C.c_double(gm), C.c_double(qav), C.c_double(safety), C.c_double(dtmin),
+                          C.c_double(floor), C.c_double(tend), C.c_int(nstop), C.byref(tc), C.byref(nc),
+                          C.byref(mc), C.byref(sc))
+        return tc.value, nc.value, mc.value, sc.value
 
     # ---- 2D
     def mlw_m_2d(self, ro, pr, vx, vy, bx, by, az, dt, qav, gm, dx, dxm, dy, dym):

@@ -48,6 +48,11 @@ def refhlld():
 
 
 @pytest.fixture(scope="session")
+def refsp():
+    return _ref("spicule")
+
+
+@pytest.fixture(scope="session")
 def dev():
     import torch
     if not torch.cuda.is_available():

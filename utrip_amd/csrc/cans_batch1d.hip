@@ -19,68 +19,13 @@
 // FP64.
 #include "cans_common.hpp"
 #include "cans_kernels.hpp"
+#include "cans_wave.hpp"
 
 namespace cans {
 namespace {
 
 constexpr double pi8i = 1.0 / kPi / 8.0;   // mlw_m.f:55
 constexpr double pi4i = 1.0 / kPi / 4.0;   // mlw_m.f:56
-constexpr double cfl_pi4i = 0.25 / kPi;    // cfl_m.f (1D)
-
-CANS_DEV double shl1(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
-    return __hiloint2double(hi, lo);
-}
-CANS_DEV double shr1(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, true);
-    return __hiloint2double(hi, lo);
-}
-CANS_DEV float shl1(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
-}
-CANS_DEV float shr1(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
-}
-
-// one 1D cfl_m term (cfl_m.f: onero, ca2, cs2, dtcfl)
-CANS_DEV double dtq1(double ro, double pr, double vx, double by, double bx, double dx, double gm, double safety) {
-    const double onero = 1.0 / ro;
-    const double ca2 = (bx * bx + by * by) * cfl_pi4i * onero;
-    const double cs2 = gm * pr * onero;
-    const double dtcfl = dx / (__builtin_fabs(vx) + __builtin_sqrt(cs2 + ca2));
-    return safety * dtcfl;
-}
-
-// Values every lane of the workgroup holds identically, made uniform for the
-// compiler.  The loops below contain barriers; their exit tests must be
-// scalar branches: with a loop exit the compiler believes divergent, the
-// control-flow structurizer may send part of a wave back to a barrier ahead
-// of lanes still finishing the previous iteration (the queue pop), so the
-// workgroup would re-read a stale tube index and never leave the loop.
-CANS_DEV int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-CANS_DEV double uni(double v) {
-    return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
-                            __builtin_amdgcn_readfirstlane(__double2loint(v)));
-}
-CANS_DEV long long uni(long long v) {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffll));
-    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
-    return (long long)(((unsigned long long)hi << 32) | lo);
-}
-
-template <int BT>
-CANS_DEV double block_min(double v, double *red) {
-    constexpr int NW = BT / 64;
-    v = wave_min(v);                       // the same value in every lane
-    red[threadIdx.x >> 6] = v;             // all lanes store it: no divergent branch
-    __syncthreads();
-    double m = red[0];
-#pragma unroll
-    for (int w = 1; w < NW; w++) m = (red[w] < m) ? red[w] : m;
-    return uni(m);
-}
 
 // One wave updates the 62 interior cells c0..c0+61 of a tube (lane l holds
 // cell c0-1+l): cur -> nxt, ghost copies, and the min of the new dtq terms.

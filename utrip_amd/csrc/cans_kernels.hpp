@@ -126,6 +126,55 @@ struct Batch1dArgs {
 };
 hipError_t launch_batch1d(const Batch1dArgs &a, int mixed, int block_threads, int nblocks, hipStream_t s);
 
+// UTRIP spicule (cans1d/md_spicule/hdmlw): geometry arrays of mlw_m_bg, in
+// this order.
+enum { SG_BX, SG_BXM, SG_GX, SG_GXM, SG_SC, SG_DSC, SG_SCM, SG_DSCM, SG_RR, SG_RRM, SG_DRR, SG_DRRM, SG_DX,
+       SG_DXM, SG_N };
+
+// One mlw_m_bg call: in -> out (cells 2..ix-1); met = 1/dx, 1/dxm, ux0, ux1
+// as filled by launch_spicule_prep.
+struct MlwBgArgs {
+    const double *in[5];
+    double *out[5];
+    const double *geo[SG_N];
+    const double *met[4];
+    double dt, qav, gm;
+    int ix;
+};
+hipError_t launch_mlw1d_bg(const MlwBgArgs &a, hipStream_t s);
+
+// Many realizations of the spicule time loop (main.f:169-227), each run by
+// one workgroup of a persistent kernel.  Run r's fields sit at r*ix of f[];
+// its geometry at r*gstride of geo[], met[] and trq (gstride 0: shared).
+struct SpiculeArgs {
+    int nrun, ix, margin;
+    long long gstride;
+    double *f[5];                   // ro pr vx vy by (in place)
+    double *g[5];                   // scratch of the same size
+    const double *geo[SG_N];
+    const double *met[4];           // filled by launch_spicule_prep
+    const double *trq;              // torque profile (trqalfven.f:23-28)
+    const long long *pw;            // 16807^(i+1) mod (2^31-1), i < ix (prep)
+    double gm, qav, safety, dtmin, floor, tend, dtout;
+    int nstop;
+    double *t;                      // per run: in t, out t
+    int *ns, *merr, *seed;          // per run: in/out (ns: loop entries so far)
+    double *snap, *snap_t;          // NULL, or [run][nsnap_max][5][ix] and [run][nsnap_max]
+    int nsnap_max;
+    int *nsnap;                     // per run: snapshots written by the launch
+    unsigned *next;                 // queue counter (zeroed by the prep kernel)
+};
+hipError_t launch_spicule(const SpiculeArgs &a, int block_threads, int nblocks, hipStream_t s);
+// metric of nset geometry sets (stride gstride), the draw multipliers, and
+// the queue counter (may be NULL)
+hipError_t launch_spicule_prep(const double *dx, const double *dxm, double *const *met, int ix, int nset,
+                               long long gstride, long long *pw, unsigned *next, hipStream_t s);
+// trqalfven.f with the Park-Miller stream starting at seed (one call)
+hipError_t launch_trq1d(double *vy, double dt, const double *trq, const double *rr, const long long *pw,
+                        long long seed, int ix, hipStream_t s);
+// chkdav.f on n cells; *count (device) = number of floored cells
+hipError_t launch_chkdav1d(double *da, double *vx, double floor_, int n, int *count, hipStream_t s);
+
 // Reverse mode of one fused periodic 2D MLW step (6 fields ro..by).
 // B1-B3 of the adjoint of step n.  dtp: the dt step n used; acc: 2 doubles
 // of zeroed accumulators (dL/ddt, ties) for the step; pend_dt/pend_acc: dt and
