@@ -91,3 +91,26 @@ def test_cans2d_main(dev, oracle, tmp_path, engine, resident):
     for k in F2:
         assert_bitwise(dac.read_last(outd / f"{k}.dac"), f[k], k)
     assert float(dac.read_last(outd / "t.dac")) == t
+
+
+@pytest.mark.parametrize("resident", [0, 1])
+def test_spicule_main(dev, oracle, tmp_path, resident):
+    """The UTRIP spicule loop as a Fortran main program on the drop-in
+    library (mlw_m_bg, trqalfven, chkdav, cfl_m, bd*), against the C
+    restatement's loop."""
+    from utrip_amd import spicule
+    sp = spicule.setup()
+    nsteps, seed = 60, 246810
+    st = {k: np.asfortranarray(v.copy()) for k, v in sp.state.items()}
+    g = {k: np.asfortranarray(v) for k, v in sp.geom.items()}
+    t, ns, merr, _ = oracle.spicule_run(st, g, np.asfortranarray(sp.x), sp.margin, sp.gm, spicule.TEND, nsteps,
+                                        0.0, 0, seed)
+    assert (ns, merr) == (nsteps, 0)
+    ind, outd = tmp_path / "in", tmp_path / "out"
+    dac.write_state(ind, sp.state, dict(sp.geom, x=sp.x))
+    outd.mkdir()
+    out = run_main([BIN / "spicule_main", ind, outd, nsteps, repr(spicule.TEND), repr(sp.gm), seed, resident])
+    assert f"step={nsteps:8d}" in out, out
+    for k in F1:
+        assert_bitwise(dac.read_last(outd / f"{k}.dac"), st[k], k)
+    assert float(dac.read_last(outd / "t.dac")) == t

@@ -100,6 +100,14 @@ def build_fortran() -> list[Path]:
             _run([FLANG, "-O2", "-J", str(objdir), "-o", str(exe), str(FSRC / "dac_io.f90"), str(main),
                   f"-L{PKG}", f"-lcans{dim}_gpu", "-lcans_gpu", "-Wl,-rpath,$ORIGIN/.."], cwd=objdir)
             out.append(exe)
+    main = FSRC / "spicule_main.f90"          # the UTRIP model, on the 1D library
+    if main.exists() and (PKG / "libcans1d_gpu.so").exists():
+        bindir = PKG / "bin"
+        bindir.mkdir(exist_ok=True)
+        exe = bindir / "spicule_main"
+        _run([FLANG, "-O2", "-J", str(objdir), "-o", str(exe), str(FSRC / "dac_io.f90"), str(main),
+              f"-L{PKG}", "-lcans1d_gpu", "-lcans_gpu", "-Wl,-rpath,$ORIGIN/.."], cwd=objdir)
+        out.append(exe)
     return out
 
 

@@ -1,7 +1,9 @@
 !=======================================================================
 ! libcans1d_gpu -- the cans1d hot-path routines with the reference's names
 ! and argument lists (cans1d/hdmlw/mlw_m.f:2, cans1d/hdroe/roe_m.f:2,
-! cans1d/common/cfl_m.f:2, cans1d/bc/bd*x.f), computed on the GPU by
+! cans1d/common/cfl_m.f:2, cans1d/bc/bd*x.f, and for the UTRIP spicule model
+! cans1d/hdmlw/mlw_m_bg.f:2, cans1d/common/chkdav.f:2,
+! cans1d/md_spicule/hdmlw/trqalfven.f:2), computed on the GPU by
 ! libcans_gpu.so.  A cans1d main program links this library instead of
 ! libcans1d.a and runs unchanged.
 !=======================================================================
@@ -17,6 +19,90 @@ subroutine mlw_m(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm, ix)
   call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vy, 1); call dunmap(by, 1)
   call dunmap(bx, 0); call dunmap(bxm, 0); call dunmap(dx, 0); call dunmap(dxm, 0)
 end subroutine mlw_m
+
+subroutine mlw_m_bg(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, gx, gxm, sc, dsc, scm, dscm, rr, rrm, drr, drrm, &
+     dx, dxm, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: ro(ix), pr(ix), vx(ix), vy(ix), by(ix)
+  double precision, target, intent(in) :: bx(ix), bxm(ix), gx(ix), gxm(ix), sc(ix), dsc(ix), scm(ix), dscm(ix)
+  double precision, target, intent(in) :: rr(ix), rrm(ix), drr(ix), drrm(ix), dx(ix), dxm(ix)
+  double precision, intent(in) :: dt, qav, gm
+  call check(c_mlw1d_bg(dmap(ro, ix), dmap(pr, ix), dmap(vx, ix), dmap(vy, ix), dmap(by, ix), dmap(bx, ix), &
+       dmap(bxm, ix), dt, qav, gm, dmap(gx, ix), dmap(gxm, ix), dmap(sc, ix), dmap(dsc, ix), dmap(scm, ix), &
+       dmap(dscm, ix), dmap(rr, ix), dmap(rrm, ix), dmap(drr, ix), dmap(drrm, ix), dmap(dx, ix), dmap(dxm, ix), &
+       int(ix, c_int), c_null_ptr), 'mlw_m_bg')
+  call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vy, 1); call dunmap(by, 1)
+  call dunmap(bx, 0); call dunmap(bxm, 0); call dunmap(gx, 0); call dunmap(gxm, 0); call dunmap(sc, 0)
+  call dunmap(dsc, 0); call dunmap(scm, 0); call dunmap(dscm, 0); call dunmap(rr, 0); call dunmap(rrm, 0)
+  call dunmap(drr, 0); call dunmap(drrm, 0); call dunmap(dx, 0); call dunmap(dxm, 0)
+end subroutine mlw_m_bg
+
+subroutine chkdav(n_events, da, vx, floor, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  integer, intent(out) :: n_events
+  double precision, target :: da(ix), vx(ix)
+  double precision, intent(in) :: floor
+  integer(c_int) :: n
+  call check(c_chkdav1d(n, dmap(da, ix), dmap(vx, ix), floor, int(ix, c_int), c_null_ptr), 'chkdav')
+  call dunmap(da, 1); call dunmap(vx, 1)
+  n_events = n
+end subroutine chkdav
+
+!-----------------------------------------------------------------------
+! trqalfven with the Park-Miller stream of cans1d/common/rangen.f in place of
+! the compiler's RANDOM_NUMBER (one draw per cell per call, cell order).
+! The seed is this library's (cans_gpu_rangen_seed sets it; default 1).
+! The torque profile tanh((x-0.75)/0.075)-1 depends on x only: it is
+! evaluated on the host (this compiler's libm tanh) when x changes.
+!-----------------------------------------------------------------------
+module cans_spicule_state
+  implicit none
+  integer, save :: mseed = 1
+  double precision, allocatable, target, save :: trq(:), xsaved(:)
+end module cans_spicule_state
+
+subroutine cans_gpu_rangen_seed(iseed)
+  use cans_spicule_state
+  implicit none
+  integer, intent(in) :: iseed
+  mseed = iseed
+end subroutine cans_gpu_rangen_seed
+
+subroutine trqalfven(vy, dt, time, x, ix, rr)
+  use cans_gpu_iface
+  use cans_spicule_state
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: vy(ix)
+  double precision, target, intent(in) :: x(ix), rr(ix)
+  double precision, intent(in) :: dt, time
+  integer :: i
+  integer(c_int) :: s
+  logical :: fresh
+  fresh = .not. allocated(trq)
+  if (.not. fresh) fresh = size(trq) /= ix
+  if (.not. fresh) fresh = any(xsaved /= x)
+  if (fresh) then
+     if (allocated(trq)) then
+        call check(c_invalidate(c_loc(trq(1))), 'trqalfven')
+        deallocate(trq, xsaved)
+     end if
+     allocate(trq(ix), xsaved(ix))
+     do i = 1, ix
+        trq(i) = tanh((x(i) - 0.75) / 0.075) - 1
+     end do
+     trq(ix) = trq(ix - 2)
+     xsaved = x
+  end if
+  s = mseed
+  call check(c_trq1d(dmap(vy, ix), dt, dmap(trq, ix), dmap(rr, ix), int(ix, c_int), s, c_null_ptr), 'trqalfven')
+  mseed = s
+  call dunmap(vy, 1); call dunmap(trq, 0); call dunmap(rr, 0)
+end subroutine trqalfven
 
 subroutine roe_m(ro, pr, vx, vy, by, bx, bxm, dt, gm, dx, ix)
   use cans_gpu_iface

@@ -53,6 +53,27 @@ module cans_gpu_iface
        real(c_double), value :: dt, qav, gm
        integer(c_int), value :: ix
      end function c_mlw1d
+     integer(c_int) function c_mlw1d_bg(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, gx, gxm, sc, dsc, scm, dscm, &
+          rr, rrm, drr, drrm, dx, dxm, ix, s) bind(C, name="cans1d_mlw_m_bg")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vy, by, bx, bxm, gx, gxm, sc, dsc, scm, dscm, rr, rrm, drr, drrm, dx, dxm, s
+       real(c_double), value :: dt, qav, gm
+       integer(c_int), value :: ix
+     end function c_mlw1d_bg
+     integer(c_int) function c_trq1d(vy, dt, trq, rr, ix, mseed, s) bind(C, name="cans1d_trqalfven")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: vy, trq, rr, s
+       real(c_double), value :: dt
+       integer(c_int), value :: ix
+       integer(c_int) :: mseed
+     end function c_trq1d
+     integer(c_int) function c_chkdav1d(n_events, da, vx, floor, ix, s) bind(C, name="cans1d_chkdav")
+       import :: c_ptr, c_double, c_int
+       integer(c_int) :: n_events
+       type(c_ptr), value :: da, vx, s
+       real(c_double), value :: floor
+       integer(c_int), value :: ix
+     end function c_chkdav1d
      integer(c_int) function c_roe1d(ro, pr, vx, vy, by, bx, bxm, dt, gm, dx, ix, s) &
           bind(C, name="cans1d_roe_m")
        import :: c_ptr, c_double, c_int
