@@ -58,6 +58,16 @@ int cans1d_roe_m(double *ro, double *pr, double *vx, double *vy, double *by,
 int cans1d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
                  const double *bx, const double *ro, const double *pr, const double *vx,
                  const double *vy, const double *by, const double *dx, int ix, void *stream);
+/* cans1d/hdmlw/mlw_m3.f:2 mlw_m3(ro,pr,vx,vy,vz,by,bz,bx,bxm,dt,qav,gm,dx,dxm,ix):
+ * 1D MHD with three velocity and field components; updated in place on
+ * i=2..ix-1. */
+int cans1d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *by, double *bz,
+                  const double *bx, const double *bxm, double dt, double qav, double gm, const double *dx,
+                  const double *dxm, int ix, void *stream);
+/* cans1d/common/cfl_m3.f:2 cfl_m3(dt,safety,dtmin,merr,gm,bx,ro,pr,vx,vy,by,vz,bz,dx,ix) */
+int cans1d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *bx,
+                  const double *ro, const double *pr, const double *vx, const double *vy, const double *by,
+                  const double *vz, const double *bz, const double *dx, int ix, void *stream);
 /* cans1d/bc/bd*x.f: kind = CANS_BD_*; mbnd = side (0 low, 1 high), or for
  * CANS_BD_PER margin0 with margin = margin1; q0 for bdcnsx, dxm for bdfrdx. */
 int cans1d_bnd(int kind, int mbnd, int margin, double *qq, double q0, const double *dxm,
@@ -84,6 +94,18 @@ int cans2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
 /* cans2d/bc/bd*{x,y}.f; dir 0 = x, 1 = y */
 int cans2d_bnd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
                const double *dm, int ix, int jx, void *stream);
+
+/* cans2d/hdmlw/mlw_m3.f:2-3 mlw_m3(ro,pr,vx,vy,vz,bx,by,bz,az,dt,qav,gm,dx,dxm,ix,dy,dym,jx):
+ * 2D MHD with vz, bz (fuses mlwhalf.f, mlwfull.f, mlwartv.f, mlwsrch.f,
+ * mlwsrcf.f); updated in place on i=2..ix-1, j=2..jx-1. */
+int cans2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by, double *bz,
+                  double *az, double dt, double qav, double gm, const double *dx, const double *dxm, int ix,
+                  const double *dy, const double *dym, int jx, void *stream);
+/* cans2d/common/cfl_m3.f:2-3 cfl_m3(dt,safety,dtmin,merr,gm,ro,pr,vx,vy,vz,bx,by,bz,dx,ix,dy,jx) */
+int cans2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *ro,
+                  const double *pr, const double *vx, const double *vy, const double *vz, const double *bx,
+                  const double *by, const double *bz, const double *dx, int ix, const double *dy, int jx,
+                  void *stream);
 
 /* ---------------------------------------------------------------- 3D
  * cans3d/hdmlw/mlw_m.f:2-3
@@ -193,6 +215,7 @@ int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *
 #define CANS_ENGINE_MLW 0   /* cans{1,2,3}d/hdmlw/mlw_m.f */
 #define CANS_ENGINE_ROE 1   /* cans{1,2,3}d/hdroe/roe_m.f */
 #define CANS_ENGINE_HLLD 2  /* cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f (3D only) */
+#define CANS_ENGINE_MLW3 3  /* cans{1,2}d/hdmlw/mlw_m3.f with cfl_m3 (1D/2D only) */
 
 /* Device workspace (bytes) a cans_run of this shape needs. */
 size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx);

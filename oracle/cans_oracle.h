@@ -48,6 +48,15 @@ void or1d_mlw_m_bg(double *ro, double *pr, double *vx, double *vy, double *by,
                    const double *dscm, const double *rr, const double *rrm,
                    const double *drr, const double *drrm, const double *dx,
                    const double *dxm, int ix);
+void or1d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
+                 double *by, double *bz, const double *bx, const double *bxm,
+                 double dt, double qav, double gm, const double *dx,
+                 const double *dxm, int ix);
+void or1d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm,
+                 const double *bx, const double *ro, const double *pr,
+                 const double *vx, const double *vy, const double *by,
+                 const double *vz, const double *bz, const double *dx, int ix,
+                 int *imin);
 void or1d_scrdy(double *dsc, double *dscm, const double *sc, const double *scm,
                 const double *dx, const double *dxm, int ix);
 void or1d_chkdav(int *n_events, double *da, double *vx, double floor_, int ix);
@@ -79,6 +88,15 @@ void or2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
                 const double *vy, const double *bx, const double *by,
                 const double *dx, int ix, const double *dy, int jx,
                 int *imin, int *jmin);
+void or2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
+                 double *bx, double *by, double *bz, double *az, double dt,
+                 double qav, double gm, const double *dx, const double *dxm,
+                 int ix, const double *dy, const double *dym, int jx);
+void or2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm,
+                 const double *ro, const double *pr, const double *vx,
+                 const double *vy, const double *vz, const double *bx,
+                 const double *by, const double *bz, const double *dx, int ix,
+                 const double *dy, int jx, int *imin, int *jmin);
 /* dir: 0 = x, 1 = y; kind as in or1d_bd, 7 = periodic (mbnd = margin0,
  * margin = margin1) */
 void or2d_bd(int dir, int kind, int mbnd, int margin, double *qq, double q0,

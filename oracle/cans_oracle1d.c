@@ -423,3 +423,161 @@ void or1d_mlw_m_bg(double *ro, double *pr, double *vx, double *vy, double *by,
     free(drosc); free(deesc); free(drxsc); free(drysc); free(dbysc);
     free(fx); free(qx); free(ss);
 }
+
+/* hdmlw/mlw_m3.f:2-252: 1D MHD with three velocity and field components
+ * (Bx constant, not in the energy) */
+void or1d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
+                 double *by, double *bz, const double *bx, const double *bxm,
+                 double dt, double qav, double gm, const double *dx,
+                 const double *dxm, int ix) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0; /* :57 */
+    const double pi4i = 1.0 / pi / 4.0;
+    double *dxi = dalloc(ix), *dxim = dalloc(ix);
+    double *ee = dalloc(ix), *rx = dalloc(ix), *ry = dalloc(ix), *rz = dalloc(ix), *ey = dalloc(ix), *ez = dalloc(ix);
+    double *roh = dalloc(ix), *eeh = dalloc(ix), *rxh = dalloc(ix), *ryh = dalloc(ix), *rzh = dalloc(ix);
+    double *byh = dalloc(ix), *bzh = dalloc(ix), *prh = dalloc(ix), *vxh = dalloc(ix), *vyh = dalloc(ix);
+    double *vzh = dalloc(ix), *eyh = dalloc(ix), *ezh = dalloc(ix);
+    double *dro = dalloc(ix), *dee = dalloc(ix), *drx = dalloc(ix), *dry = dalloc(ix), *drz = dalloc(ix);
+    double *dby = dalloc(ix), *dbz = dalloc(ix), *fx = dalloc(ix), *qx = dalloc(ix);
+    for (int i = 1; i <= ix; i++) {
+        dxim[I1(i)] = 1.0 / dxm[I1(i)];
+        dxi[I1(i)] = 1.0 / dx[I1(i)];
+    }
+    for (int i = 1; i <= ix; i++) { /* :71-79 */
+        dro[I1(i)] = 0.0; dee[I1(i)] = 0.0; drx[I1(i)] = 0.0; dry[I1(i)] = 0.0;
+        dby[I1(i)] = 0.0; drz[I1(i)] = 0.0; dbz[I1(i)] = 0.0;
+    }
+    for (int i = 1; i <= ix; i++) { /* :83-91 */
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]) + sq(vz[I1(i)]);
+        double bb = sq(by[I1(i)]) + sq(bz[I1(i)]);
+        ee[I1(i)] = pr[I1(i)] / (gm - 1.0) + 0.5 * ro[I1(i)] * vv + pi8i * bb;
+        rx[I1(i)] = ro[I1(i)] * vx[I1(i)];
+        ry[I1(i)] = ro[I1(i)] * vy[I1(i)];
+        rz[I1(i)] = ro[I1(i)] * vz[I1(i)];
+    }
+    for (int i = 1; i <= ix; i++) { /* :92-95 */
+        ey[I1(i)] = -vz[I1(i)] * bx[I1(i)] + vx[I1(i)] * bz[I1(i)];
+        ez[I1(i)] = -vx[I1(i)] * by[I1(i)] + vy[I1(i)] * bx[I1(i)];
+    }
+    /* half step, :99-143 */
+    for (int i = 1; i <= ix; i++) fx[I1(i)] = ro[I1(i)] * vx[I1(i)];
+    mlwhalf1(ro, roh, dro, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) {
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]) + sq(vz[I1(i)]);
+        double ep = pr[I1(i)] * gm / (gm - 1.0) + 0.5 * ro[I1(i)] * vv;
+        fx[I1(i)] = ep * vx[I1(i)] - by[I1(i)] * ez[I1(i)] * pi4i + bz[I1(i)] * ey[I1(i)] * pi4i;
+    }
+    mlwhalf1(ee, eeh, dee, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) {
+        double bbm = sq(by[I1(i)]) + sq(bz[I1(i)]);
+        fx[I1(i)] = ro[I1(i)] * sq(vx[I1(i)]) + pr[I1(i)] + bbm * pi8i;
+    }
+    mlwhalf1(rx, rxh, drx, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) fx[I1(i)] = ro[I1(i)] * vx[I1(i)] * vy[I1(i)] - bx[I1(i)] * by[I1(i)] * pi4i;
+    mlwhalf1(ry, ryh, dry, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) fx[I1(i)] = ro[I1(i)] * vx[I1(i)] * vz[I1(i)] - bx[I1(i)] * bz[I1(i)] * pi4i;
+    mlwhalf1(rz, rzh, drz, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) fx[I1(i)] = -ez[I1(i)];
+    mlwhalf1(by, byh, dby, dt, fx, dxi, dxim, ix);
+    for (int i = 1; i <= ix; i++) fx[I1(i)] = ey[I1(i)];
+    mlwhalf1(bz, bzh, dbz, dt, fx, dxi, dxim, ix);
+    /* half-step primitives, :147-159 */
+    for (int i = 1; i <= ix - 1; i++) {
+        vxh[I1(i)] = rxh[I1(i)] / roh[I1(i)];
+        vyh[I1(i)] = ryh[I1(i)] / roh[I1(i)];
+        vzh[I1(i)] = rzh[I1(i)] / roh[I1(i)];
+        double vv = sq(vxh[I1(i)]) + sq(vyh[I1(i)]) + sq(vzh[I1(i)]);
+        double bb = sq(byh[I1(i)]) + sq(bzh[I1(i)]);
+        prh[I1(i)] = (gm - 1.0) * (eeh[I1(i)] - 0.5 * roh[I1(i)] * vv - pi8i * bb);
+    }
+    for (int i = 1; i <= ix - 1; i++) {
+        eyh[I1(i)] = -vzh[I1(i)] * bxm[I1(i)] + vxh[I1(i)] * bzh[I1(i)];
+        ezh[I1(i)] = -vxh[I1(i)] * byh[I1(i)] + vyh[I1(i)] * bxm[I1(i)];
+    }
+    /* full step, :163-207 */
+    for (int i = 1; i <= ix - 1; i++) fx[I1(i)] = roh[I1(i)] * vxh[I1(i)];
+    mlwfull1(dro, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) {
+        double vv = sq(vxh[I1(i)]) + sq(vyh[I1(i)]) + sq(vzh[I1(i)]);
+        double eph = prh[I1(i)] * gm / (gm - 1.0) + 0.5 * roh[I1(i)] * vv;
+        fx[I1(i)] = eph * vxh[I1(i)] - byh[I1(i)] * ezh[I1(i)] * pi4i + bzh[I1(i)] * eyh[I1(i)] * pi4i;
+    }
+    mlwfull1(dee, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) {
+        double bbm = sq(byh[I1(i)]) + sq(bzh[I1(i)]);
+        fx[I1(i)] = roh[I1(i)] * sq(vxh[I1(i)]) + prh[I1(i)] + bbm * pi8i;
+    }
+    mlwfull1(drx, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++)
+        fx[I1(i)] = roh[I1(i)] * vxh[I1(i)] * vyh[I1(i)] - bxm[I1(i)] * byh[I1(i)] * pi4i;
+    mlwfull1(dry, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++)
+        fx[I1(i)] = roh[I1(i)] * vxh[I1(i)] * vzh[I1(i)] - bxm[I1(i)] * bzh[I1(i)] * pi4i;
+    mlwfull1(drz, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) fx[I1(i)] = -ezh[I1(i)];
+    mlwfull1(dby, dt, fx, dxi, ix);
+    for (int i = 1; i <= ix - 1; i++) fx[I1(i)] = eyh[I1(i)];
+    mlwfull1(dbz, dt, fx, dxi, ix);
+    /* artificial viscosity, :212-225 */
+    const double zero = 0.0;
+    for (int i = 1; i <= ix - 1; i++)
+        qx[I1(i)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I1(i + 1)] - vx[I1(i)]) - R4_1EM4);
+    mlwartv1(ro, dro, dt, qx, dxi, dxim, ix);
+    mlwartv1(ee, dee, dt, qx, dxi, dxim, ix);
+    mlwartv1(rx, drx, dt, qx, dxi, dxim, ix);
+    mlwartv1(ry, dry, dt, qx, dxi, dxim, ix);
+    mlwartv1(rz, drz, dt, qx, dxi, dxim, ix);
+    mlwartv1(by, dby, dt, qx, dxi, dxim, ix);
+    mlwartv1(bz, dbz, dt, qx, dxi, dxim, ix);
+    /* update and primitive recovery, :229-249 */
+    for (int i = 2; i <= ix - 1; i++) {
+        ro[I1(i)] = ro[I1(i)] + dro[I1(i)];
+        rx[I1(i)] = rx[I1(i)] + drx[I1(i)];
+        ry[I1(i)] = ry[I1(i)] + dry[I1(i)];
+        rz[I1(i)] = rz[I1(i)] + drz[I1(i)];
+        by[I1(i)] = by[I1(i)] + dby[I1(i)];
+        bz[I1(i)] = bz[I1(i)] + dbz[I1(i)];
+        ee[I1(i)] = ee[I1(i)] + dee[I1(i)];
+    }
+    for (int i = 2; i <= ix - 1; i++) {
+        vx[I1(i)] = rx[I1(i)] / ro[I1(i)];
+        vy[I1(i)] = ry[I1(i)] / ro[I1(i)];
+        vz[I1(i)] = rz[I1(i)] / ro[I1(i)];
+        double vv = sq(vx[I1(i)]) + sq(vy[I1(i)]) + sq(vz[I1(i)]);
+        double bb = sq(by[I1(i)]) + sq(bz[I1(i)]);
+        pr[I1(i)] = (gm - 1.0) * (ee[I1(i)] - 0.5 * ro[I1(i)] * vv - pi8i * bb);
+    }
+    free(dxi); free(dxim); free(ee); free(rx); free(ry); free(rz); free(ey); free(ez);
+    free(roh); free(eeh); free(rxh); free(ryh); free(rzh); free(byh); free(bzh); free(prh);
+    free(vxh); free(vyh); free(vzh); free(eyh); free(ezh);
+    free(dro); free(dee); free(drx); free(dry); free(drz); free(dby); free(dbz); free(fx); free(qx);
+}
+
+/* common/cfl_m3.f:42-71 (1D, interior only, |vx| + fast speed with bz) */
+void or1d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm,
+                 const double *bx, const double *ro, const double *pr,
+                 const double *vx, const double *vy, const double *by,
+                 const double *vz, const double *bz, const double *dx, int ix,
+                 int *imin) {
+    (void)vy; (void)vz;
+    const double pi = acos(-1.0);
+    const double pi4i = 0.25 / pi;
+    double *dtq = dalloc(ix);
+    double d = R4_1E20;
+    int im = 0;
+    for (int i = 2; i <= ix - 1; i++) {
+        double onero = 1.0 / ro[I1(i)];
+        double b2 = sq(bx[I1(i)]) + sq(by[I1(i)]) + sq(bz[I1(i)]);
+        double ca2 = b2 * pi4i * onero;
+        double cs2 = gm * pr[I1(i)] * onero;
+        double dtcfl = dx[I1(i)] / (fabs(vx[I1(i)]) + sqrt(cs2 + ca2));
+        dtq[I1(i)] = safety * dtcfl;
+    }
+    for (int i = 2; i <= ix - 1; i++)
+        if (dtq[I1(i)] < d) { im = i; d = dtq[I1(i)]; }
+    *merr = (d < dtmin) ? 9001 : 0;
+    *dt = d;
+    if (imin) *imin = im;
+    free(dtq);
+}

@@ -295,3 +295,224 @@ void or2d_bbtoaa(double *az, const double *bx, const double *by,
             az[I2(ii, j)] = az[I2(ii + 1, j)] + 0.5 * (by[I2(ii, j)] + by[I2(ii + 1, j)]) * dxm[I1(ii)];
     }
 }
+
+/* hdmlw/mlw_m3.f:2-377: as mlw_m with the third components vz, bz (2D
+ * fields in 3D space); Az evolves by -Ez as in mlw_m */
+void or2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
+                 double *bx, double *by, double *bz, double *az, double dt,
+                 double qav, double gm, const double *dx, const double *dxm,
+                 int ix, const double *dy, const double *dym, int jx) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0; /* :65 */
+    const double pi4i = 1.0 / pi / 4.0;
+    size_t n = (size_t)ix * jx;
+    double *dxi = dalloc(ix), *dxim = dalloc(ix), *ux0 = dalloc(ix), *ux1 = dalloc(ix);
+    double *dyi = dalloc(jx), *dyim = dalloc(jx), *uy0 = dalloc(jx), *uy1 = dalloc(jx);
+    double *ee = dalloc(n), *rx = dalloc(n), *ry = dalloc(n), *rz = dalloc(n);
+    double *ex = dalloc(n), *ey = dalloc(n), *ez = dalloc(n);
+    double *roh = dalloc(n), *eeh = dalloc(n), *rxh = dalloc(n), *ryh = dalloc(n), *rzh = dalloc(n);
+    double *bxh = dalloc(n), *byh = dalloc(n), *bzh = dalloc(n), *azh = dalloc(n);
+    double *prh = dalloc(n), *vxh = dalloc(n), *vyh = dalloc(n), *vzh = dalloc(n);
+    double *exh = dalloc(n), *eyh = dalloc(n), *ezh = dalloc(n);
+    double *dro = dalloc(n), *dee = dalloc(n), *drx = dalloc(n), *dry = dalloc(n), *drz = dalloc(n);
+    double *dbx = dalloc(n), *dby = dalloc(n), *dbz = dalloc(n), *daz = dalloc(n);
+    double *fx = dalloc(n), *fy = dalloc(n), *qx = dalloc(n), *qy = dalloc(n), *ss = dalloc(n);
+
+    for (int i = 1; i <= ix; i++) { dxi[I1(i)] = 1.0 / dx[I1(i)]; dxim[I1(i)] = 1.0 / dxm[I1(i)]; }
+    for (int i = 2; i <= ix - 1; i++) {
+        ux1[I1(i)] = 0.5 * dxm[I1(i - 1)] / dx[I1(i)];
+        ux0[I1(i)] = 0.5 * dxm[I1(i)] / dx[I1(i)];
+    }
+    for (int j = 1; j <= jx; j++) { dyi[I1(j)] = 1.0 / dy[I1(j)]; dyim[I1(j)] = 1.0 / dym[I1(j)]; }
+    for (int j = 2; j <= jx - 1; j++) {
+        uy1[I1(j)] = 0.5 * dym[I1(j - 1)] / dy[I1(j)];
+        uy0[I1(j)] = 0.5 * dym[I1(j)] / dy[I1(j)];
+    }
+    for (size_t p = 0; p < n; p++) {
+        dro[p] = 0.0; dee[p] = 0.0; drx[p] = 0.0; dry[p] = 0.0; drz[p] = 0.0;
+        dbx[p] = 0.0; dby[p] = 0.0; dbz[p] = 0.0; daz[p] = 0.0;
+    }
+    for (size_t p = 0; p < n; p++) { /* :107-115 */
+        double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+        double bb = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+        ee[p] = pr[p] / (gm - 1.0) + 0.5 * ro[p] * vv + pi8i * bb;
+        rx[p] = ro[p] * vx[p];
+        ry[p] = ro[p] * vy[p];
+        rz[p] = ro[p] * vz[p];
+    }
+    for (size_t p = 0; p < n; p++) { /* :117-123 */
+        ex[p] = -vy[p] * bz[p] + vz[p] * by[p];
+        ey[p] = -vz[p] * bx[p] + vx[p] * bz[p];
+        ez[p] = -vx[p] * by[p] + vy[p] * bx[p];
+    }
+
+    /* half step, :129-210 */
+    for (size_t p = 0; p < n; p++) { fx[p] = ro[p] * vx[p]; fy[p] = ro[p] * vy[p]; }
+    mlwhalf2(ro, roh, dro, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) {
+        double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+        double ep = pr[p] * gm / (gm - 1.0) + 0.5 * ro[p] * vv;
+        fx[p] = ep * vx[p] + (bz[p] * ey[p] - by[p] * ez[p]) * pi4i;
+        fy[p] = ep * vy[p] + (bx[p] * ez[p] - bz[p] * ex[p]) * pi4i;
+    }
+    mlwhalf2(ee, eeh, dee, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) {
+        fx[p] = ro[p] * sq(vx[p]) + pr[p] + pi8i * (sq(by[p]) + sq(bz[p]) - sq(bx[p]));
+        fy[p] = ro[p] * vx[p] * vy[p] - pi4i * bx[p] * by[p];
+    }
+    mlwhalf2(rx, rxh, drx, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) {
+        fx[p] = ro[p] * vy[p] * vx[p] - pi4i * by[p] * bx[p];
+        fy[p] = ro[p] * sq(vy[p]) + pr[p] + pi8i * (sq(bx[p]) + sq(bz[p]) - sq(by[p]));
+    }
+    mlwhalf2(ry, ryh, dry, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) {
+        fx[p] = ro[p] * vz[p] * vx[p] - pi4i * bz[p] * bx[p];
+        fy[p] = ro[p] * vz[p] * vy[p] - pi4i * bz[p] * by[p];
+    }
+    mlwhalf2(rz, rzh, drz, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) { fx[p] = 0.0; fy[p] = ez[p]; }
+    mlwhalf2(bx, bxh, dbx, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) { fx[p] = -ez[p]; fy[p] = 0.0; }
+    mlwhalf2(by, byh, dby, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) { fx[p] = ey[p]; fy[p] = -ex[p]; }
+    mlwhalf2(bz, bzh, dbz, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+    for (size_t p = 0; p < n; p++) ss[p] = -ez[p];
+    mlwsrch2(azh, daz, dt, ss, ix, jx);
+
+    /* half-step primitives, :214-229 (the E loop runs over every cell in the
+     * reference; only corners i<ix, j<jx are ever used) */
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            vxh[p] = rxh[p] / roh[p];
+            vyh[p] = ryh[p] / roh[p];
+            vzh[p] = rzh[p] / roh[p];
+            double vv = sq(vxh[p]) + sq(vyh[p]) + sq(vzh[p]);
+            double bb = sq(bxh[p]) + sq(byh[p]) + sq(bzh[p]);
+            prh[p] = (gm - 1.0) * (eeh[p] - 0.5 * roh[p] * vv - pi8i * bb);
+            exh[p] = -vyh[p] * bzh[p] + vzh[p] * byh[p];
+            eyh[p] = -vzh[p] * bxh[p] + vxh[p] * bzh[p];
+            ezh[p] = -vxh[p] * byh[p] + vyh[p] * bxh[p];
+        }
+
+    /* full step, :233-318 */
+#define HLOOP for (int j = 1; j <= jx - 1; j++) for (int i = 1; i <= ix - 1; i++)
+    HLOOP { size_t p = I2(i, j); fx[p] = roh[p] * vxh[p]; fy[p] = roh[p] * vyh[p]; }
+    mlwfull2(dro, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP {
+        size_t p = I2(i, j);
+        double vv = sq(vxh[p]) + sq(vyh[p]) + sq(vzh[p]);
+        double ep = prh[p] * gm / (gm - 1.0) + 0.5 * roh[p] * vv;
+        fx[p] = ep * vxh[p] + (bzh[p] * eyh[p] - byh[p] * ezh[p]) * pi4i;
+        fy[p] = ep * vyh[p] + (bxh[p] * ezh[p] - bzh[p] * exh[p]) * pi4i;
+    }
+    mlwfull2(dee, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP {
+        size_t p = I2(i, j);
+        fx[p] = roh[p] * sq(vxh[p]) + prh[p] + pi8i * (sq(byh[p]) + sq(bzh[p]) - sq(bxh[p]));
+        fy[p] = roh[p] * vxh[p] * vyh[p] - pi4i * bxh[p] * byh[p];
+    }
+    mlwfull2(drx, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP {
+        size_t p = I2(i, j);
+        fx[p] = roh[p] * vyh[p] * vxh[p] - pi4i * byh[p] * bxh[p];
+        fy[p] = roh[p] * sq(vyh[p]) + prh[p] + pi8i * (sq(bxh[p]) + sq(bzh[p]) - sq(byh[p]));
+    }
+    mlwfull2(dry, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP {   /* :277 loops over every cell; mlwfull reads only i<ix, j<jx */
+        size_t p = I2(i, j);
+        fx[p] = roh[p] * vzh[p] * vxh[p] - pi4i * bzh[p] * bxh[p];
+        fy[p] = roh[p] * vzh[p] * vyh[p] - pi4i * bzh[p] * byh[p];
+    }
+    mlwfull2(drz, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP { size_t p = I2(i, j); fx[p] = 0.0; fy[p] = ezh[p]; }
+    mlwfull2(dbx, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP { size_t p = I2(i, j); fx[p] = -ezh[p]; fy[p] = 0.0; }
+    mlwfull2(dby, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP { size_t p = I2(i, j); fx[p] = eyh[p]; fy[p] = -exh[p]; }
+    mlwfull2(dbz, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+    HLOOP { size_t p = I2(i, j); ss[p] = -ezh[p]; }
+#undef HLOOP
+    mlwsrcf2(daz, dt, ss, ux0, ux1, ix, uy0, uy1, jx);
+
+    /* artificial viscosity, :324-345 */
+    const double zero = 0.0;
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++)
+            qx[I2(i, j)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I2(i + 1, j)] - vx[I2(i, j)]) - R4_1EM4);
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix; i++)
+            qy[I2(i, j)] = qav * dym[I1(j)] * fmaxF(zero, fabs(vy[I2(i, j + 1)] - vy[I2(i, j)]) - R4_1EM4);
+    mlwartv2(ro, dro, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(ee, dee, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(rx, drx, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(ry, dry, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(rz, drz, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(bx, dbx, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(by, dby, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+    mlwartv2(bz, dbz, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+
+    /* update and primitive recovery, :349-374 */
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            ro[p] = ro[p] + dro[p];
+            ee[p] = ee[p] + dee[p];
+            rx[p] = rx[p] + drx[p];
+            ry[p] = ry[p] + dry[p];
+            rz[p] = rz[p] + drz[p];
+            bx[p] = bx[p] + dbx[p];
+            by[p] = by[p] + dby[p];
+            bz[p] = bz[p] + dbz[p];
+            az[p] = az[p] + daz[p];
+        }
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            vx[p] = rx[p] / ro[p];
+            vy[p] = ry[p] / ro[p];
+            vz[p] = rz[p] / ro[p];
+            double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+            double bb = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+            pr[p] = (gm - 1.0) * (ee[p] - 0.5 * ro[p] * vv - pi8i * bb);
+        }
+    free(dxi); free(dxim); free(ux0); free(ux1); free(dyi); free(dyim); free(uy0); free(uy1);
+    free(ee); free(rx); free(ry); free(rz); free(ex); free(ey); free(ez);
+    free(roh); free(eeh); free(rxh); free(ryh); free(rzh); free(bxh); free(byh); free(bzh); free(azh);
+    free(prh); free(vxh); free(vyh); free(vzh); free(exh); free(eyh); free(ezh);
+    free(dro); free(dee); free(drx); free(dry); free(drz); free(dbx); free(dby); free(dbz); free(daz);
+    free(fx); free(fy); free(qx); free(qy); free(ss);
+}
+
+/* common/cfl_m3.f:42-78: over every cell, min(dx,dy)/sqrt(v^2+cs^2+ca^2) */
+void or2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm,
+                 const double *ro, const double *pr, const double *vx,
+                 const double *vy, const double *vz, const double *bx,
+                 const double *by, const double *bz, const double *dx, int ix,
+                 const double *dy, int jx, int *imin, int *jmin) {
+    const double pi = acos(-1.0);
+    const double pi4i = 1.0 / pi / 4.0;
+    size_t n = (size_t)ix * jx;
+    double *dtq = dalloc(n);
+    for (int j = 1; j <= jx; j++)
+        for (int i = 1; i <= ix; i++) {
+            size_t p = I2(i, j);
+            double b2 = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+            double ca2 = b2 * pi4i / ro[p];
+            double cs2 = gm * pr[p] / ro[p];
+            double v2 = vx[p] * vx[p] + vy[p] * vy[p] + sq(vz[p]);
+            double dtcfl = fminF(dx[I1(i)], dy[I1(j)]) / sqrt(v2 + cs2 + ca2);
+            dtq[p] = safety * dtcfl;
+        }
+    double d = R4_1E20;
+    int im = 0, jm = 0;
+    for (int j = 1; j <= jx; j++)
+        for (int i = 1; i <= ix; i++)
+            if (dtq[I2(i, j)] < d) { im = i; jm = j; d = dtq[I2(i, j)]; }
+    *merr = (d < dtmin) ? 9001 : 0;
+    *dt = d;
+    if (imin) *imin = im;
+    if (jmin) *jmin = jm;
+    free(dtq);
+}

@@ -107,6 +107,30 @@ class Oracle:
         self.lib.or1d_grdrdy(_arr(dx), _arr(xm), _arr(dxm), _arr(x), C.c_int(ix))
         return dx, xm, dxm
 
+    # ---- three-component variants (mlw_m3, cfl_m3)
+    def mlw_m3_1d(self, ro, pr, vx, vy, vz, by, bz, bx, bxm, dt, qav, gm, dx, dxm):
+        self.lib.or1d_mlw_m3(*map(_arr, (ro, pr, vx, vy, vz, by, bz, bx, bxm)), C.c_double(dt), C.c_double(qav),
+                             C.c_double(gm), _arr(dx), _arr(dxm), C.c_int(ro.shape[0]))
+
+    def cfl_m3_1d(self, safety, dtmin, gm, bx, ro, pr, vx, vy, by, vz, bz, dx):
+        dt, merr, imin = C.c_double(), C.c_int(), C.c_int()
+        self.lib.or1d_cfl_m3(C.byref(dt), C.c_double(safety), C.c_double(dtmin), C.byref(merr), C.c_double(gm),
+                             *map(_arr, (bx, ro, pr, vx, vy, by, vz, bz, dx)), C.c_int(ro.shape[0]), C.byref(imin))
+        return dt.value, merr.value, imin.value
+
+    def mlw_m3_2d(self, ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, dy, dym):
+        ix, jx = ro.shape
+        self.lib.or2d_mlw_m3(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz, az)), C.c_double(dt), C.c_double(qav),
+                             C.c_double(gm), _arr(dx), _arr(dxm), C.c_int(ix), _arr(dy), _arr(dym), C.c_int(jx))
+
+    def cfl_m3_2d(self, safety, dtmin, gm, ro, pr, vx, vy, vz, bx, by, bz, dx, dy):
+        ix, jx = ro.shape
+        dt, merr, im, jm = C.c_double(), C.c_int(), C.c_int(), C.c_int()
+        self.lib.or2d_cfl_m3(C.byref(dt), C.c_double(safety), C.c_double(dtmin), C.byref(merr), C.c_double(gm),
+                             *map(_arr, (ro, pr, vx, vy, vz, bx, by, bz)), _arr(dx), C.c_int(ix), _arr(dy),
+                             C.c_int(jx), C.byref(im), C.byref(jm))
+        return dt.value, merr.value, (im.value, jm.value)
+
     # ---- UTRIP spicule model (cans1d/hdmlw/mlw_m_bg.f, md_spicule/hdmlw)
     GEOM = ("bx", "bxm", "gx", "gxm", "sc", "dsc", "scm", "dscm", "rr", "rrm", "drr", "drrm", "dx", "dxm")
 

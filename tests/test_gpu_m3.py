@@ -1,0 +1,135 @@
+"""The three-component engines on the GPU against the C restatement:
+mlw_m3 and cfl_m3 in 1D and 2D as single calls (ragged and exact tile
+shapes) and as a resident loop (cans_run with CANS_ENGINE_MLW3), bit for
+bit in FP64."""
+import numpy as np
+import pytest
+import torch
+
+from tests.helpers import assert_bitwise, stretched_grid, to_dev, to_host
+from tests.test_oracle_m3 import F1, F2, state1, state2
+from utrip_amd import cans
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+
+
+@pytest.mark.parametrize("ix", [5, 64, 301, 1030])
+def test_mlw_m3_1d_call(oracle, dev, ix):
+    print(f"mlw_m3 1d ix={ix}", flush=True)
+    s = state1(ix, ix)
+    _, dx, dxm = stretched_grid(ix, seed=ix)
+    a = {k: v.copy() for k, v in s.items()}
+    oracle.mlw_m3_1d(*[a[k] for k in F1], a["bx"], a["bxm"], 1.7e-3, 3.0, 5 / 3, dx, dxm)
+    d = {k: T(v, dev) for k, v in s.items()}
+    cans.mlw_m3_1d(*[d[k] for k in F1], d["bx"], d["bxm"], 1.7e-3, 3.0, 5 / 3, T(dx, dev), T(dxm, dev))
+    torch.cuda.synchronize()
+    for k in F1:
+        assert_bitwise(d[k].cpu().numpy(), a[k], k)
+
+
+def test_cfl_m3_1d_call(oracle, dev):
+    ix = 300
+    s = state1(ix, 8)
+    _, dx, _ = stretched_grid(ix)
+    want = oracle.cfl_m3_1d(0.4, 1e-10, 5 / 3, s["bx"], s["ro"], s["pr"], s["vx"], s["vy"], s["by"], s["vz"], s["bz"],
+                            dx)
+    d = {k: T(v, dev) for k, v in s.items()}
+    got = cans.cfl_m3_1d(0.4, 1e-10, 5 / 3, d["bx"], d["ro"], d["pr"], d["vx"], d["vy"], d["by"], d["vz"], d["bz"],
+                         T(dx, dev))
+    assert got == want[:2]
+
+
+@pytest.mark.parametrize("ix,jx", [(5, 6), (33, 9), (40, 33), (97, 64), (130, 71)])
+def test_mlw_m3_2d_call(oracle, dev, ix, jx):
+    print(f"mlw_m3 2d {ix}x{jx}", flush=True)
+    s = state2(ix, jx, ix + jx)
+    _, dx, dxm = stretched_grid(ix, seed=ix)
+    _, dy, dym = stretched_grid(jx, seed=jx + 1)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m3_2d(*[a[k] for k in F2], 1.3e-3, 3.0, 5 / 3, dx, dxm, dy, dym)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m3_2d(*[d[k] for k in F2], 1.3e-3, 3.0, 5 / 3, T(dx, dev), T(dxm, dev), T(dy, dev), T(dym, dev))
+    torch.cuda.synchronize()
+    for k in F2:
+        assert_bitwise(to_host(d[k]), a[k], k)
+
+
+def test_cfl_m3_2d_call(oracle, dev):
+    ix, jx = 70, 45
+    s = state2(ix, jx, 12)
+    _, dx, _ = stretched_grid(ix)
+    _, dy, _ = stretched_grid(jx, seed=4)
+    names = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")
+    want = oracle.cfl_m3_2d(0.4, 1e-10, 5 / 3, *[s[k] for k in names], dx, dy)
+    d = {k: to_dev(s[k], dev) for k in names}
+    got = cans.cfl_m3_2d(0.4, 1e-10, 5 / 3, *[d[k] for k in names], T(dx, dev), T(dy, dev))
+    assert got == want[:2]
+
+
+def periodic2(oracle, f, margin):
+    for dr in (0, 1):
+        for k in F2:
+            oracle.bnd_2d(dr, 7, margin, margin, f[k])
+
+
+def test_mlw_m3_2d_resident_loop(oracle, dev):
+    """40 steps of {cfl_m3; t=t+dt; mlw_m3; bdper x, y} in the resident loop
+    against the restatement's loop (periodic 2.5D MHD with all components)."""
+    ix, jx, margin, nsteps = 66, 40, 1, 40
+    s = state2(ix, jx, 77)
+    _, dx, dxm = stretched_grid(ix, seed=5)
+    _, dy, dym = stretched_grid(jx, seed=6)
+    periodic2(oracle, s, margin)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    t = 0.0
+    names = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")
+    for _ in range(nsteps):
+        dt, merr, _ = oracle.cfl_m3_2d(0.4, 1e-10, 5 / 3, *[a[k] for k in names], dx, dy)
+        assert merr == 0
+        t = t + dt
+        oracle.mlw_m3_2d(*[a[k] for k in F2], dt, 3.0, 5 / 3, dx, dxm, dy, dym)
+        periodic2(oracle, a, margin)
+    st = [to_dev(s[k], dev) for k in F2]
+    grid = {k: T(v, dev) for k, v in dict(dx=dx, dxm=dxm, dy=dy, dym=dym).items()}
+    run = cans.Run(2, cans.ENGINE_MLW3, st, grid, 5 / 3, 0.4, qav=3.0, margin=margin, bc=7)
+    run.reset()
+    run.steps(nsteps)
+    cur, info = run.state()
+    torch.cuda.synchronize()
+    assert info["nexec"] == nsteps and info["merr"] == 0 and info["t"] == t
+    for k, g in zip(F2, cur):
+        assert_bitwise(to_host(g), a[k], k)
+
+
+def test_mlw_m3_1d_resident_loop(oracle, dev):
+    ix, margin, nsteps = 200, 2, 60
+    s = state1(ix, 21)
+    _, dx, dxm = stretched_grid(ix, seed=9)
+    for side in (0, 1):
+        for k in F1:
+            oracle.bnd_1d(0, side, margin, s[k])
+    a = {k: v.copy() for k, v in s.items()}
+    t = 0.0
+    for _ in range(nsteps):
+        dt, merr, _ = oracle.cfl_m3_1d(0.4, 1e-10, 5 / 3, a["bx"], a["ro"], a["pr"], a["vx"], a["vy"], a["by"], a["vz"],
+                                       a["bz"], dx)
+        t = t + dt
+        oracle.mlw_m3_1d(*[a[k] for k in F1], a["bx"], a["bxm"], dt, 3.0, 5 / 3, dx, dxm)
+        for side in (0, 1):
+            for k in F1:
+                oracle.bnd_1d(0, side, margin, a[k])
+    st = [T(s[k], dev) for k in F1]
+    grid = {k: T(v, dev) for k, v in dict(dx=dx, dxm=dxm).items()}
+    run = cans.Run(1, cans.ENGINE_MLW3, st, grid, 5 / 3, 0.4, qav=3.0, margin=margin, bc=0,
+                   bx1d=T(s["bx"], dev), bxm1d=T(s["bxm"], dev))
+    run.reset()
+    run.steps(nsteps)
+    cur, info = run.state()
+    torch.cuda.synchronize()
+    assert info["nexec"] == nsteps and info["t"] == t
+    for k, g in zip(F1, cur):
+        assert_bitwise(g.cpu().numpy(), a[k], k)

@@ -1,0 +1,104 @@
+"""The three-component engine variants (cans1d/hdmlw/mlw_m3.f,
+cans2d/hdmlw/mlw_m3.f, cans{1,2}d/common/cfl_m3.f) restated in C, against
+the reference Fortran compiled from its own sources: bit-identical FP64 on
+smooth random states over stretched grids."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.helpers import assert_bitwise, stretched_grid
+
+F1 = ("ro", "pr", "vx", "vy", "vz", "by", "bz")
+F2 = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz", "az")
+
+
+def state1(ix, seed):
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0, 1, ix)
+    k = rng.uniform(1, 4, 7)
+    s = dict(ro=1.0 + 0.3 * np.sin(2 * np.pi * k[0] * x), pr=0.8 + 0.2 * np.cos(2 * np.pi * k[1] * x),
+             vx=0.3 * np.sin(2 * np.pi * k[2] * x), vy=0.2 * np.cos(2 * np.pi * k[3] * x),
+             vz=0.25 * np.sin(2 * np.pi * k[4] * x + 1.0), by=1.5 * np.sin(2 * np.pi * k[5] * x),
+             bz=0.7 * np.cos(2 * np.pi * k[6] * x))
+    s["bx"] = np.full(ix, 1.1)
+    s["bxm"] = s["bx"].copy()
+    return {q: np.asfortranarray(v) for q, v in s.items()}
+
+
+def state2(ix, jx, seed):
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0, 1, ix)[:, None]
+    y = np.linspace(0, 1, jx)[None, :]
+    k = rng.uniform(1, 3, (9, 2))
+    w = lambda a, b, c: np.sin(2 * np.pi * (k[c, 0] * x + k[c, 1] * y) + a) * b  # noqa: E731
+    s = dict(ro=1.0 + w(0.1, 0.3, 0), pr=0.8 + w(0.5, 0.2, 1), vx=w(0.2, 0.3, 2), vy=w(1.1, 0.2, 3),
+             vz=w(2.1, 0.25, 4), bx=0.9 + w(0.7, 0.4, 5), by=w(0.3, 0.5, 6), bz=w(1.7, 0.6, 7), az=w(0.9, 0.2, 8))
+    return {q: np.asfortranarray(np.broadcast_to(v, (ix, jx)).copy()) for q, v in s.items()}
+
+
+@pytest.mark.parametrize("ix", [5, 64, 301])
+def test_mlw_m3_1d_vs_ref(oracle, ref1d, ix):
+    s = state1(ix, ix)
+    _, dx, dxm = stretched_grid(ix, seed=ix)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m3_1d(*[a[k] for k in F1], a["bx"], a["bxm"], 1.7e-3, 3.0, 5 / 3, dx, dxm)
+    ref1d.call("mlw_m3", *[b[k] for k in F1], b["bx"], b["bxm"], 1.7e-3, 3.0, 5 / 3, dx, dxm, ix)
+    for k in F1:
+        assert_bitwise(a[k], b[k], k)
+
+
+def test_cfl_m3_1d_vs_ref(oracle, ref1d):
+    ix = 77
+    s = state1(ix, 3)
+    _, dx, _ = stretched_grid(ix)
+    d = oracle.cfl_m3_1d(0.4, 1e-10, 5 / 3, s["bx"], s["ro"], s["pr"], s["vx"], s["vy"], s["by"], s["vz"], s["bz"], dx)
+    dt, merr = C.c_double(), C.c_int()
+    ref1d.call("cfl_m3", dt, 0.4, 1e-10, merr, 5 / 3, s["bx"], s["ro"], s["pr"], s["vx"], s["vy"], s["by"], s["vz"],
+               s["bz"], dx, ix)
+    assert d[0] == dt.value and d[1] == merr.value == 0
+
+
+@pytest.mark.parametrize("ix,jx", [(5, 6), (40, 33), (97, 64)])
+def test_mlw_m3_2d_vs_ref(oracle, ref2d, ix, jx):
+    s = state2(ix, jx, ix + jx)
+    _, dx, dxm = stretched_grid(ix, seed=ix)
+    _, dy, dym = stretched_grid(jx, seed=jx + 1)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m3_2d(*[a[k] for k in F2], 1.3e-3, 3.0, 5 / 3, dx, dxm, dy, dym)
+    ref2d.call("mlw_m3", *[b[k] for k in F2], 1.3e-3, 3.0, 5 / 3, dx, dxm, ix, dy, dym, jx)
+    for k in F2:
+        assert_bitwise(a[k], b[k], k)
+
+
+def test_cfl_m3_2d_vs_ref(oracle, ref2d):
+    ix, jx = 30, 22
+    s = state2(ix, jx, 9)
+    _, dx, _ = stretched_grid(ix)
+    _, dy, _ = stretched_grid(jx, seed=4)
+    d = oracle.cfl_m3_2d(0.4, 1e-10, 5 / 3, *[s[k] for k in ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")], dx, dy)
+    dt, merr = C.c_double(), C.c_int()
+    ref2d.call("cfl_m3", dt, 0.4, 1e-10, merr, 5 / 3, *[s[k] for k in ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")],
+               dx, ix, dy, jx)
+    assert d[0] == dt.value and d[1] == merr.value == 0
+
+
+def test_mlw_m3_2d_reduces_to_mlw_m(oracle):
+    """With vz = bz = 0 the 2.5D engine is the 2D one: the same formulas
+    with zero third components (x + 0 = x and 0*y = 0 exactly for finite
+    y), so the two restatements agree bit for bit."""
+    ix, jx = 33, 29
+    s = state2(ix, jx, 5)
+    s["vz"][:] = 0.0
+    s["bz"][:] = 0.0
+    _, dx, dxm = stretched_grid(ix, seed=2)
+    _, dy, dym = stretched_grid(jx, seed=3)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m3_2d(*[a[k] for k in F2], 1.3e-3, 3.0, 5 / 3, dx, dxm, dy, dym)
+    oracle.mlw_m_2d(*[b[k] for k in ("ro", "pr", "vx", "vy", "bx", "by", "az")], 1.3e-3, 3.0, 5 / 3, dx, dxm, dy, dym)
+    for k in ("ro", "pr", "vx", "vy", "bx", "by", "az"):
+        assert_bitwise(a[k][1:-1, 1:-1], b[k][1:-1, 1:-1], k)
+    assert np.all(a["vz"] == 0.0) and np.all(a["bz"] == 0.0)

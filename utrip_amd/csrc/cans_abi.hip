@@ -160,6 +160,16 @@ int cans1d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
     return cfl_common(a, dt, dtmin, merr, (hipStream_t)stream);
 }
 
+int cans1d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *bx,
+                  const double *ro, const double *pr, const double *vx, const double *vy, const double *by,
+                  const double *vz, const double *bz, const double *dx, int ix, void *stream) {
+    if (ix < 3) return fail(CANS_ERR_ARG, "cans1d_cfl_m3: ix=%d < 3", ix);
+    CflArgs a{};
+    a.ro = ro; a.pr = pr; a.vx = vx; a.vy = vy; a.vz = vz; a.bx = bx; a.by = by; a.bz = bz; a.dx = dx;
+    a.safety = safety; a.gm = gm; a.ix = ix; a.jx = 1; a.kx = 1; a.dim = 1; a.comp3 = 1;
+    return cfl_common(a, dt, dtmin, merr, (hipStream_t)stream);
+}
+
 static int bnd_common(int dir, int kind, int mbnd, int margin, double *qq, double q0,
                       const double *dm, int ix, int jx, int kx, void *stream) {
     const int n = dir == 0 ? ix : (dir == 1 ? jx : kx);
@@ -224,6 +234,16 @@ int cans2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
     CflArgs a{};
     a.ro = ro; a.pr = pr; a.vx = vx; a.vy = vy; a.bx = bx; a.by = by; a.dx = dx; a.dy = dy;
     a.safety = safety; a.gm = gm; a.ix = ix; a.jx = jx; a.kx = 1; a.dim = 2;
+    return cfl_common(a, dt, dtmin, merr, (hipStream_t)stream);
+}
+
+int cans2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *ro,
+                  const double *pr, const double *vx, const double *vy, const double *vz, const double *bx,
+                  const double *by, const double *bz, const double *dx, int ix, const double *dy, int jx,
+                  void *stream) {
+    CflArgs a{};
+    a.ro = ro; a.pr = pr; a.vx = vx; a.vy = vy; a.vz = vz; a.bx = bx; a.by = by; a.bz = bz; a.dx = dx; a.dy = dy;
+    a.safety = safety; a.gm = gm; a.ix = ix; a.jx = jx; a.kx = 1; a.dim = 2; a.comp3 = 1;
     return cfl_common(a, dt, dtmin, merr, (hipStream_t)stream);
 }
 

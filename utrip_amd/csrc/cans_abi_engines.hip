@@ -174,6 +174,49 @@ int cans3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz, dou
     return 0;
 }
 
+// ====================================================================== mlw_m3 drop-in
+static int mlw3_dropin(int dim, double *const *f, const double *bx, const double *bxm, double dt, double qav,
+                       double gm, const double *dx, const double *dxm, int ix, const double *dy, const double *dym,
+                       int jx, void *stream) {
+    const int nf = dim == 1 ? 7 : 9;
+    if (ix < 3 || (dim == 2 && jx < 3)) return abi_fail(CANS_ERR_ARG, "mlw_m3: %d x %d", ix, jx);
+    for (int q = 0; q < nf; q++)
+        if (!f[q]) return abi_fail(CANS_ERR_ARG, "mlw_m3: NULL field %d", q);
+    if (dim == 1 && (!bx || !bxm)) return abi_fail(CANS_ERR_ARG, "mlw_m3: NULL bx/bxm");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t N = (size_t)ix * (dim == 2 ? jx : 1);
+    void *w = nullptr;
+    if (int rc = abi_scratch((metric_doubles(ix, jx, 1) + nf * al(N)) * sizeof(double), &w)) return rc;
+    Carver c{(double *)w};
+    Metric m[3];
+    carve_metric(c, ix, jx, 1, m);
+    HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
+    if (dim == 2) HIPCHK(launch_metric(dy, dym, jx, m[1], s));
+    Mlw3Args a{};
+    for (int q = 0; q < nf; q++) { a.in[q] = f[q]; a.out[q] = c.take(N); }
+    a.bx = bx; a.bxm = bxm; a.dxm = dxm; a.dym = dym; a.mx = m[0]; a.my = m[1];
+    a.qav = qav; a.gm = gm; a.dt = dt; a.clk = nullptr; a.ix = ix; a.jx = dim == 2 ? jx : 1;
+    HIPCHK(launch_mlw_m3(a, dim, s));
+    const double *src[9];
+    for (int q = 0; q < nf; q++) src[q] = a.out[q];
+    HIPCHK(launch_copy_interior(f, src, nf, ix, dim == 2 ? jx : 1, 1, 2, s));
+    return 0;
+}
+
+int cans1d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *by, double *bz,
+                  const double *bx, const double *bxm, double dt, double qav, double gm, const double *dx,
+                  const double *dxm, int ix, void *stream) {
+    double *f[7] = {ro, pr, vx, vy, vz, by, bz};
+    return mlw3_dropin(1, f, bx, bxm, dt, qav, gm, dx, dxm, ix, nullptr, nullptr, 1, stream);
+}
+
+int cans2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by, double *bz,
+                  double *az, double dt, double qav, double gm, const double *dx, const double *dxm, int ix,
+                  const double *dy, const double *dym, int jx, void *stream) {
+    double *f[9] = {ro, pr, vx, vy, vz, bx, by, bz, az};
+    return mlw3_dropin(2, f, nullptr, nullptr, dt, qav, gm, dx, dxm, ix, dy, dym, jx, stream);
+}
+
 // ====================================================================== HLLD drop-in
 static int hlld_dropin(bool mixed, double *const *prim, double gm, double mu, double dt, const double *dx,
                        const double *dy, const double *dz, int ix, int jx, int kx, const int *mfdim, void *stream) {
@@ -219,7 +262,8 @@ size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx) {
 // One resident run of `nsteps` CANS loop iterations {cfl_m; t=t+dt; engine;
 // bnd} for any dimension/engine with generic boundaries.  State set a holds
 // step n (n even), b step n+1, alternating.  nf fields per set:
-// 1D 5 (ro pr vx vy by), 2D 7 (ro pr vx vy bx by az), 3D 8 (ro pr vx vy vz bx by bz).
+// 1D 5 (ro pr vx vy by), 2D 7 (ro pr vx vy bx by az), 3D 8 (ro pr vx vy vz bx by bz);
+// mlw_m3: 1D 7 (ro pr vx vy vz by bz), 2D 9 (ro pr vx vy vz bx by bz az).
 int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
              const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
              double safety, const double *dx, const double *dxm, int ix, const double *dy, const double *dym,
@@ -228,14 +272,17 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     if (dim < 1 || dim > 3) return abi_fail(CANS_ERR_ARG, "cans_run: dim %d", dim);
     if (!a || !b || !clk || !ws) return abi_fail(CANS_ERR_ARG, "cans_run: NULL argument");
     if (engine == CANS_ENGINE_HLLD && dim != 3) return abi_fail(CANS_ERR_ARG, "cans_run: HLLD is 3D");
+    if (engine == CANS_ENGINE_MLW3 && dim == 3) return abi_fail(CANS_ERR_ARG, "cans_run: mlw_m3 is 1D/2D");
+    if (engine < CANS_ENGINE_MLW || engine > CANS_ENGINE_MLW3) return abi_fail(CANS_ERR_ARG, "cans_run: engine %d", engine);
     if (engine == CANS_ENGINE_MLW && dim == 2 && !bc)
         return cans2d_mlw_run(a, b, clk, step0, nsteps, qav, gm, safety, dx, dxm, ix, dy, dym, jx, nullptr, margin,
                               (double *)ws, events, stream);
     if (!bc) return abi_fail(CANS_ERR_ARG, "cans_run: bc is NULL (only 2D MLW has the fused periodic path)");
     if (dim == 1) { jx = 1; kx = 1; }
     if (dim == 2) kx = 1;
-    const int nf = dim == 1 ? 5 : (dim == 2 ? 7 : 8);
-    const int lo = engine == CANS_ENGINE_MLW ? 3 : 5;
+    const bool m3 = engine == CANS_ENGINE_MLW3;
+    const int nf = m3 ? (dim == 1 ? 7 : 9) : (dim == 1 ? 5 : (dim == 2 ? 7 : 8));
+    const int lo = (engine == CANS_ENGINE_MLW || m3) ? 3 : 5;
     const int n[3] = {ix, jx, kx};
     for (int d = 0; d < dim; d++)
         if (n[d] < lo || n[d] < 2 * margin + 1) return abi_fail(CANS_ERR_ARG, "cans_run: dimension %d too small", n[d]);
@@ -245,7 +292,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     Carver cv{(double *)ws};
     Metric m[3];
     carve_metric(cv, ix, jx, kx, m);
-    if (engine == CANS_ENGINE_MLW && step0 == 0) {   // the first call of a run fills the metric
+    if ((engine == CANS_ENGINE_MLW || m3) && step0 == 0) {   // the first call of a run fills the metric
         HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
         if (dim >= 2) HIPCHK(launch_metric(dy, dym, jx, m[1], s));
         if (dim == 3) HIPCHK(launch_metric(dz, dzm, kx, m[2], s));
@@ -256,7 +303,14 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         double *const *out = (st & 1) ? a : b;
         // map the field list onto the engine slots (ro pr vx vy vz bx by bz)
         double *pin[8] = {nullptr}, *pout[8] = {nullptr};
-        if (dim == 1) {
+        if (m3) {   // ro pr vx vy vz (by bz) or (bx by bz), az
+            const int map1[7] = {0, 1, 2, 3, 4, 6, 7};
+            for (int f = 0; f < (dim == 1 ? 7 : 8); f++) {
+                const int slot = dim == 1 ? map1[f] : f;
+                pin[slot] = in[f];
+                pout[slot] = out[f];
+            }
+        } else if (dim == 1) {
             const int map[5] = {0, 1, 2, 3, 6};
             for (int f = 0; f < 5; f++) { pin[map[f]] = in[f]; pout[map[f]] = out[f]; }
         } else if (dim == 2) {
@@ -271,12 +325,18 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         ca.ro = pin[0]; ca.pr = pin[1]; ca.vx = pin[2]; ca.vy = pin[3]; ca.vz = pin[4];
         ca.bx = dim == 1 ? bx1d : pin[5]; ca.by = pin[6]; ca.bz = pin[7];
         ca.dx = dx; ca.dy = dy; ca.dz = dz; ca.safety = safety; ca.gm = gm;
-        ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st;
+        ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st; ca.comp3 = m3 ? 1 : 0;
         HIPCHK(launch_cfl(ca, s));
         HIPCHK(launch_clock_step(c, st, s));
         const int e = st - step0;
         if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e], s));
-        if (engine == CANS_ENGINE_MLW) {
+        if (m3) {
+            Mlw3Args g{};
+            for (int f = 0; f < nf; f++) { g.in[f] = in[f]; g.out[f] = out[f]; }
+            g.bx = bx1d; g.bxm = bxm1d; g.dxm = dxm; g.dym = dym; g.mx = m[0]; g.my = m[1];
+            g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx;
+            HIPCHK(launch_mlw_m3(g, dim, s));
+        } else if (engine == CANS_ENGINE_MLW) {
             if (dim == 1) {
                 Mlw1dArgs g{};
                 g.ro = pin[0]; g.pr = pin[1]; g.vx = pin[2]; g.vy = pin[3]; g.by = pin[6];

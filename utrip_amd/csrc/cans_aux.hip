@@ -98,6 +98,7 @@ __global__ void bnd_kernel(BndArgs a) {
 //     |vx| + sqrt(cs2+ca2), pi4i = 0.25/pi, onero = 1.0/ro.
 // 2D: cans2d/common/cfl_m.f:44-62, all cells, min(dx,dy)/sqrt(v2+cs2+ca2),
 //     pi4i = 1./pi/4.
+// cfl_m3 (comp3): the 1D/2D formulas with bz in b2 and (2D) vz in v2.
 // 3D: cans3d/common/cfl_m.f:39-58, all cells, min(dx,dy,dz)/sqrt(...),
 //     onep4 = 0.25/pai with pai the REAL*4 acos(-1.0).
 CANS_DEV double cfl_cell(const CflArgs &a, int i, int j, int k, size_t p) {
@@ -105,16 +106,20 @@ CANS_DEV double cfl_cell(const CflArgs &a, int i, int j, int k, size_t p) {
     if (a.dim == 1) {
         const double pi4i = 0.25 / kPi;
         const double onero = 1.0 / a.ro[p];
-        const double ca2 = (a.bx[p] * a.bx[p] + a.by[p] * a.by[p]) * pi4i * onero;
+        double b2 = a.bx[p] * a.bx[p] + a.by[p] * a.by[p];
+        if (a.comp3) b2 = b2 + a.bz[p] * a.bz[p];           // cans1d/common/cfl_m3.f:49
+        const double ca2 = b2 * pi4i * onero;
         const double cs2 = gm * a.pr[p] * onero;
         const double dtcfl = a.dx[i - 1] / (__builtin_fabs(a.vx[p]) + __builtin_sqrt(cs2 + ca2));
         return a.safety * dtcfl;
     } else if (a.dim == 2) {
         const double pi4i = 1.0 / kPi / 4.0;
-        const double b2 = a.bx[p] * a.bx[p] + a.by[p] * a.by[p];
+        double b2 = a.bx[p] * a.bx[p] + a.by[p] * a.by[p];
+        if (a.comp3) b2 = b2 + a.bz[p] * a.bz[p];           // cans2d/common/cfl_m3.f:50-54
         const double ca2 = b2 * pi4i / a.ro[p];
         const double cs2 = gm * a.pr[p] / a.ro[p];
-        const double v2 = a.vx[p] * a.vx[p] + a.vy[p] * a.vy[p];
+        double v2 = a.vx[p] * a.vx[p] + a.vy[p] * a.vy[p];
+        if (a.comp3) v2 = v2 + a.vz[p] * a.vz[p];
         const double dtcfl = fminF(a.dx[i - 1], a.dy[j - 1]) / __builtin_sqrt(v2 + cs2 + ca2);
         return a.safety * dtcfl;
     } else {

@@ -101,6 +101,7 @@ struct CflArgs {
     double safety, gm;
     int ix, jx, kx;
     int dim;                    // 1, 2 or 3 (selects the reference's formula)
+    int comp3;                  // 1: cfl_m3 of 1D/2D (adds vz, bz)
     unsigned long long *key;    // order-preserving key of the min (pre-armed)
     Clock *clk;                 // non-null: reduce into clk->dtkey[step%3] if the step runs
     int step;
@@ -125,6 +126,22 @@ struct Batch1dArgs {
     unsigned *next;              // queue counter (zeroed by the launch)
 };
 hipError_t launch_batch1d(const Batch1dArgs &a, int mixed, int block_threads, int nblocks, hipStream_t s);
+
+// mlw_m3 (cans{1,2}d/hdmlw/mlw_m3.f): 1D fields ro pr vx vy vz by bz (bx,
+// bxm constant); 2D fields ro pr vx vy vz bx by bz az.  in -> out on the
+// interior cells (2..n-1).
+struct Mlw3Args {
+    const double *in[9];
+    double *out[9];
+    const double *bx, *bxm;          // 1D
+    const double *dxm, *dym;
+    Metric mx, my;
+    double qav, gm, dt;
+    Clock *clk;                      // non-null: dt and run flag of step `step` (after its clock kernel)
+    int step;
+    int ix, jx;
+};
+hipError_t launch_mlw_m3(const Mlw3Args &a, int dim, hipStream_t s);
 
 // UTRIP spicule (cans1d/md_spicule/hdmlw): geometry arrays of mlw_m_bg, in
 // this order.

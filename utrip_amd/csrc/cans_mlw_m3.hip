@@ -1,0 +1,388 @@
+// cans_mlw_m3.hip -- mlw_m3, the modified Lax-Wendroff engine for MHD with
+// all three vector components (cans1d/hdmlw/mlw_m3.f:2-252,
+// cans2d/hdmlw/mlw_m3.f:2-377), on gfx950, FP64 bit-identical to the
+// reference compiled without FMA contraction.
+//
+// 1D: a wave updates a segment of 62 cells (lane l holds cell c0-1+l; the
+//     face between a cell and its east neighbour belongs to the cell's lane;
+//     neighbours travel by DPP shifts), as the other 1D kernels.
+// 2D: one workgroup of 256 threads updates a tile of 31 x 7 cells in one
+//     pass through LDS: (1) the 33 x 9 cells of the tile and its halo load
+//     their state and leave conserved variables, fluxes and velocities in
+//     LDS; (2) each thread forms one of the 32 x 8 corner half steps
+//     (mlwhalf.f) and its corner fluxes, while the interior threads take the
+//     centred half-step term of their cell; (3) the interior threads apply
+//     the full step (mlwfull.f), the Az source (mlwsrch.f, mlwsrcf.f), the
+//     viscosity (mlwartv.f) and recover the primitives.  Corner values
+//     overwrite the cell fluxes in LDS once the last reader is past a
+//     barrier, so a tile needs 54.6 KB.
+#include "cans_clock.hpp"
+#include "cans_common.hpp"
+#include "cans_kernels.hpp"
+#include "cans_wave.hpp"
+
+namespace cans {
+namespace {
+
+constexpr double pi8i = 1.0 / kPi / 8.0;   // mlw_m3.f: pi8i=1./pi/8.
+constexpr double pi4i = 1.0 / kPi / 4.0;
+
+CANS_DEV bool m3_step(const Mlw3Args &a, double &dt) {
+    if (!a.clk) { dt = a.dt; return true; }
+    const StepCtl c = clock_peek(a.clk, a.step);
+    dt = c.dt;
+    return c.run;
+}
+
+// ---------------------------------------------------------------- 1D
+// fields: ro pr vx vy vz by bz
+__global__ __launch_bounds__(256) void mlw1d_m3_kernel(Mlw3Args a) {
+    double dt;
+    if (!m3_step(a, dt)) return;
+    const int lane = threadIdx.x & 63;
+    const int seg = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int ix = a.ix;
+    const int c = 2 + 62 * seg - 1 + lane;                    // this lane's cell, 1-based
+    const int p = (c < 1 ? 1 : (c > ix ? ix : c)) - 1;
+    const double gm = a.gm;
+    const double ro = a.in[0][p], pr = a.in[1][p], vx = a.in[2][p], vy = a.in[3][p], vz = a.in[4][p];
+    const double by = a.in[5][p], bz = a.in[6][p], bx = a.bx[p];
+    // cells (mlw_m3.f:83-143)
+    double u[7], f[7];
+    {
+        const double vv = vx * vx + vy * vy + vz * vz;
+        const double bb = by * by + bz * bz;
+        u[0] = ro;
+        u[1] = pr / (gm - 1.0) + 0.5 * ro * vv + pi8i * bb;
+        u[2] = ro * vx;
+        u[3] = ro * vy;
+        u[4] = ro * vz;
+        u[5] = by;
+        u[6] = bz;
+        const double ey = -vz * bx + vx * bz;
+        const double ez = -vx * by + vy * bx;
+        const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
+        f[0] = ro * vx;
+        f[1] = ep * vx - by * ez * pi4i + bz * ey * pi4i;
+        f[2] = ro * (vx * vx) + pr + bb * pi8i;
+        f[3] = ro * vx * vy - bx * by * pi4i;
+        f[4] = ro * vx * vz - bx * bz * pi4i;
+        f[5] = -ez;
+        f[6] = ey;
+    }
+    // the face to the east: half step and face fluxes (mlw_m3.f:147-207)
+    const double dxim = a.mx.dim[p];
+    double uE[7], fE[7], ff[7];
+    {
+        double un[7];
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            uE[q] = shl1(u[q]);
+            fE[q] = shl1(f[q]);
+            const double uh = 0.5 * (uE[q] + u[q]);
+            const double dfdx = dxim * (fE[q] - f[q]);
+            un[q] = uh - dt * dfdx;
+        }
+        const double bxm = a.bxm[p];
+        const double roh = un[0], eeh = un[1], byh = un[5], bzh = un[6];
+        const double vxh = un[2] / roh;
+        const double vyh = un[3] / roh;
+        const double vzh = un[4] / roh;
+        const double vv = vxh * vxh + vyh * vyh + vzh * vzh;
+        const double bb = byh * byh + bzh * bzh;
+        const double prh = (gm - 1.0) * (eeh - 0.5 * roh * vv - pi8i * bb);
+        const double eyh = -vzh * bxm + vxh * bzh;
+        const double ezh = -vxh * byh + vyh * bxm;
+        const double eph = prh * gm / (gm - 1.0) + 0.5 * roh * vv;
+        ff[0] = roh * vxh;
+        ff[1] = eph * vxh - byh * ezh * pi4i + bzh * eyh * pi4i;
+        ff[2] = roh * (vxh * vxh) + prh + bb * pi8i;
+        ff[3] = roh * vxh * vyh - bxm * byh * pi4i;
+        ff[4] = roh * vxh * vzh - bxm * bzh * pi4i;
+        ff[5] = -ezh;
+        ff[6] = eyh;
+    }
+    // own cell: centred half-step term, full step, viscosity, update
+    const double dxi = a.mx.di[p];
+    const double zero = 0.0;
+    const double qxc = a.qav * a.dxm[p] * fmaxF(zero, __builtin_fabs(shl1(vx) - vx) - kR4_1em4);
+    const double qxw = shr1(qxc);
+    const double dxim1 = shr1(dxim);
+    double nu[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        double du = 0.0;
+        du = du - 0.5 * dt * (0.5 * dxi * (fE[q] - shr1(f[q])));          // mlwhalf.f:33-35
+        const double dfdx = dxi * (ff[q] - shr1(ff[q]));                   // mlwfull.f:30-33
+        du = du - 0.5 * dt * dfdx;
+        const double dEc = uE[q] - u[q];                                  // mlwartv.f:31-34
+        const double dcW = shr1(dEc);
+        du = du + dt * (dxi * (qxc * dxim * dEc - qxw * dxim1 * dcW));
+        nu[q] = u[q] + du;
+    }
+    if (lane >= 1 && lane <= 62 && c <= ix - 1) {                         // mlw_m3.f:229-249
+        const double nro = nu[0];
+        const double nvx = nu[2] / nro;
+        const double nvy = nu[3] / nro;
+        const double nvz = nu[4] / nro;
+        const double vv = nvx * nvx + nvy * nvy + nvz * nvz;
+        const double bb = nu[5] * nu[5] + nu[6] * nu[6];
+        a.out[0][p] = nro;
+        a.out[1][p] = (gm - 1.0) * (nu[1] - 0.5 * nro * vv - pi8i * bb);
+        a.out[2][p] = nvx;
+        a.out[3][p] = nvy;
+        a.out[4][p] = nvz;
+        a.out[5][p] = nu[5];
+        a.out[6][p] = nu[6];
+    }
+}
+
+// ---------------------------------------------------------------- 2D
+// fields: ro pr vx vy vz bx by bz az
+constexpr int TX = 31, TY = 7;               // interior cells of a tile
+constexpr int CX = TX + 2, CY = TY + 2;      // cells with the halo (33 x 9)
+constexpr int NC = CX * CY;                  // 297
+constexpr int KX = TX + 1;                   // corners per tile row (32 x 8 = 256)
+// LDS rows of the flux block: fx of u0..u4 and u7, fy of the same, ez.
+// fx(bx) = 0, fy(by) = 0, fy(bx) = ez, fx(by) = -ez (mlw_m3.f:178-201).
+enum { FX0 = 0, FX7 = 5, FY0 = 6, FY7 = 11, FEZ = 12, NF = 13 };
+
+// mlw_m3.f:107-202 for one cell (cell values and the half-step fluxes)
+CANS_DEV void cell_values(double ro, double pr, double vx, double vy, double vz, double bx, double by, double bz,
+                          double gm, double *u, double *fxo, double *fyo, double &ezo) {
+    const double vv = vx * vx + vy * vy + vz * vz;
+    const double bb = bx * bx + by * by + bz * bz;
+    u[0] = ro;
+    u[1] = pr / (gm - 1.0) + 0.5 * ro * vv + pi8i * bb;
+    u[2] = ro * vx;
+    u[3] = ro * vy;
+    u[4] = ro * vz;
+    u[5] = bx;
+    u[6] = by;
+    u[7] = bz;
+    const double ex = -vy * bz + vz * by;
+    const double ey = -vz * bx + vx * bz;
+    const double ez = -vx * by + vy * bx;
+    const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
+    fxo[0] = ro * vx;
+    fyo[0] = ro * vy;
+    fxo[1] = ep * vx + (bz * ey - by * ez) * pi4i;
+    fyo[1] = ep * vy + (bx * ez - bz * ex) * pi4i;
+    fxo[2] = ro * (vx * vx) + pr + pi8i * (by * by + bz * bz - bx * bx);
+    fyo[2] = ro * vx * vy - pi4i * bx * by;
+    fxo[3] = ro * vy * vx - pi4i * by * bx;
+    fyo[3] = ro * (vy * vy) + pr + pi8i * (bx * bx + bz * bz - by * by);
+    fxo[4] = ro * vz * vx - pi4i * bz * bx;
+    fyo[4] = ro * vz * vy - pi4i * bz * by;
+    fxo[5] = ey;      // slot 5 carries the bz fluxes (u index 7)
+    fyo[5] = -ex;
+    ezo = ez;
+}
+
+__global__ __launch_bounds__(256) void mlw2d_m3_kernel(Mlw3Args a) {
+    double dt;
+    if (!m3_step(a, dt)) return;
+    __shared__ double sU[8][NC];
+    __shared__ double sF[NF][NC];
+    __shared__ double sV[2][NC];
+    const int ix = a.ix, jx = a.jx;
+    const int ntx = (ix - 2 + TX - 1) / TX;
+    const int nty = (jx - 2 + TY - 1) / TY;
+    const int b = xcd_remap((int)blockIdx.x, ntx * nty);
+    const int i0 = 2 + TX * (b % ntx), j0 = 2 + TY * (b / ntx);   // first interior cell of the tile
+    const int t = threadIdx.x;
+    const double gm = a.gm;
+    // (1) cells of the tile and its halo
+    for (int lc = t; lc < NC; lc += 256) {
+        const int li = lc % CX, lj = lc / CX;
+        int i = i0 - 1 + li, j = j0 - 1 + lj;
+        i = i > ix ? ix : i;
+        j = j > jx ? jx : j;
+        const size_t p = (size_t)(i - 1) + (size_t)ix * (size_t)(j - 1);
+        double u[8], fx[6], fy[6], ez;
+        const double vx = a.in[2][p], vy = a.in[3][p];
+        cell_values(a.in[0][p], a.in[1][p], vx, vy, a.in[4][p], a.in[5][p], a.in[6][p], a.in[7][p], gm, u, fx, fy,
+                    ez);
+#pragma unroll
+        for (int q = 0; q < 8; q++) sU[q][lc] = u[q];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            sF[FX0 + q][lc] = fx[q];
+            sF[FY0 + q][lc] = fy[q];
+        }
+        sF[FEZ][lc] = ez;
+        sV[0][lc] = vx;
+        sV[1][lc] = vy;
+    }
+    __syncthreads();
+    // (2a) centred half-step term of the interior cell (mlwhalf.f, du part)
+    const bool own = t < TX * TY;
+    const int li = 1 + t % TX, lj = 1 + t / TX;
+    const int ci = i0 - 1 + li, cj = j0 - 1 + lj;                 // this thread's cell (1-based)
+    const int cic = ci > ix ? ix : ci, cjc = cj > jx ? jx : cj;
+    const int lc = lj * CX + li;
+    double du[8], daz = 0.0;
+    if (own) {
+        const double dxi = a.mx.di[cic - 1], dyi = a.my.di[cjc - 1];
+        // slot of u index q in the flux block, and the ez-derived fluxes
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            double fxE, fxW, fyN, fyS;
+            if (q < 5) {
+                fxE = sF[FX0 + q][lc + 1]; fxW = sF[FX0 + q][lc - 1];
+                fyN = sF[FY0 + q][lc + CX]; fyS = sF[FY0 + q][lc - CX];
+            } else if (q == 5) {      // bx: fx = 0, fy = ez
+                fxE = 0.0; fxW = 0.0;
+                fyN = sF[FEZ][lc + CX]; fyS = sF[FEZ][lc - CX];
+            } else if (q == 6) {      // by: fx = -ez, fy = 0
+                fxE = -sF[FEZ][lc + 1]; fxW = -sF[FEZ][lc - 1];
+                fyN = 0.0; fyS = 0.0;
+            } else {                  // bz: fx = ey, fy = -ex
+                fxE = sF[FX7][lc + 1]; fxW = sF[FX7][lc - 1];
+                fyN = sF[FY7][lc + CX]; fyS = sF[FY7][lc - CX];
+            }
+            du[q] = 0.0;
+            du[q] = du[q] - 0.5 * dt * (0.5 * dxi * (fxE - fxW)) - 0.5 * dt * (0.5 * dyi * (fyN - fyS));
+        }
+        daz = daz + 0.5 * dt * (-sF[FEZ][lc]);                      // mlwsrch.f, du part, s = -ez
+    }
+    // (2b) corner (i, j) between cells i, i+1 and j, j+1 (mlwhalf.f, un part)
+    double cr[NF];
+    {
+        const int ki = t % KX, kj = t / KX;                         // 0..31, 0..7
+        int i = i0 - 1 + ki, j = j0 - 1 + kj;
+        i = i > ix ? ix : i;
+        j = j > jx ? jx : j;
+        const double dxim = a.mx.dim[i - 1], dyim = a.my.dim[j - 1];
+        const int c00 = kj * CX + ki, c10 = c00 + 1, c01 = c00 + CX, c11 = c01 + 1;
+        double un[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            double x00, x10, x01, x11, y00, y10, y01, y11;
+            if (q < 5) {
+                x00 = sF[FX0 + q][c00]; x10 = sF[FX0 + q][c10]; x01 = sF[FX0 + q][c01]; x11 = sF[FX0 + q][c11];
+                y00 = sF[FY0 + q][c00]; y10 = sF[FY0 + q][c10]; y01 = sF[FY0 + q][c01]; y11 = sF[FY0 + q][c11];
+            } else if (q == 5) {
+                x00 = 0.0; x10 = 0.0; x01 = 0.0; x11 = 0.0;
+                y00 = sF[FEZ][c00]; y10 = sF[FEZ][c10]; y01 = sF[FEZ][c01]; y11 = sF[FEZ][c11];
+            } else if (q == 6) {
+                x00 = -sF[FEZ][c00]; x10 = -sF[FEZ][c10]; x01 = -sF[FEZ][c01]; x11 = -sF[FEZ][c11];
+                y00 = 0.0; y10 = 0.0; y01 = 0.0; y11 = 0.0;
+            } else {
+                x00 = sF[FX7][c00]; x10 = sF[FX7][c10]; x01 = sF[FX7][c01]; x11 = sF[FX7][c11];
+                y00 = sF[FY7][c00]; y10 = sF[FY7][c10]; y01 = sF[FY7][c01]; y11 = sF[FY7][c11];
+            }
+            const double uh = 0.25 * (sU[q][c10] + sU[q][c00] + sU[q][c11] + sU[q][c01]);
+            const double dfdx = dxim * (x10 - x00 + x11 - x01) / 2.0;
+            const double dfdy = dyim * (y01 - y00 + y11 - y10) / 2.0;
+            un[q] = uh - dt * dfdx - dt * dfdy;
+        }
+        // half-step primitives, E and corner fluxes (mlw_m3.f:214-310)
+        const double roh = un[0], eeh = un[1], bxh = un[5], byh = un[6], bzh = un[7];
+        const double vxh = un[2] / roh;
+        const double vyh = un[3] / roh;
+        const double vzh = un[4] / roh;
+        const double vv = vxh * vxh + vyh * vyh + vzh * vzh;
+        const double bb = bxh * bxh + byh * byh + bzh * bzh;
+        const double prh = (gm - 1.0) * (eeh - 0.5 * roh * vv - pi8i * bb);
+        const double exh = -vyh * bzh + vzh * byh;
+        const double eyh = -vzh * bxh + vxh * bzh;
+        const double ezh = -vxh * byh + vyh * bxh;
+        const double ep = prh * gm / (gm - 1.0) + 0.5 * roh * vv;
+        cr[FX0 + 0] = roh * vxh;
+        cr[FY0 + 0] = roh * vyh;
+        cr[FX0 + 1] = ep * vxh + (bzh * eyh - byh * ezh) * pi4i;
+        cr[FY0 + 1] = ep * vyh + (bxh * ezh - bzh * exh) * pi4i;
+        cr[FX0 + 2] = roh * (vxh * vxh) + prh + pi8i * (byh * byh + bzh * bzh - bxh * bxh);
+        cr[FY0 + 2] = roh * vxh * vyh - pi4i * bxh * byh;
+        cr[FX0 + 3] = roh * vyh * vxh - pi4i * byh * bxh;
+        cr[FY0 + 3] = roh * (vyh * vyh) + prh + pi8i * (bxh * bxh + bzh * bzh - byh * byh);
+        cr[FX0 + 4] = roh * vzh * vxh - pi4i * bzh * bxh;
+        cr[FY0 + 4] = roh * vzh * vyh - pi4i * bzh * byh;
+        cr[FX7] = eyh;
+        cr[FY7] = -exh;
+        cr[FEZ] = ezh;
+    }
+    __syncthreads();            // every reader of the cell fluxes is past this point
+#pragma unroll
+    for (int r = 0; r < NF; r++) sF[r][t] = cr[r];
+    __syncthreads();
+    // (3) full step, Az source, viscosity, update (mlw_m3.f:233-374)
+    if (!own || ci > ix - 1 || cj > jx - 1) return;
+    const int kSW = (lj - 1) * KX + (li - 1), kSE = kSW + 1, kNW = kSW + KX, kNE = kNW + 1;
+    const double dxi = a.mx.di[ci - 1], dyi = a.my.di[cj - 1];
+    const double ux0 = a.mx.u0[ci - 1], ux1 = a.mx.u1[ci - 1], uy0 = a.my.u0[cj - 1], uy1 = a.my.u1[cj - 1];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        double xSW, xSE, xNW, xNE, ySW, ySE, yNW, yNE;
+        if (q < 5) {
+            xSW = sF[FX0 + q][kSW]; xSE = sF[FX0 + q][kSE]; xNW = sF[FX0 + q][kNW]; xNE = sF[FX0 + q][kNE];
+            ySW = sF[FY0 + q][kSW]; ySE = sF[FY0 + q][kSE]; yNW = sF[FY0 + q][kNW]; yNE = sF[FY0 + q][kNE];
+        } else if (q == 5) {
+            xSW = 0.0; xSE = 0.0; xNW = 0.0; xNE = 0.0;
+            ySW = sF[FEZ][kSW]; ySE = sF[FEZ][kSE]; yNW = sF[FEZ][kNW]; yNE = sF[FEZ][kNE];
+        } else if (q == 6) {
+            xSW = -sF[FEZ][kSW]; xSE = -sF[FEZ][kSE]; xNW = -sF[FEZ][kNW]; xNE = -sF[FEZ][kNE];
+            ySW = 0.0; ySE = 0.0; yNW = 0.0; yNE = 0.0;
+        } else {
+            xSW = sF[FX7][kSW]; xSE = sF[FX7][kSE]; xNW = sF[FX7][kNW]; xNE = sF[FX7][kNE];
+            ySW = sF[FY7][kSW]; ySE = sF[FY7][kSE]; yNW = sF[FY7][kNW]; yNE = sF[FY7][kNE];
+        }
+        const double dfdx = dxi * (uy0 * (xSE - xSW) + uy1 * (xNE - xNW));   // mlwfull.f:31-41
+        const double dfdy = dyi * (ux0 * (yNW - ySW) + ux1 * (yNE - ySE));
+        du[q] = du[q] - 0.5 * dt * dfdx - 0.5 * dt * dfdy;
+    }
+    {   // mlwsrcf.f:33-40 with s = -ezh at the corners
+        const double sh = uy0 * (ux0 * (-sF[FEZ][kSW]) + ux1 * (-sF[FEZ][kSE]))
+                        + uy1 * (ux0 * (-sF[FEZ][kNW]) + ux1 * (-sF[FEZ][kNE]));
+        daz = daz + 0.5 * dt * sh;
+    }
+    const double zero = 0.0;
+    const double vx = sV[0][lc], vy = sV[1][lc];
+    const double qxE = a.qav * a.dxm[ci - 1] * fmaxF(zero, __builtin_fabs(sV[0][lc + 1] - vx) - kR4_1em4);
+    const double qxW = a.qav * a.dxm[ci - 2] * fmaxF(zero, __builtin_fabs(vx - sV[0][lc - 1]) - kR4_1em4);
+    const double qyN = a.qav * a.dym[cj - 1] * fmaxF(zero, __builtin_fabs(sV[1][lc + CX] - vy) - kR4_1em4);
+    const double qyS = a.qav * a.dym[cj - 2] * fmaxF(zero, __builtin_fabs(vy - sV[1][lc - CX]) - kR4_1em4);
+    const double dxim = a.mx.dim[ci - 1], dxim1 = a.mx.dim[ci - 2];
+    const double dyim = a.my.dim[cj - 1], dyim1 = a.my.dim[cj - 2];
+    double nu[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {                                          // mlwartv.f:33-41
+        const double uc = sU[q][lc];
+        du[q] = du[q]
+              + dt * (dxi * (qxE * dxim * (sU[q][lc + 1] - uc) - qxW * dxim1 * (uc - sU[q][lc - 1])))
+              + dt * (dyi * (qyN * dyim * (sU[q][lc + CX] - uc) - qyS * dyim1 * (uc - sU[q][lc - CX])));
+        nu[q] = uc + du[q];
+    }
+    const size_t p = (size_t)(ci - 1) + (size_t)ix * (size_t)(cj - 1);
+    const double nro = nu[0];
+    const double nvx = nu[2] / nro;
+    const double nvy = nu[3] / nro;
+    const double nvz = nu[4] / nro;
+    const double vv = nvx * nvx + nvy * nvy + nvz * nvz;
+    const double bb = nu[5] * nu[5] + nu[6] * nu[6] + nu[7] * nu[7];
+    a.out[0][p] = nro;
+    a.out[1][p] = (gm - 1.0) * (nu[1] - 0.5 * nro * vv - pi8i * bb);
+    a.out[2][p] = nvx;
+    a.out[3][p] = nvy;
+    a.out[4][p] = nvz;
+    a.out[5][p] = nu[5];
+    a.out[6][p] = nu[6];
+    a.out[7][p] = nu[7];
+    a.out[8][p] = a.in[8][p] + daz;
+}
+
+}  // namespace
+
+hipError_t launch_mlw_m3(const Mlw3Args &a, int dim, hipStream_t s) {
+    if (dim == 1) {
+        const int nseg = (a.ix - 2 + 61) / 62;
+        hipLaunchKernelGGL(mlw1d_m3_kernel, dim3((nseg + 3) / 4), dim3(256), 0, s, a);
+    } else {
+        const int ntx = (a.ix - 2 + TX - 1) / TX, nty = (a.jx - 2 + TY - 1) / TY;
+        hipLaunchKernelGGL(mlw2d_m3_kernel, dim3(ntx * nty), dim3(256), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cans
