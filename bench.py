@@ -46,6 +46,9 @@ def parse():
     p.add_argument("--c2-n", type=int, default=384, help="interior cells per side of the 3D blast")
     p.add_argument("--tubes", type=int, default=4096, help="batched 1D tubes to t=0.1 (0: skip)")
     p.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint timing")
+    p.add_argument("--spicule-runs", type=int, default=1024,
+                   help="realizations of the UTRIP spicule ensemble figure (0: skip the spicule figures)")
+    p.add_argument("--spicule-steps", type=int, default=100000, help="steps of the single UTRIP spicule run")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only run 5 device-to-device copies of this many MiB (the byte-counter calibration "
                         "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) and exit")
@@ -219,6 +222,125 @@ def tubes_figures(ntube: int = 4096, tend: float = 0.1):
     return out
 
 
+def m3_figures(n: int = 1024, steps: int = 100):
+    """The three-component engine (cans2d/hdmlw/mlw_m3.f with cfl_m3 and
+    bdperx/bdpery, CANS_ENGINE_MLW3 of the resident loop) on the 2.5D
+    Orszag-Tang vortex at n^2: the engine kernel's average launch (HIP events
+    around it every step) against 144 B per zone-update (9 fields read and
+    written), and the whole step (cfl pass + engine + boundary passes)."""
+    import numpy as np
+    import torch
+
+    from utrip_amd import cans, problems
+    pb = problems.orszag_tang_25d(n=n, margin=1)
+    names = cans.FIELDS_M3[2]
+
+    def td(a):
+        return torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda:0")
+
+    st = [td(pb.fields[k]) for k in names]
+    g = {k: torch.from_numpy(pb.grid[k]).to("cuda:0") for k in ("dx", "dxm", "dy", "dym")}
+    pp = pb.params
+    run = cans.Run(2, cans.ENGINE_MLW3, st, g, pp["gm"], pp["safety"], qav=pp["qav"], margin=1, bc=7)
+    run.reset(dtmin=pp["dtmin"])
+    run.steps(5)
+    ev = cans.EventSet(2 * steps + 2)
+    torch.cuda.synchronize()
+    ev.record(2 * steps)
+    run.steps(steps, events=ev)
+    ev.record(2 * steps + 1)
+    torch.cuda.synchronize()
+    total = ev.elapsed_ms(2 * steps, 2 * steps + 1)
+    kern = statistics.fmean(ev.elapsed_ms(2 * k, 2 * k + 1) for k in range(steps))
+    _, info = run.state()
+    assert info["merr"] == 0 and info["nexec"] == steps + 5, info
+    z = n * n
+    return {"cells": z, "steps": steps, "ms_per_step": total / steps, "zone_updates_per_s": z * steps / (total / 1e3),
+            "engine_avg_us": kern * 1e3, "engine_hbm_frac": 144 * z / (kern / 1e3) / 1e9 / HBM_PEAK_GBS}
+
+
+def spicule_cpu(steps: int):
+    """CPU leg of the spicule figures: the C restatement's main.f loop
+    (oracle/liboracle.so, gcc -O2, one core) for `steps` steps from t = 0."""
+    import numpy as np
+
+    from oracle.oracle import Oracle
+    from utrip_amd import spicule
+    sp = spicule.setup()
+    st = {k: np.asfortranarray(v.copy()) for k, v in sp.state.items()}
+    g = {k: np.asfortranarray(v) for k, v in sp.geom.items()}
+    o = Oracle()
+    t0 = time.perf_counter()
+    o.spicule_run(st, g, np.asfortranarray(sp.x), sp.margin, sp.gm, spicule.TEND, steps, 0.0, 0, 1)
+    sec = time.perf_counter() - t0
+    return {"steps": steps, "s": sec, "steps_per_s": steps / sec, "zone_updates_per_s": steps * 1024 / sec,
+            "cores": 1, "kind": "port", "sample": f"{steps} steps of the 1024-cell spicule loop from t=0"}
+
+
+def spicule_figures(nrun: int, steps_single: int, cpu_steps: int = 10000, no_cpu: bool = False):
+    """The UTRIP spicule model (md_spicule/hdmlw/main.f loop, 1024 cells,
+    mlw_m_bg + Park-Miller torque + bnd + chkdav, FP64 bit-exact):
+      single   -- one realization for the shipped run's length (100000 steps)
+                  in one persistent workgroup; block size picked by a probe;
+      ensemble -- nrun realizations (seeds 1..nrun), 2000 steps each, one
+                  workgroup per realization.
+    zone-updates = realizations x 1024 interior cells x steps; the HBM
+    fraction counts 80 B per zone-update (5 fields read and written; the
+    geometry is shared by all realizations and stays in cache)."""
+    import torch
+
+    from utrip_amd import cans, spicule
+    sp = spicule.setup()
+    out = {"cells": 1024}
+
+    def timed(runs, nstop, bt):
+        ev = cans.EventSet(2)
+        torch.cuda.synchronize()
+        ev.record(0)
+        runs.run(nstop=nstop, block_threads=bt)
+        ev.record(1)
+        torch.cuda.synchronize()
+        return ev.elapsed_ms(0, 1)
+
+    probe = {}
+    for bt in (256, 512, 1024):
+        r = spicule.SpiculeRuns(sp, [1], device="cuda:0")
+        timed(r, 200, bt)
+        probe[bt] = timed(r, 2200, bt)
+    bt = min(probe, key=probe.get)
+    r = spicule.SpiculeRuns(sp, [1], device="cuda:0")
+    ms = timed(r, steps_single, bt)
+    info = r.info(0)
+    assert info["merr"] == 0 and info["ns"] == steps_single, info
+    out["single"] = {"steps": steps_single, "ms": ms, "us_per_step": ms * 1e3 / steps_single,
+                     "t_reached": info["t"], "block_threads": bt,
+                     "zone_updates_per_s": 1024 * steps_single / (ms / 1e3)}
+    if nrun > 0:
+        steps = 2000
+        best = None
+        for bte in (64, 128, 256):
+            e = spicule.SpiculeRuns(sp, list(range(1, nrun + 1)), device="cuda:0")
+            timed(e, 50, bte)
+            m = timed(e, 250, bte)
+            if best is None or m < best[1]:
+                best = (bte, m)
+        bte = best[0]
+        e = spicule.SpiculeRuns(sp, list(range(1, nrun + 1)), device="cuda:0")
+        ms = timed(e, steps, bte)
+        assert int(e.merr.abs().max().item()) == 0 and int(e.ns.min().item()) == steps
+        zu = nrun * 1024 * steps
+        out["ensemble"] = {"realizations": nrun, "steps": steps, "ms": ms, "block_threads": bte,
+                           "zone_updates_per_s": zu / (ms / 1e3),
+                           "hbm_frac_alg": 80 * zu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    if not no_cpu and cpu_steps > 0:
+        c = spicule_cpu(cpu_steps)
+        out["cpu_baseline"] = c
+        out["single"]["vs_cpu"] = out["single"]["zone_updates_per_s"] / c["zone_updates_per_s"]
+        if "ensemble" in out:
+            out["ensemble"]["vs_cpu"] = out["ensemble"]["zone_updates_per_s"] / c["zone_updates_per_s"]
+    return out
+
+
 def adjoint_figures(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     """Gradient of L = sum (rho_final - rho_target)^2 dx dy over an n^2 OT run
     of nsteps MLW steps with respect to the initial velocities and a pressure
@@ -339,6 +461,12 @@ def main():
     adj = None
     if args.adjoint:
         adj = adjoint_figures()
+    m3 = None
+    if not args.no_extra:
+        m3 = m3_figures()
+    spc = None
+    if args.spicule_runs > 0 or args.spicule_steps > 0:
+        spc = spicule_figures(args.spicule_runs, args.spicule_steps, no_cpu=args.no_cpu)
     tb = None
     if args.tubes > 0:
         tb = tubes_figures(args.tubes)
@@ -391,6 +519,8 @@ def main():
         "blast384": c2,
         "tubes": tb,
         "adjoint_ot512": adj,
+        "spicule": spc,
+        "ot1024_mlw_m3": m3,
     }
     print(json.dumps(out), flush=True)
 

@@ -114,3 +114,54 @@ def test_spicule_main(dev, oracle, tmp_path, resident):
     for k in F1:
         assert_bitwise(dac.read_last(outd / f"{k}.dac"), st[k], k)
     assert float(dac.read_last(outd / "t.dac")) == t
+
+
+def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
+    """mlw_m3_ and cfl_m3_ of libcans{1,2}d_gpu.so called as a Fortran main
+    program calls them (every argument by reference, host arrays, strict
+    mode), against the restatement."""
+    import ctypes as C
+
+    from tests.helpers import stretched_grid
+    from tests.test_oracle_m3 import state1, state2
+
+    def call(lib, name, *args):
+        keep, cargs = [], []
+        for a in args:
+            if isinstance(a, np.ndarray):
+                cargs.append(C.c_void_p(a.ctypes.data))
+            elif isinstance(a, (C.c_double, C.c_int)):
+                cargs.append(C.byref(a))
+            elif isinstance(a, int):
+                v = C.c_int(a); keep.append(v); cargs.append(C.byref(v))
+            else:
+                v = C.c_double(a); keep.append(v); cargs.append(C.byref(v))
+        getattr(lib, name + "_")(*cargs)
+
+    lib2 = C.CDLL(str(PKG_DIR / "libcans2d_gpu.so"))
+    ix, jx = 70, 41
+    s = state2(ix, jx, 4)
+    _, dx, dxm = stretched_grid(ix, seed=1)
+    _, dy, dym = stretched_grid(jx, seed=2)
+    F = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz", "az")
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m3_2d(*[a[k] for k in F], 1.1e-3, 3.0, 5 / 3, dx, dxm, dy, dym)
+    call(lib2, "mlw_m3", *[b[k] for k in F], 1.1e-3, 3.0, 5 / 3, dx, dxm, ix, dy, dym, jx)
+    for k in F:
+        assert_bitwise(b[k], a[k], k)
+    dt, merr = C.c_double(), C.c_int()
+    call(lib2, "cfl_m3", dt, 0.4, 1e-10, merr, 5 / 3, *[s[k] for k in F[:8]], dx, ix, dy, jx)
+    want = oracle.cfl_m3_2d(0.4, 1e-10, 5 / 3, *[s[k] for k in F[:8]], dx, dy)
+    assert (dt.value, merr.value) == want[:2]
+    lib1 = C.CDLL(str(PKG_DIR / "libcans1d_gpu.so"))
+    ix = 123
+    s1 = state1(ix, 2)
+    _, dx, dxm = stretched_grid(ix, seed=3)
+    F1m = ("ro", "pr", "vx", "vy", "vz", "by", "bz")
+    a = {k: v.copy() for k, v in s1.items()}
+    b = {k: v.copy() for k, v in s1.items()}
+    oracle.mlw_m3_1d(*[a[k] for k in F1m], a["bx"], a["bxm"], 2e-3, 3.0, 5 / 3, dx, dxm)
+    call(lib1, "mlw_m3", *[b[k] for k in F1m], b["bx"], b["bxm"], 2e-3, 3.0, 5 / 3, dx, dxm, ix)
+    for k in F1m:
+        assert_bitwise(b[k], a[k], k)

@@ -1,7 +1,8 @@
 !=======================================================================
 ! libcans2d_gpu -- the cans2d hot-path routines with the reference's names
 ! and argument lists (cans2d/hdmlw/mlw_m.f:2-3, cans2d/hdroe/roe_m.f:2,
-! cans2d/common/cfl_m.f:2-3, cans2d/bc/bd*{x,y}.f), computed on the GPU by
+! cans2d/common/cfl_m.f:2-3, cans2d/hdmlw/mlw_m3.f:2-3,
+! cans2d/common/cfl_m3.f:2-3, cans2d/bc/bd*{x,y}.f), computed on the GPU by
 ! libcans_gpu.so.
 !=======================================================================
 subroutine mlw_m(ro, pr, vx, vy, bx, by, az, dt, qav, gm, dx, dxm, ix, dy, dym, jx)
@@ -20,6 +21,46 @@ subroutine mlw_m(ro, pr, vx, vy, bx, by, az, dt, qav, gm, dx, dxm, ix, dy, dym, 
   call dunmap(bx, 1); call dunmap(by, 1); call dunmap(az, 1)
   call dunmap(dx, 0); call dunmap(dxm, 0); call dunmap(dy, 0); call dunmap(dym, 0)
 end subroutine mlw_m
+
+subroutine mlw_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, ix, dy, dym, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target :: ro(ix, jx), pr(ix, jx), vx(ix, jx), vy(ix, jx), vz(ix, jx)
+  double precision, target :: bx(ix, jx), by(ix, jx), bz(ix, jx), az(ix, jx)
+  double precision, target, intent(in) :: dx(ix), dxm(ix), dy(jx), dym(jx)
+  double precision, intent(in) :: dt, qav, gm
+  integer :: n
+  n = ix * jx
+  call check(c_mlw2d_m3(dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vy, n), dmap(vz, n), dmap(bx, n), dmap(by, n), &
+       dmap(bz, n), dmap(az, n), dt, qav, gm, dmap(dx, ix), dmap(dxm, ix), int(ix, c_int), dmap(dy, jx), &
+       dmap(dym, jx), int(jx, c_int), c_null_ptr), 'mlw_m3')
+  call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vy, 1); call dunmap(vz, 1)
+  call dunmap(bx, 1); call dunmap(by, 1); call dunmap(bz, 1); call dunmap(az, 1)
+  call dunmap(dx, 0); call dunmap(dxm, 0); call dunmap(dy, 0); call dunmap(dym, 0)
+end subroutine mlw_m3
+
+subroutine cfl_m3(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, vz, bx, by, bz, dx, ix, dy, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  integer, intent(out) :: merr
+  double precision, intent(out) :: dt
+  double precision, intent(in) :: safety, dtmin, gm
+  double precision, target, intent(in) :: ro(ix, jx), pr(ix, jx), vx(ix, jx), vy(ix, jx), vz(ix, jx)
+  double precision, target, intent(in) :: bx(ix, jx), by(ix, jx), bz(ix, jx), dx(ix), dy(jx)
+  real(c_double) :: d
+  integer(c_int) :: m
+  integer :: n
+  n = ix * jx
+  call check(c_cfl2d_m3(d, safety, dtmin, m, gm, dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vy, n), dmap(vz, n), &
+       dmap(bx, n), dmap(by, n), dmap(bz, n), dmap(dx, ix), int(ix, c_int), dmap(dy, jx), int(jx, c_int), &
+       c_null_ptr), 'cfl_m3')
+  call dunmap(ro, 0); call dunmap(pr, 0); call dunmap(vx, 0); call dunmap(vy, 0); call dunmap(vz, 0)
+  call dunmap(bx, 0); call dunmap(by, 0); call dunmap(bz, 0); call dunmap(dx, 0); call dunmap(dy, 0)
+  dt = d
+  merr = m
+end subroutine cfl_m3
 
 subroutine roe_m(ro, pr, vx, vy, bx, by, az, dt, gm, dx, ix, dy, jx)
   use cans_gpu_iface

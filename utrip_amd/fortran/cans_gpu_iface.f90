@@ -97,6 +97,23 @@ module cans_gpu_iface
        real(c_double), value :: q0
      end function c_bnd1d
 
+     integer(c_int) function c_mlw1d_m3(ro, pr, vx, vy, vz, by, bz, bx, bxm, dt, qav, gm, dx, dxm, ix, s) &
+          bind(C, name="cans1d_mlw_m3")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vy, vz, by, bz, bx, bxm, dx, dxm, s
+       real(c_double), value :: dt, qav, gm
+       integer(c_int), value :: ix
+     end function c_mlw1d_m3
+     integer(c_int) function c_cfl1d_m3(dt, safety, dtmin, merr, gm, bx, ro, pr, vx, vy, by, vz, bz, dx, ix, s) &
+          bind(C, name="cans1d_cfl_m3")
+       import :: c_ptr, c_double, c_int
+       real(c_double) :: dt
+       integer(c_int) :: merr
+       real(c_double), value :: safety, dtmin, gm
+       type(c_ptr), value :: bx, ro, pr, vx, vy, by, vz, bz, dx, s
+       integer(c_int), value :: ix
+     end function c_cfl1d_m3
+
      ! ---------------------------------------------------------- 2D
      integer(c_int) function c_mlw2d(ro, pr, vx, vy, bx, by, az, dt, qav, gm, dx, dxm, ix, dy, dym, jx, s) &
           bind(C, name="cans2d_mlw_m")
@@ -112,6 +129,22 @@ module cans_gpu_iface
        real(c_double), value :: dt, gm
        integer(c_int), value :: ix, jx
      end function c_roe2d
+     integer(c_int) function c_mlw2d_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, ix, dy, dym, &
+          jx, s) bind(C, name="cans2d_mlw_m3")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vy, vz, bx, by, bz, az, dx, dxm, dy, dym, s
+       real(c_double), value :: dt, qav, gm
+       integer(c_int), value :: ix, jx
+     end function c_mlw2d_m3
+     integer(c_int) function c_cfl2d_m3(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, vz, bx, by, bz, dx, ix, dy, &
+          jx, s) bind(C, name="cans2d_cfl_m3")
+       import :: c_ptr, c_double, c_int
+       real(c_double) :: dt
+       integer(c_int) :: merr
+       real(c_double), value :: safety, dtmin, gm
+       type(c_ptr), value :: ro, pr, vx, vy, vz, bx, by, bz, dx, dy, s
+       integer(c_int), value :: ix, jx
+     end function c_cfl2d_m3
      integer(c_int) function c_cfl2d(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, bx, by, dx, ix, dy, jx, s) &
           bind(C, name="cans2d_cfl_m")
        import :: c_ptr, c_double, c_int

@@ -143,6 +143,21 @@ def orszag_tang_2d(n: int = 1024, margin: int = 1, gm: float = 5.0 / 3.0) -> Pro
                    dict(gm=gm, margin=margin, safety=0.4, qav=3.0, dtmin=1e-10))
 
 
+def orszag_tang_25d(n: int = 1024, margin: int = 1, gm: float = 5.0 / 3.0) -> Problem:
+    """The Orszag-Tang vortex with out-of-plane components for mlw_m3:
+    as orszag_tang_2d plus vz = 0.2 sin(x + y), bz = 0.3 cos(x - y)
+    (Gaussian units), ghosts periodic."""
+    pb = orszag_tang_2d(n, margin, gm)
+    X, Y = np.meshgrid(pb.grid["x"], pb.grid["y"], indexing="ij")
+    vz = np.asfortranarray(0.2 * np.sin(X + Y))
+    bz = np.asfortranarray(0.3 * np.cos(X - Y))
+    for q in (vz, bz):
+        periodic_fill(q, margin)
+    f = {k: pb.fields[k] for k in ("ro", "pr", "vx", "vy")}
+    f.update(vz=vz, bx=pb.fields["bx"], by=pb.fields["by"], bz=bz, az=pb.fields["az"])
+    return Problem("orszag_tang_25d", f, pb.grid, pb.params)
+
+
 def blast_3d(n: int = 384, margin: int = 2, gm: float = 5.0 / 3.0, gaussian_b: bool = True,
              p0=0.1, p1=10.0, r0=0.1) -> Problem:
     """MHD blast wave in the periodic unit cube: rho = 1, p = 0.1, an
