@@ -64,6 +64,11 @@ int cans1d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
 int cans1d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *by, double *bz,
                   const double *bx, const double *bxm, double dt, double qav, double gm, const double *dx,
                   const double *dxm, int ix, void *stream);
+/* cans1d/hdroe/roe_m3.f:2 roe_m3(ro,pr,vx,vy,vz,by,bz,bx,bxm,dt,gm,dx,ix) (fuses
+ * roeflux_m3.f, tvdminmod.f); updated in place on i=3..ix-2. */
+int cans1d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *by, double *bz,
+                  const double *bx, const double *bxm, double dt, double gm, const double *dx, int ix,
+                  void *stream);
 /* cans1d/common/cfl_m3.f:2 cfl_m3(dt,safety,dtmin,merr,gm,bx,ro,pr,vx,vy,by,vz,bz,dx,ix) */
 int cans1d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *bx,
                   const double *ro, const double *pr, const double *vx, const double *vy, const double *by,
@@ -101,6 +106,11 @@ int cans2d_bnd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
 int cans2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by, double *bz,
                   double *az, double dt, double qav, double gm, const double *dx, const double *dxm, int ix,
                   const double *dy, const double *dym, int jx, void *stream);
+/* cans2d/hdroe/roe_m3.f:2 roe_m3(ro,pr,vx,vy,vz,bx,by,bz,az,dt,gm,dx,ix,dy,jx);
+ * updated in place on i=3..ix-2, j=3..jx-2. */
+int cans2d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by, double *bz,
+                  double *az, double dt, double gm, const double *dx, int ix, const double *dy, int jx,
+                  void *stream);
 /* cans2d/common/cfl_m3.f:2-3 cfl_m3(dt,safety,dtmin,merr,gm,ro,pr,vx,vy,vz,bx,by,bz,dx,ix,dy,jx) */
 int cans2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *ro,
                   const double *pr, const double *vx, const double *vy, const double *vz, const double *bx,
@@ -216,6 +226,7 @@ int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *
 #define CANS_ENGINE_ROE 1   /* cans{1,2,3}d/hdroe/roe_m.f */
 #define CANS_ENGINE_HLLD 2  /* cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f (3D only) */
 #define CANS_ENGINE_MLW3 3  /* cans{1,2}d/hdmlw/mlw_m3.f with cfl_m3 (1D/2D only) */
+#define CANS_ENGINE_ROE3 4  /* cans{1,2}d/hdroe/roe_m3.f with cfl_m3 (1D/2D only) */
 
 /* Device workspace (bytes) a cans_run of this shape needs. */
 size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx);

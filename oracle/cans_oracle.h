@@ -48,6 +48,9 @@ void or1d_mlw_m_bg(double *ro, double *pr, double *vx, double *vy, double *by,
                    const double *dscm, const double *rr, const double *rrm,
                    const double *drr, const double *drrm, const double *dx,
                    const double *dxm, int ix);
+void or1d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
+                 double *by, double *bz, const double *bx, const double *bxm,
+                 double dt, double gm, const double *dx, int ix);
 void or1d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
                  double *by, double *bz, const double *bx, const double *bxm,
                  double dt, double qav, double gm, const double *dx,
@@ -88,6 +91,9 @@ void or2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
                 const double *vy, const double *bx, const double *by,
                 const double *dx, int ix, const double *dy, int jx,
                 int *imin, int *jmin);
+void or2d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
+                 double *bx, double *by, double *bz, double *az, double dt,
+                 double gm, const double *dx, int ix, const double *dy, int jx);
 void or2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
                  double *bx, double *by, double *bz, double *az, double dt,
                  double qav, double gm, const double *dx, const double *dxm,

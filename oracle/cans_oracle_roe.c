@@ -157,10 +157,12 @@ static void roeflux12(double gm, double rhol, double prl, double vxl, double vyl
 }
 
 /* One interface of the cans3d roeflux_m (cans3d/hdroe/roeflux_m.f:33-221,
- * with the alphs "+1-sp2" fix of :79).  out = fro, fee, frx, fry, frz, fby, fbz. */
-static void roeflux3(double gm, double rhol, double prl, double vxl, double vyl, double vzl, double bxl,
-                     double byl, double bzl, double rhor, double prr, double vxr, double vyr, double vzr,
-                     double bxr, double byr, double bzr, double *out) {
+ * with the alphs "+1-sp2" fix of :79 when fix2006) -- without the fix it is
+ * roeflux_m3 of cans1d/cans2d (cans1d/hdroe/roeflux_m3.f:52-285, alphs at
+ * :124).  out = fro, fee, frx, fry, frz, fby, fbz. */
+static void roeflux3(int fix2006, double gm, double rhol, double prl, double vxl, double vyl, double vzl,
+                     double bxl, double byl, double bzl, double rhor, double prr, double vxr, double vyr,
+                     double vzr, double bxr, double byr, double bzr, double *out) {
     const double pi = acos(-1.0);
     const double pi4 = 4.0 * pi;
     const double pi4i = 1.0 / pi4;
@@ -207,7 +209,8 @@ static void roeflux3(double gm, double rhol, double prl, double vxl, double vyl,
     double cfcs = fmaxF(0.0, cfast2 - cslow2);
     double cfa = fmaxF(0.0, cfast2 - cs2);
     double alphf = sp2 * sqrt(cfca / (cfcs + 1.0 - sp2)) + 1.0 - sp2;
-    double alphs = sp2 * sqrt(cfa / (cfcs + 1.0 - sp2)) + 1.0 - sp2;
+    double alphs = fix2006 ? sp2 * sqrt(cfa / (cfcs + 1.0 - sp2)) + 1.0 - sp2
+                           : sp2 * sqrt(cfa / (cfcs + 1.0 - sp2));
     double sgnbx = signF(1.0, bxbar);
     double eeps = (vxr - vxl + fabs(vxr - vxl)) * 2.5e-1;
     double elpf = -fmaxF(fabs(vxbar + cfast), eeps);
@@ -414,6 +417,99 @@ void or1d_roe_m(double *ro, double *pr, double *vx, double *vy, double *by,
     free(roh); free(eeh); free(rxh); free(ryh); free(byh); free(prh); free(vxh); free(vyh);
 }
 
+
+/* cans1d/hdroe/roe_m3.f:2-151: roe_m with vz, bz (Bx constant) and the
+ * roeflux_m3 flux */
+void or1d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
+                 double *by, double *bz, const double *bx, const double *bxm,
+                 double dt, double gm, const double *dx, int ix) {
+    const double pi = acos(-1.0);
+    const double pi4 = 4.0 * pi;
+    const double pi4i = 1.0 / pi4;
+    const double pi8i = 5.0e-1 * pi4i;
+    enum { RO, PR, VX, VY, VZ, BX, BY, BZ, NW };
+    enum { FRO, FEE, FRX, FRY, FRZ, FBY, FBZ, NF };
+    double *ee = dalloc(ix), *rx = dalloc(ix), *ry = dalloc(ix), *rz = dalloc(ix);
+    double *f[NF], *w[NW][2];
+    for (int q = 0; q < NF; q++) f[q] = dalloc(ix);
+    for (int q = 0; q < NW; q++) { w[q][0] = dalloc(ix); w[q][1] = dalloc(ix); }
+    double *roh = dalloc(ix), *eeh = dalloc(ix), *rxh = dalloc(ix), *ryh = dalloc(ix), *rzh = dalloc(ix);
+    double *byh = dalloc(ix), *bzh = dalloc(ix), *prh = dalloc(ix), *vxh = dalloc(ix), *vyh = dalloc(ix);
+    double *vzh = dalloc(ix);
+    for (int i = 1; i <= ix; i++) { /* :59-68 */
+        rx[I1(i)] = ro[I1(i)] * vx[I1(i)];
+        ry[I1(i)] = ro[I1(i)] * vy[I1(i)];
+        rz[I1(i)] = ro[I1(i)] * vz[I1(i)];
+        double v2 = sq(vx[I1(i)]) + sq(vy[I1(i)]) + sq(vz[I1(i)]);
+        double b2 = sq(bx[I1(i)]) + sq(by[I1(i)]) + sq(bz[I1(i)]);
+        ee[I1(i)] = pr[I1(i)] / (gm - 1.0) + 0.5 * ro[I1(i)] * v2 + pi8i * b2;
+    }
+    const double *prim[NW] = {ro, pr, vx, vy, vz, bxm, by, bz};
+    for (int i = 1; i <= ix - 1; i++) /* :70-90 (bx at the face: bxm on both sides) */
+        for (int q = 0; q < NW; q++) {
+            w[q][0][I1(i)] = prim[q][I1(i)];
+            w[q][1][I1(i)] = q == BX ? bxm[I1(i)] : prim[q][I1(i + 1)];
+        }
+#define FLUX3()                                                                                              \
+    for (int i = 1; i <= ix - 1; i++) {                                                                      \
+        double o[7];                                                                                         \
+        roeflux3(0, gm, w[RO][0][I1(i)], w[PR][0][I1(i)], w[VX][0][I1(i)], w[VY][0][I1(i)], w[VZ][0][I1(i)], \
+                 w[BX][0][I1(i)], w[BY][0][I1(i)], w[BZ][0][I1(i)], w[RO][1][I1(i)], w[PR][1][I1(i)],       \
+                 w[VX][1][I1(i)], w[VY][1][I1(i)], w[VZ][1][I1(i)], w[BX][1][I1(i)], w[BY][1][I1(i)],       \
+                 w[BZ][1][I1(i)], o);                                                                        \
+        for (int q = 0; q < NF; q++) f[q][I1(i)] = o[q];                                                     \
+    }
+    FLUX3()
+    for (int i = 2; i <= ix - 1; i++) { /* :94-101 */
+        roh[I1(i)] = ro[I1(i)] + 0.5 * dt * ((f[FRO][I1(i - 1)] - f[FRO][I1(i)]) / dx[I1(i)]);
+        eeh[I1(i)] = ee[I1(i)] + 0.5 * dt * ((f[FEE][I1(i - 1)] - f[FEE][I1(i)]) / dx[I1(i)]);
+        rxh[I1(i)] = rx[I1(i)] + 0.5 * dt * ((f[FRX][I1(i - 1)] - f[FRX][I1(i)]) / dx[I1(i)]);
+        ryh[I1(i)] = ry[I1(i)] + 0.5 * dt * ((f[FRY][I1(i - 1)] - f[FRY][I1(i)]) / dx[I1(i)]);
+        rzh[I1(i)] = rz[I1(i)] + 0.5 * dt * ((f[FRZ][I1(i - 1)] - f[FRZ][I1(i)]) / dx[I1(i)]);
+        byh[I1(i)] = by[I1(i)] + 0.5 * dt * ((f[FBY][I1(i - 1)] - f[FBY][I1(i)]) / dx[I1(i)]);
+        bzh[I1(i)] = bz[I1(i)] + 0.5 * dt * ((f[FBZ][I1(i - 1)] - f[FBZ][I1(i)]) / dx[I1(i)]);
+    }
+    for (int i = 2; i <= ix - 1; i++) { /* :105-113 */
+        vxh[I1(i)] = rxh[I1(i)] / roh[I1(i)];
+        vyh[I1(i)] = ryh[I1(i)] / roh[I1(i)];
+        vzh[I1(i)] = rzh[I1(i)] / roh[I1(i)];
+        double v2 = sq(vxh[I1(i)]) + sq(vyh[I1(i)]) + sq(vzh[I1(i)]);
+        double b2 = sq(bx[I1(i)]) + sq(byh[I1(i)]) + sq(bzh[I1(i)]);
+        prh[I1(i)] = (gm - 1.0) * (eeh[I1(i)] - 0.5 * roh[I1(i)] * v2 - pi8i * b2);
+    }
+    tvd1(roh, w[RO][0], w[RO][1], ix);
+    tvd1(prh, w[PR][0], w[PR][1], ix);
+    tvd1(vxh, w[VX][0], w[VX][1], ix);
+    tvd1(vyh, w[VY][0], w[VY][1], ix);
+    tvd1(byh, w[BY][0], w[BY][1], ix);
+    tvd1(vzh, w[VZ][0], w[VZ][1], ix);
+    tvd1(bzh, w[BZ][0], w[BZ][1], ix);
+    FLUX3()
+#undef FLUX3
+    for (int i = 3; i <= ix - 2; i++) { /* :128-137 */
+        ro[I1(i)] = ro[I1(i)] + dt * ((f[FRO][I1(i - 1)] - f[FRO][I1(i)]) / dx[I1(i)]);
+        ee[I1(i)] = ee[I1(i)] + dt * ((f[FEE][I1(i - 1)] - f[FEE][I1(i)]) / dx[I1(i)]);
+        rx[I1(i)] = rx[I1(i)] + dt * ((f[FRX][I1(i - 1)] - f[FRX][I1(i)]) / dx[I1(i)]);
+        ry[I1(i)] = ry[I1(i)] + dt * ((f[FRY][I1(i - 1)] - f[FRY][I1(i)]) / dx[I1(i)]);
+        rz[I1(i)] = rz[I1(i)] + dt * ((f[FRZ][I1(i - 1)] - f[FRZ][I1(i)]) / dx[I1(i)]);
+        by[I1(i)] = by[I1(i)] + dt * ((f[FBY][I1(i - 1)] - f[FBY][I1(i)]) / dx[I1(i)]);
+        bz[I1(i)] = bz[I1(i)] + dt * ((f[FBZ][I1(i - 1)] - f[FBZ][I1(i)]) / dx[I1(i)]);
+    }
+    for (int i = 3; i <= ix - 2; i++) { /* :139-148 */
+        vx[I1(i)] = rx[I1(i)] / ro[I1(i)];
+        vy[I1(i)] = ry[I1(i)] / ro[I1(i)];
+        vz[I1(i)] = rz[I1(i)] / ro[I1(i)];
+        double v2 = sq(vx[I1(i)]) + sq(vy[I1(i)]) + sq(vz[I1(i)]);
+        double b2 = sq(bx[I1(i)]) + sq(by[I1(i)]) + sq(bz[I1(i)]);
+        pr[I1(i)] = (gm - 1.0) * (ee[I1(i)] - 0.5 * ro[I1(i)] * v2 - pi8i * b2);
+    }
+    free(ee); free(rx); free(ry); free(rz);
+    for (int q = 0; q < NF; q++) free(f[q]);
+    for (int q = 0; q < NW; q++) { free(w[q][0]); free(w[q][1]); }
+    free(roh); free(eeh); free(rxh); free(ryh); free(rzh); free(byh); free(bzh);
+    free(prh); free(vxh); free(vyh); free(vzh);
+}
+
 /* ================================================================== 2D */
 /* cans2d/hdroe/tvdminmod.f:11-34 */
 static void tvd2(int mdir, const double *da, double *w1, double *w2, int ix, int jx) {
@@ -556,6 +652,151 @@ void or2d_roe_m(double *ro, double *pr, double *vx, double *vy, double *bx,
     for (int q = 0; q < NW; q++) { free(w0[q]); free(w1[q]); }
 }
 
+
+/* roeflux_m3 over the 2D w arrays, j=1..jx-1, i=1..ix-1, frame (vn, v1, v2,
+ * bn, b1, b2) as the reference's calls permute it (cans2d/hdroe/roe_m3.f:85-86,
+ * 124-125: x = (x, y, z), y = (y, x, z)) */
+static void roeflux23_arr(double gm, double *const *w0, double *const *w1, int vn, int v1, int v2, int bn,
+                          int b1, int b2, double *fro, double *fee, double *frn, double *fr1, double *fr2,
+                          double *fb1, double *fb2, int ix, int jx) {
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            double o[7];
+            roeflux3(0, gm, w0[0][p], w0[1][p], w0[vn][p], w0[v1][p], w0[v2][p], w0[bn][p], w0[b1][p], w0[b2][p],
+                     w1[0][p], w1[1][p], w1[vn][p], w1[v1][p], w1[v2][p], w1[bn][p], w1[b1][p], w1[b2][p], o);
+            fro[p] = o[0]; fee[p] = o[1]; frn[p] = o[2]; fr1[p] = o[3]; fr2[p] = o[4]; fb1[p] = o[5]; fb2[p] = o[6];
+        }
+}
+
+/* cans2d/hdroe/roe_m3.f:2-236: roe_m with vz, bz and roeflux_m3 */
+void or2d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
+                 double *bx, double *by, double *bz, double *az, double dt,
+                 double gm, const double *dx, int ix, const double *dy, int jx) {
+    const double pi = acos(-1.0);
+    const double pi4 = 4.0 * pi;
+    const double pi4i = 1.0 / pi4;
+    const double pi8i = 5.0e-1 * pi4i;
+    size_t n = (size_t)ix * jx;
+    enum { RO, PR, VX, VY, VZ, BX, BY, BZ, NW };
+    double *ee = dalloc(n), *rx = dalloc(n), *ry = dalloc(n), *rz = dalloc(n);
+    double *fro = dalloc(n), *fee = dalloc(n), *frx = dalloc(n), *fry = dalloc(n), *frz = dalloc(n);
+    double *fbx = dalloc(n), *fby = dalloc(n), *fbz = dalloc(n);
+    double *roh = dalloc(n), *eeh = dalloc(n), *rxh = dalloc(n), *ryh = dalloc(n), *rzh = dalloc(n);
+    double *prh = dalloc(n), *vxh = dalloc(n), *vyh = dalloc(n), *vzh = dalloc(n);
+    double *bxh = dalloc(n), *byh = dalloc(n), *bzh = dalloc(n);
+    double *w0[NW], *w1[NW];
+    for (int q = 0; q < NW; q++) { w0[q] = dalloc(n); w1[q] = dalloc(n); }
+    const double *prim[NW] = {ro, pr, vx, vy, vz, bx, by, bz};
+
+    for (size_t p = 0; p < n; p++) { /* :34-44 */
+        rx[p] = ro[p] * vx[p];
+        ry[p] = ro[p] * vy[p];
+        rz[p] = ro[p] * vz[p];
+        double v2 = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+        double b2 = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+        ee[p] = pr[p] / (gm - 1.0) + 0.5 * ro[p] * v2 + pi8i * b2;
+    }
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            roh[p] = ro[p]; eeh[p] = ee[p]; rxh[p] = rx[p]; ryh[p] = ry[p]; rzh[p] = rz[p];
+            bxh[p] = bx[p]; byh[p] = by[p]; bzh[p] = bz[p];
+        }
+    /* x sweep, first order (:64-99) */
+    for (int j = 1; j <= jx; j++)
+        for (int i = 1; i <= ix - 1; i++)
+            for (int q = 0; q < NW; q++) { w0[q][I2(i, j)] = prim[q][I2(i, j)]; w1[q][I2(i, j)] = prim[q][I2(i + 1, j)]; }
+    roeflux23_arr(gm, w0, w1, VX, VY, VZ, BX, BY, BZ, fro, fee, frx, fry, frz, fby, fbz, ix, jx);
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j), m = I2(i - 1, j);
+            roh[p] = roh[p] + 0.5 * dt * ((fro[m] - fro[p]) / dx[I1(i)]);
+            eeh[p] = eeh[p] + 0.5 * dt * ((fee[m] - fee[p]) / dx[I1(i)]);
+            rxh[p] = rxh[p] + 0.5 * dt * ((frx[m] - frx[p]) / dx[I1(i)]);
+            ryh[p] = ryh[p] + 0.5 * dt * ((fry[m] - fry[p]) / dx[I1(i)]);
+            rzh[p] = rzh[p] + 0.5 * dt * ((frz[m] - frz[p]) / dx[I1(i)]);
+            byh[p] = byh[p] + 0.5 * dt * ((fby[m] - fby[p]) / dx[I1(i)]);
+            bzh[p] = bzh[p] + 0.5 * dt * ((fbz[m] - fbz[p]) / dx[I1(i)]);
+        }
+    /* y sweep, first order (:103-139) */
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix; i++)
+            for (int q = 0; q < NW; q++) { w0[q][I2(i, j)] = prim[q][I2(i, j)]; w1[q][I2(i, j)] = prim[q][I2(i, j + 1)]; }
+    roeflux23_arr(gm, w0, w1, VY, VX, VZ, BY, BX, BZ, fro, fee, fry, frx, frz, fbx, fbz, ix, jx);
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j), m = I2(i, j - 1);
+            roh[p] = roh[p] + 0.5 * dt * ((fro[m] - fro[p]) / dy[I1(j)]);
+            eeh[p] = eeh[p] + 0.5 * dt * ((fee[m] - fee[p]) / dy[I1(j)]);
+            rxh[p] = rxh[p] + 0.5 * dt * ((frx[m] - frx[p]) / dy[I1(j)]);
+            ryh[p] = ryh[p] + 0.5 * dt * ((fry[m] - fry[p]) / dy[I1(j)]);
+            rzh[p] = rzh[p] + 0.5 * dt * ((frz[m] - frz[p]) / dy[I1(j)]);
+            bxh[p] = bxh[p] + 0.5 * dt * ((fbx[m] - fbx[p]) / dy[I1(j)]);
+            bzh[p] = bzh[p] + 0.5 * dt * ((fbz[m] - fbz[p]) / dy[I1(j)]);
+        }
+    for (int j = 2; j <= jx - 1; j++) /* :143-152 */
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            vxh[p] = rxh[p] / roh[p];
+            vyh[p] = ryh[p] / roh[p];
+            vzh[p] = rzh[p] / roh[p];
+            double v2 = sq(vxh[p]) + sq(vyh[p]) + sq(vzh[p]);
+            double b2 = sq(bxh[p]) + sq(byh[p]) + sq(bzh[p]);
+            prh[p] = (gm - 1.0) * (eeh[p] - 0.5 * roh[p] * v2 - pi8i * b2);
+        }
+    const double *half[NW] = {roh, prh, vxh, vyh, vzh, bxh, byh, bzh};
+    /* MUSCL x (:158-181) */
+    for (int q = 0; q < NW; q++) tvd2(1, half[q], w0[q], w1[q], ix, jx);
+    roeflux23_arr(gm, w0, w1, VX, VY, VZ, BX, BY, BZ, fro, fee, frx, fry, frz, fby, fbz, ix, jx);
+    for (int j = 3; j <= jx - 2; j++)
+        for (int i = 3; i <= ix - 2; i++) {
+            size_t p = I2(i, j), m = I2(i - 1, j);
+            ro[p] = ro[p] + dt * ((fro[m] - fro[p]) / dx[I1(i)]);
+            ee[p] = ee[p] + dt * ((fee[m] - fee[p]) / dx[I1(i)]);
+            rx[p] = rx[p] + dt * ((frx[m] - frx[p]) / dx[I1(i)]);
+            ry[p] = ry[p] + dt * ((fry[m] - fry[p]) / dx[I1(i)]);
+            rz[p] = rz[p] + dt * ((frz[m] - frz[p]) / dx[I1(i)]);
+            by[p] = by[p] + dt * ((fby[m] - fby[p]) / dx[I1(i)]);
+            bz[p] = bz[p] + dt * ((fbz[m] - fbz[p]) / dx[I1(i)]);
+        }
+    /* MUSCL y (:185-210) */
+    for (int q = 0; q < NW; q++) tvd2(2, half[q], w0[q], w1[q], ix, jx);
+    roeflux23_arr(gm, w0, w1, VY, VX, VZ, BY, BX, BZ, fro, fee, fry, frx, frz, fbx, fbz, ix, jx);
+    for (int j = 3; j <= jx - 2; j++)
+        for (int i = 3; i <= ix - 2; i++) {
+            size_t p = I2(i, j), m = I2(i, j - 1);
+            ro[p] = ro[p] + dt * ((fro[m] - fro[p]) / dy[I1(j)]);
+            ee[p] = ee[p] + dt * ((fee[m] - fee[p]) / dy[I1(j)]);
+            rx[p] = rx[p] + dt * ((frx[m] - frx[p]) / dy[I1(j)]);
+            ry[p] = ry[p] + dt * ((fry[m] - fry[p]) / dy[I1(j)]);
+            rz[p] = rz[p] + dt * ((frz[m] - frz[p]) / dy[I1(j)]);
+            bx[p] = bx[p] + dt * ((fbx[m] - fbx[p]) / dy[I1(j)]);
+            bz[p] = bz[p] + dt * ((fbz[m] - fbz[p]) / dy[I1(j)]);
+        }
+    for (int j = 3; j <= jx - 2; j++) /* :214-220 */
+        for (int i = 3; i <= ix - 2; i++) {
+            size_t p = I2(i, j);
+            double ez = -vxh[p] * byh[p] + vyh[p] * bxh[p];
+            double saz = -ez;
+            az[p] = az[p] + dt * saz;
+        }
+    for (int j = 3; j <= jx - 2; j++) /* :222-232 */
+        for (int i = 3; i <= ix - 2; i++) {
+            size_t p = I2(i, j);
+            vx[p] = rx[p] / ro[p];
+            vy[p] = ry[p] / ro[p];
+            vz[p] = rz[p] / ro[p];
+            double v2 = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+            double b2 = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+            pr[p] = (gm - 1.0) * (ee[p] - 0.5 * ro[p] * v2 - pi8i * b2);
+        }
+    double *tofree[] = {ee, rx, ry, rz, fro, fee, frx, fry, frz, fbx, fby, fbz, roh, eeh, rxh, ryh, rzh,
+                        prh, vxh, vyh, vzh, bxh, byh, bzh};
+    for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
+    for (int q = 0; q < NW; q++) { free(w0[q]); free(w1[q]); }
+}
+
 /* ================================================================== 3D */
 /* cans3d/hdroe/tvdminmod.f:15-55 (argument order of flmt as in the 3D file) */
 static void tvd3(int mdir, const double *da, double *w1, double *w2, int ix, int jx, int kx) {
@@ -581,7 +822,7 @@ static void roeflux3_arr(double gm, double *const *w0, double *const *w1, int vn
             for (int i = 1; i <= ix - 1; i++) {
                 size_t p = I3(i, j, k);
                 double o[7];
-                roeflux3(gm, w0[W_RO][p], w0[W_PR][p], w0[vn][p], w0[v1][p], w0[v2][p], w0[bn][p], w0[b1][p], w0[b2][p],
+                roeflux3(1, gm, w0[W_RO][p], w0[W_PR][p], w0[vn][p], w0[v1][p], w0[v2][p], w0[bn][p], w0[b1][p], w0[b2][p],
                          w1[W_RO][p], w1[W_PR][p], w1[vn][p], w1[v1][p], w1[v2][p], w1[bn][p], w1[b1][p], w1[b2][p], o);
                 fro[p] = o[0]; fee[p] = o[1]; frn[p] = o[2]; fr1[p] = o[3]; fr2[p] = o[4]; fb1[p] = o[5]; fb2[p] = o[6];
             }

@@ -118,6 +118,15 @@ class Oracle:
                              *map(_arr, (bx, ro, pr, vx, vy, by, vz, bz, dx)), C.c_int(ro.shape[0]), C.byref(imin))
         return dt.value, merr.value, imin.value
 
+    def roe_m3_1d(self, ro, pr, vx, vy, vz, by, bz, bx, bxm, dt, gm, dx):
+        self.lib.or1d_roe_m3(*map(_arr, (ro, pr, vx, vy, vz, by, bz, bx, bxm)), C.c_double(dt), C.c_double(gm),
+                             _arr(dx), C.c_int(ro.shape[0]))
+
+    def roe_m3_2d(self, ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, dy):
+        ix, jx = ro.shape
+        self.lib.or2d_roe_m3(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz, az)), C.c_double(dt), C.c_double(gm),
+                             _arr(dx), C.c_int(ix), _arr(dy), C.c_int(jx))
+
     def mlw_m3_2d(self, ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, dy, dym):
         ix, jx = ro.shape
         self.lib.or2d_mlw_m3(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz, az)), C.c_double(dt), C.c_double(qav),

@@ -117,7 +117,7 @@ def test_spicule_main(dev, oracle, tmp_path, resident):
 
 
 def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
-    """mlw_m3_ and cfl_m3_ of libcans{1,2}d_gpu.so called as a Fortran main
+    """mlw_m3_, roe_m3_ and cfl_m3_ of libcans{1,2}d_gpu.so called as a Fortran main
     program calls them (every argument by reference, host arrays, strict
     mode), against the restatement."""
     import ctypes as C
@@ -154,6 +154,12 @@ def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
     call(lib2, "cfl_m3", dt, 0.4, 1e-10, merr, 5 / 3, *[s[k] for k in F[:8]], dx, ix, dy, jx)
     want = oracle.cfl_m3_2d(0.4, 1e-10, 5 / 3, *[s[k] for k in F[:8]], dx, dy)
     assert (dt.value, merr.value) == want[:2]
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.roe_m3_2d(*[a[k] for k in F], 9e-4, 5 / 3, dx, dy)
+    call(lib2, "roe_m3", *[b[k] for k in F], 9e-4, 5 / 3, dx, ix, dy, jx)
+    for k in F:   # Roe: the interior 4..n-3 (see tests/test_gpu_m3.py)
+        assert_bitwise(b[k][3:-3, 3:-3], a[k][3:-3, 3:-3], k)
     lib1 = C.CDLL(str(PKG_DIR / "libcans1d_gpu.so"))
     ix = 123
     s1 = state1(ix, 2)
@@ -165,3 +171,9 @@ def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
     call(lib1, "mlw_m3", *[b[k] for k in F1m], b["bx"], b["bxm"], 2e-3, 3.0, 5 / 3, dx, dxm, ix)
     for k in F1m:
         assert_bitwise(b[k], a[k], k)
+    a = {k: v.copy() for k, v in s1.items()}
+    b = {k: v.copy() for k, v in s1.items()}
+    oracle.roe_m3_1d(*[a[k] for k in F1m], a["bx"], a["bxm"], 1e-3, 5 / 3, dx)
+    call(lib1, "roe_m3", *[b[k] for k in F1m], b["bx"], b["bxm"], 1e-3, 5 / 3, dx, ix)
+    for k in F1m:
+        assert_bitwise(b[k][3:-3], a[k][3:-3], k)

@@ -133,3 +133,90 @@ def test_mlw_m3_1d_resident_loop(oracle, dev):
     assert info["nexec"] == nsteps and info["t"] == t
     for k, g in zip(F1, cur):
         assert_bitwise(g.cpu().numpy(), a[k], k)
+
+
+@pytest.mark.parametrize("ix", [9, 80, 1030])
+def test_roe_m3_1d_call(oracle, dev, ix):
+    print(f"roe_m3 1d ix={ix}", flush=True)
+    s = state1(ix, ix + 3)
+    _, dx, _ = stretched_grid(ix, seed=ix)
+    a = {k: v.copy() for k, v in s.items()}
+    oracle.roe_m3_1d(*[a[k] for k in F1], a["bx"], a["bxm"], 1.1e-3, 5 / 3, dx)
+    d = {k: T(v, dev) for k, v in s.items()}
+    cans.roe_m3_1d(*[d[k] for k in F1], d["bx"], d["bxm"], 1.1e-3, 5 / 3, T(dx, dev))
+    torch.cuda.synchronize()
+    # cells 3 and ix-2 read the half step of cells 1 and ix, which roe_m3.f
+    # never writes (margin >= 3 in its drivers): compare 4..ix-3
+    for k in F1:
+        assert_bitwise(d[k].cpu().numpy()[3:-3], a[k][3:-3], k)
+
+
+@pytest.mark.parametrize("ix,jx", [(9, 10), (40, 33), (130, 71)])
+def test_roe_m3_2d_call(oracle, dev, ix, jx):
+    print(f"roe_m3 2d {ix}x{jx}", flush=True)
+    s = state2(ix, jx, 2 * ix + jx)
+    _, dx, _ = stretched_grid(ix, seed=ix)
+    _, dy, _ = stretched_grid(jx, seed=jx + 2)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.roe_m3_2d(*[a[k] for k in F2], 9e-4, 5 / 3, dx, dy)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.roe_m3_2d(*[d[k] for k in F2], 9e-4, 5 / 3, T(dx, dev), T(dy, dev))
+    torch.cuda.synchronize()
+    for k in F2:   # as in 1D: the interior 4..n-3 in each direction
+        assert_bitwise(to_host(d[k])[3:-3, 3:-3], a[k][3:-3, 3:-3], k)
+
+
+def test_roe_m3_2d_resident_loop(oracle, dev):
+    """30 steps of {cfl_m3; t=t+dt; roe_m3; bdper x, y} (margin 3)."""
+    ix, jx, margin, nsteps = 50, 38, 3, 30
+    s = state2(ix, jx, 61)
+    _, dx, _ = stretched_grid(ix, seed=5)
+    _, dy, _ = stretched_grid(jx, seed=6)
+    periodic2(oracle, s, margin)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    t = 0.0
+    names = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")
+    for _ in range(nsteps):
+        dt, merr, _ = oracle.cfl_m3_2d(0.4, 1e-10, 5 / 3, *[a[k] for k in names], dx, dy)
+        t = t + dt
+        oracle.roe_m3_2d(*[a[k] for k in F2], dt, 5 / 3, dx, dy)
+        periodic2(oracle, a, margin)
+    st = [to_dev(s[k], dev) for k in F2]
+    grid = {"dx": T(dx, dev), "dy": T(dy, dev)}
+    run = cans.Run(2, cans.ENGINE_ROE3, st, grid, 5 / 3, 0.4, margin=margin, bc=7)
+    run.reset()
+    run.steps(nsteps)
+    cur, info = run.state()
+    torch.cuda.synchronize()
+    assert info["nexec"] == nsteps and info["t"] == t
+    for k, g in zip(F2, cur):
+        assert_bitwise(to_host(g), a[k], k)
+
+
+def test_roe_m3_1d_resident_loop(oracle, dev):
+    ix, margin, nsteps = 160, 3, 40
+    s = state1(ix, 33)
+    _, dx, dxm = stretched_grid(ix, seed=10)
+    for side in (0, 1):
+        for k in F1:
+            oracle.bnd_1d(0, side, margin, s[k])
+    a = {k: v.copy() for k, v in s.items()}
+    t = 0.0
+    for _ in range(nsteps):
+        dt, merr, _ = oracle.cfl_m3_1d(0.4, 1e-10, 5 / 3, a["bx"], a["ro"], a["pr"], a["vx"], a["vy"], a["by"], a["vz"],
+                                       a["bz"], dx)
+        t = t + dt
+        oracle.roe_m3_1d(*[a[k] for k in F1], a["bx"], a["bxm"], dt, 5 / 3, dx)
+        for side in (0, 1):
+            for k in F1:
+                oracle.bnd_1d(0, side, margin, a[k])
+    st = [T(s[k], dev) for k in F1]
+    run = cans.Run(1, cans.ENGINE_ROE3, st, {"dx": T(dx, dev), "dxm": T(dxm, dev)}, 5 / 3, 0.4, margin=margin, bc=0,
+                   bx1d=T(s["bx"], dev), bxm1d=T(s["bxm"], dev))
+    run.reset()
+    run.steps(nsteps)
+    cur, info = run.state()
+    torch.cuda.synchronize()
+    assert info["nexec"] == nsteps and info["t"] == t
+    for k, g in zip(F1, cur):
+        assert_bitwise(g.cpu().numpy(), a[k], k)

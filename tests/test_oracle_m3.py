@@ -102,3 +102,46 @@ def test_mlw_m3_2d_reduces_to_mlw_m(oracle):
     for k in ("ro", "pr", "vx", "vy", "bx", "by", "az"):
         assert_bitwise(a[k][1:-1, 1:-1], b[k][1:-1, 1:-1], k)
     assert np.all(a["vz"] == 0.0) and np.all(a["bz"] == 0.0)
+
+
+@pytest.mark.parametrize("ix", [9, 80, 257])
+def test_roe_m3_1d_vs_ref(oracle, ref1d, ix):
+    s = state1(ix, ix + 3)
+    _, dx, _ = stretched_grid(ix, seed=ix)
+    a = {k: v.copy() for k, v in s.items()}
+    b = {k: v.copy() for k, v in s.items()}
+    oracle.roe_m3_1d(*[a[k] for k in F1], a["bx"], a["bxm"], 1.1e-3, 5 / 3, dx)
+    ref1d.call("roe_m3", *[b[k] for k in F1], b["bx"], b["bxm"], 1.1e-3, 5 / 3, dx, ix)
+    for k in F1:
+        assert_bitwise(a[k][3:-3], b[k][3:-3], k)
+
+
+@pytest.mark.parametrize("ix,jx", [(9, 10), (40, 33), (70, 51)])
+def test_roe_m3_2d_vs_ref(oracle, ref2d, ix, jx):
+    s = state2(ix, jx, 2 * ix + jx)
+    _, dx, _ = stretched_grid(ix, seed=ix)
+    _, dy, _ = stretched_grid(jx, seed=jx + 2)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.roe_m3_2d(*[a[k] for k in F2], 9e-4, 5 / 3, dx, dy)
+    ref2d.call("roe_m3", *[b[k] for k in F2], 9e-4, 5 / 3, dx, ix, dy, jx)
+    for k in F2:
+        assert_bitwise(a[k][3:-3, 3:-3], b[k][3:-3, 3:-3], k)
+
+
+def test_roe_m3_2d_reduces_to_roe_m(oracle):
+    """vz = bz = 0: roeflux_m3 without the alphs fix is roeflux_m with zero
+    third components, so roe_m3 equals roe_m (2D) bit for bit; and the 2D
+    y sweep's (y, x, z) frame is pinned against roe_m's (y, x) one."""
+    ix, jx = 36, 30
+    s = state2(ix, jx, 8)
+    s["vz"][:] = 0.0
+    s["bz"][:] = 0.0
+    _, dx, _ = stretched_grid(ix, seed=2)
+    _, dy, _ = stretched_grid(jx, seed=3)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.roe_m3_2d(*[a[k] for k in F2], 9e-4, 5 / 3, dx, dy)
+    oracle.roe_m_2d(*[b[k] for k in ("ro", "pr", "vx", "vy", "bx", "by", "az")], 9e-4, 5 / 3, dx, dy)
+    for k in ("ro", "pr", "vx", "vy", "bx", "by", "az"):
+        assert_bitwise(a[k][3:-3, 3:-3], b[k][3:-3, 3:-3], k)

@@ -355,10 +355,10 @@ def hlld_engine_3d_mixed(ro, pr, vx, vy, vz, bx, by, bz, gm, mu, dt, dx, dy, dz,
                                           C.c_int(ix), C.c_int(jx), C.c_int(kx), mf, _stream()))
 
 
-ENGINE_MLW, ENGINE_ROE, ENGINE_HLLD, ENGINE_MLW3 = 0, 1, 2, 3
+ENGINE_MLW, ENGINE_ROE, ENGINE_HLLD, ENGINE_MLW3, ENGINE_ROE3 = 0, 1, 2, 3, 4
 FIELDS = {1: ("ro", "pr", "vx", "vy", "by"), 2: ("ro", "pr", "vx", "vy", "bx", "by", "az"),
           3: ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")}
-# fields of the three-component engine mlw_m3 (ENGINE_MLW3)
+# fields of the three-component engines mlw_m3 / roe_m3 (ENGINE_MLW3, ENGINE_ROE3)
 FIELDS_M3 = {1: ("ro", "pr", "vx", "vy", "vz", "by", "bz"), 2: ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz", "az")}
 
 
@@ -379,6 +379,22 @@ def cfl_m3_1d(safety, dtmin, gm, bx, ro, pr, vx, vy, by, vz, bz, dx):
     _check(lib().cans1d_cfl_m3(C.byref(dt), _d(safety), _d(dtmin), C.byref(merr), _d(gm),
                                *[_ptr(t, sh) for t in (bx, ro, pr, vx, vy, by, vz, bz, dx)], C.c_int(ix), _stream()))
     return dt.value, merr.value
+
+
+def roe_m3_1d(ro, pr, vx, vy, vz, by, bz, bx, bxm, dt, gm, dx):
+    """cans1d/hdroe/roe_m3.f:2  (in place on i=3..ix-2)."""
+    ix = ro.shape[0]
+    sh = (ix,)
+    _check(lib().cans1d_roe_m3(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, by, bz, bx, bxm)], _d(dt), _d(gm),
+                               _ptr(dx, sh), C.c_int(ix), _stream()))
+
+
+def roe_m3_2d(ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, dy):
+    """cans2d/hdroe/roe_m3.f:2  (in place on i=3..ix-2, j=3..jx-2)."""
+    jx, ix = ro.shape
+    sh = (jx, ix)
+    _check(lib().cans2d_roe_m3(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz, az)], _d(dt), _d(gm),
+                               _ptr(dx, (ix,)), C.c_int(ix), _ptr(dy, (jx,)), C.c_int(jx), _stream()))
 
 
 def mlw_m3_2d(ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, dy, dym):
@@ -414,7 +430,7 @@ class Run:
     def __init__(self, dim, engine, state, grid, gm, safety, qav=3.0, mu=1.0, margin=1, bc=None, mixed=False,
                  bx1d=None, bxm1d=None, clock=None):
         self.dim, self.engine, self.mixed = dim, engine, mixed
-        self.nf = len(FIELDS_M3[dim] if engine == ENGINE_MLW3 else FIELDS[dim])
+        self.nf = len(FIELDS_M3[dim] if engine in (ENGINE_MLW3, ENGINE_ROE3) else FIELDS[dim])
         assert len(state) == self.nf, f"{dim}D needs {self.nf} fields"
         self.a = list(state)
         self.b = [torch.empty_like(t) for t in state]

@@ -55,14 +55,16 @@ size_t engine_doubles(int dim, int engine, size_t N) {
     const size_t a = al(N);
     switch (engine) {
     case CANS_ENGINE_MLW: return dim == 3 ? (size_t)(kMlw3dCellFields + kMlw3dCornerFields) * a : 0;
-    case CANS_ENGINE_ROE: return (8 + 3 * 8) * a;
+    case CANS_ENGINE_ROE:
+    case CANS_ENGINE_ROE3: return (8 + 3 * 8) * a;
     case CANS_ENGINE_HLLD: return 4 * 8 * a;
     default: return 0;
     }
 }
 
-void roe_scratch(RoeArgs &r, Carver &c, size_t N, int dim) {
-    const bool used[8] = {true, true, true, true, dim == 3, dim >= 2, true, dim == 3};
+void roe_scratch(RoeArgs &r, Carver &c, size_t N, int dim, bool comp3 = false) {
+    const bool z = dim == 3 || comp3;
+    const bool used[8] = {true, true, true, true, z, dim >= 2, true, z};
     for (int q = 0; q < 8; q++) r.half[q] = used[q] ? c.take(N) : (c.take(N), nullptr);
     for (int d = 0; d < 3; d++)
         for (int q = 0; q < 8; q++) r.flux[d][q] = c.take(N);
@@ -87,7 +89,7 @@ extern "C" {
 // ====================================================================== Roe drop-in
 static int roe_dropin(int dim, bool mixed, double *const *prim, double *az, const double *bx, const double *bxm,
                       double dt, double gm, const double *dx, int ix, const double *dy, int jx, const double *dz,
-                      int kx, void *stream) {
+                      int kx, void *stream, bool comp3 = false) {
     const int n[3] = {ix, jx, kx};
     for (int d = 0; d < dim; d++)
         if (int rc = check_margin("roe_m", n[d], 5)) return rc;
@@ -98,14 +100,14 @@ static int roe_dropin(int dim, bool mixed, double *const *prim, double *az, cons
     Carver c{(double *)w};
     RoeArgs r{};
     for (int q = 0; q < 8; q++) r.prim[q] = prim[q];
-    roe_scratch(r, c, N, dim);
+    roe_scratch(r, c, N, dim, comp3);
     double *outbuf[8];
     for (int q = 0; q < 8; q++) { outbuf[q] = c.take(N); r.out[q] = prim[q] ? outbuf[q] : nullptr; }
     double *azo = c.take(N);
     r.az = az; r.out_az = dim == 2 ? azo : nullptr;
     r.bx = bx; r.bxm = bxm; r.dx = dx; r.dy = dy; r.dz = dz;
     r.gm = gm; r.dt = dt; r.clk = nullptr; r.ix = ix; r.jx = jx; r.kx = kx;
-    HIPCHK(launch_roe(r, dim, mixed, s));
+    HIPCHK(launch_roe(r, dim, mixed, s, comp3));
     double *dst[9];
     const double *src[9];
     int nf = 0;
@@ -126,6 +128,20 @@ int cans2d_roe_m(double *ro, double *pr, double *vx, double *vy, double *bx, dou
                  double dt, double gm, const double *dx, int ix, const double *dy, int jx, void *stream) {
     double *prim[8] = {ro, pr, vx, vy, nullptr, bx, by, nullptr};
     return roe_dropin(2, false, prim, az, nullptr, nullptr, dt, gm, dx, ix, dy, jx, nullptr, 1, stream);
+}
+
+int cans1d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *by, double *bz,
+                  const double *bx, const double *bxm, double dt, double gm, const double *dx, int ix,
+                  void *stream) {
+    double *prim[8] = {ro, pr, vx, vy, vz, nullptr, by, bz};
+    return roe_dropin(1, false, prim, nullptr, bx, bxm, dt, gm, dx, ix, nullptr, 1, nullptr, 1, stream, true);
+}
+
+int cans2d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by, double *bz,
+                  double *az, double dt, double gm, const double *dx, int ix, const double *dy, int jx,
+                  void *stream) {
+    double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
+    return roe_dropin(2, false, prim, az, nullptr, nullptr, dt, gm, dx, ix, dy, jx, nullptr, 1, stream, true);
 }
 
 int cans3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
@@ -263,7 +279,7 @@ size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx) {
 // bnd} for any dimension/engine with generic boundaries.  State set a holds
 // step n (n even), b step n+1, alternating.  nf fields per set:
 // 1D 5 (ro pr vx vy by), 2D 7 (ro pr vx vy bx by az), 3D 8 (ro pr vx vy vz bx by bz);
-// mlw_m3: 1D 7 (ro pr vx vy vz by bz), 2D 9 (ro pr vx vy vz bx by bz az).
+// mlw_m3 and roe_m3: 1D 7 (ro pr vx vy vz by bz), 2D 9 (ro pr vx vy vz bx by bz az).
 int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
              const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
              double safety, const double *dx, const double *dxm, int ix, const double *dy, const double *dym,
@@ -272,8 +288,9 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     if (dim < 1 || dim > 3) return abi_fail(CANS_ERR_ARG, "cans_run: dim %d", dim);
     if (!a || !b || !clk || !ws) return abi_fail(CANS_ERR_ARG, "cans_run: NULL argument");
     if (engine == CANS_ENGINE_HLLD && dim != 3) return abi_fail(CANS_ERR_ARG, "cans_run: HLLD is 3D");
-    if (engine == CANS_ENGINE_MLW3 && dim == 3) return abi_fail(CANS_ERR_ARG, "cans_run: mlw_m3 is 1D/2D");
-    if (engine < CANS_ENGINE_MLW || engine > CANS_ENGINE_MLW3) return abi_fail(CANS_ERR_ARG, "cans_run: engine %d", engine);
+    if ((engine == CANS_ENGINE_MLW3 || engine == CANS_ENGINE_ROE3) && dim == 3)
+        return abi_fail(CANS_ERR_ARG, "cans_run: the three-component engines are 1D/2D");
+    if (engine < CANS_ENGINE_MLW || engine > CANS_ENGINE_ROE3) return abi_fail(CANS_ERR_ARG, "cans_run: engine %d", engine);
     if (engine == CANS_ENGINE_MLW && dim == 2 && !bc)
         return cans2d_mlw_run(a, b, clk, step0, nsteps, qav, gm, safety, dx, dxm, ix, dy, dym, jx, nullptr, margin,
                               (double *)ws, events, stream);
@@ -281,8 +298,9 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     if (dim == 1) { jx = 1; kx = 1; }
     if (dim == 2) kx = 1;
     const bool m3 = engine == CANS_ENGINE_MLW3;
-    const int nf = m3 ? (dim == 1 ? 7 : 9) : (dim == 1 ? 5 : (dim == 2 ? 7 : 8));
-    const int lo = (engine == CANS_ENGINE_MLW || m3) ? 3 : 5;
+    const bool c3 = m3 || engine == CANS_ENGINE_ROE3;      // three-component field lists
+    const int nf = c3 ? (dim == 1 ? 7 : 9) : (dim == 1 ? 5 : (dim == 2 ? 7 : 8));
+    const int lo = (engine == CANS_ENGINE_MLW || m3) ? 3 : 5;   // Roe: margin >= 3 (see the drop-in)
     const int n[3] = {ix, jx, kx};
     for (int d = 0; d < dim; d++)
         if (n[d] < lo || n[d] < 2 * margin + 1) return abi_fail(CANS_ERR_ARG, "cans_run: dimension %d too small", n[d]);
@@ -303,7 +321,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         double *const *out = (st & 1) ? a : b;
         // map the field list onto the engine slots (ro pr vx vy vz bx by bz)
         double *pin[8] = {nullptr}, *pout[8] = {nullptr};
-        if (m3) {   // ro pr vx vy vz (by bz) or (bx by bz), az
+        if (c3) {   // ro pr vx vy vz (by bz) or (bx by bz), az
             const int map1[7] = {0, 1, 2, 3, 4, 6, 7};
             for (int f = 0; f < (dim == 1 ? 7 : 8); f++) {
                 const int slot = dim == 1 ? map1[f] : f;
@@ -325,7 +343,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         ca.ro = pin[0]; ca.pr = pin[1]; ca.vx = pin[2]; ca.vy = pin[3]; ca.vz = pin[4];
         ca.bx = dim == 1 ? bx1d : pin[5]; ca.by = pin[6]; ca.bz = pin[7];
         ca.dx = dx; ca.dy = dy; ca.dz = dz; ca.safety = safety; ca.gm = gm;
-        ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st; ca.comp3 = m3 ? 1 : 0;
+        ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st; ca.comp3 = c3 ? 1 : 0;
         HIPCHK(launch_cfl(ca, s));
         HIPCHK(launch_clock_step(c, st, s));
         const int e = st - step0;
@@ -364,16 +382,17 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
                 g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx; g.kx = kx;
                 HIPCHK(launch_mlw3d(g, s));
             }
-        } else if (engine == CANS_ENGINE_ROE) {
+        } else if (engine == CANS_ENGINE_ROE || engine == CANS_ENGINE_ROE3) {
             RoeArgs r{};
             for (int q = 0; q < 8; q++) { r.prim[q] = pin[q]; r.out[q] = pout[q]; }
             Carver ec{eng};
-            roe_scratch(r, ec, N, dim);
-            r.az = dim == 2 ? in[6] : nullptr;
-            r.out_az = dim == 2 ? out[6] : nullptr;
+            roe_scratch(r, ec, N, dim, c3);
+            const int faz = c3 ? 8 : 6;            // Az's place in the 2D field list
+            r.az = dim == 2 ? in[faz] : nullptr;
+            r.out_az = dim == 2 ? out[faz] : nullptr;
             r.bx = bx1d; r.bxm = bxm1d; r.dx = dx; r.dy = dy; r.dz = dz;
             r.gm = gm; r.clk = c; r.step = st; r.ix = ix; r.jx = jx; r.kx = kx;
-            HIPCHK(launch_roe(r, dim, mixed != 0, s));
+            HIPCHK(launch_roe(r, dim, mixed != 0, s, c3));
         } else {
             HlldArgs h{};
             for (int q = 0; q < 8; q++) { h.prim[q] = pin[q]; h.out[q] = pout[q]; }

@@ -11,6 +11,9 @@
 //
 // Fields: U  = ro, ee, rx, ry, rz, bx, by, bz (8; 2D: no rz/bz; 1D: ro ee rx ry by)
 //         P  = ro, pr, vx, vy, vz, bx, by, bz (primitives, same slots)
+// C3 (1D/2D): roe_m3 (cans{1,2}d/hdroe/roe_m3.f), the same steps with the
+// z components and the roeflux_m3 flux; its 2D y sweep uses the frame
+// (y, x, z) (roe_m3.f:124-125), not the cyclic (y, z, x) of 3D.
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_kernels.hpp"
@@ -52,8 +55,9 @@ CANS_DEV bool step_dt(const RoeArgs &a, double &dt) {
 // minmod states from the half-step primitives Ph.  The face range is what
 // the following cell kernel reads: along dir [lo_n, n-1 or n-2], across
 // [lo_t, n+1-lo_t].
-template <int D, class T>
+template <int D, class T, bool C3>
 __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int stage) {
+    constexpr bool FULL = D == 3 || C3;      // all three components present
     double dtv;
     if (!step_dt(a, dtv)) return;
     const int ix = a.ix, jx = a.jx, kx = a.kx;
@@ -77,17 +81,18 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
     const size_t s = dir == 0 ? 1 : (dir == 1 ? (size_t)ix : (size_t)ix * jx);
     const double *const *P = stage == 1 ? a.prim : a.half;
     // left / right states of the needed slots
-    constexpr int NS = (D == 3) ? 8 : (D == 2 ? 6 : 5);
-    // slots used: 1D: ro pr vx vy by ; 2D: + bx ; 3D: all
+    constexpr int NS = FULL ? (D == 1 ? 7 : 8) : (D == 2 ? 6 : 5);
+    // slots used: 1D: ro pr vx vy by ; 2D: + bx ; 3D: all ; 1D C3: all but bx
     const int slots3[8] = {S_RO, S_E, S_X, S_Y, S_Z, S_BX, S_BY, S_BZ};
     const int slots2[6] = {S_RO, S_E, S_X, S_Y, S_BX, S_BY};
     const int slots1[5] = {S_RO, S_E, S_X, S_Y, S_BY};
+    const int slots13[7] = {S_RO, S_E, S_X, S_Y, S_Z, S_BY, S_BZ};
     T L[8], R[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) { L[q] = T(0); R[q] = T(0); }
 #pragma unroll
     for (int m = 0; m < NS; m++) {
-        const int q = D == 3 ? slots3[m] : (D == 2 ? slots2[m] : slots1[m]);
+        const int q = FULL ? (D == 1 ? slots13[m] : slots3[m]) : (D == 2 ? slots2[m] : slots1[m]);
         const double *da = P[q];
         if (stage == 1) {
             L[q] = T(da[p]);
@@ -118,6 +123,16 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
         F[S_RO][p] = double(o[0]); F[S_E][p] = double(o[1]);
         F[vn][p] = double(o[2]); F[v1][p] = double(o[3]); F[v2][p] = double(o[4]);
         F[b1][p] = double(o[5]); F[b2][p] = double(o[6]);
+    } else if (C3) {
+        // roe_m3.f: x = (x, y, z); 2D y = (y, x, z)
+        const int vn = dir == 0 ? S_X : S_Y, v1 = dir == 0 ? S_Y : S_X;
+        const int bn = dir == 0 ? S_BX : S_BY, b1 = dir == 0 ? S_BY : S_BX;
+        T o[7];
+        roeflux3<T, false>(gm, L[S_RO], L[S_E], L[vn], L[v1], L[S_Z], L[bn], L[b1], L[S_BZ], R[S_RO], R[S_E], R[vn],
+                           R[v1], R[S_Z], R[bn], R[b1], R[S_BZ], o);
+        F[S_RO][p] = double(o[0]); F[S_E][p] = double(o[1]);
+        F[vn][p] = double(o[2]); F[v1][p] = double(o[3]); F[S_Z][p] = double(o[4]);
+        F[b1][p] = double(o[5]); F[S_BZ][p] = double(o[6]);
     } else {
         // 2D x: (vx,vy,bx,by) -> ro ee rx ry by ; 2D y: (vy,vx,by,bx) -> ro ee ry rx bx ; 1D as 2D x
         const int vn = dir == 0 ? S_X : S_Y, vt = dir == 0 ? S_Y : S_X;
@@ -131,7 +146,7 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
 
 // conserved variables of a cell from the primitives, in the reference's
 // expressions (roe_m.f: rx=ro*vx, ee=pr/(gm-1)+0.5*ro*v2+pi8i*b2)
-template <int D>
+template <int D, bool C3>
 CANS_DEV void prim_to_cons(const RoeArgs &a, size_t p, int i, double *U) {
     const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
     const double gm = a.gm;
@@ -139,8 +154,9 @@ CANS_DEV void prim_to_cons(const RoeArgs &a, size_t p, int i, double *U) {
     U[S_RO] = ro;
     U[S_X] = ro * vx;
     U[S_Y] = ro * vy;
-    if (D == 3) {
-        const double vz = a.prim[S_Z][p], bx = a.prim[S_BX][p], by = a.prim[S_BY][p], bz = a.prim[S_BZ][p];
+    if (D == 3 || C3) {
+        const double vz = a.prim[S_Z][p], by = a.prim[S_BY][p], bz = a.prim[S_BZ][p];
+        const double bx = D == 1 ? a.bx[i - 1] : a.prim[S_BX][p];
         U[S_Z] = ro * vz;
         const double v2 = vx * vx + vy * vy + vz * vz;
         const double b2 = bx * bx + by * by + bz * bz;
@@ -160,8 +176,9 @@ CANS_DEV void prim_to_cons(const RoeArgs &a, size_t p, int i, double *U) {
 // (each sweep a separate rounding, as roe_m.f); skip the face-normal field.
 // Cells on the outer layer keep their primitives (the reference leaves the
 // half-step there unset; it only reaches ghost cells when margin >= 3).
-template <int D>
+template <int D, bool C3>
 __global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
+    constexpr bool FULL = D == 3 || C3;
     double dt;
     if (!step_dt(a, dt)) return;
     const int ix = a.ix, jx = a.jx, kx = a.kx;
@@ -182,7 +199,7 @@ __global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
         return;
     }
     double U[8];
-    prim_to_cons<D>(a, p, i, U);
+    prim_to_cons<D, C3>(a, p, i, U);
     const double *dd[3] = {a.dx, a.dy, a.dz};
     for (int d = 0; d < D; d++) {
         const size_t s = d == 0 ? 1 : (d == 1 ? (size_t)ix : (size_t)ix * jx);
@@ -191,7 +208,7 @@ __global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
         const int skipB = D == 1 ? S_BX : S_BX + d;    // no normal-field update
         for (int q = 0; q < 8; q++) {
             if (q == skipB || !a.prim[q] || (D == 1 && q == S_BX)) continue;
-            if (D < 3 && (q == S_Z || q == S_BZ)) continue;
+            if (!FULL && (q == S_Z || q == S_BZ)) continue;
             U[q] = U[q] + 0.5 * dt * ((F[q][p - s] - F[q][p]) / dl);
         }
     }
@@ -202,7 +219,7 @@ __global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
     const double vx = U[S_X] / ro;
     const double vy = U[S_Y] / ro;
     double v2, b2, vz = 0.0;
-    if (D == 3) {
+    if (FULL) {
         vz = U[S_Z] / ro;
         v2 = vx * vx + vy * vy + vz * vz;
         b2 = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY] + U[S_BZ] * U[S_BZ];
@@ -216,14 +233,15 @@ __global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
     a.half[S_Y][p] = vy;
     a.half[S_BY][p] = U[S_BY];
     if (D >= 2) a.half[S_BX][p] = U[S_BX];
-    if (D == 3) { a.half[S_Z][p] = vz; a.half[S_BZ][p] = U[S_BZ]; }
+    if (FULL) { a.half[S_Z][p] = vz; a.half[S_BZ][p] = U[S_BZ]; }
 }
 
 // ------------------------------------------------------------------ corrector
 // cells [3,n-2]^D: U + dt (F(i-1)-F(i))/dx per sweep, then (2D) the Az
 // source from the half-step state, then primitives -> out arrays.
-template <int D>
+template <int D, bool C3>
 __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
+    constexpr bool FULL = D == 3 || C3;
     double dt;
     if (!step_dt(a, dt)) return;
     const int ix = a.ix, jx = a.jx, kx = a.kx;
@@ -239,7 +257,7 @@ __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
     const int idx[3] = {i, j, k};
     const size_t p = lin(i, j, k, ix, jx);
     double U[8];
-    prim_to_cons<D>(a, p, i, U);
+    prim_to_cons<D, C3>(a, p, i, U);
     const double *dd[3] = {a.dx, a.dy, a.dz};
     for (int d = 0; d < D; d++) {
         const size_t s = d == 0 ? 1 : (d == 1 ? (size_t)ix : (size_t)ix * jx);
@@ -248,7 +266,7 @@ __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
         const int skipB = S_BX + d;
         for (int q = 0; q < 8; q++) {
             if (q == skipB || (D == 1 && q == S_BX)) continue;
-            if (D < 3 && (q == S_Z || q == S_BZ)) continue;
+            if (!FULL && (q == S_Z || q == S_BZ)) continue;
             if (D == 1 && q == S_BX) continue;
             U[q] = U[q] + dt * ((F[q][p - s] - F[q][p]) / dl);
         }
@@ -265,7 +283,7 @@ __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
     const double vx = U[S_X] / ro;
     const double vy = U[S_Y] / ro;
     double v2, b2;
-    if (D == 3) {
+    if (FULL) {
         const double vz = U[S_Z] / ro;
         v2 = vx * vx + vy * vy + vz * vz;
         b2 = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY] + U[S_BZ] * U[S_BZ];
@@ -284,7 +302,7 @@ __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
     if (D >= 2) a.out[S_BX][p] = U[S_BX];
 }
 
-template <int D, class T>
+template <int D, class T, bool C3>
 hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
     const int n[3] = {a.ix, a.jx, a.kx};
     const size_t N = (size_t)a.ix * a.jx * a.kx;
@@ -298,25 +316,27 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                 total *= (size_t)(c > 0 ? c : 0);
             }
             if (total == 0) continue;
-            hipLaunchKernelGGL((roe_flux_kernel<D, T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a,
-                               dir, stage);
+            hipLaunchKernelGGL((roe_flux_kernel<D, T, C3>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                               a, dir, stage);
         }
         if (stage == 1)
-            hipLaunchKernelGGL(roe_half_kernel<D>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((roe_half_kernel<D, C3>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
     }
     size_t cells = 1;
     for (int d = 0; d < D; d++) cells *= (size_t)(n[d] - 4 > 0 ? n[d] - 4 : 0);
     if (cells)
-        hipLaunchKernelGGL(roe_update_kernel<D>, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((roe_update_kernel<D, C3>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s) {
-    if (dim == 1) return mixed ? launch_roe_t<1, float>(a, s) : launch_roe_t<1, double>(a, s);
-    if (dim == 2) return mixed ? launch_roe_t<2, float>(a, s) : launch_roe_t<2, double>(a, s);
-    return mixed ? launch_roe_t<3, float>(a, s) : launch_roe_t<3, double>(a, s);
+hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s, bool comp3) {
+    if (dim == 1 && comp3) return mixed ? launch_roe_t<1, float, true>(a, s) : launch_roe_t<1, double, true>(a, s);
+    if (dim == 2 && comp3) return mixed ? launch_roe_t<2, float, true>(a, s) : launch_roe_t<2, double, true>(a, s);
+    if (dim == 1) return mixed ? launch_roe_t<1, float, false>(a, s) : launch_roe_t<1, double, false>(a, s);
+    if (dim == 2) return mixed ? launch_roe_t<2, float, false>(a, s) : launch_roe_t<2, double, false>(a, s);
+    return mixed ? launch_roe_t<3, float, false>(a, s) : launch_roe_t<3, double, false>(a, s);
 }
 
 }  // namespace cans

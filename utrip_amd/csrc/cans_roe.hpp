@@ -23,7 +23,8 @@ struct RoeArgs {
     int step;
     int ix, jx, kx;
 };
-hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s);
+// comp3: roe_m3 (1D/2D with vz, bz in slots S_Z, S_BZ)
+hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s, bool comp3 = false);
 
 // 3D modified Lax-Wendroff (cans3d/hdmlw/mlw_m.f), three kernels with
 // global scratch: cell quantities, corner half-step fluxes, update.

@@ -4,7 +4,8 @@
 //
 //   roeflux12: cans1d/hdroe/roeflux_m.f:48-242 (identical body in cans2d)
 //   roeflux3 : cans3d/hdroe/roeflux_m.f:33-221 (with vz, bz and the 2006
-//              "+1-sp2" term in alphs, :79)
+//              "+1-sp2" term in alphs, :79); without that term it is
+//              roeflux_m3 of cans1d/cans2d (cans1d/hdroe/roeflux_m3.f)
 // Every expression keeps the reference's association order.
 #pragma once
 #include "cans_common.hpp"
@@ -165,8 +166,10 @@ CANS_DEV void roeflux12(T gm, T rhol, T prl, T vxl, T vyl, T bxl, T byl, T rhor,
     out[4] = half * (fluxlby + fluxrby + ef * rpfby + mf * rmfby + es * rpsby + ms * rmsby + ea * rpaby + ma * rmaby + ez * rzeby);
 }
 
-// out: fro, fee, frx(normal), fry, frz (tangential), fby, fbz (tangential B)
-template <class T>
+// out: fro, fee, frx(normal), fry, frz (tangential), fby, fbz (tangential B).
+// FIX2006 = false is roeflux_m3 of cans1d/cans2d (cans1d/hdroe/roeflux_m3.f:
+// 52-285, the same body without the "+1-sp2" term in alphs, :124).
+template <class T, bool FIX2006 = true>
 CANS_DEV void roeflux3(T gm, T rhol, T prl, T vxl, T vyl, T vzl, T bxl, T byl, T bzl, T rhor, T prr, T vxr, T vyr,
                        T vzr, T bxr, T byr, T bzr, T *out) {
     const T pi4 = T(RoePi<T>::pi4d), pi4i = T(RoePi<T>::pi4id), pi8i = T(RoePi<T>::pi8id);
@@ -214,7 +217,7 @@ CANS_DEV void roeflux3(T gm, T rhol, T prl, T vxl, T vyl, T vzl, T bxl, T byl, T
     const T cfcs = fmaxF(zero, cfast2 - cslow2);
     const T cfa = fmaxF(zero, cfast2 - cs2);
     const T alphf = sp2 * tsqrt(cfca / (cfcs + one - sp2)) + one - sp2;
-    const T alphs = sp2 * tsqrt(cfa / (cfcs + one - sp2)) + one - sp2;
+    const T alphs = FIX2006 ? sp2 * tsqrt(cfa / (cfcs + one - sp2)) + one - sp2 : sp2 * tsqrt(cfa / (cfcs + one - sp2));
     const T sgnbx = signF(one, bxbar);
     const T eeps = (vxr - vxl + tabs(vxr - vxl)) * T(2.5e-1);
     const T elpf = -fmaxF(tabs(vxbar + cfast), eeps);

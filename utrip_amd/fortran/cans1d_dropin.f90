@@ -2,7 +2,8 @@
 ! libcans1d_gpu -- the cans1d hot-path routines with the reference's names
 ! and argument lists (cans1d/hdmlw/mlw_m.f:2, cans1d/hdroe/roe_m.f:2,
 ! cans1d/common/cfl_m.f:2, cans1d/bc/bd*x.f, cans1d/hdmlw/mlw_m3.f:2,
-! cans1d/common/cfl_m3.f:2, and for the UTRIP spicule model
+! cans1d/hdroe/roe_m3.f:2, cans1d/common/cfl_m3.f:2, and for the UTRIP
+! spicule model
 ! cans1d/hdmlw/mlw_m_bg.f:2, cans1d/common/chkdav.f:2,
 ! cans1d/md_spicule/hdmlw/trqalfven.f:2), computed on the GPU by
 ! libcans_gpu.so.  A cans1d main program links this library instead of
@@ -119,6 +120,20 @@ subroutine mlw_m3(ro, pr, vx, vy, vz, by, bz, bx, bxm, dt, qav, gm, dx, dxm, ix)
   call dunmap(by, 1); call dunmap(bz, 1)
   call dunmap(bx, 0); call dunmap(bxm, 0); call dunmap(dx, 0); call dunmap(dxm, 0)
 end subroutine mlw_m3
+
+subroutine roe_m3(ro, pr, vx, vy, vz, by, bz, bx, bxm, dt, gm, dx, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: ro(ix), pr(ix), vx(ix), vy(ix), vz(ix), by(ix), bz(ix)
+  double precision, target, intent(in) :: bx(ix), bxm(ix), dx(ix)
+  double precision, intent(in) :: dt, gm
+  call check(c_roe1d_m3(dmap(ro, ix), dmap(pr, ix), dmap(vx, ix), dmap(vy, ix), dmap(vz, ix), dmap(by, ix), &
+       dmap(bz, ix), dmap(bx, ix), dmap(bxm, ix), dt, gm, dmap(dx, ix), int(ix, c_int), c_null_ptr), 'roe_m3')
+  call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vy, 1); call dunmap(vz, 1)
+  call dunmap(by, 1); call dunmap(bz, 1)
+  call dunmap(bx, 0); call dunmap(bxm, 0); call dunmap(dx, 0)
+end subroutine roe_m3
 
 subroutine cfl_m3(dt, safety, dtmin, merr, gm, bx, ro, pr, vx, vy, by, vz, bz, dx, ix)
   use cans_gpu_iface

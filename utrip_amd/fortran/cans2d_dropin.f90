@@ -1,7 +1,7 @@
 !=======================================================================
 ! libcans2d_gpu -- the cans2d hot-path routines with the reference's names
 ! and argument lists (cans2d/hdmlw/mlw_m.f:2-3, cans2d/hdroe/roe_m.f:2,
-! cans2d/common/cfl_m.f:2-3, cans2d/hdmlw/mlw_m3.f:2-3,
+! cans2d/common/cfl_m.f:2-3, cans2d/hdmlw/mlw_m3.f:2-3, cans2d/hdroe/roe_m3.f:2,
 ! cans2d/common/cfl_m3.f:2-3, cans2d/bc/bd*{x,y}.f), computed on the GPU by
 ! libcans_gpu.so.
 !=======================================================================
@@ -39,6 +39,24 @@ subroutine mlw_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, ix, 
   call dunmap(bx, 1); call dunmap(by, 1); call dunmap(bz, 1); call dunmap(az, 1)
   call dunmap(dx, 0); call dunmap(dxm, 0); call dunmap(dy, 0); call dunmap(dym, 0)
 end subroutine mlw_m3
+
+subroutine roe_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, ix, dy, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target :: ro(ix, jx), pr(ix, jx), vx(ix, jx), vy(ix, jx), vz(ix, jx)
+  double precision, target :: bx(ix, jx), by(ix, jx), bz(ix, jx), az(ix, jx)
+  double precision, target, intent(in) :: dx(ix), dy(jx)
+  double precision, intent(in) :: dt, gm
+  integer :: n
+  n = ix * jx
+  call check(c_roe2d_m3(dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vy, n), dmap(vz, n), dmap(bx, n), dmap(by, n), &
+       dmap(bz, n), dmap(az, n), dt, gm, dmap(dx, ix), int(ix, c_int), dmap(dy, jx), int(jx, c_int), c_null_ptr), &
+       'roe_m3')
+  call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vy, 1); call dunmap(vz, 1)
+  call dunmap(bx, 1); call dunmap(by, 1); call dunmap(bz, 1); call dunmap(az, 1)
+  call dunmap(dx, 0); call dunmap(dy, 0)
+end subroutine roe_m3
 
 subroutine cfl_m3(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, vz, bx, by, bz, dx, ix, dy, jx)
   use cans_gpu_iface

@@ -104,6 +104,13 @@ module cans_gpu_iface
        real(c_double), value :: dt, qav, gm
        integer(c_int), value :: ix
      end function c_mlw1d_m3
+     integer(c_int) function c_roe1d_m3(ro, pr, vx, vy, vz, by, bz, bx, bxm, dt, gm, dx, ix, s) &
+          bind(C, name="cans1d_roe_m3")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vy, vz, by, bz, bx, bxm, dx, s
+       real(c_double), value :: dt, gm
+       integer(c_int), value :: ix
+     end function c_roe1d_m3
      integer(c_int) function c_cfl1d_m3(dt, safety, dtmin, merr, gm, bx, ro, pr, vx, vy, by, vz, bz, dx, ix, s) &
           bind(C, name="cans1d_cfl_m3")
        import :: c_ptr, c_double, c_int
@@ -136,6 +143,13 @@ module cans_gpu_iface
        real(c_double), value :: dt, qav, gm
        integer(c_int), value :: ix, jx
      end function c_mlw2d_m3
+     integer(c_int) function c_roe2d_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, ix, dy, jx, s) &
+          bind(C, name="cans2d_roe_m3")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vy, vz, bx, by, bz, az, dx, dy, s
+       real(c_double), value :: dt, gm
+       integer(c_int), value :: ix, jx
+     end function c_roe2d_m3
      integer(c_int) function c_cfl2d_m3(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, vz, bx, by, bz, dx, ix, dy, &
           jx, s) bind(C, name="cans2d_cfl_m3")
        import :: c_ptr, c_double, c_int
