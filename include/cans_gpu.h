@@ -111,6 +111,12 @@ int cans2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, do
 int cans2d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by, double *bz,
                   double *az, double dt, double gm, const double *dx, int ix, const double *dy, int jx,
                   void *stream);
+/* cans2d/hdmlw/mlw_m_c.f:2-3 mlw_m_c(ro,pr,vx,vz,bx,bz,ay,dt,qav,gm,x,xm,dx,dxm,ix,dz,dzm,jx):
+ * axisymmetric cylindrical (r, z) = (x, z) MLW with the potential Ay; x, xm
+ * are the cell and face radii; updated in place on i=2..ix-1, j=2..jx-1. */
+int cans2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx, double *bz, double *ay, double dt,
+                   double qav, double gm, const double *x, const double *xm, const double *dx, const double *dxm,
+                   int ix, const double *dz, const double *dzm, int jx, void *stream);
 /* cans2d/common/cfl_m3.f:2-3 cfl_m3(dt,safety,dtmin,merr,gm,ro,pr,vx,vy,vz,bx,by,bz,dx,ix,dy,jx) */
 int cans2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *ro,
                   const double *pr, const double *vx, const double *vy, const double *vz, const double *bx,
@@ -227,6 +233,7 @@ int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *
 #define CANS_ENGINE_HLLD 2  /* cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f (3D only) */
 #define CANS_ENGINE_MLW3 3  /* cans{1,2}d/hdmlw/mlw_m3.f with cfl_m3 (1D/2D only) */
 #define CANS_ENGINE_ROE3 4  /* cans{1,2}d/hdroe/roe_m3.f with cfl_m3 (1D/2D only) */
+#define CANS_ENGINE_MLWC 5  /* cans2d/hdmlw/mlw_m_c.f, cylindrical (r,z) (2D only; x, xm in bx1d, bxm1d) */
 
 /* Device workspace (bytes) a cans_run of this shape needs. */
 size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx);

@@ -94,6 +94,11 @@ void or2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
 void or2d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
                  double *bx, double *by, double *bz, double *az, double dt,
                  double gm, const double *dx, int ix, const double *dy, int jx);
+void or2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx,
+                  double *bz, double *ay, double dt, double qav, double gm,
+                  const double *x, const double *xm, const double *dx,
+                  const double *dxm, int ix, const double *dz,
+                  const double *dzm, int jx);
 void or2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
                  double *bx, double *by, double *bz, double *az, double dt,
                  double qav, double gm, const double *dx, const double *dxm,

@@ -516,3 +516,179 @@ void or2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm,
     if (jmin) *jmin = jm;
     free(dtq);
 }
+
+/* hdmlw/mlw_m_c.f:2-350: mlw_m in axisymmetric cylindrical (r, z) = (x, z)
+ * coordinates; fields ro, pr, vx, vz, bx, bz and the potential Ay.  Every
+ * variable but bx carries the geometric source -F_x/r through mlwsrch (cell,
+ * r = x) and mlwsrcf (corner, r = xm; for bz the reference divides by x). */
+void or2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx,
+                  double *bz, double *ay, double dt, double qav, double gm,
+                  const double *x, const double *xm, const double *dx,
+                  const double *dxm, int ix, const double *dz,
+                  const double *dzm, int jx) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0;
+    const double pi4i = 1.0 / pi / 4.0;
+    size_t n = (size_t)ix * jx;
+    double *dxi = dalloc(ix), *dxim = dalloc(ix), *ux0 = dalloc(ix), *ux1 = dalloc(ix);
+    double *dzi = dalloc(jx), *dzim = dalloc(jx), *uz0 = dalloc(jx), *uz1 = dalloc(jx);
+    double *ee = dalloc(n), *rx = dalloc(n), *rz = dalloc(n), *ey = dalloc(n);
+    double *roh = dalloc(n), *eeh = dalloc(n), *rxh = dalloc(n), *rzh = dalloc(n), *bxh = dalloc(n), *bzh = dalloc(n);
+    double *ayh = dalloc(n), *eyh = dalloc(n), *prh = dalloc(n), *vxh = dalloc(n), *vzh = dalloc(n);
+    double *dro = dalloc(n), *dee = dalloc(n), *drx = dalloc(n), *drz = dalloc(n), *dbx = dalloc(n), *dbz = dalloc(n);
+    double *day = dalloc(n), *fx = dalloc(n), *fz = dalloc(n), *qx = dalloc(n), *qz = dalloc(n), *ss = dalloc(n);
+
+    for (int i = 1; i <= ix; i++) { dxi[I1(i)] = 1.0 / dx[I1(i)]; dxim[I1(i)] = 1.0 / dxm[I1(i)]; } /* :70-73 */
+    for (int i = 2; i <= ix - 1; i++) {
+        ux1[I1(i)] = 0.5 * dxm[I1(i - 1)] / dx[I1(i)];
+        ux0[I1(i)] = 0.5 * dxm[I1(i)] / dx[I1(i)];
+    }
+    for (int j = 1; j <= jx; j++) { dzi[I1(j)] = 1.0 / dz[I1(j)]; dzim[I1(j)] = 1.0 / dzm[I1(j)]; }
+    for (int j = 2; j <= jx - 1; j++) {
+        uz1[I1(j)] = 0.5 * dzm[I1(j - 1)] / dz[I1(j)];
+        uz0[I1(j)] = 0.5 * dzm[I1(j)] / dz[I1(j)];
+    }
+    for (size_t p = 0; p < n; p++) {
+        dro[p] = 0.0; dee[p] = 0.0; drx[p] = 0.0; drz[p] = 0.0; dbx[p] = 0.0; dbz[p] = 0.0; day[p] = 0.0;
+    }
+#define ALL for (int j = 1; j <= jx; j++) for (int i = 1; i <= ix; i++)
+#define HLF for (int j = 1; j <= jx - 1; j++) for (int i = 1; i <= ix - 1; i++)
+    ALL { /* :105-112 */
+        size_t p = I2(i, j);
+        double vv = sq(vx[p]) + sq(vz[p]);
+        double bb = sq(bx[p]) + sq(bz[p]);
+        ee[p] = pr[p] / (gm - 1.0) + 0.5 * ro[p] * vv + pi8i * bb;
+        rx[p] = ro[p] * vx[p];
+        rz[p] = ro[p] * vz[p];
+    }
+    ALL { size_t p = I2(i, j); ey[p] = -vz[p] * bx[p] + vx[p] * bz[p]; }
+
+    /* half step with the cell sources, :123-198 */
+    ALL { size_t p = I2(i, j); fx[p] = ro[p] * vx[p]; fz[p] = ro[p] * vz[p]; ss[p] = -fx[p] / x[I1(i)]; }
+    mlwhalf2(ro, roh, dro, dt, fx, dxi, dxim, ix, fz, dzi, dzim, jx);
+    mlwsrch2(roh, dro, dt, ss, ix, jx);
+    ALL {
+        size_t p = I2(i, j);
+        double vv = sq(vx[p]) + sq(vz[p]);
+        double ep = pr[p] * gm / (gm - 1.0) + 0.5 * ro[p] * vv;
+        fx[p] = ep * vx[p] + (bz[p] * ey[p]) * pi4i;
+        fz[p] = ep * vz[p] + (-bx[p] * ey[p]) * pi4i;
+        ss[p] = -fx[p] / x[I1(i)];
+    }
+    mlwhalf2(ee, eeh, dee, dt, fx, dxi, dxim, ix, fz, dzi, dzim, jx);
+    mlwsrch2(eeh, dee, dt, ss, ix, jx);
+    ALL {
+        size_t p = I2(i, j);
+        fx[p] = ro[p] * sq(vx[p]) + pr[p] + pi8i * (sq(bz[p]) - sq(bx[p]));
+        fz[p] = ro[p] * vx[p] * vz[p] - pi4i * bx[p] * bz[p];
+        ss[p] = -(ro[p] * sq(vx[p]) + pi4i * (-sq(bx[p]))) / x[I1(i)];
+    }
+    mlwhalf2(rx, rxh, drx, dt, fx, dxi, dxim, ix, fz, dzi, dzim, jx);
+    mlwsrch2(rxh, drx, dt, ss, ix, jx);
+    ALL {
+        size_t p = I2(i, j);
+        fx[p] = ro[p] * vz[p] * vx[p] - pi4i * bz[p] * bx[p];
+        fz[p] = ro[p] * sq(vz[p]) + pr[p] + pi8i * (sq(bx[p]) - sq(bz[p]));
+        ss[p] = -fx[p] / x[I1(i)];
+    }
+    mlwhalf2(rz, rzh, drz, dt, fx, dxi, dxim, ix, fz, dzi, dzim, jx);
+    mlwsrch2(rzh, drz, dt, ss, ix, jx);
+    ALL { size_t p = I2(i, j); fx[p] = 0.0; fz[p] = -ey[p]; }
+    mlwhalf2(bx, bxh, dbx, dt, fx, dxi, dxim, ix, fz, dzi, dzim, jx);
+    ALL { size_t p = I2(i, j); fx[p] = ey[p]; fz[p] = 0.0; ss[p] = -fx[p] / x[I1(i)]; }
+    mlwhalf2(bz, bzh, dbz, dt, fx, dxi, dxim, ix, fz, dzi, dzim, jx);
+    mlwsrch2(bzh, dbz, dt, ss, ix, jx);
+    ALL { size_t p = I2(i, j); ss[p] = -ey[p]; }
+    mlwsrch2(ayh, day, dt, ss, ix, jx);
+
+    /* half-step primitives, :202-215 (E over the corners that are used) */
+    HLF {
+        size_t p = I2(i, j);
+        vxh[p] = rxh[p] / roh[p];
+        vzh[p] = rzh[p] / roh[p];
+        double vv = sq(vxh[p]) + sq(vzh[p]);
+        double bb = sq(bxh[p]) + sq(bzh[p]);
+        prh[p] = (gm - 1.0) * (eeh[p] - 0.5 * roh[p] * vv - pi8i * bb);
+        eyh[p] = -vzh[p] * bxh[p] + vxh[p] * bzh[p];
+    }
+
+    /* full step with the corner sources, :219-296 */
+    HLF { size_t p = I2(i, j); fx[p] = roh[p] * vxh[p]; fz[p] = roh[p] * vzh[p]; ss[p] = -fx[p] / xm[I1(i)]; }
+    mlwfull2(dro, dt, fx, dxi, ux0, ux1, ix, fz, dzi, uz0, uz1, jx);
+    mlwsrcf2(dro, dt, ss, ux0, ux1, ix, uz0, uz1, jx);
+    HLF {
+        size_t p = I2(i, j);
+        double vv = sq(vxh[p]) + sq(vzh[p]);
+        double ep = prh[p] * gm / (gm - 1.0) + 0.5 * roh[p] * vv;
+        fx[p] = ep * vxh[p] + (bzh[p] * eyh[p]) * pi4i;
+        fz[p] = ep * vzh[p] + (-bxh[p] * eyh[p]) * pi4i;
+        ss[p] = -fx[p] / xm[I1(i)];
+    }
+    mlwfull2(dee, dt, fx, dxi, ux0, ux1, ix, fz, dzi, uz0, uz1, jx);
+    mlwsrcf2(dee, dt, ss, ux0, ux1, ix, uz0, uz1, jx);
+    HLF {
+        size_t p = I2(i, j);
+        fx[p] = roh[p] * sq(vxh[p]) + prh[p] + pi8i * (sq(bzh[p]) - sq(bxh[p]));
+        fz[p] = roh[p] * vxh[p] * vzh[p] - pi4i * bxh[p] * bzh[p];
+        ss[p] = -(roh[p] * sq(vxh[p]) + pi4i * (-sq(bxh[p]))) / xm[I1(i)];
+    }
+    mlwfull2(drx, dt, fx, dxi, ux0, ux1, ix, fz, dzi, uz0, uz1, jx);
+    mlwsrcf2(drx, dt, ss, ux0, ux1, ix, uz0, uz1, jx);
+    HLF {
+        size_t p = I2(i, j);
+        fx[p] = roh[p] * vzh[p] * vxh[p] - pi4i * bzh[p] * bxh[p];
+        fz[p] = roh[p] * sq(vzh[p]) + prh[p] + pi8i * (sq(bxh[p]) - sq(bzh[p]));
+        ss[p] = -fx[p] / xm[I1(i)];
+    }
+    mlwfull2(drz, dt, fx, dxi, ux0, ux1, ix, fz, dzi, uz0, uz1, jx);
+    mlwsrcf2(drz, dt, ss, ux0, ux1, ix, uz0, uz1, jx);
+    HLF { size_t p = I2(i, j); fx[p] = 0.0; fz[p] = -eyh[p]; }
+    mlwfull2(dbx, dt, fx, dxi, ux0, ux1, ix, fz, dzi, uz0, uz1, jx);
+    HLF { size_t p = I2(i, j); fx[p] = eyh[p]; fz[p] = 0.0; ss[p] = -fx[p] / x[I1(i)]; }   /* :284: x, not xm */
+    mlwfull2(dbz, dt, fx, dxi, ux0, ux1, ix, fz, dzi, uz0, uz1, jx);
+    mlwsrcf2(dbz, dt, ss, ux0, ux1, ix, uz0, uz1, jx);
+    HLF { size_t p = I2(i, j); ss[p] = -eyh[p]; }
+    mlwsrcf2(day, dt, ss, ux0, ux1, ix, uz0, uz1, jx);
+#undef ALL
+#undef HLF
+
+    /* artificial viscosity, :302-321 */
+    const double zero = 0.0;
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++)
+            qx[I2(i, j)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I2(i + 1, j)] - vx[I2(i, j)]) - R4_1EM4);
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix; i++)
+            qz[I2(i, j)] = qav * dzm[I1(j)] * fmaxF(zero, fabs(vz[I2(i, j + 1)] - vz[I2(i, j)]) - R4_1EM4);
+    mlwartv2(ro, dro, dt, qx, dxi, dxim, ix, qz, dzi, dzim, jx);
+    mlwartv2(ee, dee, dt, qx, dxi, dxim, ix, qz, dzi, dzim, jx);
+    mlwartv2(rx, drx, dt, qx, dxi, dxim, ix, qz, dzi, dzim, jx);
+    mlwartv2(rz, drz, dt, qx, dxi, dxim, ix, qz, dzi, dzim, jx);
+    mlwartv2(bx, dbx, dt, qx, dxi, dxim, ix, qz, dzi, dzim, jx);
+    mlwartv2(bz, dbz, dt, qx, dxi, dxim, ix, qz, dzi, dzim, jx);
+
+    /* update and primitive recovery, :325-346 */
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            ro[p] = ro[p] + dro[p];
+            ee[p] = ee[p] + dee[p];
+            rx[p] = rx[p] + drx[p];
+            rz[p] = rz[p] + drz[p];
+            bx[p] = bx[p] + dbx[p];
+            bz[p] = bz[p] + dbz[p];
+            ay[p] = ay[p] + day[p];
+        }
+    for (int j = 2; j <= jx - 1; j++)
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j);
+            vx[p] = rx[p] / ro[p];
+            vz[p] = rz[p] / ro[p];
+            double vv = sq(vx[p]) + sq(vz[p]);
+            double bb = sq(bx[p]) + sq(bz[p]);
+            pr[p] = (gm - 1.0) * (ee[p] - 0.5 * ro[p] * vv - pi8i * bb);
+        }
+    double *tofree[] = {dxi, dxim, ux0, ux1, dzi, dzim, uz0, uz1, ee, rx, rz, ey, roh, eeh, rxh, rzh, bxh, bzh,
+                        ayh, eyh, prh, vxh, vzh, dro, dee, drx, drz, dbx, dbz, day, fx, fz, qx, qz, ss};
+    for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
+}

@@ -127,6 +127,12 @@ class Oracle:
         self.lib.or2d_roe_m3(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz, az)), C.c_double(dt), C.c_double(gm),
                              _arr(dx), C.c_int(ix), _arr(dy), C.c_int(jx))
 
+    def mlw_m_c_2d(self, ro, pr, vx, vz, bx, bz, ay, dt, qav, gm, x, xm, dx, dxm, dz, dzm):
+        ix, jx = ro.shape
+        self.lib.or2d_mlw_m_c(*map(_arr, (ro, pr, vx, vz, bx, bz, ay)), C.c_double(dt), C.c_double(qav),
+                              C.c_double(gm), _arr(x), _arr(xm), _arr(dx), _arr(dxm), C.c_int(ix), _arr(dz), _arr(dzm),
+                              C.c_int(jx))
+
     def mlw_m3_2d(self, ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, dy, dym):
         ix, jx = ro.shape
         self.lib.or2d_mlw_m3(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz, az)), C.c_double(dt), C.c_double(qav),

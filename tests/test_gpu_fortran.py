@@ -117,7 +117,7 @@ def test_spicule_main(dev, oracle, tmp_path, resident):
 
 
 def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
-    """mlw_m3_, roe_m3_ and cfl_m3_ of libcans{1,2}d_gpu.so called as a Fortran main
+    """mlw_m3_, roe_m3_, cfl_m3_ and mlw_m_c_ of libcans{1,2}d_gpu.so called as a Fortran main
     program calls them (every argument by reference, host arrays, strict
     mode), against the restatement."""
     import ctypes as C
@@ -160,6 +160,14 @@ def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
     call(lib2, "roe_m3", *[b[k] for k in F], 9e-4, 5 / 3, dx, ix, dy, jx)
     for k in F:   # Roe: the interior 4..n-3 (see tests/test_gpu_m3.py)
         assert_bitwise(b[k][3:-3, 3:-3], a[k][3:-3, 3:-3], k)
+    from tests.test_oracle_m3 import FC, state_c
+    sc, x, xm, dxc, dxmc = state_c(ix, jx, 6)
+    a = {k: v.copy(order="F") for k, v in sc.items()}
+    b = {k: v.copy(order="F") for k, v in sc.items()}
+    oracle.mlw_m_c_2d(*[a[k] for k in FC], 1e-3, 3.0, 5 / 3, x, xm, dxc, dxmc, dy, dym)
+    call(lib2, "mlw_m_c", *[b[k] for k in FC], 1e-3, 3.0, 5 / 3, x, xm, dxc, dxmc, ix, dy, dym, jx)
+    for k in FC:
+        assert_bitwise(b[k], a[k], k)
     lib1 = C.CDLL(str(PKG_DIR / "libcans1d_gpu.so"))
     ix = 123
     s1 = state1(ix, 2)

@@ -220,3 +220,66 @@ def test_roe_m3_1d_resident_loop(oracle, dev):
     assert info["nexec"] == nsteps and info["t"] == t
     for k, g in zip(F1, cur):
         assert_bitwise(g.cpu().numpy(), a[k], k)
+
+
+from tests.test_oracle_m3 import FC, state_c  # noqa: E402
+
+
+@pytest.mark.parametrize("ix,jx", [(5, 6), (33, 9), (70, 51), (130, 71)])
+def test_mlw_m_c_call(oracle, dev, ix, jx):
+    print(f"mlw_m_c {ix}x{jx}", flush=True)
+    s, x, xm, dx, dxm = state_c(ix, jx, ix + 3 * jx)
+    _, dz, dzm = stretched_grid(jx, seed=jx + 5)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_2d(*[a[k] for k in FC], 1.2e-3, 3.0, 5 / 3, x, xm, dx, dxm, dz, dzm)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_c_2d(*[d[k] for k in FC], 1.2e-3, 3.0, 5 / 3, *[T(v, dev) for v in (x, xm, dx, dxm, dz, dzm)])
+    torch.cuda.synchronize()
+    for k in FC:
+        assert_bitwise(to_host(d[k]), a[k], k)
+
+
+def test_mlw_m_c_axis_face_call(oracle, dev):
+    ix, jx = 24, 20
+    s, x, xm, dx, dxm = state_c(ix, jx, 3, r0=0.0)
+    x = np.asfortranarray(x - x[1] + 0.5 * dxm[1])
+    xm = np.asfortranarray(np.append(0.5 * (x[:-1] + x[1:]), x[-1] + 0.5 * dx[-1]))
+    xm[0] = 0.0
+    _, dz, dzm = stretched_grid(jx, seed=8)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_2d(*[a[k] for k in FC], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, dz, dzm)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_c_2d(*[d[k] for k in FC], 1e-3, 3.0, 5 / 3, *[T(v, dev) for v in (x, xm, dx, dxm, dz, dzm)])
+    torch.cuda.synchronize()
+    for k in FC:
+        assert_bitwise(to_host(d[k]), a[k], k)
+
+
+def test_mlw_m_c_resident_loop(oracle, dev):
+    """30 steps of {cfl_m (vx, vz, bx, bz); t=t+dt; mlw_m_c; bdper x, z}."""
+    ix, jx, margin, nsteps = 50, 40, 1, 30
+    s, x, xm, dx, dxm = state_c(ix, jx, 17)
+    _, dz, dzm = stretched_grid(jx, seed=18)
+    for dr in (0, 1):
+        for k in FC:
+            oracle.bnd_2d(dr, 7, margin, margin, s[k])
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    t = 0.0
+    for _ in range(nsteps):
+        dt, merr, _ = oracle.cfl_m_2d(0.4, 1e-10, 5 / 3, a["ro"], a["pr"], a["vx"], a["vz"], a["bx"], a["bz"], dx, dz)
+        t = t + dt
+        oracle.mlw_m_c_2d(*[a[k] for k in FC], dt, 3.0, 5 / 3, x, xm, dx, dxm, dz, dzm)
+        for dr in (0, 1):
+            for k in FC:
+                oracle.bnd_2d(dr, 7, margin, margin, a[k])
+    st = [to_dev(s[k], dev) for k in FC]
+    grid = {"dx": T(dx, dev), "dxm": T(dxm, dev), "dy": T(dz, dev), "dym": T(dzm, dev)}
+    run = cans.Run(2, cans.ENGINE_MLWC, st, grid, 5 / 3, 0.4, qav=3.0, margin=margin, bc=7, bx1d=T(x, dev),
+                   bxm1d=T(xm, dev))
+    run.reset()
+    run.steps(nsteps)
+    cur, info = run.state()
+    torch.cuda.synchronize()
+    assert info["nexec"] == nsteps and info["t"] == t
+    for k, g in zip(FC, cur):
+        assert_bitwise(to_host(g), a[k], k)

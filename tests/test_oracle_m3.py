@@ -145,3 +145,67 @@ def test_roe_m3_2d_reduces_to_roe_m(oracle):
     oracle.roe_m_2d(*[b[k] for k in ("ro", "pr", "vx", "vy", "bx", "by", "az")], 9e-4, 5 / 3, dx, dy)
     for k in ("ro", "pr", "vx", "vy", "bx", "by", "az"):
         assert_bitwise(a[k][3:-3, 3:-3], b[k][3:-3, 3:-3], k)
+
+
+FC = ("ro", "pr", "vx", "vz", "bx", "bz", "ay")
+
+
+def state_c(ix, jx, seed, r0=0.4):
+    """An r-z state on r in [r0, r0 + 1] (r > 0 everywhere) with all of
+    mlw_m_c's fields non-trivial; x, xm from a stretched grid."""
+    s2 = state2(ix, jx, seed)
+    s = {"ro": s2["ro"], "pr": s2["pr"], "vx": s2["vx"], "vz": s2["vz"], "bx": s2["bx"], "bz": s2["bz"],
+         "ay": s2["az"]}
+    x, dx, dxm = stretched_grid(ix, seed=seed, lo=r0, hi=r0 + 1.0)
+    xm = np.asfortranarray(np.append(0.5 * (x[:-1] + x[1:]), x[-1] + 0.5 * dx[-1]))
+    return s, np.asfortranarray(x), xm, dx, dxm
+
+
+@pytest.mark.parametrize("ix,jx", [(5, 6), (40, 33), (70, 51)])
+def test_mlw_m_c_2d_vs_ref(oracle, ref2d, ix, jx):
+    s, x, xm, dx, dxm = state_c(ix, jx, ix + 3 * jx)
+    _, dz, dzm = stretched_grid(jx, seed=jx + 5)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_2d(*[a[k] for k in FC], 1.2e-3, 3.0, 5 / 3, x, xm, dx, dxm, dz, dzm)
+    ref2d.call("mlw_m_c", *[b[k] for k in FC], 1.2e-3, 3.0, 5 / 3, x, xm, dx, dxm, ix, dz, dzm, jx)
+    for k in FC:
+        assert_bitwise(a[k], b[k], k)
+
+
+def test_mlw_m_c_axis_face(oracle, ref2d):
+    """A grid whose first face sits on the axis (xm = 0 there): the corner
+    sources divide by zero exactly as in the reference (inf/NaN patterns)."""
+    ix, jx = 24, 20
+    s, x, xm, dx, dxm = state_c(ix, jx, 3, r0=0.0)
+    x = np.asfortranarray(x - x[1] + 0.5 * dxm[1])   # cell 2 centred half a cell off the axis
+    xm = np.asfortranarray(np.append(0.5 * (x[:-1] + x[1:]), x[-1] + 0.5 * dx[-1]))
+    xm[0] = 0.0
+    _, dz, dzm = stretched_grid(jx, seed=8)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_2d(*[a[k] for k in FC], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, dz, dzm)
+    ref2d.call("mlw_m_c", *[b[k] for k in FC], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, ix, dz, dzm, jx)
+    for k in FC:
+        assert_bitwise(a[k], b[k], k)
+
+
+def test_mlw_m_c_large_radius_limit(oracle):
+    """Far from the axis the geometric sources vanish: at r ~ 1e12 the
+    cylindrical engine equals the Cartesian mlw_m on the (x, z) plane to
+    rounding (the sources are ~1e-12 of the fluxes)."""
+    ix, jx = 30, 26
+    s, x, xm, dx, dxm = state_c(ix, jx, 4, r0=1.0e12)
+    x = np.asfortranarray(1.0e12 + np.linspace(0, 1, ix))
+    xm = np.asfortranarray(np.append(0.5 * (x[:-1] + x[1:]), x[-1] + 0.5 * dx[-1]))
+    _, dz, dzm = stretched_grid(jx, seed=9)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_2d(*[a[k] for k in FC], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, dz, dzm)
+    # mlw_m on (x, y=z) with vy=vz, by=bz and Az = -Ay: E_z(x,y) = -E_y(x,z)
+    c = {"ro": b["ro"], "pr": b["pr"], "vx": b["vx"], "vy": b["vz"], "bx": b["bx"], "by": b["bz"],
+         "az": np.asfortranarray(-b["ay"])}
+    oracle.mlw_m_2d(*[c[k] for k in ("ro", "pr", "vx", "vy", "bx", "by", "az")], 1e-3, 3.0, 5 / 3, dx, dxm, dz, dzm)
+    for ka, kc in (("ro", "ro"), ("pr", "pr"), ("vx", "vx"), ("vz", "vy"), ("bx", "bx"), ("bz", "by")):
+        np.testing.assert_allclose(a[ka][1:-1, 1:-1], c[kc][1:-1, 1:-1], rtol=1e-9, atol=1e-12, err_msg=ka)
+    np.testing.assert_allclose(a["ay"][1:-1, 1:-1], -c["az"][1:-1, 1:-1], rtol=1e-9, atol=1e-12)

@@ -355,7 +355,7 @@ def hlld_engine_3d_mixed(ro, pr, vx, vy, vz, bx, by, bz, gm, mu, dt, dx, dy, dz,
                                           C.c_int(ix), C.c_int(jx), C.c_int(kx), mf, _stream()))
 
 
-ENGINE_MLW, ENGINE_ROE, ENGINE_HLLD, ENGINE_MLW3, ENGINE_ROE3 = 0, 1, 2, 3, 4
+ENGINE_MLW, ENGINE_ROE, ENGINE_HLLD, ENGINE_MLW3, ENGINE_ROE3, ENGINE_MLWC = 0, 1, 2, 3, 4, 5
 FIELDS = {1: ("ro", "pr", "vx", "vy", "by"), 2: ("ro", "pr", "vx", "vy", "bx", "by", "az"),
           3: ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")}
 # fields of the three-component engines mlw_m3 / roe_m3 (ENGINE_MLW3, ENGINE_ROE3)
@@ -395,6 +395,15 @@ def roe_m3_2d(ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, dy):
     sh = (jx, ix)
     _check(lib().cans2d_roe_m3(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz, az)], _d(dt), _d(gm),
                                _ptr(dx, (ix,)), C.c_int(ix), _ptr(dy, (jx,)), C.c_int(jx), _stream()))
+
+
+def mlw_m_c_2d(ro, pr, vx, vz, bx, bz, ay, dt, qav, gm, x, xm, dx, dxm, dz, dzm):
+    """cans2d/hdmlw/mlw_m_c.f:2-3, cylindrical (r, z)  (in place on i=2..ix-1, j=2..jx-1)."""
+    jx, ix = ro.shape
+    sh = (jx, ix)
+    _check(lib().cans2d_mlw_m_c(*[_ptr(t, sh) for t in (ro, pr, vx, vz, bx, bz, ay)], _d(dt), _d(qav), _d(gm),
+                                _ptr(x, (ix,)), _ptr(xm, (ix,)), _ptr(dx, (ix,)), _ptr(dxm, (ix,)), C.c_int(ix),
+                                _ptr(dz, (jx,)), _ptr(dzm, (jx,)), C.c_int(jx), _stream()))
 
 
 def mlw_m3_2d(ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, dy, dym):

@@ -219,6 +219,34 @@ static int mlw3_dropin(int dim, double *const *f, const double *bx, const double
     return 0;
 }
 
+int cans2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx, double *bz, double *ay, double dt,
+                   double qav, double gm, const double *x, const double *xm, const double *dx, const double *dxm,
+                   int ix, const double *dz, const double *dzm, int jx, void *stream) {
+    double *f[7] = {ro, pr, vx, vz, bx, bz, ay};
+    for (int q = 0; q < 7; q++)
+        if (!f[q]) return abi_fail(CANS_ERR_ARG, "mlw_m_c: NULL field %d", q);
+    if (!x || !xm || !dx || !dxm || !dz || !dzm) return abi_fail(CANS_ERR_ARG, "mlw_m_c: NULL grid array");
+    if (ix < 3 || jx < 3) return abi_fail(CANS_ERR_ARG, "mlw_m_c: %d x %d", ix, jx);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t N = (size_t)ix * jx;
+    void *w = nullptr;
+    if (int rc = abi_scratch((metric_doubles(ix, jx, 1) + 7 * al(N)) * sizeof(double), &w)) return rc;
+    Carver c{(double *)w};
+    Metric m[3];
+    carve_metric(c, ix, jx, 1, m);
+    HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
+    HIPCHK(launch_metric(dz, dzm, jx, m[1], s));
+    Mlw3Args a{};
+    for (int q = 0; q < 7; q++) { a.in[q] = f[q]; a.out[q] = c.take(N); }
+    a.x = x; a.xm = xm; a.dxm = dxm; a.dym = dzm; a.mx = m[0]; a.my = m[1];
+    a.qav = qav; a.gm = gm; a.dt = dt; a.clk = nullptr; a.ix = ix; a.jx = jx;
+    HIPCHK(launch_mlw_m_c(a, s));
+    const double *src[7];
+    for (int q = 0; q < 7; q++) src[q] = a.out[q];
+    HIPCHK(launch_copy_interior(f, src, 7, ix, jx, 1, 2, s));
+    return 0;
+}
+
 int cans1d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *by, double *bz,
                   const double *bx, const double *bxm, double dt, double qav, double gm, const double *dx,
                   const double *dxm, int ix, void *stream) {
@@ -279,7 +307,9 @@ size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx) {
 // bnd} for any dimension/engine with generic boundaries.  State set a holds
 // step n (n even), b step n+1, alternating.  nf fields per set:
 // 1D 5 (ro pr vx vy by), 2D 7 (ro pr vx vy bx by az), 3D 8 (ro pr vx vy vz bx by bz);
-// mlw_m3 and roe_m3: 1D 7 (ro pr vx vy vz by bz), 2D 9 (ro pr vx vy vz bx by bz az).
+// mlw_m3 and roe_m3: 1D 7 (ro pr vx vy vz by bz), 2D 9 (ro pr vx vy vz bx by bz az);
+// mlw_m_c: 2D 7 (ro pr vx vz bx bz ay) on the (x, z) plane (dy, dym are dz,
+// dzm; bx1d, bxm1d carry the radii x, xm), cfl_m (2D) on (vx, vz, bx, bz).
 int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
              const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
              double safety, const double *dx, const double *dxm, int ix, const double *dy, const double *dym,
@@ -290,7 +320,9 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     if (engine == CANS_ENGINE_HLLD && dim != 3) return abi_fail(CANS_ERR_ARG, "cans_run: HLLD is 3D");
     if ((engine == CANS_ENGINE_MLW3 || engine == CANS_ENGINE_ROE3) && dim == 3)
         return abi_fail(CANS_ERR_ARG, "cans_run: the three-component engines are 1D/2D");
-    if (engine < CANS_ENGINE_MLW || engine > CANS_ENGINE_ROE3) return abi_fail(CANS_ERR_ARG, "cans_run: engine %d", engine);
+    if (engine == CANS_ENGINE_MLWC && (dim != 2 || !bx1d || !bxm1d))
+        return abi_fail(CANS_ERR_ARG, "cans_run: mlw_m_c is 2D and needs the radii x, xm (in bx1d, bxm1d)");
+    if (engine < CANS_ENGINE_MLW || engine > CANS_ENGINE_MLWC) return abi_fail(CANS_ERR_ARG, "cans_run: engine %d", engine);
     if (engine == CANS_ENGINE_MLW && dim == 2 && !bc)
         return cans2d_mlw_run(a, b, clk, step0, nsteps, qav, gm, safety, dx, dxm, ix, dy, dym, jx, nullptr, margin,
                               (double *)ws, events, stream);
@@ -298,9 +330,10 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     if (dim == 1) { jx = 1; kx = 1; }
     if (dim == 2) kx = 1;
     const bool m3 = engine == CANS_ENGINE_MLW3;
+    const bool cyl = engine == CANS_ENGINE_MLWC;            // fields ro pr vx vz bx bz ay
     const bool c3 = m3 || engine == CANS_ENGINE_ROE3;      // three-component field lists
     const int nf = c3 ? (dim == 1 ? 7 : 9) : (dim == 1 ? 5 : (dim == 2 ? 7 : 8));
-    const int lo = (engine == CANS_ENGINE_MLW || m3) ? 3 : 5;   // Roe: margin >= 3 (see the drop-in)
+    const int lo = (engine == CANS_ENGINE_MLW || m3 || cyl) ? 3 : 5;   // Roe: margin >= 3 (see the drop-in)
     const int n[3] = {ix, jx, kx};
     for (int d = 0; d < dim; d++)
         if (n[d] < lo || n[d] < 2 * margin + 1) return abi_fail(CANS_ERR_ARG, "cans_run: dimension %d too small", n[d]);
@@ -310,7 +343,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     Carver cv{(double *)ws};
     Metric m[3];
     carve_metric(cv, ix, jx, kx, m);
-    if ((engine == CANS_ENGINE_MLW || m3) && step0 == 0) {   // the first call of a run fills the metric
+    if ((engine == CANS_ENGINE_MLW || m3 || cyl) && step0 == 0) {   // the first call of a run fills the metric
         HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
         if (dim >= 2) HIPCHK(launch_metric(dy, dym, jx, m[1], s));
         if (dim == 3) HIPCHK(launch_metric(dz, dzm, kx, m[2], s));
@@ -348,7 +381,13 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         HIPCHK(launch_clock_step(c, st, s));
         const int e = st - step0;
         if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e], s));
-        if (m3) {
+        if (cyl) {
+            Mlw3Args g{};
+            for (int f = 0; f < 7; f++) { g.in[f] = in[f]; g.out[f] = out[f]; }
+            g.x = bx1d; g.xm = bxm1d; g.dxm = dxm; g.dym = dym; g.mx = m[0]; g.my = m[1];
+            g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx;
+            HIPCHK(launch_mlw_m_c(g, s));
+        } else if (m3) {
             Mlw3Args g{};
             for (int f = 0; f < nf; f++) { g.in[f] = in[f]; g.out[f] = out[f]; }
             g.bx = bx1d; g.bxm = bxm1d; g.dxm = dxm; g.dym = dym; g.mx = m[0]; g.my = m[1];

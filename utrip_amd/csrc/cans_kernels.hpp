@@ -134,6 +134,7 @@ struct Mlw3Args {
     const double *in[9];
     double *out[9];
     const double *bx, *bxm;          // 1D
+    const double *x, *xm;            // cylindrical (mlw_m_c): cell and face radii
     const double *dxm, *dym;
     Metric mx, my;
     double qav, gm, dt;
@@ -142,6 +143,9 @@ struct Mlw3Args {
     int ix, jx;
 };
 hipError_t launch_mlw_m3(const Mlw3Args &a, int dim, hipStream_t s);
+// mlw_m_c (cans2d/hdmlw/mlw_m_c.f), fields ro pr vx vz bx bz ay on the (x, z)
+// plane with the radii a.x, a.xm; my is the z metric
+hipError_t launch_mlw_m_c(const Mlw3Args &a, hipStream_t s);
 
 // UTRIP spicule (cans1d/md_spicule/hdmlw): geometry arrays of mlw_m_bg, in
 // this order.

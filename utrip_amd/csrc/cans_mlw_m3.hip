@@ -385,4 +385,235 @@ hipError_t launch_mlw_m3(const Mlw3Args &a, int dim, hipStream_t s) {
     return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------- 2D cylindrical
+// mlw_m_c (cans2d/hdmlw/mlw_m_c.f:2-350): fields ro pr vx vz bx bz ay on the
+// (r, z) = (x, z) plane.  The same one-pass LDS tile as mlw2d_m3_kernel; in
+// addition every variable but bx carries the geometric source -F_x/r, which
+// enters the corner half step (mlwsrch.f) and the cell (mlwsrch.f, cell
+// value; mlwsrcf.f, corner values).
+namespace {
+
+// LDS rows: cell u (6), x and z fluxes of ro ee rx rz, Ey, cell sources of
+// ro ee rx rz bz, vx, vz.  Corner values overwrite rows CF0..CSBZ.
+enum { CU0 = 0, CFX = 6, CFZ = 10, CEY = 14, CS = 15, CSBZ = 19, CVX = 20, CVZ = 21, CNR = 22 };
+
+CANS_DEV void cyl_fluxes(double ro, double pr, double vx, double vz, double bx, double bz, double ey, double r,
+                         double gm, double *fx, double *fz, double *ss) {
+    fx[0] = ro * vx;
+    fz[0] = ro * vz;
+    ss[0] = -fx[0] / r;
+    const double vv = vx * vx + vz * vz;
+    const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
+    fx[1] = ep * vx + (bz * ey) * pi4i;
+    fz[1] = ep * vz + (-bx * ey) * pi4i;
+    ss[1] = -fx[1] / r;
+    fx[2] = ro * (vx * vx) + pr + pi8i * (bz * bz - bx * bx);
+    fz[2] = ro * vx * vz - pi4i * bx * bz;
+    ss[2] = -(ro * (vx * vx) + pi4i * (-(bx * bx))) / r;
+    fx[3] = ro * vz * vx - pi4i * bz * bx;
+    fz[3] = ro * (vz * vz) + pr + pi8i * (bx * bx - bz * bz);
+    ss[3] = -fx[3] / r;
+}
+
+__global__ __launch_bounds__(256) void mlw2d_c_kernel(Mlw3Args a) {
+    double dt;
+    if (!m3_step(a, dt)) return;
+    __shared__ double sA[CNR][NC];
+    const int ix = a.ix, jx = a.jx;
+    const int ntx = (ix - 2 + TX - 1) / TX;
+    const int nty = (jx - 2 + TY - 1) / TY;
+    const int b = xcd_remap((int)blockIdx.x, ntx * nty);
+    const int i0 = 2 + TX * (b % ntx), j0 = 2 + TY * (b / ntx);
+    const int t = threadIdx.x;
+    const double gm = a.gm;
+    // (1) cells of the tile and its halo (mlw_m_c.f:105-198)
+    for (int lc = t; lc < NC; lc += 256) {
+        const int li = lc % CX, lj = lc / CX;
+        int i = i0 - 1 + li, j = j0 - 1 + lj;
+        i = i > ix ? ix : i;
+        j = j > jx ? jx : j;
+        const size_t p = (size_t)(i - 1) + (size_t)ix * (size_t)(j - 1);
+        const double ro = a.in[0][p], pr = a.in[1][p], vx = a.in[2][p], vz = a.in[3][p];
+        const double bx = a.in[4][p], bz = a.in[5][p];
+        const double vv = vx * vx + vz * vz;
+        const double bb = bx * bx + bz * bz;
+        sA[CU0 + 0][lc] = ro;
+        sA[CU0 + 1][lc] = pr / (gm - 1.0) + 0.5 * ro * vv + pi8i * bb;
+        sA[CU0 + 2][lc] = ro * vx;
+        sA[CU0 + 3][lc] = ro * vz;
+        sA[CU0 + 4][lc] = bx;
+        sA[CU0 + 5][lc] = bz;
+        const double ey = -vz * bx + vx * bz;
+        const double r = a.x[i - 1];
+        double fx[4], fz[4], ss[4];
+        cyl_fluxes(ro, pr, vx, vz, bx, bz, ey, r, gm, fx, fz, ss);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            sA[CFX + q][lc] = fx[q];
+            sA[CFZ + q][lc] = fz[q];
+            sA[CS + q][lc] = ss[q];
+        }
+        sA[CEY][lc] = ey;
+        sA[CSBZ][lc] = -ey / r;          // bz: fx = ey, ss = -fx/x
+        sA[CVX][lc] = vx;
+        sA[CVZ][lc] = vz;
+    }
+    __syncthreads();
+    // (2a) centred half-step term and cell sources (mlwhalf.f, mlwsrch.f)
+    const bool own = t < TX * TY;
+    const int li = 1 + t % TX, lj = 1 + t / TX;
+    const int ci = i0 - 1 + li, cj = j0 - 1 + lj;
+    const int cic = ci > ix ? ix : ci, cjc = cj > jx ? jx : cj;
+    const int lc = lj * CX + li;
+    double du[6], day = 0.0;
+    if (own) {
+        const double dxi = a.mx.di[cic - 1], dzi = a.my.di[cjc - 1];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            double fxE, fxW, fzN, fzS;
+            if (q < 4) {
+                fxE = sA[CFX + q][lc + 1]; fxW = sA[CFX + q][lc - 1];
+                fzN = sA[CFZ + q][lc + CX]; fzS = sA[CFZ + q][lc - CX];
+            } else if (q == 4) {      // bx: fx = 0, fz = -ey
+                fxE = 0.0; fxW = 0.0;
+                fzN = -sA[CEY][lc + CX]; fzS = -sA[CEY][lc - CX];
+            } else {                  // bz: fx = ey, fz = 0
+                fxE = sA[CEY][lc + 1]; fxW = sA[CEY][lc - 1];
+                fzN = 0.0; fzS = 0.0;
+            }
+            du[q] = 0.0;
+            du[q] = du[q] - 0.5 * dt * (0.5 * dxi * (fxE - fxW)) - 0.5 * dt * (0.5 * dzi * (fzN - fzS));
+            if (q < 4) du[q] = du[q] + 0.5 * dt * sA[CS + q][lc];
+            if (q == 5) du[q] = du[q] + 0.5 * dt * sA[CSBZ][lc];
+        }
+        day = day + 0.5 * dt * (-sA[CEY][lc]);
+    }
+    // (2b) corner (i, j): half step with the sources, primitives, fluxes
+    double cr[CSBZ - CFX + 1];
+    {
+        const int ki = t % KX, kj = t / KX;
+        int i = i0 - 1 + ki, j = j0 - 1 + kj;
+        i = i > ix ? ix : i;
+        j = j > jx ? jx : j;
+        const double dxim = a.mx.dim[i - 1], dzim = a.my.dim[j - 1];
+        const int c00 = kj * CX + ki, c10 = c00 + 1, c01 = c00 + CX, c11 = c01 + 1;
+        double un[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            double x00, x10, x01, x11, z00, z10, z01, z11;
+            if (q < 4) {
+                x00 = sA[CFX + q][c00]; x10 = sA[CFX + q][c10]; x01 = sA[CFX + q][c01]; x11 = sA[CFX + q][c11];
+                z00 = sA[CFZ + q][c00]; z10 = sA[CFZ + q][c10]; z01 = sA[CFZ + q][c01]; z11 = sA[CFZ + q][c11];
+            } else if (q == 4) {
+                x00 = 0.0; x10 = 0.0; x01 = 0.0; x11 = 0.0;
+                z00 = -sA[CEY][c00]; z10 = -sA[CEY][c10]; z01 = -sA[CEY][c01]; z11 = -sA[CEY][c11];
+            } else {
+                x00 = sA[CEY][c00]; x10 = sA[CEY][c10]; x01 = sA[CEY][c01]; x11 = sA[CEY][c11];
+                z00 = 0.0; z10 = 0.0; z01 = 0.0; z11 = 0.0;
+            }
+            const double uh = 0.25 * (sA[CU0 + q][c10] + sA[CU0 + q][c00] + sA[CU0 + q][c11] + sA[CU0 + q][c01]);
+            const double dfdx = dxim * (x10 - x00 + x11 - x01) / 2.0;
+            const double dfdz = dzim * (z01 - z00 + z11 - z10) / 2.0;
+            un[q] = uh - dt * dfdx - dt * dfdz;
+            if (q != 4) {   // mlwsrch.f:38-44, every variable but bx
+                const int rs = q < 4 ? CS + q : CSBZ;
+                const double sh = (sA[rs][c10] + sA[rs][c00] + sA[rs][c11] + sA[rs][c01]) / 4.0;
+                un[q] = un[q] + dt * sh;
+            }
+        }
+        const double roh = un[0], eeh = un[1], bxh = un[4], bzh = un[5];
+        const double vxh = un[2] / roh;
+        const double vzh = un[3] / roh;
+        const double vv = vxh * vxh + vzh * vzh;
+        const double bb = bxh * bxh + bzh * bzh;
+        const double prh = (gm - 1.0) * (eeh - 0.5 * roh * vv - pi8i * bb);
+        const double eyh = -vzh * bxh + vxh * bzh;
+        double fx[4], fz[4], ss[4];
+        cyl_fluxes(roh, prh, vxh, vzh, bxh, bzh, eyh, a.xm[i - 1], gm, fx, fz, ss);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            cr[CFX - CFX + q] = fx[q];
+            cr[CFZ - CFX + q] = fz[q];
+            cr[CS - CFX + q] = ss[q];
+        }
+        cr[CEY - CFX] = eyh;
+        cr[CSBZ - CFX] = -eyh / a.x[i - 1];   // mlw_m_c.f:284 divides the bz source by x, not xm
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r <= CSBZ - CFX; r++) sA[CFX + r][t] = cr[r];
+    __syncthreads();
+    // (3) full step, corner sources, viscosity, update (mlw_m_c.f:219-346)
+    if (!own || ci > ix - 1 || cj > jx - 1) return;
+    const int kSW = (lj - 1) * KX + (li - 1), kSE = kSW + 1, kNW = kSW + KX, kNE = kNW + 1;
+    const double dxi = a.mx.di[ci - 1], dzi = a.my.di[cj - 1];
+    const double ux0 = a.mx.u0[ci - 1], ux1 = a.mx.u1[ci - 1], uz0 = a.my.u0[cj - 1], uz1 = a.my.u1[cj - 1];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        double xSW, xSE, xNW, xNE, zSW, zSE, zNW, zNE;
+        if (q < 4) {
+            xSW = sA[CFX + q][kSW]; xSE = sA[CFX + q][kSE]; xNW = sA[CFX + q][kNW]; xNE = sA[CFX + q][kNE];
+            zSW = sA[CFZ + q][kSW]; zSE = sA[CFZ + q][kSE]; zNW = sA[CFZ + q][kNW]; zNE = sA[CFZ + q][kNE];
+        } else if (q == 4) {
+            xSW = 0.0; xSE = 0.0; xNW = 0.0; xNE = 0.0;
+            zSW = -sA[CEY][kSW]; zSE = -sA[CEY][kSE]; zNW = -sA[CEY][kNW]; zNE = -sA[CEY][kNE];
+        } else {
+            xSW = sA[CEY][kSW]; xSE = sA[CEY][kSE]; xNW = sA[CEY][kNW]; xNE = sA[CEY][kNE];
+            zSW = 0.0; zSE = 0.0; zNW = 0.0; zNE = 0.0;
+        }
+        const double dfdx = dxi * (uz0 * (xSE - xSW) + uz1 * (xNE - xNW));
+        const double dfdz = dzi * (ux0 * (zNW - zSW) + ux1 * (zNE - zSE));
+        du[q] = du[q] - 0.5 * dt * dfdx - 0.5 * dt * dfdz;
+        if (q != 4) {   // mlwsrcf.f:33-40
+            const int rs = q < 4 ? CS + q : CSBZ;
+            const double sh = uz0 * (ux0 * sA[rs][kSW] + ux1 * sA[rs][kSE]) + uz1 * (ux0 * sA[rs][kNW] + ux1 * sA[rs][kNE]);
+            du[q] = du[q] + 0.5 * dt * sh;
+        }
+    }
+    {
+        const double sh = uz0 * (ux0 * (-sA[CEY][kSW]) + ux1 * (-sA[CEY][kSE]))
+                        + uz1 * (ux0 * (-sA[CEY][kNW]) + ux1 * (-sA[CEY][kNE]));
+        day = day + 0.5 * dt * sh;
+    }
+    const double zero = 0.0;
+    const double vx = sA[CVX][lc], vz = sA[CVZ][lc];
+    const double qxE = a.qav * a.dxm[ci - 1] * fmaxF(zero, __builtin_fabs(sA[CVX][lc + 1] - vx) - kR4_1em4);
+    const double qxW = a.qav * a.dxm[ci - 2] * fmaxF(zero, __builtin_fabs(vx - sA[CVX][lc - 1]) - kR4_1em4);
+    const double qzN = a.qav * a.dym[cj - 1] * fmaxF(zero, __builtin_fabs(sA[CVZ][lc + CX] - vz) - kR4_1em4);
+    const double qzS = a.qav * a.dym[cj - 2] * fmaxF(zero, __builtin_fabs(vz - sA[CVZ][lc - CX]) - kR4_1em4);
+    const double dxim = a.mx.dim[ci - 1], dxim1 = a.mx.dim[ci - 2];
+    const double dzim = a.my.dim[cj - 1], dzim1 = a.my.dim[cj - 2];
+    double nu[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const double uc = sA[CU0 + q][lc];
+        du[q] = du[q]
+              + dt * (dxi * (qxE * dxim * (sA[CU0 + q][lc + 1] - uc) - qxW * dxim1 * (uc - sA[CU0 + q][lc - 1])))
+              + dt * (dzi * (qzN * dzim * (sA[CU0 + q][lc + CX] - uc) - qzS * dzim1 * (uc - sA[CU0 + q][lc - CX])));
+        nu[q] = uc + du[q];
+    }
+    const size_t p = (size_t)(ci - 1) + (size_t)ix * (size_t)(cj - 1);
+    const double nro = nu[0];
+    const double nvx = nu[2] / nro;
+    const double nvz = nu[3] / nro;
+    const double vv = nvx * nvx + nvz * nvz;
+    const double bb = nu[4] * nu[4] + nu[5] * nu[5];
+    a.out[0][p] = nro;
+    a.out[1][p] = (gm - 1.0) * (nu[1] - 0.5 * nro * vv - pi8i * bb);
+    a.out[2][p] = nvx;
+    a.out[3][p] = nvz;
+    a.out[4][p] = nu[4];
+    a.out[5][p] = nu[5];
+    a.out[6][p] = a.in[6][p] + day;
+}
+
+}  // namespace
+
+hipError_t launch_mlw_m_c(const Mlw3Args &a, hipStream_t s) {
+    const int ntx = (a.ix - 2 + TX - 1) / TX, nty = (a.jx - 2 + TY - 1) / TY;
+    hipLaunchKernelGGL(mlw2d_c_kernel, dim3(ntx * nty), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 }  // namespace cans

@@ -2,6 +2,7 @@
 ! libcans2d_gpu -- the cans2d hot-path routines with the reference's names
 ! and argument lists (cans2d/hdmlw/mlw_m.f:2-3, cans2d/hdroe/roe_m.f:2,
 ! cans2d/common/cfl_m.f:2-3, cans2d/hdmlw/mlw_m3.f:2-3, cans2d/hdroe/roe_m3.f:2,
+! cans2d/hdmlw/mlw_m_c.f:2-3,
 ! cans2d/common/cfl_m3.f:2-3, cans2d/bc/bd*{x,y}.f), computed on the GPU by
 ! libcans_gpu.so.
 !=======================================================================
@@ -39,6 +40,23 @@ subroutine mlw_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, ix, 
   call dunmap(bx, 1); call dunmap(by, 1); call dunmap(bz, 1); call dunmap(az, 1)
   call dunmap(dx, 0); call dunmap(dxm, 0); call dunmap(dy, 0); call dunmap(dym, 0)
 end subroutine mlw_m3
+
+subroutine mlw_m_c(ro, pr, vx, vz, bx, bz, ay, dt, qav, gm, x, xm, dx, dxm, ix, dz, dzm, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target :: ro(ix, jx), pr(ix, jx), vx(ix, jx), vz(ix, jx), bx(ix, jx), bz(ix, jx), ay(ix, jx)
+  double precision, target, intent(in) :: x(ix), xm(ix), dx(ix), dxm(ix), dz(jx), dzm(jx)
+  double precision, intent(in) :: dt, qav, gm
+  integer :: n
+  n = ix * jx
+  call check(c_mlw2d_c(dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vz, n), dmap(bx, n), dmap(bz, n), dmap(ay, n), &
+       dt, qav, gm, dmap(x, ix), dmap(xm, ix), dmap(dx, ix), dmap(dxm, ix), int(ix, c_int), dmap(dz, jx), &
+       dmap(dzm, jx), int(jx, c_int), c_null_ptr), 'mlw_m_c')
+  call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vz, 1)
+  call dunmap(bx, 1); call dunmap(bz, 1); call dunmap(ay, 1)
+  call dunmap(x, 0); call dunmap(xm, 0); call dunmap(dx, 0); call dunmap(dxm, 0); call dunmap(dz, 0); call dunmap(dzm, 0)
+end subroutine mlw_m_c
 
 subroutine roe_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, ix, dy, jx)
   use cans_gpu_iface

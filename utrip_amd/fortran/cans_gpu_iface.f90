@@ -143,6 +143,13 @@ module cans_gpu_iface
        real(c_double), value :: dt, qav, gm
        integer(c_int), value :: ix, jx
      end function c_mlw2d_m3
+     integer(c_int) function c_mlw2d_c(ro, pr, vx, vz, bx, bz, ay, dt, qav, gm, x, xm, dx, dxm, ix, dz, dzm, &
+          jx, s) bind(C, name="cans2d_mlw_m_c")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vz, bx, bz, ay, x, xm, dx, dxm, dz, dzm, s
+       real(c_double), value :: dt, qav, gm
+       integer(c_int), value :: ix, jx
+     end function c_mlw2d_c
      integer(c_int) function c_roe2d_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, ix, dy, jx, s) &
           bind(C, name="cans2d_roe_m3")
        import :: c_ptr, c_double, c_int
