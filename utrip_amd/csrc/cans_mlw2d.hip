@@ -11,7 +11,11 @@
 // stencil rows live in registers; x neighbours come from the adjacent lanes
 // through DPP wave shifts (v_mov_b32_dpp wave_shl:1 / wave_shr:1), so there is
 // no LDS traffic and no barrier, and each field is read once per strip and
-// row with coalesced 512-byte wave loads.
+// row with coalesced 512-byte wave loads.  The carried row state is kept to
+// what the next row cannot re-derive by a lane shift (the east cell is
+// shifted again rather than carried), so that a wave fits in 256 VGPRs and
+// two waves share each SIMD: the FP64 chains of one hide the latency of the
+// other's.
 //
 // Per output row j a lane forms the cell state of row j+1 (mlw_m.f:101-176),
 // the corner half step (i,j) between rows j and j+1 (mlwhalf.f:45-52,
@@ -80,7 +84,6 @@ struct Corner {
 // Row state carried from output row j-1 to output row j.
 struct RowState {
     Cell c;          // cell (i, j)
-    CellE e;         // cell (i+1, j)
     double sy[5];    // y fluxes ry eey rxy ryy ez of cell (i, j-1)
     double dS[NU];   // u(i,j) - u(i,j-1)               (= uc - uS of row j)
     double qys;      // AV coefficient of face j-1/2
@@ -89,9 +92,7 @@ struct RowState {
     double PS;       // ux0*(-ezh(i-1,j-1)) + ux1*(-ezh(i,j-1))
     Prim nxt;        // primitives of cell (i, j+1), loaded
     double az;       // az of cell (i, j), loaded
-    // updated conserved variables and az of cell (i, j-1): its primitive
-    // recovery, CFL term and stores run during row j, off the critical path
-    double pro, pee, prx, pry, pbx, pby, paz;
+    double pro, pee, prx, pry, pbx, pby, paz;   // updated cell (i, j)
 };
 
 // Per-column constants of a strip.
@@ -354,10 +355,7 @@ CANS_DEV void finish_row(const Mlw2dArgs &a, const ColCtx &x, int j, const RowSt
 
 // One output row j: s holds the state entering row j, o receives the state
 // entering row j+1.  Everything but the stores runs on all 64 lanes (the
-// lane shifts need every lane's operands).  With PREV the row also finishes
-// row j-1 (primitives, CFL, stores), so that those dependent chains overlap
-// this row's independent work.
-template <bool PREV>
+// lane shifts need every lane's operands).
 CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bool do_cfl, const RowState &s,
                        RowState &o, double &dtmin_local) {
     const double gm = a.gm;
@@ -366,11 +364,12 @@ CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bo
     o.az = load_az(a, x, j + 1);
 
     const double dyim = a.my.dim[j - 1];
+    const CellE E = shl_cell(s.c);                 // cell (i+1, j)
     const Cell R2 = make_cell(s.nxt, gm);          // cell (i, j+1)
     const CellE R2p = shl_cell(R2);                // cell (i+1, j+1)
-    const Corner C = make_corner(s.c, s.e, R2, R2p, x.dxim, dyim, dt, gm);
+    const Corner C = make_corner(s.c, E, R2, R2p, x.dxim, dyim, dt, gm);
 
-    const double qxc = av_coef(x.qxs, s.e.vx, s.c.vx);
+    const double qxc = av_coef(x.qxs, E.vx, s.c.vx);
     const double qyc = av_coef(a.qav * a.dym[j - 1], R2.vy, s.c.vy);
     RowCoef r;
     r.hdt = 0.5 * dt;
@@ -387,18 +386,18 @@ CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bo
     r.qys = s.qys * a.my.dim[j - 2];
 
     double du[NU];
-    du[U_RO] = eq_xy(r, s.e.u[U_RX], s.c.u[U_RX], R2.u[U_RY], s.sy[0], C.rox, s.dXS[0], C.roy, s.ky[0],
-                     s.e.u[U_RO], s.c.u[U_RO], R2.u[U_RO], s.dS[U_RO], o.dXS[0], o.dS[U_RO]);
-    du[U_EE] = eq_xy(r, s.e.eex, s.c.eex, R2.eey, s.sy[1], C.eex, s.dXS[1], C.eey, s.ky[1], s.e.u[U_EE],
+    du[U_RO] = eq_xy(r, E.u[U_RX], s.c.u[U_RX], R2.u[U_RY], s.sy[0], C.rox, s.dXS[0], C.roy, s.ky[0],
+                     E.u[U_RO], s.c.u[U_RO], R2.u[U_RO], s.dS[U_RO], o.dXS[0], o.dS[U_RO]);
+    du[U_EE] = eq_xy(r, E.eex, s.c.eex, R2.eey, s.sy[1], C.eex, s.dXS[1], C.eey, s.ky[1], E.u[U_EE],
                      s.c.u[U_EE], R2.u[U_EE], s.dS[U_EE], o.dXS[1], o.dS[U_EE]);
-    du[U_RX] = eq_xy(r, s.e.rxx, s.c.rxx, R2.rxy, s.sy[2], C.rxx, s.dXS[2], C.rxy, s.ky[2], s.e.u[U_RX],
+    du[U_RX] = eq_xy(r, E.rxx, s.c.rxx, R2.rxy, s.sy[2], C.rxx, s.dXS[2], C.rxy, s.ky[2], E.u[U_RX],
                      s.c.u[U_RX], R2.u[U_RX], s.dS[U_RX], o.dXS[2], o.dS[U_RX]);
-    du[U_RY] = eq_xy(r, s.e.ryx, s.c.ryx, R2.ryy, s.sy[3], C.ryx, s.dXS[3], C.ryy, s.ky[3], s.e.u[U_RY],
+    du[U_RY] = eq_xy(r, E.ryx, s.c.ryx, R2.ryy, s.sy[3], C.ryx, s.dXS[3], C.ryy, s.ky[3], E.u[U_RY],
                      s.c.u[U_RY], R2.u[U_RY], s.dS[U_RY], o.dXS[3], o.dS[U_RY]);
     const double ezW = shr1(C.ez);                 // ezh(i-1, j)
     {   // bx: x flux 0, y flux ez
         const double kE = C.ez - s.ky[4];
-        const double dE = s.e.u[U_BX] - s.c.u[U_BX];
+        const double dE = E.u[U_BX] - s.c.u[U_BX];
         const double dN = R2.u[U_BX] - s.c.u[U_BX];
         du[U_BX] = du1_y(r, R2.ez - s.sy[4], shr1(kE), kE, dE, shr1(dE), dN, s.dS[U_BX]);
         o.dS[U_BX] = dN;
@@ -406,9 +405,9 @@ CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bo
     {   // by: x flux -ez, y flux 0.  (-a)-(-b) is b-a bit for bit (signed
         // zeros included), so the x differences are taken as west - east.
         const double hN = ezW - C.ez;
-        const double dE = s.e.u[U_BY] - s.c.u[U_BY];
+        const double dE = E.u[U_BY] - s.c.u[U_BY];
         const double dN = R2.u[U_BY] - s.c.u[U_BY];
-        du[U_BY] = du1_x(r, shr1(s.c.ez) - s.e.ez, s.dXS[4], hN, dE, shr1(dE), dN, s.dS[U_BY]);
+        du[U_BY] = du1_x(r, shr1(s.c.ez) - E.ez, s.dXS[4], hN, dE, shr1(dE), dN, s.dS[U_BY]);
         o.dXS[4] = hN;
         o.dS[U_BY] = dN;
     }
@@ -418,7 +417,7 @@ CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bo
     daz = daz + r.hdt * (-s.c.ez);
     const double sh = r.uy0 * s.PS + r.uy1 * PN;
     daz = daz + r.hdt * sh;
-    // update (mlw_m.f:299-304); the rest of the cell's work runs next row
+    // update (mlw_m.f:299-304)
     o.pro = s.c.u[U_RO] + du[U_RO];
     o.pee = s.c.u[U_EE] + du[U_EE];
     o.prx = s.c.u[U_RX] + du[U_RX];
@@ -429,13 +428,12 @@ CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bo
 
     // state entering row j+1
     o.c = R2;
-    o.e = R2p;
     o.sy[0] = s.c.u[U_RY]; o.sy[1] = s.c.eey; o.sy[2] = s.c.rxy; o.sy[3] = s.c.ryy; o.sy[4] = s.c.ez;
     o.qys = qyc;
     o.ky[0] = C.roy; o.ky[1] = C.eey; o.ky[2] = C.rxy; o.ky[3] = C.ryy; o.ky[4] = C.ez;
     o.PS = PN;
 
-    if (PREV) finish_row(a, x, j - 1, s, do_cfl, dtmin_local);
+    finish_row(a, x, j, o, do_cfl, dtmin_local);
 }
 
 template <int OCC>
@@ -483,11 +481,11 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
         const Cell R0 = make_cell(load_prim(a, x, jb - 1), a.gm);
         RowState s0, s1;
         s0.c = make_cell(load_prim(a, x, jb), a.gm);
-        s0.e = shl_cell(s0.c);
+        const CellE s0e = shl_cell(s0.c);
         s0.nxt = load_prim(a, x, jb + 1);
         s0.az = load_az(a, x, jb);
         const CellE R0p = shl_cell(R0);
-        const Corner Cm = make_corner(R0, R0p, s0.c, s0.e, x.dxim, a.my.dim[jb - 2], dt, a.gm);
+        const Corner Cm = make_corner(R0, R0p, s0.c, s0e, x.dxim, a.my.dim[jb - 2], dt, a.gm);
         const double ezW = shr1(Cm.ez);
         s0.dXS[0] = Cm.rox - shr1(Cm.rox);
         s0.dXS[1] = Cm.eex - shr1(Cm.eex);
@@ -501,21 +499,14 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
         for (int q = 0; q < NU; q++) s0.dS[q] = s0.c.u[q] - R0.u[q];
         s0.qys = av_coef(a.qav * a.dym[jb - 2], s0.c.vy, R0.vy);
 
-        // Row jb, then two rows per trip so that the carried state
-        // ping-pongs between s1 and s0 instead of being copied; every row
-        // finishes the one before it, and the last row is finished after.
-        row_step<false>(a, x, jb, dt, do_cfl, s0, s1, dtmin_local);
-        int j = jb + 1;
+        // two rows per trip so that the carried state ping-pongs between
+        // s0 and s1 instead of being copied
+        int j = jb;
         for (; j + 1 <= je; j += 2) {
-            row_step<true>(a, x, j, dt, do_cfl, s1, s0, dtmin_local);
-            row_step<true>(a, x, j + 1, dt, do_cfl, s0, s1, dtmin_local);
+            row_step(a, x, j, dt, do_cfl, s0, s1, dtmin_local);
+            row_step(a, x, j + 1, dt, do_cfl, s1, s0, dtmin_local);
         }
-        if (j <= je) {
-            row_step<true>(a, x, j, dt, do_cfl, s1, s0, dtmin_local);
-            finish_row(a, x, je, s0, do_cfl, dtmin_local);
-        } else {
-            finish_row(a, x, je, s1, do_cfl, dtmin_local);
-        }
+        if (j <= je) row_step(a, x, j, dt, do_cfl, s0, s1, dtmin_local);
     }
     if (do_cfl) block_min_to_global(dtmin_local, red, &a.clk->dtkey[(a.step + 1) % 3]);
 }
@@ -523,7 +514,7 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
 // Build-time tuning: waves per SIMD the register budget is compiled for,
 // and a fixed rows-per-wave (0: chosen per grid below).
 #ifndef CANS_MLW2D_OCC
-#define CANS_MLW2D_OCC 1
+#define CANS_MLW2D_OCC 2
 #endif
 #ifndef CANS_MLW2D_ROWS
 #define CANS_MLW2D_ROWS 0
