@@ -95,6 +95,11 @@ struct RowState {
     double pro, pee, prx, pry, pbx, pby, paz;   // updated cell (i, j)
 };
 
+// Per-row (y) constants, read through non-aliased pointers.
+struct YMet {
+    const double *dim, *di, *u0, *u1, *dym, *dy;
+};
+
 // Per-column constants of a strip.
 struct ColCtx {
     int i;
@@ -121,14 +126,20 @@ CANS_DEV void st(double *row, unsigned boff, double v) {
 
 CANS_DEV Prim load_prim(const Mlw2dArgs &a, const ColCtx &x, int j) {
     const size_t r = row_of(a.ro, a.ix, a.jx, j) - a.ro;
+    // Loads as scalar row base + 32-bit lane offset; the empty asm keeps the
+    // compiler from hoisting six 64-bit per-lane field pointers into VGPRs.
+    unsigned bo = x.boff;
+    asm volatile("" : "+v"(bo));
     Prim p;
-    p.ro = ld(a.ro + r, x.boff); p.pr = ld(a.pr + r, x.boff); p.vx = ld(a.vx + r, x.boff);
-    p.vy = ld(a.vy + r, x.boff); p.bx = ld(a.bx + r, x.boff); p.by = ld(a.by + r, x.boff);
+    p.ro = ld(a.ro + r, bo); p.pr = ld(a.pr + r, bo); p.vx = ld(a.vx + r, bo);
+    p.vy = ld(a.vy + r, bo); p.bx = ld(a.bx + r, bo); p.by = ld(a.by + r, bo);
     return p;
 }
 
 CANS_DEV double load_az(const Mlw2dArgs &a, const ColCtx &x, int j) {
-    return ld(row_of(a.az, a.ix, a.jx, j), x.boff);
+    unsigned bo = x.boff;
+    asm volatile("" : "+v"(bo));
+    return ld(row_of(a.az, a.ix, a.jx, j), bo);
 }
 
 // mlw_m.f:101-176 for one cell
@@ -296,8 +307,8 @@ CANS_DEV double eq_xy(const RowCoef &r, double fE, double fC, double gN, double 
 // Primitive recovery (mlw_m.f:299-321), the CFL term of the new state
 // (cans2d/common/cfl_m.f:46-55) and the stores of the updated cell (i, j),
 // with its periodic ghost images.
-CANS_DEV void finish_row(const Mlw2dArgs &a, const ColCtx &x, int j, const RowState &s, bool do_cfl,
-                         double &dtmin_local) {
+CANS_DEV void finish_row(const Mlw2dArgs &a, const YMet &y, const ColCtx &x, int j, const RowState &s,
+                         bool do_cfl, double &dtmin_local) {
     const double gm = a.gm;
     const double ro = s.pro, ee = s.pee, rx = s.prx, ry = s.pry, bx = s.pbx, by = s.pby, az = s.paz;
     const double vx = rx / ro;
@@ -312,7 +323,7 @@ CANS_DEV void finish_row(const Mlw2dArgs &a, const ColCtx &x, int j, const RowSt
         const double cs2 = gm * pr / ro;
         const double v2 = vx * vx + vy * vy;
         sq = __builtin_sqrt(v2 + cs2 + ca2);
-        const double dtq = a.safety * (fminF(x.dxc, a.dy[j - 1]) / sq);
+        const double dtq = a.safety * (fminF(x.dxc, y.dy[j - 1]) / sq);
         dtmin_local = (x.out && dtq < dtmin_local) ? dtq : dtmin_local;
     }
     if (x.out) {
@@ -356,34 +367,35 @@ CANS_DEV void finish_row(const Mlw2dArgs &a, const ColCtx &x, int j, const RowSt
 // One output row j: s holds the state entering row j, o receives the state
 // entering row j+1.  Everything but the stores runs on all 64 lanes (the
 // lane shifts need every lane's operands).
-CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bool do_cfl, const RowState &s,
-                       RowState &o, double &dtmin_local) {
+CANS_DEV void row_step(const Mlw2dArgs &a, const YMet &y, const ColCtx &x, int j, double dt, bool do_cfl,
+                       const RowState &s, RowState &o, double &dtmin_local) {
+    j = __builtin_amdgcn_readfirstlane(j);   // wave-uniform: SGPR row bases and scalar loads
     const double gm = a.gm;
     // prefetch the row after next and the az of the next row
     o.nxt = load_prim(a, x, j + 2);
     o.az = load_az(a, x, j + 1);
 
-    const double dyim = a.my.dim[j - 1];
+    const double dyim = y.dim[j - 1];
     const CellE E = shl_cell(s.c);                 // cell (i+1, j)
     const Cell R2 = make_cell(s.nxt, gm);          // cell (i, j+1)
     const CellE R2p = shl_cell(R2);                // cell (i+1, j+1)
     const Corner C = make_corner(s.c, E, R2, R2p, x.dxim, dyim, dt, gm);
 
     const double qxc = av_coef(x.qxs, E.vx, s.c.vx);
-    const double qyc = av_coef(a.qav * a.dym[j - 1], R2.vy, s.c.vy);
+    const double qyc = av_coef(a.qav * y.dym[j - 1], R2.vy, s.c.vy);
     RowCoef r;
     r.hdt = 0.5 * dt;
     r.dt = dt;
     r.dxi = x.dxi;
     r.hdxi = 0.5 * x.dxi;
-    r.dyi = a.my.di[j - 1];
+    r.dyi = y.di[j - 1];
     r.hdyi = 0.5 * r.dyi;
     r.ux0 = x.ux0; r.ux1 = x.ux1;
-    r.uy0 = a.my.u0[j - 1]; r.uy1 = a.my.u1[j - 1];
+    r.uy0 = y.u0[j - 1]; r.uy1 = y.u1[j - 1];
     r.qxe = qxc * x.dxim;
     r.qxw = shr1(qxc) * x.dxim1;
     r.qyn = qyc * dyim;
-    r.qys = s.qys * a.my.dim[j - 2];
+    r.qys = s.qys * y.dim[j - 2];
 
     double du[NU];
     du[U_RO] = eq_xy(r, E.u[U_RX], s.c.u[U_RX], R2.u[U_RY], s.sy[0], C.rox, s.dXS[0], C.roy, s.ky[0],
@@ -433,12 +445,18 @@ CANS_DEV void row_step(const Mlw2dArgs &a, const ColCtx &x, int j, double dt, bo
     o.ky[0] = C.roy; o.ky[1] = C.eey; o.ky[2] = C.rxy; o.ky[3] = C.ryy; o.ky[4] = C.ez;
     o.PS = PN;
 
-    finish_row(a, x, j, o, do_cfl, dtmin_local);
+    finish_row(a, y, x, j, o, do_cfl, dtmin_local);
 }
 
+// The y metric arrays come as separate non-aliased arguments (the same
+// pointers as a.my, a.dym, a.dy) so that their per-row uniform loads go
+// through the scalar cache.
 template <int OCC>
-__global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC))) void mlw2d_kernel(Mlw2dArgs a,
-                                                                                                        int H) {
+__global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC))) void mlw2d_kernel(
+    Mlw2dArgs a, int H, const double *__restrict__ ydim, const double *__restrict__ ydi,
+    const double *__restrict__ yu0, const double *__restrict__ yu1, const double *__restrict__ ydym,
+    const double *__restrict__ ydy) {
+    const YMet y{ydim, ydi, yu0, yu1, ydym, ydy};
     __shared__ double red[WPB];
     const int ix = a.ix, jx = a.jx;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -503,10 +521,10 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
         // s0 and s1 instead of being copied
         int j = jb;
         for (; j + 1 <= je; j += 2) {
-            row_step(a, x, j, dt, do_cfl, s0, s1, dtmin_local);
-            row_step(a, x, j + 1, dt, do_cfl, s1, s0, dtmin_local);
+            row_step(a, y, x, j, dt, do_cfl, s0, s1, dtmin_local);
+            row_step(a, y, x, j + 1, dt, do_cfl, s1, s0, dtmin_local);
         }
-        if (j <= je) row_step(a, x, j, dt, do_cfl, s0, s1, dtmin_local);
+        if (j <= je) row_step(a, y, x, j, dt, do_cfl, s0, s1, dtmin_local);
     }
     if (do_cfl) block_min_to_global(dtmin_local, red, &a.clk->dtkey[(a.step + 1) % 3]);
 }
@@ -546,7 +564,8 @@ hipError_t launch_mlw2d(const Mlw2dArgs &a, hipStream_t s) {
     const int nstrip = (a.ix - 2 + COLS - 1) / COLS;
     const int nrowb = (a.jx - 2 + h - 1) / h;
     const int nblocks = (nstrip * nrowb + WPB - 1) / WPB;
-    hipLaunchKernelGGL(mlw2d_kernel<kOcc>, dim3(nblocks), dim3(WAVE * WPB), 0, s, a, h);
+    hipLaunchKernelGGL(mlw2d_kernel<kOcc>, dim3(nblocks), dim3(WAVE * WPB), 0, s, a, h, a.my.dim, a.my.di, a.my.u0,
+                       a.my.u1, a.dym, a.dy);
     return hipGetLastError();
 }
 
