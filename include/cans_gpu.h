@@ -131,6 +131,13 @@ int cans3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz, dou
                  const double *dx, const double *dxm, int ix, const double *dy,
                  const double *dym, int jx, const double *dz, const double *dzm, int kx,
                  void *stream);
+/* cans3d/hdmlw/mlw_m_c.f:2-3 mlw_m_c(ro,pr,vx,vy,vz,bx,by,bz,dt,qav,gm,x,xm,dx,dxm,ix,dy,dym,jx,
+ * dz,dzm,kx): cylindrical (r, phi, z) = (x, y, z) MLW (md_mhdsn/hdmlw/main.f:179); x, xm are the cell
+ * and face radii; updated in place on the interior. */
+int cans3d_mlw_m_c(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
+                   double *bz, double dt, double qav, double gm, const double *x, const double *xm,
+                   const double *dx, const double *dxm, int ix, const double *dy, const double *dym, int jx,
+                   const double *dz, const double *dzm, int kx, void *stream);
 /* cans3d/hdroe/roe_m.f:2 roe_m(ro,pr,vx,vy,vz,bx,by,bz,dt,gm,dx,ix,dy,jx,dz,kx) */
 int cans3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx,
                  double *by, double *bz, double dt, double gm, const double *dx, int ix,

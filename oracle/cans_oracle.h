@@ -122,6 +122,11 @@ void or3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz,
                 double gm, const double *dx, const double *dxm, int ix,
                 const double *dy, const double *dym, int jx,
                 const double *dz, const double *dzm, int kx);
+void or3d_mlw_m_c(double *ro, double *pr, double *vx, double *vy, double *vz,
+                  double *bx, double *by, double *bz, double dt, double qav,
+                  double gm, const double *x, const double *xm, const double *dx, const double *dxm, int ix,
+                  const double *dy, const double *dym, int jx,
+                  const double *dz, const double *dzm, int kx);
 void or3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz,
                 double *bx, double *by, double *bz, double dt, double gm,
                 const double *dx, int ix, const double *dy, int jx,

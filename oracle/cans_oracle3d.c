@@ -263,6 +263,250 @@ void or3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz,
     for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
 }
 
+/* hdmlw/mlwsrch.f:32-47 (3D): cell source into du, corner-averaged source into un */
+static void mlwsrch3(double *un, double *du, double dt, const double *s, int ix, int jx, int kx) {
+    for (int k = 2; k <= kx - 1; k++)
+        for (int j = 2; j <= jx - 1; j++)
+            for (int i = 2; i <= ix - 1; i++) du[I3(i, j, k)] = du[I3(i, j, k)] + 0.5 * dt * s[I3(i, j, k)];
+    for (int k = 1; k <= kx - 1; k++)
+        for (int j = 1; j <= jx - 1; j++)
+            for (int i = 1; i <= ix - 1; i++) {
+                double sh = (s[I3(i + 1, j, k)] + s[I3(i, j, k)] + s[I3(i + 1, j + 1, k)] + s[I3(i, j + 1, k)]
+                             + s[I3(i + 1, j, k + 1)] + s[I3(i, j, k + 1)] + s[I3(i + 1, j + 1, k + 1)]
+                             + s[I3(i, j + 1, k + 1)]) / 8.0;
+                un[I3(i, j, k)] = un[I3(i, j, k)] + dt * sh;
+            }
+}
+
+/* hdmlw/mlwsrcf.f:33-46 (3D): corner source, trilinear weights */
+static void mlwsrcf3(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix,
+                     const double *uy0, const double *uy1, int jx, const double *uz0, const double *uz1, int kx) {
+    for (int k = 2; k <= kx - 1; k++)
+        for (int j = 2; j <= jx - 1; j++)
+            for (int i = 2; i <= ix - 1; i++) {
+                double sh = uz0[I1(k)] * (uy0[I1(j)] * (ux0[I1(i)] * s[I3(i - 1, j - 1, k - 1)] + ux1[I1(i)] * s[I3(i, j - 1, k - 1)])
+                                          + uy1[I1(j)] * (ux0[I1(i)] * s[I3(i - 1, j, k - 1)] + ux1[I1(i)] * s[I3(i, j, k - 1)]))
+                          + uz1[I1(k)] * (uy0[I1(j)] * (ux0[I1(i)] * s[I3(i - 1, j - 1, k)] + ux1[I1(i)] * s[I3(i, j - 1, k)])
+                                          + uy1[I1(j)] * (ux0[I1(i)] * s[I3(i - 1, j, k)] + ux1[I1(i)] * s[I3(i, j, k)]));
+                du[I3(i, j, k)] = du[I3(i, j, k)] + 0.5 * dt * sh;
+            }
+}
+
+/* hdmlw/mlw_m_c.f:2-489 (3D cylindrical (r, phi, z) MHD, 8 fields): mlw_m
+ * with the phi fluxes divided by the radius (x at cells, xm at corners) and
+ * the geometric sources of ro, ee, rx, ry, rz, bz through mlwsrch/mlwsrcf */
+void or3d_mlw_m_c(double *ro, double *pr, double *vx, double *vy, double *vz,
+                  double *bx, double *by, double *bz, double dt, double qav,
+                  double gm, const double *x, const double *xm, const double *dx, const double *dxm, int ix,
+                  const double *dy, const double *dym, int jx,
+                  const double *dz, const double *dzm, int kx) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0;
+    const double pi4i = 1.0 / pi / 4.0;
+    size_t n = (size_t)ix * jx * kx;
+    double *dxi = dalloc(ix), *dxim = dalloc(ix), *ux0 = dalloc(ix), *ux1 = dalloc(ix);
+    double *dyi = dalloc(jx), *dyim = dalloc(jx), *uy0 = dalloc(jx), *uy1 = dalloc(jx);
+    double *dzi = dalloc(kx), *dzim = dalloc(kx), *uz0 = dalloc(kx), *uz1 = dalloc(kx);
+    double *ex = dalloc(n), *ey = dalloc(n), *ez = dalloc(n);
+    double *ee = dalloc(n), *rx = dalloc(n), *ry = dalloc(n), *rz = dalloc(n);
+    double *roh = dalloc(n), *eeh = dalloc(n), *rxh = dalloc(n), *ryh = dalloc(n), *rzh = dalloc(n);
+    double *bxh = dalloc(n), *byh = dalloc(n), *bzh = dalloc(n);
+    double *exh = dalloc(n), *eyh = dalloc(n), *ezh = dalloc(n);
+    double *prh = dalloc(n), *vxh = dalloc(n), *vyh = dalloc(n), *vzh = dalloc(n);
+    double *dro = dalloc(n), *dee = dalloc(n), *drx = dalloc(n), *dry = dalloc(n), *drz = dalloc(n);
+    double *dbx = dalloc(n), *dby = dalloc(n), *dbz = dalloc(n);
+    double *fx = dalloc(n), *fy = dalloc(n), *fz = dalloc(n), *ss = dalloc(n);
+    double *qx = dalloc(n), *qy = dalloc(n), *qz = dalloc(n);
+
+    for (int i = 1; i <= ix; i++) { dxi[I1(i)] = 1.0 / dx[I1(i)]; dxim[I1(i)] = 1.0 / dxm[I1(i)]; }
+    for (int i = 2; i <= ix - 1; i++) { ux1[I1(i)] = 0.5 * dxm[I1(i - 1)] / dx[I1(i)]; ux0[I1(i)] = 0.5 * dxm[I1(i)] / dx[I1(i)]; }
+    for (int j = 1; j <= jx; j++) { dyi[I1(j)] = 1.0 / dy[I1(j)]; dyim[I1(j)] = 1.0 / dym[I1(j)]; }
+    for (int j = 2; j <= jx - 1; j++) { uy1[I1(j)] = 0.5 * dym[I1(j - 1)] / dy[I1(j)]; uy0[I1(j)] = 0.5 * dym[I1(j)] / dy[I1(j)]; }
+    for (int k = 1; k <= kx; k++) { dzi[I1(k)] = 1.0 / dz[I1(k)]; dzim[I1(k)] = 1.0 / dzm[I1(k)]; }
+    for (int k = 2; k <= kx - 1; k++) { uz1[I1(k)] = 0.5 * dzm[I1(k - 1)] / dz[I1(k)]; uz0[I1(k)] = 0.5 * dzm[I1(k)] / dz[I1(k)]; }
+    for (size_t p = 0; p < n; p++) dro[p] = dee[p] = drx[p] = dry[p] = drz[p] = dbx[p] = dby[p] = dbz[p] = 0.0;
+    for (size_t p = 0; p < n; p++) { /* :84-94 */
+        double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+        double bb = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+        ee[p] = pr[p] / (gm - 1.0) + 0.5 * ro[p] * vv + pi8i * bb;
+        rx[p] = ro[p] * vx[p];
+        ry[p] = ro[p] * vy[p];
+        rz[p] = ro[p] * vz[p];
+    }
+    for (size_t p = 0; p < n; p++) { /* :96-104 */
+        ex[p] = -vy[p] * bz[p] + vz[p] * by[p];
+        ey[p] = -vz[p] * bx[p] + vx[p] * bz[p];
+        ez[p] = -vx[p] * by[p] + vy[p] * bx[p];
+    }
+#define CELLS for (int k = 1; k <= kx; k++) for (int j = 1; j <= jx; j++) for (int i = 1; i <= ix; i++)
+#define HALF(u, un, du) mlwhalf3(u, un, du, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx, fz, dzi, dzim, kx)
+    CELLS { /* :109-121 */
+        size_t p = I3(i, j, k); double r = x[I1(i)];
+        fx[p] = ro[p] * vx[p]; fy[p] = ro[p] * vy[p] / r; fz[p] = ro[p] * vz[p]; ss[p] = -(fx[p] / r);
+    }
+    HALF(ro, roh, dro); mlwsrch3(roh, dro, dt, ss, ix, jx, kx);
+    CELLS { /* :125-142 */
+        size_t p = I3(i, j, k); double r = x[I1(i)];
+        double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+        double ep = pr[p] * gm / (gm - 1.0) + 0.5 * ro[p] * vv;
+        fx[p] = ep * vx[p] + (bz[p] * ey[p] - by[p] * ez[p]) * pi4i;
+        fy[p] = ep * vy[p] + (bx[p] * ez[p] - bz[p] * ex[p]) * pi4i;
+        fy[p] = fy[p] / r;
+        fz[p] = ep * vz[p] + (by[p] * ex[p] - bx[p] * ey[p]) * pi4i;
+        ss[p] = -(fx[p] / r);
+    }
+    HALF(ee, eeh, dee); mlwsrch3(eeh, dee, dt, ss, ix, jx, kx);
+    CELLS { /* :145-162 */
+        size_t p = I3(i, j, k); double r = x[I1(i)];
+        fx[p] = ro[p] * sq(vx[p]) + pr[p] + pi8i * (sq(by[p]) + sq(bz[p]) - sq(bx[p]));
+        fy[p] = ro[p] * vx[p] * vy[p] - pi4i * bx[p] * by[p];
+        fy[p] = fy[p] / r;
+        fz[p] = ro[p] * vx[p] * vz[p] - pi4i * bx[p] * bz[p];
+        ss[p] = -(ro[p] / r * (sq(vx[p]) - sq(vy[p]))) + pi4i / r * (sq(bx[p]) - sq(by[p]));
+    }
+    HALF(rx, rxh, drx); mlwsrch3(rxh, drx, dt, ss, ix, jx, kx);
+    CELLS { /* :165-182 */
+        size_t p = I3(i, j, k); double r = x[I1(i)];
+        fx[p] = ro[p] * vy[p] * vx[p] - pi4i * by[p] * bx[p];
+        fy[p] = ro[p] * sq(vy[p]) + pr[p] + pi8i * (sq(bx[p]) + sq(bz[p]) - sq(by[p]));
+        fy[p] = fy[p] / r;
+        fz[p] = ro[p] * vy[p] * vz[p] - pi4i * by[p] * bz[p];
+        ss[p] = -(2.0 * ro[p] / r * vx[p] * vy[p]) + 2.0 * pi4i / r * bx[p] * by[p];
+    }
+    HALF(ry, ryh, dry); mlwsrch3(ryh, dry, dt, ss, ix, jx, kx);
+    CELLS { /* :185-201 */
+        size_t p = I3(i, j, k); double r = x[I1(i)];
+        fx[p] = ro[p] * vz[p] * vx[p] - pi4i * bz[p] * bx[p];
+        fy[p] = ro[p] * vz[p] * vy[p] - pi4i * bz[p] * by[p];
+        fy[p] = fy[p] / r;
+        fz[p] = ro[p] * sq(vz[p]) + pr[p] + pi8i * (sq(bx[p]) + sq(by[p]) - sq(bz[p]));
+        ss[p] = -(fx[p] / r);
+    }
+    HALF(rz, rzh, drz); mlwsrch3(rzh, drz, dt, ss, ix, jx, kx);
+    CELLS { size_t p = I3(i, j, k); fx[p] = 0.0; fy[p] = ez[p] / x[I1(i)]; fz[p] = -ey[p]; } /* :205-213 */
+    HALF(bx, bxh, dbx);
+    CELLS { size_t p = I3(i, j, k); fx[p] = -ez[p]; fy[p] = 0.0; fz[p] = ex[p]; }          /* :218-226 */
+    HALF(by, byh, dby);
+    CELLS { /* :231-243 */
+        size_t p = I3(i, j, k); double r = x[I1(i)];
+        fx[p] = ey[p]; fy[p] = -ex[p] / r; fz[p] = 0.0; ss[p] = -(fx[p] / r);
+    }
+    HALF(bz, bzh, dbz); mlwsrch3(bzh, dbz, dt, ss, ix, jx, kx);
+#undef HALF
+#undef CELLS
+    for (int k = 1; k <= kx - 1; k++) /* :248-258 */
+        for (int j = 1; j <= jx - 1; j++)
+            for (int i = 1; i <= ix - 1; i++) {
+                size_t p = I3(i, j, k);
+                vxh[p] = rxh[p] / roh[p];
+                vyh[p] = ryh[p] / roh[p];
+                vzh[p] = rzh[p] / roh[p];
+                double vv = sq(vxh[p]) + sq(vyh[p]) + sq(vzh[p]);
+                double bb = sq(bxh[p]) + sq(byh[p]) + sq(bzh[p]);
+                prh[p] = (gm - 1.0) * (eeh[p] - 0.5 * roh[p] * vv - pi8i * bb);
+            }
+    for (int k = 1; k <= kx - 1; k++) /* :260-268 (exh..ezh used at corners 1..n-1 only) */
+        for (int j = 1; j <= jx - 1; j++)
+            for (int i = 1; i <= ix - 1; i++) {
+                size_t p = I3(i, j, k);
+                exh[p] = -vyh[p] * bzh[p] + vzh[p] * byh[p];
+                eyh[p] = -vzh[p] * bxh[p] + vxh[p] * bzh[p];
+                ezh[p] = -vxh[p] * byh[p] + vyh[p] * bxh[p];
+            }
+#define FULL(du) mlwfull3(du, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx, fz, dzi, uz0, uz1, kx)
+#define SRCF(du) mlwsrcf3(du, dt, ss, ux0, ux1, ix, uy0, uy1, jx, uz0, uz1, kx)
+#define HL for (int k = 1; k <= kx - 1; k++) for (int j = 1; j <= jx - 1; j++) for (int i = 1; i <= ix - 1; i++)
+    HL { /* :273-287 */
+        size_t p = I3(i, j, k); double r = xm[I1(i)];
+        fx[p] = roh[p] * vxh[p]; fy[p] = roh[p] * vyh[p] / r; fz[p] = roh[p] * vzh[p]; ss[p] = -(fx[p] / r);
+    }
+    FULL(dro); SRCF(dro);
+    HL { /* :290-308 */
+        size_t p = I3(i, j, k); double r = xm[I1(i)];
+        double vv = sq(vxh[p]) + sq(vyh[p]) + sq(vzh[p]);
+        double ep = prh[p] * gm / (gm - 1.0) + 0.5 * roh[p] * vv;
+        fx[p] = ep * vxh[p] + (bzh[p] * eyh[p] - byh[p] * ezh[p]) * pi4i;
+        fy[p] = ep * vyh[p] + (bxh[p] * ezh[p] - bzh[p] * exh[p]) * pi4i;
+        fy[p] = fy[p] / r;
+        fz[p] = ep * vzh[p] + (byh[p] * exh[p] - bxh[p] * eyh[p]) * pi4i;
+        ss[p] = -(fx[p] / r);
+    }
+    FULL(dee); SRCF(dee);
+    HL { /* :311-328 */
+        size_t p = I3(i, j, k); double r = xm[I1(i)];
+        fx[p] = roh[p] * sq(vxh[p]) + prh[p] + pi8i * (sq(byh[p]) + sq(bzh[p]) - sq(bxh[p]));
+        fy[p] = roh[p] * vxh[p] * vyh[p] - pi4i * bxh[p] * byh[p];
+        fy[p] = fy[p] / r;
+        fz[p] = roh[p] * vxh[p] * vzh[p] - pi4i * bxh[p] * bzh[p];
+        ss[p] = -(roh[p] / r * (sq(vxh[p]) - sq(vyh[p]))) + pi4i / r * (sq(bxh[p]) - sq(byh[p]));
+    }
+    FULL(drx); SRCF(drx);
+    HL { /* :331-348 */
+        size_t p = I3(i, j, k); double r = xm[I1(i)];
+        fx[p] = roh[p] * vyh[p] * vxh[p] - pi4i * byh[p] * bxh[p];
+        fy[p] = roh[p] * sq(vyh[p]) + prh[p] + pi8i * (sq(bxh[p]) + sq(bzh[p]) - sq(byh[p]));
+        fy[p] = fy[p] / r;
+        fz[p] = roh[p] * vyh[p] * vzh[p] - pi4i * byh[p] * bzh[p];
+        ss[p] = -(2.0 * roh[p] / r * vxh[p] * vyh[p]) + 2.0 * pi4i / r * bxh[p] * byh[p];
+    }
+    FULL(dry); SRCF(dry);
+    HL { /* :351-367 */
+        size_t p = I3(i, j, k); double r = xm[I1(i)];
+        fx[p] = roh[p] * vzh[p] * vxh[p] - pi4i * bzh[p] * bxh[p];
+        fy[p] = roh[p] * vzh[p] * vyh[p] - pi4i * bzh[p] * byh[p];
+        fy[p] = fy[p] / r;
+        fz[p] = roh[p] * sq(vzh[p]) + prh[p] + pi8i * (sq(bxh[p]) + sq(byh[p]) - sq(bzh[p]));
+        ss[p] = -(fx[p] / r);
+    }
+    FULL(drz); SRCF(drz);
+    HL { size_t p = I3(i, j, k); fx[p] = 0.0; fy[p] = ezh[p] / xm[I1(i)]; fz[p] = -eyh[p]; } /* :370-379 */
+    FULL(dbx);
+    HL { size_t p = I3(i, j, k); fx[p] = -ezh[p]; fy[p] = 0.0; fz[p] = exh[p]; }           /* :383-391 */
+    FULL(dby);
+    HL { /* :395-408 */
+        size_t p = I3(i, j, k); double r = xm[I1(i)];
+        fx[p] = eyh[p]; fy[p] = -exh[p] / r; fz[p] = 0.0; ss[p] = -(fx[p] / r);
+    }
+    FULL(dbz); SRCF(dbz);
+#undef HL
+#undef SRCF
+#undef FULL
+    const double zero = 0.0;   /* :414-436 */
+    for (int k = 1; k <= kx; k++)
+        for (int j = 1; j <= jx; j++)
+            for (int i = 1; i <= ix - 1; i++)
+                qx[I3(i, j, k)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I3(i + 1, j, k)] - vx[I3(i, j, k)]) - R4_1EM4);
+    for (int k = 1; k <= kx; k++)
+        for (int j = 1; j <= jx - 1; j++)
+            for (int i = 1; i <= ix; i++)
+                qy[I3(i, j, k)] = qav * dym[I1(j)] * fmaxF(zero, fabs(vy[I3(i, j + 1, k)] - vy[I3(i, j, k)]) - R4_1EM4);
+    for (int k = 1; k <= kx - 1; k++)
+        for (int j = 1; j <= jx; j++)
+            for (int i = 1; i <= ix; i++)
+                qz[I3(i, j, k)] = qav * dzm[I1(k)] * fmaxF(zero, fabs(vz[I3(i, j, k + 1)] - vz[I3(i, j, k)]) - R4_1EM4);
+#define ARTV(u, du) mlwartv3(u, du, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx, qz, dzi, dzim, kx)
+    ARTV(ro, dro); ARTV(ee, dee); ARTV(rx, drx); ARTV(ry, dry); ARTV(rz, drz);
+    ARTV(bx, dbx); ARTV(by, dby); ARTV(bz, dbz);
+#undef ARTV
+    for (int k = 2; k <= kx - 1; k++) /* :458-485 */
+        for (int j = 2; j <= jx - 1; j++)
+            for (int i = 2; i <= ix - 1; i++) {
+                size_t p = I3(i, j, k);
+                ro[p] = ro[p] + dro[p]; ee[p] = ee[p] + dee[p]; rx[p] = rx[p] + drx[p]; ry[p] = ry[p] + dry[p];
+                rz[p] = rz[p] + drz[p]; bx[p] = bx[p] + dbx[p]; by[p] = by[p] + dby[p]; bz[p] = bz[p] + dbz[p];
+                vx[p] = rx[p] / ro[p];
+                vy[p] = ry[p] / ro[p];
+                vz[p] = rz[p] / ro[p];
+                double vv = sq(vx[p]) + sq(vy[p]) + sq(vz[p]);
+                double bb = sq(bx[p]) + sq(by[p]) + sq(bz[p]);
+                pr[p] = (gm - 1.0) * (ee[p] - 0.5 * ro[p] * vv - pi8i * bb);
+            }
+    double *tofree[] = {dxi, dxim, ux0, ux1, dyi, dyim, uy0, uy1, dzi, dzim, uz0, uz1, ex, ey, ez, ee, rx, ry, rz,
+                        roh, eeh, rxh, ryh, rzh, bxh, byh, bzh, exh, eyh, ezh, prh, vxh, vyh, vzh,
+                        dro, dee, drx, dry, drz, dbx, dby, dbz, fx, fy, fz, ss, qx, qy, qz};
+    for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
+}
+
 /* common/cfl_m.f:39-80 (all cells; REAL*4 pi) */
 void or3d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
                 const double *ro, const double *pr, const double *vx,

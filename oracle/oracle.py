@@ -235,6 +235,12 @@ class Oracle:
                             C.c_double(gm), _arr(dx), _arr(dxm), C.c_int(ix), _arr(dy), _arr(dym), C.c_int(jx),
                             _arr(dz), _arr(dzm), C.c_int(kx))
 
+    def mlw_m_c_3d(self, ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, dy, dym, dz, dzm):
+        ix, jx, kx = ro.shape
+        self.lib.or3d_mlw_m_c(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz)), C.c_double(dt), C.c_double(qav),
+                              C.c_double(gm), _arr(x), _arr(xm), _arr(dx), _arr(dxm), C.c_int(ix), _arr(dy),
+                              _arr(dym), C.c_int(jx), _arr(dz), _arr(dzm), C.c_int(kx))
+
     def roe_m_3d(self, ro, pr, vx, vy, vz, bx, by, bz, dt, gm, dx, dy, dz):
         ix, jx, kx = ro.shape
         self.lib.or3d_roe_m(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz)), C.c_double(dt), C.c_double(gm),

@@ -117,7 +117,7 @@ def test_spicule_main(dev, oracle, tmp_path, resident):
 
 
 def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
-    """mlw_m3_, roe_m3_, cfl_m3_ and mlw_m_c_ of libcans{1,2}d_gpu.so called as a Fortran main
+    """mlw_m3_, roe_m3_, cfl_m3_ and mlw_m_c_ of libcans{1,2,3}d_gpu.so called as a Fortran main
     program calls them (every argument by reference, host arrays, strict
     mode), against the restatement."""
     import ctypes as C
@@ -185,3 +185,18 @@ def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
     call(lib1, "roe_m3", *[b[k] for k in F1m], b["bx"], b["bxm"], 1e-3, 5 / 3, dx, ix)
     for k in F1m:
         assert_bitwise(b[k][3:-3], a[k][3:-3], k)
+    # 3D cylindrical mlw_m_c_ of libcans3d_gpu.so (md_mhdsn/hdmlw/main.f:179)
+    from tests.helpers import grid3, smooth_state_3d
+    from tests.test_oracle_m3 import F3, radii
+    lib3 = C.CDLL(str(PKG_DIR / "libcans3d_gpu.so"))
+    shape = (17, 9, 11)
+    s3 = smooth_state_3d(*shape, seed=41)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(shape[0], 0.25, dx)
+    a = {k: v.copy(order="F") for k, v in s3.items()}
+    b = {k: v.copy(order="F") for k, v in s3.items()}
+    oracle.mlw_m_c_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, dy, dym, dz, dzm)
+    call(lib3, "mlw_m_c", *[b[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, shape[0], dy, dym, shape[1],
+         dz, dzm, shape[2])
+    for k in F3:
+        assert_bitwise(b[k], a[k], k)

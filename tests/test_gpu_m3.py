@@ -283,3 +283,70 @@ def test_mlw_m_c_resident_loop(oracle, dev):
     assert info["nexec"] == nsteps and info["t"] == t
     for k, g in zip(FC, cur):
         assert_bitwise(to_host(g), a[k], k)
+
+
+# ---------------------------------------------------------------- mlw_m_c, 3D (r, phi, z)
+from tests.helpers import grid3, smooth_state_3d  # noqa: E402
+from tests.test_oracle_m3 import F3, radii  # noqa: E402
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (6, 5, 7), (21, 8, 12), (40, 33, 20)])
+def test_mlw_m_c_3d_call(oracle, dev, shape):
+    s = smooth_state_3d(*shape, seed=sum(shape) + 1)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(shape[0], 0.3, dx)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, dy, dym, dz, dzm)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_c_3d(*[d[k] for k in F3], 1e-3, 3.0, 5 / 3, *[T(v, dev) for v in (x, xm, dx, dxm, dy, dym, dz, dzm)])
+    torch.cuda.synchronize()
+    for k in F3:
+        assert_bitwise(to_host(d[k]), a[k], k)
+
+
+def test_mlw_m_c_3d_axis_face_call(oracle, dev):
+    shape = (10, 7, 6)
+    s = smooth_state_3d(*shape, seed=17)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(shape[0], 0.0, dx)
+    xm[0] = 0.0
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, dy, dym, dz, dzm)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_c_3d(*[d[k] for k in F3], 1e-3, 3.0, 5 / 3, *[T(v, dev) for v in (x, xm, dx, dxm, dy, dym, dz, dzm)])
+    torch.cuda.synchronize()
+    for k in F3:
+        assert_bitwise(to_host(d[k]), a[k], k)
+    assert not np.isfinite(a["ro"][1, 1:-1, 1:-1]).all()
+
+
+def test_mlw_m_c_3d_resident_loop(oracle, dev):
+    """12 steps of {cfl_m (3D); t=t+dt; mlw_m_c; bdper x, y, z} -- the
+    md_mhdsn 3D loop body (main.f:169-183) with periodic faces."""
+    shape, margin, nsteps = (24, 20, 16), 1, 12
+    s = smooth_state_3d(*shape, seed=31)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(shape[0], 0.5, dx)
+    for dr in (0, 1, 2):
+        for k in F3:
+            oracle.bnd_3d(dr, 7, margin, margin, s[k])
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    t = 0.0
+    for _ in range(nsteps):
+        dt, merr, _ = oracle.cfl_m_3d(0.4, 1e-10, 5 / 3, *[a[k] for k in F3], dx, dy, dz)
+        t = t + dt
+        oracle.mlw_m_c_3d(*[a[k] for k in F3], dt, 3.0, 5 / 3, x, xm, dx, dxm, dy, dym, dz, dzm)
+        for dr in (0, 1, 2):
+            for k in F3:
+                oracle.bnd_3d(dr, 7, margin, margin, a[k])
+    st = [to_dev(s[k], dev) for k in F3]
+    grid = {k: T(v, dev) for k, v in zip(("dx", "dxm", "dy", "dym", "dz", "dzm"), (dx, dxm, dy, dym, dz, dzm))}
+    run = cans.Run(3, cans.ENGINE_MLWC, st, grid, 5 / 3, 0.4, qav=3.0, margin=margin, bc=7, bx1d=T(x, dev),
+                   bxm1d=T(xm, dev))
+    run.reset()
+    run.steps(nsteps)
+    cur, info = run.state()
+    torch.cuda.synchronize()
+    assert info["nexec"] == nsteps and info["t"] == t
+    for k, g in zip(F3, cur):
+        assert_bitwise(to_host(g), a[k], k)

@@ -7,7 +7,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from tests.helpers import assert_bitwise, stretched_grid
+from tests.helpers import assert_bitwise, grid3, smooth_state_3d, stretched_grid
 
 F1 = ("ro", "pr", "vx", "vy", "vz", "by", "bz")
 F2 = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz", "az")
@@ -209,3 +209,68 @@ def test_mlw_m_c_large_radius_limit(oracle):
     for ka, kc in (("ro", "ro"), ("pr", "pr"), ("vx", "vx"), ("vz", "vy"), ("bx", "bx"), ("bz", "by")):
         np.testing.assert_allclose(a[ka][1:-1, 1:-1], c[kc][1:-1, 1:-1], rtol=1e-9, atol=1e-12, err_msg=ka)
     np.testing.assert_allclose(a["ay"][1:-1, 1:-1], -c["az"][1:-1, 1:-1], rtol=1e-9, atol=1e-12)
+
+
+# ---------------------------------------------------------------- mlw_m_c, 3D (r, phi, z)
+F3 = ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")
+
+
+def radii(ix, r0, dx):
+    """Cell radii x (cell 2 at r0 + dx/2) and face radii xm = x + dx/2 of a
+    stretched grid with widths dx."""
+    x = r0 + np.cumsum(dx) - 1.5 * dx[0] - 0.5 * dx
+    xm = x + 0.5 * dx
+    return np.asfortranarray(x), np.asfortranarray(xm)
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (6, 5, 7), (13, 11, 10), (21, 8, 12)])
+def test_mlw_m_c_3d_vs_ref(oracle, ref3d, shape):
+    ix, jx, kx = shape
+    s = smooth_state_3d(*shape, seed=sum(shape) + 1)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(ix, 0.3, dx)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, dy, dym, dz, dzm)
+    ref3d.call("mlw_m_c", *[b[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, ix, dy, dym, jx, dz, dzm, kx)
+    for k in F3:
+        assert_bitwise(a[k], b[k], k)
+
+
+def test_mlw_m_c_3d_axis_face(oracle, ref3d):
+    """First face on the axis (xm = 0): the corner fluxes and sources there
+    are inf/NaN exactly as in the reference."""
+    shape = (10, 7, 6)
+    ix, jx, kx = shape
+    s = smooth_state_3d(*shape, seed=17)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(ix, 0.0, dx)
+    xm[0] = 0.0
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, dy, dym, dz, dzm)
+    ref3d.call("mlw_m_c", *[b[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, ix, dy, dym, jx, dz, dzm, kx)
+    for k in F3:
+        assert_bitwise(a[k], b[k], k)
+    assert not np.isfinite(a["ro"][1, 1:-1, 1:-1]).all()   # the axis reaches the first interior cells
+
+
+def test_mlw_m_c_3d_large_radius_limit(oracle):
+    """At r ~ 1e12 with azimuthal widths dy/r the cylindrical engine is the
+    Cartesian mlw_m on widths (dx, dy, dz) to rounding: the phi fluxes /r
+    over dphi become d/dy and the geometric sources ~1/r vanish.  qav = 0,
+    since the artificial viscosity in phi uses dphi without the radius."""
+    shape = (12, 10, 9)
+    ix, jx, kx = shape
+    R = 1.0e12
+    s = smooth_state_3d(*shape, seed=23)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(ix, R, dx)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_c_3d(*[a[k] for k in F3], 1e-3, 0.0, 5 / 3, x, xm, dx, dxm,
+                      np.asfortranarray(dy / R), np.asfortranarray(dym / R), dz, dzm)
+    oracle.mlw_m_3d(*[b[k] for k in F3], 1e-3, 0.0, 5 / 3, dx, dxm, dy, dym, dz, dzm)
+    for k in F3:
+        np.testing.assert_allclose(a[k][1:-1, 1:-1, 1:-1], b[k][1:-1, 1:-1, 1:-1], rtol=1e-9, atol=1e-12,
+                                   err_msg=k)

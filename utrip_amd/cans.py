@@ -170,6 +170,17 @@ def mlw_m_3d(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, dx, dxm, dy, dym, dz, 
                               _ptr(dz, (kx,)), _ptr(dzm, (kx,)), C.c_int(kx), _stream()))
 
 
+def mlw_m_c_3d(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, dy, dym, dz, dzm):
+    """cans3d/hdmlw/mlw_m_c.f:2-3, cylindrical (r, phi, z)  (in place on the interior)."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    _check(lib().cans3d_mlw_m_c(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                                _d(dt), _d(qav), _d(gm), _ptr(x, (ix,)), _ptr(xm, (ix,)),
+                                _ptr(dx, (ix,)), _ptr(dxm, (ix,)), C.c_int(ix),
+                                _ptr(dy, (jx,)), _ptr(dym, (jx,)), C.c_int(jx),
+                                _ptr(dz, (kx,)), _ptr(dzm, (kx,)), C.c_int(kx), _stream()))
+
+
 def roe_m_3d(ro, pr, vx, vy, vz, bx, by, bz, dt, gm, dx, dy, dz):
     """cans3d/hdroe/roe_m.f:2."""
     kx, jx, ix = ro.shape

@@ -10,6 +10,12 @@
 //            the corner fluxes (mlw_m.f:193-315), 18 values per corner;
 //   update   centre term of mlwhalf, mlwfull.f:34-55, mlwartv.f:36-48,
 //            update and primitive recovery (mlw_m.f:316-453).
+// The same kernels, with CYL = true, do the cylindrical (r, phi, z) engine
+// cans3d/hdmlw/mlw_m_c.f: phi fluxes divided by the radius (x at cells, xm
+// at corners) in slots of their own, and the geometric sources of ro, ee,
+// rx, ry, rz, bz -- at cells into the corner half step (mlwsrch.f:39-46) and
+// the centre term (mlwsrch.f:33-38), at corners into the update
+// (mlwsrcf.f:33-46).
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_roe.hpp"
@@ -40,6 +46,43 @@ __constant__ int kCellFlux[8][3][2] = {
     {{C_EZ, -1}, {-1, 1}, {C_EX, 1}},     // by: -ez, 0, ex
     {{C_EY, 1}, {C_EX, -1}, {-1, 1}},     // bz: ey, -ex, 0
 };
+// cylindrical cell slots (mlw_m_c.f:109-243): conserved, x fluxes, raw E,
+// phi fluxes / x, z fluxes, sources
+enum { Y_EEX = 8, Y_RXX, Y_RYX, Y_RZX, Y_EX, Y_EY, Y_EZ,
+       Y_FRO, Y_FEE, Y_FRX, Y_FRY, Y_FRZ, Y_FBX, Y_FBZ,
+       Y_EEZ, Y_RXZ, Y_RYZ, Y_RZZ,
+       Y_SRO, Y_SEE, Y_SRX, Y_SRY, Y_SRZ, Y_SBZ, Y_NCELL };
+static_assert(Y_NCELL == kMlw3dCylCellFields, "cylindrical cell slots");
+// cylindrical corner slots (mlw_m_c.f:273-408)
+enum { Z_ROX, Z_EEX, Z_RXX, Z_RYX, Z_RZX, Z_EX, Z_EY, Z_EZ,
+       Z_FRO, Z_FEE, Z_FRX, Z_FRY, Z_FRZ, Z_FBX, Z_FBZ,
+       Z_ROZ, Z_EEZ, Z_RXZ, Z_RYZ, Z_RZZ,
+       Z_SRO, Z_SEE, Z_SRX, Z_SRY, Z_SRZ, Z_SBZ, Z_NCORN };
+static_assert(Z_NCORN == kMlw3dCylCornerFields, "cylindrical corner slots");
+__constant__ int kCellFluxC[8][3][2] = {
+    {{C_RX, 1}, {Y_FRO, 1}, {C_RZ, 1}},
+    {{Y_EEX, 1}, {Y_FEE, 1}, {Y_EEZ, 1}},
+    {{Y_RXX, 1}, {Y_FRX, 1}, {Y_RXZ, 1}},
+    {{Y_RYX, 1}, {Y_FRY, 1}, {Y_RYZ, 1}},
+    {{Y_RZX, 1}, {Y_FRZ, 1}, {Y_RZZ, 1}},
+    {{-1, 1}, {Y_FBX, 1}, {Y_EY, -1}},    // bx: 0, ez/x, -ey
+    {{Y_EZ, -1}, {-1, 1}, {Y_EX, 1}},     // by: -ez, 0, ex
+    {{Y_EY, 1}, {Y_FBZ, 1}, {-1, 1}},     // bz: ey, -ex/x, 0
+};
+__constant__ int kCornFluxC[8][3][2] = {
+    {{Z_ROX, 1}, {Z_FRO, 1}, {Z_ROZ, 1}},
+    {{Z_EEX, 1}, {Z_FEE, 1}, {Z_EEZ, 1}},
+    {{Z_RXX, 1}, {Z_FRX, 1}, {Z_RXZ, 1}},
+    {{Z_RYX, 1}, {Z_FRY, 1}, {Z_RYZ, 1}},
+    {{Z_RZX, 1}, {Z_FRZ, 1}, {Z_RZZ, 1}},
+    {{-1, 1}, {Z_FBX, 1}, {Z_EY, -1}},
+    {{Z_EZ, -1}, {-1, 1}, {Z_EX, 1}},
+    {{Z_EY, 1}, {Z_FBZ, 1}, {-1, 1}},
+};
+// source slot of each equation (bx, by: none)
+__constant__ int kCellSrc[8] = {Y_SRO, Y_SEE, Y_SRX, Y_SRY, Y_SRZ, -1, -1, Y_SBZ};
+__constant__ int kCornSrc[8] = {Z_SRO, Z_SEE, Z_SRX, Z_SRY, Z_SRZ, -1, -1, Z_SBZ};
+
 __constant__ int kCornFlux[8][3][2] = {
     {{H_ROX, 1}, {H_ROY, 1}, {H_ROZ, 1}},
     {{H_EEX, 1}, {H_EEY, 1}, {H_EEZ, 1}},
@@ -59,6 +102,7 @@ CANS_DEV bool step_dt3(const Mlw3dArgs &a, double &dt) {
     return true;
 }
 
+template <bool CYL>
 __global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
     double dt;
     if (!step_dt3(a, dt)) return;
@@ -84,6 +128,37 @@ __global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
     const double ey = -vz * bx + vx * bz;
     const double ez = -vx * by + vy * bx;
     const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
+    if (CYL) {   // mlw_m_c.f:109-243, r = x(i)
+        const double r = a.x[p % (size_t)a.ix];
+        const double fro = ro * vx;
+        W(Y_FRO, ro * vy / r);
+        W(Y_SRO, -(fro / r));
+        const double fee = ep * vx + (bz * ey - by * ez) * pi4i;
+        W(Y_EEX, fee);
+        W(Y_FEE, (ep * vy + (bx * ez - bz * ex) * pi4i) / r);
+        W(Y_EEZ, ep * vz + (by * ex - bx * ey) * pi4i);
+        W(Y_SEE, -(fee / r));
+        W(Y_RXX, ro * (vx * vx) + pr + pi8i * (by * by + bz * bz - bx * bx));
+        W(Y_FRX, (ro * vx * vy - pi4i * bx * by) / r);
+        W(Y_RXZ, ro * vx * vz - pi4i * bx * bz);
+        W(Y_SRX, -(ro / r * (vx * vx - vy * vy)) + pi4i / r * (bx * bx - by * by));
+        W(Y_RYX, ro * vy * vx - pi4i * by * bx);
+        W(Y_FRY, (ro * (vy * vy) + pr + pi8i * (bx * bx + bz * bz - by * by)) / r);
+        W(Y_RYZ, ro * vy * vz - pi4i * by * bz);
+        W(Y_SRY, -(2.0 * ro / r * vx * vy) + 2.0 * pi4i / r * bx * by);
+        const double frz = ro * vz * vx - pi4i * bz * bx;
+        W(Y_RZX, frz);
+        W(Y_FRZ, (ro * vz * vy - pi4i * bz * by) / r);
+        W(Y_RZZ, ro * (vz * vz) + pr + pi8i * (bx * bx + by * by - bz * bz));
+        W(Y_SRZ, -(frz / r));
+        W(Y_EX, ex);
+        W(Y_EY, ey);
+        W(Y_EZ, ez);
+        W(Y_FBX, ez / r);
+        W(Y_FBZ, -ex / r);
+        W(Y_SBZ, -(ey / r));
+        return;
+    }
     W(C_EEX, ep * vx + (bz * ey - by * ez) * pi4i);
     W(C_EEY, ep * vy + (bx * ez - bz * ex) * pi4i);
     W(C_EEZ, ep * vz + (by * ex - bx * ey) * pi4i);
@@ -102,20 +177,23 @@ __global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
 #undef W
 }
 
+template <bool CYL>
 CANS_DEV double cflux(const double *c, size_t N, int q, int d, size_t p) {
-    const int slot = kCellFlux[q][d][0];
+    const int slot = CYL ? kCellFluxC[q][d][0] : kCellFlux[q][d][0];
     if (slot < 0) return 0.0;
     const double v = c[(size_t)slot * N + p];
-    return kCellFlux[q][d][1] < 0 ? -v : v;
+    return (CYL ? kCellFluxC[q][d][1] : kCellFlux[q][d][1]) < 0 ? -v : v;
 }
+template <bool CYL>
 CANS_DEV double hflux(const double *h, size_t N, int q, int d, size_t p) {
-    const int slot = kCornFlux[q][d][0];
+    const int slot = CYL ? kCornFluxC[q][d][0] : kCornFlux[q][d][0];
     if (slot < 0) return 0.0;
     const double v = h[(size_t)slot * N + p];
-    return kCornFlux[q][d][1] < 0 ? -v : v;
+    return (CYL ? kCornFluxC[q][d][1] : kCornFlux[q][d][1]) < 0 ? -v : v;
 }
 
 // corner (i,j,k) between cells i..i+1, j..j+1, k..k+1, i,j,k in [1,n-1]
+template <bool CYL>
 __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
     double dt;
     if (!step_dt3(a, dt)) return;
@@ -138,16 +216,22 @@ __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
     for (int q = 0; q < 8; q++) {
         const double *u = c + (size_t)q * N;    // C_RO..C_BZ are slots 0..7
         const double uh = (u[p100] + u[p000] + u[p110] + u[p010] + u[p101] + u[p001] + u[p111] + u[p011]) / 8.0;
-        const double dfdx = (cflux(c, N, q, 0, p100) - cflux(c, N, q, 0, p000) + cflux(c, N, q, 0, p110)
-                             - cflux(c, N, q, 0, p010) + cflux(c, N, q, 0, p101) - cflux(c, N, q, 0, p001)
-                             + cflux(c, N, q, 0, p111) - cflux(c, N, q, 0, p011)) / 4.0 * dxim;
-        const double dfdy = (cflux(c, N, q, 1, p010) - cflux(c, N, q, 1, p000) + cflux(c, N, q, 1, p110)
-                             - cflux(c, N, q, 1, p100) + cflux(c, N, q, 1, p011) - cflux(c, N, q, 1, p001)
-                             + cflux(c, N, q, 1, p111) - cflux(c, N, q, 1, p101)) / 4.0 * dyim;
-        const double dfdz = (cflux(c, N, q, 2, p001) - cflux(c, N, q, 2, p000) + cflux(c, N, q, 2, p101)
-                             - cflux(c, N, q, 2, p100) + cflux(c, N, q, 2, p011) - cflux(c, N, q, 2, p010)
-                             + cflux(c, N, q, 2, p111) - cflux(c, N, q, 2, p110)) / 4.0 * dzim;
+        const double dfdx = (cflux<CYL>(c, N, q, 0, p100) - cflux<CYL>(c, N, q, 0, p000) + cflux<CYL>(c, N, q, 0, p110)
+                             - cflux<CYL>(c, N, q, 0, p010) + cflux<CYL>(c, N, q, 0, p101) - cflux<CYL>(c, N, q, 0, p001)
+                             + cflux<CYL>(c, N, q, 0, p111) - cflux<CYL>(c, N, q, 0, p011)) / 4.0 * dxim;
+        const double dfdy = (cflux<CYL>(c, N, q, 1, p010) - cflux<CYL>(c, N, q, 1, p000) + cflux<CYL>(c, N, q, 1, p110)
+                             - cflux<CYL>(c, N, q, 1, p100) + cflux<CYL>(c, N, q, 1, p011) - cflux<CYL>(c, N, q, 1, p001)
+                             + cflux<CYL>(c, N, q, 1, p111) - cflux<CYL>(c, N, q, 1, p101)) / 4.0 * dyim;
+        const double dfdz = (cflux<CYL>(c, N, q, 2, p001) - cflux<CYL>(c, N, q, 2, p000) + cflux<CYL>(c, N, q, 2, p101)
+                             - cflux<CYL>(c, N, q, 2, p100) + cflux<CYL>(c, N, q, 2, p011) - cflux<CYL>(c, N, q, 2, p010)
+                             + cflux<CYL>(c, N, q, 2, p111) - cflux<CYL>(c, N, q, 2, p110)) / 4.0 * dzim;
         un[q] = uh - dt * dfdx - dt * dfdy - dt * dfdz;
+        if (CYL && kCellSrc[q] >= 0) {   // mlwsrch.f:39-46
+            const double *sc = c + (size_t)kCellSrc[q] * N;
+            const double sh = (sc[p100] + sc[p000] + sc[p110] + sc[p010] + sc[p101] + sc[p001] + sc[p111] + sc[p011])
+                              / 8.0;
+            un[q] = un[q] + dt * sh;
+        }
     }
     const double gm = a.gm;
     const double roh = un[0], eeh = un[1], rxh = un[2], ryh = un[3], rzh = un[4];
@@ -165,6 +249,39 @@ __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
     double *h = a.corner;
     const size_t pc = p000;   // corner (i,j,k) stored at the index of cell (i,j,k)
 #define W(slot, v) h[(size_t)(slot) * N + pc] = (v)
+    if (CYL) {   // mlw_m_c.f:273-408, r = xm(i)
+        const double r = a.xm[i - 1];
+        const double fro = roh * vxh;
+        W(Z_ROX, fro);
+        W(Z_FRO, roh * vyh / r);
+        W(Z_ROZ, roh * vzh);
+        W(Z_SRO, -(fro / r));
+        const double fee = ep * vxh + (bzh * eyh - byh * ezh) * pi4i;
+        W(Z_EEX, fee);
+        W(Z_FEE, (ep * vyh + (bxh * ezh - bzh * exh) * pi4i) / r);
+        W(Z_EEZ, ep * vzh + (byh * exh - bxh * eyh) * pi4i);
+        W(Z_SEE, -(fee / r));
+        W(Z_RXX, roh * (vxh * vxh) + prh + pi8i * (byh * byh + bzh * bzh - bxh * bxh));
+        W(Z_FRX, (roh * vxh * vyh - pi4i * bxh * byh) / r);
+        W(Z_RXZ, roh * vxh * vzh - pi4i * bxh * bzh);
+        W(Z_SRX, -(roh / r * (vxh * vxh - vyh * vyh)) + pi4i / r * (bxh * bxh - byh * byh));
+        W(Z_RYX, roh * vyh * vxh - pi4i * byh * bxh);
+        W(Z_FRY, (roh * (vyh * vyh) + prh + pi8i * (bxh * bxh + bzh * bzh - byh * byh)) / r);
+        W(Z_RYZ, roh * vyh * vzh - pi4i * byh * bzh);
+        W(Z_SRY, -(2.0 * roh / r * vxh * vyh) + 2.0 * pi4i / r * bxh * byh);
+        const double frz = roh * vzh * vxh - pi4i * bzh * bxh;
+        W(Z_RZX, frz);
+        W(Z_FRZ, (roh * vzh * vyh - pi4i * bzh * byh) / r);
+        W(Z_RZZ, roh * (vzh * vzh) + prh + pi8i * (bxh * bxh + byh * byh - bzh * bzh));
+        W(Z_SRZ, -(frz / r));
+        W(Z_EX, exh);
+        W(Z_EY, eyh);
+        W(Z_EZ, ezh);
+        W(Z_FBX, ezh / r);
+        W(Z_FBZ, -exh / r);
+        W(Z_SBZ, -(eyh / r));
+        return;
+    }
     W(H_ROX, roh * vxh);
     W(H_ROY, roh * vyh);
     W(H_ROZ, roh * vzh);
@@ -187,6 +304,7 @@ __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
 }
 
 // interior cell (i,j,k) in [2,n-1]^3
+template <bool CYL>
 __global__ __launch_bounds__(256) void mlw3d_update(Mlw3dArgs a) {
     double dt;
     if (!step_dt3(a, dt)) return;
@@ -225,23 +343,31 @@ __global__ __launch_bounds__(256) void mlw3d_update(Mlw3dArgs a) {
     for (int q = 0; q < 8; q++) {
         double du = 0.0;
         // mlwhalf centre term
-        du = du - 0.5 * dt * (0.5 * dxi * (cflux(c, N, q, 0, p + sx) - cflux(c, N, q, 0, p - sx)))
-                - 0.5 * dt * (0.5 * dyi * (cflux(c, N, q, 1, p + sy) - cflux(c, N, q, 1, p - sy)))
-                - 0.5 * dt * (0.5 * dzi * (cflux(c, N, q, 2, p + sz) - cflux(c, N, q, 2, p - sz)));
+        du = du - 0.5 * dt * (0.5 * dxi * (cflux<CYL>(c, N, q, 0, p + sx) - cflux<CYL>(c, N, q, 0, p - sx)))
+                - 0.5 * dt * (0.5 * dyi * (cflux<CYL>(c, N, q, 1, p + sy) - cflux<CYL>(c, N, q, 1, p - sy)))
+                - 0.5 * dt * (0.5 * dzi * (cflux<CYL>(c, N, q, 2, p + sz) - cflux<CYL>(c, N, q, 2, p - sz)));
+        if (CYL && kCellSrc[q] >= 0) du = du + 0.5 * dt * c[(size_t)kCellSrc[q] * N + p];   // mlwsrch.f:33-38
         // mlwfull
-        const double dfdx = dxi * (uz0 * (uy0 * (hflux(h, N, q, 0, c100) - hflux(h, N, q, 0, c000))
-                                          + uy1 * (hflux(h, N, q, 0, c110) - hflux(h, N, q, 0, c010)))
-                                   + uz1 * (uy0 * (hflux(h, N, q, 0, c101) - hflux(h, N, q, 0, c001))
-                                            + uy1 * (hflux(h, N, q, 0, c111) - hflux(h, N, q, 0, c011))));
-        const double dfdy = dyi * (ux0 * (uz0 * (hflux(h, N, q, 1, c010) - hflux(h, N, q, 1, c000))
-                                          + uz1 * (hflux(h, N, q, 1, c011) - hflux(h, N, q, 1, c001)))
-                                   + ux1 * (uz0 * (hflux(h, N, q, 1, c110) - hflux(h, N, q, 1, c100))
-                                            + uz1 * (hflux(h, N, q, 1, c111) - hflux(h, N, q, 1, c101))));
-        const double dfdz = dzi * (uy0 * (ux0 * (hflux(h, N, q, 2, c001) - hflux(h, N, q, 2, c000))
-                                          + ux1 * (hflux(h, N, q, 2, c101) - hflux(h, N, q, 2, c100)))
-                                   + uy1 * (ux0 * (hflux(h, N, q, 2, c011) - hflux(h, N, q, 2, c010))
-                                            + ux1 * (hflux(h, N, q, 2, c111) - hflux(h, N, q, 2, c110))));
+        const double dfdx = dxi * (uz0 * (uy0 * (hflux<CYL>(h, N, q, 0, c100) - hflux<CYL>(h, N, q, 0, c000))
+                                          + uy1 * (hflux<CYL>(h, N, q, 0, c110) - hflux<CYL>(h, N, q, 0, c010)))
+                                   + uz1 * (uy0 * (hflux<CYL>(h, N, q, 0, c101) - hflux<CYL>(h, N, q, 0, c001))
+                                            + uy1 * (hflux<CYL>(h, N, q, 0, c111) - hflux<CYL>(h, N, q, 0, c011))));
+        const double dfdy = dyi * (ux0 * (uz0 * (hflux<CYL>(h, N, q, 1, c010) - hflux<CYL>(h, N, q, 1, c000))
+                                          + uz1 * (hflux<CYL>(h, N, q, 1, c011) - hflux<CYL>(h, N, q, 1, c001)))
+                                   + ux1 * (uz0 * (hflux<CYL>(h, N, q, 1, c110) - hflux<CYL>(h, N, q, 1, c100))
+                                            + uz1 * (hflux<CYL>(h, N, q, 1, c111) - hflux<CYL>(h, N, q, 1, c101))));
+        const double dfdz = dzi * (uy0 * (ux0 * (hflux<CYL>(h, N, q, 2, c001) - hflux<CYL>(h, N, q, 2, c000))
+                                          + ux1 * (hflux<CYL>(h, N, q, 2, c101) - hflux<CYL>(h, N, q, 2, c100)))
+                                   + uy1 * (ux0 * (hflux<CYL>(h, N, q, 2, c011) - hflux<CYL>(h, N, q, 2, c010))
+                                            + ux1 * (hflux<CYL>(h, N, q, 2, c111) - hflux<CYL>(h, N, q, 2, c110))));
         du = du - 0.5 * dt * dfdx - 0.5 * dt * dfdy - 0.5 * dt * dfdz;
+        if (CYL && kCornSrc[q] >= 0) {   // mlwsrcf.f:33-46
+            const double *sh_ = h + (size_t)kCornSrc[q] * N;
+            const double sh = uz0 * (uy0 * (ux0 * sh_[c000] + ux1 * sh_[c100]) + uy1 * (ux0 * sh_[c010] + ux1 * sh_[c110]))
+                              + uz1 * (uy0 * (ux0 * sh_[c001] + ux1 * sh_[c101])
+                                       + uy1 * (ux0 * sh_[c011] + ux1 * sh_[c111]));
+            du = du + 0.5 * dt * sh;
+        }
         // mlwartv
         const double *u = c + (size_t)q * N;
         const double uc = u[p];
@@ -274,9 +400,16 @@ hipError_t launch_mlw3d(const Mlw3dArgs &a, hipStream_t s) {
     const size_t N = (size_t)a.ix * a.jx * a.kx;
     const size_t nc = (size_t)(a.ix - 1) * (a.jx - 1) * (a.kx - 1);
     const size_t nu = (size_t)(a.ix - 2) * (a.jx - 2) * (a.kx - 2);
-    hipLaunchKernelGGL(mlw3d_cells, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(mlw3d_corners, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(mlw3d_update, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, a);
+    if (a.x) {
+        if (!a.xm) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(mlw3d_cells<true>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_corners<true>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_update<true>, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(mlw3d_cells<false>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_corners<false>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_update<false>, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -31,8 +31,9 @@ hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s, bool
 struct Mlw3dArgs {
     const double *prim[8];   // ro pr vx vy vz bx by bz (in)
     double *out[8];          // same (out), interior cells
-    double *cell;            // scratch: 23 doubles per cell (SoA)
-    double *corner;          // scratch: 18 doubles per corner (SoA)
+    double *cell;            // scratch: 23 (cylindrical 32) doubles per cell (SoA)
+    double *corner;          // scratch: 18 (cylindrical 26) doubles per corner (SoA)
+    const double *x, *xm;    // non-null: cylindrical mlw_m_c, cell and face radii
     const double *dxm, *dym, *dzm;
     Metric mx, my, mz;
     double qav, gm, dt;
@@ -43,6 +44,8 @@ struct Mlw3dArgs {
 hipError_t launch_mlw3d(const Mlw3dArgs &a, hipStream_t s);
 constexpr int kMlw3dCellFields = 23;
 constexpr int kMlw3dCornerFields = 18;
+constexpr int kMlw3dCylCellFields = 32;
+constexpr int kMlw3dCylCornerFields = 26;
 
 // HLLD MUSCL-Hancock (cansxx-nosupport/hdhlld, mu units)
 struct HlldArgs {

@@ -1,6 +1,6 @@
 !=======================================================================
 ! libcans3d_gpu -- the cans3d hot-path routines with the reference's names
-! and argument lists (cans3d/hdmlw/mlw_m.f:2-3, cans3d/hdroe/roe_m.f:2,
+! and argument lists (cans3d/hdmlw/mlw_m.f:2-3, cans3d/hdmlw/mlw_m_c.f:2-3, cans3d/hdroe/roe_m.f:2,
 ! cans3d/common/cfl_m.f:2-3, cans3d/bc/bd*{x,y,z}.f) and the HLLD driver
 ! engine of cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f:2, computed on
 ! the GPU by libcans_gpu.so.
@@ -21,6 +21,27 @@ subroutine mlw_m(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, dx, dxm, ix, dy, d
   call dunmap(dx, 0); call dunmap(dxm, 0); call dunmap(dy, 0); call dunmap(dym, 0)
   call dunmap(dz, 0); call dunmap(dzm, 0)
 end subroutine mlw_m
+
+
+! cans3d/hdmlw/mlw_m_c.f:2-3: cylindrical (r, phi, z), cell and face radii x, xm
+subroutine mlw_m_c(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, ix, dy, dym, jx, dz, dzm, kx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx, kx
+  double precision, target :: ro(*), pr(*), vx(*), vy(*), vz(*), bx(*), by(*), bz(*)
+  double precision, target, intent(in) :: x(ix), xm(ix), dx(ix), dxm(ix), dy(jx), dym(jx), dz(kx), dzm(kx)
+  double precision, intent(in) :: dt, qav, gm
+  integer :: n
+  n = ix * jx * kx
+  call check(c_mlw3d_c(dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vy, n), dmap(vz, n), dmap(bx, n), dmap(by, n), &
+       dmap(bz, n), dt, qav, gm, dmap(x, ix), dmap(xm, ix), dmap(dx, ix), dmap(dxm, ix), int(ix, c_int), &
+       dmap(dy, jx), dmap(dym, jx), int(jx, c_int), dmap(dz, kx), dmap(dzm, kx), int(kx, c_int), c_null_ptr), &
+       'mlw_m_c')
+  call unmap8(ro, pr, vx, vy, vz, bx, by, bz)
+  call dunmap(x, 0); call dunmap(xm, 0)
+  call dunmap(dx, 0); call dunmap(dxm, 0); call dunmap(dy, 0); call dunmap(dym, 0)
+  call dunmap(dz, 0); call dunmap(dzm, 0)
+end subroutine mlw_m_c
 
 
 subroutine roe_m(ro, pr, vx, vy, vz, bx, by, bz, dt, gm, dx, ix, dy, jx, dz, kx)

@@ -190,6 +190,13 @@ module cans_gpu_iface
        real(c_double), value :: dt, qav, gm
        integer(c_int), value :: ix, jx, kx
      end function c_mlw3d
+     integer(c_int) function c_mlw3d_c(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, ix, &
+          dy, dym, jx, dz, dzm, kx, s) bind(C, name="cans3d_mlw_m_c")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vy, vz, bx, by, bz, x, xm, dx, dxm, dy, dym, dz, dzm, s
+       real(c_double), value :: dt, qav, gm
+       integer(c_int), value :: ix, jx, kx
+     end function c_mlw3d_c
      integer(c_int) function c_roe3d(ro, pr, vx, vy, vz, bx, by, bz, dt, gm, dx, ix, dy, jx, dz, kx, s) &
           bind(C, name="cans3d_roe_m")
        import :: c_ptr, c_double, c_int
