@@ -138,6 +138,15 @@ int cans3d_mlw_m_c(double *ro, double *pr, double *vx, double *vy, double *vz, d
                    double *bz, double dt, double qav, double gm, const double *x, const double *xm,
                    const double *dx, const double *dxm, int ix, const double *dy, const double *dym, int jx,
                    const double *dz, const double *dzm, int kx, void *stream);
+/* cans3d/hdmlw/mlw_m_s.f:2-3 mlw_m_s(ro,pr,vx,vy,vz,bx,by,bz,dt,qav,gm,x,xm,dx,dxm,ix,y,ym,dy,dym,jx,
+ * dz,dzm,kx): spherical (r, theta, phi) = (x, y, z) MLW; x, xm the cell and face radii, y, ym the cell and
+ * face colatitudes (their sin, cos, tan are taken on the host with the C library); updated in place on the
+ * interior. */
+int cans3d_mlw_m_s(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
+                   double *bz, double dt, double qav, double gm, const double *x, const double *xm,
+                   const double *dx, const double *dxm, int ix, const double *y, const double *ym,
+                   const double *dy, const double *dym, int jx, const double *dz, const double *dzm, int kx,
+                   void *stream);
 /* cans3d/hdroe/roe_m.f:2 roe_m(ro,pr,vx,vy,vz,bx,by,bz,dt,gm,dx,ix,dy,jx,dz,kx) */
 int cans3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx,
                  double *by, double *bz, double dt, double gm, const double *dx, int ix,
@@ -149,6 +158,13 @@ int cans3d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
                  const double *vz, const double *bx, const double *by, const double *bz,
                  const double *dx, int ix, const double *dy, int jx, const double *dz, int kx,
                  void *stream);
+/* cans3d/common/cfl_m_s.f:1-2 cfl_m_s(dt,safety,dtmin,merr,gm,ro,pr,vx,vy,vz,bx,by,bz,x,dx,ix,y,dy,jx,dz,kx):
+ * cfl_m with the widths dx, x dy, x sin(y) dz. */
+int cans3d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm,
+                   const double *ro, const double *pr, const double *vx, const double *vy,
+                   const double *vz, const double *bx, const double *by, const double *bz,
+                   const double *x, const double *dx, int ix, const double *y, const double *dy, int jx,
+                   const double *dz, int kx, void *stream);
 /* cans3d/bc/bd*{x,y,z}.f; dir 0 = x, 1 = y, 2 = z */
 int cans3d_bnd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
                const double *dm, int ix, int jx, int kx, void *stream);
@@ -240,7 +256,9 @@ int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *
 #define CANS_ENGINE_HLLD 2  /* cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f (3D only) */
 #define CANS_ENGINE_MLW3 3  /* cans{1,2}d/hdmlw/mlw_m3.f with cfl_m3 (1D/2D only) */
 #define CANS_ENGINE_ROE3 4  /* cans{1,2}d/hdroe/roe_m3.f with cfl_m3 (1D/2D only) */
-#define CANS_ENGINE_MLWC 5  /* cans2d/hdmlw/mlw_m_c.f, cylindrical (r,z) (2D only; x, xm in bx1d, bxm1d) */
+#define CANS_ENGINE_MLWC 5  /* cans{2,3}d/hdmlw/mlw_m_c.f, cylindrical (2D (r,z), 3D (r,phi,z); x, xm in bx1d, bxm1d) */
+#define CANS_ENGINE_MLWS 6  /* cans3d/hdmlw/mlw_m_s.f + common/cfl_m_s.f, spherical (r,theta,phi) (3D; bx1d = the
+                             * device array x(ix), xm(ix), y(jx), ym(jx) back to back; bxm1d unused) */
 
 /* Device workspace (bytes) a cans_run of this shape needs. */
 size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx);

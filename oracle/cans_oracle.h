@@ -127,6 +127,11 @@ void or3d_mlw_m_c(double *ro, double *pr, double *vx, double *vy, double *vz,
                   double gm, const double *x, const double *xm, const double *dx, const double *dxm, int ix,
                   const double *dy, const double *dym, int jx,
                   const double *dz, const double *dzm, int kx);
+void or3d_mlw_m_s(double *ro, double *pr, double *vx, double *vy, double *vz,
+                  double *bx, double *by, double *bz, double dt, double qav,
+                  double gm, const double *x, const double *xm, const double *dx, const double *dxm, int ix,
+                  const double *y, const double *ym, const double *dy, const double *dym, int jx,
+                  const double *dz, const double *dzm, int kx);
 void or3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz,
                 double *bx, double *by, double *bz, double dt, double gm,
                 const double *dx, int ix, const double *dy, int jx,
@@ -137,6 +142,12 @@ void or3d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
                 const double *by, const double *bz, const double *dx, int ix,
                 const double *dy, int jx, const double *dz, int kx,
                 int *imin, int *jmin, int *kmin);
+void or3d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm,
+                  const double *ro, const double *pr, const double *vx,
+                  const double *vy, const double *vz, const double *bx,
+                  const double *by, const double *bz, const double *x, const double *dx, int ix,
+                  const double *y, const double *dy, int jx, const double *dz, int kx,
+                  int *imin, int *jmin, int *kmin);
 /* dir: 0 x, 1 y, 2 z; kind as or2d_bd */
 void or3d_bd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
              const double *dm, int ix, int jx, int kx);

@@ -241,6 +241,12 @@ class Oracle:
                               C.c_double(gm), _arr(x), _arr(xm), _arr(dx), _arr(dxm), C.c_int(ix), _arr(dy),
                               _arr(dym), C.c_int(jx), _arr(dz), _arr(dzm), C.c_int(kx))
 
+    def mlw_m_s_3d(self, ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm):
+        ix, jx, kx = ro.shape
+        self.lib.or3d_mlw_m_s(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz)), C.c_double(dt), C.c_double(qav),
+                              C.c_double(gm), _arr(x), _arr(xm), _arr(dx), _arr(dxm), C.c_int(ix), _arr(y),
+                              _arr(ym), _arr(dy), _arr(dym), C.c_int(jx), _arr(dz), _arr(dzm), C.c_int(kx))
+
     def roe_m_3d(self, ro, pr, vx, vy, vz, bx, by, bz, dt, gm, dx, dy, dz):
         ix, jx, kx = ro.shape
         self.lib.or3d_roe_m(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz)), C.c_double(dt), C.c_double(gm),
@@ -252,6 +258,14 @@ class Oracle:
         self.lib.or3d_cfl_m(C.byref(dt), C.c_double(safety), C.c_double(dtmin), C.byref(merr), C.c_double(gm),
                             *map(_arr, (ro, pr, vx, vy, vz, bx, by, bz)), _arr(dx), C.c_int(ix), _arr(dy),
                             C.c_int(jx), _arr(dz), C.c_int(kx), C.byref(im), C.byref(jm), C.byref(km))
+        return dt.value, merr.value, (im.value, jm.value, km.value)
+
+    def cfl_m_s_3d(self, safety, dtmin, gm, ro, pr, vx, vy, vz, bx, by, bz, x, dx, y, dy, dz):
+        ix, jx, kx = ro.shape
+        dt, merr, im, jm, km = C.c_double(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self.lib.or3d_cfl_m_s(C.byref(dt), C.c_double(safety), C.c_double(dtmin), C.byref(merr), C.c_double(gm),
+                              *map(_arr, (ro, pr, vx, vy, vz, bx, by, bz)), _arr(x), _arr(dx), C.c_int(ix), _arr(y),
+                              _arr(dy), C.c_int(jx), _arr(dz), C.c_int(kx), C.byref(im), C.byref(jm), C.byref(km))
         return dt.value, merr.value, (im.value, jm.value, km.value)
 
     def bnd_3d(self, dir, kind, mbnd, margin, qq, q0=0.0, dm=None):

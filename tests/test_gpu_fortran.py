@@ -117,7 +117,7 @@ def test_spicule_main(dev, oracle, tmp_path, resident):
 
 
 def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
-    """mlw_m3_, roe_m3_, cfl_m3_ and mlw_m_c_ of libcans{1,2,3}d_gpu.so called as a Fortran main
+    """mlw_m3_, roe_m3_, cfl_m3_, mlw_m_c_, mlw_m_s_ and cfl_m_s_ of libcans{1,2,3}d_gpu.so called as a Fortran main
     program calls them (every argument by reference, host arrays, strict
     mode), against the restatement."""
     import ctypes as C
@@ -200,3 +200,18 @@ def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
          dz, dzm, shape[2])
     for k in F3:
         assert_bitwise(b[k], a[k], k)
+    # 3D spherical mlw_m_s_ and cfl_m_s_
+    from tests.test_oracle_m3 import polar
+    y, ym = polar(shape[1], 0.7, dy)
+    a = {k: v.copy(order="F") for k, v in s3.items()}
+    b = {k: v.copy(order="F") for k, v in s3.items()}
+    oracle.mlw_m_s_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm)
+    call(lib3, "mlw_m_s", *[b[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, shape[0], y, ym, dy, dym,
+         shape[1], dz, dzm, shape[2])
+    for k in F3:
+        assert_bitwise(b[k], a[k], k)
+    ref = oracle.cfl_m_s_3d(0.4, 1e-10, 5 / 3, *[s3[k] for k in F3], x, dx, y, dy, dz)
+    dt, merr = C.c_double(), C.c_int()
+    call(lib3, "cfl_m_s", dt, 0.4, 1e-10, merr, 5 / 3, *[s3[k] for k in F3], x, dx, shape[0], y, dy, shape[1],
+         dz, shape[2])
+    assert dt.value == ref[0] and merr.value == ref[1]

@@ -350,3 +350,86 @@ def test_mlw_m_c_3d_resident_loop(oracle, dev):
     assert info["nexec"] == nsteps and info["t"] == t
     for k, g in zip(F3, cur):
         assert_bitwise(to_host(g), a[k], k)
+
+
+# ---------------------------------------------------------------- mlw_m_s, cfl_m_s, 3D (r, theta, phi)
+from tests.test_oracle_m3 import polar  # noqa: E402
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (6, 5, 7), (21, 8, 12), (40, 33, 20)])
+def test_mlw_m_s_3d_call(oracle, dev, shape):
+    s = smooth_state_3d(*shape, seed=sum(shape) + 2)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(shape[0], 0.4, dx)
+    y, ym = polar(shape[1], 0.6, dy)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_s_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_s_3d(*[d[k] for k in F3], 1e-3, 3.0, 5 / 3,
+                    *[T(v, dev) for v in (x, xm, dx, dxm, y, ym, dy, dym, dz, dzm)])
+    torch.cuda.synchronize()
+    for k in F3:
+        assert_bitwise(to_host(d[k]), a[k], k)
+
+
+def test_mlw_m_s_3d_pole_face_call(oracle, dev):
+    shape = (8, 9, 6)
+    s = smooth_state_3d(*shape, seed=29)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(shape[0], 0.5, dx)
+    y, ym = polar(shape[1], 0.0, dy)
+    ym[0] = 0.0
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_s_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_s_3d(*[d[k] for k in F3], 1e-3, 3.0, 5 / 3,
+                    *[T(v, dev) for v in (x, xm, dx, dxm, y, ym, dy, dym, dz, dzm)])
+    torch.cuda.synchronize()
+    for k in F3:
+        assert_bitwise(to_host(d[k]), a[k], k)
+    assert not np.isfinite(a["ro"][1:-1, 1, 1:-1]).all()
+
+
+def test_cfl_m_s_3d_call(oracle, dev):
+    shape = (30, 21, 17)
+    s = smooth_state_3d(*shape, seed=43)
+    dx, _, dy, _, dz, _ = grid3(*shape)
+    x, _ = radii(shape[0], 0.3, dx)
+    y, _ = polar(shape[1], 0.4, dy)
+    ref = oracle.cfl_m_s_3d(0.4, 1e-10, 5 / 3, *[s[k] for k in F3], x, dx, y, dy, dz)
+    d = [to_dev(s[k], dev) for k in F3]
+    got = cans.cfl_m_s_3d(0.4, 1e-10, 5 / 3, *d, *[T(v, dev) for v in (x, dx, y, dy, dz)])
+    assert got[0] == ref[0] and got[1] == ref[1]
+
+
+def test_mlw_m_s_3d_resident_loop(oracle, dev):
+    """10 steps of {cfl_m_s; t=t+dt; mlw_m_s; bdper x, y, z}: the spherical
+    engine as a resident run (faces periodic, a test of the loop only)."""
+    shape, margin, nsteps = (22, 18, 14), 1, 10
+    s = smooth_state_3d(*shape, seed=47)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(shape[0], 0.6, dx)
+    y, ym = polar(shape[1], 0.5, dy)
+    for dr in (0, 1, 2):
+        for k in F3:
+            oracle.bnd_3d(dr, 7, margin, margin, s[k])
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    t = 0.0
+    for _ in range(nsteps):
+        dt, merr, _ = oracle.cfl_m_s_3d(0.4, 1e-10, 5 / 3, *[a[k] for k in F3], x, dx, y, dy, dz)
+        t = t + dt
+        oracle.mlw_m_s_3d(*[a[k] for k in F3], dt, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm)
+        for dr in (0, 1, 2):
+            for k in F3:
+                oracle.bnd_3d(dr, 7, margin, margin, a[k])
+    st = [to_dev(s[k], dev) for k in F3]
+    grid = {k: T(v, dev) for k, v in zip(("dx", "dxm", "dy", "dym", "dz", "dzm"), (dx, dxm, dy, dym, dz, dzm))}
+    geo = T(np.concatenate([x, xm, y, ym]), dev)
+    run = cans.Run(3, cans.ENGINE_MLWS, st, grid, 5 / 3, 0.4, qav=3.0, margin=margin, bc=7, bx1d=geo)
+    run.reset()
+    run.steps(nsteps)
+    cur, info = run.state()
+    torch.cuda.synchronize()
+    assert info["nexec"] == nsteps and info["t"] == t
+    for k, g in zip(F3, cur):
+        assert_bitwise(to_host(g), a[k], k)

@@ -274,3 +274,89 @@ def test_mlw_m_c_3d_large_radius_limit(oracle):
     for k in F3:
         np.testing.assert_allclose(a[k][1:-1, 1:-1, 1:-1], b[k][1:-1, 1:-1, 1:-1], rtol=1e-9, atol=1e-12,
                                    err_msg=k)
+
+
+# ---------------------------------------------------------------- mlw_m_s, 3D (r, theta, phi)
+def polar(jx, t0, dy):
+    """Cell colatitudes y (cell 2 at t0 + dy/2) and face colatitudes ym."""
+    y = t0 + np.cumsum(dy) - 1.5 * dy[0] - 0.5 * dy
+    return np.asfortranarray(y), np.asfortranarray(y + 0.5 * dy)
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (6, 5, 7), (13, 11, 10), (21, 8, 12)])
+def test_mlw_m_s_3d_vs_ref(oracle, ref3d, shape):
+    ix, jx, kx = shape
+    s = smooth_state_3d(*shape, seed=sum(shape) + 2)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(ix, 0.4, dx)
+    y, ym = polar(jx, 0.6, dy)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_s_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm)
+    ref3d.call("mlw_m_s", *[b[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, ix, y, ym, dy, dym, jx,
+               dz, dzm, kx)
+    for k in F3:
+        assert_bitwise(a[k], b[k], k)
+
+
+def test_mlw_m_s_3d_pole_face(oracle, ref3d):
+    """A theta face on the pole (ym = 0, sin = 0): the corner unscaling
+    divides by zero exactly as in the reference."""
+    shape = (8, 9, 6)
+    ix, jx, kx = shape
+    s = smooth_state_3d(*shape, seed=29)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(ix, 0.5, dx)
+    y, ym = polar(jx, 0.0, dy)
+    ym[0] = 0.0
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_s_3d(*[a[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm)
+    ref3d.call("mlw_m_s", *[b[k] for k in F3], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, ix, y, ym, dy, dym, jx,
+               dz, dzm, kx)
+    for k in F3:
+        assert_bitwise(a[k], b[k], k)
+    assert not np.isfinite(a["ro"][1:-1, 1, 1:-1]).all()
+
+
+def test_mlw_m_s_3d_local_cartesian_limit(oracle):
+    """Far from the origin and near the equator, on a patch of widths
+    (dx, dy/R, dz/R): the spherical engine is the Cartesian mlw_m on (dx, dy,
+    dz) to rounding -- the metric factors become constants and the
+    geometric sources ~1/R vanish.  qav = 0 (the artificial viscosity uses
+    the coordinate widths without the metric)."""
+    shape = (12, 10, 9)
+    ix, jx, kx = shape
+    R = 1.0e8
+    s = smooth_state_3d(*shape, seed=37)
+    dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+    x, xm = radii(ix, R, dx)
+    y, ym = polar(jx, np.pi / 2, dy / R)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    b = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_s_3d(*[a[k] for k in F3], 1e-3, 0.0, 5 / 3, x, xm, dx, dxm, y, ym,
+                      np.asfortranarray(dy / R), np.asfortranarray(dym / R), np.asfortranarray(dz / R),
+                      np.asfortranarray(dzm / R))
+    oracle.mlw_m_3d(*[b[k] for k in F3], 1e-3, 0.0, 5 / 3, dx, dxm, dy, dym, dz, dzm)
+    for k in F3:
+        np.testing.assert_allclose(a[k][1:-1, 1:-1, 1:-1], b[k][1:-1, 1:-1, 1:-1], rtol=1e-6, atol=1e-9,
+                                   err_msg=k)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_cfl_m_s_3d_vs_ref(oracle, ref3d, seed):
+    shape = (12, 11, 9)
+    ix, jx, kx = shape
+    s = smooth_state_3d(*shape, seed=seed)
+    dx, _, dy, _, dz, _ = grid3(*shape)
+    x, _ = radii(ix, 0.2 * seed, dx)
+    y, _ = polar(jx, 0.1 + 0.5 * seed, dy)
+    d1 = oracle.cfl_m_s_3d(0.4, 1e-10, 5 / 3, *[s[k] for k in F3], x, dx, y, dy, dz)
+    dt, merr = C.c_double(), C.c_int()
+    ref3d.call("cfl_m_s", dt, 0.4, 1e-10, merr, 5 / 3, *[s[k] for k in F3], x, dx, ix, y, dy, jx, dz, kx)
+    assert d1[0] == dt.value and d1[1] == merr.value
+    # with x = 1 and y = pi/2 it is cfl_m up to the rounding of sin(pi/2) = 1
+    one = np.ones(ix)
+    d2 = oracle.cfl_m_s_3d(0.4, 1e-10, 5 / 3, *[s[k] for k in F3], one, dx, np.full(jx, np.pi / 2), dy, dz)
+    d3 = oracle.cfl_m_3d(0.4, 1e-10, 5 / 3, *[s[k] for k in F3], dx, dy, dz)
+    assert d2[0] == d3[0] and d2[2] == d3[2]

@@ -181,6 +181,30 @@ def mlw_m_c_3d(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, dy, 
                                 _ptr(dz, (kx,)), _ptr(dzm, (kx,)), C.c_int(kx), _stream()))
 
 
+def mlw_m_s_3d(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm):
+    """cans3d/hdmlw/mlw_m_s.f:2-3, spherical (r, theta, phi)  (in place on the interior)."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    _check(lib().cans3d_mlw_m_s(*[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                                _d(dt), _d(qav), _d(gm), _ptr(x, (ix,)), _ptr(xm, (ix,)),
+                                _ptr(dx, (ix,)), _ptr(dxm, (ix,)), C.c_int(ix), _ptr(y, (jx,)), _ptr(ym, (jx,)),
+                                _ptr(dy, (jx,)), _ptr(dym, (jx,)), C.c_int(jx),
+                                _ptr(dz, (kx,)), _ptr(dzm, (kx,)), C.c_int(kx), _stream()))
+
+
+def cfl_m_s_3d(safety, dtmin, gm, ro, pr, vx, vy, vz, bx, by, bz, x, dx, y, dy, dz):
+    """cans3d/common/cfl_m_s.f:1-2  -> (dt, merr)."""
+    kx, jx, ix = ro.shape
+    sh = (kx, jx, ix)
+    dt = C.c_double()
+    merr = C.c_int()
+    _check(lib().cans3d_cfl_m_s(C.byref(dt), _d(safety), _d(dtmin), C.byref(merr), _d(gm),
+                                *[_ptr(t, sh) for t in (ro, pr, vx, vy, vz, bx, by, bz)],
+                                _ptr(x, (ix,)), _ptr(dx, (ix,)), C.c_int(ix), _ptr(y, (jx,)), _ptr(dy, (jx,)),
+                                C.c_int(jx), _ptr(dz, (kx,)), C.c_int(kx), _stream()))
+    return dt.value, merr.value
+
+
 def roe_m_3d(ro, pr, vx, vy, vz, bx, by, bz, dt, gm, dx, dy, dz):
     """cans3d/hdroe/roe_m.f:2."""
     kx, jx, ix = ro.shape
@@ -366,7 +390,7 @@ def hlld_engine_3d_mixed(ro, pr, vx, vy, vz, bx, by, bz, gm, mu, dt, dx, dy, dz,
                                           C.c_int(ix), C.c_int(jx), C.c_int(kx), mf, _stream()))
 
 
-ENGINE_MLW, ENGINE_ROE, ENGINE_HLLD, ENGINE_MLW3, ENGINE_ROE3, ENGINE_MLWC = 0, 1, 2, 3, 4, 5
+ENGINE_MLW, ENGINE_ROE, ENGINE_HLLD, ENGINE_MLW3, ENGINE_ROE3, ENGINE_MLWC, ENGINE_MLWS = 0, 1, 2, 3, 4, 5, 6
 FIELDS = {1: ("ro", "pr", "vx", "vy", "by"), 2: ("ro", "pr", "vx", "vy", "bx", "by", "az"),
           3: ("ro", "pr", "vx", "vy", "vz", "bx", "by", "bz")}
 # fields of the three-component engines mlw_m3 / roe_m3 (ENGINE_MLW3, ENGINE_ROE3)
@@ -480,12 +504,14 @@ class Run:
 
     def steps(self, n: int, events=None):
         g = self.grid
+        # bx1d: 1D bx, or the radii x (cylindrical), or x, xm, y, ym back to back (spherical)
+        nb1 = 2 * (self.ix + self.jx) if self.engine == ENGINE_MLWS else self.ix
 
         def gp(name, ln):
             return _ptr(g[name], (ln,)) if name in g and g[name] is not None else C.c_void_p(0)
         _check(lib().cans_run(C.c_int(self.dim), C.c_int(self.engine), C.c_int(1 if self.mixed else 0),
                               self._pa, self._pb,
-                              _ptr(self.bx1d, (self.ix,)) if self.bx1d is not None else C.c_void_p(0),
+                              _ptr(self.bx1d, (nb1,)) if self.bx1d is not None else C.c_void_p(0),
                               _ptr(self.bxm1d, (self.ix,)) if self.bxm1d is not None else C.c_void_p(0),
                               self.clock.handle, C.c_int(self.clock.issued), C.c_int(n),
                               _d(self.qav), _d(self.gm), _d(self.mu), _d(self.safety),

@@ -10,6 +10,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -30,6 +31,7 @@ int abi_fail(int code, const char *fmt, ...) {
     return code;
 }
 int abi_scratch(size_t bytes, void **out);
+int host_trig(const double *y, const double *ym, int jx, double *dev, hipStream_t s);
 }  // namespace cans
 
 namespace {
@@ -77,6 +79,26 @@ inline size_t al(size_t n) { return (n + 63) & ~size_t(63); }  // doubles, 512-b
 }  // namespace
 
 int cans::abi_scratch(size_t bytes, void **out) { return scratch(bytes, out); }
+
+// sin, cos, tan of the colatitudes y (and ym) on the host -- the C library's
+// functions, as the reference's compiled intrinsics -- into dev[0..3jx)
+// (y) and dev[3jx..6jx) (ym, when given): the tables of the spherical engine.
+int cans::host_trig(const double *y, const double *ym, int jx, double *dev, hipStream_t s) {
+    std::vector<double> h(2 * (size_t)jx), t(6 * (size_t)jx, 0.0);
+    HIPCHK(hipMemcpyAsync(h.data(), y, jx * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (ym) HIPCHK(hipMemcpyAsync(h.data() + jx, ym, jx * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int side = 0; side < (ym ? 2 : 1); side++)
+        for (int j = 0; j < jx; j++) {
+            const double v = h[(size_t)side * jx + j];
+            t[(size_t)(3 * side) * jx + j] = std::sin(v);
+            t[(size_t)(3 * side + 1) * jx + j] = std::cos(v);
+            t[(size_t)(3 * side + 2) * jx + j] = std::tan(v);
+        }
+    HIPCHK(hipMemcpyAsync(dev, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
 
 extern "C" {
 
@@ -264,6 +286,24 @@ int cans3d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
     a.dx = dx; a.dy = dy; a.dz = dz;
     a.safety = safety; a.gm = gm; a.ix = ix; a.jx = jx; a.kx = kx; a.dim = 3;
     return cfl_common(a, dt, dtmin, merr, (hipStream_t)stream);
+}
+
+int cans3d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm,
+                   const double *ro, const double *pr, const double *vx, const double *vy,
+                   const double *vz, const double *bx, const double *by, const double *bz,
+                   const double *x, const double *dx, int ix, const double *y, const double *dy, int jx,
+                   const double *dz, int kx, void *stream) {
+    if (!x || !y) return fail(CANS_ERR_ARG, "cans3d_cfl_m_s: NULL x or y");
+    hipStream_t s = (hipStream_t)stream;
+    double *tab = nullptr;
+    if (int rc = scratch(64 + 6 * (size_t)jx * sizeof(double), (void **)&tab)) return rc;
+    tab += 8;                                   // the first 64 bytes hold cfl_common's key
+    if (int rc = cans::host_trig(y, nullptr, jx, tab, s)) return rc;
+    CflArgs a{};
+    a.ro = ro; a.pr = pr; a.vx = vx; a.vy = vy; a.vz = vz; a.bx = bx; a.by = by; a.bz = bz;
+    a.dx = dx; a.dy = dy; a.dz = dz; a.xr = x; a.trig = tab;
+    a.safety = safety; a.gm = gm; a.ix = ix; a.jx = jx; a.kx = kx; a.dim = 3;
+    return cfl_common(a, dt, dtmin, merr, s);
 }
 
 int cans3d_bnd(int dir, int kind, int mbnd, int margin, double *qq, double q0,

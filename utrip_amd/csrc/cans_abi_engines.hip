@@ -12,6 +12,7 @@
 namespace cans {
 int abi_fail(int code, const char *fmt, ...);
 int abi_scratch(size_t bytes, void **out);
+int host_trig(const double *y, const double *ym, int jx, double *dev, hipStream_t s);
 }  // namespace cans
 
 using namespace cans;
@@ -55,7 +56,8 @@ size_t engine_doubles(int dim, int engine, size_t N) {
     const size_t a = al(N);
     switch (engine) {
     case CANS_ENGINE_MLW: return dim == 3 ? (size_t)(kMlw3dCellFields + kMlw3dCornerFields) * a : 0;
-    case CANS_ENGINE_MLWC: return dim == 3 ? (size_t)(kMlw3dCylCellFields + kMlw3dCylCornerFields) * a : 0;
+    case CANS_ENGINE_MLWC:
+    case CANS_ENGINE_MLWS: return dim == 3 ? (size_t)(kMlw3dCylCellFields + kMlw3dCylCornerFields) * a : 0;
     case CANS_ENGINE_ROE:
     case CANS_ENGINE_ROE3: return (8 + 3 * 8) * a;
     case CANS_ENGINE_HLLD: return 4 * 8 * a;
@@ -160,20 +162,24 @@ int cans3d_roe_m_mixed(double *ro, double *pr, double *vx, double *vy, double *v
 }
 
 // ====================================================================== 3D MLW drop-in
-// mlw_m (x = NULL) or the cylindrical mlw_m_c (cell/face radii x, xm)
-static int mlw3d_dropin(const char *who, double *const *prim, const double *x, const double *xm, double dt,
-                        double qav, double gm, const double *dx, const double *dxm, int ix, const double *dy,
-                        const double *dym, int jx, const double *dz, const double *dzm, int kx, void *stream) {
+// mlw_m (x = NULL), the cylindrical mlw_m_c (cell/face radii x, xm) or the
+// spherical mlw_m_s (and the colatitudes y, ym)
+static int mlw3d_dropin(const char *who, double *const *prim, const double *x, const double *xm, const double *y,
+                        const double *ym, double dt, double qav, double gm, const double *dx, const double *dxm,
+                        int ix, const double *dy, const double *dym, int jx, const double *dz, const double *dzm,
+                        int kx, void *stream) {
     if (ix < 3 || jx < 3 || kx < 3) return abi_fail(CANS_ERR_ARG, "%s: %d x %d x %d", who, ix, jx, kx);
     for (int q = 0; q < 8; q++)
         if (!prim[q]) return abi_fail(CANS_ERR_ARG, "%s: NULL field %d", who, q);
-    const bool cyl = x != nullptr;
+    const bool cyl = x != nullptr, sph = y != nullptr;
     if (cyl && !xm) return abi_fail(CANS_ERR_ARG, "%s: NULL xm", who);
+    if (sph && (!ym || !cyl)) return abi_fail(CANS_ERR_ARG, "%s: NULL ym or radii", who);
     hipStream_t s = (hipStream_t)stream;
     const size_t N = (size_t)ix * jx * kx;
     void *w = nullptr;
     const int engine = cyl ? CANS_ENGINE_MLWC : CANS_ENGINE_MLW;
-    const size_t need = metric_doubles(ix, jx, kx) + engine_doubles(3, engine, N) + 8 * al(N);
+    const size_t need =
+        metric_doubles(ix, jx, kx) + engine_doubles(3, engine, N) + 8 * al(N) + (sph ? 6 * al(jx) : 0);
     if (int rc = abi_scratch(need * sizeof(double), &w)) return rc;
     Carver c{(double *)w};
     Metric m[3];
@@ -187,6 +193,11 @@ static int mlw3d_dropin(const char *who, double *const *prim, const double *x, c
     a.corner = c.take((size_t)(cyl ? kMlw3dCylCornerFields : kMlw3dCornerFields) * al(N));
     for (int q = 0; q < 8; q++) a.out[q] = c.take(N);
     a.x = x; a.xm = xm;
+    if (sph) {
+        double *tab = c.take(6 * (size_t)jx);
+        if (int rc = host_trig(y, ym, jx, tab, s)) return rc;
+        a.trig = tab;
+    }
     a.dxm = dxm; a.dym = dym; a.dzm = dzm; a.mx = m[0]; a.my = m[1]; a.mz = m[2];
     a.qav = qav; a.gm = gm; a.dt = dt; a.clk = nullptr; a.ix = ix; a.jx = jx; a.kx = kx;
     HIPCHK(launch_mlw3d(a, s));
@@ -201,8 +212,8 @@ int cans3d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *vz, dou
                  const double *dy, const double *dym, int jx, const double *dz, const double *dzm, int kx,
                  void *stream) {
     double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
-    return mlw3d_dropin("cans3d_mlw_m", prim, nullptr, nullptr, dt, qav, gm, dx, dxm, ix, dy, dym, jx, dz, dzm, kx,
-                        stream);
+    return mlw3d_dropin("cans3d_mlw_m", prim, nullptr, nullptr, nullptr, nullptr, dt, qav, gm, dx, dxm, ix, dy, dym,
+                        jx, dz, dzm, kx, stream);
 }
 
 int cans3d_mlw_m_c(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
@@ -211,7 +222,19 @@ int cans3d_mlw_m_c(double *ro, double *pr, double *vx, double *vy, double *vz, d
                    const double *dz, const double *dzm, int kx, void *stream) {
     if (!x || !xm) return abi_fail(CANS_ERR_ARG, "cans3d_mlw_m_c: NULL radius");
     double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
-    return mlw3d_dropin("cans3d_mlw_m_c", prim, x, xm, dt, qav, gm, dx, dxm, ix, dy, dym, jx, dz, dzm, kx, stream);
+    return mlw3d_dropin("cans3d_mlw_m_c", prim, x, xm, nullptr, nullptr, dt, qav, gm, dx, dxm, ix, dy, dym, jx, dz,
+                        dzm, kx, stream);
+}
+
+int cans3d_mlw_m_s(double *ro, double *pr, double *vx, double *vy, double *vz, double *bx, double *by,
+                   double *bz, double dt, double qav, double gm, const double *x, const double *xm,
+                   const double *dx, const double *dxm, int ix, const double *y, const double *ym,
+                   const double *dy, const double *dym, int jx, const double *dz, const double *dzm, int kx,
+                   void *stream) {
+    if (!x || !xm || !y || !ym) return abi_fail(CANS_ERR_ARG, "cans3d_mlw_m_s: NULL geometry array");
+    double *prim[8] = {ro, pr, vx, vy, vz, bx, by, bz};
+    return mlw3d_dropin("cans3d_mlw_m_s", prim, x, xm, y, ym, dt, qav, gm, dx, dxm, ix, dy, dym, jx, dz, dzm, kx,
+                        stream);
 }
 
 // ====================================================================== mlw_m3 drop-in
@@ -324,7 +347,8 @@ int cans3d_hlld_engine_mixed(double *ro, double *pr, double *vx, double *vy, dou
 // ====================================================================== resident runs
 size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx) {
     const size_t N = (size_t)ix * jx * kx;
-    return (metric_doubles(ix, jx, kx) + engine_doubles(dim, engine, N) + 64) * sizeof(double);
+    const size_t trig = engine == CANS_ENGINE_MLWS ? 6 * al(jx) : 0;
+    return (metric_doubles(ix, jx, kx) + trig + engine_doubles(dim, engine, N) + 64) * sizeof(double);
 }
 
 // One resident run of `nsteps` CANS loop iterations {cfl_m; t=t+dt; engine;
@@ -335,7 +359,9 @@ size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx) {
 // mlw_m_c: 2D 7 (ro pr vx vz bx bz ay) on the (x, z) plane (dy, dym are dz,
 // dzm; bx1d, bxm1d carry the radii x, xm), cfl_m (2D) on (vx, vz, bx, bz);
 // 3D 8 (ro pr vx vy vz bx by bz) on (r, phi, z), radii in bx1d, bxm1d,
-// cfl_m (3D) as the md_mhdsn 3D driver calls it.
+// cfl_m (3D) as the md_mhdsn 3D driver calls it;
+// mlw_m_s: 3D 8 on (r, theta, phi), bx1d = x, xm, y, ym back to back, with
+// cfl_m_s.
 int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
              const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
              double safety, const double *dx, const double *dxm, int ix, const double *dy, const double *dym,
@@ -348,7 +374,9 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         return abi_fail(CANS_ERR_ARG, "cans_run: the three-component engines are 1D/2D");
     if (engine == CANS_ENGINE_MLWC && (dim == 1 || !bx1d || !bxm1d))
         return abi_fail(CANS_ERR_ARG, "cans_run: mlw_m_c is 2D/3D and needs the radii x, xm (in bx1d, bxm1d)");
-    if (engine < CANS_ENGINE_MLW || engine > CANS_ENGINE_MLWC) return abi_fail(CANS_ERR_ARG, "cans_run: engine %d", engine);
+    if (engine == CANS_ENGINE_MLWS && (dim != 3 || !bx1d))
+        return abi_fail(CANS_ERR_ARG, "cans_run: mlw_m_s is 3D and needs x, xm, y, ym (packed in bx1d)");
+    if (engine < CANS_ENGINE_MLW || engine > CANS_ENGINE_MLWS) return abi_fail(CANS_ERR_ARG, "cans_run: engine %d", engine);
     if (engine == CANS_ENGINE_MLW && dim == 2 && !bc)
         return cans2d_mlw_run(a, b, clk, step0, nsteps, qav, gm, safety, dx, dxm, ix, dy, dym, jx, nullptr, margin,
                               (double *)ws, events, stream);
@@ -358,9 +386,10 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     const bool m3 = engine == CANS_ENGINE_MLW3;
     const bool cyl = engine == CANS_ENGINE_MLWC && dim == 2;   // fields ro pr vx vz bx bz ay
     const bool cyl3 = engine == CANS_ENGINE_MLWC && dim == 3;  // the 3D field list, radii x, xm
+    const bool sph = engine == CANS_ENGINE_MLWS;               // 3D, x xm y ym packed in bx1d
     const bool c3 = m3 || engine == CANS_ENGINE_ROE3;      // three-component field lists
     const int nf = c3 ? (dim == 1 ? 7 : 9) : (dim == 1 ? 5 : (dim == 2 ? 7 : 8));
-    const int lo = (engine == CANS_ENGINE_MLW || m3 || cyl || cyl3) ? 3 : 5;   // Roe: margin >= 3 (see the drop-in)
+    const int lo = (engine == CANS_ENGINE_MLW || m3 || cyl || cyl3 || sph) ? 3 : 5;   // Roe: margin >= 3 (drop-in)
     const int n[3] = {ix, jx, kx};
     for (int d = 0; d < dim; d++)
         if (n[d] < lo || n[d] < 2 * margin + 1) return abi_fail(CANS_ERR_ARG, "cans_run: dimension %d too small", n[d]);
@@ -370,7 +399,11 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     Carver cv{(double *)ws};
     Metric m[3];
     carve_metric(cv, ix, jx, kx, m);
-    if ((engine == CANS_ENGINE_MLW || m3 || cyl || cyl3) && step0 == 0) {   // the first call fills the metric
+    const double *gx = sph ? bx1d : nullptr, *gxm = sph ? bx1d + ix : nullptr;
+    double *trig = sph ? cv.take(6 * (size_t)jx) : nullptr;
+    if (sph && step0 == 0)   // sin, cos, tan of y and ym (mlw_m_s.f:109-112, 243, 274; cfl_m_s.f:26)
+        if (int rc = host_trig(bx1d + 2 * ix, bx1d + 2 * ix + jx, jx, trig, s)) return rc;
+    if ((engine == CANS_ENGINE_MLW || m3 || cyl || cyl3 || sph) && step0 == 0) {   // the first call fills the metric
         HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
         if (dim >= 2) HIPCHK(launch_metric(dy, dym, jx, m[1], s));
         if (dim == 3) HIPCHK(launch_metric(dz, dzm, kx, m[2], s));
@@ -403,6 +436,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         ca.ro = pin[0]; ca.pr = pin[1]; ca.vx = pin[2]; ca.vy = pin[3]; ca.vz = pin[4];
         ca.bx = dim == 1 ? bx1d : pin[5]; ca.by = pin[6]; ca.bz = pin[7];   // (3D mlw_m_c: the plain cfl_m)
         ca.dx = dx; ca.dy = dy; ca.dz = dz; ca.safety = safety; ca.gm = gm;
+        ca.xr = gx; ca.trig = trig;   // cfl_m_s for the spherical engine
         ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st; ca.comp3 = c3 ? 1 : 0;
         HIPCHK(launch_cfl(ca, s));
         HIPCHK(launch_clock_step(c, st, s));
@@ -420,6 +454,16 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
             g.bx = bx1d; g.bxm = bxm1d; g.dxm = dxm; g.dym = dym; g.mx = m[0]; g.my = m[1];
             g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx;
             HIPCHK(launch_mlw_m3(g, dim, s));
+        } else if (sph) {
+            Mlw3dArgs g{};
+            for (int q = 0; q < 8; q++) { g.prim[q] = pin[q]; g.out[q] = pout[q]; }
+            Carver ec{eng};
+            g.cell = ec.take((size_t)kMlw3dCylCellFields * N);
+            g.corner = ec.take((size_t)kMlw3dCylCornerFields * N);
+            g.x = gx; g.xm = gxm; g.trig = trig;
+            g.dxm = dxm; g.dym = dym; g.dzm = dzm; g.mx = m[0]; g.my = m[1]; g.mz = m[2];
+            g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx; g.kx = kx;
+            HIPCHK(launch_mlw3d(g, s));
         } else if (cyl3) {
             Mlw3dArgs g{};
             for (int q = 0; q < 8; q++) { g.prim[q] = pin[q]; g.out[q] = pout[q]; }

@@ -100,7 +100,8 @@ __global__ void bnd_kernel(BndArgs a) {
 //     pi4i = 1./pi/4.
 // cfl_m3 (comp3): the 1D/2D formulas with bz in b2 and (2D) vz in v2.
 // 3D: cans3d/common/cfl_m.f:39-58, all cells, min(dx,dy,dz)/sqrt(...),
-//     onep4 = 0.25/pai with pai the REAL*4 acos(-1.0).
+//     onep4 = 0.25/pai with pai the REAL*4 acos(-1.0); spherical cfl_m_s
+//     (trig set): min(dx, x dy, x sin(y) dz).
 CANS_DEV double cfl_cell(const CflArgs &a, int i, int j, int k, size_t p) {
     const double gm = a.gm;
     if (a.dim == 1) {
@@ -129,6 +130,12 @@ CANS_DEV double cfl_cell(const CflArgs &a, int i, int j, int k, size_t p) {
         const double ca2 = b2 * pi4i / a.ro[p];
         const double cs2 = gm * a.pr[p] / a.ro[p];
         const double v2 = a.vx[p] * a.vx[p] + a.vy[p] * a.vy[p] + a.vz[p] * a.vz[p];
+        if (a.trig) {   // cans3d/common/cfl_m_s.f:26-27: min(dx, x dy, x sin(y) dz)
+            const double r = a.xr[i - 1];
+            const double dtcfl = fminF(fminF(a.dx[i - 1], r * a.dy[j - 1]), r * a.trig[j - 1] * a.dz[k - 1])
+                                 / __builtin_sqrt(v2 + cs2 + ca2);
+            return a.safety * dtcfl;
+        }
         const double dtcfl = fminF(fminF(a.dx[i - 1], a.dy[j - 1]), a.dz[k - 1]) / __builtin_sqrt(v2 + cs2 + ca2);
         return a.safety * dtcfl;
     }

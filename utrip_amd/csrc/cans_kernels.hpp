@@ -102,6 +102,7 @@ struct CflArgs {
     int ix, jx, kx;
     int dim;                    // 1, 2 or 3 (selects the reference's formula)
     int comp3;                  // 1: cfl_m3 of 1D/2D (adds vz, bz)
+    const double *xr, *trig;    // 3D spherical cfl_m_s: radii x and sin(y) (trig[0..jx)); NULL: cfl_m
     unsigned long long *key;    // order-preserving key of the min (pre-armed)
     Clock *clk;                 // non-null: reduce into clk->dtkey[step%3] if the step runs
     int step;

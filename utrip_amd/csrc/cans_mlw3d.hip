@@ -10,11 +10,17 @@
 //            the corner fluxes (mlw_m.f:193-315), 18 values per corner;
 //   update   centre term of mlwhalf, mlwfull.f:34-55, mlwartv.f:36-48,
 //            update and primitive recovery (mlw_m.f:316-453).
-// The same kernels, with CYL = true, do the cylindrical (r, phi, z) engine
-// cans3d/hdmlw/mlw_m_c.f: phi fluxes divided by the radius (x at cells, xm
-// at corners) in slots of their own, and the geometric sources of ro, ee,
-// rx, ry, rz, bz -- at cells into the corner half step (mlwsrch.f:39-46) and
-// the centre term (mlwsrch.f:33-38), at corners into the update
+// The same kernels do the curvilinear engines (template G):
+//   G = 1  cylindrical (r, phi, z), cans3d/hdmlw/mlw_m_c.f: phi fluxes divided
+//          by the radius (x at cells, xm at corners) in slots of their own,
+//          and the geometric sources of ro, ee, rx, ry, rz, bz;
+//   G = 2  spherical (r, theta, phi), cans3d/hdmlw/mlw_m_s.f: the conserved
+//          variables and fluxes carry the metric factors (r^2 sin, r sin, r,
+//          ...), the momenta the geometric sources; the state is unscaled at
+//          the corners and after the update.  sin, cos, tan of the
+//          colatitudes come as per-row tables made on the host.
+// Sources enter at cells into the corner half step (mlwsrch.f:39-46) and the
+// centre term (mlwsrch.f:33-38), at corners into the update
 // (mlwsrcf.f:33-46).
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
@@ -83,6 +89,46 @@ __constant__ int kCornFluxC[8][3][2] = {
 __constant__ int kCellSrc[8] = {Y_SRO, Y_SEE, Y_SRX, Y_SRY, Y_SRZ, -1, -1, Y_SBZ};
 __constant__ int kCornSrc[8] = {Z_SRO, Z_SEE, Z_SRX, Z_SRY, Z_SRZ, -1, -1, Z_SBZ};
 
+// spherical cell slots (mlw_m_s.f:158-321): scaled conserved variables in
+// 0..7, then the scaled x, y, z fluxes and the momentum sources
+enum { P_XRO = 8, P_XEE, P_XRX, P_XRY, P_XRZ, P_XBY, P_XBZ,
+       P_YRO, P_YEE, P_YRX, P_YRY, P_YRZ, P_YBX, P_YBZ,
+       P_ZRO, P_ZEE, P_ZRX, P_ZRY, P_ZRZ, P_ZBX, P_ZBY,
+       P_SRX, P_SRY, P_SRZ, P_NCELL };
+static_assert(P_NCELL <= kMlw3dCylCellFields, "spherical cell slots");
+// spherical corner slots (mlw_m_s.f:356-503)
+enum { Q_XRO, Q_XEE, Q_XRX, Q_XRY, Q_XRZ, Q_XBY, Q_XBZ,
+       Q_YRO, Q_YEE, Q_YRX, Q_YRY, Q_YRZ, Q_YBX, Q_YBZ,
+       Q_ZRO, Q_ZEE, Q_ZRX, Q_ZRY, Q_ZRZ, Q_ZBX, Q_ZBY,
+       Q_SRX, Q_SRY, Q_SRZ, Q_NCORN };
+static_assert(Q_NCORN <= kMlw3dCylCornerFields, "spherical corner slots");
+__constant__ int kCellFluxS[8][3][2] = {
+    {{P_XRO, 1}, {P_YRO, 1}, {P_ZRO, 1}},
+    {{P_XEE, 1}, {P_YEE, 1}, {P_ZEE, 1}},
+    {{P_XRX, 1}, {P_YRX, 1}, {P_ZRX, 1}},
+    {{P_XRY, 1}, {P_YRY, 1}, {P_ZRY, 1}},
+    {{P_XRZ, 1}, {P_YRZ, 1}, {P_ZRZ, 1}},
+    {{-1, 1}, {P_YBX, 1}, {P_ZBX, 1}},
+    {{P_XBY, 1}, {-1, 1}, {P_ZBY, 1}},
+    {{P_XBZ, 1}, {P_YBZ, 1}, {-1, 1}},
+};
+__constant__ int kCornFluxS[8][3][2] = {
+    {{Q_XRO, 1}, {Q_YRO, 1}, {Q_ZRO, 1}},
+    {{Q_XEE, 1}, {Q_YEE, 1}, {Q_ZEE, 1}},
+    {{Q_XRX, 1}, {Q_YRX, 1}, {Q_ZRX, 1}},
+    {{Q_XRY, 1}, {Q_YRY, 1}, {Q_ZRY, 1}},
+    {{Q_XRZ, 1}, {Q_YRZ, 1}, {Q_ZRZ, 1}},
+    {{-1, 1}, {Q_YBX, 1}, {Q_ZBX, 1}},
+    {{Q_XBY, 1}, {-1, 1}, {Q_ZBY, 1}},
+    {{Q_XBZ, 1}, {Q_YBZ, 1}, {-1, 1}},
+};
+__constant__ int kCellSrcS[8] = {-1, -1, P_SRX, P_SRY, P_SRZ, -1, -1, -1};
+__constant__ int kCornSrcS[8] = {-1, -1, Q_SRX, Q_SRY, Q_SRZ, -1, -1, -1};
+
+enum { G_CART = 0, G_CYL = 1, G_SPH = 2 };
+template <int G> CANS_DEV int csrc(int q) { return G == G_CYL ? kCellSrc[q] : (G == G_SPH ? kCellSrcS[q] : -1); }
+template <int G> CANS_DEV int hsrc(int q) { return G == G_CYL ? kCornSrc[q] : (G == G_SPH ? kCornSrcS[q] : -1); }
+
 __constant__ int kCornFlux[8][3][2] = {
     {{H_ROX, 1}, {H_ROY, 1}, {H_ROZ, 1}},
     {{H_EEX, 1}, {H_EEY, 1}, {H_EEZ, 1}},
@@ -102,7 +148,7 @@ CANS_DEV bool step_dt3(const Mlw3dArgs &a, double &dt) {
     return true;
 }
 
-template <bool CYL>
+template <int G>
 __global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
     double dt;
     if (!step_dt3(a, dt)) return;
@@ -116,6 +162,62 @@ __global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
 #define W(slot, v) c[(size_t)(slot) * N + p] = (v)
     const double vv = vx * vx + vy * vy + vz * vz;
     const double bb = bx * bx + by * by + bz * bz;
+    if (G == G_SPH) {   // mlw_m_s.f:137-321, r = x(i), st = sin(y(j))
+        const size_t i0 = p % (size_t)a.ix, j0 = (p / (size_t)a.ix) % (size_t)a.jx;
+        const double r = a.x[i0];
+        const double st = a.trig[j0], ct = a.trig[a.jx + j0], tt = a.trig[2 * a.jx + j0];
+        const double r2 = r * r;
+        const double ee = pr / (gm - 1.0) + 0.5 * ro * vv + pi8i * bb;
+        W(0, ro * r2 * st);
+        W(1, ee * r2 * st);
+        W(2, ro * vx * r2 * st);
+        W(3, ro * vy * r * st);
+        W(4, ro * vz * r);
+        W(5, bx * r2 * st);
+        W(6, by * r * st);
+        W(7, bz * r);
+        const double ex = -vy * bz + vz * by;
+        const double ey = -vz * bx + vx * bz;
+        const double ez = -vx * by + vy * bx;
+        W(P_XRO, ro * vx * r2 * st);
+        W(P_YRO, ro * vy * r * st);
+        W(P_ZRO, ro * vz * r);
+        const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
+        const double px = (bz * ey - by * ez) * pi4i;
+        const double py = (bx * ez - bz * ex) * pi4i;
+        const double pz = (by * ex - bx * ey) * pi4i;
+        W(P_XEE, (ep * vx + px) * r2 * st);
+        W(P_YEE, (ep * vy + py) * r * st);
+        W(P_ZEE, (ep * vz + pz) * r);
+        const double rxx = ro * (vx * vx) + pr + pi8i * (by * by + bz * bz - bx * bx);
+        const double rxy = ro * vx * vy - pi4i * bx * by;
+        const double rxz = ro * vx * vz - pi4i * bx * bz;
+        W(P_XRX, rxx * r2 * st);
+        W(P_YRX, rxy * r * st);
+        W(P_ZRX, rxz * r);
+        W(P_SRX, (ro * (vy * vy + vz * vz) + 2.0 * pr + pi4i * (bx * bx)) * r * st);
+        const double ryx = ro * vy * vx - pi4i * by * bx;
+        const double ryy = ro * (vy * vy) + pr + pi8i * (bx * bx + bz * bz - by * by);
+        const double ryz = ro * vy * vz - pi4i * by * bz;
+        const double rzz = ro * (vz * vz) + pr + pi8i * (bx * bx + by * by - bz * bz);
+        W(P_XRY, ryx * r * st);
+        W(P_YRY, ryy * st);
+        W(P_ZRY, ryz);
+        W(P_SRY, rxy * (-2.0) * st + rzz * ct);
+        const double rzx = ro * vz * vx - pi4i * bz * bx;
+        const double rzy = ro * vz * vy - pi4i * bz * by;
+        W(P_XRZ, rzx * r);
+        W(P_YRZ, rzy);
+        W(P_ZRZ, rzz / st);
+        W(P_SRZ, rxz * (-2.0) + ryz * (-2.0) / tt);
+        W(P_YBX, ez * r * st);
+        W(P_ZBX, -ey * r);
+        W(P_XBY, -ez * r * st);
+        W(P_ZBY, ex);
+        W(P_XBZ, ey * r);
+        W(P_YBZ, -ex);
+        return;
+    }
     W(C_RO, ro);
     W(C_EE, pr / (gm - 1.0) + 0.5 * ro * vv + pi8i * bb);
     W(C_RX, ro * vx);
@@ -128,7 +230,7 @@ __global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
     const double ey = -vz * bx + vx * bz;
     const double ez = -vx * by + vy * bx;
     const double ep = pr * gm / (gm - 1.0) + 0.5 * ro * vv;
-    if (CYL) {   // mlw_m_c.f:109-243, r = x(i)
+    if (G == G_CYL) {   // mlw_m_c.f:109-243, r = x(i)
         const double r = a.x[p % (size_t)a.ix];
         const double fro = ro * vx;
         W(Y_FRO, ro * vy / r);
@@ -177,23 +279,25 @@ __global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
 #undef W
 }
 
-template <bool CYL>
+template <int G>
 CANS_DEV double cflux(const double *c, size_t N, int q, int d, size_t p) {
-    const int slot = CYL ? kCellFluxC[q][d][0] : kCellFlux[q][d][0];
+    const int(*t)[3][2] = G == G_CYL ? kCellFluxC : (G == G_SPH ? kCellFluxS : kCellFlux);
+    const int slot = t[q][d][0];
     if (slot < 0) return 0.0;
     const double v = c[(size_t)slot * N + p];
-    return (CYL ? kCellFluxC[q][d][1] : kCellFlux[q][d][1]) < 0 ? -v : v;
+    return t[q][d][1] < 0 ? -v : v;
 }
-template <bool CYL>
+template <int G>
 CANS_DEV double hflux(const double *h, size_t N, int q, int d, size_t p) {
-    const int slot = CYL ? kCornFluxC[q][d][0] : kCornFlux[q][d][0];
+    const int(*t)[3][2] = G == G_CYL ? kCornFluxC : (G == G_SPH ? kCornFluxS : kCornFlux);
+    const int slot = t[q][d][0];
     if (slot < 0) return 0.0;
     const double v = h[(size_t)slot * N + p];
-    return (CYL ? kCornFluxC[q][d][1] : kCornFlux[q][d][1]) < 0 ? -v : v;
+    return t[q][d][1] < 0 ? -v : v;
 }
 
 // corner (i,j,k) between cells i..i+1, j..j+1, k..k+1, i,j,k in [1,n-1]
-template <bool CYL>
+template <int G>
 __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
     double dt;
     if (!step_dt3(a, dt)) return;
@@ -216,29 +320,44 @@ __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
     for (int q = 0; q < 8; q++) {
         const double *u = c + (size_t)q * N;    // C_RO..C_BZ are slots 0..7
         const double uh = (u[p100] + u[p000] + u[p110] + u[p010] + u[p101] + u[p001] + u[p111] + u[p011]) / 8.0;
-        const double dfdx = (cflux<CYL>(c, N, q, 0, p100) - cflux<CYL>(c, N, q, 0, p000) + cflux<CYL>(c, N, q, 0, p110)
-                             - cflux<CYL>(c, N, q, 0, p010) + cflux<CYL>(c, N, q, 0, p101) - cflux<CYL>(c, N, q, 0, p001)
-                             + cflux<CYL>(c, N, q, 0, p111) - cflux<CYL>(c, N, q, 0, p011)) / 4.0 * dxim;
-        const double dfdy = (cflux<CYL>(c, N, q, 1, p010) - cflux<CYL>(c, N, q, 1, p000) + cflux<CYL>(c, N, q, 1, p110)
-                             - cflux<CYL>(c, N, q, 1, p100) + cflux<CYL>(c, N, q, 1, p011) - cflux<CYL>(c, N, q, 1, p001)
-                             + cflux<CYL>(c, N, q, 1, p111) - cflux<CYL>(c, N, q, 1, p101)) / 4.0 * dyim;
-        const double dfdz = (cflux<CYL>(c, N, q, 2, p001) - cflux<CYL>(c, N, q, 2, p000) + cflux<CYL>(c, N, q, 2, p101)
-                             - cflux<CYL>(c, N, q, 2, p100) + cflux<CYL>(c, N, q, 2, p011) - cflux<CYL>(c, N, q, 2, p010)
-                             + cflux<CYL>(c, N, q, 2, p111) - cflux<CYL>(c, N, q, 2, p110)) / 4.0 * dzim;
+        const double dfdx = (cflux<G>(c, N, q, 0, p100) - cflux<G>(c, N, q, 0, p000) + cflux<G>(c, N, q, 0, p110)
+                             - cflux<G>(c, N, q, 0, p010) + cflux<G>(c, N, q, 0, p101) - cflux<G>(c, N, q, 0, p001)
+                             + cflux<G>(c, N, q, 0, p111) - cflux<G>(c, N, q, 0, p011)) / 4.0 * dxim;
+        const double dfdy = (cflux<G>(c, N, q, 1, p010) - cflux<G>(c, N, q, 1, p000) + cflux<G>(c, N, q, 1, p110)
+                             - cflux<G>(c, N, q, 1, p100) + cflux<G>(c, N, q, 1, p011) - cflux<G>(c, N, q, 1, p001)
+                             + cflux<G>(c, N, q, 1, p111) - cflux<G>(c, N, q, 1, p101)) / 4.0 * dyim;
+        const double dfdz = (cflux<G>(c, N, q, 2, p001) - cflux<G>(c, N, q, 2, p000) + cflux<G>(c, N, q, 2, p101)
+                             - cflux<G>(c, N, q, 2, p100) + cflux<G>(c, N, q, 2, p011) - cflux<G>(c, N, q, 2, p010)
+                             + cflux<G>(c, N, q, 2, p111) - cflux<G>(c, N, q, 2, p110)) / 4.0 * dzim;
         un[q] = uh - dt * dfdx - dt * dfdy - dt * dfdz;
-        if (CYL && kCellSrc[q] >= 0) {   // mlwsrch.f:39-46
-            const double *sc = c + (size_t)kCellSrc[q] * N;
+        if (csrc<G>(q) >= 0) {   // mlwsrch.f:39-46
+            const double *sc = c + (size_t)csrc<G>(q) * N;
             const double sh = (sc[p100] + sc[p000] + sc[p110] + sc[p010] + sc[p101] + sc[p001] + sc[p111] + sc[p011])
                               / 8.0;
             un[q] = un[q] + dt * sh;
         }
     }
     const double gm = a.gm;
-    const double roh = un[0], eeh = un[1], rxh = un[2], ryh = un[3], rzh = un[4];
-    const double bxh = un[5], byh = un[6], bzh = un[7];
-    const double vxh = rxh / roh;
-    const double vyh = ryh / roh;
-    const double vzh = rzh / roh;
+    double roh, eeh, vxh, vyh, vzh, bxh, byh, bzh;
+    double r = 0.0, r2 = 0.0, st = 0.0;
+    if (G == G_SPH) {   // mlw_m_s.f:325-343, r = xm(i), st = sin(ym(j))
+        r = a.xm[i - 1];
+        st = a.trig[3 * a.jx + j - 1];
+        r2 = r * r;
+        roh = un[0] / r2 / st;
+        eeh = un[1] / r2 / st;
+        vxh = un[2] / roh / r2 / st;
+        vyh = un[3] / roh / r / st;
+        vzh = un[4] / roh / r;
+        bxh = un[5] / r2 / st;
+        byh = un[6] / r / st;
+        bzh = un[7] / r;
+    } else {
+        roh = un[0]; eeh = un[1]; bxh = un[5]; byh = un[6]; bzh = un[7];
+        vxh = un[2] / roh;
+        vyh = un[3] / roh;
+        vzh = un[4] / roh;
+    }
     const double vv = vxh * vxh + vyh * vyh + vzh * vzh;
     const double bb = bxh * bxh + byh * byh + bzh * bzh;
     const double prh = (gm - 1.0) * (eeh - 0.5 * roh * vv - pi8i * bb);
@@ -249,7 +368,47 @@ __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
     double *h = a.corner;
     const size_t pc = p000;   // corner (i,j,k) stored at the index of cell (i,j,k)
 #define W(slot, v) h[(size_t)(slot) * N + pc] = (v)
-    if (CYL) {   // mlw_m_c.f:273-408, r = xm(i)
+    if (G == G_SPH) {   // mlw_m_s.f:356-503
+        const double ct = a.trig[4 * a.jx + j - 1], tt = a.trig[5 * a.jx + j - 1];
+        W(Q_XRO, roh * vxh * r2 * st);
+        W(Q_YRO, roh * vyh * r * st);
+        W(Q_ZRO, roh * vzh * r);
+        const double px = (bzh * eyh - byh * ezh) * pi4i;
+        const double py = (bxh * ezh - bzh * exh) * pi4i;
+        const double pz = (byh * exh - bxh * eyh) * pi4i;
+        W(Q_XEE, (ep * vxh + px) * r2 * st);
+        W(Q_YEE, (ep * vyh + py) * r * st);
+        W(Q_ZEE, (ep * vzh + pz) * r);
+        const double rxx = roh * (vxh * vxh) + prh + pi8i * (byh * byh + bzh * bzh - bxh * bxh);
+        const double rxy = roh * vxh * vyh - pi4i * bxh * byh;
+        const double rxz = roh * vxh * vzh - pi4i * bxh * bzh;
+        W(Q_XRX, rxx * r2 * st);
+        W(Q_YRX, rxy * r * st);
+        W(Q_ZRX, rxz * r);
+        W(Q_SRX, (roh * (vyh * vyh + vzh * vzh) + 2.0 * prh + pi4i * (bxh * bxh)) * r * st);
+        const double ryx = roh * vyh * vxh - pi4i * byh * bxh;
+        const double ryy = roh * (vyh * vyh) + prh + pi8i * (bxh * bxh + bzh * bzh - byh * byh);
+        const double ryz = roh * vyh * vzh - pi4i * byh * bzh;
+        const double rzz = roh * (vzh * vzh) + prh + pi8i * (bxh * bxh + byh * byh - bzh * bzh);
+        W(Q_XRY, ryx * r * st);
+        W(Q_YRY, ryy * st);
+        W(Q_ZRY, ryz);
+        W(Q_SRY, rxy * (-2.0) * st + rzz * ct);
+        const double rzx = roh * vzh * vxh - pi4i * bzh * bxh;
+        const double rzy = roh * vzh * vyh - pi4i * bzh * byh;
+        W(Q_XRZ, rzx * r);
+        W(Q_YRZ, rzy);
+        W(Q_ZRZ, rzz / st);
+        W(Q_SRZ, rxz * (-2.0) + ryz * (-2.0) / tt);
+        W(Q_YBX, ezh * r * st);
+        W(Q_ZBX, -eyh * r);
+        W(Q_XBY, -ezh * r * st);
+        W(Q_ZBY, exh);
+        W(Q_XBZ, eyh * r);
+        W(Q_YBZ, -exh);
+        return;
+    }
+    if (G == G_CYL) {   // mlw_m_c.f:273-408, r = xm(i)
         const double r = a.xm[i - 1];
         const double fro = roh * vxh;
         W(Z_ROX, fro);
@@ -304,7 +463,7 @@ __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
 }
 
 // interior cell (i,j,k) in [2,n-1]^3
-template <bool CYL>
+template <int G>
 __global__ __launch_bounds__(256) void mlw3d_update(Mlw3dArgs a) {
     double dt;
     if (!step_dt3(a, dt)) return;
@@ -343,26 +502,26 @@ __global__ __launch_bounds__(256) void mlw3d_update(Mlw3dArgs a) {
     for (int q = 0; q < 8; q++) {
         double du = 0.0;
         // mlwhalf centre term
-        du = du - 0.5 * dt * (0.5 * dxi * (cflux<CYL>(c, N, q, 0, p + sx) - cflux<CYL>(c, N, q, 0, p - sx)))
-                - 0.5 * dt * (0.5 * dyi * (cflux<CYL>(c, N, q, 1, p + sy) - cflux<CYL>(c, N, q, 1, p - sy)))
-                - 0.5 * dt * (0.5 * dzi * (cflux<CYL>(c, N, q, 2, p + sz) - cflux<CYL>(c, N, q, 2, p - sz)));
-        if (CYL && kCellSrc[q] >= 0) du = du + 0.5 * dt * c[(size_t)kCellSrc[q] * N + p];   // mlwsrch.f:33-38
+        du = du - 0.5 * dt * (0.5 * dxi * (cflux<G>(c, N, q, 0, p + sx) - cflux<G>(c, N, q, 0, p - sx)))
+                - 0.5 * dt * (0.5 * dyi * (cflux<G>(c, N, q, 1, p + sy) - cflux<G>(c, N, q, 1, p - sy)))
+                - 0.5 * dt * (0.5 * dzi * (cflux<G>(c, N, q, 2, p + sz) - cflux<G>(c, N, q, 2, p - sz)));
+        if (csrc<G>(q) >= 0) du = du + 0.5 * dt * c[(size_t)csrc<G>(q) * N + p];   // mlwsrch.f:33-38
         // mlwfull
-        const double dfdx = dxi * (uz0 * (uy0 * (hflux<CYL>(h, N, q, 0, c100) - hflux<CYL>(h, N, q, 0, c000))
-                                          + uy1 * (hflux<CYL>(h, N, q, 0, c110) - hflux<CYL>(h, N, q, 0, c010)))
-                                   + uz1 * (uy0 * (hflux<CYL>(h, N, q, 0, c101) - hflux<CYL>(h, N, q, 0, c001))
-                                            + uy1 * (hflux<CYL>(h, N, q, 0, c111) - hflux<CYL>(h, N, q, 0, c011))));
-        const double dfdy = dyi * (ux0 * (uz0 * (hflux<CYL>(h, N, q, 1, c010) - hflux<CYL>(h, N, q, 1, c000))
-                                          + uz1 * (hflux<CYL>(h, N, q, 1, c011) - hflux<CYL>(h, N, q, 1, c001)))
-                                   + ux1 * (uz0 * (hflux<CYL>(h, N, q, 1, c110) - hflux<CYL>(h, N, q, 1, c100))
-                                            + uz1 * (hflux<CYL>(h, N, q, 1, c111) - hflux<CYL>(h, N, q, 1, c101))));
-        const double dfdz = dzi * (uy0 * (ux0 * (hflux<CYL>(h, N, q, 2, c001) - hflux<CYL>(h, N, q, 2, c000))
-                                          + ux1 * (hflux<CYL>(h, N, q, 2, c101) - hflux<CYL>(h, N, q, 2, c100)))
-                                   + uy1 * (ux0 * (hflux<CYL>(h, N, q, 2, c011) - hflux<CYL>(h, N, q, 2, c010))
-                                            + ux1 * (hflux<CYL>(h, N, q, 2, c111) - hflux<CYL>(h, N, q, 2, c110))));
+        const double dfdx = dxi * (uz0 * (uy0 * (hflux<G>(h, N, q, 0, c100) - hflux<G>(h, N, q, 0, c000))
+                                          + uy1 * (hflux<G>(h, N, q, 0, c110) - hflux<G>(h, N, q, 0, c010)))
+                                   + uz1 * (uy0 * (hflux<G>(h, N, q, 0, c101) - hflux<G>(h, N, q, 0, c001))
+                                            + uy1 * (hflux<G>(h, N, q, 0, c111) - hflux<G>(h, N, q, 0, c011))));
+        const double dfdy = dyi * (ux0 * (uz0 * (hflux<G>(h, N, q, 1, c010) - hflux<G>(h, N, q, 1, c000))
+                                          + uz1 * (hflux<G>(h, N, q, 1, c011) - hflux<G>(h, N, q, 1, c001)))
+                                   + ux1 * (uz0 * (hflux<G>(h, N, q, 1, c110) - hflux<G>(h, N, q, 1, c100))
+                                            + uz1 * (hflux<G>(h, N, q, 1, c111) - hflux<G>(h, N, q, 1, c101))));
+        const double dfdz = dzi * (uy0 * (ux0 * (hflux<G>(h, N, q, 2, c001) - hflux<G>(h, N, q, 2, c000))
+                                          + ux1 * (hflux<G>(h, N, q, 2, c101) - hflux<G>(h, N, q, 2, c100)))
+                                   + uy1 * (ux0 * (hflux<G>(h, N, q, 2, c011) - hflux<G>(h, N, q, 2, c010))
+                                            + ux1 * (hflux<G>(h, N, q, 2, c111) - hflux<G>(h, N, q, 2, c110))));
         du = du - 0.5 * dt * dfdx - 0.5 * dt * dfdy - 0.5 * dt * dfdz;
-        if (CYL && kCornSrc[q] >= 0) {   // mlwsrcf.f:33-46
-            const double *sh_ = h + (size_t)kCornSrc[q] * N;
+        if (hsrc<G>(q) >= 0) {   // mlwsrcf.f:33-46
+            const double *sh_ = h + (size_t)hsrc<G>(q) * N;
             const double sh = uz0 * (uy0 * (ux0 * sh_[c000] + ux1 * sh_[c100]) + uy1 * (ux0 * sh_[c010] + ux1 * sh_[c110]))
                               + uz1 * (uy0 * (ux0 * sh_[c001] + ux1 * sh_[c101])
                                        + uy1 * (ux0 * sh_[c011] + ux1 * sh_[c111]));
@@ -377,6 +536,28 @@ __global__ __launch_bounds__(256) void mlw3d_update(Mlw3dArgs a) {
         U[q] = uc + du;
     }
     const double gm = a.gm;
+    if (G == G_SPH) {   // mlw_m_s.f:570-586, r = x(i), st = sin(y(j))
+        const double r = a.x[i - 1], st = a.trig[j - 1], r2 = r * r;
+        const double ro = U[0] / r2 / st;
+        const double ee = U[1] / r2 / st;
+        const double vx = U[2] / ro / r2 / st;
+        const double vy = U[3] / ro / r / st;
+        const double vz = U[4] / ro / r;
+        const double bx = U[5] / r2 / st;
+        const double by = U[6] / r / st;
+        const double bz = U[7] / r;
+        const double vv = vx * vx + vy * vy + vz * vz;
+        const double bb = bx * bx + by * by + bz * bz;
+        a.out[0][p] = ro;
+        a.out[1][p] = (gm - 1.0) * (ee - 0.5 * ro * vv - pi8i * bb);
+        a.out[2][p] = vx;
+        a.out[3][p] = vy;
+        a.out[4][p] = vz;
+        a.out[5][p] = bx;
+        a.out[6][p] = by;
+        a.out[7][p] = bz;
+        return;
+    }
     const double ro = U[0];
     const double vx = U[2] / ro;
     const double vy = U[3] / ro;
@@ -400,15 +581,21 @@ hipError_t launch_mlw3d(const Mlw3dArgs &a, hipStream_t s) {
     const size_t N = (size_t)a.ix * a.jx * a.kx;
     const size_t nc = (size_t)(a.ix - 1) * (a.jx - 1) * (a.kx - 1);
     const size_t nu = (size_t)(a.ix - 2) * (a.jx - 2) * (a.kx - 2);
-    if (a.x) {
-        if (!a.xm) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(mlw3d_cells<true>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(mlw3d_corners<true>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(mlw3d_update<true>, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, a);
+    if (a.x && !a.xm) return hipErrorInvalidValue;
+    if (a.trig && !a.x) return hipErrorInvalidValue;
+    const dim3 gc((unsigned)((N + 255) / 256)), gk((unsigned)((nc + 255) / 256)), gu((unsigned)((nu + 255) / 256));
+    if (a.trig) {
+        hipLaunchKernelGGL(mlw3d_cells<G_SPH>, gc, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_corners<G_SPH>, gk, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_update<G_SPH>, gu, dim3(256), 0, s, a);
+    } else if (a.x) {
+        hipLaunchKernelGGL(mlw3d_cells<G_CYL>, gc, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_corners<G_CYL>, gk, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_update<G_CYL>, gu, dim3(256), 0, s, a);
     } else {
-        hipLaunchKernelGGL(mlw3d_cells<false>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(mlw3d_corners<false>, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(mlw3d_update<false>, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_cells<G_CART>, gc, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_corners<G_CART>, gk, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mlw3d_update<G_CART>, gu, dim3(256), 0, s, a);
     }
     return hipGetLastError();
 }

@@ -31,9 +31,10 @@ hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s, bool
 struct Mlw3dArgs {
     const double *prim[8];   // ro pr vx vy vz bx by bz (in)
     double *out[8];          // same (out), interior cells
-    double *cell;            // scratch: 23 (cylindrical 32) doubles per cell (SoA)
-    double *corner;          // scratch: 18 (cylindrical 26) doubles per corner (SoA)
-    const double *x, *xm;    // non-null: cylindrical mlw_m_c, cell and face radii
+    double *cell;            // scratch: 23 (curvilinear 32) doubles per cell (SoA)
+    double *corner;          // scratch: 18 (curvilinear 26) doubles per corner (SoA)
+    const double *x, *xm;    // non-null: curvilinear (mlw_m_c / mlw_m_s), cell and face radii
+    const double *trig;      // non-null: spherical mlw_m_s; 6 tables of jx: sin, cos, tan of y, then of ym
     const double *dxm, *dym, *dzm;
     Metric mx, my, mz;
     double qav, gm, dt;

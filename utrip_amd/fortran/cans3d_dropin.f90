@@ -1,6 +1,7 @@
 !=======================================================================
 ! libcans3d_gpu -- the cans3d hot-path routines with the reference's names
-! and argument lists (cans3d/hdmlw/mlw_m.f:2-3, cans3d/hdmlw/mlw_m_c.f:2-3, cans3d/hdroe/roe_m.f:2,
+! and argument lists (cans3d/hdmlw/mlw_m.f:2-3, cans3d/hdmlw/mlw_m_c.f:2-3, cans3d/hdmlw/mlw_m_s.f:2-3,
+! cans3d/common/cfl_m_s.f:1-2, cans3d/hdroe/roe_m.f:2,
 ! cans3d/common/cfl_m.f:2-3, cans3d/bc/bd*{x,y,z}.f) and the HLLD driver
 ! engine of cansxx-nosupport/md_mhdshktb/hdhlld-x/engine.f:2, computed on
 ! the GPU by libcans_gpu.so.
@@ -82,6 +83,52 @@ subroutine cfl_m(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, vz, bx, by, bz, dx
   dt = d
   merr = m
 end subroutine cfl_m
+
+! cans3d/hdmlw/mlw_m_s.f:2-3: spherical (r, theta, phi), radii x, xm and colatitudes y, ym
+subroutine mlw_m_s(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, ix, y, ym, dy, dym, jx, &
+     dz, dzm, kx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx, kx
+  double precision, target :: ro(*), pr(*), vx(*), vy(*), vz(*), bx(*), by(*), bz(*)
+  double precision, target, intent(in) :: x(ix), xm(ix), dx(ix), dxm(ix), y(jx), ym(jx), dy(jx), dym(jx)
+  double precision, target, intent(in) :: dz(kx), dzm(kx)
+  double precision, intent(in) :: dt, qav, gm
+  integer :: n
+  n = ix * jx * kx
+  call check(c_mlw3d_s(dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vy, n), dmap(vz, n), dmap(bx, n), dmap(by, n), &
+       dmap(bz, n), dt, qav, gm, dmap(x, ix), dmap(xm, ix), dmap(dx, ix), dmap(dxm, ix), int(ix, c_int), &
+       dmap(y, jx), dmap(ym, jx), dmap(dy, jx), dmap(dym, jx), int(jx, c_int), dmap(dz, kx), dmap(dzm, kx), &
+       int(kx, c_int), c_null_ptr), 'mlw_m_s')
+  call unmap8(ro, pr, vx, vy, vz, bx, by, bz)
+  call dunmap(x, 0); call dunmap(xm, 0); call dunmap(dx, 0); call dunmap(dxm, 0)
+  call dunmap(y, 0); call dunmap(ym, 0); call dunmap(dy, 0); call dunmap(dym, 0)
+  call dunmap(dz, 0); call dunmap(dzm, 0)
+end subroutine mlw_m_s
+
+! cans3d/common/cfl_m_s.f:1-2
+subroutine cfl_m_s(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, vz, bx, by, bz, x, dx, ix, y, dy, jx, dz, kx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx, kx
+  integer, intent(out) :: merr
+  double precision, intent(out) :: dt
+  double precision, intent(in) :: safety, dtmin, gm
+  double precision, target, intent(in) :: ro(*), pr(*), vx(*), vy(*), vz(*), bx(*), by(*), bz(*)
+  double precision, target, intent(in) :: x(ix), dx(ix), y(jx), dy(jx), dz(kx)
+  real(c_double) :: d
+  integer(c_int) :: m
+  integer :: n
+  n = ix * jx * kx
+  call check(c_cfl3d_s(d, safety, dtmin, m, gm, dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vy, n), &
+       dmap(vz, n), dmap(bx, n), dmap(by, n), dmap(bz, n), dmap(x, ix), dmap(dx, ix), int(ix, c_int), &
+       dmap(y, jx), dmap(dy, jx), int(jx, c_int), dmap(dz, kx), int(kx, c_int), c_null_ptr), 'cfl_m_s')
+  call dunmap(ro, 0); call dunmap(pr, 0); call dunmap(vx, 0); call dunmap(vy, 0)
+  call dunmap(vz, 0); call dunmap(bx, 0); call dunmap(by, 0); call dunmap(bz, 0)
+  call dunmap(x, 0); call dunmap(dx, 0); call dunmap(y, 0); call dunmap(dy, 0); call dunmap(dz, 0)
+  dt = d
+  merr = m
+end subroutine cfl_m_s
 
 subroutine engine(ro, pr, vx, vy, vz, bx, by, bz, gm, mu, dt, dx, dxm, dy, dym, dz, dzm, ix, jx, kx, mfdim)
   use cans_gpu_iface

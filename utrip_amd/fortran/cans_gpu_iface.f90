@@ -213,6 +213,22 @@ module cans_gpu_iface
        type(c_ptr), value :: ro, pr, vx, vy, vz, bx, by, bz, dx, dy, dz, s
        integer(c_int), value :: ix, jx, kx
      end function c_cfl3d
+     integer(c_int) function c_cfl3d_s(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, vz, bx, by, bz, &
+          x, dx, ix, y, dy, jx, dz, kx, s) bind(C, name="cans3d_cfl_m_s")
+       import :: c_ptr, c_double, c_int
+       real(c_double) :: dt
+       integer(c_int) :: merr
+       real(c_double), value :: safety, dtmin, gm
+       type(c_ptr), value :: ro, pr, vx, vy, vz, bx, by, bz, x, dx, y, dy, dz, s
+       integer(c_int), value :: ix, jx, kx
+     end function c_cfl3d_s
+     integer(c_int) function c_mlw3d_s(ro, pr, vx, vy, vz, bx, by, bz, dt, qav, gm, x, xm, dx, dxm, ix, &
+          y, ym, dy, dym, jx, dz, dzm, kx, s) bind(C, name="cans3d_mlw_m_s")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vy, vz, bx, by, bz, x, xm, dx, dxm, y, ym, dy, dym, dz, dzm, s
+       real(c_double), value :: dt, qav, gm
+       integer(c_int), value :: ix, jx, kx
+     end function c_mlw3d_s
      integer(c_int) function c_bnd3d(dir, kind, mbnd, margin, qq, q0, dm, ix, jx, kx, s) bind(C, name="cans3d_bnd")
        import :: c_ptr, c_double, c_int
        integer(c_int), value :: dir, kind, mbnd, margin, ix, jx, kx
