@@ -117,6 +117,18 @@ int cans2d_roe_m3(double *ro, double *pr, double *vx, double *vy, double *vz, do
 int cans2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx, double *bz, double *ay, double dt,
                    double qav, double gm, const double *x, const double *xm, const double *dx, const double *dxm,
                    int ix, const double *dz, const double *dzm, int jx, void *stream);
+/* cans2d/hdmlw/mlw_m_s.f:1-2 mlw_m_s(ro,pr,vx,vy,bx,by,az,dt,qav,gm,x,xm,dx,dxm,ix,y,ym,dy,dym,jx):
+ * spherical (r, theta) = (x, y) MLW with Az; x, xm cell and face radii, y, ym cell and face colatitudes
+ * (sin, cos taken on the host with the C library); updated in place on i=2..ix-1, j=2..jx-1. */
+int cans2d_mlw_m_s(double *ro, double *pr, double *vx, double *vy, double *bx, double *by, double *az, double dt,
+                   double qav, double gm, const double *x, const double *xm, const double *dx, const double *dxm,
+                   int ix, const double *y, const double *ym, const double *dy, const double *dym, int jx,
+                   void *stream);
+/* cans2d/common/cfl_m_s.f:1-2 cfl_m_s(dt,safety,dtmin,merr,gm,ro,pr,vx,vy,bx,by,x,dx,ix,dy,jx):
+ * cfl_m with the widths dx, x dy. */
+int cans2d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm, const double *ro,
+                   const double *pr, const double *vx, const double *vy, const double *bx, const double *by,
+                   const double *x, const double *dx, int ix, const double *dy, int jx, void *stream);
 /* cans2d/common/cfl_m3.f:2-3 cfl_m3(dt,safety,dtmin,merr,gm,ro,pr,vx,vy,vz,bx,by,bz,dx,ix,dy,jx) */
 int cans2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *ro,
                   const double *pr, const double *vx, const double *vy, const double *vz, const double *bx,
@@ -257,8 +269,9 @@ int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *
 #define CANS_ENGINE_MLW3 3  /* cans{1,2}d/hdmlw/mlw_m3.f with cfl_m3 (1D/2D only) */
 #define CANS_ENGINE_ROE3 4  /* cans{1,2}d/hdroe/roe_m3.f with cfl_m3 (1D/2D only) */
 #define CANS_ENGINE_MLWC 5  /* cans{2,3}d/hdmlw/mlw_m_c.f, cylindrical (2D (r,z), 3D (r,phi,z); x, xm in bx1d, bxm1d) */
-#define CANS_ENGINE_MLWS 6  /* cans3d/hdmlw/mlw_m_s.f + common/cfl_m_s.f, spherical (r,theta,phi) (3D; bx1d = the
-                             * device array x(ix), xm(ix), y(jx), ym(jx) back to back; bxm1d unused) */
+#define CANS_ENGINE_MLWS 6  /* cans{2,3}d/hdmlw/mlw_m_s.f + common/cfl_m_s.f, spherical (2D (r,theta) with az,
+                             * 3D (r,theta,phi); bx1d = the device array x(ix), xm(ix), y(jx), ym(jx) back to
+                             * back; bxm1d unused) */
 
 /* Device workspace (bytes) a cans_run of this shape needs. */
 size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx);

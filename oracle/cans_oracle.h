@@ -99,6 +99,14 @@ void or2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx,
                   const double *x, const double *xm, const double *dx,
                   const double *dxm, int ix, const double *dz,
                   const double *dzm, int jx);
+void or2d_mlw_m_s(double *ro, double *pr, double *vx, double *vy, double *bx, double *by, double *az,
+                  double dt, double qav, double gm, const double *x, const double *xm, const double *dx,
+                  const double *dxm, int ix, const double *y, const double *ym, const double *dy,
+                  const double *dym, int jx);
+void or2d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm,
+                  const double *ro, const double *pr, const double *vx, const double *vy, const double *bx,
+                  const double *by, const double *x, const double *dx, int ix, const double *dy, int jx,
+                  int *imin, int *jmin);
 void or2d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz,
                  double *bx, double *by, double *bz, double *az, double dt,
                  double qav, double gm, const double *dx, const double *dxm,

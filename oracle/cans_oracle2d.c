@@ -692,3 +692,197 @@ void or2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx,
                         ayh, eyh, prh, vxh, vzh, dro, dee, drx, drz, dbx, dbz, day, fx, fz, qx, qz, ss};
     for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
 }
+
+/* hdmlw/mlw_m_s.f:1-398 (2D spherical (r, theta) MHD with Az): the conserved
+ * variables and fluxes carry the metric factors, rx and ry the geometric
+ * sources; the ry x flux at the corners carries sin(ym) alone
+ * (mlw_m_s.f:305), kept as in the reference.  sin, cos of y and ym from the
+ * C library, as the reference's compiled intrinsics. */
+void or2d_mlw_m_s(double *ro, double *pr, double *vx, double *vy, double *bx, double *by, double *az,
+                  double dt, double qav, double gm, const double *x, const double *xm, const double *dx,
+                  const double *dxm, int ix, const double *y, const double *ym, const double *dy,
+                  const double *dym, int jx) {
+    const double pi = acos(-1.0);
+    const double pi8i = 1.0 / pi / 8.0;
+    const double pi4i = 1.0 / pi / 4.0;
+    size_t n = (size_t)ix * jx;
+    double *dxi = dalloc(ix), *dxim = dalloc(ix), *ux0 = dalloc(ix), *ux1 = dalloc(ix);
+    double *dyi = dalloc(jx), *dyim = dalloc(jx), *uy0 = dalloc(jx), *uy1 = dalloc(jx);
+    double *siny = dalloc(jx), *sinym = dalloc(jx);
+    double *ee = dalloc(n), *rx = dalloc(n), *ry = dalloc(n), *ez = dalloc(n);
+    double *sc[6], *sch[6], *dsc[6];
+    for (int q = 0; q < 6; q++) { sc[q] = dalloc(n); sch[q] = dalloc(n); dsc[q] = dalloc(n); }
+    double *daz = dalloc(n), *azh = dalloc(n);
+    double *roh = dalloc(n), *eeh = dalloc(n), *prh = dalloc(n), *vxh = dalloc(n), *vyh = dalloc(n);
+    double *bxh = dalloc(n), *byh = dalloc(n), *ezh = dalloc(n);
+    double *fx = dalloc(n), *fy = dalloc(n), *ss = dalloc(n), *qx = dalloc(n), *qy = dalloc(n);
+    for (int i = 1; i <= ix; i++) { dxi[I1(i)] = 1.0 / dx[I1(i)]; dxim[I1(i)] = 1.0 / dxm[I1(i)]; }
+    for (int i = 2; i <= ix - 1; i++) { ux1[I1(i)] = 0.5 * dxm[I1(i - 1)] / dx[I1(i)]; ux0[I1(i)] = 0.5 * dxm[I1(i)] / dx[I1(i)]; }
+    for (int j = 1; j <= jx; j++) { dyi[I1(j)] = 1.0 / dy[I1(j)]; dyim[I1(j)] = 1.0 / dym[I1(j)]; }
+    for (int j = 2; j <= jx - 1; j++) { uy1[I1(j)] = 0.5 * dym[I1(j - 1)] / dy[I1(j)]; uy0[I1(j)] = 0.5 * dym[I1(j)] / dy[I1(j)]; }
+    for (int j = 1; j <= jx; j++) { siny[I1(j)] = sin(y[I1(j)]); sinym[I1(j)] = sin(ym[I1(j)]); }   /* :99-102 */
+    for (size_t p = 0; p < n; p++) {   /* :107-133 */
+        for (int q = 0; q < 6; q++) dsc[q][p] = 0.0;
+        daz[p] = 0.0; azh[p] = 0.0;
+        double vv = sq(vx[p]) + sq(vy[p]);
+        double bb = sq(bx[p]) + sq(by[p]);
+        ee[p] = pr[p] / (gm - 1.0) + 0.5 * ro[p] * vv + pi8i * bb;
+        rx[p] = ro[p] * vx[p];
+        ry[p] = ro[p] * vy[p];
+        ez[p] = -vx[p] * by[p] + vy[p] * bx[p];
+    }
+#define CELLS for (int j = 1; j <= jx; j++) for (int i = 1; i <= ix; i++)
+#define HALF(q) mlwhalf2(sc[q], sch[q], dsc[q], dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx)
+    CELLS { /* :136-145 */
+        size_t p = I2(i, j); double r = x[I1(i)], st = siny[I1(j)];
+        sc[0][p] = ro[p] * sq(r) * st; sc[1][p] = ee[p] * sq(r) * st; sc[2][p] = rx[p] * sq(r) * st;
+        sc[3][p] = ry[p] * r * st;      sc[4][p] = bx[p] * sq(r) * st; sc[5][p] = by[p] * r * st;
+    }
+    CELLS { size_t p = I2(i, j); double r = x[I1(i)], st = siny[I1(j)];   /* :148-157 */
+            fx[p] = ro[p] * vx[p]; fy[p] = ro[p] * vy[p];
+            fx[p] = fx[p] * sq(r) * st; fy[p] = fy[p] * r * st; }
+    HALF(0);
+    CELLS { /* :160-170 */
+        size_t p = I2(i, j); double r = x[I1(i)], st = siny[I1(j)];
+        double vv = sq(vx[p]) + sq(vy[p]);
+        double ep = pr[p] * gm / (gm - 1.0) + 0.5 * ro[p] * vv;
+        fx[p] = ep * vx[p] - by[p] * ez[p] * pi4i;
+        fy[p] = ep * vy[p] + bx[p] * ez[p] * pi4i;
+        fx[p] = fx[p] * sq(r) * st; fy[p] = fy[p] * r * st;
+    }
+    HALF(1);
+    CELLS { /* :173-184 */
+        size_t p = I2(i, j); double r = x[I1(i)], st = siny[I1(j)];
+        fx[p] = ro[p] * sq(vx[p]) + pr[p] + pi8i * (sq(by[p]) - sq(bx[p]));
+        fy[p] = ro[p] * vx[p] * vy[p] - pi4i * bx[p] * by[p];
+        fx[p] = fx[p] * sq(r) * st; fy[p] = fy[p] * r * st;
+        ss[p] = (ro[p] * sq(vy[p]) + 2.0 * pr[p] + pi4i * sq(bx[p])) * r * st;
+    }
+    HALF(2); mlwsrch2(sch[2], dsc[2], dt, ss, ix, jx);
+    CELLS { /* :188-201 */
+        size_t p = I2(i, j); double r = x[I1(i)], st = siny[I1(j)];
+        fx[p] = ro[p] * vy[p] * vx[p] - pi4i * by[p] * bx[p];
+        fy[p] = ro[p] * sq(vy[p]) + pr[p] + pi8i * (sq(bx[p]) - sq(by[p]));
+        fx[p] = fx[p] * r * st; fy[p] = fy[p] * st;
+        ss[p] = (ro[p] * vx[p] * vy[p] - pi4i * bx[p] * by[p]) * (-2.0) * st
+              + (pr[p] + pi8i * (sq(bx[p]) + sq(by[p]))) * cos(y[I1(j)]);
+    }
+    HALF(3); mlwsrch2(sch[3], dsc[3], dt, ss, ix, jx);
+    CELLS { size_t p = I2(i, j); fx[p] = 0.0; fy[p] = ez[p]; fy[p] = fy[p] * x[I1(i)] * siny[I1(j)]; }   /* :206-212 */
+    HALF(4);
+    CELLS { size_t p = I2(i, j); fx[p] = -ez[p]; fy[p] = 0.0; fx[p] = fx[p] * x[I1(i)] * siny[I1(j)]; }  /* :216-222 */
+    HALF(5);
+    CELLS { size_t p = I2(i, j); ss[p] = -ez[p]; }                                                      /* :226-231 */
+    mlwsrch2(azh, daz, dt, ss, ix, jx);
+#undef HALF
+#undef CELLS
+    for (int j = 1; j <= jx - 1; j++) /* :234-247 */
+        for (int i = 1; i <= ix - 1; i++) {
+            size_t p = I2(i, j); double r = xm[I1(i)], st = sinym[I1(j)];
+            roh[p] = sch[0][p] / sq(r) / st;
+            eeh[p] = sch[1][p] / sq(r) / st;
+            vxh[p] = sch[2][p] / roh[p] / sq(r) / st;
+            vyh[p] = sch[3][p] / roh[p] / r / st;
+            bxh[p] = sch[4][p] / sq(r) / st;
+            byh[p] = sch[5][p] / r / st;
+            double vv = sq(vxh[p]) + sq(vyh[p]);
+            double bb = sq(bxh[p]) + sq(byh[p]);
+            prh[p] = (gm - 1.0) * (eeh[p] - 0.5 * roh[p] * vv - pi8i * bb);
+            ezh[p] = -vxh[p] * byh[p] + vyh[p] * bxh[p];
+        }
+#define FULL(q) mlwfull2(dsc[q], dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx)
+#define SRCF(du) mlwsrcf2(du, dt, ss, ux0, ux1, ix, uy0, uy1, jx)
+#define HL for (int j = 1; j <= jx - 1; j++) for (int i = 1; i <= ix - 1; i++)
+    HL { size_t p = I2(i, j); double r = xm[I1(i)], st = sinym[I1(j)];   /* :258-265 */
+         fx[p] = roh[p] * vxh[p]; fy[p] = roh[p] * vyh[p];
+         fx[p] = fx[p] * sq(r) * st; fy[p] = fy[p] * r * st; }
+    FULL(0);
+    HL { /* :268-278 */
+        size_t p = I2(i, j); double r = xm[I1(i)], st = sinym[I1(j)];
+        double vv = sq(vxh[p]) + sq(vyh[p]);
+        double eph = prh[p] * gm / (gm - 1.0) + 0.5 * roh[p] * vv;
+        fx[p] = eph * vxh[p] - byh[p] * ezh[p] * pi4i;
+        fy[p] = eph * vyh[p] + bxh[p] * ezh[p] * pi4i;
+        fx[p] = fx[p] * sq(r) * st; fy[p] = fy[p] * r * st;
+    }
+    FULL(1);
+    HL { /* :282-293 */
+        size_t p = I2(i, j); double r = xm[I1(i)], st = sinym[I1(j)];
+        fx[p] = roh[p] * sq(vxh[p]) + prh[p] + pi8i * (sq(byh[p]) - sq(bxh[p]));
+        fy[p] = roh[p] * vxh[p] * vyh[p] - pi4i * bxh[p] * byh[p];
+        fx[p] = fx[p] * sq(r) * st; fy[p] = fy[p] * r * st;
+        ss[p] = (roh[p] * sq(vyh[p]) + 2.0 * prh[p] + pi4i * sq(bxh[p])) * r * st;
+    }
+    FULL(2); SRCF(dsc[2]);
+    HL { /* :298-310 */
+        size_t p = I2(i, j); double st = sinym[I1(j)];
+        fx[p] = roh[p] * vyh[p] * vxh[p] - pi4i * byh[p] * bxh[p];
+        fy[p] = roh[p] * sq(vyh[p]) + prh[p] + pi8i * (sq(bxh[p]) - sq(byh[p]));
+        fx[p] = fx[p] * st; fy[p] = fy[p] * st;
+        ss[p] = (roh[p] * vxh[p] * vyh[p] - pi4i * bxh[p] * byh[p]) * (-2.0) * st
+              + (prh[p] + pi8i * (sq(bxh[p]) + sq(byh[p]))) * cos(ym[I1(j)]);
+    }
+    FULL(3); SRCF(dsc[3]);
+    HL { size_t p = I2(i, j); fx[p] = 0.0; fy[p] = ezh[p]; fy[p] = fy[p] * xm[I1(i)] * sinym[I1(j)]; }   /* :315-321 */
+    FULL(4);
+    HL { size_t p = I2(i, j); fx[p] = -ezh[p]; fy[p] = 0.0; fx[p] = fx[p] * xm[I1(i)] * sinym[I1(j)]; }  /* :325-331 */
+    FULL(5);
+    HL { size_t p = I2(i, j); ss[p] = -ezh[p]; }                                                        /* :335-340 */
+    SRCF(daz);
+#undef HL
+#undef SRCF
+#undef FULL
+    const double zero = 0.0;   /* :346-357 */
+    for (int j = 1; j <= jx; j++)
+        for (int i = 1; i <= ix - 1; i++)
+            qx[I2(i, j)] = qav * dxm[I1(i)] * fmaxF(zero, fabs(vx[I2(i + 1, j)] - vx[I2(i, j)]) - R4_1EM4);
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix; i++)
+            qy[I2(i, j)] = qav * dym[I1(j)] * fmaxF(zero, fabs(vy[I2(i, j + 1)] - vy[I2(i, j)]) - R4_1EM4);
+    for (int q = 0; q < 6; q++) mlwartv2(sc[q], dsc[q], dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);   /* :360-365 */
+    for (int j = 2; j <= jx - 1; j++) /* :369-396 */
+        for (int i = 2; i <= ix - 1; i++) {
+            size_t p = I2(i, j); double r = x[I1(i)], st = siny[I1(j)];
+            double u[6];
+            for (int q = 0; q < 6; q++) u[q] = sc[q][p] + dsc[q][p];
+            az[p] = az[p] + daz[p];
+            ro[p] = u[0] / sq(r) / st;
+            double e = u[1] / sq(r) / st;
+            vx[p] = u[2] / ro[p] / sq(r) / st;
+            vy[p] = u[3] / ro[p] / r / st;
+            bx[p] = u[4] / sq(r) / st;
+            by[p] = u[5] / r / st;
+            double vv = sq(vx[p]) + sq(vy[p]);
+            double bb = sq(bx[p]) + sq(by[p]);
+            pr[p] = (gm - 1.0) * (e - 0.5 * ro[p] * vv - pi8i * bb);
+        }
+    double *tofree[] = {dxi, dxim, ux0, ux1, dyi, dyim, uy0, uy1, siny, sinym, ee, rx, ry, ez, daz, azh, roh, eeh, prh,
+                        vxh, vyh, bxh, byh, ezh, fx, fy, ss, qx, qy};
+    for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
+    for (int q = 0; q < 6; q++) { free(sc[q]); free(sch[q]); free(dsc[q]); }
+}
+
+/* common/cfl_m_s.f:1-71: cfl_m with the widths dx, x dy (pi in double) */
+void or2d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm,
+                  const double *ro, const double *pr, const double *vx, const double *vy, const double *bx,
+                  const double *by, const double *x, const double *dx, int ix, const double *dy, int jx,
+                  int *imin, int *jmin) {
+    const double pi = acos(-1.0);
+    const double pi4i = 1.0 / pi / 4.0;
+    double d = R4_1E20;
+    int im = 0, jm = 0;
+    for (int j = 1; j <= jx; j++)   /* :48-67, the two loops fused in the same order */
+        for (int i = 1; i <= ix; i++) {
+            size_t p = I2(i, j);
+            double b2 = bx[p] * bx[p] + by[p] * by[p];
+            double ca2 = b2 * pi4i / ro[p];
+            double cs2 = gm * pr[p] / ro[p];
+            double v2 = vx[p] * vx[p] + vy[p] * vy[p];
+            double q = safety * (fminF(dx[I1(i)], x[I1(i)] * dy[I1(j)]) / sqrt(v2 + cs2 + ca2));
+            if (q < d) { im = i; jm = j; d = q; }
+        }
+    *merr = (d < dtmin) ? 9001 : 0;
+    *dt = d;
+    if (imin) *imin = im;
+    if (jmin) *jmin = jm;
+}

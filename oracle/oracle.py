@@ -133,6 +133,20 @@ class Oracle:
                               C.c_double(gm), _arr(x), _arr(xm), _arr(dx), _arr(dxm), C.c_int(ix), _arr(dz), _arr(dzm),
                               C.c_int(jx))
 
+    def mlw_m_s_2d(self, ro, pr, vx, vy, bx, by, az, dt, qav, gm, x, xm, dx, dxm, y, ym, dy, dym):
+        ix, jx = ro.shape
+        self.lib.or2d_mlw_m_s(*map(_arr, (ro, pr, vx, vy, bx, by, az)), C.c_double(dt), C.c_double(qav),
+                              C.c_double(gm), _arr(x), _arr(xm), _arr(dx), _arr(dxm), C.c_int(ix), _arr(y), _arr(ym),
+                              _arr(dy), _arr(dym), C.c_int(jx))
+
+    def cfl_m_s_2d(self, safety, dtmin, gm, ro, pr, vx, vy, bx, by, x, dx, dy):
+        ix, jx = ro.shape
+        dt, merr, im, jm = C.c_double(), C.c_int(), C.c_int(), C.c_int()
+        self.lib.or2d_cfl_m_s(C.byref(dt), C.c_double(safety), C.c_double(dtmin), C.byref(merr), C.c_double(gm),
+                              *map(_arr, (ro, pr, vx, vy, bx, by)), _arr(x), _arr(dx), C.c_int(ix), _arr(dy),
+                              C.c_int(jx), C.byref(im), C.byref(jm))
+        return dt.value, merr.value, (im.value, jm.value)
+
     def mlw_m3_2d(self, ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, dy, dym):
         ix, jx = ro.shape
         self.lib.or2d_mlw_m3(*map(_arr, (ro, pr, vx, vy, vz, bx, by, bz, az)), C.c_double(dt), C.c_double(qav),

@@ -168,6 +168,19 @@ def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
     call(lib2, "mlw_m_c", *[b[k] for k in FC], 1e-3, 3.0, 5 / 3, x, xm, dxc, dxmc, ix, dy, dym, jx)
     for k in FC:
         assert_bitwise(b[k], a[k], k)
+    from tests.test_oracle_m3 import F2S, state_s2
+    ss2, xs, xms, dxs, dxms, ys, yms, dys, dyms = state_s2(ix, jx, 8)
+    a = {k: v.copy(order="F") for k, v in ss2.items()}
+    b = {k: v.copy(order="F") for k, v in ss2.items()}
+    oracle.mlw_m_s_2d(*[a[k] for k in F2S], 1e-3, 3.0, 5 / 3, xs, xms, dxs, dxms, ys, yms, dys, dyms)
+    call(lib2, "mlw_m_s", *[b[k] for k in F2S], 1e-3, 3.0, 5 / 3, xs, xms, dxs, dxms, ix, ys, yms, dys, dyms, jx)
+    for k in F2S:
+        assert_bitwise(b[k], a[k], k)
+    f6 = ("ro", "pr", "vx", "vy", "bx", "by")
+    ref = oracle.cfl_m_s_2d(0.4, 1e-10, 5 / 3, *[ss2[k] for k in f6], xs, dxs, dys)
+    dt, merr = C.c_double(), C.c_int()
+    call(lib2, "cfl_m_s", dt, 0.4, 1e-10, merr, 5 / 3, *[ss2[k] for k in f6], xs, dxs, ix, dys, jx)
+    assert dt.value == ref[0] and merr.value == ref[1]
     lib1 = C.CDLL(str(PKG_DIR / "libcans1d_gpu.so"))
     ix = 123
     s1 = state1(ix, 2)

@@ -433,3 +433,71 @@ def test_mlw_m_s_3d_resident_loop(oracle, dev):
     assert info["nexec"] == nsteps and info["t"] == t
     for k, g in zip(F3, cur):
         assert_bitwise(to_host(g), a[k], k)
+
+
+# ---------------------------------------------------------------- mlw_m_s, cfl_m_s, 2D (r, theta)
+from tests.test_oracle_m3 import F2S, state_s2  # noqa: E402
+
+
+@pytest.mark.parametrize("ix,jx", [(5, 6), (33, 9), (70, 51), (130, 71)])
+def test_mlw_m_s_2d_call(oracle, dev, ix, jx):
+    s, x, xm, dx, dxm, y, ym, dy, dym = state_s2(ix, jx, ix + jx)
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_s_2d(*[a[k] for k in F2S], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_s_2d(*[d[k] for k in F2S], 1e-3, 3.0, 5 / 3, *[T(v, dev) for v in (x, xm, dx, dxm, y, ym, dy, dym)])
+    torch.cuda.synchronize()
+    for k in F2S:
+        assert_bitwise(to_host(d[k]), a[k], k)
+
+
+def test_mlw_m_s_2d_pole_face_call(oracle, dev):
+    ix, jx = 20, 16
+    s, x, xm, dx, dxm, y, ym, dy, dym = state_s2(ix, jx, 5, t0=0.0)
+    ym[0] = 0.0
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    oracle.mlw_m_s_2d(*[a[k] for k in F2S], 1e-3, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym)
+    d = {k: to_dev(v, dev) for k, v in s.items()}
+    cans.mlw_m_s_2d(*[d[k] for k in F2S], 1e-3, 3.0, 5 / 3, *[T(v, dev) for v in (x, xm, dx, dxm, y, ym, dy, dym)])
+    torch.cuda.synchronize()
+    for k in F2S:
+        assert_bitwise(to_host(d[k]), a[k], k)
+
+
+def test_cfl_m_s_2d_call(oracle, dev):
+    ix, jx = 61, 47
+    s, x, xm, dx, dxm, y, ym, dy, dym = state_s2(ix, jx, 12)
+    f6 = ("ro", "pr", "vx", "vy", "bx", "by")
+    ref = oracle.cfl_m_s_2d(0.4, 1e-10, 5 / 3, *[s[k] for k in f6], x, dx, dy)
+    got = cans.cfl_m_s_2d(0.4, 1e-10, 5 / 3, *[to_dev(s[k], dev) for k in f6], T(x, dev), T(dx, dev), T(dy, dev))
+    assert got[0] == ref[0] and got[1] == ref[1]
+
+
+def test_mlw_m_s_2d_resident_loop(oracle, dev):
+    """20 steps of {cfl_m_s; t=t+dt; mlw_m_s; bdper x, y} as a resident run."""
+    ix, jx, margin, nsteps = 50, 40, 1, 20
+    s, x, xm, dx, dxm, y, ym, dy, dym = state_s2(ix, jx, 21)
+    for dr in (0, 1):
+        for k in F2S:
+            oracle.bnd_2d(dr, 7, margin, margin, s[k])
+    a = {k: v.copy(order="F") for k, v in s.items()}
+    t = 0.0
+    for _ in range(nsteps):
+        dt, merr, _ = oracle.cfl_m_s_2d(0.4, 1e-10, 5 / 3, *[a[k] for k in ("ro", "pr", "vx", "vy", "bx", "by")],
+                                        x, dx, dy)
+        t = t + dt
+        oracle.mlw_m_s_2d(*[a[k] for k in F2S], dt, 3.0, 5 / 3, x, xm, dx, dxm, y, ym, dy, dym)
+        for dr in (0, 1):
+            for k in F2S:
+                oracle.bnd_2d(dr, 7, margin, margin, a[k])
+    st = [to_dev(s[k], dev) for k in F2S]
+    grid = {"dx": T(dx, dev), "dxm": T(dxm, dev), "dy": T(dy, dev), "dym": T(dym, dev)}
+    geo = T(np.concatenate([x, xm, y, ym]), dev)
+    run = cans.Run(2, cans.ENGINE_MLWS, st, grid, 5 / 3, 0.4, qav=3.0, margin=margin, bc=7, bx1d=geo)
+    run.reset()
+    run.steps(nsteps)
+    cur, info = run.state()
+    torch.cuda.synchronize()
+    assert info["nexec"] == nsteps and info["t"] == t
+    for k, g in zip(F2S, cur):
+        assert_bitwise(to_host(g), a[k], k)

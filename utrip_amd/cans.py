@@ -441,6 +441,27 @@ def mlw_m_c_2d(ro, pr, vx, vz, bx, bz, ay, dt, qav, gm, x, xm, dx, dxm, dz, dzm)
                                 _ptr(dz, (jx,)), _ptr(dzm, (jx,)), C.c_int(jx), _stream()))
 
 
+def mlw_m_s_2d(ro, pr, vx, vy, bx, by, az, dt, qav, gm, x, xm, dx, dxm, y, ym, dy, dym):
+    """cans2d/hdmlw/mlw_m_s.f:1-2, spherical (r, theta)  (in place on i=2..ix-1, j=2..jx-1)."""
+    jx, ix = ro.shape
+    sh = (jx, ix)
+    _check(lib().cans2d_mlw_m_s(*[_ptr(t, sh) for t in (ro, pr, vx, vy, bx, by, az)], _d(dt), _d(qav), _d(gm),
+                                _ptr(x, (ix,)), _ptr(xm, (ix,)), _ptr(dx, (ix,)), _ptr(dxm, (ix,)), C.c_int(ix),
+                                _ptr(y, (jx,)), _ptr(ym, (jx,)), _ptr(dy, (jx,)), _ptr(dym, (jx,)), C.c_int(jx),
+                                _stream()))
+
+
+def cfl_m_s_2d(safety, dtmin, gm, ro, pr, vx, vy, bx, by, x, dx, dy):
+    """cans2d/common/cfl_m_s.f:1-2  -> (dt, merr)."""
+    jx, ix = ro.shape
+    sh = (jx, ix)
+    dt, merr = C.c_double(), C.c_int()
+    _check(lib().cans2d_cfl_m_s(C.byref(dt), _d(safety), _d(dtmin), C.byref(merr), _d(gm),
+                                *[_ptr(t, sh) for t in (ro, pr, vx, vy, bx, by)], _ptr(x, (ix,)), _ptr(dx, (ix,)),
+                                C.c_int(ix), _ptr(dy, (jx,)), C.c_int(jx), _stream()))
+    return dt.value, merr.value
+
+
 def mlw_m3_2d(ro, pr, vx, vy, vz, bx, by, bz, az, dt, qav, gm, dx, dxm, dy, dym):
     """cans2d/hdmlw/mlw_m3.f:2-3  (in place on i=2..ix-1, j=2..jx-1)."""
     jx, ix = ro.shape

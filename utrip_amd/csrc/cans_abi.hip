@@ -259,6 +259,16 @@ int cans2d_cfl_m(double *dt, double safety, double dtmin, int *merr, double gm,
     return cfl_common(a, dt, dtmin, merr, (hipStream_t)stream);
 }
 
+int cans2d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm, const double *ro,
+                   const double *pr, const double *vx, const double *vy, const double *bx, const double *by,
+                   const double *x, const double *dx, int ix, const double *dy, int jx, void *stream) {
+    if (!x) return fail(CANS_ERR_ARG, "cans2d_cfl_m_s: NULL x");
+    CflArgs a{};
+    a.ro = ro; a.pr = pr; a.vx = vx; a.vy = vy; a.bx = bx; a.by = by; a.dx = dx; a.dy = dy; a.xr = x;
+    a.safety = safety; a.gm = gm; a.ix = ix; a.jx = jx; a.kx = 1; a.dim = 2;
+    return cfl_common(a, dt, dtmin, merr, (hipStream_t)stream);
+}
+
 int cans2d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm, const double *ro,
                   const double *pr, const double *vx, const double *vy, const double *vz, const double *bx,
                   const double *by, const double *bz, const double *dx, int ix, const double *dy, int jx,

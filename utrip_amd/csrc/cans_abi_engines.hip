@@ -56,8 +56,9 @@ size_t engine_doubles(int dim, int engine, size_t N) {
     const size_t a = al(N);
     switch (engine) {
     case CANS_ENGINE_MLW: return dim == 3 ? (size_t)(kMlw3dCellFields + kMlw3dCornerFields) * a : 0;
-    case CANS_ENGINE_MLWC:
-    case CANS_ENGINE_MLWS: return dim == 3 ? (size_t)(kMlw3dCylCellFields + kMlw3dCylCornerFields) * a : 0;
+    case CANS_ENGINE_MLWC: return dim == 3 ? (size_t)(kMlw3dCylCellFields + kMlw3dCylCornerFields) * a : 0;
+    case CANS_ENGINE_MLWS:
+        return (size_t)(dim == 3 ? kMlw3dCylCellFields + kMlw3dCylCornerFields : kMlw2sCellFields + kMlw2sCornerFields) * a;
     case CANS_ENGINE_ROE:
     case CANS_ENGINE_ROE3: return (8 + 3 * 8) * a;
     case CANS_ENGINE_HLLD: return 4 * 8 * a;
@@ -294,6 +295,40 @@ int cans2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx, d
     return 0;
 }
 
+int cans2d_mlw_m_s(double *ro, double *pr, double *vx, double *vy, double *bx, double *by, double *az, double dt,
+                   double qav, double gm, const double *x, const double *xm, const double *dx, const double *dxm,
+                   int ix, const double *y, const double *ym, const double *dy, const double *dym, int jx,
+                   void *stream) {
+    double *f[7] = {ro, pr, vx, vy, bx, by, az};
+    for (int q = 0; q < 7; q++)
+        if (!f[q]) return abi_fail(CANS_ERR_ARG, "mlw_m_s: NULL field %d", q);
+    if (!x || !xm || !dx || !dxm || !y || !ym || !dy || !dym) return abi_fail(CANS_ERR_ARG, "mlw_m_s: NULL grid array");
+    if (ix < 3 || jx < 3) return abi_fail(CANS_ERR_ARG, "mlw_m_s: %d x %d", ix, jx);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t N = (size_t)ix * jx;
+    void *w = nullptr;
+    const size_t need = metric_doubles(ix, jx, 1) + engine_doubles(2, CANS_ENGINE_MLWS, N) + 7 * al(N) + 6 * al(jx);
+    if (int rc = abi_scratch(need * sizeof(double), &w)) return rc;
+    Carver c{(double *)w};
+    Metric m[3];
+    carve_metric(c, ix, jx, 1, m);
+    HIPCHK(launch_metric(dx, dxm, ix, m[0], s));
+    HIPCHK(launch_metric(dy, dym, jx, m[1], s));
+    Mlw2sArgs a{};
+    for (int q = 0; q < 7; q++) { a.prim[q] = f[q]; a.out[q] = c.take(N); }
+    a.cell = c.take((size_t)kMlw2sCellFields * al(N));
+    a.corner = c.take((size_t)kMlw2sCornerFields * al(N));
+    double *tab = c.take(6 * (size_t)jx);
+    if (int rc = host_trig(y, ym, jx, tab, s)) return rc;
+    a.x = x; a.xm = xm; a.trig = tab; a.dxm = dxm; a.dym = dym; a.mx = m[0]; a.my = m[1];
+    a.qav = qav; a.gm = gm; a.dt = dt; a.clk = nullptr; a.ix = ix; a.jx = jx;
+    HIPCHK(launch_mlw2d_s(a, s));
+    const double *src[7];
+    for (int q = 0; q < 7; q++) src[q] = a.out[q];
+    HIPCHK(launch_copy_interior(f, src, 7, ix, jx, 1, 2, s));
+    return 0;
+}
+
 int cans1d_mlw_m3(double *ro, double *pr, double *vx, double *vy, double *vz, double *by, double *bz,
                   const double *bx, const double *bxm, double dt, double qav, double gm, const double *dx,
                   const double *dxm, int ix, void *stream) {
@@ -360,8 +395,8 @@ size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx) {
 // dzm; bx1d, bxm1d carry the radii x, xm), cfl_m (2D) on (vx, vz, bx, bz);
 // 3D 8 (ro pr vx vy vz bx by bz) on (r, phi, z), radii in bx1d, bxm1d,
 // cfl_m (3D) as the md_mhdsn 3D driver calls it;
-// mlw_m_s: 3D 8 on (r, theta, phi), bx1d = x, xm, y, ym back to back, with
-// cfl_m_s.
+// mlw_m_s: 2D 7 (ro pr vx vy bx by az) on (r, theta) or 3D 8 on (r, theta,
+// phi), bx1d = x, xm, y, ym back to back, with cfl_m_s.
 int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
              const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
              double safety, const double *dx, const double *dxm, int ix, const double *dy, const double *dym,
@@ -374,8 +409,8 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         return abi_fail(CANS_ERR_ARG, "cans_run: the three-component engines are 1D/2D");
     if (engine == CANS_ENGINE_MLWC && (dim == 1 || !bx1d || !bxm1d))
         return abi_fail(CANS_ERR_ARG, "cans_run: mlw_m_c is 2D/3D and needs the radii x, xm (in bx1d, bxm1d)");
-    if (engine == CANS_ENGINE_MLWS && (dim != 3 || !bx1d))
-        return abi_fail(CANS_ERR_ARG, "cans_run: mlw_m_s is 3D and needs x, xm, y, ym (packed in bx1d)");
+    if (engine == CANS_ENGINE_MLWS && (dim == 1 || !bx1d))
+        return abi_fail(CANS_ERR_ARG, "cans_run: mlw_m_s is 2D/3D and needs x, xm, y, ym (packed in bx1d)");
     if (engine < CANS_ENGINE_MLW || engine > CANS_ENGINE_MLWS) return abi_fail(CANS_ERR_ARG, "cans_run: engine %d", engine);
     if (engine == CANS_ENGINE_MLW && dim == 2 && !bc)
         return cans2d_mlw_run(a, b, clk, step0, nsteps, qav, gm, safety, dx, dxm, ix, dy, dym, jx, nullptr, margin,
@@ -386,7 +421,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     const bool m3 = engine == CANS_ENGINE_MLW3;
     const bool cyl = engine == CANS_ENGINE_MLWC && dim == 2;   // fields ro pr vx vz bx bz ay
     const bool cyl3 = engine == CANS_ENGINE_MLWC && dim == 3;  // the 3D field list, radii x, xm
-    const bool sph = engine == CANS_ENGINE_MLWS;               // 3D, x xm y ym packed in bx1d
+    const bool sph = engine == CANS_ENGINE_MLWS;               // x xm y ym packed in bx1d
     const bool c3 = m3 || engine == CANS_ENGINE_ROE3;      // three-component field lists
     const int nf = c3 ? (dim == 1 ? 7 : 9) : (dim == 1 ? 5 : (dim == 2 ? 7 : 8));
     const int lo = (engine == CANS_ENGINE_MLW || m3 || cyl || cyl3 || sph) ? 3 : 5;   // Roe: margin >= 3 (drop-in)
@@ -436,7 +471,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         ca.ro = pin[0]; ca.pr = pin[1]; ca.vx = pin[2]; ca.vy = pin[3]; ca.vz = pin[4];
         ca.bx = dim == 1 ? bx1d : pin[5]; ca.by = pin[6]; ca.bz = pin[7];   // (3D mlw_m_c: the plain cfl_m)
         ca.dx = dx; ca.dy = dy; ca.dz = dz; ca.safety = safety; ca.gm = gm;
-        ca.xr = gx; ca.trig = trig;   // cfl_m_s for the spherical engine
+        ca.xr = gx; ca.trig = dim == 3 ? trig : nullptr;   // cfl_m_s for the spherical engine
         ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st; ca.comp3 = c3 ? 1 : 0;
         HIPCHK(launch_cfl(ca, s));
         HIPCHK(launch_clock_step(c, st, s));
@@ -454,6 +489,15 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
             g.bx = bx1d; g.bxm = bxm1d; g.dxm = dxm; g.dym = dym; g.mx = m[0]; g.my = m[1];
             g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx;
             HIPCHK(launch_mlw_m3(g, dim, s));
+        } else if (sph && dim == 2) {
+            Mlw2sArgs g{};
+            for (int f = 0; f < 7; f++) { g.prim[f] = in[f]; g.out[f] = out[f]; }
+            Carver ec{eng};
+            g.cell = ec.take((size_t)kMlw2sCellFields * N);
+            g.corner = ec.take((size_t)kMlw2sCornerFields * N);
+            g.x = gx; g.xm = gxm; g.trig = trig; g.dxm = dxm; g.dym = dym; g.mx = m[0]; g.my = m[1];
+            g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx;
+            HIPCHK(launch_mlw2d_s(g, s));
         } else if (sph) {
             Mlw3dArgs g{};
             for (int q = 0; q < 8; q++) { g.prim[q] = pin[q]; g.out[q] = pout[q]; }

@@ -101,7 +101,7 @@ __global__ void bnd_kernel(BndArgs a) {
 // cfl_m3 (comp3): the 1D/2D formulas with bz in b2 and (2D) vz in v2.
 // 3D: cans3d/common/cfl_m.f:39-58, all cells, min(dx,dy,dz)/sqrt(...),
 //     onep4 = 0.25/pai with pai the REAL*4 acos(-1.0); spherical cfl_m_s
-//     (trig set): min(dx, x dy, x sin(y) dz).
+//     (trig set): min(dx, x dy, x sin(y) dz); 2D cfl_m_s (xr set): min(dx, x dy).
 CANS_DEV double cfl_cell(const CflArgs &a, int i, int j, int k, size_t p) {
     const double gm = a.gm;
     if (a.dim == 1) {
@@ -121,6 +121,10 @@ CANS_DEV double cfl_cell(const CflArgs &a, int i, int j, int k, size_t p) {
         const double cs2 = gm * a.pr[p] / a.ro[p];
         double v2 = a.vx[p] * a.vx[p] + a.vy[p] * a.vy[p];
         if (a.comp3) v2 = v2 + a.vz[p] * a.vz[p];
+        if (a.xr) {   // cans2d/common/cfl_m_s.f:54: min(dx, x dy)
+            const double dtcfl = fminF(a.dx[i - 1], a.xr[i - 1] * a.dy[j - 1]) / __builtin_sqrt(v2 + cs2 + ca2);
+            return a.safety * dtcfl;
+        }
         const double dtcfl = fminF(a.dx[i - 1], a.dy[j - 1]) / __builtin_sqrt(v2 + cs2 + ca2);
         return a.safety * dtcfl;
     } else {

@@ -102,7 +102,7 @@ struct CflArgs {
     int ix, jx, kx;
     int dim;                    // 1, 2 or 3 (selects the reference's formula)
     int comp3;                  // 1: cfl_m3 of 1D/2D (adds vz, bz)
-    const double *xr, *trig;    // 3D spherical cfl_m_s: radii x and sin(y) (trig[0..jx)); NULL: cfl_m
+    const double *xr, *trig;    // spherical cfl_m_s: radii x (2D, 3D) and sin(y) (3D, trig[0..jx)); NULL: cfl_m
     unsigned long long *key;    // order-preserving key of the min (pre-armed)
     Clock *clk;                 // non-null: reduce into clk->dtkey[step%3] if the step runs
     int step;
@@ -147,6 +147,25 @@ hipError_t launch_mlw_m3(const Mlw3Args &a, int dim, hipStream_t s);
 // mlw_m_c (cans2d/hdmlw/mlw_m_c.f), fields ro pr vx vz bx bz ay on the (x, z)
 // plane with the radii a.x, a.xm; my is the z metric
 hipError_t launch_mlw_m_c(const Mlw3Args &a, hipStream_t s);
+
+// 2D spherical mlw_m_s (cans2d/hdmlw/mlw_m_s.f): fields ro pr vx vy bx by az,
+// SoA scratch of kMlw2sCellFields per cell and kMlw2sCornerFields per corner;
+// trig: 6 tables of jx (sin, cos, tan of y, then of ym).
+constexpr int kMlw2sCellFields = 19;
+constexpr int kMlw2sCornerFields = 13;
+struct Mlw2sArgs {
+    const double *prim[7];
+    double *out[7];                  // interior cells
+    double *cell, *corner;
+    const double *x, *xm, *trig;
+    const double *dxm, *dym;
+    Metric mx, my;
+    double qav, gm, dt;
+    Clock *clk;                      // non-null: dt and run flag of step `step` (after its clock kernel)
+    int step;
+    int ix, jx;
+};
+hipError_t launch_mlw2d_s(const Mlw2sArgs &a, hipStream_t s);
 
 // UTRIP spicule (cans1d/md_spicule/hdmlw): geometry arrays of mlw_m_bg, in
 // this order.

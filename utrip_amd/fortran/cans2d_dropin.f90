@@ -58,6 +58,47 @@ subroutine mlw_m_c(ro, pr, vx, vz, bx, bz, ay, dt, qav, gm, x, xm, dx, dxm, ix, 
   call dunmap(x, 0); call dunmap(xm, 0); call dunmap(dx, 0); call dunmap(dxm, 0); call dunmap(dz, 0); call dunmap(dzm, 0)
 end subroutine mlw_m_c
 
+! cans2d/hdmlw/mlw_m_s.f:1-2: spherical (r, theta) with Az
+subroutine mlw_m_s(ro, pr, vx, vy, bx, by, az, dt, qav, gm, x, xm, dx, dxm, ix, y, ym, dy, dym, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target :: ro(ix, jx), pr(ix, jx), vx(ix, jx), vy(ix, jx), bx(ix, jx), by(ix, jx), az(ix, jx)
+  double precision, target, intent(in) :: x(ix), xm(ix), dx(ix), dxm(ix), y(jx), ym(jx), dy(jx), dym(jx)
+  double precision, intent(in) :: dt, qav, gm
+  integer :: n
+  n = ix * jx
+  call check(c_mlw2d_s(dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vy, n), dmap(bx, n), dmap(by, n), dmap(az, n), &
+       dt, qav, gm, dmap(x, ix), dmap(xm, ix), dmap(dx, ix), dmap(dxm, ix), int(ix, c_int), dmap(y, jx), &
+       dmap(ym, jx), dmap(dy, jx), dmap(dym, jx), int(jx, c_int), c_null_ptr), 'mlw_m_s')
+  call dunmap(ro, 1); call dunmap(pr, 1); call dunmap(vx, 1); call dunmap(vy, 1)
+  call dunmap(bx, 1); call dunmap(by, 1); call dunmap(az, 1)
+  call dunmap(x, 0); call dunmap(xm, 0); call dunmap(dx, 0); call dunmap(dxm, 0)
+  call dunmap(y, 0); call dunmap(ym, 0); call dunmap(dy, 0); call dunmap(dym, 0)
+end subroutine mlw_m_s
+
+! cans2d/common/cfl_m_s.f:1-2
+subroutine cfl_m_s(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, bx, by, x, dx, ix, dy, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  integer, intent(out) :: merr
+  double precision, intent(out) :: dt
+  double precision, intent(in) :: safety, dtmin, gm
+  double precision, target, intent(in) :: ro(ix, jx), pr(ix, jx), vx(ix, jx), vy(ix, jx), bx(ix, jx), by(ix, jx)
+  double precision, target, intent(in) :: x(ix), dx(ix), dy(jx)
+  real(c_double) :: d
+  integer(c_int) :: m
+  integer :: n
+  n = ix * jx
+  call check(c_cfl2d_s(d, safety, dtmin, m, gm, dmap(ro, n), dmap(pr, n), dmap(vx, n), dmap(vy, n), dmap(bx, n), &
+       dmap(by, n), dmap(x, ix), dmap(dx, ix), int(ix, c_int), dmap(dy, jx), int(jx, c_int), c_null_ptr), 'cfl_m_s')
+  call dunmap(ro, 0); call dunmap(pr, 0); call dunmap(vx, 0); call dunmap(vy, 0)
+  call dunmap(bx, 0); call dunmap(by, 0); call dunmap(x, 0); call dunmap(dx, 0); call dunmap(dy, 0)
+  dt = d
+  merr = m
+end subroutine cfl_m_s
+
 subroutine roe_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, ix, dy, jx)
   use cans_gpu_iface
   implicit none

@@ -150,6 +150,22 @@ module cans_gpu_iface
        real(c_double), value :: dt, qav, gm
        integer(c_int), value :: ix, jx
      end function c_mlw2d_c
+     integer(c_int) function c_mlw2d_s(ro, pr, vx, vy, bx, by, az, dt, qav, gm, x, xm, dx, dxm, ix, y, ym, &
+          dy, dym, jx, s) bind(C, name="cans2d_mlw_m_s")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: ro, pr, vx, vy, bx, by, az, x, xm, dx, dxm, y, ym, dy, dym, s
+       real(c_double), value :: dt, qav, gm
+       integer(c_int), value :: ix, jx
+     end function c_mlw2d_s
+     integer(c_int) function c_cfl2d_s(dt, safety, dtmin, merr, gm, ro, pr, vx, vy, bx, by, x, dx, ix, dy, jx, s) &
+          bind(C, name="cans2d_cfl_m_s")
+       import :: c_ptr, c_double, c_int
+       real(c_double) :: dt
+       integer(c_int) :: merr
+       real(c_double), value :: safety, dtmin, gm
+       type(c_ptr), value :: ro, pr, vx, vy, bx, by, x, dx, dy, s
+       integer(c_int), value :: ix, jx
+     end function c_cfl2d_s
      integer(c_int) function c_roe2d_m3(ro, pr, vx, vy, vz, bx, by, bz, az, dt, gm, dx, ix, dy, jx, s) &
           bind(C, name="cans2d_roe_m3")
        import :: c_ptr, c_double, c_int
