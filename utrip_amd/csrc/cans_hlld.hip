@@ -2,14 +2,19 @@
 // Miyoshi & Kusano 2005; mu units) on gfx950, FP64 (bit-exact) and FP32-flux
 // mixed precision.
 //
-// engine.f:2-73 as kernels: conserved state, then per stage and direction a
-// face-flux kernel (inp_const / inp_tvdminmod + hlld_flux.f90) and an
-// accumulation kernel (adddfdx.f), then the primitive recovery.  Each face
-// flux is computed once; the state accumulates in FP64.
+// engine.f:2-73 as kernels: conserved state, then per stage and direction one
+// sweep kernel that computes the face fluxes (inp_const / inp_tvdminmod +
+// hlld_flux.f90) and applies them to the cells (adddfdx.f), then the
+// primitive recovery.  The face fluxes never go to memory: along x a wave
+// covers 64 faces and takes the west face of each cell from the
+// neighbouring lane (DPP); along y and z each thread marches over a segment
+// of cells and carries the previous face (one face per segment is computed
+// twice, by the same operations).  The state accumulates in FP64.
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_roe.hpp"
 #include "cans_roeflux.hpp"
+#include "cans_wave.hpp"
 
 namespace cans {
 namespace {
@@ -178,79 +183,112 @@ __global__ __launch_bounds__(256) void hlld_cons(HlldArgs a) {
     for (int q = 0; q < 8; q++) { a.u1[q][p] = U[q]; a.u2[q][p] = U[q]; }
 }
 
-// faces along `dir` with index [1, n_dir-1], every transverse index
+// Flux of the face between cell p and cell p+s along `dir` (face index
+// aidx in [1, n_dir-1]), in the U slots ro ee rx ry rz bx by bz (the normal
+// field's flux is 0, hlld_mh.f:56-94).
 template <class T>
-__global__ __launch_bounds__(256) void hlld_flux_kernel(HlldArgs a, int dir, int stage) {
-    double dt;
-    if (!hstep(a, dt)) return;
-    const int ix = a.ix, jx = a.jx, kx = a.kx;
-    const int n[3] = {ix, jx, kx};
-    int cnt[3] = {ix, jx, kx};
-    cnt[dir] = n[dir] - 1;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)cnt[0] * cnt[1] * cnt[2]) return;
-    const int i = 1 + (int)(t % cnt[0]);
-    const int j = 1 + (int)((t / cnt[0]) % cnt[1]);
-    const int k = 1 + (int)(t / ((size_t)cnt[0] * cnt[1]));
-    const int idx[3] = {i, j, k};
-    const size_t p = (size_t)(i - 1) + (size_t)ix * ((size_t)(j - 1) + (size_t)jx * (size_t)(k - 1));
-    const size_t s = dir == 0 ? 1 : (dir == 1 ? (size_t)ix : (size_t)ix * jx);
-    double *const *P = stage == 1 ? a.prim : a.half;
-    const int aidx = idx[dir];
-    const bool muscl = stage == 2 && aidx >= 2 && aidx <= n[dir] - 2;   // inp_tvdminmod.f
+CANS_DEV void hlld_face_flux(const HlldArgs &a, const double *const *P, int dir, int stage, int aidx, int ndir,
+                             size_t p, size_t s, double *F) {
+    const bool muscl = stage == 2 && aidx >= 2 && aidx <= ndir - 2;   // inp_tvdminmod.f
+    // four cells, clamped to the face's two where the states are piecewise
+    // constant (inp_const.f)
+    const size_t pm = muscl ? p - s : p, p3 = muscl ? p + 2 * s : p + s;
     T L[8], R[8];
+    double bn0 = 0.0, bn1 = 0.0;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const double *da = P[q];
-        if (!muscl) {   // inp_const.f (and the end faces of inp_tvdminmod.f)
-            L[q] = T(da[p]);
-            R[q] = T(da[p + s]);
-        } else {
-            const double d0 = da[p - s], d1 = da[p], d2 = da[p + s], d3 = da[p + 2 * s];
-            L[q] = T(d1 + 0.5 * flmtH(d1 - d0, d2 - d1));
-            R[q] = T(d2 - 0.5 * flmtH(d3 - d2, d2 - d1));
-        }
+        const double d0 = da[pm], d1 = da[p], d2 = da[p + s], d3 = da[p3];
+        L[q] = muscl ? T(d1 + 0.5 * flmtH(d1 - d0, d2 - d1)) : T(d1);
+        R[q] = muscl ? T(d2 - 0.5 * flmtH(d3 - d2, d2 - d1)) : T(d2);
+        if (q == P_BX + dir) { bn0 = d1; bn1 = d2; }
     }
-    const int bn = P_BX + dir;
-    const T bb = T((P[bn][p] + P[bn][p + s]) / 2.0);   // hlld_mh.f:47/65/82
+    const T bb = T((bn0 + bn1) / 2.0);   // hlld_mh.f:47/65/82
     const int vn = P_VX + dir, v1 = P_VX + (dir + 1) % 3, v2 = P_VX + (dir + 2) % 3;
     const int b1 = P_BX + (dir + 1) % 3, b2 = P_BX + (dir + 2) % 3;
     T o[7];
     hlld_face<T>(T(a.gm), T(1.0e-10), bb, L[P_RO], L[P_PR], L[vn], L[v1], L[v2], L[b1], L[b2], R[P_RO], R[P_PR],
                  R[vn], R[v1], R[v2], R[b1], R[b2], o);
-    double *const *F = a.flux;
-    F[U_RO][p] = double(o[0]);
-    F[U_EE][p] = double(o[1]);
-    F[U_RX + dir][p] = double(o[2]);
-    F[U_RX + (dir + 1) % 3][p] = double(o[3]);
-    F[U_RX + (dir + 2) % 3][p] = double(o[4]);
-    F[U_BX + (dir + 1) % 3][p] = double(o[5]);
-    F[U_BX + (dir + 2) % 3][p] = double(o[6]);
-    F[U_BX + dir][p] = 0.0;   // normal-field flux, hlld_mh.f:56-94
+    F[U_RO] = double(o[0]);
+    F[U_EE] = double(o[1]);
+    F[U_RX + dir] = double(o[2]);
+    F[U_RX + (dir + 1) % 3] = double(o[3]);
+    F[U_RX + (dir + 2) % 3] = double(o[4]);
+    F[U_BX + (dir + 1) % 3] = double(o[5]);
+    F[U_BX + (dir + 2) % 3] = double(o[6]);
+    F[U_BX + dir] = 0.0;
 }
 
-// common/adddfdx.f: cells [2, n_dir-1] along dir, every transverse index
-__global__ __launch_bounds__(256) void hlld_add_kernel(HlldArgs a, int dir, int stage) {
+// adddfdx.f for one cell: u <- u - dts*((F_hi - F_lo)/dd)
+CANS_DEV void hlld_apply(double *const *u, size_t p, double dts, double dd, const double *hi, const double *lo) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) u[q][p] = u[q][p] - dts * ((hi[q] - lo[q]) / dd);
+}
+
+// x sweep: a wave covers faces i = 63*strip + lane (lane 0 overlaps the
+// previous strip), cells i = 2..ix-1 take their west face from lane-1.
+template <class T>
+__global__ __launch_bounds__(256) void hlld_sweep_x(HlldArgs a, int stage) {
     double dt;
     if (!hstep(a, dt)) return;
     const double dts = stage == 1 ? 0.5 * dt : dt;   // engine.f: dth = 0.5d0*dt
     const int ix = a.ix, jx = a.jx, kx = a.kx;
-    const int n[3] = {ix, jx, kx};
-    int cnt[3] = {ix, jx, kx};
-    cnt[dir] = n[dir] - 2;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)cnt[0] * cnt[1] * cnt[2]) return;
-    int idx[3];
-    idx[0] = 1 + (int)(t % cnt[0]);
-    idx[1] = 1 + (int)((t / cnt[0]) % cnt[1]);
-    idx[2] = 1 + (int)(t / ((size_t)cnt[0] * cnt[1]));
-    idx[dir] += 1;
-    const size_t p = (size_t)(idx[0] - 1) + (size_t)ix * ((size_t)(idx[1] - 1) + (size_t)jx * (size_t)(idx[2] - 1));
-    const size_t s = dir == 0 ? 1 : (dir == 1 ? (size_t)ix : (size_t)ix * jx);
-    const double dd = (dir == 0 ? a.dx : (dir == 1 ? a.dy : a.dz))[idx[dir] - 1];
-    double *const *u = stage == 1 ? a.u1 : a.u2;
+    const int lane = threadIdx.x & 63;
+    const int w = uni((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int nstrip = (ix + 62) / 63;
+    if (w >= nstrip * jx * kx) return;                // whole waves
+    const int strip = w % nstrip, row = w / nstrip;
+    const int j = 1 + row % jx, k = 1 + row / jx;
+    const int i = 63 * strip + lane;
+    const size_t p = (size_t)(i - 1) + (size_t)ix * ((size_t)(j - 1) + (size_t)jx * (size_t)(k - 1));
+    const double *P[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) u[q][p] = u[q][p] - dts * ((a.flux[q][p] - a.flux[q][p - s]) / dd);
+    for (int q = 0; q < 8; q++) P[q] = stage == 1 ? a.prim[q] : a.half[q];
+    double F[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i >= 1 && i <= ix - 1) hlld_face_flux<T>(a, P, 0, stage, i, ix, p, 1, F);
+    double FW[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) FW[q] = shr1(F[q]);
+    if (lane >= 1 && i >= 2 && i <= ix - 1) {
+        double *u[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) u[q] = stage == 1 ? a.u1[q] : a.u2[q];
+        hlld_apply(u, p, dts, a.dx[i - 1], F, FW);
+    }
+}
+
+// y or z sweep: thread (i, other transverse index, segment) marches over the
+// cells cb..ce of its segment, computing faces cb-1..ce once each.
+template <class T>
+__global__ __launch_bounds__(256) void hlld_sweep_yz(HlldArgs a, int dir, int stage, int H) {
+    double dt;
+    if (!hstep(a, dt)) return;
+    const double dts = stage == 1 ? 0.5 * dt : dt;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int nd = dir == 1 ? jx : kx, no = dir == 1 ? kx : jx;
+    const int nseg = (nd - 2 + H - 1) / H;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ix * no * nseg) return;
+    const int i = 1 + (int)(t % ix);
+    const int o = 1 + (int)((t / ix) % no);
+    const int seg = (int)(t / ((size_t)ix * no));
+    const int cb = 2 + seg * H, ce = min(cb + H - 1, nd - 1);
+    const size_t s = dir == 1 ? (size_t)ix : (size_t)ix * jx;
+    const size_t base = (size_t)(i - 1) + (dir == 1 ? (size_t)ix * jx * (size_t)(o - 1) : (size_t)ix * (size_t)(o - 1));
+    const double *P[8];
+    double *u[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) { P[q] = stage == 1 ? a.prim[q] : a.half[q]; u[q] = stage == 1 ? a.u1[q] : a.u2[q]; }
+    const double *dd = dir == 1 ? a.dy : a.dz;
+    double Fl[8], Fh[8];
+    hlld_face_flux<T>(a, P, dir, stage, cb - 1, nd, base + s * (size_t)(cb - 2), s, Fl);
+    for (int c = cb; c <= ce; c++) {
+        const size_t p = base + s * (size_t)(c - 1);
+        hlld_face_flux<T>(a, P, dir, stage, c, nd, p, s, Fh);
+        hlld_apply(u, p, dts, dd[c - 1], Fh, Fl);
+#pragma unroll
+        for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
+    }
 }
 
 // rvtovv.f, ee2eh.f, eh2pr.f: primitives of every cell from u (stage 1 -> half,
@@ -290,10 +328,21 @@ hipError_t launch_hlld_t(const HlldArgs &a, hipStream_t s) {
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < 3; dir++) {
             if (a.mfdim[dir] != 1) continue;
-            const size_t nf = N / n[dir] * (size_t)(n[dir] - 1);
-            const size_t na = N / n[dir] * (size_t)(n[dir] - 2);
-            if (nf) hipLaunchKernelGGL(hlld_flux_kernel<T>, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, a, dir, stage);
-            if (na) hipLaunchKernelGGL(hlld_add_kernel, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, a, dir, stage);
+            if (n[dir] < 3) continue;                 // no interior cell along dir
+            if (dir == 0) {
+                const size_t waves = (size_t)((a.ix + 62) / 63) * a.jx * a.kx;
+                hipLaunchKernelGGL(hlld_sweep_x<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a, stage);
+            } else {
+                // segments so that the launch has >= 8192 waves
+                const size_t cols = N / n[dir];
+                int nseg = (int)((8192ull * 64 + cols - 1) / cols);
+                nseg = nseg < 1 ? 1 : (nseg > n[dir] - 2 ? n[dir] - 2 : nseg);
+                const int H = (n[dir] - 2 + nseg - 1) / nseg;
+                nseg = (n[dir] - 2 + H - 1) / H;
+                const size_t nt = cols * (size_t)nseg;
+                hipLaunchKernelGGL(hlld_sweep_yz<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a, dir, stage,
+                                   H);
+            }
         }
         hipLaunchKernelGGL(hlld_prim, dim3(gN), dim3(256), 0, s, a, stage);
     }
