@@ -61,7 +61,7 @@ size_t engine_doubles(int dim, int engine, size_t N) {
         return (size_t)(dim == 3 ? kMlw3dCylCellFields + kMlw3dCylCornerFields : kMlw2sCellFields + kMlw2sCornerFields) * a;
     case CANS_ENGINE_ROE:
     case CANS_ENGINE_ROE3: return (8 + 3 * 8) * a;
-    case CANS_ENGINE_HLLD: return 4 * 8 * a;
+    case CANS_ENGINE_HLLD: return 3 * 8 * a;
     default: return 0;
     }
 }
@@ -78,7 +78,6 @@ void hlld_scratch(HlldArgs &h, Carver &c, size_t N) {
     for (int q = 0; q < 8; q++) h.half[q] = c.take(N);
     for (int q = 0; q < 8; q++) h.u1[q] = c.take(N);
     for (int q = 0; q < 8; q++) h.u2[q] = c.take(N);
-    for (int q = 0; q < 8; q++) h.flux[q] = c.take(N);
 }
 
 int check_margin(const char *who, int n, int lo) {

@@ -55,7 +55,6 @@ struct HlldArgs {
     double *half[8];         // scratch: half-step primitives
     double *u1[8];           // scratch: conserved state of stage 1
     double *u2[8];           // scratch: conserved state of stage 2
-    double *flux[8];         // scratch: face fluxes of the current direction
     const double *dx, *dy, *dz;
     double gm, mu, dt;
     int mfdim[3];
