@@ -228,3 +228,27 @@ def test_dropin_mlw_m3_by_fortran_names(dev, oracle):
     call(lib3, "cfl_m_s", dt, 0.4, 1e-10, merr, 5 / 3, *[s3[k] for k in F3], x, dx, shape[0], y, dy, shape[1],
          dz, shape[2])
     assert dt.value == ref[0] and merr.value == ref[1]
+
+
+@pytest.mark.gpu
+def test_mirror_remap_smaller_array(dev):
+    """The host mirror of the Fortran layer is keyed by the host address; an
+    address that comes back for a smaller array (the allocator reuses it)
+    must be copied back with the smaller size, never past its end."""
+    import ctypes as C
+    lib = C.CDLL(str(PKG_DIR / "libcans_gpu.so"))
+    lib.cans_gpu_map.restype = C.c_void_p
+    lib.cans_gpu_map.argtypes = [C.c_void_p, C.c_size_t, C.c_int]
+    lib.cans_gpu_unmap.argtypes = [C.c_void_p, C.c_int]
+    big = np.arange(4096, dtype=np.float64)
+    addr = big.ctypes.data
+    assert lib.cans_gpu_map(addr, big.nbytes, 1)
+    assert lib.cans_gpu_unmap(addr, 1) == 0
+    n = 1000
+    big[n:] = -7.0                       # host memory past the smaller array
+    small = big[:n]
+    assert small.ctypes.data == addr
+    assert lib.cans_gpu_map(addr, small.nbytes, 1)
+    assert lib.cans_gpu_unmap(addr, 1) == 0
+    assert np.array_equal(big[:n], np.arange(n, dtype=np.float64))
+    assert np.all(big[n:] == -7.0)

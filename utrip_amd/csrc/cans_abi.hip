@@ -597,7 +597,8 @@ int cans_event_elapsed_ms(void *start, void *stop, float *ms) {
 namespace {
 struct Mirror {
     void *dev = nullptr;
-    size_t bytes = 0;
+    size_t cap = 0;       // device allocation
+    size_t bytes = 0;     // size of the current mapping (what unmap copies back)
     bool valid = false;   // device copy is current
 };
 std::map<const void *, Mirror> g_mirror;
@@ -607,7 +608,7 @@ int g_resident = 0;
 void *cans_gpu_map(const void *host, size_t bytes, int copy_in) {
     std::lock_guard<std::mutex> lk(g_mu);
     Mirror &m = g_mirror[host];
-    if (m.bytes < bytes) {
+    if (m.cap < bytes) {
         if (m.dev) (void)hipFree(m.dev);
         m.dev = nullptr;
         if (hipMalloc(&m.dev, bytes) != hipSuccess) {
@@ -615,9 +616,15 @@ void *cans_gpu_map(const void *host, size_t bytes, int copy_in) {
             g_mirror.erase(host);
             return nullptr;
         }
-        m.bytes = bytes;
+        m.cap = bytes;
         m.valid = false;
     }
+    // A host address can come back for an array of another size (the
+    // allocator reuses it): the mapping's size is this call's, so that unmap
+    // never copies past the end of the host array, and a size change makes
+    // the device copy stale.
+    if (m.bytes != bytes) m.valid = false;
+    m.bytes = bytes;
     const bool need = copy_in && (!g_resident || !m.valid);
     if (need) {
         if (hipMemcpy(m.dev, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
