@@ -163,6 +163,41 @@ def test_mlw2d_ot_resident(dev, oracle, fused):
         assert_bitwise(to_host(tt), ref[k], k)
 
 
+@pytest.mark.parametrize("margin,axis", [(1, "x"), (2, "x"), (1, "y"), (2, "y")])
+def test_mlw2d_resident_periodic_ghost_cfl(dev, oracle, margin, axis):
+    """Fused periodic run on a stretched grid whose smallest spacings lie in
+    ghost cells (dx(1), dx(ix) or dy(1), dy(jx)): the reference's cfl_m takes
+    dt from a ghost image, which the fused kernel folds into one term per
+    cell."""
+    from utrip_amd import cans
+    from utrip_amd.problems import Problem
+    _print_case("mlw2d resident periodic ghost cfl, margin", margin, "axis", axis)
+    base = P.orszag_tang_2d(n=40, margin=margin)
+    ix, jx = base.fields["ro"].shape
+    _, dx, dxm = stretched_grid(ix, seed=11)
+    _, dy, dym = stretched_grid(jx, seed=12)
+    if axis == "x":
+        dx[0], dx[ix - 1] = 0.3 * dx.min(), 0.35 * dx.min()
+    else:
+        dy[0], dy[jx - 1] = 0.35 * dy.min(), 0.3 * dy.min()
+    grid = dict(base.grid, dx=dx, dxm=dxm, dy=dy, dym=dym)
+    pb = Problem(base.name, base.fields, grid, base.params)
+    nsteps = 12
+    ref, dts = _loop_2d_oracle(oracle, pb, nsteps)
+    pp = pb.params
+    st = [to_dev(pb.fields[k], dev) for k in F2]
+    gd = {k: to_dev(v, dev) for k, v in grid.items()}
+    run = cans.MlwRun2D(st, gd["dx"], gd["dxm"], gd["dy"], gd["dym"], pp["qav"], pp["gm"], pp["safety"],
+                        margin=margin)
+    run.reset(dtmin=pp["dtmin"])
+    run.steps(nsteps)
+    cur, info = run.state()
+    assert info["nexec"] == nsteps and info["merr"] == 0
+    assert info["dt"] == dts[-1]
+    for k, tt in zip(F2, cur):
+        assert_bitwise(to_host(tt), ref[k], k)
+
+
 def sum_sequential(xs):
     t = 0.0
     for x in xs:

@@ -109,6 +109,7 @@ struct ColCtx {
     double ux0, ux1;              // 0.5*dxm(i)/dx(i), 0.5*dxm(i-1)/dx(i)
     double qxs;                   // qav*dxm(i)
     double dxc;                   // dx(i)
+    double dxg;                   // min of dx(i) and dx of i's periodic x images
 };
 
 // Row j (clamped to the grid, so every lane reads a real cell) of a field:
@@ -306,9 +307,15 @@ CANS_DEV double eq_xy(const RowCoef &r, double fE, double fC, double gN, double 
 
 // Primitive recovery (mlw_m.f:299-321), the CFL term of the new state
 // (cans2d/common/cfl_m.f:46-55) and the stores of the updated cell (i, j),
-// with its periodic ghost images.
+// with its periodic ghost images.  When the kernel writes the ghost images
+// (per_margin = m > 0) only interior cells, i in [m+1, ix-m] and j in
+// [m+1, jx-m], store and enter the CFL: the reference's bdperx/bdpery
+// overwrite what mlw_m computed in the ghost cells 2..m and ix-m+1..ix-1
+// with interior images.
 CANS_DEV void finish_row(const Mlw2dArgs &a, const YMet &y, const ColCtx &x, int j, const RowState &s,
                          bool do_cfl, double &dtmin_local) {
+    const int mgj = a.per_margin;
+    const bool out = x.out && (mgj == 0 || (j >= mgj + 1 && j <= a.jx - mgj));   // uniform in j
     const double gm = a.gm;
     const double ro = s.pro, ee = s.pee, rx = s.prx, ry = s.pry, bx = s.pbx, by = s.pby, az = s.paz;
     const double vx = rx / ro;
@@ -323,10 +330,21 @@ CANS_DEV void finish_row(const Mlw2dArgs &a, const YMet &y, const ColCtx &x, int
         const double cs2 = gm * pr / ro;
         const double v2 = vx * vx + vy * vy;
         sq = __builtin_sqrt(v2 + cs2 + ca2);
-        const double dtq = a.safety * (fminF(x.dxc, y.dy[j - 1]) / sq);
-        dtmin_local = (x.out && dtq < dtmin_local) ? dtq : dtmin_local;
+        // The reference's cfl_m also visits the periodic ghost images of
+        // this cell, which hold the same state: their terms are
+        // safety*(min(dx(gi), dy(gj))/sq) over the images (gi, gj).  For
+        // sq >= 0, m -> safety*(m/sq) is monotone non-decreasing in IEEE
+        // arithmetic (a NaN sq gives NaN terms that never win either way), so
+        // the smallest of those terms is safety*(min(min dx, min dy)/sq) bit
+        // for bit: one division per cell and no per-image metric loads.
+        const int mg = a.per_margin, jx = a.jx;
+        double dyg = y.dy[j - 1];
+        if (mg > 0 && j >= jx - 2 * mg + 1) dyg = fminF(dyg, y.dy[j - (jx - 2 * mg) - 1]);
+        if (mg > 0 && j <= 2 * mg) dyg = fminF(dyg, y.dy[j + (jx - 2 * mg) - 1]);
+        const double dtq = a.safety * (fminF(x.dxg, dyg) / sq);
+        dtmin_local = (out && dtq < dtmin_local) ? dtq : dtmin_local;
     }
-    if (x.out) {
+    if (out) {
         const int ix = a.ix, jx = a.jx;
         // Stores as scalar row address + 32-bit lane offset.  The empty asm
         // makes the offset opaque so that the compiler does not hoist seven
@@ -354,10 +372,6 @@ CANS_DEV void finish_row(const Mlw2dArgs &a, const YMet &y, const ColCtx &x, int
                     const size_t q = (size_t)(xi[c] - 1) + (size_t)ix * (size_t)(yj[b] - 1);
                     a.ro_o[q] = ro; a.pr_o[q] = pr; a.vx_o[q] = vx; a.vy_o[q] = vy;
                     a.bx_o[q] = bx; a.by_o[q] = by; a.az_o[q] = az;
-                    if (do_cfl) {
-                        const double dtq = a.safety * (fminF(a.dx[xi[c] - 1], a.dy[yj[b] - 1]) / sq);
-                        if (dtq < dtmin_local) dtmin_local = dtq;
-                    }
                 }
             }
         }
@@ -484,7 +498,8 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
         const int je = min(jb + H - 1, jx - 1);
         ColCtx x;
         x.i = 1 + strip * COLS + lane;         // lane 0 is the west halo column
-        x.out = lane >= 1 && lane <= COLS && x.i <= ix - 1;
+        x.out = lane >= 1 && lane <= COLS && x.i <= ix - 1 &&
+                (a.per_margin == 0 || (x.i >= a.per_margin + 1 && x.i <= ix - a.per_margin));
         const int ic = min(x.i, ix);           // lanes past the grid read column ix
         x.boff = (unsigned)(ic - 1) * 8u;
         x.dxi = a.mx.di[ic - 1];
@@ -494,6 +509,12 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
         x.ux1 = a.mx.u1[ic - 1];
         x.qxs = a.qav * a.dxm[ic - 1];
         x.dxc = a.dx[ic - 1];
+        x.dxg = x.dxc;
+        {
+            const int mg = a.per_margin;
+            if (mg > 0 && ic >= ix - 2 * mg + 1) x.dxg = fminF(x.dxg, a.dx[ic - (ix - 2 * mg) - 1]);
+            if (mg > 0 && ic <= 2 * mg) x.dxg = fminF(x.dxg, a.dx[ic + (ix - 2 * mg) - 1]);
+        }
 
         // prologue: cells of rows jb-1 and jb, corner row jb-1
         const Cell R0 = make_cell(load_prim(a, x, jb - 1), a.gm);
