@@ -7,7 +7,8 @@
 // the corrector U^{n+1} = U + dt sum_d (F_{i-1}-F_i)/dx.  Roe is FP64-VALU
 // bound (~500-700 flops per face), so each face flux is computed exactly
 // once (a flux kernel per direction writing a face array), and the per-cell
-// kernels only combine fluxes.  Arrays are SoA FP64, column-major.
+// kernels only combine fluxes.  Arrays are SoA FP64, column-major (the face
+// arrays of an FP32-flux run hold FP32).
 //
 // Fields: U  = ro, ee, rx, ry, rz, bx, by, bz (8; 2D: no rz/bz; 1D: ro ee rx ry by)
 //         P  = ro, pr, vx, vy, vz, bx, by, bz (primitives, same slots)
@@ -48,6 +49,12 @@ CANS_DEV bool step_dt(const RoeArgs &a, double &dt) {
     dt = a.clk->dt_last[d];
     return true;
 }
+
+// Face fluxes are stored in the type they are computed in: FP32-flux runs
+// keep FP32 face arrays (half the scratch traffic); widening a float to
+// double is exact, so the cell kernels see the same values either way.
+template <class T> CANS_DEV void fput(double *f, size_t p, T v) { reinterpret_cast<T *>(f)[p] = v; }
+template <class T> CANS_DEV double fget(const double *f, size_t p) { return double(reinterpret_cast<const T *>(f)[p]); }
 
 // ------------------------------------------------------------------ fluxes
 // Face (i,j,k) in direction `dir` lies between cell (i,j,k) and the next
@@ -120,9 +127,9 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
         T o[7];
         roeflux3<T>(gm, L[S_RO], L[S_E], L[vn], L[v1], L[v2], L[bn], L[b1], L[b2], R[S_RO], R[S_E], R[vn], R[v1],
                     R[v2], R[bn], R[b1], R[b2], o);
-        F[S_RO][p] = double(o[0]); F[S_E][p] = double(o[1]);
-        F[vn][p] = double(o[2]); F[v1][p] = double(o[3]); F[v2][p] = double(o[4]);
-        F[b1][p] = double(o[5]); F[b2][p] = double(o[6]);
+        fput<T>(F[S_RO], p, o[0]); fput<T>(F[S_E], p, o[1]);
+        fput<T>(F[vn], p, o[2]); fput<T>(F[v1], p, o[3]); fput<T>(F[v2], p, o[4]);
+        fput<T>(F[b1], p, o[5]); fput<T>(F[b2], p, o[6]);
     } else if (C3) {
         // roe_m3.f: x = (x, y, z); 2D y = (y, x, z)
         const int vn = dir == 0 ? S_X : S_Y, v1 = dir == 0 ? S_Y : S_X;
@@ -130,17 +137,17 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
         T o[7];
         roeflux3<T, false>(gm, L[S_RO], L[S_E], L[vn], L[v1], L[S_Z], L[bn], L[b1], L[S_BZ], R[S_RO], R[S_E], R[vn],
                            R[v1], R[S_Z], R[bn], R[b1], R[S_BZ], o);
-        F[S_RO][p] = double(o[0]); F[S_E][p] = double(o[1]);
-        F[vn][p] = double(o[2]); F[v1][p] = double(o[3]); F[S_Z][p] = double(o[4]);
-        F[b1][p] = double(o[5]); F[S_BZ][p] = double(o[6]);
+        fput<T>(F[S_RO], p, o[0]); fput<T>(F[S_E], p, o[1]);
+        fput<T>(F[vn], p, o[2]); fput<T>(F[v1], p, o[3]); fput<T>(F[S_Z], p, o[4]);
+        fput<T>(F[b1], p, o[5]); fput<T>(F[S_BZ], p, o[6]);
     } else {
         // 2D x: (vx,vy,bx,by) -> ro ee rx ry by ; 2D y: (vy,vx,by,bx) -> ro ee ry rx bx ; 1D as 2D x
         const int vn = dir == 0 ? S_X : S_Y, vt = dir == 0 ? S_Y : S_X;
         const int bn = dir == 0 ? S_BX : S_BY, bt = dir == 0 ? S_BY : S_BX;
         T o[5];
         roeflux12<T>(gm, L[S_RO], L[S_E], L[vn], L[vt], L[bn], L[bt], R[S_RO], R[S_E], R[vn], R[vt], R[bn], R[bt], o);
-        F[S_RO][p] = double(o[0]); F[S_E][p] = double(o[1]);
-        F[vn][p] = double(o[2]); F[vt][p] = double(o[3]); F[bt][p] = double(o[4]);
+        fput<T>(F[S_RO], p, o[0]); fput<T>(F[S_E], p, o[1]);
+        fput<T>(F[vn], p, o[2]); fput<T>(F[vt], p, o[3]); fput<T>(F[bt], p, o[4]);
     }
 }
 
@@ -176,7 +183,7 @@ CANS_DEV void prim_to_cons(const RoeArgs &a, size_t p, int i, double *U) {
 // (each sweep a separate rounding, as roe_m.f); skip the face-normal field.
 // Cells on the outer layer keep their primitives (the reference leaves the
 // half-step there unset; it only reaches ghost cells when margin >= 3).
-template <int D, bool C3>
+template <int D, class T, bool C3>
 __global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
     constexpr bool FULL = D == 3 || C3;
     double dt;
@@ -209,7 +216,7 @@ __global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
         for (int q = 0; q < 8; q++) {
             if (q == skipB || !a.prim[q] || (D == 1 && q == S_BX)) continue;
             if (!FULL && (q == S_Z || q == S_BZ)) continue;
-            U[q] = U[q] + 0.5 * dt * ((F[q][p - s] - F[q][p]) / dl);
+            U[q] = U[q] + 0.5 * dt * ((fget<T>(F[q], p - s) - fget<T>(F[q], p)) / dl);
         }
     }
     // half-step primitives (roe_m.f; 1D: b2 with the cell bx, roe_m.f:94-100)
@@ -239,7 +246,7 @@ __global__ __launch_bounds__(256) void roe_half_kernel(RoeArgs a) {
 // ------------------------------------------------------------------ corrector
 // cells [3,n-2]^D: U + dt (F(i-1)-F(i))/dx per sweep, then (2D) the Az
 // source from the half-step state, then primitives -> out arrays.
-template <int D, bool C3>
+template <int D, class T, bool C3>
 __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
     constexpr bool FULL = D == 3 || C3;
     double dt;
@@ -268,7 +275,7 @@ __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
             if (q == skipB || (D == 1 && q == S_BX)) continue;
             if (!FULL && (q == S_Z || q == S_BZ)) continue;
             if (D == 1 && q == S_BX) continue;
-            U[q] = U[q] + dt * ((F[q][p - s] - F[q][p]) / dl);
+            U[q] = U[q] + dt * ((fget<T>(F[q], p - s) - fget<T>(F[q], p)) / dl);
         }
     }
     if (D == 2) {   // roe_m.f:187-193
@@ -320,12 +327,12 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                                a, dir, stage);
         }
         if (stage == 1)
-            hipLaunchKernelGGL((roe_half_kernel<D, C3>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((roe_half_kernel<D, T, C3>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a);
     }
     size_t cells = 1;
     for (int d = 0; d < D; d++) cells *= (size_t)(n[d] - 4 > 0 ? n[d] - 4 : 0);
     if (cells)
-        hipLaunchKernelGGL((roe_update_kernel<D, C3>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((roe_update_kernel<D, T, C3>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
