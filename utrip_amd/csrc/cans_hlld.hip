@@ -186,24 +186,20 @@ __global__ __launch_bounds__(256) void hlld_cons(HlldArgs a) {
 // Flux of the face between cell p and cell p+s along `dir` (face index
 // aidx in [1, n_dir-1]), in the U slots ro ee rx ry rz bx by bz (the normal
 // field's flux is 0, hlld_mh.f:56-94).
+// Face flux from the values of the four cells around it (d0..d3 along the
+// sweep, the face between d1 and d2): MUSCL-minmod states where muscl
+// (inp_tvdminmod.f), else the piecewise-constant ones (inp_const.f, d0 and
+// d3 unused).
 template <class T>
-CANS_DEV void hlld_face_flux(const HlldArgs &a, const double *const *P, int dir, int stage, int aidx, int ndir,
-                             size_t p, size_t s, double *F) {
-    const bool muscl = stage == 2 && aidx >= 2 && aidx <= ndir - 2;   // inp_tvdminmod.f
-    // four cells, clamped to the face's two where the states are piecewise
-    // constant (inp_const.f)
-    const size_t pm = muscl ? p - s : p, p3 = muscl ? p + 2 * s : p + s;
+CANS_DEV void hlld_face_w(const HlldArgs &a, int dir, bool muscl, const double *d0, const double *d1,
+                          const double *d2, const double *d3, double *F) {
     T L[8], R[8];
-    double bn0 = 0.0, bn1 = 0.0;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-        const double *da = P[q];
-        const double d0 = da[pm], d1 = da[p], d2 = da[p + s], d3 = da[p3];
-        L[q] = muscl ? T(d1 + 0.5 * flmtH(d1 - d0, d2 - d1)) : T(d1);
-        R[q] = muscl ? T(d2 - 0.5 * flmtH(d3 - d2, d2 - d1)) : T(d2);
-        if (q == P_BX + dir) { bn0 = d1; bn1 = d2; }
+        L[q] = muscl ? T(d1[q] + 0.5 * flmtH(d1[q] - d0[q], d2[q] - d1[q])) : T(d1[q]);
+        R[q] = muscl ? T(d2[q] - 0.5 * flmtH(d3[q] - d2[q], d2[q] - d1[q])) : T(d2[q]);
     }
-    const T bb = T((bn0 + bn1) / 2.0);   // hlld_mh.f:47/65/82
+    const T bb = T((d1[P_BX + dir] + d2[P_BX + dir]) / 2.0);   // hlld_mh.f:47/65/82
     const int vn = P_VX + dir, v1 = P_VX + (dir + 1) % 3, v2 = P_VX + (dir + 2) % 3;
     const int b1 = P_BX + (dir + 1) % 3, b2 = P_BX + (dir + 2) % 3;
     T o[7];
@@ -219,6 +215,23 @@ CANS_DEV void hlld_face_flux(const HlldArgs &a, const double *const *P, int dir,
     F[U_BX + dir] = 0.0;
 }
 
+// Face aidx (of ndir along the sweep) between cell p and p+s.
+template <class T>
+CANS_DEV void hlld_face_flux(const HlldArgs &a, const double *const *P, int dir, int stage, int aidx, int ndir,
+                             size_t p, size_t s, double *F) {
+    const bool muscl = stage == 2 && aidx >= 2 && aidx <= ndir - 2;   // inp_tvdminmod.f
+    // four cells, clamped to the face's two where the states are piecewise
+    // constant (inp_const.f)
+    const size_t pm = muscl ? p - s : p, p3 = muscl ? p + 2 * s : p + s;
+    double d0[8], d1[8], d2[8], d3[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const double *da = P[q];
+        d0[q] = da[pm]; d1[q] = da[p]; d2[q] = da[p + s]; d3[q] = da[p3];
+    }
+    hlld_face_w<T>(a, dir, muscl, d0, d1, d2, d3, F);
+}
+
 // adddfdx.f for one cell: u <- u - dts*((F_hi - F_lo)/dd)
 CANS_DEV void hlld_apply(double *const *u, size_t p, double dts, double dd, const double *hi, const double *lo) {
 #pragma unroll
@@ -227,10 +240,9 @@ CANS_DEV void hlld_apply(double *const *u, size_t p, double dts, double dd, cons
 
 // x sweep: a wave covers faces i = 63*strip + lane (lane 0 overlaps the
 // previous strip), cells i = 2..ix-1 take their west face from lane-1.
-// FP64 fluxes are compiled for four waves per SIMD (128 VGPRs, a few
-// spilled) and the y/z sweep for three (168): at 384^3 an FP64 step takes
-// 31.3 ms against 38.0 ms at the compiler's own choice (134 and 170 VGPRs,
-// three and two waves).
+// With FP64 fluxes it is compiled for four waves per SIMD (128 VGPRs, a few
+// spilled): at 384^3 that took an FP64 step from 38.0 to 31.3 ms against the
+// compiler's own choice (134 VGPRs, three waves).
 template <class T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 1))) void hlld_sweep_x(HlldArgs a, int stage) {
     double dt;
@@ -262,9 +274,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 }
 
 // y or z sweep: thread (i, other transverse index, segment) marches over the
-// cells cb..ce of its segment, computing faces cb-1..ce once each.
+// cells cb..ce of its segment, computing faces cb-1..ce once each from a
+// sliding window of four cells (8 loads per face instead of 32: at 384^3
+// the FP64 step went from 31.3 to 26.4 ms, FP32-flux from 26.0 to 21.9 ms).
+// The window needs two waves per SIMD with FP64 fluxes (231 VGPRs; at three
+// the spills made it slower), three with FP32.
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 1))) void hlld_sweep_yz(HlldArgs a, int dir, int stage, int H) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 1))) void hlld_sweep_yz(HlldArgs a, int dir, int stage, int H) {
     double dt;
     if (!hstep(a, dt)) return;
     const double dts = stage == 1 ? 0.5 * dt : dt;
@@ -284,11 +300,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 #pragma unroll
     for (int q = 0; q < 8; q++) { P[q] = stage == 1 ? a.prim[q] : a.half[q]; u[q] = stage == 1 ? a.u1[q] : a.u2[q]; }
     const double *dd = dir == 1 ? a.dy : a.dz;
+    // cells c-1..c+2 of face c in a sliding window: one new cell per face
+    // (cell indices clamped to [1, nd]; a clamped cell only feeds a face
+    // whose states are piecewise constant, which does not read it)
+    double w0[8], w1[8], w2[8], w3[8];
+    auto load_cell = [&](int cell, double *d) {
+        const int cc = cell < 1 ? 1 : (cell > nd ? nd : cell);
+        const size_t pc = base + s * (size_t)(cc - 1);
+#pragma unroll
+        for (int q = 0; q < 8; q++) d[q] = P[q][pc];
+    };
+    load_cell(cb - 2, w0); load_cell(cb - 1, w1); load_cell(cb, w2); load_cell(cb + 1, w3);
     double Fl[8], Fh[8];
-    hlld_face_flux<T>(a, P, dir, stage, cb - 1, nd, base + s * (size_t)(cb - 2), s, Fl);
+    hlld_face_w<T>(a, dir, stage == 2 && cb - 1 >= 2 && cb - 1 <= nd - 2, w0, w1, w2, w3, Fl);
     for (int c = cb; c <= ce; c++) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) { w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; }
+        load_cell(c + 2, w3);
+        hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
         const size_t p = base + s * (size_t)(c - 1);
-        hlld_face_flux<T>(a, P, dir, stage, c, nd, p, s, Fh);
         hlld_apply(u, p, dts, dd[c - 1], Fh, Fl);
 #pragma unroll
         for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
