@@ -259,6 +259,41 @@ def m3_figures(n: int = 1024, steps: int = 100):
             "engine_avg_us": kern * 1e3, "engine_hbm_frac": 144 * z / (kern / 1e3) / 1e9 / HBM_PEAK_GBS}
 
 
+def roe2d_figures(n: int = 1024, steps: int = 100):
+    """The same Orszag-Tang vortex at n^2 with the 2D Roe-MUSCL engine
+    (cans2d/hdroe/roe_m.f; margin 3 as the reference drivers use, periodic
+    bdper, cfl_m each step): whole-step throughput and the engine's average
+    time (HIP events around it every step) against 112 B per zone-update."""
+    import numpy as np
+    import torch
+
+    from utrip_amd import cans, problems
+    pb = problems.orszag_tang_2d(n=n, margin=3)
+
+    def td(a):
+        return torch.from_numpy(np.ascontiguousarray(a.T)).to("cuda:0")
+
+    st = [td(pb.fields[k]) for k in cans.FIELDS[2]]
+    g = {k: torch.from_numpy(pb.grid[k]).to("cuda:0") for k in ("dx", "dxm", "dy", "dym")}
+    pp = pb.params
+    run = cans.Run(2, cans.ENGINE_ROE, st, g, pp["gm"], pp["safety"], qav=pp["qav"], margin=3, bc=7)
+    run.reset(dtmin=pp["dtmin"])
+    run.steps(5)
+    ev = cans.EventSet(2 * steps + 2)
+    torch.cuda.synchronize()
+    ev.record(2 * steps)
+    run.steps(steps, events=ev)
+    ev.record(2 * steps + 1)
+    torch.cuda.synchronize()
+    total = ev.elapsed_ms(2 * steps, 2 * steps + 1)
+    kern = statistics.fmean(ev.elapsed_ms(2 * k, 2 * k + 1) for k in range(steps))
+    _, info = run.state()
+    assert info["merr"] == 0 and info["nexec"] == steps + 5, info
+    z = n * n
+    return {"cells": z, "steps": steps, "ms_per_step": total / steps, "zone_updates_per_s": z * steps / (total / 1e3),
+            "engine_avg_us": kern * 1e3, "engine_hbm_frac": B_ALG_2D * z / (kern / 1e3) / 1e9 / HBM_PEAK_GBS}
+
+
 def spicule_cpu(steps: int):
     """CPU leg of the spicule figures: the C restatement's main.f loop
     (oracle/liboracle.so, gcc -O2, one core) for `steps` steps from t = 0."""
@@ -461,9 +496,10 @@ def main():
     adj = None
     if args.adjoint:
         adj = adjoint_figures()
-    m3 = None
+    m3 = roe2 = None
     if not args.no_extra:
         m3 = m3_figures()
+        roe2 = roe2d_figures()
     spc = None
     if args.spicule_runs > 0 or args.spicule_steps > 0:
         spc = spicule_figures(args.spicule_runs, args.spicule_steps, no_cpu=args.no_cpu)
@@ -521,6 +557,7 @@ def main():
         "adjoint_ot512": adj,
         "spicule": spc,
         "ot1024_mlw_m3": m3,
+        "ot1024_roe": roe2,
     }
     print(json.dumps(out), flush=True)
 
