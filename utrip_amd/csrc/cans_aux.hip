@@ -154,14 +154,19 @@ __global__ __launch_bounds__(256) void cfl_kernel(CflArgs a) {
         if (!clock_live(a.clk, a.step)) return;   // uniform across the launch
         key = &a.clk->dtkey[a.step % 3];
     }
-    const size_t n = (size_t)a.ix * a.jx * a.kx;
     double m = kR4_1e20;
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(p % a.ix) + 1;
-        const size_t r = p / a.ix;
-        const int j = (int)(r % a.jx) + 1;
-        const int k = (int)(r / a.jx) + 1;
-        if (!cfl_cell_in(a, i)) continue;
+    // blocks take (row, 256-column chunk) items: the row indices are
+    // block-uniform, so no per-cell 64-bit division
+    const int nch = (a.ix + 255) / 256;
+    const long items = (long)nch * a.jx * a.kx;
+    for (long it = blockIdx.x; it < items; it += gridDim.x) {
+        const int ch = (int)(it % nch);
+        const long row = it / nch;
+        const int j = (int)(row % a.jx) + 1;
+        const int k = (int)(row / a.jx) + 1;
+        const int i = ch * 256 + (int)threadIdx.x + 1;
+        if (i > a.ix || !cfl_cell_in(a, i)) continue;
+        const size_t p = (size_t)(i - 1) + (size_t)row * (size_t)a.ix;
         const double q = cfl_cell(a, i, j, k, p);
         if (q < m) m = q;   // the reference's `if (dtq < dt)`: NaN never wins
     }
@@ -170,13 +175,16 @@ __global__ __launch_bounds__(256) void cfl_kernel(CflArgs a) {
 
 // first cell in the reference's loop order (i fastest) with dtq == dt
 __global__ void cfl_locate_kernel(CflArgs a, double dt, unsigned long long *first) {
-    const size_t n = (size_t)a.ix * a.jx * a.kx;
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(p % a.ix) + 1;
-        const size_t r = p / a.ix;
-        const int j = (int)(r % a.jx) + 1;
-        const int k = (int)(r / a.jx) + 1;
-        if (!cfl_cell_in(a, i)) continue;
+    const int nch = (a.ix + 255) / 256;
+    const long items = (long)nch * a.jx * a.kx;
+    for (long it = blockIdx.x; it < items; it += gridDim.x) {
+        const int ch = (int)(it % nch);
+        const long row = it / nch;
+        const int j = (int)(row % a.jx) + 1;
+        const int k = (int)(row / a.jx) + 1;
+        const int i = ch * 256 + (int)threadIdx.x + 1;
+        if (i > a.ix || !cfl_cell_in(a, i)) continue;
+        const size_t p = (size_t)(i - 1) + (size_t)row * (size_t)a.ix;
         if (cfl_cell(a, i, j, k, p) == dt) atomicMin(first, (unsigned long long)p);
     }
 }
@@ -253,22 +261,21 @@ hipError_t launch_copy_interior(double *const *dst, const double *const *src, in
     return hipGetLastError();
 }
 
+// blocks of 256 threads over (row, 256-column chunk) items, at most 2048
+static unsigned cfl_blocks(const CflArgs &a) {
+    const size_t items = (size_t)((a.ix + 255) / 256) * (size_t)a.jx * (size_t)a.kx;
+    return (unsigned)(items < 1 ? 1 : (items > 2048 ? 2048 : items));
+}
+
 hipError_t launch_cfl(const CflArgs &a, hipStream_t s) {
-    const size_t n = (size_t)a.ix * a.jx * a.kx;
-    unsigned blocks = (unsigned)((n + 255) / 256);
-    if (blocks > 2048u) blocks = 2048u;
-    if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(cfl_kernel, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(cfl_kernel, dim3(cfl_blocks(a)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_cfl_locate(const CflArgs &a, double dt, int *loc_dev, hipStream_t s) {
     unsigned long long *first = reinterpret_cast<unsigned long long *>(loc_dev + 4);
     hipLaunchKernelGGL(locate_arm, dim3(1), dim3(1), 0, s, first);
-    const size_t n = (size_t)a.ix * a.jx * a.kx;
-    unsigned blocks = (unsigned)((n + 255) / 256);
-    if (blocks > 2048u) blocks = 2048u;
-    hipLaunchKernelGGL(cfl_locate_kernel, dim3(blocks), dim3(256), 0, s, a, dt, first);
+    hipLaunchKernelGGL(cfl_locate_kernel, dim3(cfl_blocks(a)), dim3(256), 0, s, a, dt, first);
     hipLaunchKernelGGL(locate_finish, dim3(1), dim3(1), 0, s, first, a.ix, a.jx, loc_dev);
     return hipGetLastError();
 }
