@@ -40,6 +40,11 @@ extern "C" {
 const char *cans_gpu_last_error(void);
 int cans_gpu_init(int device);
 int cans_gpu_sync(void *stream);
+/* Tuning, no counterpart in the reference: the marching face kernels (HLLD
+ * y/z sweeps, 3D Roe y/z fluxes) cut each column of faces into segments so
+ * that one launch has at least `waves` wavefronts (default 32768; fewer
+ * waves = longer segments).  Results do not depend on it. */
+int cans_gpu_set_march_waves(int waves);
 
 /* ---------------------------------------------------------------- 1D
  * cans1d/hdmlw/mlw_m.f:2 mlw_m(ro,pr,vx,vy,by,bx,bxm,dt,qav,gm,dx,dxm,ix)

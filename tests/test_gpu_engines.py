@@ -241,6 +241,37 @@ def test_run_3d_blast(dev, oracle, engine, margin):
         assert_bitwise(got[k], ref[k], k)
 
 
+@pytest.mark.parametrize("waves", [1, 8])
+@pytest.mark.parametrize("engine", ["roe", "hlld"])
+def test_run_3d_blast_long_segments(dev, oracle, engine, waves):
+    """The marching face kernels with long segments: at this size the default
+    wave target gives one face per thread, so force whole columns (1 wave)
+    and a few segments per column (8 waves); bit-exact either way."""
+    from utrip_amd import cans
+    case("run 3d blast, march waves", engine, waves)
+    margin = 3
+    pb = P.blast_3d(n=14, margin=margin, gaussian_b=(engine != "hlld"), r0=0.25)
+    ref, dts = oracle_loop(oracle, 3, engine, pb.fields, pb.grid, pb.params, 6, 7, margin)
+    cans.set_march_waves(waves)
+    try:
+        got, info = run_gpu(dev, 3, engine, pb.fields, pb.grid, pb.params, 6, 7, margin)
+        got32, _ = run_gpu(dev, 3, engine, pb.fields, pb.grid, pb.params, 6, 7, margin, mixed=True)
+    finally:
+        cans.set_march_waves(32768)
+    assert info["nexec"] == 6 and info["dt"] == dts[-1]
+    for k in F3:
+        assert_bitwise(got[k], ref[k], k)
+    ref32, _ = run_gpu(dev, 3, engine, pb.fields, pb.grid, pb.params, 6, 7, margin, mixed=True)
+    for k in F3:
+        assert_bitwise(got32[k], ref32[k], "fp32-flux " + k)
+
+
+def test_march_waves_rejects_zero(dev):
+    from utrip_amd import cans
+    with pytest.raises(cans.CansError):
+        cans.set_march_waves(0)
+
+
 @pytest.mark.parametrize("engine", ["roe", "hlld"])
 def test_mixed_precision_3d(dev, oracle, engine):
     """FP32 face fluxes + FP64 state: density within 1e-5 relative L1 of the

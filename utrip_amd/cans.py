@@ -46,6 +46,13 @@ def _check(rc: int) -> None:
         raise CansError(f"libcans_gpu error {rc}: {lib().cans_gpu_last_error().decode()}")
 
 
+def set_march_waves(waves: int) -> None:
+    """Minimum wavefronts per launch of the marching face kernels (HLLD y/z
+    sweeps, 3D Roe y/z fluxes; default 32768): fewer waves, longer segments.
+    Results do not depend on it."""
+    _check(lib().cans_gpu_set_march_waves(int(waves)))
+
+
 def _stream() -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 

@@ -11,12 +11,14 @@
 #include <stdio.h>
 #include <string.h>
 #include <cmath>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <vector>
 
 #include "cans_common.hpp"
 #include "cans_kernels.hpp"
+#include "cans_roe.hpp"
 #include "../../include/cans_gpu.h"
 
 using namespace cans;
@@ -80,6 +82,11 @@ inline size_t al(size_t n) { return (n + 63) & ~size_t(63); }  // doubles, 512-b
 
 int cans::abi_scratch(size_t bytes, void **out) { return scratch(bytes, out); }
 
+namespace {
+std::atomic<int> g_march_waves{32768};
+}
+int cans::march_waves() { return g_march_waves.load(); }
+
 // sin, cos, tan of the colatitudes y (and ym) on the host -- the C library's
 // functions, as the reference's compiled intrinsics -- into dev[0..3jx)
 // (y) and dev[3jx..6jx) (ym, when given): the tables of the spherical engine.
@@ -114,6 +121,12 @@ int cans_gpu_init(int device) {
 
 int cans_gpu_sync(void *stream) {
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+int cans_gpu_set_march_waves(int waves) {
+    if (waves < 1) return fail(CANS_ERR_ARG, "march waves %d < 1", waves);
+    g_march_waves.store(waves);
     return 0;
 }
 

@@ -367,10 +367,10 @@ hipError_t launch_hlld_t(const HlldArgs &a, hipStream_t s) {
                 const size_t waves = (size_t)((a.ix + 62) / 63) * a.jx * a.kx;
                 hipLaunchKernelGGL(hlld_sweep_x<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a, stage);
             } else {
-                // segments so that the launch has >= 32768 waves (a sweep of
-                // 2048, 8192, 32768, 131072 at 384^3: 32768 is fastest)
+                // segments so that the launch has >= march_waves() waves (a
+                // sweep of 2048, 8192, 32768, 131072 at 384^3: 32768 is fastest)
                 const size_t cols = N / n[dir];
-                int nseg = (int)((32768ull * 64 + cols - 1) / cols);
+                int nseg = (int)(((unsigned long long)march_waves() * 64 + cols - 1) / cols);
                 nseg = nseg < 1 ? 1 : (nseg > n[dir] - 2 ? n[dir] - 2 : nseg);
                 const int H = (n[dir] - 2 + nseg - 1) / nseg;
                 nseg = (n[dir] - 2 + H - 1) / H;

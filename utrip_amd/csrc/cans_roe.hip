@@ -392,12 +392,12 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
             if (D == 3 && dir >= 1) {
                 // y and z faces marched (roe_flux_march3: 384^3 FP64 step 26.4 ->
                 // 23.7 ms, FP32-flux 19.5 -> 17.7 ms); segments along dir so that
-                // the launch has >= 32768 waves
+                // the launch has >= march_waves() waves
                 const int nf = stage == 1 ? n[dir] - 1 : n[dir] - 3;     // faces along dir
                 const int od = dir == 1 ? 2 : 1;
                 const size_t cols = (size_t)(stage == 1 ? n[0] - 2 : n[0] - 4) *
                                     (size_t)(stage == 1 ? n[od] - 2 : n[od] - 4);
-                int nseg = (int)((32768ull * 64 + cols - 1) / cols);
+                int nseg = (int)(((unsigned long long)march_waves() * 64 + cols - 1) / cols);
                 nseg = nseg < 1 ? 1 : (nseg > nf ? nf : nseg);
                 const int H = (nf + nseg - 1) / nseg;
                 nseg = (nf + H - 1) / H;

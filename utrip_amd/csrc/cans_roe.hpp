@@ -64,4 +64,9 @@ struct HlldArgs {
 };
 hipError_t launch_hlld(const HlldArgs &a, bool mixed, hipStream_t s);
 
+// Marching face kernels (HLLD y/z sweeps, Roe 3D y/z fluxes) split each
+// column into segments so that a launch has at least this many waves
+// (default 32768; cans_gpu_set_march_waves changes it).
+int march_waves();
+
 }  // namespace cans
