@@ -4,6 +4,7 @@
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_kernels.hpp"
+#include "cans_wave.hpp"
 
 namespace cans {
 namespace {
@@ -155,16 +156,18 @@ __global__ __launch_bounds__(256) void cfl_kernel(CflArgs a) {
         key = &a.clk->dtkey[a.step % 3];
     }
     double m = kR4_1e20;
-    // blocks take (row, 256-column chunk) items: the row indices are
-    // block-uniform, so no per-cell 64-bit division
-    const int nch = (a.ix + 255) / 256;
+    // waves take (row, 64-column chunk) items: the row indices are
+    // wave-uniform, so no per-cell 64-bit division
+    const int lane = threadIdx.x & 63;
+    const int nch = (a.ix + 63) / 64;
     const long items = (long)nch * a.jx * a.kx;
-    for (long it = blockIdx.x; it < items; it += gridDim.x) {
+    const long wstride = (long)gridDim.x * (blockDim.x >> 6);
+    for (long it = uni((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6))); it < items; it += wstride) {
         const int ch = (int)(it % nch);
         const long row = it / nch;
         const int j = (int)(row % a.jx) + 1;
         const int k = (int)(row / a.jx) + 1;
-        const int i = ch * 256 + (int)threadIdx.x + 1;
+        const int i = ch * 64 + lane + 1;
         if (i > a.ix || !cfl_cell_in(a, i)) continue;
         const size_t p = (size_t)(i - 1) + (size_t)row * (size_t)a.ix;
         const double q = cfl_cell(a, i, j, k, p);
@@ -175,14 +178,16 @@ __global__ __launch_bounds__(256) void cfl_kernel(CflArgs a) {
 
 // first cell in the reference's loop order (i fastest) with dtq == dt
 __global__ void cfl_locate_kernel(CflArgs a, double dt, unsigned long long *first) {
-    const int nch = (a.ix + 255) / 256;
+    const int lane = threadIdx.x & 63;
+    const int nch = (a.ix + 63) / 64;
     const long items = (long)nch * a.jx * a.kx;
-    for (long it = blockIdx.x; it < items; it += gridDim.x) {
+    const long wstride = (long)gridDim.x * (blockDim.x >> 6);
+    for (long it = uni((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6))); it < items; it += wstride) {
         const int ch = (int)(it % nch);
         const long row = it / nch;
         const int j = (int)(row % a.jx) + 1;
         const int k = (int)(row / a.jx) + 1;
-        const int i = ch * 256 + (int)threadIdx.x + 1;
+        const int i = ch * 64 + lane + 1;
         if (i > a.ix || !cfl_cell_in(a, i)) continue;
         const size_t p = (size_t)(i - 1) + (size_t)row * (size_t)a.ix;
         if (cfl_cell(a, i, j, k, p) == dt) atomicMin(first, (unsigned long long)p);
@@ -261,10 +266,12 @@ hipError_t launch_copy_interior(double *const *dst, const double *const *src, in
     return hipGetLastError();
 }
 
-// blocks of 256 threads over (row, 256-column chunk) items, at most 2048
+// blocks of 256 threads (4 waves) over (row, 64-column chunk) items, at
+// most 2048 blocks
 static unsigned cfl_blocks(const CflArgs &a) {
-    const size_t items = (size_t)((a.ix + 255) / 256) * (size_t)a.jx * (size_t)a.kx;
-    return (unsigned)(items < 1 ? 1 : (items > 2048 ? 2048 : items));
+    const size_t items = (size_t)((a.ix + 63) / 64) * (size_t)a.jx * (size_t)a.kx;
+    const size_t b = (items + 3) / 4;
+    return (unsigned)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
 }
 
 hipError_t launch_cfl(const CflArgs &a, hipStream_t s) {
