@@ -167,6 +167,16 @@ CANS_DEV double flmtH(double a, double b) {
     return fmaxF(0.0, fminF(b * signF(1.0, a), __builtin_fabs(a))) * signF(1.0, a);
 }
 
+// conserved state of cell p from the primitives (vvtorv.f, pr2eh.f, eh2ee.f)
+CANS_DEV void hlld_cons_of(const HlldArgs &a, size_t p, double *U) {
+    const double ro = a.prim[P_RO][p], pr = a.prim[P_PR][p], vx = a.prim[P_VX][p], vy = a.prim[P_VY][p];
+    const double vz = a.prim[P_VZ][p], bx = a.prim[P_BX][p], by = a.prim[P_BY][p], bz = a.prim[P_BZ][p];
+    const double eh = pr / (a.gm - 1.0);
+    const double ek = (vx * vx + vy * vy + vz * vz) * ro / 2.0;
+    const double em = (bx * bx + by * by + bz * bz) / (2.0 * a.mu);
+    U[0] = ro; U[1] = eh + ek + em; U[2] = vx * ro; U[3] = vy * ro; U[4] = vz * ro; U[5] = bx; U[6] = by; U[7] = bz;
+}
+
 // conserved state of every cell: vvtorv.f, pr2eh.f, eh2ee.f; into u1 and u2
 __global__ __launch_bounds__(256) void hlld_cons(HlldArgs a) {
     double dt;
@@ -174,12 +184,8 @@ __global__ __launch_bounds__(256) void hlld_cons(HlldArgs a) {
     const size_t N = (size_t)a.ix * a.jx * a.kx;
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
-    const double ro = a.prim[P_RO][p], pr = a.prim[P_PR][p], vx = a.prim[P_VX][p], vy = a.prim[P_VY][p];
-    const double vz = a.prim[P_VZ][p], bx = a.prim[P_BX][p], by = a.prim[P_BY][p], bz = a.prim[P_BZ][p];
-    const double eh = pr / (a.gm - 1.0);
-    const double ek = (vx * vx + vy * vy + vz * vz) * ro / 2.0;
-    const double em = (bx * bx + by * by + bz * bz) / (2.0 * a.mu);
-    const double U[8] = {ro, eh + ek + em, vx * ro, vy * ro, vz * ro, bx, by, bz};
+    double U[8];
+    hlld_cons_of(a, p, U);
     for (int q = 0; q < 8; q++) { a.u1[q][p] = U[q]; a.u2[q][p] = U[q]; }
 }
 
@@ -244,7 +250,8 @@ CANS_DEV void hlld_apply(double *const *u, size_t p, double dts, double dd, cons
 // spilled): at 384^3 that took an FP64 step from 38.0 to 31.3 ms against the
 // compiler's own choice (134 VGPRs, three waves).
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 1))) void hlld_sweep_x(HlldArgs a, int stage) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 1))) void hlld_sweep_x(HlldArgs a, int stage,
+                                                                                                          int cons) {
     double dt;
     if (!hstep(a, dt)) return;
     const double dts = stage == 1 ? 0.5 * dt : dt;   // engine.f: dth = 0.5d0*dt
@@ -265,10 +272,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     double FW[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) FW[q] = shr1(F[q]);
-    if (lane >= 1 && i >= 2 && i <= ix - 1) {
-        double *u[8];
+    double *u[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++) u[q] = stage == 1 ? a.u1[q] : a.u2[q];
+    for (int q = 0; q < 8; q++) u[q] = stage == 1 ? a.u1[q] : a.u2[q];
+    if (cons) {
+        // the first sweep of the stage also starts u from the primitives
+        // (hlld_cons fused): every cell i = 1..ix of the row
+        if (lane >= 1 && i >= 1 && i <= ix) {
+            double U[8];
+            hlld_cons_of(a, p, U);
+            if (i >= 2 && i <= ix - 1) {
+                const double dd = a.dx[i - 1];
+#pragma unroll
+                for (int q = 0; q < 8; q++) u[q][p] = U[q] - dts * ((F[q] - FW[q]) / dd);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; q++) u[q][p] = U[q];
+            }
+        }
+    } else if (lane >= 1 && i >= 2 && i <= ix - 1) {
         hlld_apply(u, p, dts, a.dx[i - 1], F, FW);
     }
 }
@@ -358,14 +380,16 @@ hipError_t launch_hlld_t(const HlldArgs &a, hipStream_t s) {
     const size_t N = (size_t)a.ix * a.jx * a.kx;
     const int n[3] = {a.ix, a.jx, a.kx};
     const unsigned gN = (unsigned)((N + 255) / 256);
-    hipLaunchKernelGGL(hlld_cons, dim3(gN), dim3(256), 0, s, a);
+    // u1, u2 = U(prim): fused into the x sweeps when x is the first sweep
+    const int fuse = a.mfdim[0] == 1 && n[0] >= 3;
+    if (!fuse) hipLaunchKernelGGL(hlld_cons, dim3(gN), dim3(256), 0, s, a);
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < 3; dir++) {
             if (a.mfdim[dir] != 1) continue;
             if (n[dir] < 3) continue;                 // no interior cell along dir
             if (dir == 0) {
                 const size_t waves = (size_t)((a.ix + 62) / 63) * a.jx * a.kx;
-                hipLaunchKernelGGL(hlld_sweep_x<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a, stage);
+                hipLaunchKernelGGL(hlld_sweep_x<T>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, a, stage, fuse);
             } else {
                 // segments so that the launch has >= march_waves() waves (a
                 // sweep of 2048, 8192, 32768, 131072 at 384^3: 32768 is fastest)
