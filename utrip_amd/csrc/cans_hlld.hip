@@ -238,6 +238,26 @@ CANS_DEV void hlld_face_flux(const HlldArgs &a, const double *const *P, int dir,
     hlld_face_w<T>(a, dir, muscl, d0, d1, d2, d3, F);
 }
 
+// primitives of one cell from its conserved state (rvtovv.f, ee2eh.f, eh2pr.f)
+CANS_DEV void hlld_prim_of(const HlldArgs &a, const double *U, double *const *P, size_t p) {
+    const double ro = U[U_RO];
+    const double vx = U[U_RX] / ro;
+    const double vy = U[U_RY] / ro;
+    const double vz = U[U_RZ] / ro;
+    const double bx = U[U_BX], by = U[U_BY], bz = U[U_BZ];
+    const double ek = (vx * vx + vy * vy + vz * vz) * ro / 2.0;
+    const double em = (bx * bx + by * by + bz * bz) / (2.0 * a.mu);
+    const double eh = U[U_EE] - ek - em;
+    P[P_RO][p] = ro;
+    P[P_PR][p] = (a.gm - 1.0) * eh;
+    P[P_VX][p] = vx;
+    P[P_VY][p] = vy;
+    P[P_VZ][p] = vz;
+    P[P_BX][p] = bx;
+    P[P_BY][p] = by;
+    P[P_BZ][p] = bz;
+}
+
 // adddfdx.f for one cell: u <- u - dts*((F_hi - F_lo)/dd)
 CANS_DEV void hlld_apply(double *const *u, size_t p, double dts, double dd, const double *hi, const double *lo) {
 #pragma unroll
@@ -302,7 +322,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 // The window needs two waves per SIMD with FP64 fluxes (231 VGPRs; at three
 // the spills made it slower), three with FP32.
 template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 1))) void hlld_sweep_yz(HlldArgs a, int dir, int stage, int H) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 1))) void hlld_sweep_yz(HlldArgs a, int dir, int stage, int H,
+                                                                                                           int to_prim) {
     double dt;
     if (!hstep(a, dt)) return;
     const double dts = stage == 1 ? 0.5 * dt : dt;
@@ -341,9 +362,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         load_cell(c + 2, w3);
         hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
         const size_t p = base + s * (size_t)(c - 1);
-        hlld_apply(u, p, dts, dd[c - 1], Fh, Fl);
+        if (to_prim) {
+            // the last sweep of the stage also recovers the primitives
+            // (hlld_prim fused): half (stage 1) or out (stage 2)
+            double U[8];
+            const double dl = dd[c - 1];
+#pragma unroll
+            for (int q = 0; q < 8; q++) U[q] = u[q][p] - dts * ((Fh[q] - Fl[q]) / dl);
+            hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p);
+        } else {
+            hlld_apply(u, p, dts, dd[c - 1], Fh, Fl);
+        }
 #pragma unroll
         for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
+    }
+    if (to_prim) {   // the end cells 1 and nd of the column, not swept
+        for (int c = 1; c <= nd; c += nd - 1) {
+            if ((c == 1 && cb != 2) || (c == nd && ce != nd - 1)) continue;
+            const size_t p = base + s * (size_t)(c - 1);
+            double U[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) U[q] = u[q][p];
+            hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p);
+        }
     }
 }
 
@@ -356,23 +397,10 @@ __global__ __launch_bounds__(256) void hlld_prim(HlldArgs a, int stage) {
     const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N) return;
     double *const *u = stage == 1 ? a.u1 : a.u2;
-    double *const *P = stage == 1 ? a.half : a.out;
-    const double ro = u[U_RO][p];
-    const double vx = u[U_RX][p] / ro;
-    const double vy = u[U_RY][p] / ro;
-    const double vz = u[U_RZ][p] / ro;
-    const double bx = u[U_BX][p], by = u[U_BY][p], bz = u[U_BZ][p];
-    const double ek = (vx * vx + vy * vy + vz * vz) * ro / 2.0;
-    const double em = (bx * bx + by * by + bz * bz) / (2.0 * a.mu);
-    const double eh = u[U_EE][p] - ek - em;
-    P[P_RO][p] = ro;
-    P[P_PR][p] = (a.gm - 1.0) * eh;
-    P[P_VX][p] = vx;
-    P[P_VY][p] = vy;
-    P[P_VZ][p] = vz;
-    P[P_BX][p] = bx;
-    P[P_BY][p] = by;
-    P[P_BZ][p] = bz;
+    double U[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) U[q] = u[q][p];
+    hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p);
 }
 
 template <class T>
@@ -380,8 +408,10 @@ hipError_t launch_hlld_t(const HlldArgs &a, hipStream_t s) {
     const size_t N = (size_t)a.ix * a.jx * a.kx;
     const int n[3] = {a.ix, a.jx, a.kx};
     const unsigned gN = (unsigned)((N + 255) / 256);
-    // u1, u2 = U(prim): fused into the x sweeps when x is the first sweep
+    // u1, u2 = U(prim): fused into the x sweeps when x is the first sweep;
+    // the primitives of each stage: fused into the z sweeps when z is the last
     const int fuse = a.mfdim[0] == 1 && n[0] >= 3;
+    const int fuse_p = a.mfdim[2] == 1 && n[2] >= 3;
     if (!fuse) hipLaunchKernelGGL(hlld_cons, dim3(gN), dim3(256), 0, s, a);
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < 3; dir++) {
@@ -400,10 +430,10 @@ hipError_t launch_hlld_t(const HlldArgs &a, hipStream_t s) {
                 nseg = (n[dir] - 2 + H - 1) / H;
                 const size_t nt = cols * (size_t)nseg;
                 hipLaunchKernelGGL(hlld_sweep_yz<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a, dir, stage,
-                                   H);
+                                   H, dir == 2 && fuse_p);
             }
         }
-        hipLaunchKernelGGL(hlld_prim, dim3(gN), dim3(256), 0, s, a, stage);
+        if (!fuse_p) hipLaunchKernelGGL(hlld_prim, dim3(gN), dim3(256), 0, s, a, stage);
     }
     return hipGetLastError();
 }
