@@ -2,14 +2,18 @@
 // Miyoshi & Kusano 2005; mu units) on gfx950, FP64 (bit-exact) and FP32-flux
 // mixed precision.
 //
-// engine.f:2-73 as kernels: conserved state, then per stage and direction one
-// sweep kernel that computes the face fluxes (inp_const / inp_tvdminmod +
-// hlld_flux.f90) and applies them to the cells (adddfdx.f), then the
-// primitive recovery.  The face fluxes never go to memory: along x a wave
-// covers 64 faces and takes the west face of each cell from the
-// neighbouring lane (DPP); along y and z each thread marches over a segment
-// of cells and carries the previous face (one face per segment is computed
-// twice, by the same operations).  The state accumulates in FP64.
+// engine.f:2-73 as kernels: per stage and direction one sweep kernel that
+// computes the face fluxes (inp_const / inp_tvdminmod + hlld_flux.f90) and
+// applies them to the cells (adddfdx.f).  The face fluxes never go to
+// memory: along x a wave covers 64 faces and takes the west face of each
+// cell from the neighbouring lane (DPP); along y and z each thread marches
+// over a segment of cells with the four stencil cells of its face in a
+// sliding register window and carries the previous face (one face per
+// segment is computed twice, by the same operations).  The first (x) sweep
+// of a stage also sets up the conserved state from the primitives and the
+// last (z) sweep recovers the primitives (the separate hlld_cons/hlld_prim
+// passes remain for runs without an x or z sweep).  The state accumulates
+// in FP64.
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_roe.hpp"
