@@ -488,7 +488,7 @@ int cans2d_mlw_adjoint_step(const double *const *pn6, const double *const *pn1_6
     c.dx = dx; c.dy = dy; c.safety = safety; c.gm = gm; c.ix = ix; c.jx = jx; c.kx = 1; c.dim = 2; c.key = key;
     HIPCHK(launch_cfl(c, s));
     HIPCHK(launch_key_to_dt(key, tail + 1, s));
-    HIPCHK(launch_zero(tail + 2, 2, s));
+    HIPCHK(launch_zero(tail + 2, kAdjAccSlot, s));
     HIPCHK(launch_adj2d_step(pn6, pn1_6, lam_in6, lam_out6, (double *)ws, tail + 1, tail + 2, nullptr, nullptr,
                              nullptr, dx, dxm, dy, dym, mx, my, qav, gm, safety, ix, jx, margin, s));
     HIPCHK(launch_adj2d_dt(pn6, lam_out6, (double *)ws, tail + 1, tail + 2, dx, dy, gm, safety, ix, jx, s));
@@ -559,15 +559,16 @@ int cans2d_mlw_adjoint_run(const double *const *states, const double *dt_log, in
     Metric my{(double *)m, (double *)m + myn, (double *)m + 2 * myn, (double *)m + 3 * myn};
     double *const *cur = lam6;
     double *const *oth = tmp6;
-    double *acc = (double *)ws + 12 * al((size_t)ix * jx) + 4;   // two slots of 2 accumulators
-    HIPCHK(launch_zero(acc, 4, s));
+    double *acc = (double *)ws + 12 * al((size_t)ix * jx) + 4;   // two accumulator slots
+    HIPCHK(launch_zero(acc, 2 * kAdjAccSlot, s));
     // step n counts into acc slot n&1; its B1 applies the dt gradient of step
     // n+1 (slot (n+1)&1), which its B2 then clears for step n-1
     for (int n = nsteps - 1; n >= 0; n--) {
         const bool pend = n < nsteps - 1;
         HIPCHK(launch_adj2d_step(states + 7 * n, states + 7 * (n + 1), cur, oth, (double *)ws, dt_log + n,
-                                 acc + 2 * (n & 1), pend ? dt_log + n + 1 : nullptr,
-                                 pend ? acc + 2 * ((n + 1) & 1) : nullptr, acc + 2 * ((n + 1) & 1), dx, dxm, dy,
+                                 acc + kAdjAccSlot * (n & 1), pend ? dt_log + n + 1 : nullptr,
+                                 pend ? acc + kAdjAccSlot * ((n + 1) & 1) : nullptr,
+                                 acc + kAdjAccSlot * ((n + 1) & 1), dx, dxm, dy,
                                  dym, mx, my, qav, gm, safety, ix, jx, margin, s));
         double *const *t = cur;
         cur = oth;

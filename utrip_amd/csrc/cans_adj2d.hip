@@ -45,10 +45,10 @@ struct AdjArgs {
     double qav, gm, safety;
     int ix, jx, margin;
     const double *dtp;               // the dt of the step
-    double *ldt;                     // dL/ddt accumulator (zeroed)
+    double *ldt;                     // dL/ddt accumulator stripes (zeroed)
     unsigned *nties;                 // cells whose cfl term equals dt (zeroed)
     const double *pend_dt;           // the dt of step n+1, whose dt gradient B1 applies (or NULL)
-    const double *pend_acc;          // its accumulators (ldt, nties)
+    const double *pend_acc;          // its accumulator slot (ldt stripes, nties)
     double *acc_zero;                // accumulators B2 clears for step n-1 (or NULL)
 };
 
@@ -154,6 +154,14 @@ CANS_DEV double step_dt(const AdjArgs &a) { return *a.dtp; }
 
 CANS_DEV bool interior(const AdjArgs &a, int i, int j) { return i >= 2 && i <= a.ix - 1 && j >= 2 && j <= a.jx - 1; }
 
+// the dL/ddt of a slot: its stripes summed in order
+CANS_DEV double acc_ldt(const double *acc) {
+    double s = acc[0];
+    for (int k = 1; k < kAdjStripes; k++) s += acc[k];
+    return s;
+}
+
+// block sum added to one of the kAdjStripes stripes at dst
 template <int BT>
 CANS_DEV void block_sum_add(double v, double *dst) {
     __shared__ double red[BT / 64];
@@ -163,7 +171,7 @@ CANS_DEV void block_sum_add(double v, double *dst) {
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int w = 0; w < BT / 64; w++) s += red[w];
-        atomicAdd(dst, s);
+        atomicAdd(dst + (blockIdx.x % kAdjStripes), s);
     }
 }
 
@@ -252,8 +260,8 @@ __global__ __launch_bounds__(256) void adj_b1(AdjArgs a) {
     // cell, ghost images included): added at the images that tie
     const bool pend = a.pend_dt != nullptr;
     const double pdt = pend ? *a.pend_dt : 0.0;
-    const double pldt = pend ? a.pend_acc[0] : 0.0;
-    const unsigned pnt = pend ? *reinterpret_cast<const unsigned *>(a.pend_acc + 1) : 0u;
+    const double pldt = pend ? acc_ldt(a.pend_acc) : 0.0;
+    const unsigned pnt = pend ? *reinterpret_cast<const unsigned *>(a.pend_acc + kAdjStripes) : 0u;
     P6 l{0, 0, 0, 0, 0, 0};
     for (int b = 0; b < nyj; b++)
         for (int c = 0; c < nxi; c++) {
@@ -280,8 +288,7 @@ __global__ __launch_bounds__(256) void adj_b2(AdjArgs a) {
         const int ci = (int)(t % (ix - 1)) + 1, cj = (int)(t / (ix - 1)) + 1;
         const double gm = a.gm, dt = step_dt(a), hdt = 0.5 * dt;
         if (t == 0 && a.acc_zero) {   // B1 of this step has read them; B1 of step n-1 counts into them
-            a.acc_zero[0] = 0.0;
-            a.acc_zero[1] = 0.0;
+            for (int q = 0; q < kAdjAccSlot; q++) a.acc_zero[q] = 0.0;
         }
         const size_t kA = at(ci, cj, ix), kB = at(ci + 1, cj, ix), kC = at(ci, cj + 1, ix), kD = at(ci + 1, cj + 1, ix);
         const CellV A = cellv(load6(a.p, kA), gm), B = cellv(load6(a.p, kB), gm);
@@ -455,7 +462,7 @@ __global__ __launch_bounds__(256) void adj_dt(AdjArgs a) {
     const double q = cfl_term(a, i, j, p, &S);
     if (q != step_dt(a) || *a.nties == 0u) return;
     P6 l{0, 0, 0, 0, 0, 0};
-    dt_grad(p, q, S, *a.ldt, *a.nties, a.gm, l);
+    dt_grad(p, q, S, acc_ldt(a.ldt), *a.nties, a.gm, l);
     a.lout[0][k] += l.ro; a.lout[1][k] += l.pr; a.lout[2][k] += l.vx;
     a.lout[3][k] += l.vy; a.lout[4][k] += l.bx; a.lout[5][k] += l.by;
 }
@@ -477,7 +484,7 @@ AdjArgs make_adj(const double *const *pn, const double *const *pn1, const double
     }
     a.dtp = dtp;
     a.ldt = acc;
-    a.nties = reinterpret_cast<unsigned *>(acc + 1);
+    a.nties = reinterpret_cast<unsigned *>(acc + kAdjStripes);
     a.dx = dx; a.dxm = dxm; a.dy = dy; a.dym = dym;
     a.mx = mx; a.my = my;
     a.qav = qav; a.gm = gm; a.safety = safety;
@@ -561,6 +568,7 @@ hipError_t launch_copy6(double *const *dst, double *const *src, size_t n, hipStr
 size_t adj2d_ws_doubles(int ix, int jx) {
     const size_t n = (size_t)ix * jx;
     const size_t na = (n + 63) & ~size_t(63);
+    static_assert(4 + 2 * kAdjAccSlot <= 64, "adjoint accumulator tail");
     return 12 * na + 64;
 }
 

@@ -234,6 +234,11 @@ hipError_t launch_key_to_dt(const unsigned long long *key, double *dt, hipStream
 hipError_t launch_zero(double *p, size_t n, hipStream_t s);
 hipError_t launch_periodic_fold(double *const *lam, int ix, int jx, int margin, hipStream_t s);
 size_t adj2d_ws_doubles(int ix, int jx);
+// One accumulator slot of the adjoint step: dL/ddt in kAdjStripes partial
+// sums (blocks add to stripe blockIdx % kAdjStripes, so that same-address
+// atomics do not serialize), then the tie count (an unsigned in a double).
+constexpr int kAdjStripes = 16;
+constexpr int kAdjAccSlot = kAdjStripes + 1;
 hipError_t launch_copy6(double *const *dst, double *const *src, size_t n, hipStream_t s);
 
 hipError_t launch_metric(const double *d, const double *dm, int n, Metric m, hipStream_t s);
