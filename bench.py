@@ -211,6 +211,8 @@ def tubes_figures(ntube: int = 4096, tend: float = 0.1):
                      "steps_min": int(ns.min()), "steps_max": int(ns.max()), "launch_shape": [bt, bpc],
                      "hbm_frac_alg": 112 * zu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
         finals[name] = ([t.clone() for t in b.f], b)
+        if not mixed:
+            ns_fp64 = ns
     a, _ = finals["fp64"]
     c, bb = finals["fp32flux"]
     rel = []
@@ -219,6 +221,32 @@ def tubes_figures(ntube: int = 4096, tend: float = 0.1):
         rel.append(float((c[0][sl] - a[0][sl]).abs().sum() / a[0][sl].abs().sum()))
     rel = np.array(rel)
     out["rho_rel_l1_fp32flux_vs_fp64"] = {"max": float(rel.max()), "median": float(np.median(rel))}
+    # the alternative the batch replaces: the plain single-run engine looped
+    # over the tubes (one resident 1D run per tube, cfl + mlw_m + bdfre per
+    # step), timed on the first `nloop` tubes of the set for the same steps
+    nloop = 16
+    zu = 0
+    ms = 0.0
+    for k in range(nloop):
+        d = tubes[k]
+        st = [torch.from_numpy(np.ascontiguousarray(d[q])).to("cuda:0") for q in cans.FIELDS[1]]
+        gd = {"dx": torch.from_numpy(d["dx"]).to("cuda:0"), "dxm": torch.from_numpy(d["dxm"]).to("cuda:0")}
+        run = cans.Run(1, cans.ENGINE_MLW, st, gd, 2.0, 0.4, qav=3.0, margin=2, bc=0,
+                       bx1d=torch.from_numpy(d["bx"]).to("cuda:0"), bxm1d=torch.from_numpy(d["bxm"]).to("cuda:0"))
+        run.reset(tend=tend)
+        ev = cans.EventSet(2)
+        torch.cuda.synchronize()
+        ev.record(0)
+        run.steps(int(ns_fp64[k]))
+        ev.record(1)
+        torch.cuda.synchronize()
+        _, info = run.state()
+        assert info["merr"] == 0 and info["nexec"] == int(ns_fp64[k]), info
+        zu += int(nx[k]) * int(ns_fp64[k])
+        ms += ev.elapsed_ms(0, 1)
+    out["looped_single_runs"] = {"tubes": nloop, "zone_updates": zu, "ms": ms,
+                                 "zone_updates_per_s": zu / (ms / 1e3),
+                                 "batched_fp64_over_looped": out["fp64"]["zone_updates_per_s"] / (zu / (ms / 1e3))}
     return out
 
 
