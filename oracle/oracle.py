@@ -35,6 +35,10 @@ def build(ref: bool = True) -> None:
     subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
     if ref and REFDIR_SRC.exists():
         subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+        # the reference's shock-tube program, linked with the reference
+        # routines and with the drop-in library (needs it built first)
+        if (HERE.parent / "utrip_amd" / "libcans1d_gpu.so").exists():
+            subprocess.run(["make", "-s", "-C", str(HERE), "drivers"], check=True)
 
 
 def _arr(a) -> C.c_void_p:

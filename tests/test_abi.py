@@ -66,3 +66,24 @@ def test_python_mirror_rejects_bad_arrays():
         cans._ptr(a.float(), (10,), "ro")    # wrong dtype
     with pytest.raises(TypeError):
         cans._ptr([1.0], None, "ro")
+
+
+def test_fortran_dropin_libraries_export_only_cans_names():
+    """The drop-in libraries keep the Fortran runtime they carry local: a
+    CANS main program linked with its own runtime must not bind its I/O unit
+    tables to a second copy."""
+    import subprocess
+    from utrip_amd import PKG_DIR
+    found = False
+    for dim in ("1d", "2d", "3d"):
+        p = PKG_DIR / f"libcans{dim}_gpu.so"
+        if not p.exists():
+            continue
+        found = True
+        out = subprocess.run(["nm", "-D", "--defined-only", str(p)], capture_output=True, text=True, check=True).stdout
+        names = [ln.split()[-1] for ln in out.splitlines() if ln.strip()]
+        bad = [n for n in names if not (n.endswith("_") and n == n.lower() and not n.startswith("_"))]
+        assert not bad, (dim, bad[:10])
+        assert "mlw_m_" in names and "cfl_m_" in names
+    if not found:
+        pytest.skip("Fortran drop-in libraries not built")

@@ -182,3 +182,32 @@ def test_hlld(oracle, refhlld, shape, mf):
                  np.array(mf, dtype=np.int32))
     for k in F3:
         assert_bitwise(inner(a[k], 3), inner(b[k], 3), k)
+
+
+@pytest.mark.parametrize("margin", [1, 2, 3])
+def test_bnd_3d_all_kinds(oracle, ref3d, margin):
+    """Every 3D boundary routine (cans3d/bc/bd*{x,y,z}.f), both sides, against
+    the reference build: x, y, z faces of a 9x8x7 block, kinds fre, spp, spn,
+    smp, smn, cns (q0 = 0.25) and frd (with the face widths), and bdper with
+    unequal margins."""
+    ix, jx, kx = 9, 8, 7
+    _, _, dxm = stretched_grid(ix, 3)
+    _, _, dym = stretched_grid(jx, 5)
+    _, _, dzm = stretched_grid(kx, 7)
+    rng = np.random.default_rng(100 + margin)
+    names = {0: "bdfre", 1: "bdspp", 2: "bdspn", 3: "bdsmp", 4: "bdsmn", 5: "bdcns", 6: "bdfrd"}
+    for d, ax, dm in ((0, "x", dxm), (1, "y", dym), (2, "z", dzm)):
+        for kind, nm in names.items():
+            for mbnd in (0, 1):
+                q = np.asfortranarray(rng.standard_normal((ix, jx, kx)))
+                r = q.copy(order="F")
+                oracle.bnd_3d(d, kind, mbnd, margin, q, 0.25, dm)
+                args = (0.25,) if kind == 5 else ((dm,) if kind == 6 else ())
+                ref3d.call(nm + ax, mbnd, margin, r, *args, ix, jx, kx)
+                assert_bitwise(q, r, f"{nm}{ax} mbnd={mbnd} margin={margin}")
+        q = np.asfortranarray(rng.standard_normal((ix, jx, kx)))
+        r = q.copy(order="F")
+        m0 = max(1, margin - 1)
+        oracle.bnd_3d(d, 7, m0, margin, q)
+        ref3d.call("bdper" + ax, m0, margin, r, ix, jx, kx)
+        assert_bitwise(q, r, f"bdper{ax} margins {m0},{margin}")

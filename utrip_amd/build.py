@@ -10,6 +10,7 @@ cross-compiles for gfx950 without a GPU present.
 from __future__ import annotations
 
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -76,6 +77,22 @@ def build_gpu(jobs: int = 8) -> Path:
     return LIB_GPU
 
 
+def export_map(*sources: Path) -> str:
+    """Linker version script of a drop-in library: only the external
+    subroutines of its Fortran sources (the CANS routine names, lower case
+    with one trailing underscore) are exported; the Fortran runtime the
+    compiler links into the library stays local, so a main program linked
+    with its own runtime sees one copy of it."""
+    names = []
+    for src in sources:
+        for line in src.read_text().splitlines():
+            m = re.match(r"^subroutine\s+(\w+)", line, flags=re.I)
+            if m:
+                names.append(m.group(1).lower() + "_")
+    body = "".join(f"    {n};\n" for n in sorted(set(names)))
+    return "{\n  global:\n" + body + "  local: *;\n};\n"
+
+
 def build_fortran() -> list[Path]:
     """Fortran drop-in layer: one library per dimensionality, like the
     reference's libcans1d.a / libcans2d.a / libcans3d.a."""
@@ -90,8 +107,11 @@ def build_fortran() -> list[Path]:
         if not src.exists():
             continue
         lib = PKG / f"libcans{dim}_gpu.so"
+        vers = objdir / f"cans{dim}_exports.map"
+        vers.write_text(export_map(common, src))
         _run([FLANG, "-O2", "-fPIC", "-shared", "-J", str(objdir), "-o", str(lib),
-              str(common), str(src), f"-L{PKG}", "-lcans_gpu", "-Wl,-rpath,$ORIGIN"], cwd=objdir)
+              str(common), str(src), f"-L{PKG}", "-lcans_gpu", "-Wl,-rpath,$ORIGIN",
+              f"-Wl,--version-script={vers}"], cwd=objdir)
         out.append(lib)
         main = FSRC / f"cans{dim}_main.f90"
         if main.exists():
