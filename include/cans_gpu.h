@@ -53,6 +53,18 @@ int cans_gpu_set_march_waves(int waves);
 int cans1d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *by,
                  const double *bx, const double *bxm, double dt, double qav, double gm,
                  const double *dx, const double *dxm, int ix, void *stream);
+/* cans1d_mlw_m without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as cans1d_mlw_m; its workspace is a static device
+ * array of 9*CANS_NOALLOC_MAX_IX doubles (4.5 MiB per GPU, held until the
+ * process ends), so ix > CANS_NOALLOC_MAX_IX returns CANS_ERR_ARG.  All calls
+ * on one GPU share that array: calls that overlap in time on one GPU (other
+ * streams, other threads) race. */
+#define CANS_NOALLOC_MAX_IX 65536
+int cans1d_mlw_m_noalloc(double *ro, double *pr, double *vx, double *vy, double *by,
+                         const double *bx, const double *bxm, double dt, double qav, double gm,
+                         const double *dx, const double *dxm, int ix, void *stream);
 /* cans1d/hdroe/roe_m.f:2 roe_m(ro,pr,vx,vy,by,bx,bxm,dt,gm,dx,ix)
  * (fuses roeflux_m.f, tvdminmod.f); updated in place on i=3..ix-2. */
 int cans1d_roe_m(double *ro, double *pr, double *vx, double *vy, double *by,

@@ -98,6 +98,22 @@ def mlw_m_1d(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm):
                               _stream()))
 
 
+NOALLOC_MAX_IX = 65536   # CANS_NOALLOC_MAX_IX of include/cans_gpu.h
+
+
+def mlw_m_1d_noalloc(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm):
+    """cans1d/hdmlw/mlw_m.f:2 as mlw_m_1d, through the entry point that
+    allocates no device memory (workspace: a static device array, so
+    ix <= NOALLOC_MAX_IX; calls that overlap on one GPU race)."""
+    ix = ro.shape[0]
+    sh = (ix,)
+    _check(lib().cans1d_mlw_m_noalloc(*[_ptr(t, sh, n) for t, n in
+                                        zip((ro, pr, vx, vy, by, bx, bxm),
+                                            ("ro", "pr", "vx", "vy", "by", "bx", "bxm"))],
+                                      _d(dt), _d(qav), _d(gm), _ptr(dx, sh, "dx"), _ptr(dxm, sh, "dxm"),
+                                      C.c_int(ix), _stream()))
+
+
 def roe_m_1d(ro, pr, vx, vy, by, bx, bxm, dt, gm, dx):
     """cans1d/hdroe/roe_m.f:2  (in place on i=3..ix-2)."""
     ix = ro.shape[0]

@@ -57,7 +57,7 @@ struct Pool {
 };
 std::map<int, Pool> g_pool;
 
-int scratch(size_t bytes, void **out) {
+int pool_scratch(size_t bytes, void **out) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_mu);
@@ -80,7 +80,7 @@ inline size_t al(size_t n) { return (n + 63) & ~size_t(63); }  // doubles, 512-b
 
 }  // namespace
 
-int cans::abi_scratch(size_t bytes, void **out) { return scratch(bytes, out); }
+int cans::abi_scratch(size_t bytes, void **out) { return pool_scratch(bytes, out); }
 
 namespace {
 std::atomic<int> g_march_waves{32768};
@@ -131,16 +131,12 @@ int cans_gpu_set_march_waves(int waves) {
 }
 
 // ====================================================================== 1D
-int cans1d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *by,
-                 const double *bx, const double *bxm, double dt, double qav, double gm,
-                 const double *dx, const double *dxm, int ix, void *stream) {
-    if (ix < 3) return fail(CANS_ERR_ARG, "cans1d_mlw_m: ix=%d < 3", ix);
-    hipStream_t s = (hipStream_t)stream;
+// One mlw_m 1D call on the workspace ws of 9*al(ix) doubles: the five
+// updated fields, then 1/dx, 1/dxm, ux0, ux1.
+static int mlw1d_on_ws(double *ro, double *pr, double *vx, double *vy, double *by, const double *bx,
+                       const double *bxm, double dt, double qav, double gm, const double *dx, const double *dxm,
+                       int ix, double *ws, hipStream_t s) {
     const size_t n = al(ix);
-    void *w = nullptr;
-    int rc = scratch((5 * n + 4 * n) * sizeof(double), &w);
-    if (rc) return rc;
-    double *ws = (double *)w;
     Metric mx{ws + 5 * n, ws + 6 * n, ws + 7 * n, ws + 8 * n};
     HIPCHK(launch_metric(dx, dxm, ix, mx, s));
     Mlw1dArgs a{};
@@ -155,9 +151,64 @@ int cans1d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *by,
     return 0;
 }
 
+int cans1d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *by,
+                 const double *bx, const double *bxm, double dt, double qav, double gm,
+                 const double *dx, const double *dxm, int ix, void *stream) {
+    if (ix < 3) return fail(CANS_ERR_ARG, "cans1d_mlw_m: ix=%d < 3", ix);
+    void *w = nullptr;
+    int rc = pool_scratch(9 * al(ix) * sizeof(double), &w);
+    if (rc) return rc;
+    return mlw1d_on_ws(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm, ix, (double *)w, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// Workspace of cans1d_mlw_m_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never passes `scratch` itself
+// (on the host it names a stand-in, not device memory): it passes the address
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device and kept in g_noalloc_ptr.
+static __device__ __align__(256) double scratch[9 * CANS_NOALLOC_MAX_IX];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double *g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(double **out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(CANS_ERR_ARG, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CANS_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL on device %d", dev);
+        g_noalloc_ptr[dev] = (double *)p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int cans1d_mlw_m_noalloc(double *ro, double *pr, double *vx, double *vy, double *by, const double *bx,
+                         const double *bxm, double dt, double qav, double gm, const double *dx, const double *dxm,
+                         int ix, void *stream) {
+    if (ix < 3) return fail(CANS_ERR_ARG, "cans1d_mlw_m_noalloc: ix=%d < 3", ix);
+    if (ix > CANS_NOALLOC_MAX_IX)
+        return fail(CANS_ERR_ARG, "cans1d_mlw_m_noalloc: ix=%d exceeds the static workspace (ix <= %d)", ix,
+                    CANS_NOALLOC_MAX_IX);
+    double *ws = nullptr;
+    int rc = noalloc_scratch(&ws);
+    if (rc) return rc;
+    return mlw1d_on_ws(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm, ix, ws, (hipStream_t)stream);
+}
+
 static int cfl_common(CflArgs &a, double *dt, double dtmin, int *merr, hipStream_t s) {
     void *w = nullptr;
-    int rc = scratch(64, &w);
+    int rc = pool_scratch(64, &w);
     if (rc) return rc;
     unsigned long long *key = (unsigned long long *)w;
     int *loc = (int *)((char *)w + 8);
@@ -238,7 +289,7 @@ int cans2d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *bx, dou
     hipStream_t s = (hipStream_t)stream;
     const size_t n = al((size_t)ix * jx), mxn = al(ix), myn = al(jx);
     void *w = nullptr;
-    int rc = scratch((7 * n + 4 * mxn + 4 * myn) * sizeof(double), &w);
+    int rc = pool_scratch((7 * n + 4 * mxn + 4 * myn) * sizeof(double), &w);
     if (rc) return rc;
     double *ws = (double *)w;
     double *o[7];
@@ -319,7 +370,7 @@ int cans3d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm
     if (!x || !y) return fail(CANS_ERR_ARG, "cans3d_cfl_m_s: NULL x or y");
     hipStream_t s = (hipStream_t)stream;
     double *tab = nullptr;
-    if (int rc = scratch(64 + 6 * (size_t)jx * sizeof(double), (void **)&tab)) return rc;
+    if (int rc = pool_scratch(64 + 6 * (size_t)jx * sizeof(double), (void **)&tab)) return rc;
     tab += 8;                                   // the first 64 bytes hold cfl_common's key
     if (int rc = cans::host_trig(y, nullptr, jx, tab, s)) return rc;
     CflArgs a{};
