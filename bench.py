@@ -39,10 +39,14 @@ def parse():
     p.add_argument("--steps", type=int, default=2000)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--n", type=int, default=1024, help="interior cells per side")
-    p.add_argument("--cpu-steps", type=int, default=40, help="reference steps timed on the host")
+    p.add_argument("--cpu-steps", type=int, default=100, help="reference steps timed on the host (one core)")
+    p.add_argument("--cpu-procs", type=int, default=0,
+                   help="independent reference processes of the all-cores figure (0: the host's usable cores, at most 16)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true", help="only the main workload (no 4096^2 line)")
     p.add_argument("--c2-steps", type=int, default=10, help="timed steps of the 3D blast figures (0: skip)")
+    p.add_argument("--c2-l1-steps", type=int, default=500,
+                   help="steps after which the FP32-flux vs FP64 L1 per field is taken")
     p.add_argument("--c2-n", type=int, default=384, help="interior cells per side of the 3D blast")
     p.add_argument("--tubes", type=int, default=4096, help="batched 1D tubes to t=0.1 (0: skip)")
     p.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint timing")
@@ -82,53 +86,104 @@ def traffic_from_profile(name: str):
     return None
 
 
-def cpu_baseline(pb, nsteps: int):
-    """The reference Fortran (oracle/_ref/libcans2d_ref.so, built from the
-    reference sources with amdflang -O2 -ffp-contract=off) on one host core:
-    cfl_m + mlw_m + bdperx/bdpery per step, 1 warm-up step, median step time."""
-    import ctypes as C
+def host_info():
+    """Core count and model of the host the CPU legs run on."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "cpu_model": model}
 
-    from oracle.oracle import REF_LIBS, Ref
-    if not REF_LIBS["2d"].exists():
-        return None
-    R = Ref("2d")
-    f = {k: v.copy(order="F") for k, v in pb.fields.items()}
-    g, pp = pb.grid, pb.params
+
+def _ot_ref_setup(n: int):
+    from oracle.oracle import Ref
+    from utrip_amd import problems
+    pb = problems.orszag_tang_2d(n=n, margin=1)
+    return Ref("2d"), {k: v.copy(order="F") for k, v in pb.fields.items()}, pb.grid, pb.params
+
+
+def _ot_ref_step(R, f, g, pp):
+    """cfl_m + mlw_m + bdperx/bdpery of the reference build (cans2d main.f order)."""
+    import ctypes as C
     ix, jx = f["ro"].shape
     names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+    dt = C.c_double()
+    merr = C.c_int()
+    R.call("cfl_m", dt, pp["safety"], pp["dtmin"], merr, pp["gm"], *[f[k] for k in names[:6]], g["dx"], ix,
+           g["dy"], jx)
+    R.call("mlw_m", *[f[k] for k in names], dt.value, pp["qav"], pp["gm"], g["dx"], g["dxm"], ix, g["dy"],
+           g["dym"], jx)
+    for k in names:
+        R.call("bdperx", 1, 1, f[k], ix, jx)
+    for k in names:
+        R.call("bdpery", 1, 1, f[k], ix, jx)
 
-    def step():
-        dt = C.c_double()
-        merr = C.c_int()
-        R.call("cfl_m", dt, pp["safety"], pp["dtmin"], merr, pp["gm"], *[f[k] for k in names[:6]], g["dx"], ix,
-               g["dy"], jx)
-        R.call("mlw_m", *[f[k] for k in names], dt.value, pp["qav"], pp["gm"], g["dx"], g["dxm"], ix, g["dy"],
-               g["dym"], jx)
-        for k in names:
-            R.call("bdperx", 1, 1, f[k], ix, jx)
-        for k in names:
-            R.call("bdpery", 1, 1, f[k], ix, jx)
 
-    step()  # warm-up
-    times = []
+def _ot_ref_worker(n: int, nsteps: int):
+    """One process of the all-cores figure: its own copy of the n^2 problem,
+    1 warm-up step, then nsteps timed steps; returns the step times."""
+    R, f, g, pp = _ot_ref_setup(n)
+    _ot_ref_step(R, f, g, pp)
+    out = []
     for _ in range(nsteps):
         t0 = time.perf_counter()
-        step()
-        times.append(time.perf_counter() - t0)
+        _ot_ref_step(R, f, g, pp)
+        out.append(time.perf_counter() - t0)
+    return out
+
+
+def cpu_baseline(n: int, nsteps: int, procs: int):
+    """The reference Fortran (oracle/_ref/libcans2d_ref.so, built from the
+    reference sources with amdflang -O2 -ffp-contract=off) on the host:
+      one core -- cfl_m + mlw_m + bdperx/bdpery per step, 1 warm-up step,
+                  median of nsteps step times;
+      all cores -- `procs` independent processes, each running its own copy
+                  of the same problem (the reference has no threads; this is
+                  the host's whole-chip throughput on independent runs),
+                  10 timed steps each, the sum of their rates."""
+    from oracle.oracle import REF_LIBS
+    if not REF_LIBS["2d"].exists():
+        return None
+    times = _ot_ref_worker(n, nsteps)
     med = statistics.median(times)
-    zones = (ix - 2) * (jx - 2)
-    return {"value": zones / med, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
-            "sample": f"{nsteps} timed steps (after 1 warm-up) of the same {ix - 2}^2 Orszag-Tang MLW loop, "
-                      f"median step {med:.3f} s; reference cfl_m/mlw_m/bdperx/bdpery compiled from source "
-                      f"with amdflang -O2 -ffp-contract=off, single thread"}
+    zones = n * n
+    info = host_info()
+    out = {"value": zones / med, "unit": "zone-updates/s", "cores": 1, "kind": "reference",
+           "sample": f"{nsteps} timed steps (after 1 warm-up) of the same {n}^2 Orszag-Tang MLW loop, "
+                     f"median step {med:.3f} s; reference cfl_m/mlw_m/bdperx/bdpery compiled from source "
+                     f"with amdflang -O2 -ffp-contract=off, single thread", **info}
+    procs = procs or min(16, info["usable_cores"] or 1)
+    if procs > 1:
+        import multiprocessing as mp
+        ctx = mp.get_context("spawn")
+        t0 = time.perf_counter()
+        with ctx.Pool(procs) as pool:
+            res = pool.starmap(_ot_ref_worker, [(n, 10)] * procs)
+        wall = time.perf_counter() - t0
+        rate = sum(zones / statistics.median(r) for r in res)
+        out["all_cores"] = {"value": rate, "unit": "zone-updates/s", "processes": procs,
+                            "median_step_s": statistics.median([statistics.median(r) for r in res]),
+                            "wall_s": wall,
+                            "sample": f"{procs} independent processes, each its own {n}^2 problem, 10 timed steps "
+                                      f"after 1 warm-up; sum of the per-process rates"}
+    return out
 
 
-def blast3d_figures(n: int, steps: int):
+def blast3d_figures(n: int, steps: int, l1_steps: int = 500):
     """The 384^3 periodic MHD blast wave (rho=1, p=0.1, p=10 inside r<0.1,
     B_HL=(1,1,0)/sqrt(2)) on the resident run: 3D MLW (FP64), Roe-MUSCL and
-    HLLD in FP64 and with FP32 face fluxes; zone-updates/s of each and the
-    relative L1 difference per field between the FP32-flux and FP64 runs
-    after the same number of steps."""
+    HLLD in FP64 and with FP32 face fluxes.  Each engine is timed over
+    `steps` steps after 2 warm-up steps; the Roe and HLLD runs then continue
+    to l1_steps steps (the workload's run length), where the relative L1
+    difference per field between the FP32-flux and FP64 states is taken."""
     import gc
 
     import numpy as np
@@ -137,7 +192,7 @@ def blast3d_figures(n: int, steps: int):
     from utrip_amd import cans, problems
     dev = torch.device("cuda:0")
     F3 = cans.FIELDS[3]
-    out = {"cells": n ** 3, "steps": steps, "warmup": 2}
+    out = {"cells": n ** 3, "steps": steps, "warmup": 2, "l1_after_steps": l1_steps}
     finals = {}
     for engine, mixed in (("mlw", False), ("roe", False), ("roe", True), ("hlld", False), ("hlld", True)):
         margin = 2 if engine == "mlw" else 3
@@ -157,12 +212,18 @@ def blast3d_figures(n: int, steps: int):
         ev.record(1)
         torch.cuda.synchronize()
         ms = ev.elapsed_ms(0, 1)
-        cur, info = run.state()
-        assert info["merr"] == 0 and info["nexec"] == steps + 2, info
         name = engine + ("_fp32flux" if mixed else "_fp64")
         out[name] = {"zone_updates_per_s": n ** 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
                      "hbm_frac_alg": 128 * n ** 3 * steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        total = steps + 2
+        if engine != "mlw" and l1_steps > total:
+            run.steps(l1_steps - total)
+            total = l1_steps
+        cur, info = run.state()
+        assert info["merr"] == 0 and info["nexec"] == total, info
+        print(f"bench: blast {name} done", file=sys.stderr, flush=True)
         if engine != "mlw":
+            out[name]["t_at_l1"] = info["t"]
             finals[name] = [t.clone() for t in cur]   # kept on the GPU (3.7 GB per run at 384^3)
         del run, st, gd, cur
         gc.collect()
@@ -172,6 +233,76 @@ def blast3d_figures(n: int, steps: int):
         out[engine + "_rel_l1_fp32flux_vs_fp64"] = {
             k: float((x - y).abs().sum() / y.abs().sum()) if float(y.abs().sum()) > 0 else float((x - y).abs().sum())
             for k, x, y in zip(F3, b, a)}
+        out[engine + "_fp32flux_speedup"] = out[engine + "_fp64"]["ms_per_step"] / out[engine + "_fp32flux"]["ms_per_step"]
+        del a, b
+    finals.clear()
+    gc.collect()
+    torch.cuda.empty_cache()
+    return out
+
+
+def c0_figures(no_cpu: bool = False, reps: int = 5):
+    """The c0 workload: one Brio-Wu shock tube (nx = 1024, margin 2, free
+    boundaries, gamma 2, safety 0.4, qav 3) run to t = 0.1 on the device
+    clock (cfl_m, mlw_m, bdfrex launches per step, no host round trip);
+    median of `reps` whole runs after a warm-up run.  The CPU leg runs the
+    same loop over the reference routines built from source (oracle/_ref,
+    libcans1d_ref.so) from C (oracle's orref_loop_1d), one core, median of
+    `reps` whole runs."""
+    import numpy as np
+    import torch
+
+    from utrip_amd import cans, problems
+    pb = problems.shock_tube_1d(nx=1024, margin=2)
+    g, pp = pb.grid, pb.params
+
+    def td(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+    gd = {k: td(g[k]) for k in ("dx", "dxm")}
+    ms = []
+    issue = 2000                            # warm-up run: the clock stops it at t >= 0.1
+    for rep in range(reps + 1):
+        st = [td(pb.fields[k]) for k in cans.FIELDS[1]]
+        run = cans.Run(1, cans.ENGINE_MLW, st, gd, pp["gm"], pp["safety"], qav=pp["qav"], margin=2, bc=0,
+                       bx1d=td(pb.fields["bx"]), bxm1d=td(pb.fields["bxm"]))
+        run.reset(tend=pp["tend"], dtmin=pp["dtmin"])
+        ev = cans.EventSet(2)
+        torch.cuda.synchronize()
+        ev.record(0)
+        run.steps(issue)
+        ev.record(1)
+        torch.cuda.synchronize()
+        _, info = run.state()
+        assert info["merr"] == 0 and info["t"] >= pp["tend"], info
+        if rep:
+            assert info["nexec"] == issue, info
+            ms.append(ev.elapsed_ms(0, 1))
+        issue = info["nexec"]               # timed runs issue exactly the steps the run takes
+    nsteps = info["nexec"]
+    med = statistics.median(ms)
+    out = {"cells": 1024, "steps": nsteps, "t": info["t"], "ms": med, "us_per_step": med * 1e3 / nsteps,
+           "zone_updates_per_s": 1024 * nsteps / (med / 1e3)}
+    if not no_cpu:
+        from oracle.oracle import REF_LIBS, Oracle, Ref
+        if REF_LIBS["1d"].exists():
+            o, R = Oracle(), Ref("1d")
+            secs = []
+            for rep in range(reps + 1):
+                f = {k: v.copy() for k, v in pb.fields.items()}
+                t0 = time.perf_counter()
+                t, ns, merr = o.ref_loop_1d(R, f, f["bx"], f["bxm"], g["dx"], g["dxm"], 2, pp["gm"], pp["safety"],
+                                            pp["qav"], pp["dtmin"], pp["tend"], 10 ** 6)
+                if rep:
+                    secs.append(time.perf_counter() - t0)
+            assert merr == 0 and ns == nsteps, (ns, nsteps)
+            sec = statistics.median(secs)
+            out["cpu_baseline"] = {"value": 1024 * ns / sec, "unit": "zone-updates/s", "cores": 1,
+                                   "kind": "reference", "s": sec,
+                                   "sample": f"the whole run ({ns} steps to t=0.1), median of {reps} runs; "
+                                             "reference cfl_m/mlw_m/bdfrex (amdflang -O2 -ffp-contract=off) "
+                                             "called from a C loop"}
+            out["gpu_vs_cpu"] = out["zone_updates_per_s"] / out["cpu_baseline"]["value"]
     return out
 
 
@@ -213,14 +344,32 @@ def tubes_figures(ntube: int = 4096, tend: float = 0.1):
         finals[name] = ([t.clone() for t in b.f], b)
         if not mixed:
             ns_fp64 = ns
+            ns_fp64_dev = b.ns.clone()
     a, _ = finals["fp64"]
     c, bb = finals["fp32flux"]
-    rel = []
-    for k in range(ntube):
-        sl = slice(int(bb.off_host[k]), int(bb.off_host[k] + bb.ix_host[k]))
-        rel.append(float((c[0][sl] - a[0][sl]).abs().sum() / a[0][sl].abs().sum()))
-    rel = np.array(rel)
-    out["rho_rel_l1_fp32flux_vs_fp64"] = {"max": float(rel.max()), "median": float(np.median(rel))}
+
+    def rel_rho(f):
+        r = []
+        for k in range(ntube):
+            sl = slice(int(bb.off_host[k]), int(bb.off_host[k] + bb.ix_host[k]))
+            r.append(float((f[0][sl] - a[0][sl]).abs().sum() / a[0][sl].abs().sum()))
+        return np.array(r)
+
+    rel = rel_rho(c)
+    ns32 = bb.ns.cpu().numpy()
+    same = ns32 == ns_fp64
+    # the FP32-flux run again from the initial state, each tube for exactly
+    # the steps its FP64 run took
+    for t_, i0 in zip(bb.f, init):
+        t_.copy_(i0)
+    bb.run_steps(ns_fp64_dev)
+    torch.cuda.synchronize()
+    rel_same = rel_rho(bb.f)
+    out["rho_rel_l1_fp32flux_vs_fp64"] = {
+        "same_steps": {"max": float(rel_same.max()), "median": float(np.median(rel_same))},
+        "to_tend": {"max": float(rel.max()), "median": float(np.median(rel)),
+                    "tubes_other_step_count": int((~same).sum()),
+                    "max_over_tubes_same_step_count": float(rel[same].max())}}
     # the alternative the batch replaces: the plain single-run engine looped
     # over the tubes (one resident 1D run per tube, cfl + mlw_m + bdfre per
     # step), timed on the first `nloop` tubes of the set for the same steps
@@ -323,8 +472,28 @@ def roe2d_figures(n: int = 1024, steps: int = 100):
 
 
 def spicule_cpu(steps: int):
-    """CPU leg of the spicule figures: the C restatement's main.f loop
-    (oracle/liboracle.so, gcc -O2, one core) for `steps` steps from t = 0."""
+    """CPU leg of the spicule figures: the reference's own UTRIP program
+    (cans1d/md_spicule/hdmlw main.f with its model, bnd, bndsc, trqalfven and
+    the reference routines, built from source by oracle/Makefile into
+    oracle/_ref/spicule_ref), its whole 100000-step run in a scratch folder,
+    one core.  Where that program is not built, the C restatement's loop
+    (oracle/liboracle.so) for `steps` steps instead."""
+    import subprocess
+    import tempfile
+
+    exe = ROOT / "oracle" / "_ref" / "spicule_ref"
+    if exe.exists():
+        with tempfile.TemporaryDirectory() as d:
+            t0 = time.perf_counter()
+            p = subprocess.run([str(exe)], cwd=d, capture_output=True, text=True, timeout=300)
+            sec = time.perf_counter() - t0
+        assert p.returncode == 0 and "normal stop" in p.stdout, p.stdout[-500:]
+        ns = int(p.stdout.split("stop ")[1].split("step=")[1].split()[0])
+        return {"steps": ns, "s": sec, "steps_per_s": ns / sec, "zone_updates_per_s": ns * 1024 / sec,
+                "cores": 1, "kind": "reference",
+                "sample": f"the reference md_spicule/hdmlw program built from source (amdflang -O2 "
+                          f"-ffp-contract=off), its whole run: {ns} steps of 1024 cells, wall time incl. its DAC "
+                          f"output"}
     import numpy as np
 
     from oracle.oracle import Oracle
@@ -435,11 +604,14 @@ def adjoint_figures(n: int = 512, nsteps: int = 200, ckpt: int = 20):
     run.reset()
     run.steps(nsteps)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run.reset()
-    run.steps(nsteps)
-    torch.cuda.synchronize()
-    plain = time.perf_counter() - t0
+    plains = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run.reset()
+        run.steps(nsteps)
+        torch.cuda.synchronize()
+        plains.append(time.perf_counter() - t0)
+    plain = statistics.median(plains)
     fwd, bwd = [], []
     for _ in range(3):
         for t in (vx0, vy0, beta):
@@ -456,9 +628,14 @@ def adjoint_figures(n: int = 512, nsteps: int = 200, ckpt: int = 20):
         bwd.append(t2 - t1)
     f, b = statistics.median(fwd), statistics.median(bwd)
     return {"cells": n * n, "steps": nsteps, "checkpoint_every": ckpt, "forward_plain_ms": plain * 1e3,
-            "forward_ms": f * 1e3, "backward_ms": b * 1e3, "adj_over_fwd": b / f,
-            "adj_over_plain_fwd": b / plain, "loss": float(L.detach()),
-            "dL_dbeta": float(beta.grad)}
+            "forward_ckpt_ms": f * 1e3, "backward_ms": b * 1e3, "adj_over_fwd": b / plain,
+            "adj_over_ckpt_fwd": b / f, "loss": float(L.detach()), "dL_dbeta": float(beta.grad),
+            "note": "adj_over_fwd = backward / plain forward (no checkpoints) of the same 200 steps"}
+
+
+def progress(what: str) -> None:
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print(f"bench: {what} ({time.strftime('%H:%M:%S')})", file=sys.stderr, flush=True)
 
 
 def main():
@@ -478,10 +655,13 @@ def main():
     dev = torch.device("cuda:0")
 
     def ot_run(n: int, K: int, warmup: int):
-        """K timed steps of the OT n^2 periodic MLW loop after `warmup`
-        steps.  Every EVERY-th step of the timed region has its engine launch
-        bracketed by a pair of HIP events on the launch stream (the average
-        launch duration of the roofline); the other steps run back to back."""
+        """K timed steps of the OT n^2 periodic MLW loop after `warmup` steps,
+        bracketed by two HIP events on the launch stream.  In this periodic
+        path a step is exactly one kernel launch (mlw2d_kernel: the engine
+        with the ghost fill and the next step's cfl_m fused in), so the timed
+        region / K is the average launch duration of that kernel including
+        the gaps between back-to-back launches (never shorter than the kernel,
+        never longer than the step)."""
         pb = problems.orszag_tang_2d(n=n, margin=1)
         names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
         g, pp = pb.grid, pb.params
@@ -495,57 +675,59 @@ def main():
         run.reset(dtmin=pp["dtmin"])
         mass0 = float(st[0][1:-1, 1:-1].sum())
         run.steps(warmup)
+        ev = cans.EventSet(2)
         torch.cuda.synchronize()
-        EVERY = 8
-        sampled = list(range(0, K, EVERY))
-        M = len(sampled)
-        ev = cans.EventSet(2 * M + 2)
+        ev.record(0)
+        run.steps(K)
+        ev.record(1)
         torch.cuda.synchronize()
-        ev.record(2 * M)               # start of the timed region
-        for m, s0 in enumerate(sampled):
-            run.steps(1, events=ev.at(2 * m))
-            rest = min(EVERY, K - s0) - 1
-            if rest > 0:
-                run.steps(rest)
-        ev.record(2 * M + 1)           # end of the timed region
-        torch.cuda.synchronize()
-        total_ms = ev.elapsed_ms(2 * M, 2 * M + 1)
-        kern_ms = [ev.elapsed_ms(2 * m, 2 * m + 1) for m in range(M)]
+        total_ms = ev.elapsed_ms(0, 1)
         cur, info = run.state()
         assert info["merr"] == 0 and info["nexec"] == warmup + K, info
         mass1 = float(cur[0][1:-1, 1:-1].sum())
-        return dict(pb=pb, total_ms=total_ms, avg_kern_s=statistics.fmean(kern_ms) / 1e3, info=info,
+        return dict(pb=pb, total_ms=total_ms, avg_kern_s=total_ms / K / 1e3, info=info,
                     drift=(mass1 - mass0) / mass0)
 
     n = args.n
     K = args.steps
     r = ot_run(n, K, args.warmup)
+    progress(f"OT {n}^2: {K} steps")
     pb, pp, total_ms, avg_kern_s, info = r["pb"], r["pb"].params, r["total_ms"], r["avg_kern_s"], r["info"]
     adj = None
     if args.adjoint:
         adj = adjoint_figures()
+        progress("adjoint")
     m3 = roe2 = None
     if not args.no_extra:
         m3 = m3_figures()
         roe2 = roe2d_figures()
+        progress("mlw_m3 and 2D Roe")
     spc = None
     if args.spicule_runs > 0 or args.spicule_steps > 0:
         spc = spicule_figures(args.spicule_runs, args.spicule_steps, no_cpu=args.no_cpu)
+        progress("spicule")
     tb = None
     if args.tubes > 0:
         tb = tubes_figures(args.tubes)
+        progress("tubes")
     c2 = None
     if args.c2_steps > 0:
-        c2 = blast3d_figures(args.c2_n, args.c2_steps)
+        c2 = blast3d_figures(args.c2_n, args.c2_steps, args.c2_l1_steps)
+        progress("3D blast")
     big = None
     if not args.no_extra:
         # the same loop at 4096^2: 0.94 GB of state per buffer set, beyond
         # the 256 MB Infinity Cache that holds the 1024^2 state
         rb = ot_run(4096, 100, 5)
+        progress("OT 4096^2")
         zb = 4096 * 4096
+        ach = B_ALG_2D * zb / rb["avg_kern_s"] / 1e9
         big = {"cells": zb, "steps": 100, "zone_updates_per_s": zb * 100 / (rb["total_ms"] / 1e3),
                "ms_per_step": rb["total_ms"] / 100, "avg_launch_us": rb["avg_kern_s"] * 1e6,
-               "hbm_frac": B_ALG_2D * zb / rb["avg_kern_s"] / 1e9 / HBM_PEAK_GBS}
+               "hbm_frac": ach / HBM_PEAK_GBS,
+               "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": ach / HBM_PEAK_GBS, "kernel": "mlw2d_kernel",
+                            "traffic": traffic_from_profile("mlw2d_kernel_ot4096_pmc.txt")}}
     zones = n * n
     value = zones * K / (total_ms / 1e3)
     achieved = B_ALG_2D * zones / avg_kern_s / 1e9
@@ -553,7 +735,12 @@ def main():
 
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
-        cpu = cpu_baseline(pb, args.cpu_steps)
+        cpu = cpu_baseline(n, args.cpu_steps, args.cpu_procs)
+        progress("CPU baseline")
+    c0 = None
+    if not args.no_extra:
+        c0 = c0_figures(no_cpu=args.no_cpu)
+        progress("c0 shock tube")
 
     out = {
         "metric": METRIC,
@@ -578,6 +765,8 @@ def main():
                      "avg_launch_us": avg_kern_s * 1e6},
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "gpu_vs_cpu_all_cores": (value / cpu["all_cores"]["value"]) if cpu and "all_cores" in cpu else None,
+        "c0_shock_tube": c0,
         "check": {"t": info["t"], "dt_last": info["dt"], "mass_rel_drift": r["drift"]},
         "ot4096_fp64": big,
         "blast384": c2,

@@ -317,8 +317,9 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
  * order (NULL: 0..ntube-1) is the queue order of the persistent kernel --
  * longest tubes first balances the GPU.  The state ro..by is updated in
  * place.  mixed = 0 is bit-identical to the reference loop per tube;
- * mixed = 1 evaluates cell fluxes, half step and face fluxes in FP32 and
- * accumulates in FP64.  block_threads in {64,128,256,512}, blocks_per_cu
+ * mixed = 1 evaluates the cell and face fluxes in FP32 and keeps the
+ * half-step state with its pressure recovery, the accumulation, the update
+ * and cfl_m in FP64.  block_threads in {64,128,256,512}, blocks_per_cu
  * 1..32 (block_threads*blocks_per_cu <= 2048).  ws: cans1d_batch_ws_bytes
  * (tune = 1 adds room for cans1d_batch_tune's copy of the state).
  * No single counterpart in the reference, whose 1D driver runs one tube. */
@@ -328,6 +329,17 @@ int cans1d_batch_run(int ntube, const long long *off, const int *ix, const int *
                      const double *dxm, long long ncells, double gm, double qav, double safety, double dtmin,
                      double tend, int nstop, int margin, int mixed, double *t_out, int *ns_out, int *merr_out,
                      int block_threads, int blocks_per_cu, void *ws, void *stream);
+/* cans1d_batch_run with a step limit per tube: tube k runs while
+ * ns < nstop_tube[k] and t < tend (nstop_tube: device array of ntube
+ * entries).  With tend = +inf every tube takes exactly its own number of
+ * steps -- e.g. the counts an FP64 run to tend took, so that a mixed run
+ * can be compared with it at the same steps. */
+int cans1d_batch_run_steps(int ntube, const long long *off, const int *ix, const int *order, double *ro,
+                           double *pr, double *vx, double *vy, double *by, const double *bx, const double *bxm,
+                           const double *dx, const double *dxm, long long ncells, double gm, double qav,
+                           double safety, double dtmin, double tend, const int *nstop_tube, int margin, int mixed,
+                           double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
+                           void *ws, void *stream);
 /* Autotuner of the persistent kernel's launch shape: times every
  * (block_threads, blocks_per_cu) on a copy of the state for probe_steps
  * steps per tube and returns the fastest (cached per batch shape for the
@@ -405,13 +417,23 @@ int cans_event_elapsed_ms(void *start, void *stop, float *ms);
 /* ---------------------------------------------------------------- host arrays
  * Device mirrors of host arrays, used by the Fortran drop-in wrappers.  In the
  * default (strict) mode every wrapper call copies its arrays in and out; in
- * resident mode (cans_gpu_set_resident(1)) arrays stay on the device and
- * cans_gpu_sync_host() copies them all back (call it before output). */
+ * resident mode (cans_gpu_set_resident(1)) arrays stay on the device,
+ * cans_gpu_unmap(copy_out = 1) marks a mirror changed, and
+ * cans_gpu_sync_host() copies back the changed mirrors only (call it before
+ * output; leaving resident mode does it too).  A program that changes a
+ * mapped array on the host calls cans_gpu_invalidate on it; one that
+ * releases a mapped array calls cans_gpu_forget on it first, so that a later
+ * sync never writes to the released memory and a new array at the same
+ * address is copied in afresh.
+ * Workspace of the device-array entry points: one pool per (device, stream),
+ * grown to the largest call and kept until cans_gpu_release_all; calls on
+ * different streams never share one. */
 void *cans_gpu_map(const void *host, size_t bytes, int copy_in);
 int cans_gpu_unmap(void *host, int copy_out);
 int cans_gpu_set_resident(int on);
 int cans_gpu_sync_host(void);
 int cans_gpu_invalidate(const void *host);
+int cans_gpu_forget(const void *host);
 int cans_gpu_release_all(void);
 
 #ifdef __cplusplus

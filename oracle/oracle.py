@@ -200,6 +200,20 @@ class Oracle:
     def spicule_bnd(self, margin, ro, pr, vx, vy, by):
         self.lib.orsp_bnd(C.c_int(margin), *map(_arr, (ro, pr, vx, vy, by)), C.c_int(ro.shape[0]))
 
+    def ref_loop_1d(self, ref, st, bx, bxm, dx, dxm, margin, gm, safety, qav, dtmin, tend, nstop):
+        """The cans1d main.f loop (cfl_m; t=t+dt; mlw_m; bdfrex at both ends)
+        on the state dict st (in place), calling the routines of the
+        reference build `ref` (a Ref("1d")); returns (t, ns, merr)."""
+        fn = lambda name: C.cast(getattr(ref.lib, name + "_"), C.c_void_p)
+        t, merr = C.c_double(), C.c_int()
+        ix = st["ro"].shape[0]
+        ns = self.lib.orref_loop_1d(fn("cfl_m"), fn("mlw_m"), fn("bdfrex"),
+                                    *map(_arr, (st["ro"], st["pr"], st["vx"], st["vy"], st["by"], bx, bxm, dx, dxm)),
+                                    C.c_int(ix), C.c_int(margin), C.c_double(gm), C.c_double(safety),
+                                    C.c_double(qav), C.c_double(dtmin), C.c_double(tend), C.c_int(nstop),
+                                    C.byref(t), C.byref(merr))
+        return t.value, ns, merr.value
+
     def spicule_run(self, st, g, x, margin, gm, tend, nstop, t, ns, seed, qav=1.0, safety=0.8, dtmin=1e-10,
                     floor=1e-9):
         """main.f's loop from (t, ns, seed) on the state dict st (in place);

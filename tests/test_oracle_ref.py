@@ -211,3 +211,34 @@ def test_bnd_3d_all_kinds(oracle, ref3d, margin):
         oracle.bnd_3d(d, 7, m0, margin, q)
         ref3d.call("bdper" + ax, m0, margin, r, ix, jx, kx)
         assert_bitwise(q, r, f"bdper{ax} margins {m0},{margin}")
+
+
+def test_c0_loop_on_reference_build(oracle, ref1d):
+    """The c0 shock tube to t=0.1 (about a thousand steps): the main.f loop
+    running the reference routines (orref_loop_1d over libcans1d_ref.so)
+    against the same loop over the oracle's routines, bit for bit, with the
+    same step count and final time."""
+    from utrip_amd import problems as P
+    pb = P.shock_tube_1d(nx=1024, margin=2)
+    g, pp = pb.grid, pb.params
+    a = {k: v.copy() for k, v in pb.fields.items()}
+    t_r, ns_r, merr_r = oracle.ref_loop_1d(ref1d, a, a["bx"], a["bxm"], g["dx"], g["dxm"], 2, pp["gm"], pp["safety"],
+                                           pp["qav"], pp["dtmin"], pp["tend"], 10 ** 6)
+    assert merr_r == 0 and t_r >= pp["tend"]
+    b = {k: v.copy() for k, v in pb.fields.items()}
+    t, ns = 0.0, 0
+    while True:
+        ns += 1
+        dt, merr, _ = oracle.cfl_m_1d(pp["safety"], pp["dtmin"], pp["gm"], b["bx"], b["ro"], b["pr"], b["vx"],
+                                      b["vy"], b["by"], g["dx"])
+        assert merr == 0
+        t = t + dt
+        oracle.mlw_m_1d(*[b[k] for k in F1], b["bx"], b["bxm"], dt, pp["qav"], pp["gm"], g["dx"], g["dxm"])
+        for k in F1:
+            oracle.bnd_1d(0, 0, 2, b[k])
+            oracle.bnd_1d(0, 1, 2, b[k])
+        if not t < pp["tend"]:
+            break
+    assert (ns, t) == (ns_r, t_r)
+    for k in F1:
+        assert_bitwise(a[k], b[k], k)

@@ -659,6 +659,15 @@ class Batch1D:
                                       C.c_int(1 if self.mixed else 0), *self._outs(), C.c_int(bt), C.c_int(bpc),
                                       _ptr(self.ws), _stream()))
 
+    def run_steps(self, nstop_tube: torch.Tensor, tend: float = 1e300):
+        """Run tube k while ns < nstop_tube[k] and t < tend (nstop_tube: an
+        int32 device tensor of one entry per tube)."""
+        bt, bpc = self.shape
+        _check(lib().cans1d_batch_run_steps(*self._common(), _d(tend),
+                                            _iptr(nstop_tube, torch.int32, self.n, "nstop_tube"),
+                                            C.c_int(self.margin), C.c_int(1 if self.mixed else 0), *self._outs(),
+                                            C.c_int(bt), C.c_int(bpc), _ptr(self.ws), _stream()))
+
     def graph(self, tend: float, nstop: int = 2**31 - 1, init=None):
         """An instantiated hipGraph of the whole run; ``init`` (5 packed
         device tensors ro..by) is copied into the state at the start of

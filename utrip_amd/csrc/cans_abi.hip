@@ -32,7 +32,7 @@ int abi_fail(int code, const char *fmt, ...) {
     va_end(ap);
     return code;
 }
-int abi_scratch(size_t bytes, void **out);
+int abi_scratch(size_t bytes, void **out, hipStream_t s);
 int host_trig(const double *y, const double *ym, int jx, double *dev, hipStream_t s);
 }  // namespace cans
 
@@ -48,20 +48,23 @@ std::mutex g_mu;
                         __FILE__, __LINE__);                                            \
     } while (0)
 
-// Scratch device memory of the drop-in entry points, one pool per device.
-// Calls on one device that overlap in time share it (the reference engines
-// are called one at a time by a single-threaded main program).
+// Scratch device memory of the drop-in entry points: one pool per (device,
+// stream), grown to the largest request and kept until
+// cans_gpu_release_all.  Calls queued on one stream run in order and may
+// share their pool; calls on different streams get different pools, so they
+// never overwrite each other's workspace.  Host threads that enqueue on the
+// same stream at once still race on it (as they would on the stream).
 struct Pool {
     void *ptr = nullptr;
     size_t bytes = 0;
 };
-std::map<int, Pool> g_pool;
+std::map<std::pair<int, hipStream_t>, Pool> g_pool;
 
-int pool_scratch(size_t bytes, void **out) {
+int pool_scratch(size_t bytes, void **out, hipStream_t stream) {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_mu);
-    Pool &p = g_pool[dev];
+    Pool &p = g_pool[{dev, stream}];
     if (p.bytes < bytes) {
         if (p.ptr) {
             HIPCHK(hipDeviceSynchronize());
@@ -80,7 +83,7 @@ inline size_t al(size_t n) { return (n + 63) & ~size_t(63); }  // doubles, 512-b
 
 }  // namespace
 
-int cans::abi_scratch(size_t bytes, void **out) { return pool_scratch(bytes, out); }
+int cans::abi_scratch(size_t bytes, void **out, hipStream_t s) { return pool_scratch(bytes, out, s); }
 
 namespace {
 std::atomic<int> g_march_waves{32768};
@@ -156,7 +159,7 @@ int cans1d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *by,
                  const double *dx, const double *dxm, int ix, void *stream) {
     if (ix < 3) return fail(CANS_ERR_ARG, "cans1d_mlw_m: ix=%d < 3", ix);
     void *w = nullptr;
-    int rc = pool_scratch(9 * al(ix) * sizeof(double), &w);
+    int rc = pool_scratch(9 * al(ix) * sizeof(double), &w, (hipStream_t)stream);
     if (rc) return rc;
     return mlw1d_on_ws(ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm, ix, (double *)w, (hipStream_t)stream);
 }
@@ -208,7 +211,7 @@ int cans1d_mlw_m_noalloc(double *ro, double *pr, double *vx, double *vy, double 
 
 static int cfl_common(CflArgs &a, double *dt, double dtmin, int *merr, hipStream_t s) {
     void *w = nullptr;
-    int rc = pool_scratch(64, &w);
+    int rc = pool_scratch(64, &w, s);
     if (rc) return rc;
     unsigned long long *key = (unsigned long long *)w;
     int *loc = (int *)((char *)w + 8);
@@ -289,7 +292,7 @@ int cans2d_mlw_m(double *ro, double *pr, double *vx, double *vy, double *bx, dou
     hipStream_t s = (hipStream_t)stream;
     const size_t n = al((size_t)ix * jx), mxn = al(ix), myn = al(jx);
     void *w = nullptr;
-    int rc = pool_scratch((7 * n + 4 * mxn + 4 * myn) * sizeof(double), &w);
+    int rc = pool_scratch((7 * n + 4 * mxn + 4 * myn) * sizeof(double), &w, s);
     if (rc) return rc;
     double *ws = (double *)w;
     double *o[7];
@@ -370,7 +373,7 @@ int cans3d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm
     if (!x || !y) return fail(CANS_ERR_ARG, "cans3d_cfl_m_s: NULL x or y");
     hipStream_t s = (hipStream_t)stream;
     double *tab = nullptr;
-    if (int rc = pool_scratch(64 + 6 * (size_t)jx * sizeof(double), (void **)&tab)) return rc;
+    if (int rc = pool_scratch(64 + 6 * (size_t)jx * sizeof(double), (void **)&tab, s)) return rc;
     tab += 8;                                   // the first 64 bytes hold cfl_common's key
     if (int rc = cans::host_trig(y, nullptr, jx, tab, s)) return rc;
     CflArgs a{};
@@ -665,6 +668,7 @@ struct Mirror {
     size_t cap = 0;       // device allocation
     size_t bytes = 0;     // size of the current mapping (what unmap copies back)
     bool valid = false;   // device copy is current
+    bool dirty = false;   // resident mode: a routine changed the device copy since the last sync
 };
 std::map<const void *, Mirror> g_mirror;
 int g_resident = 0;
@@ -688,7 +692,10 @@ void *cans_gpu_map(const void *host, size_t bytes, int copy_in) {
     // allocator reuses it): the mapping's size is this call's, so that unmap
     // never copies past the end of the host array, and a size change makes
     // the device copy stale.
-    if (m.bytes != bytes) m.valid = false;
+    if (m.bytes != bytes) {
+        m.valid = false;
+        m.dirty = false;   // the device data belonged to the array that lived here before
+    }
     m.bytes = bytes;
     const bool need = copy_in && (!g_resident || !m.valid);
     if (need) {
@@ -706,12 +713,29 @@ int cans_gpu_unmap(void *host, int copy_out) {
     auto it = g_mirror.find(host);
     if (it == g_mirror.end()) return fail(CANS_ERR_ARG, "cans_gpu_unmap: %p not mapped", host);
     if (copy_out && !g_resident) HIPCHK(hipMemcpy(host, it->second.dev, it->second.bytes, hipMemcpyDeviceToHost));
+    if (copy_out && g_resident) it->second.dirty = true;
     if (!copy_out && !g_resident) HIPCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+// Copy back the mirrors a routine changed since the last sync (and only
+// those: read-only inputs and arrays the program has since released stay
+// untouched on the host).  Caller holds g_mu.
+static int flush_dirty() {
+    HIPCHK(hipDeviceSynchronize());
+    for (auto &kv : g_mirror) {
+        Mirror &m = kv.second;
+        if (!m.valid || !m.dirty) continue;
+        HIPCHK(hipMemcpy(const_cast<void *>(kv.first), m.dev, m.bytes, hipMemcpyDeviceToHost));
+        m.dirty = false;
+    }
     return 0;
 }
 
 int cans_gpu_set_resident(int on) {
     std::lock_guard<std::mutex> lk(g_mu);
+    if (g_resident && !on)
+        if (int rc = flush_dirty()) return rc;   // leaving resident mode: the host gets the results first
     g_resident = on ? 1 : 0;
     if (!on)
         for (auto &kv : g_mirror) kv.second.valid = false;
@@ -720,17 +744,26 @@ int cans_gpu_set_resident(int on) {
 
 int cans_gpu_sync_host(void) {
     std::lock_guard<std::mutex> lk(g_mu);
-    HIPCHK(hipDeviceSynchronize());
-    for (auto &kv : g_mirror)
-        if (kv.second.valid)
-            HIPCHK(hipMemcpy(const_cast<void *>(kv.first), kv.second.dev, kv.second.bytes, hipMemcpyDeviceToHost));
-    return 0;
+    return flush_dirty();
 }
 
 int cans_gpu_invalidate(const void *host) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_mirror.find(host);
-    if (it != g_mirror.end()) it->second.valid = false;
+    if (it != g_mirror.end()) {
+        it->second.valid = false;
+        it->second.dirty = false;
+    }
+    return 0;
+}
+
+int cans_gpu_forget(const void *host) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_mirror.find(host);
+    if (it == g_mirror.end()) return 0;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(it->second.dev));
+    g_mirror.erase(it);
     return 0;
 }
 

@@ -59,6 +59,7 @@ struct BatchIn {
     double *t_out;
     int *ns_out, *merr_out;
     double *ws;
+    const int *nstop_tube = nullptr;   // device, per tube (NULL: nstop)
 };
 
 // Host check of the per-tube layout (the kernel trusts off/ix): every tube
@@ -80,6 +81,12 @@ int validate(const BatchIn &b) {
         if (ix[k] < 2 * b.margin + 1 || ix[k] < 3 || off[k] < 0 || off[k] + ix[k] > b.ncells)
             return abi_fail(CANS_ERR_ARG, "cans1d_batch: tube %d has off=%lld ix=%d (ncells=%lld, margin=%d)", k,
                             off[k], ix[k], b.ncells, b.margin);
+    }
+    if (b.nstop_tube) {
+        std::vector<int> ns(b.ntube);
+        HIPCHK(hipMemcpy(ns.data(), b.nstop_tube, sizeof(int) * b.ntube, hipMemcpyDeviceToHost));
+        for (int k = 0; k < b.ntube; k++)
+            if (ns[k] < 0) return abi_fail(CANS_ERR_ARG, "cans1d_batch: nstop of tube %d is %d", k, ns[k]);
     }
     if (b.order) {
         order.resize(b.ntube);
@@ -111,6 +118,7 @@ Batch1dArgs make_args(const BatchIn &b, double *const *state) {
     a.bx = b.bx; a.bxm = b.bxm; a.dx = b.dx; a.dxm = b.dxm;
     a.gm = b.gm; a.qav = b.qav; a.safety = b.safety; a.dtmin = b.dtmin; a.tend = b.tend; a.t0 = 0.0;
     a.nstop = b.nstop; a.margin = b.margin;
+    a.nstop_tube = b.nstop_tube;
     a.t_out = b.t_out; a.ns_out = b.ns_out; a.merr_out = b.merr_out;
     return a;
 }
@@ -159,6 +167,24 @@ int cans1d_batch_run(int ntube, const long long *off, const int *ix, const int *
     if (ntube == 0) return 0;
     if (!shape_ok(block_threads, blocks_per_cu))
         return abi_fail(CANS_ERR_ARG, "cans1d_batch_run: block_threads=%d blocks_per_cu=%d", block_threads,
+                        blocks_per_cu);
+    const Batch1dArgs a = make_args(b, b.f);
+    HIPCHK(launch_batch1d(a, mixed, block_threads, cu_count() * blocks_per_cu, (hipStream_t)stream));
+    return 0;
+}
+
+int cans1d_batch_run_steps(int ntube, const long long *off, const int *ix, const int *order, double *ro, double *pr,
+                           double *vx, double *vy, double *by, const double *bx, const double *bxm, const double *dx,
+                           const double *dxm, long long ncells, double gm, double qav, double safety, double dtmin,
+                           double tend, const int *nstop_tube, int margin, int mixed, double *t_out, int *ns_out,
+                           int *merr_out, int block_threads, int blocks_per_cu, void *ws, void *stream) {
+    if (!nstop_tube) return abi_fail(CANS_ERR_ARG, "cans1d_batch_run_steps: NULL nstop_tube");
+    BatchIn b{ntube, off, ix, order, {ro, pr, vx, vy, by}, bx, bxm, dx, dxm, ncells, gm, qav, safety, dtmin,
+              tend, 0, margin, mixed, t_out, ns_out, merr_out, (double *)ws, nstop_tube};
+    if (int e = validate(b)) return e;
+    if (ntube == 0) return 0;
+    if (!shape_ok(block_threads, blocks_per_cu))
+        return abi_fail(CANS_ERR_ARG, "cans1d_batch_run_steps: block_threads=%d blocks_per_cu=%d", block_threads,
                         blocks_per_cu);
     const Batch1dArgs a = make_args(b, b.f);
     HIPCHK(launch_batch1d(a, mixed, block_threads, cu_count() * blocks_per_cu, (hipStream_t)stream));

@@ -11,7 +11,7 @@
 
 namespace cans {
 int abi_fail(int code, const char *fmt, ...);
-int abi_scratch(size_t bytes, void **out);
+int abi_scratch(size_t bytes, void **out, hipStream_t s);
 int host_trig(const double *y, const double *ym, int jx, double *dev, hipStream_t s);
 }  // namespace cans
 
@@ -99,7 +99,7 @@ static int roe_dropin(int dim, bool mixed, double *const *prim, double *az, cons
     hipStream_t s = (hipStream_t)stream;
     const size_t N = (size_t)ix * jx * kx;
     void *w = nullptr;
-    if (int rc = abi_scratch((engine_doubles(dim, CANS_ENGINE_ROE, N) + 9 * al(N)) * sizeof(double), &w)) return rc;
+    if (int rc = abi_scratch((engine_doubles(dim, CANS_ENGINE_ROE, N) + 9 * al(N)) * sizeof(double), &w, s)) return rc;
     Carver c{(double *)w};
     RoeArgs r{};
     for (int q = 0; q < 8; q++) r.prim[q] = prim[q];
@@ -180,7 +180,7 @@ static int mlw3d_dropin(const char *who, double *const *prim, const double *x, c
     const int engine = cyl ? CANS_ENGINE_MLWC : CANS_ENGINE_MLW;
     const size_t need =
         metric_doubles(ix, jx, kx) + engine_doubles(3, engine, N) + 8 * al(N) + (sph ? 6 * al(jx) : 0);
-    if (int rc = abi_scratch(need * sizeof(double), &w)) return rc;
+    if (int rc = abi_scratch(need * sizeof(double), &w, s)) return rc;
     Carver c{(double *)w};
     Metric m[3];
     carve_metric(c, ix, jx, kx, m);
@@ -249,7 +249,7 @@ static int mlw3_dropin(int dim, double *const *f, const double *bx, const double
     hipStream_t s = (hipStream_t)stream;
     const size_t N = (size_t)ix * (dim == 2 ? jx : 1);
     void *w = nullptr;
-    if (int rc = abi_scratch((metric_doubles(ix, jx, 1) + nf * al(N)) * sizeof(double), &w)) return rc;
+    if (int rc = abi_scratch((metric_doubles(ix, jx, 1) + nf * al(N)) * sizeof(double), &w, s)) return rc;
     Carver c{(double *)w};
     Metric m[3];
     carve_metric(c, ix, jx, 1, m);
@@ -277,7 +277,7 @@ int cans2d_mlw_m_c(double *ro, double *pr, double *vx, double *vz, double *bx, d
     hipStream_t s = (hipStream_t)stream;
     const size_t N = (size_t)ix * jx;
     void *w = nullptr;
-    if (int rc = abi_scratch((metric_doubles(ix, jx, 1) + 7 * al(N)) * sizeof(double), &w)) return rc;
+    if (int rc = abi_scratch((metric_doubles(ix, jx, 1) + 7 * al(N)) * sizeof(double), &w, s)) return rc;
     Carver c{(double *)w};
     Metric m[3];
     carve_metric(c, ix, jx, 1, m);
@@ -307,7 +307,7 @@ int cans2d_mlw_m_s(double *ro, double *pr, double *vx, double *vy, double *bx, d
     const size_t N = (size_t)ix * jx;
     void *w = nullptr;
     const size_t need = metric_doubles(ix, jx, 1) + engine_doubles(2, CANS_ENGINE_MLWS, N) + 7 * al(N) + 6 * al(jx);
-    if (int rc = abi_scratch(need * sizeof(double), &w)) return rc;
+    if (int rc = abi_scratch(need * sizeof(double), &w, s)) return rc;
     Carver c{(double *)w};
     Metric m[3];
     carve_metric(c, ix, jx, 1, m);
@@ -352,7 +352,7 @@ static int hlld_dropin(bool mixed, double *const *prim, double gm, double mu, do
     hipStream_t s = (hipStream_t)stream;
     const size_t N = (size_t)ix * jx * kx;
     void *w = nullptr;
-    if (int rc = abi_scratch(engine_doubles(3, CANS_ENGINE_HLLD, N) * sizeof(double), &w)) return rc;
+    if (int rc = abi_scratch(engine_doubles(3, CANS_ENGINE_HLLD, N) * sizeof(double), &w, s)) return rc;
     Carver c{(double *)w};
     HlldArgs h{};
     for (int q = 0; q < 8; q++) { h.prim[q] = prim[q]; h.out[q] = prim[q]; }

@@ -9,7 +9,7 @@
 
 namespace cans {
 int abi_fail(int code, const char *fmt, ...);
-int abi_scratch(size_t bytes, void **out);
+int abi_scratch(size_t bytes, void **out, hipStream_t s);
 }  // namespace cans
 
 using namespace cans;
@@ -61,7 +61,7 @@ int cans1d_mlw_m_bg(double *ro, double *pr, double *vx, double *vy, double *by, 
     hipStream_t s = (hipStream_t)stream;
     const size_t n = al((size_t)ix);
     void *w = nullptr;
-    int rc = abi_scratch((5 + 4) * n * sizeof(double), &w);
+    int rc = abi_scratch((5 + 4) * n * sizeof(double), &w, s);
     if (rc) return rc;
     double *ws = (double *)w;
     double *met[4] = {ws + 5 * n, ws + 6 * n, ws + 7 * n, ws + 8 * n};
@@ -87,7 +87,7 @@ int cans1d_trqalfven(double *vy, double dt, const double *trq, const double *rr,
     if (*mseed <= 0) return abi_fail(CANS_ERR_ARG, "cans1d_trqalfven: seed %d not in 1..2^31-2", *mseed);
     hipStream_t s = (hipStream_t)stream;
     void *w = nullptr;
-    int rc = abi_scratch(al((size_t)ix) * sizeof(double), &w);
+    int rc = abi_scratch(al((size_t)ix) * sizeof(double), &w, s);
     if (rc) return rc;
     long long *pw = (long long *)w;
     double *nomet[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -101,7 +101,7 @@ int cans1d_chkdav(int *n_events, double *da, double *vx, double floor_, int ix, 
     if (ix < 1 || !da || !vx) return abi_fail(CANS_ERR_ARG, "cans1d_chkdav: bad argument");
     hipStream_t s = (hipStream_t)stream;
     void *w = nullptr;
-    int rc = abi_scratch(64, &w);
+    int rc = abi_scratch(64, &w, s);
     if (rc) return rc;
     int *cnt = (int *)w;
     HIPCHK(launch_chkdav1d(da, vx, floor_, ix, cnt, s));

@@ -14,9 +14,9 @@
 // step costs one workgroup barrier (the dt min-reduction).
 //
 // Precision: T = double is bit-identical to mlw_m.f:2-215 + cfl_m + bdfrex;
-// T = float evaluates the cell fluxes, the half step and the face fluxes in
-// FP32 and accumulates du, the update, the primitive recovery and cfl_m in
-// FP64.
+// T = float evaluates the cell and face fluxes (and their differences) in
+// FP32 and keeps in FP64 the half-step state with its pressure recovery, the
+// accumulation of du, the update, the primitive recovery and cfl_m.
 #include "cans_common.hpp"
 #include "cans_kernels.hpp"
 #include "cans_wave.hpp"
@@ -68,34 +68,40 @@ CANS_DEV double update_segment(const Batch1dArgs &a, const P5 &cur, const P5 &nx
     }
     const double dxi = a.di[p];
     const double dxim = a.dim[p];
-    // face between this cell and the next (mlwhalf.f:39-46, mlw_m.f:129-175)
+    // face between this cell and the next (mlwhalf.f:39-46, mlw_m.f:129-175):
+    // the half-step state and its primitives, pressure included, in FP64
+    // (in the mixed form the flux difference that enters it is the FP32 one,
+    // widened); the face fluxes in T.
     T ff[5];
     T fE[5];
     double uE[5];
     {
-        T un[5];
+        double un[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) {
             uE[q] = shl1(u[q]);
             fE[q] = shl1(f[q]);
-            const T uh = (T)0.5 * ((T)uE[q] + (T)u[q]);
-            const T dfdx = (T)dxim * (fE[q] - f[q]);
-            un[q] = uh - (T)dt * dfdx;
+            const double uh = 0.5 * (uE[q] + u[q]);
+            const double dfdx = dxim * (double)(fE[q] - f[q]);
+            un[q] = uh - dt * dfdx;
         }
-        const T roh = un[0], eeh = un[1], rxh = un[2], ryh = un[3], byh = un[4];
-        const T bxm = (T)a.bxm[p];
-        const T tgm = (T)gm;
-        const T vxh = rxh / roh;
-        const T vyh = ryh / roh;
-        const T vv = vxh * vxh + vyh * vyh;
-        const T bb = byh * byh;
-        const T prh = (tgm - (T)1.0) * (eeh - (T)0.5 * roh * vv - (T)pi8i * bb);
-        const T ezh = -vxh * byh + vyh * bxm;
-        const T eph = prh * tgm / (tgm - (T)1.0) + (T)0.5 * roh * vv;
-        ff[0] = roh * vxh;
-        ff[1] = eph * vxh - byh * ezh * (T)pi4i;
-        ff[2] = roh * (vxh * vxh) + prh + bb * (T)pi8i;
-        ff[3] = roh * vxh * vyh - bxm * byh * (T)pi4i;
+        const double roh = un[0], eeh = un[1], rxh = un[2], ryh = un[3], byh = un[4];
+        const double bxm = a.bxm[p];
+        const double vxh = rxh / roh;
+        const double vyh = ryh / roh;
+        const double vv = vxh * vxh + vyh * vyh;
+        const double bb = byh * byh;
+        const double prh = (gm - 1.0) * (eeh - 0.5 * roh * vv - pi8i * bb);
+        const T troh = (T)roh, tvxh = (T)vxh, tvyh = (T)vyh, tbyh = (T)byh, tbxm = (T)bxm, tprh = (T)prh,
+                tgm = (T)gm;
+        const T tvv = tvxh * tvxh + tvyh * tvyh;
+        const T tbb = tbyh * tbyh;
+        const T ezh = -tvxh * tbyh + tvyh * tbxm;
+        const T eph = tprh * tgm / (tgm - (T)1.0) + (T)0.5 * troh * tvv;
+        ff[0] = troh * tvxh;
+        ff[1] = eph * tvxh - tbyh * ezh * (T)pi4i;
+        ff[2] = troh * (tvxh * tvxh) + tprh + tbb * (T)pi8i;
+        ff[3] = troh * tvxh * tvyh - tbxm * tbyh * (T)pi4i;
         ff[4] = -ezh;
     }
     // own cell: centre term, full step, artificial viscosity, update
@@ -190,6 +196,7 @@ __global__ __launch_bounds__(BT) void batch1d_kernel(Batch1dArgs a) {
         int ns = 0, merr = 0, nexec = 0;
         double t = a.t0;
         const int nseg = (ix - 2 * margin + 61) / 62;
+        const int nstop = a.nstop_tube ? uni(a.nstop_tube[k]) : a.nstop;
         for (;;) {
             ns = ns + 1;
             if (dt < a.dtmin) { merr = 9001; break; }    // cfl_m.f: merr=9001
@@ -204,7 +211,7 @@ __global__ __launch_bounds__(BT) void batch1d_kernel(Batch1dArgs a) {
             nexec++;
             dt = block_min<BT>(m, red[par]);              // dt of the next step; also the step's barrier
             par ^= 1;
-            if (!(ns < a.nstop && t < a.tend)) break;
+            if (!(ns < nstop && t < a.tend)) break;
         }
         if (nexec & 1) {   // the last state is in the scratch set: copy back
             for (int c = 1 + (int)threadIdx.x; c <= ix; c += BT) {

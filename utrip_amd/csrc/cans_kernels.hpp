@@ -122,6 +122,7 @@ struct Batch1dArgs {
     double *di, *dim;            // 1/dx, 1/dxm (filled by the launch)
     double gm, qav, safety, dtmin, tend, t0;
     int nstop, margin;
+    const int *nstop_tube;       // device, per tube (NULL: nstop for every tube)
     double *t_out;               // device, per tube
     int *ns_out, *merr_out;      // device, per tube
     unsigned *next;              // queue counter (zeroed by the launch)

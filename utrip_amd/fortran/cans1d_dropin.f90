@@ -90,7 +90,7 @@ subroutine trqalfven(vy, dt, time, x, ix, rr)
   if (.not. fresh) fresh = any(xsaved /= x)
   if (fresh) then
      if (allocated(trq)) then
-        call check(c_invalidate(c_loc(trq(1))), 'trqalfven')
+        call check(c_forget(c_loc(trq(1))), 'trqalfven')
         deallocate(trq, xsaved)
      end if
      allocate(trq(ix), xsaved(ix))

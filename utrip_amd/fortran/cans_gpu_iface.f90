@@ -40,6 +40,10 @@ module cans_gpu_iface
        import :: c_ptr, c_int
        type(c_ptr), value :: host
      end function c_invalidate
+     integer(c_int) function c_forget(host) bind(C, name="cans_gpu_forget")
+       import :: c_ptr, c_int
+       type(c_ptr), value :: host
+     end function c_forget
      function c_last_error() bind(C, name="cans_gpu_last_error")
        import :: c_ptr
        type(c_ptr) :: c_last_error
@@ -389,3 +393,10 @@ subroutine cans_gpu_modified(a)
   double precision, target, intent(in) :: a(*)
   call check(c_invalidate(c_loc(a(1))), 'cans_gpu_modified')
 end subroutine cans_gpu_modified
+
+subroutine cans_gpu_forget(a)
+  use cans_gpu_iface
+  implicit none
+  double precision, target, intent(in) :: a(*)
+  call check(c_forget(c_loc(a(1))), 'cans_gpu_forget')
+end subroutine cans_gpu_forget
