@@ -177,13 +177,16 @@ def cpu_baseline(n: int, nsteps: int, procs: int):
     return out
 
 
-def blast3d_figures(n: int, steps: int, l1_steps: int = 500):
+def blast3d_figures(n: int, steps: int, l1_steps: int = 500, early: int = 100):
     """The 384^3 periodic MHD blast wave (rho=1, p=0.1, p=10 inside r<0.1,
     B_HL=(1,1,0)/sqrt(2)) on the resident run: 3D MLW (FP64), Roe-MUSCL and
     HLLD in FP64 and with FP32 face fluxes.  Each engine is timed over
     `steps` steps after 2 warm-up steps; the Roe and HLLD runs then continue
     to l1_steps steps (the workload's run length), where the relative L1
-    difference per field between the FP32-flux and FP64 states is taken."""
+    difference per field between the FP32-flux and FP64 states is taken
+    (density also after `early` steps).  A control run of each, FP64 from an
+    initial density perturbed by 1e-7 relative (seeded), measures how far
+    the run itself amplifies a perturbation of that size by then."""
     import gc
 
     import numpy as np
@@ -193,16 +196,23 @@ def blast3d_figures(n: int, steps: int, l1_steps: int = 500):
     dev = torch.device("cuda:0")
     F3 = cans.FIELDS[3]
     out = {"cells": n ** 3, "steps": steps, "warmup": 2, "l1_after_steps": l1_steps}
-    finals = {}
-    for engine, mixed in (("mlw", False), ("roe", False), ("roe", True), ("hlld", False), ("hlld", True)):
+    variants = (("mlw", "fp64"), ("roe", "fp64"), ("roe", "fp32flux"), ("roe", "fp64_perturbed"),
+                ("hlld", "fp64"), ("hlld", "fp32flux"), ("hlld", "fp64_perturbed"))
+    finals, early_rho = {}, {}
+    for engine, var in variants:
         margin = 2 if engine == "mlw" else 3
         pb = problems.blast_3d(n=n, margin=margin, gaussian_b=(engine != "hlld"))
         st = [torch.from_numpy(np.ascontiguousarray(pb.fields[k].T)).to(dev) for k in F3]
         gd = {k: torch.from_numpy(pb.grid[k]).to(dev) for k in ("dx", "dxm", "dy", "dym", "dz", "dzm")}
         del pb
+        if var == "fp64_perturbed":
+            gen = torch.Generator(device="cpu").manual_seed(3)
+            xi = torch.rand(st[0].shape, generator=gen, dtype=torch.float64).to(dev) * 2 - 1
+            st[0].mul_(1 + 1e-7 * xi)
+            del xi
         eng = {"mlw": cans.ENGINE_MLW, "roe": cans.ENGINE_ROE, "hlld": cans.ENGINE_HLLD}[engine]
         run = cans.Run(3, eng, st, gd, 5.0 / 3.0, 0.4, qav=3.0, mu=1.0, margin=margin, bc=cans.BD_PER,
-                       mixed=mixed)
+                       mixed=(var == "fp32flux"))
         run.reset(dtmin=1e-10)
         run.steps(2)
         ev = cans.EventSet(2)
@@ -212,30 +222,43 @@ def blast3d_figures(n: int, steps: int, l1_steps: int = 500):
         ev.record(1)
         torch.cuda.synchronize()
         ms = ev.elapsed_ms(0, 1)
-        name = engine + ("_fp32flux" if mixed else "_fp64")
-        out[name] = {"zone_updates_per_s": n ** 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
-                     "hbm_frac_alg": 128 * n ** 3 * steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        name = engine + "_" + var
+        if var != "fp64_perturbed":
+            out[name] = {"zone_updates_per_s": n ** 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
+                         "hbm_frac_alg": 128 * n ** 3 * steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
         total = steps + 2
-        if engine != "mlw" and l1_steps > total:
-            run.steps(l1_steps - total)
+        if engine != "mlw":
+            sl = (slice(margin, -margin),) * 3
+            run.steps(early - total)
+            cur, _ = run.state()
+            early_rho[name] = cur[0][sl].clone()
+            run.steps(l1_steps - early)
             total = l1_steps
         cur, info = run.state()
         assert info["merr"] == 0 and info["nexec"] == total, info
         print(f"bench: blast {name} done", file=sys.stderr, flush=True)
         if engine != "mlw":
-            out[name]["t_at_l1"] = info["t"]
+            out.setdefault(name, {})["t_at_l1"] = info["t"]
             finals[name] = [t.clone() for t in cur]   # kept on the GPU (3.7 GB per run at 384^3)
         del run, st, gd, cur
         gc.collect()
         torch.cuda.empty_cache()
+
+    def rel(b, a):
+        return {k: float((x - y).abs().sum() / y.abs().sum()) if float(y.abs().sum()) > 0
+                else float((x - y).abs().sum()) for k, x, y in zip(F3, b, a)}
+
     for engine in ("roe", "hlld"):
-        a, b = finals[engine + "_fp64"], finals[engine + "_fp32flux"]
-        out[engine + "_rel_l1_fp32flux_vs_fp64"] = {
-            k: float((x - y).abs().sum() / y.abs().sum()) if float(y.abs().sum()) > 0 else float((x - y).abs().sum())
-            for k, x, y in zip(F3, b, a)}
+        a = finals[engine + "_fp64"]
+        out[engine + "_rel_l1_fp32flux_vs_fp64"] = rel(finals[engine + "_fp32flux"], a)
+        out[engine + "_rel_l1_fp64_perturbed_vs_fp64"] = rel(finals[engine + "_fp64_perturbed"], a)
+        e0 = early_rho[engine + "_fp64"]
+        out[engine + f"_rho_rel_l1_after_{early}_steps"] = {
+            v: float((early_rho[engine + "_" + v] - e0).abs().sum() / e0.abs().sum())
+            for v in ("fp32flux", "fp64_perturbed")}
         out[engine + "_fp32flux_speedup"] = out[engine + "_fp64"]["ms_per_step"] / out[engine + "_fp32flux"]["ms_per_step"]
-        del a, b
     finals.clear()
+    early_rho.clear()
     gc.collect()
     torch.cuda.empty_cache()
     return out

@@ -219,6 +219,96 @@ int cans3d_hlld_engine_mixed(double *ro, double *pr, double *vx, double *vy, dou
                              const double *dx, const double *dy, const double *dz, int ix, int jx,
                              int kx, const int *mfdim, void *stream);
 
+/* ---------------------------------------------------------------- helper routines
+ * The routines the engines are built from, with their reference argument
+ * lists (device arrays, in place as the reference updates them).  The fused
+ * engines above do not call them; they are there for callers that compose
+ * their own engine from them.  w arrays of roeflux_m, hlld_flux and the daw
+ * output of tvdminmod have a trailing dimension 2 (left state first). */
+/* cans1d/hdmlw/mlwhalf.f:2 mlwhalf(u,un,du,dt,f,dxi,dxim,ix) */
+int cans1d_mlwhalf(const double *u, double *un, double *du, double dt, const double *f, const double *dxi,
+                   const double *dxim, int ix, void *stream);
+/* cans1d/hdmlw/mlwfull.f:2 mlwfull(du,dt,f,dxi,ix) */
+int cans1d_mlwfull(double *du, double dt, const double *f, const double *dxi, int ix, void *stream);
+/* cans1d/hdmlw/mlwartv.f:2 mlwartv(u,du,dt,qx,dxi,dxim,ix) */
+int cans1d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi,
+                   const double *dxim, int ix, void *stream);
+/* cans1d/hdmlw/mlwsrch.f:2 mlwsrch(un,du,dt,s,ix) */
+int cans1d_mlwsrch(double *un, double *du, double dt, const double *s, int ix, void *stream);
+/* cans1d/hdmlw/mlwsrcf.f:2 mlwsrcf(du,dt,s,ux0,ux1,ix) */
+int cans1d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix,
+                   void *stream);
+/* cans1d/hdroe/roeflux_m.f:2-3 roeflux_m(fro,fee,frx,fry,fby,gm,row,prw,vxw,vyw,bxw,byw,ix), faces 1..ix-1 */
+int cans1d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *fby, double gm,
+                     const double *row, const double *prw, const double *vxw, const double *vyw,
+                     const double *bxw, const double *byw, int ix, void *stream);
+/* cans1d/hdroe/tvdminmod.f:2 tvdminmod(da,daw,ix) */
+int cans1d_tvdminmod(const double *da, double *daw, int ix, void *stream);
+/* cans2d/hdmlw/mlwhalf.f:2 mlwhalf(u,un,du,dt,fx,dxi,dxim,ix,fy,dyi,dyim,jx) */
+int cans2d_mlwhalf(const double *u, double *un, double *du, double dt, const double *fx, const double *dxi,
+                   const double *dxim, int ix, const double *fy, const double *dyi, const double *dyim, int jx,
+                   void *stream);
+/* cans2d/hdmlw/mlwfull.f:2 mlwfull(du,dt,fx,dxi,ux0,ux1,ix,fy,dyi,uy0,uy1,jx) */
+int cans2d_mlwfull(double *du, double dt, const double *fx, const double *dxi, const double *ux0,
+                   const double *ux1, int ix, const double *fy, const double *dyi, const double *uy0,
+                   const double *uy1, int jx, void *stream);
+/* cans2d/hdmlw/mlwartv.f:2 mlwartv(u,du,dt,qx,dxi,dxim,ix,qy,dyi,dyim,jx) */
+int cans2d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi,
+                   const double *dxim, int ix, const double *qy, const double *dyi, const double *dyim, int jx,
+                   void *stream);
+/* cans2d/hdmlw/mlwsrch.f:2 mlwsrch(un,du,dt,s,ix,jx) */
+int cans2d_mlwsrch(double *un, double *du, double dt, const double *s, int ix, int jx, void *stream);
+/* cans2d/hdmlw/mlwsrcf.f:2 mlwsrcf(du,dt,s,ux0,ux1,ix,uy0,uy1,jx) */
+int cans2d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix,
+                   const double *uy0, const double *uy1, int jx, void *stream);
+/* cans2d/hdroe/roeflux_m.f:1-2 roeflux_m(fro,fee,frx,fry,fby,gm,row,prw,vxw,vyw,bxw,byw,ix,jx) */
+int cans2d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *fby, double gm,
+                     const double *row, const double *prw, const double *vxw, const double *vyw,
+                     const double *bxw, const double *byw, int ix, int jx, void *stream);
+/* cans2d/hdroe/tvdminmod.f:1 tvdminmod(mdir,da,daw,ix,jx) */
+int cans2d_tvdminmod(int mdir, const double *da, double *daw, int ix, int jx, void *stream);
+/* cans3d/hdmlw/mlwhalf.f:2-3 mlwhalf(u,un,du,dt,fx,dxi,dxim,ix,fy,dyi,dyim,jx,fz,dzi,dzim,kx) */
+int cans3d_mlwhalf(const double *u, double *un, double *du, double dt, const double *fx, const double *dxi,
+                   const double *dxim, int ix, const double *fy, const double *dyi, const double *dyim, int jx,
+                   const double *fz, const double *dzi, const double *dzim, int kx, void *stream);
+/* cans3d/hdmlw/mlwfull.f:2-3 mlwfull(du,dt,fx,dxi,ux0,ux1,ix,fy,dyi,uy0,uy1,jx,fz,dzi,uz0,uz1,kx) */
+int cans3d_mlwfull(double *du, double dt, const double *fx, const double *dxi, const double *ux0,
+                   const double *ux1, int ix, const double *fy, const double *dyi, const double *uy0,
+                   const double *uy1, int jx, const double *fz, const double *dzi, const double *uz0,
+                   const double *uz1, int kx, void *stream);
+/* cans3d/hdmlw/mlwartv.f:2-3 mlwartv(u,du,dt,qx,dxi,dxim,ix,qy,dyi,dyim,jx,qz,dzi,dzim,kx) */
+int cans3d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi,
+                   const double *dxim, int ix, const double *qy, const double *dyi, const double *dyim, int jx,
+                   const double *qz, const double *dzi, const double *dzim, int kx, void *stream);
+/* cans3d/hdmlw/mlwsrch.f:2 mlwsrch(un,du,dt,s,ix,jx,kx) */
+int cans3d_mlwsrch(double *un, double *du, double dt, const double *s, int ix, int jx, int kx, void *stream);
+/* cans3d/hdmlw/mlwsrcf.f:2 mlwsrcf(du,dt,s,ux0,ux1,ix,uy0,uy1,jx,uz0,uz1,kx) */
+int cans3d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix,
+                   const double *uy0, const double *uy1, int jx, const double *uz0, const double *uz1, int kx,
+                   void *stream);
+/* cans3d/hdroe/roeflux_m.f:6-7 roeflux_m(fro,fee,frx,fry,frz,fby,fbz,gm,row,prw,vxw,vyw,vzw,bxw,byw,bzw,ix,jx,kx) */
+int cans3d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *frz, double *fby, double *fbz,
+                     double gm, const double *row, const double *prw, const double *vxw, const double *vyw,
+                     const double *vzw, const double *bxw, const double *byw, const double *bzw, int ix, int jx,
+                     int kx, void *stream);
+/* cans3d/hdroe/tvdminmod.f:4 tvdminmod(mdir,da,daw,ix,jx,kx) */
+int cans3d_tvdminmod(int mdir, const double *da, double *daw, int ix, int jx, int kx, void *stream);
+/* cansxx-nosupport/hdhlld/hlld_flux.f90:2-3
+ * hlld_flux(row,prw,vxw,vyw,vzw,bx,byw,bzw,gm,ix,jx,kx,zfloor,fro,fee,frx,fry,frz,fby,fbz), every face */
+int cans3d_hlld_flux(const double *row, const double *prw, const double *vxw, const double *vyw,
+                     const double *vzw, const double *bx, const double *byw, const double *bzw, double gm, int ix,
+                     int jx, int kx, double zfloor, double *fro, double *fee, double *frx, double *fry,
+                     double *frz, double *fby, double *fbz, void *stream);
+/* cansxx-nosupport/hdhlld/hlld_mh.f:2-4 hlld_mh(rof,prf,vxf,vyf,vzf,bxf,byf,bzf,ron,een,rxn,ryn,rzn,
+ * bxn,byn,bzn,gm,dt,dx,dy,dz,ix,jx,kx,mfdim,mstage); mfdim is a host array of 3.  Where the reference
+ * reads its unset automatic arrays (the last face along a direction, which reaches ghost cells only)
+ * the result here is NaN. */
+int cans3d_hlld_mh(const double *rof, const double *prf, const double *vxf, const double *vyf,
+                   const double *vzf, const double *bxf, const double *byf, const double *bzf, double *ron,
+                   double *een, double *rxn, double *ryn, double *rzn, double *bxn, double *byn, double *bzn,
+                   double gm, double dt, const double *dx, const double *dy, const double *dz, int ix, int jx,
+                   int kx, const int *mfdim, int mstage, void *stream);
+
 /* ---------------------------------------------------------------- resident loop
  * A device clock holds t, ns, merr and dt of the loop in
  * cans2d/md_mhdshktb/hdmlw/main.f:150-206 so that steps run back to back with

@@ -166,6 +166,67 @@ void orh_engine(double *ro, double *pr, double *vx, double *vy, double *vz,
                 double dt, const double *dx, const double *dy,
                 const double *dz, int ix, int jx, int kx, const int *mfdim);
 
+/* ---------------- helper routines with the reference argument lists
+ * (hdmlw/mlwhalf.f, mlwfull.f, mlwartv.f, mlwsrch.f, mlwsrcf.f of each
+ * dimensionality; hdroe/roeflux_m.f, tvdminmod.f; cansxx-nosupport
+ * hdhlld/hlld_flux.f90, hlld_mh.f).  w arrays and daw carry a trailing
+ * dimension 2 (left state first). */
+void or1d_mlwhalf(const double *u, double *un, double *du, double dt, const double *f, const double *dxi,
+                  const double *dxim, int ix);
+void or1d_mlwfull(double *du, double dt, const double *f, const double *dxi, int ix);
+void or1d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi, const double *dxim,
+                  int ix);
+void or1d_mlwsrch(double *un, double *du, double dt, const double *s, int ix);
+void or1d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix);
+void or2d_mlwhalf(const double *u, double *un, double *du, double dt, const double *fx, const double *dxi,
+                  const double *dxim, int ix, const double *fy, const double *dyi, const double *dyim, int jx);
+void or2d_mlwfull(double *du, double dt, const double *fx, const double *dxi, const double *ux0, const double *ux1,
+                  int ix, const double *fy, const double *dyi, const double *uy0, const double *uy1, int jx);
+void or2d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi, const double *dxim,
+                  int ix, const double *qy, const double *dyi, const double *dyim, int jx);
+void or2d_mlwsrch(double *un, double *du, double dt, const double *s, int ix, int jx);
+void or2d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix,
+                  const double *uy0, const double *uy1, int jx);
+void or3d_mlwhalf(const double *u, double *un, double *du, double dt, const double *fx, const double *dxi,
+                  const double *dxim, int ix, const double *fy, const double *dyi, const double *dyim, int jx,
+                  const double *fz, const double *dzi, const double *dzim, int kx);
+void or3d_mlwfull(double *du, double dt, const double *fx, const double *dxi, const double *ux0, const double *ux1,
+                  int ix, const double *fy, const double *dyi, const double *uy0, const double *uy1, int jx,
+                  const double *fz, const double *dzi, const double *uz0, const double *uz1, int kx);
+void or3d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi, const double *dxim,
+                  int ix, const double *qy, const double *dyi, const double *dyim, int jx, const double *qz,
+                  const double *dzi, const double *dzim, int kx);
+void or3d_mlwsrch(double *un, double *du, double dt, const double *s, int ix, int jx, int kx);
+void or3d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix,
+                  const double *uy0, const double *uy1, int jx, const double *uz0, const double *uz1, int kx);
+void or1d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *fby, double gm, const double *row,
+                    const double *prw, const double *vxw, const double *vyw, const double *bxw, const double *byw,
+                    int ix);
+void or2d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *fby, double gm, const double *row,
+                    const double *prw, const double *vxw, const double *vyw, const double *bxw, const double *byw,
+                    int ix, int jx);
+void or3d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *frz, double *fby, double *fbz,
+                    double gm, const double *row, const double *prw, const double *vxw, const double *vyw,
+                    const double *vzw, const double *bxw, const double *byw, const double *bzw, int ix, int jx,
+                    int kx);
+void or1d_tvdminmod(const double *da, double *daw, int ix);
+void or2d_tvdminmod(int mdir, const double *da, double *daw, int ix, int jx);
+void or3d_tvdminmod(int mdir, const double *da, double *daw, int ix, int jx, int kx);
+void orh_hlld_flux(const double *row, const double *prw, const double *vxw, const double *vyw, const double *vzw,
+                   const double *bx, const double *byw, const double *bzw, double gm, int ix, int jx, int kx,
+                   double zfloor, double *fro, double *fee, double *frx, double *fry, double *frz, double *fby,
+                   double *fbz);
+void orh_hlld_mh(double *rof, double *prf, double *vxf, double *vyf, double *vzf, double *bxf, double *byf,
+                 double *bzf, double *ron, double *een, double *rxn, double *ryn, double *rzn, double *bxn,
+                 double *byn, double *bzn, double gm, double dt, const double *dx, const double *dy,
+                 const double *dz, int ix, int jx, int kx, const int *mfdim, int mstage);
+/* the cans1d main.f loop over routines passed as function pointers
+ * (cans_oracle_refloop.c) */
+int orref_loop_1d(void *cfl, void *mlw, void *bdfre, double *ro, double *pr, double *vx, double *vy, double *by,
+                  const double *bx, const double *bxm, const double *dx, const double *dxm, int ix, int margin,
+                  double gm, double safety, double qav, double dtmin, double tend, int nstop, double *t_out,
+                  int *merr_out);
+
 #ifdef __cplusplus
 }
 #endif

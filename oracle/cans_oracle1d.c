@@ -581,3 +581,21 @@ void or1d_cfl_m3(double *dt, double safety, double dtmin, int *merr, double gm,
     if (imin) *imin = im;
     free(dtq);
 }
+
+/* ---- the helper routines themselves, with the reference argument lists
+ * (the names the drop-in library exports; see cans_oracle.h) */
+void or1d_mlwhalf(const double *u, double *un, double *du, double dt, const double *f, const double *dxi,
+                  const double *dxim, int ix) {
+    mlwhalf1(u, un, du, dt, f, dxi, dxim, ix);
+}
+void or1d_mlwfull(double *du, double dt, const double *f, const double *dxi, int ix) {
+    mlwfull1(du, dt, f, dxi, ix);
+}
+void or1d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi, const double *dxim,
+                  int ix) {
+    mlwartv1(u, du, dt, qx, dxi, dxim, ix);
+}
+void or1d_mlwsrch(double *un, double *du, double dt, const double *s, int ix) { mlwsrch1(un, du, dt, s, ix); }
+void or1d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix) {
+    mlwsrcf1(du, dt, s, ux0, ux1, ix);
+}

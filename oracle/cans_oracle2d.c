@@ -886,3 +886,24 @@ void or2d_cfl_m_s(double *dt, double safety, double dtmin, int *merr, double gm,
     if (imin) *imin = im;
     if (jmin) *jmin = jm;
 }
+
+/* ---- the helper routines themselves, with the reference argument lists */
+void or2d_mlwhalf(const double *u, double *un, double *du, double dt, const double *fx, const double *dxi,
+                  const double *dxim, int ix, const double *fy, const double *dyi, const double *dyim, int jx) {
+    mlwhalf2(u, un, du, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx);
+}
+void or2d_mlwfull(double *du, double dt, const double *fx, const double *dxi, const double *ux0, const double *ux1,
+                  int ix, const double *fy, const double *dyi, const double *uy0, const double *uy1, int jx) {
+    mlwfull2(du, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx);
+}
+void or2d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi, const double *dxim,
+                  int ix, const double *qy, const double *dyi, const double *dyim, int jx) {
+    mlwartv2(u, du, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx);
+}
+void or2d_mlwsrch(double *un, double *du, double dt, const double *s, int ix, int jx) {
+    mlwsrch2(un, du, dt, s, ix, jx);
+}
+void or2d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix,
+                  const double *uy0, const double *uy1, int jx) {
+    mlwsrcf2(du, dt, s, ux0, ux1, ix, uy0, uy1, jx);
+}

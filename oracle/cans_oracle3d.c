@@ -825,3 +825,27 @@ void or3d_bd(int dir, int kind, int mbnd, int margin, double *qq, double q0,
             }
     free(line);
 }
+
+/* ---- the helper routines themselves, with the reference argument lists */
+void or3d_mlwhalf(const double *u, double *un, double *du, double dt, const double *fx, const double *dxi,
+                  const double *dxim, int ix, const double *fy, const double *dyi, const double *dyim, int jx,
+                  const double *fz, const double *dzi, const double *dzim, int kx) {
+    mlwhalf3(u, un, du, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx, fz, dzi, dzim, kx);
+}
+void or3d_mlwfull(double *du, double dt, const double *fx, const double *dxi, const double *ux0, const double *ux1,
+                  int ix, const double *fy, const double *dyi, const double *uy0, const double *uy1, int jx,
+                  const double *fz, const double *dzi, const double *uz0, const double *uz1, int kx) {
+    mlwfull3(du, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx, fz, dzi, uz0, uz1, kx);
+}
+void or3d_mlwartv(const double *u, double *du, double dt, const double *qx, const double *dxi, const double *dxim,
+                  int ix, const double *qy, const double *dyi, const double *dyim, int jx, const double *qz,
+                  const double *dzi, const double *dzim, int kx) {
+    mlwartv3(u, du, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx, qz, dzi, dzim, kx);
+}
+void or3d_mlwsrch(double *un, double *du, double dt, const double *s, int ix, int jx, int kx) {
+    mlwsrch3(un, du, dt, s, ix, jx, kx);
+}
+void or3d_mlwsrcf(double *du, double dt, const double *s, const double *ux0, const double *ux1, int ix,
+                  const double *uy0, const double *uy1, int jx, const double *uz0, const double *uz1, int kx) {
+    mlwsrcf3(du, dt, s, ux0, ux1, ix, uy0, uy1, jx, uz0, uz1, kx);
+}

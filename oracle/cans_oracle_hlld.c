@@ -264,3 +264,38 @@ void orh_engine(double *ro, double *pr, double *vx, double *vy, double *vz,
     for (int q = 0; q < P_N; q++) { free(ph[q]); free(w0[q]); free(w1[q]); }
     free(bb); free(eh);
 }
+
+/* ---- the helper routines themselves, with the reference argument lists */
+/* hdhlld/hlld_flux.f90:2-3, every face i,j,k (:141-143); w arrays (ix,jx,kx,2) */
+void orh_hlld_flux(const double *row, const double *prw, const double *vxw, const double *vyw, const double *vzw,
+                   const double *bx, const double *byw, const double *bzw, double gm, int ix, int jx, int kx,
+                   double zfloor, double *fro, double *fee, double *frx, double *fry, double *frz, double *fby,
+                   double *fbz) {
+    const size_t n = (size_t)ix * jx * kx;
+    for (size_t p = 0; p < n; p++) {
+        const size_t q = p + n;
+        double o[7];
+        hlld1(gm, zfloor, bx[p], row[p], prw[p], vxw[p], vyw[p], vzw[p], byw[p], bzw[p], row[q], prw[q], vxw[q],
+              vyw[q], vzw[q], byw[q], bzw[q], o);
+        fro[p] = o[0]; fee[p] = o[1]; frx[p] = o[2]; fry[p] = o[3]; frz[p] = o[4]; fby[p] = o[5]; fbz[p] = o[6];
+    }
+}
+/* hdhlld/hlld_mh.f:2-4: the directions with mfdim = 1, in order, on one set
+ * of automatic arrays (NaN where the reference leaves them unset) */
+void orh_hlld_mh(double *rof, double *prf, double *vxf, double *vyf, double *vzf, double *bxf, double *byf,
+                 double *bzf, double *ron, double *een, double *rxn, double *ryn, double *rzn, double *bxn,
+                 double *byn, double *bzn, double gm, double dt, const double *dx, const double *dy,
+                 const double *dz, int ix, int jx, int kx, const int *mfdim, int mstage) {
+    size_t n = (size_t)ix * jx * kx;
+    double *prim[P_N] = {rof, prf, vxf, vyf, vzf, bxf, byf, bzf};
+    double *un[U_N] = {ron, een, rxn, ryn, rzn, bxn, byn, bzn};
+    double *w0[P_N], *w1[P_N], *f[U_N];
+    for (int q = 0; q < P_N; q++) { w0[q] = dalloc(n); w1[q] = dalloc(n); }
+    for (int q = 0; q < U_N; q++) f[q] = dalloc(n);
+    double *bb = dalloc(n);
+    for (int d = 1; d <= 3; d++)
+        if (mfdim[d - 1] == 1) hlld_mh_dir(prim, un, gm, dt, dx, dy, dz, ix, jx, kx, d, mstage, w0, w1, bb, f);
+    for (int q = 0; q < P_N; q++) { free(w0[q]); free(w1[q]); }
+    for (int q = 0; q < U_N; q++) free(f[q]);
+    free(bb);
+}

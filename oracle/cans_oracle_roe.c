@@ -963,3 +963,60 @@ void or3d_roe_m(double *ro, double *pr, double *vx, double *vy, double *vz,
     for (size_t t = 0; t < sizeof(tofree) / sizeof(tofree[0]); t++) free(tofree[t]);
     for (int q = 0; q < W_N; q++) { free(w0[q]); free(w1[q]); }
 }
+
+/* ================================================================== helpers
+ * roeflux_m and tvdminmod with the reference argument lists: w arrays are
+ * (n,2) / (ix,jx,2) / (ix,jx,kx,2), the left state in the first half. */
+/* cans1d/hdroe/roeflux_m.f:2-3, faces i=1..ix-1 (:48) */
+void or1d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *fby, double gm, const double *row,
+                    const double *prw, const double *vxw, const double *vyw, const double *bxw, const double *byw,
+                    int ix) {
+    for (int i = 1; i <= ix - 1; i++) {
+        size_t p = I1(i), q = p + (size_t)ix;
+        double o[5];
+        roeflux12(gm, row[p], prw[p], vxw[p], vyw[p], bxw[p], byw[p], row[q], prw[q], vxw[q], vyw[q], bxw[q], byw[q],
+                  o);
+        fro[p] = o[0]; fee[p] = o[1]; frx[p] = o[2]; fry[p] = o[3]; fby[p] = o[4];
+    }
+}
+/* cans2d/hdroe/roeflux_m.f:1-2, faces j=1..jx-1, i=1..ix-1 (:16-17) */
+void or2d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *fby, double gm, const double *row,
+                    const double *prw, const double *vxw, const double *vyw, const double *bxw, const double *byw,
+                    int ix, int jx) {
+    const size_t n = (size_t)ix * jx;
+    for (int j = 1; j <= jx - 1; j++)
+        for (int i = 1; i <= ix - 1; i++) {
+            size_t p = I2(i, j), q = p + n;
+            double o[5];
+            roeflux12(gm, row[p], prw[p], vxw[p], vyw[p], bxw[p], byw[p], row[q], prw[q], vxw[q], vyw[q], bxw[q],
+                      byw[q], o);
+            fro[p] = o[0]; fee[p] = o[1]; frx[p] = o[2]; fry[p] = o[3]; fby[p] = o[4];
+        }
+}
+/* cans3d/hdroe/roeflux_m.f:6-7, faces k,j,i up to n-1 (:26-28) */
+void or3d_roeflux_m(double *fro, double *fee, double *frx, double *fry, double *frz, double *fby, double *fbz,
+                    double gm, const double *row, const double *prw, const double *vxw, const double *vyw,
+                    const double *vzw, const double *bxw, const double *byw, const double *bzw, int ix, int jx,
+                    int kx) {
+    const size_t n = (size_t)ix * jx * kx;
+    for (int k = 1; k <= kx - 1; k++)
+        for (int j = 1; j <= jx - 1; j++)
+            for (int i = 1; i <= ix - 1; i++) {
+                size_t p = I3(i, j, k), q = p + n;
+                double o[7];
+                roeflux3(1, gm, row[p], prw[p], vxw[p], vyw[p], vzw[p], bxw[p], byw[p], bzw[p], row[q], prw[q],
+                         vxw[q], vyw[q], vzw[q], bxw[q], byw[q], bzw[q], o);
+                fro[p] = o[0]; fee[p] = o[1]; frx[p] = o[2]; fry[p] = o[3]; frz[p] = o[4]; fby[p] = o[5];
+                fbz[p] = o[6];
+            }
+}
+/* cans1d/hdroe/tvdminmod.f:2 */
+void or1d_tvdminmod(const double *da, double *daw, int ix) { tvd1(da, daw, daw + ix, ix); }
+/* cans2d/hdroe/tvdminmod.f:1 */
+void or2d_tvdminmod(int mdir, const double *da, double *daw, int ix, int jx) {
+    tvd2(mdir, da, daw, daw + (size_t)ix * jx, ix, jx);
+}
+/* cans3d/hdroe/tvdminmod.f:4 */
+void or3d_tvdminmod(int mdir, const double *da, double *daw, int ix, int jx, int kx) {
+    tvd3(mdir, da, daw, daw + (size_t)ix * jx * kx, ix, jx, kx);
+}

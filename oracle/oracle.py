@@ -83,6 +83,20 @@ class Oracle:
             raise FileNotFoundError(f"{path} not built (make -C oracle)")
         self.lib = C.CDLL(str(path))
 
+    def call(self, name, *args):
+        """Call an exported oracle function by name with C conventions:
+        numpy float64 arrays (Fortran order) and int32 arrays by address,
+        Python floats as double, ints as int, by value."""
+        cargs = []
+        for a in args:
+            if isinstance(a, np.ndarray):
+                cargs.append(C.c_void_p(a.ctypes.data) if a.dtype == np.int32 else _arr(a))
+            elif isinstance(a, bool) or isinstance(a, (int, np.integer)):
+                cargs.append(C.c_int(int(a)))
+            else:
+                cargs.append(C.c_double(float(a)))
+        getattr(self.lib, name)(*cargs)
+
     # ---- 1D
     def mlw_m_1d(self, ro, pr, vx, vy, by, bx, bxm, dt, qav, gm, dx, dxm):
         self.lib.or1d_mlw_m(*map(_arr, (ro, pr, vx, vy, by, bx, bxm)), C.c_double(dt), C.c_double(qav),

@@ -37,11 +37,13 @@ def test_library_exports_every_declared_symbol():
 
 def test_fortran_dropin_libraries_export_reference_names():
     from utrip_amd import PKG_DIR
+    helpers = ["mlwhalf_", "mlwfull_", "mlwartv_", "mlwsrch_", "mlwsrcf_", "roeflux_m_", "tvdminmod_"]
     want = {
         "1d": ["mlw_m_", "roe_m_", "cfl_m_", "bdfrex_", "bdperx_", "bdsppx_", "bdspnx_", "bdsmpx_", "bdsmnx_",
-               "bdcnsx_", "bdfrdx_"],
-        "2d": ["mlw_m_", "roe_m_", "cfl_m_", "bdperx_", "bdpery_", "bdfrex_", "bdfrey_"],
-        "3d": ["mlw_m_", "roe_m_", "cfl_m_", "bdperx_", "bdpery_", "bdperz_", "engine_"],
+               "bdcnsx_", "bdfrdx_"] + helpers,
+        "2d": ["mlw_m_", "roe_m_", "cfl_m_", "bdperx_", "bdpery_", "bdfrex_", "bdfrey_"] + helpers,
+        "3d": ["mlw_m_", "roe_m_", "cfl_m_", "bdperx_", "bdpery_", "bdperz_", "engine_", "hlld_mh_",
+               "hlld_flux_"] + helpers,
     }
     found = False
     for dim, names in want.items():

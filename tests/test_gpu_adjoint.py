@@ -158,3 +158,42 @@ def test_adjoint_ot512_200_steps(dev):
     d = g/|g| the plain relative error is checked."""
     case("OT 512^2, 200 steps, checkpoints every 20, 16 directions, qav 3")
     fd_check(512, 200, 20, 16, dev, norm="g")
+
+
+def test_adjoint_matches_cpu_reference_gradient(dev):
+    """The whole trajectory (OT 24^2, 12 steps, checkpoints every 5) against
+    an FP64 CPU reference gradient: autograd of the vectorised torch step
+    (tests/torch_mlw2d.py) run on CPU tensors for the same 12 steps, from the
+    same initial velocities and pressure scale, with the same target.  The
+    hand-written GPU adjoint must give dL/dvx0, dL/dvy0 and dL/dbeta to 1e-10
+    relative (max abs difference over max abs value)."""
+    from tests import torch_mlw2d as T
+    from utrip_amd import adjoint
+    case("OT 24^2, 12 steps vs CPU autograd of the torch step")
+    n, nsteps = 24, 12
+    solver, base, vx0, vy0, beta, target, w = setup(n, nsteps, 5, dev)
+    x = [vx0.clone().requires_grad_(True), vy0.clone().requires_grad_(True), beta.clone().requires_grad_(True)]
+    L = loss(*x, solver, base, target, w)
+    L.backward()
+    g_gpu = [t.grad.detach().cpu() for t in x]
+    # the CPU reference
+    cpu = torch.device("cpu")
+    dx, dxm, dy, dym = (t.to(cpu) for t in solver.grid)
+    b = [t.to(cpu) for t in base]
+    xc = [t.detach().to(cpu).clone().requires_grad_(True) for t in (vx0, vy0, beta)]
+    m = 1
+    vx = b[2].clone()
+    vy = b[3].clone()
+    vx = torch.cat([vx[:m], torch.cat([vx[m:-m, :m], xc[0], vx[m:-m, -m:]], dim=1), vx[-m:]], dim=0)
+    vy = torch.cat([vy[:m], torch.cat([vy[m:-m, :m], xc[1], vy[m:-m, -m:]], dim=1), vy[-m:]], dim=0)
+    P = [b[0], b[1] * xc[2], T._fill(vx), T._fill(vy), b[4], b[5]]
+    for _ in range(nsteps):
+        P, _dt = T.step(P, dx, dxm, dy, dym, solver.gm, solver.qav, solver.safety)
+    Lc = ((P[0][m:-m, m:-m] - target.to(cpu)) ** 2).sum() * w
+    Lc.backward()
+    print("L gpu", float(L), "L cpu", float(Lc), flush=True)
+    assert abs(float(Lc) - float(L)) <= 1e-10 * abs(float(L))
+    for name, a, c in zip(("vx0", "vy0", "beta"), g_gpu, (t.grad for t in xc)):
+        r = float((a - c).abs().max() / c.abs().max())
+        print(f"d L / d {name}: max rel diff {r:.2e}", flush=True)
+        assert r <= 1e-10, (name, r)

@@ -19,6 +19,7 @@
 #include "cans_roe.hpp"
 #include "cans_roeflux.hpp"
 #include "cans_wave.hpp"
+#include "../../include/cans_gpu.h"
 
 namespace cans {
 namespace {
@@ -449,3 +450,170 @@ hipError_t launch_hlld(const HlldArgs &a, bool mixed, hipStream_t s) {
 }
 
 }  // namespace cans
+
+// ======================================================================
+// hlld_flux and hlld_mh with their own reference argument lists, for callers
+// that compose an engine from them (the fused engine above does not use
+// these).  One thread per array element.
+namespace cans {
+int abi_fail(int code, const char *fmt, ...);
+int abi_scratch(size_t bytes, void **out, hipStream_t s);
+namespace {
+
+// hdhlld/hlld_flux.f90:141-143: every face i,j,k; w arrays hold the left
+// state, then the right one, n apart
+__global__ void hlld_flux_k(const double *row, const double *prw, const double *vxw, const double *vyw,
+                            const double *vzw, const double *bx, const double *byw, const double *bzw, double gm,
+                            double zfloor, double *fro, double *fee, double *frx, double *fry, double *frz,
+                            double *fby, double *fbz, size_t n) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const size_t q = p + n;
+    double o[7];
+    hlld_face<double>(gm, zfloor, bx[p], row[p], prw[p], vxw[p], vyw[p], vzw[p], byw[p], bzw[p], row[q], prw[q],
+                      vxw[q], vyw[q], vzw[q], byw[q], bzw[q], o);
+    fro[p] = o[0]; fee[p] = o[1]; frx[p] = o[2]; fry[p] = o[3]; frz[p] = o[4]; fby[p] = o[5]; fbz[p] = o[6];
+}
+
+__global__ void nan_fill_k(double *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = __builtin_nan("");
+}
+
+struct MhDir {
+    const double *prim[8];   // rof..bzf
+    double *un[8];           // ron..bzn
+    double *w0[8], *w1[8];   // interpolated left/right states
+    double *bb;              // face-normal field
+    double *f[8];            // fluxes (ro ee rx ry rz bx by bz)
+    const double *dd;        // dx, dy or dz
+    double gm, dt;
+    int ix, jx, kx, mdir, mstage;
+};
+
+CANS_DEV int along(const MhDir &a, int i, int j, int k) { return a.mdir == 1 ? i : (a.mdir == 2 ? j : k); }
+CANS_DEV size_t stride(const MhDir &a) {
+    return a.mdir == 1 ? 1 : (a.mdir == 2 ? (size_t)a.ix : (size_t)a.ix * a.jx);
+}
+
+// interpolation/inp_const.f (mstage 1) and inp_tvdminmod.f (mstage 2, with the
+// constant faces at 1 and n-1), then the face-normal field of hlld_mh.f:43-95
+__global__ void mh_interp_k(MhDir a) {
+    const size_t n = (size_t)a.ix * a.jx * a.kx;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = (int)(p % (size_t)a.ix) + 1, j = (int)((p / (size_t)a.ix) % (size_t)a.jx) + 1;
+    const int k = (int)(p / ((size_t)a.ix * a.jx)) + 1;
+    const int x = along(a, i, j, k), nd = a.mdir == 1 ? a.ix : (a.mdir == 2 ? a.jx : a.kx);
+    if (x > nd - 1) return;
+    const size_t s = stride(a);
+    for (int q = 0; q < 8; q++) {
+        const double *da = a.prim[q];
+        if (a.mstage == 1 || x == 1 || x == nd - 1) {
+            a.w0[q][p] = da[p];
+            a.w1[q][p] = da[p + s];
+        } else {
+            a.w0[q][p] = da[p] + 0.5 * flmtH(da[p] - da[p - s], da[p + s] - da[p]);
+            a.w1[q][p] = da[p + s] - 0.5 * flmtH(da[p + 2 * s] - da[p + s], da[p + s] - da[p]);
+        }
+    }
+    const double *bn = a.prim[a.mdir == 1 ? 5 : (a.mdir == 2 ? 6 : 7)];
+    a.bb[p] = (bn[p] + bn[p + s]) / 2.0;
+}
+
+// hlld_flux over every face in the sweep frame, then the normal-field flux
+// zeroed on i = 1..ix-1 (hlld_mh.f:56-94)
+__global__ void mh_flux_k(MhDir a) {
+    const size_t n = (size_t)a.ix * a.jx * a.kx;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int vn, v1, v2, b1, b2, fn, f1, f2, g1, g2, gn;
+    if (a.mdir == 1) { vn = 2; v1 = 3; v2 = 4; b1 = 6; b2 = 7; fn = 2; f1 = 3; f2 = 4; g1 = 6; g2 = 7; gn = 5; }
+    else if (a.mdir == 2) { vn = 3; v1 = 4; v2 = 2; b1 = 7; b2 = 5; fn = 3; f1 = 4; f2 = 2; g1 = 7; g2 = 5; gn = 6; }
+    else { vn = 4; v1 = 2; v2 = 3; b1 = 5; b2 = 6; fn = 4; f1 = 2; f2 = 3; g1 = 5; g2 = 6; gn = 7; }
+    double o[7];
+    hlld_face<double>(a.gm, 1.0e-10, a.bb[p], a.w0[0][p], a.w0[1][p], a.w0[vn][p], a.w0[v1][p], a.w0[v2][p],
+                      a.w0[b1][p], a.w0[b2][p], a.w1[0][p], a.w1[1][p], a.w1[vn][p], a.w1[v1][p], a.w1[v2][p],
+                      a.w1[b1][p], a.w1[b2][p], o);
+    a.f[0][p] = o[0]; a.f[1][p] = o[1]; a.f[fn][p] = o[2]; a.f[f1][p] = o[3]; a.f[f2][p] = o[4];
+    a.f[g1][p] = o[5]; a.f[g2][p] = o[6];
+    const int i = (int)(p % (size_t)a.ix) + 1;
+    if (i <= a.ix - 1) a.f[gn][p] = 0.0;
+}
+
+// common/adddfdx.f
+__global__ void mh_add_k(MhDir a) {
+    const size_t n = (size_t)a.ix * a.jx * a.kx;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = (int)(p % (size_t)a.ix) + 1, j = (int)((p / (size_t)a.ix) % (size_t)a.jx) + 1;
+    const int k = (int)(p / ((size_t)a.ix * a.jx)) + 1;
+    const int x = along(a, i, j, k), nd = a.mdir == 1 ? a.ix : (a.mdir == 2 ? a.jx : a.kx);
+    if (x < 2 || x > nd - 1) return;
+    const size_t s = stride(a);
+    const double d = a.dd[x - 1];
+    for (int q = 0; q < 8; q++) a.un[q][p] = a.un[q][p] - a.dt * ((a.f[q][p] - a.f[q][p - s]) / d);
+}
+
+}  // namespace
+}  // namespace cans
+
+extern "C" {
+
+int cans3d_hlld_flux(const double *row, const double *prw, const double *vxw, const double *vyw, const double *vzw,
+                     const double *bx, const double *byw, const double *bzw, double gm, int ix, int jx, int kx,
+                     double zfloor, double *fro, double *fee, double *frx, double *fry, double *frz, double *fby,
+                     double *fbz, void *stream) {
+    using namespace cans;
+    if (ix < 1 || jx < 1 || kx < 1) return abi_fail(CANS_ERR_ARG, "cans3d_hlld_flux: ix=%d jx=%d kx=%d", ix, jx, kx);
+    const size_t n = (size_t)ix * jx * kx;
+    hipLaunchKernelGGL(hlld_flux_k, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, row, prw,
+                       vxw, vyw, vzw, bx, byw, bzw, gm, zfloor, fro, fee, frx, fry, frz, fby, fbz, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return abi_fail(CANS_ERR_HIP, "cans3d_hlld_flux: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int cans3d_hlld_mh(const double *rof, const double *prf, const double *vxf, const double *vyf, const double *vzf,
+                   const double *bxf, const double *byf, const double *bzf, double *ron, double *een, double *rxn,
+                   double *ryn, double *rzn, double *bxn, double *byn, double *bzn, double gm, double dt,
+                   const double *dx, const double *dy, const double *dz, int ix, int jx, int kx, const int *mfdim,
+                   int mstage, void *stream) {
+    using namespace cans;
+    if (ix < 1 || jx < 1 || kx < 1 || !mfdim || (mstage != 1 && mstage != 2))
+        return abi_fail(CANS_ERR_ARG, "cans3d_hlld_mh: ix=%d jx=%d kx=%d mstage=%d", ix, jx, kx, mstage);
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)ix * jx * kx;
+    // the reference's automatic arrays (w0, w1, bb, f), NaN where it leaves
+    // them unset; kept across the directions of the call as the reference's are
+    void *w = nullptr;
+    if (int rc = abi_scratch(25 * n * sizeof(double), &w, s)) return rc;
+    double *ws = (double *)w;
+    hipLaunchKernelGGL(nan_fill_k, dim3(1024), dim3(256), 0, s, ws, 25 * n);
+    MhDir a{};
+    const double *prim[8] = {rof, prf, vxf, vyf, vzf, bxf, byf, bzf};
+    double *un[8] = {ron, een, rxn, ryn, rzn, bxn, byn, bzn};
+    for (int q = 0; q < 8; q++) {
+        a.prim[q] = prim[q];
+        a.un[q] = un[q];
+        a.w0[q] = ws + q * n;
+        a.w1[q] = ws + (8 + q) * n;
+        a.f[q] = ws + (17 + q) * n;
+    }
+    a.bb = ws + 16 * n;
+    a.gm = gm; a.dt = dt; a.ix = ix; a.jx = jx; a.kx = kx; a.mstage = mstage;
+    const dim3 g((unsigned)((n + 255) / 256));
+    for (int d = 1; d <= 3; d++) {
+        if (mfdim[d - 1] != 1) continue;
+        a.mdir = d;
+        a.dd = d == 1 ? dx : (d == 2 ? dy : dz);
+        hipLaunchKernelGGL(mh_interp_k, g, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mh_flux_k, g, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(mh_add_k, g, dim3(256), 0, s, a);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return abi_fail(CANS_ERR_HIP, "cans3d_hlld_mh: %s", hipGetErrorString(e));
+    return 0;
+}
+
+}  // extern "C"

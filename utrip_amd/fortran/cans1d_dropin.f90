@@ -244,3 +244,119 @@ subroutine bdfrdx(mbnd, margin, qq, dxm, ix)
        int(ix, c_int), c_null_ptr), 'bdfrdx')
   call dunmap(qq, 1); call dunmap(dxm, 0)
 end subroutine bdfrdx
+
+!=======================================================================
+! the helper routines the engines are built from, with the reference argument
+! lists (device counterparts; the engines above do not call them)
+!=======================================================================
+! cans1d/hdmlw/mlwhalf.f:2
+subroutine mlwhalf(u, un, du, dt, f, dxi, dxim, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target, intent(in) :: u(*)
+  double precision, target :: un(*)
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: f(*)
+  double precision, target, intent(in) :: dxi(*)
+  double precision, target, intent(in) :: dxim(*)
+  call check(c_mlwhalf1d(dmap(u, ix), dmap(un, ix), dmap(du, ix), dt, dmap(f, ix), dmap(dxi, ix), dmap(dxim, ix), &
+       int(ix, c_int), c_null_ptr), 'mlwhalf')
+  call dunmap(u, 0); call dunmap(un, 1); call dunmap(du, 1); call dunmap(f, 0)
+  call dunmap(dxi, 0); call dunmap(dxim, 0)
+end subroutine mlwhalf
+
+! cans1d/hdmlw/mlwfull.f:2
+subroutine mlwfull(du, dt, f, dxi, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: f(*)
+  double precision, target, intent(in) :: dxi(*)
+  call check(c_mlwfull1d(dmap(du, ix), dt, dmap(f, ix), dmap(dxi, ix), int(ix, c_int), c_null_ptr), 'mlwfull')
+  call dunmap(du, 1); call dunmap(f, 0); call dunmap(dxi, 0)
+end subroutine mlwfull
+
+! cans1d/hdmlw/mlwartv.f:2
+subroutine mlwartv(u, du, dt, qx, dxi, dxim, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target, intent(in) :: u(*)
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: qx(*)
+  double precision, target, intent(in) :: dxi(*)
+  double precision, target, intent(in) :: dxim(*)
+  call check(c_mlwartv1d(dmap(u, ix), dmap(du, ix), dt, dmap(qx, ix), dmap(dxi, ix), dmap(dxim, ix), int(ix, c_int), &
+       c_null_ptr), 'mlwartv')
+  call dunmap(u, 0); call dunmap(du, 1); call dunmap(qx, 0); call dunmap(dxi, 0)
+  call dunmap(dxim, 0)
+end subroutine mlwartv
+
+! cans1d/hdmlw/mlwsrch.f:2
+subroutine mlwsrch(un, du, dt, s, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: un(*)
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: s(*)
+  call check(c_mlwsrch1d(dmap(un, ix), dmap(du, ix), dt, dmap(s, ix), int(ix, c_int), c_null_ptr), 'mlwsrch')
+  call dunmap(un, 1); call dunmap(du, 1); call dunmap(s, 0)
+end subroutine mlwsrch
+
+! cans1d/hdmlw/mlwsrcf.f:2
+subroutine mlwsrcf(du, dt, s, ux0, ux1, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: s(*)
+  double precision, target, intent(in) :: ux0(*)
+  double precision, target, intent(in) :: ux1(*)
+  call check(c_mlwsrcf1d(dmap(du, ix), dt, dmap(s, ix), dmap(ux0, ix), dmap(ux1, ix), int(ix, c_int), c_null_ptr), 'mlwsrcf')
+  call dunmap(du, 1); call dunmap(s, 0); call dunmap(ux0, 0); call dunmap(ux1, 0)
+end subroutine mlwsrcf
+
+! cans1d/hdroe/roeflux_m.f:2-3
+subroutine roeflux_m(fro, fee, frx, fry, fby, gm, row, prw, vxw, vyw, bxw, byw, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target :: fro(*)
+  double precision, target :: fee(*)
+  double precision, target :: frx(*)
+  double precision, target :: fry(*)
+  double precision, target :: fby(*)
+  double precision, intent(in) :: gm
+  double precision, target, intent(in) :: row(*)
+  double precision, target, intent(in) :: prw(*)
+  double precision, target, intent(in) :: vxw(*)
+  double precision, target, intent(in) :: vyw(*)
+  double precision, target, intent(in) :: bxw(*)
+  double precision, target, intent(in) :: byw(*)
+  call check(c_roeflux_m1d(dmap(fro, ix), dmap(fee, ix), dmap(frx, ix), dmap(fry, ix), dmap(fby, ix), gm, dmap(row, 2*ix), &
+       dmap(prw, 2*ix), dmap(vxw, 2*ix), dmap(vyw, 2*ix), dmap(bxw, 2*ix), dmap(byw, 2*ix), int(ix, c_int), &
+       c_null_ptr), 'roeflux_m')
+  call dunmap(fro, 1); call dunmap(fee, 1); call dunmap(frx, 1); call dunmap(fry, 1)
+  call dunmap(fby, 1); call dunmap(row, 0); call dunmap(prw, 0); call dunmap(vxw, 0)
+  call dunmap(vyw, 0); call dunmap(bxw, 0); call dunmap(byw, 0)
+end subroutine roeflux_m
+
+! cans1d/hdroe/tvdminmod.f:2
+subroutine tvdminmod(da, daw, ix)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix
+  double precision, target, intent(in) :: da(*)
+  double precision, target :: daw(*)
+  call check(c_tvdminmod1d(dmap(da, ix), dmap(daw, 2*ix), int(ix, c_int), c_null_ptr), 'tvdminmod')
+  call dunmap(da, 0); call dunmap(daw, 1)
+end subroutine tvdminmod
+

@@ -289,3 +289,153 @@ subroutine bdfrdy(mbnd, margin, qq, dym, ix, jx)
   double precision qq(ix, jx), dym(*)
   call bnd2d_frd(1, mbnd, margin, qq, dym, jx, ix, jx)
 end subroutine bdfrdy
+
+!=======================================================================
+! the helper routines the engines are built from, with the reference argument
+! lists (device counterparts; the engines above do not call them)
+!=======================================================================
+! cans2d/hdmlw/mlwhalf.f:2
+subroutine mlwhalf(u, un, du, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target, intent(in) :: u(*)
+  double precision, target :: un(*)
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: fx(*)
+  double precision, target, intent(in) :: dxi(*)
+  double precision, target, intent(in) :: dxim(*)
+  double precision, target, intent(in) :: fy(*)
+  double precision, target, intent(in) :: dyi(*)
+  double precision, target, intent(in) :: dyim(*)
+  integer :: n
+  n = ix * jx
+  call check(c_mlwhalf2d(dmap(u, n), dmap(un, n), dmap(du, n), dt, dmap(fx, n), dmap(dxi, ix), dmap(dxim, ix), &
+       int(ix, c_int), dmap(fy, n), dmap(dyi, jx), dmap(dyim, jx), int(jx, c_int), c_null_ptr), 'mlwhalf')
+  call dunmap(u, 0); call dunmap(un, 1); call dunmap(du, 1); call dunmap(fx, 0)
+  call dunmap(dxi, 0); call dunmap(dxim, 0); call dunmap(fy, 0); call dunmap(dyi, 0)
+  call dunmap(dyim, 0)
+end subroutine mlwhalf
+
+! cans2d/hdmlw/mlwfull.f:2
+subroutine mlwfull(du, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: fx(*)
+  double precision, target, intent(in) :: dxi(*)
+  double precision, target, intent(in) :: ux0(*)
+  double precision, target, intent(in) :: ux1(*)
+  double precision, target, intent(in) :: fy(*)
+  double precision, target, intent(in) :: dyi(*)
+  double precision, target, intent(in) :: uy0(*)
+  double precision, target, intent(in) :: uy1(*)
+  integer :: n
+  n = ix * jx
+  call check(c_mlwfull2d(dmap(du, n), dt, dmap(fx, n), dmap(dxi, ix), dmap(ux0, ix), dmap(ux1, ix), int(ix, c_int), &
+       dmap(fy, n), dmap(dyi, jx), dmap(uy0, jx), dmap(uy1, jx), int(jx, c_int), c_null_ptr), 'mlwfull')
+  call dunmap(du, 1); call dunmap(fx, 0); call dunmap(dxi, 0); call dunmap(ux0, 0)
+  call dunmap(ux1, 0); call dunmap(fy, 0); call dunmap(dyi, 0); call dunmap(uy0, 0)
+  call dunmap(uy1, 0)
+end subroutine mlwfull
+
+! cans2d/hdmlw/mlwartv.f:2
+subroutine mlwartv(u, du, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target, intent(in) :: u(*)
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: qx(*)
+  double precision, target, intent(in) :: dxi(*)
+  double precision, target, intent(in) :: dxim(*)
+  double precision, target, intent(in) :: qy(*)
+  double precision, target, intent(in) :: dyi(*)
+  double precision, target, intent(in) :: dyim(*)
+  integer :: n
+  n = ix * jx
+  call check(c_mlwartv2d(dmap(u, n), dmap(du, n), dt, dmap(qx, n), dmap(dxi, ix), dmap(dxim, ix), int(ix, c_int), &
+       dmap(qy, n), dmap(dyi, jx), dmap(dyim, jx), int(jx, c_int), c_null_ptr), 'mlwartv')
+  call dunmap(u, 0); call dunmap(du, 1); call dunmap(qx, 0); call dunmap(dxi, 0)
+  call dunmap(dxim, 0); call dunmap(qy, 0); call dunmap(dyi, 0); call dunmap(dyim, 0)
+end subroutine mlwartv
+
+! cans2d/hdmlw/mlwsrch.f:2
+subroutine mlwsrch(un, du, dt, s, ix, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target :: un(*)
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: s(*)
+  integer :: n
+  n = ix * jx
+  call check(c_mlwsrch2d(dmap(un, n), dmap(du, n), dt, dmap(s, n), int(ix, c_int), int(jx, c_int), c_null_ptr), 'mlwsrch')
+  call dunmap(un, 1); call dunmap(du, 1); call dunmap(s, 0)
+end subroutine mlwsrch
+
+! cans2d/hdmlw/mlwsrcf.f:2
+subroutine mlwsrcf(du, dt, s, ux0, ux1, ix, uy0, uy1, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target :: du(*)
+  double precision, intent(in) :: dt
+  double precision, target, intent(in) :: s(*)
+  double precision, target, intent(in) :: ux0(*)
+  double precision, target, intent(in) :: ux1(*)
+  double precision, target, intent(in) :: uy0(*)
+  double precision, target, intent(in) :: uy1(*)
+  integer :: n
+  n = ix * jx
+  call check(c_mlwsrcf2d(dmap(du, n), dt, dmap(s, n), dmap(ux0, ix), dmap(ux1, ix), int(ix, c_int), dmap(uy0, jx), &
+       dmap(uy1, jx), int(jx, c_int), c_null_ptr), 'mlwsrcf')
+  call dunmap(du, 1); call dunmap(s, 0); call dunmap(ux0, 0); call dunmap(ux1, 0)
+  call dunmap(uy0, 0); call dunmap(uy1, 0)
+end subroutine mlwsrcf
+
+! cans2d/hdroe/roeflux_m.f:1-2
+subroutine roeflux_m(fro, fee, frx, fry, fby, gm, row, prw, vxw, vyw, bxw, byw, ix, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: ix, jx
+  double precision, target :: fro(*)
+  double precision, target :: fee(*)
+  double precision, target :: frx(*)
+  double precision, target :: fry(*)
+  double precision, target :: fby(*)
+  double precision, intent(in) :: gm
+  double precision, target, intent(in) :: row(*)
+  double precision, target, intent(in) :: prw(*)
+  double precision, target, intent(in) :: vxw(*)
+  double precision, target, intent(in) :: vyw(*)
+  double precision, target, intent(in) :: bxw(*)
+  double precision, target, intent(in) :: byw(*)
+  integer :: n
+  n = ix * jx
+  call check(c_roeflux_m2d(dmap(fro, n), dmap(fee, n), dmap(frx, n), dmap(fry, n), dmap(fby, n), gm, dmap(row, 2*n), &
+       dmap(prw, 2*n), dmap(vxw, 2*n), dmap(vyw, 2*n), dmap(bxw, 2*n), dmap(byw, 2*n), int(ix, c_int), &
+       int(jx, c_int), c_null_ptr), 'roeflux_m')
+  call dunmap(fro, 1); call dunmap(fee, 1); call dunmap(frx, 1); call dunmap(fry, 1)
+  call dunmap(fby, 1); call dunmap(row, 0); call dunmap(prw, 0); call dunmap(vxw, 0)
+  call dunmap(vyw, 0); call dunmap(bxw, 0); call dunmap(byw, 0)
+end subroutine roeflux_m
+
+! cans2d/hdroe/tvdminmod.f:1
+subroutine tvdminmod(mdir, da, daw, ix, jx)
+  use cans_gpu_iface
+  implicit none
+  integer, intent(in) :: mdir, ix, jx
+  double precision, target, intent(in) :: da(*)
+  double precision, target :: daw(*)
+  integer :: n
+  n = ix * jx
+  call check(c_tvdminmod2d(int(mdir, c_int), dmap(da, n), dmap(daw, 2*n), int(ix, c_int), int(jx, c_int), c_null_ptr), 'tvdminmod')
+  call dunmap(da, 0); call dunmap(daw, 1)
+end subroutine tvdminmod
+

@@ -262,6 +262,165 @@ module cans_gpu_iface
        real(c_double), value :: gm, mu, dt
        integer(c_int), value :: ix, jx, kx
      end function c_hlld
+     ! ---------------------------------------------------------- helper routines
+     integer(c_int) function c_mlwhalf1d(u, un, du, dt, f, dxi, dxim, ix, strm) &
+          bind(C, name="cans1d_mlwhalf")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: u, un, du, f, dxi, dxim, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix
+     end function c_mlwhalf1d
+     integer(c_int) function c_mlwfull1d(du, dt, f, dxi, ix, strm) &
+          bind(C, name="cans1d_mlwfull")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: du, f, dxi, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix
+     end function c_mlwfull1d
+     integer(c_int) function c_mlwartv1d(u, du, dt, qx, dxi, dxim, ix, strm) &
+          bind(C, name="cans1d_mlwartv")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: u, du, qx, dxi, dxim, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix
+     end function c_mlwartv1d
+     integer(c_int) function c_mlwsrch1d(un, du, dt, s, ix, strm) &
+          bind(C, name="cans1d_mlwsrch")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: un, du, s, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix
+     end function c_mlwsrch1d
+     integer(c_int) function c_mlwsrcf1d(du, dt, s, ux0, ux1, ix, strm) &
+          bind(C, name="cans1d_mlwsrcf")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: du, s, ux0, ux1, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix
+     end function c_mlwsrcf1d
+     integer(c_int) function c_roeflux_m1d(fro, fee, frx, fry, fby, gm, row, prw, vxw, vyw, bxw, byw, ix, strm) &
+          bind(C, name="cans1d_roeflux_m")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: fro, fee, frx, fry, fby, row, prw, vxw, vyw, bxw, byw, strm
+       real(c_double), value :: gm
+       integer(c_int), value :: ix
+     end function c_roeflux_m1d
+     integer(c_int) function c_tvdminmod1d(da, daw, ix, strm) &
+          bind(C, name="cans1d_tvdminmod")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: da, daw, strm
+       integer(c_int), value :: ix
+     end function c_tvdminmod1d
+     integer(c_int) function c_mlwhalf2d(u, un, du, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx, strm) &
+          bind(C, name="cans2d_mlwhalf")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: u, un, du, fx, dxi, dxim, fy, dyi, dyim, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx
+     end function c_mlwhalf2d
+     integer(c_int) function c_mlwfull2d(du, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx, strm) &
+          bind(C, name="cans2d_mlwfull")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: du, fx, dxi, ux0, ux1, fy, dyi, uy0, uy1, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx
+     end function c_mlwfull2d
+     integer(c_int) function c_mlwartv2d(u, du, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx, strm) &
+          bind(C, name="cans2d_mlwartv")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: u, du, qx, dxi, dxim, qy, dyi, dyim, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx
+     end function c_mlwartv2d
+     integer(c_int) function c_mlwsrch2d(un, du, dt, s, ix, jx, strm) &
+          bind(C, name="cans2d_mlwsrch")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: un, du, s, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx
+     end function c_mlwsrch2d
+     integer(c_int) function c_mlwsrcf2d(du, dt, s, ux0, ux1, ix, uy0, uy1, jx, strm) &
+          bind(C, name="cans2d_mlwsrcf")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: du, s, ux0, ux1, uy0, uy1, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx
+     end function c_mlwsrcf2d
+     integer(c_int) function c_roeflux_m2d(fro, fee, frx, fry, fby, gm, row, prw, vxw, vyw, bxw, byw, ix, jx, strm) &
+          bind(C, name="cans2d_roeflux_m")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: fro, fee, frx, fry, fby, row, prw, vxw, vyw, bxw, byw, strm
+       real(c_double), value :: gm
+       integer(c_int), value :: ix, jx
+     end function c_roeflux_m2d
+     integer(c_int) function c_tvdminmod2d(mdir, da, daw, ix, jx, strm) &
+          bind(C, name="cans2d_tvdminmod")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: da, daw, strm
+       integer(c_int), value :: mdir, ix, jx
+     end function c_tvdminmod2d
+     integer(c_int) function c_mlwhalf3d(u, un, du, dt, fx, dxi, dxim, ix, fy, dyi, dyim, jx, fz, dzi, dzim, kx, strm) &
+          bind(C, name="cans3d_mlwhalf")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: u, un, du, fx, dxi, dxim, fy, dyi, dyim, fz, dzi, dzim, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx, kx
+     end function c_mlwhalf3d
+     integer(c_int) function c_mlwfull3d(du, dt, fx, dxi, ux0, ux1, ix, fy, dyi, uy0, uy1, jx, fz, dzi, uz0, uz1, kx, strm) &
+          bind(C, name="cans3d_mlwfull")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: du, fx, dxi, ux0, ux1, fy, dyi, uy0, uy1, fz, dzi, uz0, uz1, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx, kx
+     end function c_mlwfull3d
+     integer(c_int) function c_mlwartv3d(u, du, dt, qx, dxi, dxim, ix, qy, dyi, dyim, jx, qz, dzi, dzim, kx, strm) &
+          bind(C, name="cans3d_mlwartv")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: u, du, qx, dxi, dxim, qy, dyi, dyim, qz, dzi, dzim, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx, kx
+     end function c_mlwartv3d
+     integer(c_int) function c_mlwsrch3d(un, du, dt, s, ix, jx, kx, strm) &
+          bind(C, name="cans3d_mlwsrch")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: un, du, s, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx, kx
+     end function c_mlwsrch3d
+     integer(c_int) function c_mlwsrcf3d(du, dt, s, ux0, ux1, ix, uy0, uy1, jx, uz0, uz1, kx, strm) &
+          bind(C, name="cans3d_mlwsrcf")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: du, s, ux0, ux1, uy0, uy1, uz0, uz1, strm
+       real(c_double), value :: dt
+       integer(c_int), value :: ix, jx, kx
+     end function c_mlwsrcf3d
+     integer(c_int) function c_roeflux_m3d(fro, fee, frx, fry, frz, fby, fbz, gm, row, prw, vxw, vyw, vzw, bxw, byw, bzw, ix, jx, kx, strm) &
+          bind(C, name="cans3d_roeflux_m")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: fro, fee, frx, fry, frz, fby, fbz, row, prw, vxw, vyw, vzw, bxw, byw, bzw, strm
+       real(c_double), value :: gm
+       integer(c_int), value :: ix, jx, kx
+     end function c_roeflux_m3d
+     integer(c_int) function c_tvdminmod3d(mdir, da, daw, ix, jx, kx, strm) &
+          bind(C, name="cans3d_tvdminmod")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: da, daw, strm
+       integer(c_int), value :: mdir, ix, jx, kx
+     end function c_tvdminmod3d
+     integer(c_int) function c_hlld_flux3d(row, prw, vxw, vyw, vzw, bx, byw, bzw, gm, ix, jx, kx, zfloor, fro, fee, frx, fry, frz, fby, fbz, strm) &
+          bind(C, name="cans3d_hlld_flux")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: row, prw, vxw, vyw, vzw, bx, byw, bzw, fro, fee, frx, fry, frz, fby, fbz, strm
+       real(c_double), value :: gm, zfloor
+       integer(c_int), value :: ix, jx, kx
+     end function c_hlld_flux3d
+     integer(c_int) function c_hlld_mh3d(rof, prf, vxf, vyf, vzf, bxf, byf, bzf, ron, een, rxn, ryn, rzn, bxn, byn, bzn, gm, dt, dx, dy, dz, ix, jx, kx, mfdim, mstage, strm) &
+          bind(C, name="cans3d_hlld_mh")
+       import :: c_ptr, c_double, c_int
+       type(c_ptr), value :: rof, prf, vxf, vyf, vzf, bxf, byf, bzf, ron, een, rxn, ryn, rzn, bxn, byn, bzn, dx, dy, dz, mfdim, strm
+       real(c_double), value :: gm, dt
+       integer(c_int), value :: ix, jx, kx, mstage
+     end function c_hlld_mh3d
   end interface
 
 contains
