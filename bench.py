@@ -29,6 +29,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "zone-updates/s on 1 MI355X (FP64 plain vs FP32-mixed/batched), %HBM peak, adj/fwd"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (FMA counted as two operations)
 # compulsory HBM traffic of one 2D zone-update: 7 FP64 fields read + 7 written
 B_ALG_2D = 112
 
@@ -49,6 +50,7 @@ def parse():
                    help="steps after which the FP32-flux vs FP64 L1 per field is taken")
     p.add_argument("--c2-n", type=int, default=384, help="interior cells per side of the 3D blast")
     p.add_argument("--tubes", type=int, default=4096, help="batched 1D tubes to t=0.1 (0: skip)")
+    p.add_argument("--runs2d", type=int, default=1024, help="batched uneven 2D runs to t=0.2 (0: skip)")
     p.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint timing")
     p.add_argument("--spicule-runs", type=int, default=1024,
                    help="realizations of the UTRIP spicule ensemble figure (0: skip the spicule figures)")
@@ -70,6 +72,20 @@ def copy_calibration(mib: int):
         torch.mul(src, 1.0, out=dst)   # an elementwise kernel: 8 B read + 8 B written per element
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "dispatches": 5}), flush=True)
+
+
+def profile_value(name: str, key: str):
+    """A KEY=value line of a committed profile summary (None if absent)."""
+    f = ROOT / "profiles" / name
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith(key + "="):
+            try:
+                return float(line.split("=", 1)[1])
+            except ValueError:
+                return None
+    return None
 
 
 def traffic_from_profile(name: str):
@@ -196,8 +212,11 @@ def blast3d_figures(n: int, steps: int, l1_steps: int = 500, early: int = 100):
     dev = torch.device("cuda:0")
     F3 = cans.FIELDS[3]
     out = {"cells": n ** 3, "steps": steps, "warmup": 2, "l1_after_steps": l1_steps}
+    with_l1 = l1_steps > max(early, steps + 2)
     variants = (("mlw", "fp64"), ("roe", "fp64"), ("roe", "fp32flux"), ("roe", "fp64_perturbed"),
                 ("hlld", "fp64"), ("hlld", "fp32flux"), ("hlld", "fp64_perturbed"))
+    if not with_l1:
+        variants = tuple(v for v in variants if v[1] != "fp64_perturbed")
     finals, early_rho = {}, {}
     for engine, var in variants:
         margin = 2 if engine == "mlw" else 3
@@ -227,7 +246,7 @@ def blast3d_figures(n: int, steps: int, l1_steps: int = 500, early: int = 100):
             out[name] = {"zone_updates_per_s": n ** 3 * steps / (ms / 1e3), "ms_per_step": ms / steps,
                          "hbm_frac_alg": 128 * n ** 3 * steps / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
         total = steps + 2
-        if engine != "mlw":
+        if engine != "mlw" and with_l1:
             sl = (slice(margin, -margin),) * 3
             run.steps(early - total)
             cur, _ = run.state()
@@ -237,7 +256,7 @@ def blast3d_figures(n: int, steps: int, l1_steps: int = 500, early: int = 100):
         cur, info = run.state()
         assert info["merr"] == 0 and info["nexec"] == total, info
         print(f"bench: blast {name} done", file=sys.stderr, flush=True)
-        if engine != "mlw":
+        if engine != "mlw" and with_l1:
             out.setdefault(name, {})["t_at_l1"] = info["t"]
             finals[name] = [t.clone() for t in cur]   # kept on the GPU (3.7 GB per run at 384^3)
         del run, st, gd, cur
@@ -249,6 +268,21 @@ def blast3d_figures(n: int, steps: int, l1_steps: int = 500, early: int = 100):
                 else float((x - y).abs().sum()) for k, x, y in zip(F3, b, a)}
 
     for engine in ("roe", "hlld"):
+        out[engine + "_fp32flux_speedup"] = out[engine + "_fp64"]["ms_per_step"] / out[engine + "_fp32flux"]["ms_per_step"]
+        for var in ("fp64", "fp32flux"):
+            ops = profile_value("engines3d_blast384_pmc.txt", f"FP64_OPS_PER_ZONE_UPDATE_{engine.upper()}_{var.upper()}")
+            ops32 = profile_value("engines3d_blast384_pmc.txt", f"FP32_OPS_PER_ZONE_UPDATE_{engine.upper()}_{var.upper()}")
+            zu = out[engine + "_" + var]["zone_updates_per_s"]
+            if ops is not None:
+                # FP64 arithmetic rate of the whole step (the op count per
+                # zone-update from the committed counter profile, the rate
+                # from this run) against the 78.6 TFLOP/s FP64 vector peak
+                out[engine + "_" + var]["fp64_tflops"] = ops * zu / 1e12
+                out[engine + "_" + var]["fp64_valu_frac"] = ops * zu / 1e12 / FP64_PEAK_TFLOPS
+            if ops32 is not None:
+                out[engine + "_" + var]["fp32_tflops"] = ops32 * zu / 1e12
+        if not with_l1:
+            continue
         a = finals[engine + "_fp64"]
         out[engine + "_rel_l1_fp32flux_vs_fp64"] = rel(finals[engine + "_fp32flux"], a)
         out[engine + "_rel_l1_fp64_perturbed_vs_fp64"] = rel(finals[engine + "_fp64_perturbed"], a)
@@ -256,7 +290,6 @@ def blast3d_figures(n: int, steps: int, l1_steps: int = 500, early: int = 100):
         out[engine + f"_rho_rel_l1_after_{early}_steps"] = {
             v: float((early_rho[engine + "_" + v] - e0).abs().sum() / e0.abs().sum())
             for v in ("fp32flux", "fp64_perturbed")}
-        out[engine + "_fp32flux_speedup"] = out[engine + "_fp64"]["ms_per_step"] / out[engine + "_fp32flux"]["ms_per_step"]
     finals.clear()
     early_rho.clear()
     gc.collect()
@@ -304,8 +337,35 @@ def c0_figures(no_cpu: bool = False, reps: int = 5):
         issue = info["nexec"]               # timed runs issue exactly the steps the run takes
     nsteps = info["nexec"]
     med = statistics.median(ms)
-    out = {"cells": 1024, "steps": nsteps, "t": info["t"], "ms": med, "us_per_step": med * 1e3 / nsteps,
-           "zone_updates_per_s": 1024 * nsteps / (med / 1e3)}
+    out = {"cells": 1024, "steps": nsteps, "t": info["t"],
+           "launch_per_step": {"ms": med, "us_per_step": med * 1e3 / nsteps,
+                               "zone_updates_per_s": 1024 * nsteps / (med / 1e3),
+                               "note": "cans_run: cfl_m, clock, mlw_m and bdfrex launches per step"}}
+    # the same run as one persistent workgroup (the batched-tube kernel with
+    # one tube): the whole loop on the device, one barrier per step
+    tube = {k: pb.fields[k] for k in ("ro", "pr", "vx", "vy", "by", "bx", "bxm")}
+    tube["dx"], tube["dxm"] = g["dx"], g["dxm"]
+    b = cans.Batch1D([tube], pp["gm"], pp["qav"], pp["safety"], 2)
+    init = [t.clone() for t in b.f]
+    b.tune(probe_steps=50)
+    ms = []
+    for rep in range(reps + 1):
+        for t_, i0 in zip(b.f, init):
+            t_.copy_(i0)
+        ev = cans.EventSet(2)
+        torch.cuda.synchronize()
+        ev.record(0)
+        b.run(tend=pp["tend"])
+        ev.record(1)
+        torch.cuda.synchronize()
+        if rep:
+            ms.append(ev.elapsed_ms(0, 1))
+    assert int(b.ns[0]) == nsteps and int(b.merr[0]) == 0 and float(b.t[0]) == info["t"]
+    med = statistics.median(ms)
+    out["persistent"] = {"ms": med, "us_per_step": med * 1e3 / nsteps, "zone_updates_per_s": 1024 * nsteps / (med / 1e3),
+                         "block_threads": b.shape[0]}
+    out["zone_updates_per_s"] = max(out["persistent"]["zone_updates_per_s"],
+                                    out["launch_per_step"]["zone_updates_per_s"])
     if not no_cpu:
         from oracle.oracle import REF_LIBS, Oracle, Ref
         if REF_LIBS["1d"].exists():
@@ -419,6 +479,75 @@ def tubes_figures(ntube: int = 4096, tend: float = 0.1):
     out["looped_single_runs"] = {"tubes": nloop, "zone_updates": zu, "ms": ms,
                                  "zone_updates_per_s": zu / (ms / 1e3),
                                  "batched_fp64_over_looped": out["fp64"]["zone_updates_per_s"] / (zu / (ms / 1e3))}
+    return out
+
+
+def batch2d_figures(nrun: int = 1024, tend: float = 0.2):
+    """The batched 2D workload: nrun periodic Orszag-Tang runs of uneven size
+    (nx, ny uniform in 32..256, pressure scale 0.1..10, seeded:
+    problems.ot_run_set), each running the cans2d main loop to its own
+    t >= tend in one persistent kernel (cans2d_batch_run), FP64, against the
+    same runs looped one after another through the single-run engine."""
+    import numpy as np
+    import torch
+
+    from utrip_amd import cans, problems
+    nx, ny, ps = problems.ot_run_set(nrun)
+    runs = []
+    for k in range(nrun):
+        pb = problems.orszag_tang_box(int(nx[k]), int(ny[k]), 1, float(ps[k]))
+        r = dict(pb.fields)
+        r.update({q: pb.grid[q] for q in ("dx", "dxm", "dy", "dym")})
+        runs.append(r)
+    pp = pb.params
+    cells = (nx * ny).astype(np.int64)
+    out = {"runs": nrun, "cells": int(cells.sum()), "tend": tend, "n_range": [32, 256]}
+    b = cans.Batch2D(runs, pp["gm"], pp["qav"], pp["safety"], 1)
+    init = [t.clone() for t in b.f]
+    best = None
+    for shape in ((256, 2), (128, 4), (512, 1), (64, 8)):
+        for t_, i0 in zip(b.f, init):
+            t_.copy_(i0)
+        ev = cans.EventSet(2)
+        torch.cuda.synchronize()
+        ev.record(0)
+        b.run(tend=tend, block_threads=shape[0], blocks_per_cu=shape[1])
+        ev.record(1)
+        torch.cuda.synchronize()
+        ms = ev.elapsed_ms(0, 1)
+        ns = b.ns.cpu().numpy().astype(np.int64)
+        assert (b.merr.cpu().numpy() == 0).all() and (b.t.cpu().numpy() >= tend).all()
+        zu = int((cells * ns).sum())
+        out.setdefault("by_launch_shape", {})[f"{shape[0]}x{shape[1]}"] = zu / (ms / 1e3)
+        if best is None or ms < best[0]:
+            best = (ms, zu, shape, ns)
+    ms, zu, shape, ns = best
+    out["fp64"] = {"zone_updates": zu, "ms": ms, "zone_updates_per_s": zu / (ms / 1e3), "launch_shape": list(shape),
+                   "steps_min": int(ns.min()), "steps_max": int(ns.max()),
+                   "hbm_frac_alg": B_ALG_2D * zu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    # the alternative the batch replaces: each run through the resident
+    # single-run loop (MlwRun2D, one fused launch per step), one after
+    # another, for the steps the batch took; timed on the first `nloop` runs
+    nloop = 16
+    zl, msl = 0, 0.0
+    for k in range(nloop):
+        st = [torch.from_numpy(np.ascontiguousarray(runs[k][q].T)).to("cuda:0") for q in cans.FIELDS[2]]
+        gd = [torch.from_numpy(runs[k][q]).to("cuda:0") for q in ("dx", "dxm", "dy", "dym")]
+        run = cans.MlwRun2D(st, *gd, pp["qav"], pp["gm"], pp["safety"])
+        run.reset(tend=tend, dtmin=pp["dtmin"])
+        ev = cans.EventSet(2)
+        torch.cuda.synchronize()
+        ev.record(0)
+        run.steps(int(ns[k]))
+        ev.record(1)
+        torch.cuda.synchronize()
+        msl += ev.elapsed_ms(0, 1)
+        _, info = run.state()
+        assert info["nexec"] == int(ns[k]) and info["t"] == float(b.t[k])
+        zl += int(cells[k] * ns[k])
+        del run, st, gd
+    out["looped_single_runs"] = {"runs": nloop, "zone_updates": zl, "ms": msl, "zone_updates_per_s": zl / (msl / 1e3),
+                                 "batched_over_looped": out["fp64"]["zone_updates_per_s"] / (zl / (msl / 1e3))}
     return out
 
 
@@ -733,6 +862,10 @@ def main():
     if args.tubes > 0:
         tb = tubes_figures(args.tubes)
         progress("tubes")
+    b2 = None
+    if args.runs2d > 0:
+        b2 = batch2d_figures(args.runs2d)
+        progress("batched 2D runs")
     c2 = None
     if args.c2_steps > 0:
         c2 = blast3d_figures(args.c2_n, args.c2_steps, args.c2_l1_steps)
@@ -794,6 +927,7 @@ def main():
         "ot4096_fp64": big,
         "blast384": c2,
         "tubes": tb,
+        "runs2d": b2,
         "adjoint_ot512": adj,
         "spicule": spc,
         "ot1024_mlw_m3": m3,

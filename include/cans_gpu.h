@@ -430,6 +430,33 @@ int cans1d_batch_run_steps(int ntube, const long long *off, const int *ix, const
                            double safety, double dtmin, double tend, const int *nstop_tube, int margin, int mixed,
                            double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
                            void *ws, void *stream);
+/* Batched periodic 2D runs: nrun independent problems, each running the
+ * cans2d main loop (cans2d/md_mhdshktb/hdmlw/main.f:150-206 with bdperx and
+ * bdpery at ghost width `margin`):
+ *     ns=ns+1; cfl_m (cans2d/common/cfl_m.f:2-3); stop if merr; t=t+dt;
+ *     mlw_m (cans2d/hdmlw/mlw_m.f:2-3); bdperx; bdpery; repeat while
+ *     ns < nstop and t < tend
+ * to its own end, in one persistent kernel (a workgroup per run at a time,
+ * queue order `order`, NULL: 0..nrun-1).  Run k: fields ro,pr,vx,vy,bx,by,az
+ * (state7: 7 device pointers) at cells off[k] .. off[k]+ix[k]*jx[k]-1,
+ * column-major (ix,jx) with the ghost layers (filled periodically on entry);
+ * its dx, dxm at entries offx[k] .. offx[k]+ix[k]-1 of the packed x arrays
+ * (nx1d entries), dy, dym at offy[k] .. of the packed y arrays (ny1d).  off,
+ * ix, jx, offx, offy, order, t_out, ns_out, merr_out are device arrays of
+ * nrun entries.  FP64, bit-identical per run to the reference loop.  Every
+ * call starts each run's loop at t = 0, ns = 0 from the state it finds and
+ * leaves the final state in state7, t, ns and merr (9001 if dt < dtmin).
+ * block_threads in {64,...,512} with block_threads*blocks_per_cu <= 512.
+ * ws: cans2d_batch_ws_bytes.  The call synchronises the stream once (to
+ * upload the per-run argument table), then launches asynchronously.  No
+ * single counterpart in the reference, whose 2D driver runs one problem. */
+size_t cans2d_batch_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun);
+int cans2d_batch_run(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
+                     const long long *offy, const int *order, double *const *state7, const double *dx,
+                     const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
+                     long long ny1d, double gm, double qav, double safety, double dtmin, double tend, int nstop,
+                     int margin, double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
+                     void *ws, void *stream);
 /* Autotuner of the persistent kernel's launch shape: times every
  * (block_threads, blocks_per_cu) on a copy of the state for probe_steps
  * steps per tube and returns the fastest (cached per batch shape for the

@@ -237,3 +237,23 @@ def test_blast384_conservation_and_fp32_flux(dev, engine):
     rel = float((out[True] - out[False]).abs().sum() / out[False].abs().sum())
     print("rho rel L1 fp32-flux vs fp64", rel, flush=True)
     assert rel < 1e-5
+
+
+def test_c0_shock_tube_to_tend_persistent(dev, oracle):
+    """The same run as one persistent workgroup (the batched-tube kernel with
+    one tube): same step count, t and final state as the oracle, bit for
+    bit."""
+    from utrip_amd import cans
+    case("c0 shock tube to t=0.1, persistent kernel")
+    pb = P.shock_tube_1d(nx=1024, margin=2)
+    ref, dts, tref = _c0_oracle(oracle, pb)
+    g, pp = pb.grid, pb.params
+    tube = {k: pb.fields[k] for k in ("ro", "pr", "vx", "vy", "by", "bx", "bxm")}
+    tube["dx"], tube["dxm"] = g["dx"], g["dxm"]
+    b = cans.Batch1D([tube], pp["gm"], pp["qav"], pp["safety"], 2)
+    b.run(tend=pp["tend"])
+    torch.cuda.synchronize()
+    assert int(b.ns[0]) == len(dts) and int(b.merr[0]) == 0 and float(b.t[0]) == tref
+    got = b.tube(0)
+    for k in F1:
+        assert_bitwise(got[k], ref[k], k)

@@ -690,6 +690,87 @@ class Batch1D:
         return {q: t[s].cpu().numpy() for q, t in zip(self.FIELDS, self.f)}
 
 
+class Batch2D:
+    """Many independent periodic 2D MLW runs, each running the cans2d main
+    loop {cfl_m; t=t+dt; mlw_m; bdperx; bdpery} to its own end
+    (cans2d/md_mhdshktb/hdmlw/main.f:150-206), in one persistent kernel
+    (cans2d_batch_run).
+
+    ``runs``: a list of dicts of Fortran-ordered numpy arrays ro, pr, vx, vy,
+    bx, by, az of shape (ix, jx) (ghosts included and filled periodically)
+    and dx, dxm (ix), dy, dym (jx).  Run k's fields occupy cells
+    ``off[k] .. off[k] + ix*jx - 1`` of one packed device array per field.
+    The queue order is costliest first (cells / dt0, dt0 from the initial
+    cfl_m)."""
+
+    FIELDS = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+
+    def __init__(self, runs, gm, qav, safety, margin, dtmin=1e-10, device="cuda:0"):
+        import numpy as np
+        self.n = len(runs)
+        self.gm, self.qav, self.safety, self.margin, self.dtmin = gm, qav, safety, margin, dtmin
+        ix = np.array([r["ro"].shape[0] for r in runs], dtype=np.int32)
+        jx = np.array([r["ro"].shape[1] for r in runs], dtype=np.int32)
+        cells = ix.astype(np.int64) * jx
+
+        def starts(sizes):
+            o = np.zeros(self.n, dtype=np.int64)
+            if self.n:
+                o[1:] = np.cumsum(sizes[:-1], dtype=np.int64)
+            return o
+
+        off, offx, offy = starts(cells), starts(ix.astype(np.int64)), starts(jx.astype(np.int64))
+        self.ncells, self.nx1d, self.ny1d = int(cells.sum()), int(ix.sum()), int(jx.sum())
+        self.ix_host, self.jx_host, self.off_host = ix, jx, off
+        dev = torch.device(device)
+
+        def pack(get):
+            a = np.concatenate([np.asarray(get(r), dtype=np.float64).ravel(order="F") for r in runs])
+            return torch.from_numpy(a).to(dev)
+
+        self.f = [pack(lambda r, k=k: r[k]) for k in self.FIELDS]
+        self.g1 = [pack(lambda r, k=k: r[k]) for k in ("dx", "dxm", "dy", "dym")]
+        cost = np.zeros(self.n)
+        for k, r in enumerate(runs):
+            ro, pr, vx, vy, bx, by = (np.asarray(r[q], dtype=np.float64) for q in self.FIELDS[:6])
+            c = np.sqrt(vx ** 2 + vy ** 2 + gm * pr / ro + (bx ** 2 + by ** 2) / np.pi / 4.0 / ro)
+            d = np.minimum(np.asarray(r["dx"])[:, None], np.asarray(r["dy"])[None, :])
+            cost[k] = float(cells[k]) * float(np.max(c / d))
+        self.order_host = np.argsort(-cost, kind="stable").astype(np.int32)
+        to = lambda a: torch.from_numpy(a).to(dev)
+        self.off, self.offx, self.offy = to(off), to(offx), to(offy)
+        self.ix, self.jx, self.order = to(ix), to(jx), to(self.order_host)
+        self.t = torch.zeros(self.n, dtype=torch.float64, device=dev)
+        self.ns = torch.zeros(self.n, dtype=torch.int32, device=dev)
+        self.merr = torch.zeros(self.n, dtype=torch.int32, device=dev)
+        lib().cans2d_batch_ws_bytes.restype = C.c_size_t
+        nb = lib().cans2d_batch_ws_bytes(C.c_longlong(self.ncells), C.c_longlong(self.nx1d), C.c_longlong(self.ny1d),
+                                         C.c_int(self.n))
+        self.ws = torch.empty((nb + 7) // 8, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+
+    def run(self, tend: float, nstop: int = 2**31 - 1, block_threads: int = 256, blocks_per_cu: int = 2):
+        n = self.n
+        st = (C.c_void_p * 7)(*[_ptr(t, (self.ncells,)).value for t in self.f])
+        _check(lib().cans2d_batch_run(
+            C.c_int(n), _iptr(self.off, torch.int64, n, "off"), _iptr(self.ix, torch.int32, n, "ix"),
+            _iptr(self.jx, torch.int32, n, "jx"), _iptr(self.offx, torch.int64, n, "offx"),
+            _iptr(self.offy, torch.int64, n, "offy"), _iptr(self.order, torch.int32, n, "order"), st,
+            _ptr(self.g1[0], (self.nx1d,)), _ptr(self.g1[1], (self.nx1d,)), _ptr(self.g1[2], (self.ny1d,)),
+            _ptr(self.g1[3], (self.ny1d,)), C.c_longlong(self.ncells), C.c_longlong(self.nx1d),
+            C.c_longlong(self.ny1d), _d(self.gm), _d(self.qav), _d(self.safety), _d(self.dtmin), _d(tend),
+            C.c_int(nstop), C.c_int(self.margin), _ptr(self.t, (n,)), _iptr(self.ns, torch.int32, n, "ns"),
+            _iptr(self.merr, torch.int32, n, "merr"), C.c_int(block_threads), C.c_int(blocks_per_cu),
+            _ptr(self.ws), _stream()))
+
+    def run_fields(self, k: int):
+        """Host copies (Fortran order, (ix, jx)) of run k's fields."""
+        import numpy as np
+        ix, jx, o = int(self.ix_host[k]), int(self.jx_host[k]), int(self.off_host[k])
+        return {q: np.asfortranarray(t[o:o + ix * jx].cpu().numpy().reshape((ix, jx), order="F"))
+                for q, t in zip(self.FIELDS, self.f)}
+
+
 class BatchGraph:
     def __init__(self, handle):
         self.h = handle

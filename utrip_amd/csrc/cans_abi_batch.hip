@@ -296,3 +296,132 @@ int cans_graph_destroy(void *graph) {
 }
 
 }  // extern "C"
+
+// ======================================================================
+// Batched periodic 2D runs (MLW)
+namespace {
+
+struct Batch2In {
+    int nrun;
+    const long long *off, *offx, *offy;
+    const int *ix, *jx, *order;
+    double *f[7];
+    const double *dx, *dxm, *dy, *dym;
+    long long ncells, nx1d, ny1d;
+    double gm, qav, safety, dtmin, tend;
+    int nstop, margin;
+    double *t_out;
+    int *ns_out, *merr_out;
+    void *ws;
+};
+
+// workspace: the scratch state (7 fields), the x and y metric (4 arrays
+// each), the per-run argument table (2 per run), the queue counter
+size_t b2_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun) {
+    return (7 * al((size_t)ncells) + 4 * al((size_t)nx1d) + 4 * al((size_t)ny1d)) * sizeof(double) +
+           ((2 * (size_t)nrun * sizeof(Mlw2dArgs) + 255) & ~size_t(255)) + 256;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t cans2d_batch_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun) {
+    return b2_ws_bytes(ncells, nx1d, ny1d, nrun);
+}
+
+int cans2d_batch_run(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
+                     const long long *offy, const int *order, double *const *state7, const double *dx,
+                     const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
+                     long long ny1d, double gm, double qav, double safety, double dtmin, double tend, int nstop,
+                     int margin, double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
+                     void *ws, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (nrun < 0 || ncells < 0 || nx1d < 0 || ny1d < 0 || margin < 1 || nstop < 0)
+        return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: nrun=%d ncells=%lld margin=%d nstop=%d", nrun, ncells,
+                        margin, nstop);
+    if (nrun == 0) return 0;
+    if (!state7 || !off || !ix || !jx || !offx || !offy || !dx || !dxm || !dy || !dym || !t_out || !ns_out ||
+        !merr_out || !ws)
+        return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: NULL argument");
+    for (int q = 0; q < 7; q++)
+        if (!state7[q]) return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: NULL field %d", q);
+    if (!shape_ok(block_threads, blocks_per_cu) || block_threads * blocks_per_cu > 512)
+        return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: block_threads=%d blocks_per_cu=%d (at most 512 threads "
+                        "per CU: two waves per SIMD)", block_threads, blocks_per_cu);
+    // host check of the layout (the kernel trusts it)
+    std::vector<long long> o(nrun), ox(nrun), oy(nrun);
+    std::vector<int> nx(nrun), ny(nrun);
+    HIPCHK(hipMemcpy(o.data(), off, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ox.data(), offx, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(oy.data(), offy, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nx.data(), ix, sizeof(int) * nrun, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ny.data(), jx, sizeof(int) * nrun, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nrun; k++) {
+        const bool ok = nx[k] >= 3 && ny[k] >= 3 && nx[k] - 2 * margin >= margin && ny[k] - 2 * margin >= margin &&
+                        o[k] >= 0 && o[k] + (long long)nx[k] * ny[k] <= ncells && ox[k] >= 0 &&
+                        ox[k] + nx[k] <= nx1d && oy[k] >= 0 && oy[k] + ny[k] <= ny1d;
+        if (!ok)
+            return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: run %d has ix=%d jx=%d off=%lld offx=%lld offy=%lld "
+                            "(ncells=%lld nx1d=%lld ny1d=%lld margin=%d; the periodic box needs n-2*margin >= margin)",
+                            k, nx[k], ny[k], o[k], ox[k], oy[k], ncells, nx1d, ny1d, margin);
+    }
+    if (order) {
+        std::vector<int> ord(nrun);
+        HIPCHK(hipMemcpy(ord.data(), order, sizeof(int) * nrun, hipMemcpyDeviceToHost));
+        std::vector<char> seen(nrun, 0);
+        for (int k = 0; k < nrun; k++) {
+            if (ord[k] < 0 || ord[k] >= nrun || seen[ord[k]])
+                return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: order is not a permutation (entry %d = %d)", k, ord[k]);
+            seen[ord[k]] = 1;
+        }
+    }
+    Batch2dArgs b{};
+    b.nrun = nrun;
+    b.off = off; b.offx = offx; b.offy = offy; b.ix = ix; b.jx = jx; b.order = order;
+    double *w = (double *)ws;
+    const size_t nc = al((size_t)ncells), n1x = al((size_t)nx1d), n1y = al((size_t)ny1d);
+    for (int q = 0; q < 7; q++) {
+        b.f[q] = state7[q];
+        b.g[q] = w + q * nc;
+    }
+    for (int q = 0; q < 4; q++) {
+        b.mx[q] = w + 7 * nc + q * n1x;
+        b.my[q] = w + 7 * nc + 4 * n1x + q * n1y;
+    }
+    Mlw2dArgs *table = reinterpret_cast<Mlw2dArgs *>(w + 7 * nc + 4 * n1x + 4 * n1y);
+    b.next = reinterpret_cast<unsigned *>(reinterpret_cast<char *>(table) +
+                                          ((2 * (size_t)nrun * sizeof(Mlw2dArgs) + 255) & ~size_t(255)));
+    b.runs = table;
+    b.dx = dx; b.dxm = dxm; b.dy = dy; b.dym = dym;
+    b.gm = gm; b.qav = qav; b.safety = safety; b.dtmin = dtmin; b.tend = tend;
+    b.nstop = nstop; b.margin = margin;
+    b.t_out = t_out; b.ns_out = ns_out; b.merr_out = merr_out;
+    // the per-run arguments of the strip march, one per step parity
+    std::vector<Mlw2dArgs> h(2 * (size_t)nrun);
+    for (int k = 0; k < nrun; k++)
+        for (int par = 0; par < 2; par++) {
+            Mlw2dArgs &a = h[2 * (size_t)k + par];
+            a = Mlw2dArgs{};
+            double *const *cur = par ? b.g : b.f;
+            double *const *nxt = par ? b.f : b.g;
+            a.ro = cur[0] + o[k]; a.pr = cur[1] + o[k]; a.vx = cur[2] + o[k]; a.vy = cur[3] + o[k];
+            a.bx = cur[4] + o[k]; a.by = cur[5] + o[k]; a.az = cur[6] + o[k];
+            a.ro_o = nxt[0] + o[k]; a.pr_o = nxt[1] + o[k]; a.vx_o = nxt[2] + o[k]; a.vy_o = nxt[3] + o[k];
+            a.bx_o = nxt[4] + o[k]; a.by_o = nxt[5] + o[k]; a.az_o = nxt[6] + o[k];
+            a.dx = dx + ox[k]; a.dxm = dxm + ox[k]; a.dy = dy + oy[k]; a.dym = dym + oy[k];
+            a.mx = Metric{b.mx[0] + ox[k], b.mx[1] + ox[k], b.mx[2] + ox[k], b.mx[3] + ox[k]};
+            a.my = Metric{b.my[0] + oy[k], b.my[1] + oy[k], b.my[2] + oy[k], b.my[3] + oy[k]};
+            a.qav = qav; a.gm = gm; a.dt = 0.0; a.clk = nullptr; a.step = 0;
+            a.ix = nx[k]; a.jx = ny[k];
+            a.per_margin = margin; a.safety = safety; a.peek = 0; a.dt_log = nullptr;
+        }
+    // ordered after any earlier work on the stream that reads the table; the
+    // host copy h must outlive the transfer, the launch itself stays async
+    HIPCHK(hipMemcpyAsync(table, h.data(), h.size() * sizeof(Mlw2dArgs), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(launch_batch2d(b, block_threads, cu_count() * blocks_per_cu, s));
+    return 0;
+}
+
+}  // extern "C"

@@ -129,6 +129,28 @@ struct Batch1dArgs {
 };
 hipError_t launch_batch1d(const Batch1dArgs &a, int mixed, int block_threads, int nblocks, hipStream_t s);
 
+// Batched periodic 2D runs (MLW): run k's fields occupy cells off[k] ..
+// off[k]+ix[k]*jx[k]-1 of every field array (column-major (ix,jx)); its x
+// grid arrays (dx, dxm and the x metric) entries offx[k] .. offx[k]+ix[k]-1,
+// its y arrays offy[k] .. offy[k]+jx[k]-1.
+struct Batch2dArgs {
+    int nrun;
+    const long long *off, *offx, *offy;   // device, per run
+    const int *ix, *jx;                   // device, per run (ghosts included)
+    const int *order;                     // device queue order (NULL: 0..nrun-1)
+    double *f[7];                         // ro pr vx vy bx by az (state, in place)
+    double *g[7];                         // scratch set of the same size
+    const double *dx, *dxm, *dy, *dym;    // packed grid arrays
+    double *mx[4], *my[4];                // packed metric (di, dim, u0, u1), filled by the launch
+    const Mlw2dArgs *__restrict__ runs;   // per run k: runs[2k] reads f writes g, runs[2k+1] the reverse
+    double gm, qav, safety, dtmin, tend;
+    int nstop, margin;                    // margin: the periodic ghost width
+    double *t_out;                        // device, per run
+    int *ns_out, *merr_out;               // device, per run
+    unsigned *next;                       // queue counter (zeroed by the launch)
+};
+hipError_t launch_batch2d(const Batch2dArgs &b, int block_threads, int nblocks, hipStream_t s);
+
 // mlw_m3 (cans{1,2}d/hdmlw/mlw_m3.f): 1D fields ro pr vx vy vz by bz (bx,
 // bxm constant); 2D fields ro pr vx vy vz bx by bz az.  in -> out on the
 // interior cells (2..n-1).

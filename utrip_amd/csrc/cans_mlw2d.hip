@@ -31,6 +31,7 @@
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_kernels.hpp"
+#include "cans_wave.hpp"
 
 namespace cans {
 namespace {
@@ -38,21 +39,13 @@ namespace {
 constexpr int WAVE = 64;
 constexpr int COLS = WAVE - 2;               // output columns per wave
 constexpr int WPB = 4;                       // waves per block
+constexpr int kOccBatch = 2;                 // waves per SIMD of the batched kernel (the strip's registers)
 constexpr double pi8i = 1.0 / kPi / 8.0;     // mlw_m.f:27  pi8i=1./pi/8.
 constexpr double pi4i = 1.0 / kPi / 4.0;     // mlw_m.f:28  pi4i=1./pi/4.
 
-// Lane l receives lane l+1 (shl) / lane l-1 (shr).  The edge lanes receive
-// 0; their results are never used by an output lane.
-CANS_DEV double shl1(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
-    return __hiloint2double(hi, lo);
-}
-CANS_DEV double shr1(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, true);
-    return __hiloint2double(hi, lo);
-}
+// x neighbours: shl1/shr1 of cans_wave.hpp (lane l receives lane l+1 / l-1
+// through a DPP wave shift; the edge lanes receive 0, and their results are
+// never used by an output lane).
 
 // u: ro ee rx ry bx by (the six conserved variables, mlw_m.f:101-112)
 enum { U_RO, U_EE, U_RX, U_RY, U_BX, U_BY, NU };
@@ -462,6 +455,67 @@ CANS_DEV void row_step(const Mlw2dArgs &a, const YMet &y, const ColCtx &x, int j
     finish_row(a, y, x, j, o, do_cfl, dtmin_local);
 }
 
+// One wave's strip: columns 1+strip*COLS .. (lane 0 the west halo), output
+// rows jb = 2+rowb*H .. min(jb+H-1, jx-1), marched in y (prologue: the cells
+// of rows jb-1 and jb and the corner row jb-1).
+CANS_DEV void march_strip(const Mlw2dArgs &a, const YMet &y, int H, int strip, int rowb, double dt, bool do_cfl,
+                          double &dtmin_local) {
+    const int ix = a.ix, jx = a.jx;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int jb = 2 + rowb * H;
+    const int je = min(jb + H - 1, jx - 1);
+    ColCtx x;
+    x.i = 1 + strip * COLS + lane;         // lane 0 is the west halo column
+    x.out = lane >= 1 && lane <= COLS && x.i <= ix - 1 &&
+            (a.per_margin == 0 || (x.i >= a.per_margin + 1 && x.i <= ix - a.per_margin));
+    const int ic = min(x.i, ix);           // lanes past the grid read column ix
+    x.boff = (unsigned)(ic - 1) * 8u;
+    x.dxi = a.mx.di[ic - 1];
+    x.dxim = a.mx.dim[ic - 1];
+    x.dxim1 = a.mx.dim[max(ic - 2, 0)];
+    x.ux0 = a.mx.u0[ic - 1];
+    x.ux1 = a.mx.u1[ic - 1];
+    x.qxs = a.qav * a.dxm[ic - 1];
+    x.dxc = a.dx[ic - 1];
+    x.dxg = x.dxc;
+    {
+        const int mg = a.per_margin;
+        if (mg > 0 && ic >= ix - 2 * mg + 1) x.dxg = fminF(x.dxg, a.dx[ic - (ix - 2 * mg) - 1]);
+        if (mg > 0 && ic <= 2 * mg) x.dxg = fminF(x.dxg, a.dx[ic + (ix - 2 * mg) - 1]);
+    }
+
+    // prologue: cells of rows jb-1 and jb, corner row jb-1
+    const Cell R0 = make_cell(load_prim(a, x, jb - 1), a.gm);
+    RowState s0, s1;
+    s0.c = make_cell(load_prim(a, x, jb), a.gm);
+    const CellE s0e = shl_cell(s0.c);
+    s0.nxt = load_prim(a, x, jb + 1);
+    s0.az = load_az(a, x, jb);
+    const CellE R0p = shl_cell(R0);
+    const Corner Cm = make_corner(R0, R0p, s0.c, s0e, x.dxim, a.my.dim[jb - 2], dt, a.gm);
+    const double ezW = shr1(Cm.ez);
+    s0.dXS[0] = Cm.rox - shr1(Cm.rox);
+    s0.dXS[1] = Cm.eex - shr1(Cm.eex);
+    s0.dXS[2] = Cm.rxx - shr1(Cm.rxx);
+    s0.dXS[3] = Cm.ryx - shr1(Cm.ryx);
+    s0.dXS[4] = ezW - Cm.ez;
+    s0.PS = x.ux0 * (-ezW) + x.ux1 * (-Cm.ez);
+    s0.ky[0] = Cm.roy; s0.ky[1] = Cm.eey; s0.ky[2] = Cm.rxy; s0.ky[3] = Cm.ryy; s0.ky[4] = Cm.ez;
+    s0.sy[0] = R0.u[U_RY]; s0.sy[1] = R0.eey; s0.sy[2] = R0.rxy; s0.sy[3] = R0.ryy; s0.sy[4] = R0.ez;
+#pragma unroll
+    for (int q = 0; q < NU; q++) s0.dS[q] = s0.c.u[q] - R0.u[q];
+    s0.qys = av_coef(a.qav * a.dym[jb - 2], s0.c.vy, R0.vy);
+
+    // two rows per trip so that the carried state ping-pongs between
+    // s0 and s1 instead of being copied
+    int j = jb;
+    for (; j + 1 <= je; j += 2) {
+        row_step(a, y, x, j, dt, do_cfl, s0, s1, dtmin_local);
+        row_step(a, y, x, j + 1, dt, do_cfl, s1, s0, dtmin_local);
+    }
+    if (j <= je) row_step(a, y, x, j, dt, do_cfl, s0, s1, dtmin_local);
+}
+
 // The y metric arrays come as separate non-aliased arguments (the same
 // pointers as a.my, a.dym, a.dy) so that their per-row uniform loads go
 // through the scalar cache.
@@ -473,7 +527,6 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
     const YMet y{ydim, ydi, yu0, yu1, ydym, ydy};
     __shared__ double red[WPB];
     const int ix = a.ix, jx = a.jx;
-    const int lane = threadIdx.x & (WAVE - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nstrip = (ix - 2 + COLS - 1) / COLS;
     const int nrowb = (jx - 2 + H - 1) / H;
@@ -492,62 +545,129 @@ __global__ __launch_bounds__(WAVE * WPB) __attribute__((amdgpu_waves_per_eu(OCC)
     const bool do_cfl = a.clk && a.safety > 0.0;
     double dtmin_local = kR4_1e20;
 
-    if (unit < nstrip * nrowb) {
-        const int strip = unit % nstrip, rowb = unit / nstrip;
-        const int jb = 2 + rowb * H;
-        const int je = min(jb + H - 1, jx - 1);
-        ColCtx x;
-        x.i = 1 + strip * COLS + lane;         // lane 0 is the west halo column
-        x.out = lane >= 1 && lane <= COLS && x.i <= ix - 1 &&
-                (a.per_margin == 0 || (x.i >= a.per_margin + 1 && x.i <= ix - a.per_margin));
-        const int ic = min(x.i, ix);           // lanes past the grid read column ix
-        x.boff = (unsigned)(ic - 1) * 8u;
-        x.dxi = a.mx.di[ic - 1];
-        x.dxim = a.mx.dim[ic - 1];
-        x.dxim1 = a.mx.dim[max(ic - 2, 0)];
-        x.ux0 = a.mx.u0[ic - 1];
-        x.ux1 = a.mx.u1[ic - 1];
-        x.qxs = a.qav * a.dxm[ic - 1];
-        x.dxc = a.dx[ic - 1];
-        x.dxg = x.dxc;
-        {
-            const int mg = a.per_margin;
-            if (mg > 0 && ic >= ix - 2 * mg + 1) x.dxg = fminF(x.dxg, a.dx[ic - (ix - 2 * mg) - 1]);
-            if (mg > 0 && ic <= 2 * mg) x.dxg = fminF(x.dxg, a.dx[ic + (ix - 2 * mg) - 1]);
-        }
-
-        // prologue: cells of rows jb-1 and jb, corner row jb-1
-        const Cell R0 = make_cell(load_prim(a, x, jb - 1), a.gm);
-        RowState s0, s1;
-        s0.c = make_cell(load_prim(a, x, jb), a.gm);
-        const CellE s0e = shl_cell(s0.c);
-        s0.nxt = load_prim(a, x, jb + 1);
-        s0.az = load_az(a, x, jb);
-        const CellE R0p = shl_cell(R0);
-        const Corner Cm = make_corner(R0, R0p, s0.c, s0e, x.dxim, a.my.dim[jb - 2], dt, a.gm);
-        const double ezW = shr1(Cm.ez);
-        s0.dXS[0] = Cm.rox - shr1(Cm.rox);
-        s0.dXS[1] = Cm.eex - shr1(Cm.eex);
-        s0.dXS[2] = Cm.rxx - shr1(Cm.rxx);
-        s0.dXS[3] = Cm.ryx - shr1(Cm.ryx);
-        s0.dXS[4] = ezW - Cm.ez;
-        s0.PS = x.ux0 * (-ezW) + x.ux1 * (-Cm.ez);
-        s0.ky[0] = Cm.roy; s0.ky[1] = Cm.eey; s0.ky[2] = Cm.rxy; s0.ky[3] = Cm.ryy; s0.ky[4] = Cm.ez;
-        s0.sy[0] = R0.u[U_RY]; s0.sy[1] = R0.eey; s0.sy[2] = R0.rxy; s0.sy[3] = R0.ryy; s0.sy[4] = R0.ez;
-#pragma unroll
-        for (int q = 0; q < NU; q++) s0.dS[q] = s0.c.u[q] - R0.u[q];
-        s0.qys = av_coef(a.qav * a.dym[jb - 2], s0.c.vy, R0.vy);
-
-        // two rows per trip so that the carried state ping-pongs between
-        // s0 and s1 instead of being copied
-        int j = jb;
-        for (; j + 1 <= je; j += 2) {
-            row_step(a, y, x, j, dt, do_cfl, s0, s1, dtmin_local);
-            row_step(a, y, x, j + 1, dt, do_cfl, s1, s0, dtmin_local);
-        }
-        if (j <= je) row_step(a, y, x, j, dt, do_cfl, s0, s1, dtmin_local);
-    }
+    if (unit < nstrip * nrowb) march_strip(a, y, H, unit % nstrip, unit / nstrip, dt, do_cfl, dtmin_local);
     if (do_cfl) block_min_to_global(dtmin_local, red, &a.clk->dtkey[(a.step + 1) % 3]);
+}
+
+// ---------------------------------------------------------------- batched runs
+// Many independent periodic 2D runs, each running the cans2d main loop
+// (cans2d/md_mhdshktb/hdmlw/main.f:150-206 with bdperx/bdpery):
+//     ns=ns+1; cfl_m; stop on merr; t=t+dt; mlw_m; bdperx; bdpery;
+//     repeat while ns < nstop and t < tend
+// to its own end, in one persistent kernel.  A workgroup takes a run from a
+// queue and advances it to the end before taking the next; inside a run its
+// waves share the strips of each step (the register-marching strip of the
+// single-run kernel, with the ghost images and the next step's cfl term
+// fused), and a step ends with one workgroup barrier, the dt min-reduction.
+// Writes of one step are read by other waves of the same workgroup in the
+// next, after that barrier (one CU, one L1).
+
+// rows per strip for a run of jx rows and nstrip strips on NW waves: the
+// fewest rounds of units, each unit's prologue counted as 1.5 rows
+CANS_DEV int batch2d_rows(int jx, int nstrip, int NW) {
+    int best = 8;
+    double cost = 1e300;
+    for (int h = 4; h <= 48; h++) {
+        const int units = nstrip * ((jx - 2 + h - 1) / h);
+        const double c = (double)((units + NW - 1) / NW) * (h + 1.5);
+        if (c < cost) { cost = c; best = h; }
+    }
+    return best;
+}
+
+template <int BT>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(kOccBatch))) void batch2d_kernel(Batch2dArgs b) {
+    constexpr int NW = BT / 64;
+    __shared__ double red[2][NW];
+    __shared__ int run_s;
+    const int wv = uni((int)(threadIdx.x >> 6));
+    for (;;) {
+        // queue pop by wave 0 under a uniform branch (lane 0 adds 1)
+        if (wv == 0) {
+            const unsigned got = atomicAdd(b.next, threadIdx.x == 0 ? 1u : 0u);
+            run_s = uni((int)got);
+        }
+        __syncthreads();
+        const int kq = uni(run_s);
+        __syncthreads();
+        if (kq >= b.nrun) break;
+        const int k = b.order ? uni(b.order[kq]) : kq;
+        const long long o = uni(b.off[k]);
+        // step parity p reads set p and writes the other: runs[2k+p]
+        const Mlw2dArgs &a0 = b.runs[2 * k];
+        const int ix = uni(a0.ix), jx = uni(a0.jx);
+        const size_t ncell = (size_t)ix * (size_t)jx;
+        // cfl_m of the initial state over every cell (cans2d/common/cfl_m.f:44-62)
+        double m = kR4_1e20;
+        for (size_t p = threadIdx.x; p < ncell; p += BT) {
+            const int i = (int)(p % (size_t)ix) + 1, j = (int)(p / (size_t)ix) + 1;
+            const size_t q = (size_t)o + p;
+            const double ro = b.f[0][q], pr = b.f[1][q], vx = b.f[2][q], vy = b.f[3][q], bx = b.f[4][q],
+                         by = b.f[5][q];
+            const double b2 = bx * bx + by * by;
+            const double ca2 = b2 * pi4i / ro;
+            const double cs2 = b.gm * pr / ro;
+            const double v2 = vx * vx + vy * vy;
+            const double dtq = b.safety * (fminF(a0.dx[i - 1], a0.dy[j - 1]) / __builtin_sqrt(v2 + cs2 + ca2));
+            if (dtq < m) m = dtq;
+        }
+        int par = 0;
+        double dt = block_min<BT>(m, red[par]);
+        par ^= 1;
+        int ns = 0, merr = 0, nexec = 0;
+        double t = 0.0;
+        const int nstrip = (ix - 2 + COLS - 1) / COLS;
+        const int H = batch2d_rows(jx, nstrip, NW);
+        const int nunit = nstrip * ((jx - 2 + H - 1) / H);
+        for (;;) {
+            ns = ns + 1;
+            if (dt < b.dtmin) { merr = 9001; break; }     // cfl_m.f: merr=9001, main.f stops
+            t = t + dt;
+            const Mlw2dArgs &a = b.runs[2 * k + (nexec & 1)];
+            const YMet y{a.my.dim, a.my.di, a.my.u0, a.my.u1, a.dym, a.dy};
+            double dmin = kR4_1e20;
+            for (int u = wv; u < nunit; u += NW) march_strip(a, y, H, u % nstrip, u / nstrip, dt, true, dmin);
+            nexec++;
+            dt = block_min<BT>(dmin, red[par]);           // dt of the next step; also the step's barrier
+            par ^= 1;
+            if (!(ns < b.nstop && t < b.tend)) break;
+        }
+        if (nexec & 1) {   // the last state is in the scratch set: copy back
+            for (size_t p = threadIdx.x; p < ncell; p += BT)
+#pragma unroll
+                for (int q = 0; q < 7; q++) b.f[q][(size_t)o + p] = b.g[q][(size_t)o + p];
+        }
+        if (wv == 0) {   // uniform branch; the lanes store identical values
+            b.t_out[k] = t;
+            b.ns_out[k] = ns;
+            b.merr_out[k] = merr;
+        }
+    }
+}
+
+// per-run metric of both axes (the arithmetic of the single-run metric
+// kernel) and the queue counter; block r fills run r
+__global__ void batch2d_prep(Batch2dArgs b) {
+    const int r = blockIdx.x;
+    if (r == 0 && threadIdx.x == 0) *b.next = 0u;
+    if (r >= b.nrun) return;
+    for (int axis = 0; axis < 2; axis++) {
+        const long long o = axis ? b.offy[r] : b.offx[r];
+        const int n = axis ? b.jx[r] : b.ix[r];
+        const double *d = (axis ? b.dy : b.dx) + o, *dm = (axis ? b.dym : b.dxm) + o;
+        double *const *m = axis ? b.my : b.mx;
+        for (int i = 1 + (int)threadIdx.x; i <= n; i += blockDim.x) {
+            m[0][o + i - 1] = 1.0 / d[i - 1];
+            m[1][o + i - 1] = 1.0 / dm[i - 1];
+            if (i >= 2 && i <= n - 1) {
+                m[2][o + i - 1] = 0.5 * dm[i - 1] / d[i - 1];
+                m[3][o + i - 1] = 0.5 * dm[i - 2] / d[i - 1];
+            } else {
+                m[2][o + i - 1] = 0.0;
+                m[3][o + i - 1] = 0.0;
+            }
+        }
+    }
 }
 
 // Build-time tuning: waves per SIMD the register budget is compiled for,
@@ -578,6 +698,20 @@ int pick_rows(int ix, int jx) {
 }
 
 }  // namespace
+
+hipError_t launch_batch2d(const Batch2dArgs &b, int block_threads, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(batch2d_prep, dim3(b.nrun > 0 ? b.nrun : 1), dim3(256), 0, s, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || b.nrun == 0) return e;
+    switch (block_threads) {
+    case 64: hipLaunchKernelGGL(batch2d_kernel<64>, dim3(nblocks), dim3(64), 0, s, b); break;
+    case 128: hipLaunchKernelGGL(batch2d_kernel<128>, dim3(nblocks), dim3(128), 0, s, b); break;
+    case 256: hipLaunchKernelGGL(batch2d_kernel<256>, dim3(nblocks), dim3(256), 0, s, b); break;
+    case 512: hipLaunchKernelGGL(batch2d_kernel<512>, dim3(nblocks), dim3(512), 0, s, b); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_mlw2d(const Mlw2dArgs &a, hipStream_t s) {
     if (a.ix < 3 || a.jx < 3) return hipSuccess;

@@ -143,6 +143,40 @@ def orszag_tang_2d(n: int = 1024, margin: int = 1, gm: float = 5.0 / 3.0) -> Pro
                    dict(gm=gm, margin=margin, safety=0.4, qav=3.0, dtmin=1e-10))
 
 
+def orszag_tang_box(nx: int, ny: int, margin: int = 1, pscale: float = 1.0, gm: float = 5.0 / 3.0) -> Problem:
+    """The Orszag-Tang vortex of orszag_tang_2d on an nx x ny grid of the same
+    periodic box, with the pressure scaled by pscale (the plasma beta with
+    it).  nx = ny and pscale = 1 give orszag_tang_2d's arrays bit for bit."""
+    two_pi = 2.0 * math.pi
+    x = _centres(nx, margin, 0.0, two_pi)
+    y = _centres(ny, margin, 0.0, two_pi)
+    dx, xm, dxm = grdrdy(x)
+    dy, ym, dym = grdrdy(y)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    ix, jx = X.shape
+    ro = np.full((ix, jx), 25.0 / (36.0 * math.pi), order="F")
+    pr = np.full((ix, jx), 5.0 / (12.0 * math.pi) * pscale, order="F")
+    f = dict(ro=ro, pr=pr, vx=np.asfortranarray(-np.sin(Y)), vy=np.asfortranarray(np.sin(X)),
+             bx=np.asfortranarray(-np.sin(Y)), by=np.asfortranarray(np.sin(2.0 * X)))
+    for q in f.values():
+        periodic_fill(q, margin)
+    az = bbtoaa(f["bx"], f["by"], dxm, dym, margin)
+    periodic_fill(az, margin)
+    f["az"] = az
+    return Problem("orszag_tang_box", f, dict(x=x, dx=dx, dxm=dxm, y=y, dy=dy, dym=dym),
+                   dict(gm=gm, margin=margin, safety=0.4, qav=3.0, dtmin=1e-10))
+
+
+def ot_run_set(nrun: int = 1024, seed: int = 20160811, n_lo: int = 32, n_hi: int = 256):
+    """A seeded set of uneven periodic 2D runs: nx, ny uniform in
+    [n_lo, n_hi], pressure scale log-uniform in [0.1, 10]."""
+    rng = np.random.default_rng(seed)
+    nx = rng.integers(n_lo, n_hi + 1, size=nrun)
+    ny = rng.integers(n_lo, n_hi + 1, size=nrun)
+    ps = 10.0 ** rng.uniform(-1.0, 1.0, size=nrun)
+    return nx, ny, ps
+
+
 def orszag_tang_25d(n: int = 1024, margin: int = 1, gm: float = 5.0 / 3.0) -> Problem:
     """The Orszag-Tang vortex with out-of-plane components for mlw_m3:
     as orszag_tang_2d plus vz = 0.2 sin(x + y), bz = 0.3 cos(x - y)
