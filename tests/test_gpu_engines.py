@@ -76,7 +76,10 @@ def test_roe3d_one_call(dev, oracle, shape):
 
 
 # ------------------------------------------------------------------ 3D MLW one call
-@pytest.mark.parametrize("shape", [(3, 3, 3), (13, 11, 10), (40, 7, 9)])
+# (17, 9, 70), (31, 16, 34), (16, 8, 66): several z-segments of the fused
+# Cartesian cells+corners kernel (32 corner planes each; 34 leaves one plane
+# in the last), x and y tiles cut at their edges
+@pytest.mark.parametrize("shape", [(3, 3, 3), (13, 11, 10), (40, 7, 9), (17, 9, 70), (31, 16, 34), (16, 8, 66)])
 def test_mlw3d_one_call(dev, oracle, shape):
     from utrip_amd import cans
     case("mlw3d", shape)
