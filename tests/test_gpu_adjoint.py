@@ -160,8 +160,29 @@ def test_adjoint_ot512_200_steps(dev):
     fd_check(512, 200, 20, 16, dev, norm="g")
 
 
-def test_adjoint_matches_cpu_reference_gradient(dev):
-    """The whole trajectory (OT 24^2, 12 steps, checkpoints every 5) against
+def test_adjoint_every_state_kept_equals_checkpoints(dev):
+    """ckpt = 0 (every state of the trajectory kept, one reverse sweep) and
+    checkpoints every 7 steps (segments recomputed from them) give the same
+    gradient bit for bit: the same kernels run on the same states.  (The
+    dL/ddt block sums are added atomically to 16 stripes; at 64^2 a stripe
+    receives at most two of them (17 blocks), and two additions to zero
+    commute exactly, so the result does not depend on their order.  At larger
+    sizes the order varies from run to run in the last bits.)"""
+    case("OT 64^2, 30 steps, every state kept vs checkpoints every 7")
+    grads = []
+    for ck in (0, 7):
+        solver, base, vx0, vy0, beta, target, w = setup(64, 30, ck, dev)
+        x = [vx0.clone().requires_grad_(True), vy0.clone().requires_grad_(True), beta.clone().requires_grad_(True)]
+        loss(*x, solver, base, target, w).backward()
+        grads.append([t.grad.detach().cpu() for t in x])
+    for name, a, b in zip(("vx0", "vy0", "beta"), *grads):
+        assert torch.equal(a, b), name
+
+
+@pytest.mark.parametrize("ckpt", [5, 0])
+def test_adjoint_matches_cpu_reference_gradient(dev, ckpt):
+    """The whole trajectory (OT 24^2, 12 steps, checkpoints every 5 or every
+    state kept) against
     an FP64 CPU reference gradient: autograd of the vectorised torch step
     (tests/torch_mlw2d.py) run on CPU tensors for the same 12 steps, from the
     same initial velocities and pressure scale, with the same target.  The
@@ -169,9 +190,9 @@ def test_adjoint_matches_cpu_reference_gradient(dev):
     relative (max abs difference over max abs value)."""
     from tests import torch_mlw2d as T
     from utrip_amd import adjoint
-    case("OT 24^2, 12 steps vs CPU autograd of the torch step")
+    case("OT 24^2, 12 steps vs CPU autograd of the torch step, ckpt", ckpt)
     n, nsteps = 24, 12
-    solver, base, vx0, vy0, beta, target, w = setup(n, nsteps, 5, dev)
+    solver, base, vx0, vy0, beta, target, w = setup(n, nsteps, ckpt, dev)
     x = [vx0.clone().requires_grad_(True), vy0.clone().requires_grad_(True), beta.clone().requires_grad_(True)]
     L = loss(*x, solver, base, target, w)
     L.backward()

@@ -2,9 +2,12 @@
 bdperx/bdpery and cfl_m, the loop of cans2d/md_mhdshktb/hdmlw/main.f:150-206)
 by a hand-written reverse mode on the GPU (cans2d_mlw_adjoint_step).
 
-``PeriodicMlwAdjoint`` runs the forward loop on the resident fused kernel,
-keeping a checkpoint every ``ckpt`` steps; its ``backward`` recomputes each
-segment from its checkpoint and applies the adjoint step in reverse.
+``PeriodicMlwAdjoint`` runs the forward loop on the resident fused kernel.
+With ``ckpt = 0`` it keeps every state of the trajectory in device memory
+(7 fields x (nsteps+1) states: 2.9 GB for 200 steps at 512^2) and its
+``backward`` is one reverse sweep; with ``ckpt > 0`` it keeps a checkpoint
+every ``ckpt`` steps and the backward recomputes each segment from its
+checkpoint (on a second stream) before sweeping it.
 ``FinalDensity`` wraps this as a ``torch.autograd.Function`` of the initial
 velocities (interior) and a uniform pressure scale ``beta``.
 """
@@ -24,6 +27,8 @@ class PeriodicMlwAdjoint:
     loop on a grid with ghost width ``margin`` (state tensors (jx, ix))."""
 
     def __init__(self, dx, dxm, dy, dym, qav, gm, safety, margin, nsteps, ckpt=20, dtmin=1e-10):
+        if ckpt < 0:
+            raise ValueError("ckpt must be >= 0 (0: keep every state)")
         self.grid = (dx, dxm, dy, dym)
         self.ix, self.jx = dx.numel(), dy.numel()
         self.qav, self.gm, self.safety, self.margin = qav, gm, safety, margin
@@ -48,9 +53,37 @@ class PeriodicMlwAdjoint:
     def _cur(run, k):
         return run.a if k % 2 == 0 else run.b
 
+    def _forward_all(self, state):
+        """Every state of the trajectory into self._all (cans2d_mlw_record);
+        the dt of each step into self._dt_all."""
+        dx, dxm, dy, dym = self.grid
+        dev = self.ws.device
+        if getattr(self, "_all", None) is None:
+            self._all = [[torch.empty_like(t) for t in state] for _ in range(self.nsteps + 1)]
+            self._dt_all = torch.empty(max(self.nsteps, 1), dtype=torch.float64, device=dev)
+            self._clock_all = cans.Clock()
+            self._mws_all = torch.empty_like(self.metric)
+            self._all_p = _ptr_array([t for k in range(self.nsteps + 1) for t in self._all[k]],
+                                     (self.jx, self.ix))
+        for dst, src in zip(self._all[0], state):
+            dst.copy_(src)
+        self._clock_all.reset(dtmin=self.dtmin)
+        _check(lib().cans2d_mlw_record(self._all_p, C.c_int(self.nsteps), self._clock_all.handle, _d(self.qav),
+                                       _d(self.gm), _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy),
+                                       _ptr(dym), C.c_int(self.jx), C.c_int(self.margin), _ptr(self._mws_all),
+                                       _ptr(self._dt_all), _stream()))
+        self._clock_all.issued = self.nsteps
+        info = self._clock_all.read()
+        if info["merr"] != 0 or info["nexec"] != self.nsteps:
+            raise cans.CansError(f"forward run stopped early: {info}")
+        return [t.clone() for t in self._all[self.nsteps]]
+
     def forward(self, state):
         """Run nsteps from ``state`` (7 tensors ro,pr,vx,vy,bx,by,az, ghosts
-        filled); keep the checkpoints; return the final state (clones)."""
+        filled); keep every state (ckpt = 0) or the checkpoints; return the
+        final state (clones)."""
+        if self.ckpt == 0:
+            return self._forward_all(state)
         self.ckpts = []
         run = self._run(state)
         done = 0
@@ -80,6 +113,15 @@ class PeriodicMlwAdjoint:
         lam = [t.clone().contiguous() for t in lam]
         tmp = [torch.empty_like(t) for t in lam]
         dev = self.ws.device
+        if self.ckpt == 0:   # every state kept: one reverse sweep
+            lam_p, tmp_p = _ptr_array(lam, shape), _ptr_array(tmp, shape)
+            _check(lib().cans2d_mlw_adjoint_run(self._all_p, _ptr(self._dt_all), C.c_int(self.nsteps), lam_p, tmp_p,
+                                                _d(self.qav), _d(self.gm), _d(self.safety), _ptr(dx), _ptr(dxm),
+                                                C.c_int(self.ix), _ptr(dy), _ptr(dym), C.c_int(self.jx),
+                                                C.c_int(self.margin), _ptr(self.metric), _ptr(self.ws), _stream()))
+            _check(lib().cans2d_periodic_fold(lam_p, C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin),
+                                              _stream()))
+            return lam
         if getattr(self, "_seg", None) is None:
             self._seg = [[[torch.empty_like(t) for t in self.ckpts[0]] for _ in range(self.ckpt + 1)]
                          for _ in range(2)]

@@ -62,20 +62,27 @@ CANS_DEV P6 load6(const double *const *f, size_t k) {
     return P6{f[0][k], f[1][k], f[2][k], f[3][k], f[4][k], f[5][k]};
 }
 
-// conserved variables and fluxes of one cell, the forward's expressions
+// The reverse sweep re-evaluates the forward's cell values, half step and
+// primitives only as the point its Jacobians are taken at: it divides by
+// gm-1 and by rho through reciprocals (g1i = 1/(gm-1), one 1/rho per
+// cell), which moves that point by at most an ulp and the gradient by
+// ~1e-16 relative.  The one forward value it must match bit for bit is the
+// cfl term, compared with dt for the ties (cfl_term, unchanged).
+
+// conserved variables and fluxes of one cell (the forward's expressions)
 struct CellV {
     double u[6];      // ro ee rx ry bx by
     double fx[6];     // x fluxes: rx eex rxx ryx 0 -ez
     double gy[6];     // y fluxes: ry eey rxy ryy ez 0
 };
-CANS_DEV CellV cellv(const P6 &p, double gm) {
+CANS_DEV CellV cellv(const P6 &p, double gm, double g1i) {
     CellV c;
     const double vv = p.vx * p.vx + p.vy * p.vy;
     const double bb = p.bx * p.bx + p.by * p.by;
     const double ez = -p.vx * p.by + p.vy * p.bx;
-    const double ep = p.pr * gm / (gm - 1.0) + 0.5 * p.ro * vv;
+    const double ep = p.pr * gm * g1i + 0.5 * p.ro * vv;
     c.u[0] = p.ro;
-    c.u[1] = p.pr / (gm - 1.0) + 0.5 * p.ro * vv + pi8i * bb;
+    c.u[1] = p.pr * g1i + 0.5 * p.ro * vv + pi8i * bb;
     c.u[2] = p.ro * p.vx;
     c.u[3] = p.ro * p.vy;
     c.u[4] = p.bx;
@@ -98,8 +105,9 @@ CANS_DEV CellV cellv(const P6 &p, double gm) {
 CANS_DEV P6 prims(const double *u, double gm) {   // mlw_m.f:299-321
     P6 p;
     p.ro = u[0];
-    p.vx = u[2] / u[0];
-    p.vy = u[3] / u[0];
+    const double rr = 1.0 / u[0];
+    p.vx = u[2] * rr;
+    p.vy = u[3] * rr;
     const double vv = p.vx * p.vx + p.vy * p.vy;
     const double bb = u[4] * u[4] + u[5] * u[5];
     p.pr = (gm - 1.0) * (u[1] - 0.5 * u[0] * vv - pi8i * bb);
@@ -112,19 +120,20 @@ CANS_DEV P6 prims(const double *u, double gm) {   // mlw_m.f:299-321
 CANS_DEV void prim_adj(const P6 &p, const P6 &l, double gm, double *lu) {
     const double g1 = gm - 1.0;
     const double vv = p.vx * p.vx + p.vy * p.vy;
-    lu[0] = l.ro + l.pr * g1 * 0.5 * vv - (l.vx * p.vx + l.vy * p.vy) / p.ro;
+    const double rr = 1.0 / p.ro;
+    lu[0] = l.ro + l.pr * g1 * 0.5 * vv - (l.vx * p.vx + l.vy * p.vy) * rr;
     lu[1] = l.pr * g1;
-    lu[2] = l.vx / p.ro - l.pr * g1 * p.vx;
-    lu[3] = l.vy / p.ro - l.pr * g1 * p.vy;
+    lu[2] = l.vx * rr - l.pr * g1 * p.vx;
+    lu[3] = l.vy * rr - l.pr * g1 * p.vy;
     lu[4] = l.bx - l.pr * g1 * 2.0 * pi8i * p.bx;
     lu[5] = l.by - l.pr * g1 * 2.0 * pi8i * p.by;
 }
 
 // lam of U(P) -> lam of P (added)
-CANS_DEV void cons_adj(const P6 &p, const double *lu, double gm, P6 &l) {
+CANS_DEV void cons_adj(const P6 &p, const double *lu, double g1i, P6 &l) {
     const double vv = p.vx * p.vx + p.vy * p.vy;
     l.ro += lu[0] + lu[1] * 0.5 * vv + lu[2] * p.vx + lu[3] * p.vy;
-    l.pr += lu[1] / (gm - 1.0);
+    l.pr += lu[1] * g1i;
     l.vx += lu[1] * p.ro * p.vx + lu[2] * p.ro;
     l.vy += lu[1] * p.ro * p.vy + lu[3] * p.ro;
     l.bx += lu[1] * 2.0 * pi8i * p.bx + lu[4];
@@ -132,18 +141,18 @@ CANS_DEV void cons_adj(const P6 &p, const double *lu, double gm, P6 &l) {
 }
 
 // lam of the flux quantities (x: rx eex rxx ryx, y: ry eey rxy ryy, ez) -> lam of P (added)
-CANS_DEV void flux_adj(const P6 &p, double gm, double lrx, double leex, double lrxx, double lryx, double lry,
+CANS_DEV void flux_adj(const P6 &p, double gm, double g1i, double lrx, double leex, double lrxx, double lryx, double lry,
                        double leey, double lrxy, double lryy, double lez, P6 &l) {
     const double ro = p.ro, pr = p.pr, vx = p.vx, vy = p.vy, bx = p.bx, by = p.by;
     (void)pr;
     const double vv = vx * vx + vy * vy;
     const double ez = -vx * by + vy * bx;
-    const double ep = p.pr * gm / (gm - 1.0) + 0.5 * ro * vv;
+    const double ep = p.pr * gm * g1i + 0.5 * ro * vv;
     const double lez_t = lez - leex * by * pi4i + leey * bx * pi4i;
     const double lep = leex * vx + leey * vy;
     const double lxy = lrxy + lryx;
     l.ro += lrx * vx + lry * vy + lrxx * vx * vx + lxy * vx * vy + lryy * vy * vy + lep * 0.5 * vv;
-    l.pr += lep * gm / (gm - 1.0) + lrxx + lryy;
+    l.pr += lep * gm * g1i + lrxx + lryy;
     l.vx += lrx * ro + leex * ep + lrxx * 2.0 * ro * vx + lxy * ro * vy + lep * ro * vx - lez_t * by;
     l.vy += lry * ro + leey * ep + lxy * ro * vx + lryy * 2.0 * ro * vy + lep * ro * vy + lez_t * bx;
     l.bx += leey * ez * pi4i - lrxx * 2.0 * pi8i * bx - lxy * pi4i * by + lryy * 2.0 * pi8i * bx + lez_t * vy;
@@ -286,13 +295,13 @@ __global__ __launch_bounds__(256) void adj_b2(AdjArgs a) {
     double part = 0.0;
     if (t < n) {
         const int ci = (int)(t % (ix - 1)) + 1, cj = (int)(t / (ix - 1)) + 1;
-        const double gm = a.gm, dt = step_dt(a), hdt = 0.5 * dt;
+        const double gm = a.gm, g1i = 1.0 / (gm - 1.0), dt = step_dt(a), hdt = 0.5 * dt;
         if (t == 0 && a.acc_zero) {   // B1 of this step has read them; B1 of step n-1 counts into them
             for (int q = 0; q < kAdjAccSlot; q++) a.acc_zero[q] = 0.0;
         }
         const size_t kA = at(ci, cj, ix), kB = at(ci + 1, cj, ix), kC = at(ci, cj + 1, ix), kD = at(ci + 1, cj + 1, ix);
-        const CellV A = cellv(load6(a.p, kA), gm), B = cellv(load6(a.p, kB), gm);
-        const CellV Cc = cellv(load6(a.p, kC), gm), D = cellv(load6(a.p, kD), gm);
+        const CellV A = cellv(load6(a.p, kA), gm, g1i), B = cellv(load6(a.p, kB), gm, g1i);
+        const CellV Cc = cellv(load6(a.p, kC), gm, g1i), D = cellv(load6(a.p, kD), gm, g1i);
         const double dxim = a.mx.dim[ci - 1], dyim = a.my.dim[cj - 1];
         // half step (mlwhalf.f:47-50)
         double uh[6], dh[6];
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(256) void adj_b2(AdjArgs a) {
             dh[q] = -dfdx - dfdy;                       // d uh / d dt
         }
         const P6 ph = prims(uh, gm);
-        const CellV H = cellv(ph, gm);                  // corner fluxes (H.fx, H.gy)
+        const CellV H = cellv(ph, gm, g1i);             // corner fluxes (H.fx, H.gy)
         // full-step weights of this corner in its 4 cells' du (mlwfull.f:31-41)
         const bool iA = interior(a, ci, cj), iB = interior(a, ci + 1, cj);
         const bool iC = interior(a, ci, cj + 1), iD = interior(a, ci + 1, cj + 1);
@@ -329,7 +338,7 @@ __global__ __launch_bounds__(256) void adj_b2(AdjArgs a) {
             part += s / dt;
         }
         P6 lp{0, 0, 0, 0, 0, 0};
-        flux_adj(ph, gm, lH[0], lH[1], lH[2], lH[3], lK[0], lK[1], lK[2], lK[3], -lH[5] + lK[4], lp);
+        flux_adj(ph, gm, g1i, lH[0], lH[1], lH[2], lH[3], lK[0], lK[1], lK[2], lK[3], -lH[5] + lK[4], lp);
         double lu[6];
         prim_adj(ph, lp, gm, lu);
         for (int q = 0; q < 6; q++) {
@@ -362,10 +371,10 @@ __global__ __launch_bounds__(256) void adj_b3(AdjArgs a) {
     double part = 0.0;
     if (k < n) {
         const int i = (int)(k % ix) + 1, j = (int)(k / ix) + 1;
-        const double gm = a.gm, dt = step_dt(a), hdt = 0.5 * dt;
+        const double gm = a.gm, g1i = 1.0 / (gm - 1.0), dt = step_dt(a), hdt = 0.5 * dt;
         const bool ic = interior(a, i, j);
         const P6 pc = load6(a.p, k);
-        const CellV C = cellv(pc, gm);
+        const CellV C = cellv(pc, gm, g1i);
         double lc[6];
         for (int q = 0; q < 6; q++) lc[q] = ic ? a.ldu[q][k] : 0.0;
         const double dxi_c = a.mx.di[i - 1], dyi_c = a.my.di[j - 1];
@@ -389,7 +398,7 @@ __global__ __launch_bounds__(256) void adj_b3(AdjArgs a) {
             const size_t kn = at(in_, jn, ix);
             const bool inb = interior(a, in_, jn);
             const P6 pn = load6(a.p, kn);
-            const CellV Nb = cellv(pn, gm);
+            const CellV Nb = cellv(pn, gm, g1i);
             double ln[6];
             for (int q = 0; q < 6; q++) ln[q] = inb ? a.ldu[q][kn] : 0.0;
             // the face between this cell and the neighbour: its AV coefficient
@@ -443,9 +452,9 @@ __global__ __launch_bounds__(256) void adj_b3(AdjArgs a) {
             }
         }
         P6 l{0.0, 0.0, G.lvx, G.lvy, 0.0, 0.0};
-        cons_adj(pc, G.lu, gm, l);
+        cons_adj(pc, G.lu, g1i, l);
         // x fluxes rx eex rxx ryx (bx: none, by: -ez); y fluxes ry eey rxy ryy (bx: ez, by: none)
-        flux_adj(pc, gm, G.lf[0], G.lf[1], G.lf[2], G.lf[3], G.lg[0], G.lg[1], G.lg[2], G.lg[3], -G.lf[5] + G.lg[4], l);
+        flux_adj(pc, gm, g1i, G.lf[0], G.lf[1], G.lf[2], G.lf[3], G.lg[0], G.lg[1], G.lg[2], G.lg[3], -G.lf[5] + G.lg[4], l);
         a.lout[0][k] = l.ro; a.lout[1][k] = l.pr; a.lout[2][k] = l.vx;
         a.lout[3][k] = l.vy; a.lout[4][k] = l.bx; a.lout[5][k] = l.by;
     }
