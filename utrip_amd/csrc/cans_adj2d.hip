@@ -56,6 +56,9 @@ struct P6 {
     double ro, pr, vx, vy, bx, by;
 };
 
+// The reverse kernels take their blocks in the XCD-contiguous order of
+// xcd_first_thread() (cans_common.hpp): the neighbour rows a cell's gathers
+// read are then mostly fetched into one L2.
 CANS_DEV size_t at(int i, int j, int ix) { return (size_t)(i - 1) + (size_t)ix * (size_t)(j - 1); }
 
 CANS_DEV P6 load6(const double *const *f, size_t k) {
@@ -245,7 +248,7 @@ CANS_DEV void dt_grad(const P6 &p, double q, double S, double ldt, unsigned ntie
 __global__ __launch_bounds__(256) void adj_b1(AdjArgs a) {
     const int ix = a.ix, jx = a.jx;
     const size_t n = (size_t)ix * jx;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t k = xcd_first_thread() + threadIdx.x;
     if (k >= n) return;
     const int i = (int)(k % ix) + 1, j = (int)(k / ix) + 1;
     double S;
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(256) void adj_b1(AdjArgs a) {
 __global__ __launch_bounds__(256) void adj_b2(AdjArgs a) {
     const int ix = a.ix, jx = a.jx;
     const size_t n = (size_t)(ix - 1) * (jx - 1);
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t t = xcd_first_thread() + threadIdx.x;
     double part = 0.0;
     if (t < n) {
         const int ci = (int)(t % (ix - 1)) + 1, cj = (int)(t / (ix - 1)) + 1;
@@ -367,7 +370,7 @@ struct Gather {
 __global__ __launch_bounds__(256) void adj_b3(AdjArgs a) {
     const int ix = a.ix, jx = a.jx;
     const size_t n = (size_t)ix * jx;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t k = xcd_first_thread() + threadIdx.x;
     double part = 0.0;
     if (k < n) {
         const int i = (int)(k % ix) + 1, j = (int)(k / ix) + 1;
@@ -513,10 +516,9 @@ hipError_t launch_adj2d_step(const double *const *pn, const double *const *pn1, 
     a.pend_acc = pend_acc;
     a.acc_zero = acc_zero;
     const size_t n = (size_t)ix * jx;
-    const int nb = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(adj_b1, dim3(nb), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(adj_b2, dim3((int)(((size_t)(ix - 1) * (jx - 1) + 255) / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(adj_b3, dim3(nb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(adj_b1, dim3(xcd_grid(n)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(adj_b2, dim3(xcd_grid((size_t)(ix - 1) * (jx - 1))), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(adj_b3, dim3(xcd_grid(n)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

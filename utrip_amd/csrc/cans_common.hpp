@@ -32,6 +32,23 @@ CANS_DEV float fminF(float a, float b) { return (a < b) ? a : b; }
 CANS_DEV float signF(float a, float b) { return __builtin_copysignf(__builtin_fabsf(a), b); }
 template <class T> CANS_DEV T sq(T a) { return a * a; }
 
+// Block order across the XCDs.  The hardware deals the blocks of a launch
+// to the 8 XCDs round-robin (block b to XCD b % 8), each with its own L2.
+// A kernel that takes block b as the logical block (b % 8) * (grid / 8) +
+// b / 8 of its linear order has each XCD work through one contiguous eighth
+// of the grid, so the neighbouring rows its stencils share are fetched into
+// one L2 instead of several.  Launch such kernels with xcd_grid() blocks (a
+// multiple of 8; the extra blocks fall past the end and exit).
+constexpr unsigned kXcds = 8;
+CANS_DEV size_t xcd_first_thread() {
+    const unsigned per = gridDim.x / kXcds;
+    return (size_t)((blockIdx.x % kXcds) * per + blockIdx.x / kXcds) * blockDim.x;
+}
+inline unsigned xcd_grid(size_t nthreads, unsigned block = 256) {
+    const size_t nb = (nthreads + block - 1) / block;
+    return (unsigned)((nb + kXcds - 1) / kXcds * kXcds);
+}
+
 // Order-preserving map of a double onto uint64 so that an unsigned atomic
 // min over the keys is a min over the doubles (NaNs are filtered before).
 CANS_DEV uint64_t dkey(double d) {

@@ -140,22 +140,8 @@ __constant__ int kCornFlux[8][3][2] = {
     {{H_EY, 1}, {H_EX, -1}, {-1, 1}},
 };
 
-// Block order: the hardware deals blocks to the 8 XCDs round-robin (block b
-// to XCD b % 8).  The kernels below take block b as the logical block
-// (b % 8) * per + b / 8 of the linear cell order, so each XCD runs one
-// contiguous eighth of the grid and the rows a block's stencil shares with
-// its neighbours (the next blocks in that order) are read through the same
-// L2.  The grid is padded to a multiple of 8; blocks past the end exit.
-constexpr unsigned kXcds = 8;
-CANS_DEV size_t xcd_block() {
-    const unsigned per = gridDim.x / kXcds;
-    return (size_t)((blockIdx.x % kXcds) * per + blockIdx.x / kXcds);
-}
-inline unsigned xcd_grid(size_t nthreads) {
-    const size_t nb = (nthreads + 255) / 256;
-    return (unsigned)((nb + kXcds - 1) / kXcds * kXcds);
-}
-
+// The three kernels take their blocks in the XCD-contiguous order of
+// xcd_first_thread() (cans_common.hpp).
 CANS_DEV bool step_dt3(const Mlw3dArgs &a, double &dt) {
     if (!a.clk) { dt = a.dt; return true; }
     const int d = (a.step + 1) & 1;
@@ -169,7 +155,7 @@ __global__ __launch_bounds__(256) void mlw3d_cells(Mlw3dArgs a) {
     double dt;
     if (!step_dt3(a, dt)) return;
     const size_t N = (size_t)a.ix * a.jx * a.kx;
-    const size_t p = xcd_block() * blockDim.x + threadIdx.x;
+    const size_t p = xcd_first_thread() + threadIdx.x;
     if (p >= N) return;
     const double gm = a.gm;
     const double ro = a.prim[0][p], pr = a.prim[1][p], vx = a.prim[2][p], vy = a.prim[3][p], vz = a.prim[4][p];
@@ -319,7 +305,7 @@ __global__ __launch_bounds__(256) void mlw3d_corners(Mlw3dArgs a) {
     if (!step_dt3(a, dt)) return;
     const int ix = a.ix, jx = a.jx, kx = a.kx;
     const int ci = ix - 1, cj = jx - 1, ck = kx - 1;
-    const size_t t = xcd_block() * blockDim.x + threadIdx.x;
+    const size_t t = xcd_first_thread() + threadIdx.x;
     if (t >= (size_t)ci * cj * ck) return;
     const int i = 1 + (int)(t % ci);
     const int j = 1 + (int)((t / ci) % cj);
@@ -485,7 +471,7 @@ __global__ __launch_bounds__(256) void mlw3d_update(Mlw3dArgs a) {
     if (!step_dt3(a, dt)) return;
     const int ix = a.ix, jx = a.jx, kx = a.kx;
     const int ni = ix - 2, nj = jx - 2, nk = kx - 2;
-    const size_t t = xcd_block() * blockDim.x + threadIdx.x;
+    const size_t t = xcd_first_thread() + threadIdx.x;
     if (t >= (size_t)ni * nj * nk) return;
     const int i = 2 + (int)(t % ni);
     const int j = 2 + (int)((t / ni) % nj);
