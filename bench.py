@@ -487,7 +487,9 @@ def batch2d_figures(nrun: int = 1024, tend: float = 0.2):
     (nx, ny uniform in 32..256, pressure scale 0.1..10, seeded:
     problems.ot_run_set), each running the cans2d main loop to its own
     t >= tend in one persistent kernel (cans2d_batch_run), FP64, against the
-    same runs looped one after another through the single-run engine."""
+    same runs looped one after another through the single-run engine.  The
+    launch shape comes from the autotuner (cans2d_batch_tune); the timed run
+    is a replay of the batch's hipGraph."""
     import numpy as np
     import torch
 
@@ -504,27 +506,23 @@ def batch2d_figures(nrun: int = 1024, tend: float = 0.2):
     out = {"runs": nrun, "cells": int(cells.sum()), "tend": tend, "n_range": [32, 256]}
     b = cans.Batch2D(runs, pp["gm"], pp["qav"], pp["safety"], 1)
     init = [t.clone() for t in b.f]
-    best = None
-    for shape in ((256, 2), (128, 4), (512, 1), (64, 8)):
-        for t_, i0 in zip(b.f, init):
-            t_.copy_(i0)
-        ev = cans.EventSet(2)
-        torch.cuda.synchronize()
-        ev.record(0)
-        b.run(tend=tend, block_threads=shape[0], blocks_per_cu=shape[1])
-        ev.record(1)
-        torch.cuda.synchronize()
-        ms = ev.elapsed_ms(0, 1)
-        ns = b.ns.cpu().numpy().astype(np.int64)
-        assert (b.merr.cpu().numpy() == 0).all() and (b.t.cpu().numpy() >= tend).all()
-        zu = int((cells * ns).sum())
-        out.setdefault("by_launch_shape", {})[f"{shape[0]}x{shape[1]}"] = zu / (ms / 1e3)
-        if best is None or ms < best[0]:
-            best = (ms, zu, shape, ns)
-    ms, zu, shape, ns = best
-    out["fp64"] = {"zone_updates": zu, "ms": ms, "zone_updates_per_s": zu / (ms / 1e3), "launch_shape": list(shape),
-                   "steps_min": int(ns.min()), "steps_max": int(ns.max()),
-                   "hbm_frac_alg": B_ALG_2D * zu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    bt, bpc, tune_ms = b.tune(probe_steps=10)
+    g = b.graph(tend=tend, init=init)
+    g.launch()                      # warm-up replay
+    torch.cuda.synchronize()
+    ev = cans.EventSet(2)
+    ev.record(0)
+    g.launch()                      # timed replay: the whole batch from the initial state
+    ev.record(1)
+    torch.cuda.synchronize()
+    ms = ev.elapsed_ms(0, 1)
+    ns = b.ns.cpu().numpy().astype(np.int64)
+    assert (b.merr.cpu().numpy() == 0).all() and (b.t.cpu().numpy() >= tend).all()
+    zu = int((cells * ns).sum())
+    out["fp64"] = {"zone_updates": zu, "ms": ms, "zone_updates_per_s": zu / (ms / 1e3), "launch_shape": [bt, bpc],
+                   "tune_probe_ms": tune_ms, "steps_min": int(ns.min()), "steps_max": int(ns.max()),
+                   "hbm_frac_alg": B_ALG_2D * zu / (ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                   "note": "autotuned launch shape; the hipGraph of the whole batch run, replayed"}
     # the alternative the batch replaces: each run through the resident
     # single-run loop (MlwRun2D, one fused launch per step), one after
     # another, for the steps the batch took; timed on the first `nloop` runs

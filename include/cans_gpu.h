@@ -447,16 +447,41 @@ int cans1d_batch_run_steps(int ntube, const long long *off, const int *ix, const
  * call starts each run's loop at t = 0, ns = 0 from the state it finds and
  * leaves the final state in state7, t, ns and merr (9001 if dt < dtmin).
  * block_threads in {64,...,512} with block_threads*blocks_per_cu <= 512.
- * ws: cans2d_batch_ws_bytes.  The call synchronises the stream once (to
- * upload the per-run argument table), then launches asynchronously.  No
- * single counterpart in the reference, whose 2D driver runs one problem. */
-size_t cans2d_batch_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun);
+ * ws: cans2d_batch_ws_bytes (tune = 1 adds room for cans2d_batch_tune's
+ * copy of the state).  The call synchronises the stream once (to upload the
+ * per-run argument table into ws), then launches asynchronously.  No single
+ * counterpart in the reference, whose 2D driver runs one problem. */
+size_t cans2d_batch_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun, int tune);
 int cans2d_batch_run(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
                      const long long *offy, const int *order, double *const *state7, const double *dx,
                      const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
                      long long ny1d, double gm, double qav, double safety, double dtmin, double tend, int nstop,
                      int margin, double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
                      void *ws, void *stream);
+/* Launch-shape autotuner of cans2d_batch_run: times every (block_threads,
+ * blocks_per_cu) with block_threads*blocks_per_cu <= 512 on a copy of the
+ * state (probe_steps steps of every run; the state is unchanged, t_out,
+ * ns_out, merr_out receive the probe's values) and returns the fastest;
+ * cached per (nrun, ncells, nx1d+ny1d, probe_steps).  ws must have been
+ * sized with tune = 1. */
+int cans2d_batch_tune(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
+                      const long long *offy, const int *order, double *const *state7, const double *dx,
+                      const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
+                      long long ny1d, double gm, double qav, double safety, double dtmin, int margin, int probe_steps,
+                      double *t_out, int *ns_out, int *merr_out, void *ws, int *block_threads, int *blocks_per_cu,
+                      float *best_ms, void *stream);
+/* cans2d_batch_run captured as an instantiated hipGraph (*graph; replay
+ * with cans_graph_launch, free with cans_graph_destroy).  init7 (7 device
+ * pointers, or NULL): copied into state7 at the start of every replay, so
+ * replays repeat the same run.  The per-run argument table is uploaded into
+ * ws before the capture; the graph reads it from there, so ws must not be
+ * used by other calls while the graph is in use. */
+int cans2d_batch_graph(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
+                       const long long *offy, const int *order, double *const *state7, const double *dx,
+                       const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
+                       long long ny1d, double gm, double qav, double safety, double dtmin, double tend, int nstop,
+                       int margin, double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
+                       void *ws, const double *const *init7, void **graph);
 /* Autotuner of the persistent kernel's launch shape: times every
  * (block_threads, blocks_per_cu) on a copy of the state for probe_steps
  * steps per tube and returns the fastest (cached per batch shape for the

@@ -109,3 +109,37 @@ def test_batch2d_second_run_restarts_from_state(dev, oracle):
         for q in F2:
             assert_bitwise(got[q], ref[q], f"run {k} {q}")
         assert t[k] == tr and ns[k] == nsr
+
+
+def test_batch2d_tune_and_graph_replays(dev, oracle):
+    """The autotuner leaves the state unchanged and picks a legal shape; the
+    hipGraph of the batch run, replayed three times from the same initial
+    state (with eager work and synchronisations between the replays), equals
+    the oracle's loop for every run on every replay."""
+    from utrip_amd import cans
+    print("CASE batch2d tune + graph", flush=True)
+    runs, pp = [], None
+    for nx, ny, ps in SPECS:
+        r, pp = make_run(nx, ny, 1, ps)
+        runs.append(r)
+    b = cans.Batch2D(runs, pp["gm"], pp["qav"], pp["safety"], 1)
+    init = [t.clone() for t in b.f]
+    bt, bpc, ms = b.tune(probe_steps=3)
+    torch.cuda.synchronize()
+    assert bt in (64, 128, 256, 512) and bpc >= 1 and bt * bpc <= 512
+    for t0, t1 in zip(init, b.f):
+        assert torch.equal(t0, t1)
+    refs = [oracle_run(oracle, r, pp, 1, 0.05, 10**6) for r in runs]
+    g = b.graph(tend=0.05, init=init)
+    for rep in range(3):
+        for t in b.f:            # eager work on the state between replays
+            t.mul_(2.0)
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        t, ns = b.t.cpu().numpy(), b.ns.cpu().numpy()
+        for k, (ref, tr, nsr) in enumerate(refs):
+            got = b.run_fields(k)
+            for q in F2:
+                assert_bitwise(got[q], ref[q], f"replay {rep} run {k} {q}")
+            assert t[k] == tr and ns[k] == nsr, (rep, k)

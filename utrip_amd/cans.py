@@ -705,7 +705,7 @@ class Batch2D:
 
     FIELDS = ("ro", "pr", "vx", "vy", "bx", "by", "az")
 
-    def __init__(self, runs, gm, qav, safety, margin, dtmin=1e-10, device="cuda:0"):
+    def __init__(self, runs, gm, qav, safety, margin, dtmin=1e-10, device="cuda:0", tune_room=True):
         import numpy as np
         self.n = len(runs)
         self.gm, self.qav, self.safety, self.margin, self.dtmin = gm, qav, safety, margin, dtmin
@@ -745,23 +745,57 @@ class Batch2D:
         self.merr = torch.zeros(self.n, dtype=torch.int32, device=dev)
         lib().cans2d_batch_ws_bytes.restype = C.c_size_t
         nb = lib().cans2d_batch_ws_bytes(C.c_longlong(self.ncells), C.c_longlong(self.nx1d), C.c_longlong(self.ny1d),
-                                         C.c_int(self.n))
+                                         C.c_int(self.n), C.c_int(1 if tune_room else 0))
         self.ws = torch.empty((nb + 7) // 8, dtype=torch.float64, device=dev)
+        self.shape = (512, 1)   # (block_threads, blocks_per_cu) until tuned
+        self._graphs = []
         torch.cuda.synchronize()
 
-    def run(self, tend: float, nstop: int = 2**31 - 1, block_threads: int = 256, blocks_per_cu: int = 2):
+    def _head(self):
         n = self.n
         st = (C.c_void_p * 7)(*[_ptr(t, (self.ncells,)).value for t in self.f])
-        _check(lib().cans2d_batch_run(
-            C.c_int(n), _iptr(self.off, torch.int64, n, "off"), _iptr(self.ix, torch.int32, n, "ix"),
-            _iptr(self.jx, torch.int32, n, "jx"), _iptr(self.offx, torch.int64, n, "offx"),
-            _iptr(self.offy, torch.int64, n, "offy"), _iptr(self.order, torch.int32, n, "order"), st,
-            _ptr(self.g1[0], (self.nx1d,)), _ptr(self.g1[1], (self.nx1d,)), _ptr(self.g1[2], (self.ny1d,)),
-            _ptr(self.g1[3], (self.ny1d,)), C.c_longlong(self.ncells), C.c_longlong(self.nx1d),
-            C.c_longlong(self.ny1d), _d(self.gm), _d(self.qav), _d(self.safety), _d(self.dtmin), _d(tend),
-            C.c_int(nstop), C.c_int(self.margin), _ptr(self.t, (n,)), _iptr(self.ns, torch.int32, n, "ns"),
-            _iptr(self.merr, torch.int32, n, "merr"), C.c_int(block_threads), C.c_int(blocks_per_cu),
-            _ptr(self.ws), _stream()))
+        self._st = st
+        return [C.c_int(n), _iptr(self.off, torch.int64, n, "off"), _iptr(self.ix, torch.int32, n, "ix"),
+                _iptr(self.jx, torch.int32, n, "jx"), _iptr(self.offx, torch.int64, n, "offx"),
+                _iptr(self.offy, torch.int64, n, "offy"), _iptr(self.order, torch.int32, n, "order"), st,
+                _ptr(self.g1[0], (self.nx1d,)), _ptr(self.g1[1], (self.nx1d,)), _ptr(self.g1[2], (self.ny1d,)),
+                _ptr(self.g1[3], (self.ny1d,)), C.c_longlong(self.ncells), C.c_longlong(self.nx1d),
+                C.c_longlong(self.ny1d), _d(self.gm), _d(self.qav), _d(self.safety), _d(self.dtmin)]
+
+    def _outs(self):
+        n = self.n
+        return [_ptr(self.t, (n,)), _iptr(self.ns, torch.int32, n, "ns"), _iptr(self.merr, torch.int32, n, "merr")]
+
+    def tune(self, probe_steps: int = 10):
+        """Pick (block_threads, blocks_per_cu) by timing every candidate on a
+        copy of the state (probe_steps steps of every run; the state is
+        unchanged)."""
+        bt, bpc, ms = C.c_int(), C.c_int(), C.c_float()
+        _check(lib().cans2d_batch_tune(*self._head(), C.c_int(self.margin), C.c_int(probe_steps), *self._outs(),
+                                       _ptr(self.ws), C.byref(bt), C.byref(bpc), C.byref(ms), _stream()))
+        self.shape = (bt.value, bpc.value)
+        return bt.value, bpc.value, ms.value
+
+    def run(self, tend: float, nstop: int = 2**31 - 1, block_threads: int = None, blocks_per_cu: int = None):
+        bt = block_threads if block_threads is not None else self.shape[0]
+        bpc = blocks_per_cu if blocks_per_cu is not None else self.shape[1]
+        _check(lib().cans2d_batch_run(*self._head(), _d(tend), C.c_int(nstop), C.c_int(self.margin), *self._outs(),
+                                      C.c_int(bt), C.c_int(bpc), _ptr(self.ws), _stream()))
+
+    def graph(self, tend: float, nstop: int = 2**31 - 1, init=None):
+        """An instantiated hipGraph of the whole batch run; ``init`` (7
+        packed device tensors ro..az) is copied into the state at the start
+        of every replay."""
+        bt, bpc = self.shape
+        arr = None
+        if init is not None:
+            arr = (C.c_void_p * 7)(*[_ptr(t, (self.ncells,)).value for t in init])
+        h = C.c_void_p()
+        _check(lib().cans2d_batch_graph(*self._head(), _d(tend), C.c_int(nstop), C.c_int(self.margin), *self._outs(),
+                                        C.c_int(bt), C.c_int(bpc), _ptr(self.ws), arr, C.byref(h)))
+        g = BatchGraph(h)
+        self._graphs.append(g)
+        return g
 
     def run_fields(self, k: int):
         """Host copies (Fortran order, (ix, jx)) of run k's fields."""

@@ -301,82 +301,85 @@ int cans_graph_destroy(void *graph) {
 // Batched periodic 2D runs (MLW)
 namespace {
 
-struct Batch2In {
-    int nrun;
-    const long long *off, *offx, *offy;
-    const int *ix, *jx, *order;
-    double *f[7];
-    const double *dx, *dxm, *dy, *dym;
-    long long ncells, nx1d, ny1d;
-    double gm, qav, safety, dtmin, tend;
-    int nstop, margin;
-    double *t_out;
-    int *ns_out, *merr_out;
-    void *ws;
-};
-
 // workspace: the scratch state (7 fields), the x and y metric (4 arrays
-// each), the per-run argument table (2 per run), the queue counter
-size_t b2_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun) {
+// each), the per-run argument table (2 per run), the queue counter, and
+// (tune) a copy of the state for the tuner's probes
+size_t b2_table_bytes(int nrun) { return (2 * (size_t)nrun * sizeof(Mlw2dArgs) + 255) & ~size_t(255); }
+size_t b2_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun, int tune) {
     return (7 * al((size_t)ncells) + 4 * al((size_t)nx1d) + 4 * al((size_t)ny1d)) * sizeof(double) +
-           ((2 * (size_t)nrun * sizeof(Mlw2dArgs) + 255) & ~size_t(255)) + 256;
+           b2_table_bytes(nrun) + 256 + (tune ? 7 * al((size_t)ncells) * sizeof(double) : 0);
+}
+double *b2_tune_copy(void *ws, long long ncells, long long nx1d, long long ny1d, int nrun) {
+    return (double *)((char *)ws + b2_ws_bytes(ncells, nx1d, ny1d, nrun, 0));
 }
 
-}  // namespace
-
-extern "C" {
-
-size_t cans2d_batch_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun) {
-    return b2_ws_bytes(ncells, nx1d, ny1d, nrun);
+__global__ void copy7_kernel(Batch2dArgs dst, Batch2dArgs src, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+#pragma unroll
+        for (int q = 0; q < 7; q++) dst.f[q][i] = src.f[q][i];
+}
+hipError_t copy7(double *const *dst, const double *const *src, long long n, hipStream_t s) {
+    Batch2dArgs d{}, c{};
+    for (int q = 0; q < 7; q++) {
+        d.f[q] = dst[q];
+        c.f[q] = const_cast<double *>(src[q]);
+    }
+    hipLaunchKernelGGL(copy7_kernel, dim3(2048), dim3(256), 0, s, d, c, n);
+    return hipGetLastError();
 }
 
-int cans2d_batch_run(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
-                     const long long *offy, const int *order, double *const *state7, const double *dx,
-                     const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
-                     long long ny1d, double gm, double qav, double safety, double dtmin, double tend, int nstop,
-                     int margin, double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
-                     void *ws, void *stream) {
-    hipStream_t s = (hipStream_t)stream;
+bool b2_shape_ok(int bt, int bpc) { return shape_ok(bt, bpc) && bt * bpc <= 512; }
+
+using Tune2Key = std::tuple<int, long long, long long, int>;
+std::map<Tune2Key, std::pair<int, int>> &tune2_cache() {
+    static std::map<Tune2Key, std::pair<int, int>> m;
+    return m;
+}
+
+// Checks the layout on the host (the kernel trusts it), fills the launch
+// arguments for the state `state7` and uploads the per-run argument table
+// (synchronously: the host copy must outlive the transfer).
+int b2_prepare(const char *who, int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
+               const long long *offy, const int *order, double *const *state7, const double *dx, const double *dxm,
+               const double *dy, const double *dym, long long ncells, long long nx1d, long long ny1d, double gm,
+               double qav, double safety, double dtmin, double tend, int nstop, int margin, double *t_out,
+               int *ns_out, int *merr_out, void *ws, hipStream_t s, Batch2dArgs &b) {
     if (nrun < 0 || ncells < 0 || nx1d < 0 || ny1d < 0 || margin < 1 || nstop < 0)
-        return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: nrun=%d ncells=%lld margin=%d nstop=%d", nrun, ncells,
-                        margin, nstop);
-    if (nrun == 0) return 0;
+        return abi_fail(CANS_ERR_ARG, "%s: nrun=%d ncells=%lld margin=%d nstop=%d", who, nrun, ncells, margin, nstop);
     if (!state7 || !off || !ix || !jx || !offx || !offy || !dx || !dxm || !dy || !dym || !t_out || !ns_out ||
         !merr_out || !ws)
-        return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: NULL argument");
+        return abi_fail(CANS_ERR_ARG, "%s: NULL argument", who);
     for (int q = 0; q < 7; q++)
-        if (!state7[q]) return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: NULL field %d", q);
-    if (!shape_ok(block_threads, blocks_per_cu) || block_threads * blocks_per_cu > 512)
-        return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: block_threads=%d blocks_per_cu=%d (at most 512 threads "
-                        "per CU: two waves per SIMD)", block_threads, blocks_per_cu);
-    // host check of the layout (the kernel trusts it)
+        if (!state7[q]) return abi_fail(CANS_ERR_ARG, "%s: NULL field %d", who, q);
     std::vector<long long> o(nrun), ox(nrun), oy(nrun);
     std::vector<int> nx(nrun), ny(nrun);
-    HIPCHK(hipMemcpy(o.data(), off, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ox.data(), offx, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(oy.data(), offy, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(nx.data(), ix, sizeof(int) * nrun, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(ny.data(), jx, sizeof(int) * nrun, hipMemcpyDeviceToHost));
+    if (nrun > 0) {
+        HIPCHK(hipMemcpy(o.data(), off, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ox.data(), offx, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(oy.data(), offy, sizeof(long long) * nrun, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(nx.data(), ix, sizeof(int) * nrun, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ny.data(), jx, sizeof(int) * nrun, hipMemcpyDeviceToHost));
+    }
     for (int k = 0; k < nrun; k++) {
         const bool ok = nx[k] >= 3 && ny[k] >= 3 && nx[k] - 2 * margin >= margin && ny[k] - 2 * margin >= margin &&
                         o[k] >= 0 && o[k] + (long long)nx[k] * ny[k] <= ncells && ox[k] >= 0 &&
                         ox[k] + nx[k] <= nx1d && oy[k] >= 0 && oy[k] + ny[k] <= ny1d;
         if (!ok)
-            return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: run %d has ix=%d jx=%d off=%lld offx=%lld offy=%lld "
+            return abi_fail(CANS_ERR_ARG, "%s: run %d has ix=%d jx=%d off=%lld offx=%lld offy=%lld "
                             "(ncells=%lld nx1d=%lld ny1d=%lld margin=%d; the periodic box needs n-2*margin >= margin)",
-                            k, nx[k], ny[k], o[k], ox[k], oy[k], ncells, nx1d, ny1d, margin);
+                            who, k, nx[k], ny[k], o[k], ox[k], oy[k], ncells, nx1d, ny1d, margin);
     }
-    if (order) {
+    if (order && nrun > 0) {
         std::vector<int> ord(nrun);
         HIPCHK(hipMemcpy(ord.data(), order, sizeof(int) * nrun, hipMemcpyDeviceToHost));
         std::vector<char> seen(nrun, 0);
         for (int k = 0; k < nrun; k++) {
             if (ord[k] < 0 || ord[k] >= nrun || seen[ord[k]])
-                return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: order is not a permutation (entry %d = %d)", k, ord[k]);
+                return abi_fail(CANS_ERR_ARG, "%s: order is not a permutation (entry %d = %d)", who, k, ord[k]);
             seen[ord[k]] = 1;
         }
     }
-    Batch2dArgs b{};
+    b = Batch2dArgs{};
     b.nrun = nrun;
     b.off = off; b.offx = offx; b.offy = offy; b.ix = ix; b.jx = jx; b.order = order;
     double *w = (double *)ws;
@@ -390,13 +393,13 @@ int cans2d_batch_run(int nrun, const long long *off, const int *ix, const int *j
         b.my[q] = w + 7 * nc + 4 * n1x + q * n1y;
     }
     Mlw2dArgs *table = reinterpret_cast<Mlw2dArgs *>(w + 7 * nc + 4 * n1x + 4 * n1y);
-    b.next = reinterpret_cast<unsigned *>(reinterpret_cast<char *>(table) +
-                                          ((2 * (size_t)nrun * sizeof(Mlw2dArgs) + 255) & ~size_t(255)));
+    b.next = reinterpret_cast<unsigned *>(reinterpret_cast<char *>(table) + b2_table_bytes(nrun));
     b.runs = table;
     b.dx = dx; b.dxm = dxm; b.dy = dy; b.dym = dym;
     b.gm = gm; b.qav = qav; b.safety = safety; b.dtmin = dtmin; b.tend = tend;
     b.nstop = nstop; b.margin = margin;
     b.t_out = t_out; b.ns_out = ns_out; b.merr_out = merr_out;
+    if (nrun == 0) return 0;
     // the per-run arguments of the strip march, one per step parity
     std::vector<Mlw2dArgs> h(2 * (size_t)nrun);
     for (int k = 0; k < nrun; k++)
@@ -416,11 +419,148 @@ int cans2d_batch_run(int nrun, const long long *off, const int *ix, const int *j
             a.ix = nx[k]; a.jx = ny[k];
             a.per_margin = margin; a.safety = safety; a.peek = 0; a.dt_log = nullptr;
         }
-    // ordered after any earlier work on the stream that reads the table; the
-    // host copy h must outlive the transfer, the launch itself stays async
+    // ordered after any earlier work on the stream that reads the table
     HIPCHK(hipMemcpyAsync(table, h.data(), h.size() * sizeof(Mlw2dArgs), hipMemcpyHostToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t cans2d_batch_ws_bytes(long long ncells, long long nx1d, long long ny1d, int nrun, int tune) {
+    return b2_ws_bytes(ncells, nx1d, ny1d, nrun, tune);
+}
+
+int cans2d_batch_run(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
+                     const long long *offy, const int *order, double *const *state7, const double *dx,
+                     const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
+                     long long ny1d, double gm, double qav, double safety, double dtmin, double tend, int nstop,
+                     int margin, double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
+                     void *ws, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (!b2_shape_ok(block_threads, blocks_per_cu))
+        return abi_fail(CANS_ERR_ARG, "cans2d_batch_run: block_threads=%d blocks_per_cu=%d (at most 512 threads "
+                        "per CU: two waves per SIMD)", block_threads, blocks_per_cu);
+    Batch2dArgs b;
+    if (int e = b2_prepare("cans2d_batch_run", nrun, off, ix, jx, offx, offy, order, state7, dx, dxm, dy, dym, ncells,
+                           nx1d, ny1d, gm, qav, safety, dtmin, tend, nstop, margin, t_out, ns_out, merr_out, ws, s, b))
+        return e;
+    if (nrun == 0) return 0;
     HIPCHK(launch_batch2d(b, block_threads, cu_count() * blocks_per_cu, s));
+    return 0;
+}
+
+int cans2d_batch_tune(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
+                      const long long *offy, const int *order, double *const *state7, const double *dx,
+                      const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
+                      long long ny1d, double gm, double qav, double safety, double dtmin, int margin, int probe_steps,
+                      double *t_out, int *ns_out, int *merr_out, void *ws, int *block_threads, int *blocks_per_cu,
+                      float *best_ms, void *stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (!block_threads || !blocks_per_cu) return abi_fail(CANS_ERR_ARG, "cans2d_batch_tune: NULL output");
+    if (probe_steps < 1) return abi_fail(CANS_ERR_ARG, "cans2d_batch_tune: probe_steps=%d", probe_steps);
+    if (!state7 || !ws) return abi_fail(CANS_ERR_ARG, "cans2d_batch_tune: NULL argument");
+    // the probes run on a copy of the state (room from cans2d_batch_ws_bytes(..., tune=1))
+    double *tc = b2_tune_copy(ws, ncells, nx1d, ny1d, nrun);
+    const size_t nc = al((size_t)ncells);
+    double *copy[7];
+    for (int q = 0; q < 7; q++) copy[q] = tc + q * nc;
+    Batch2dArgs b;
+    if (int e = b2_prepare("cans2d_batch_tune", nrun, off, ix, jx, offx, offy, order, copy, dx, dxm, dy, dym, ncells,
+                           nx1d, ny1d, gm, qav, safety, dtmin, 1e300, probe_steps, margin, t_out, ns_out, merr_out,
+                           ws, s, b))
+        return e;
+    const Tune2Key key{nrun, ncells, nx1d + ny1d, probe_steps};
+    {
+        std::lock_guard<std::mutex> g(tune_mu);
+        auto it = tune2_cache().find(key);
+        if (it != tune2_cache().end()) {
+            *block_threads = it->second.first;
+            *blocks_per_cu = it->second.second;
+            if (best_ms) *best_ms = -1.0f;
+            return 0;
+        }
+    }
+    if (nrun == 0) {
+        *block_threads = 512;
+        *blocks_per_cu = 1;
+        if (best_ms) *best_ms = 0.0f;
+        return 0;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int bts[] = {64, 128, 256, 512};
+    const int bpcs[] = {1, 2, 4, 8};
+    float best = 1e30f;
+    int bbt = 512, bbpc = 1;
+    for (int bt : bts) {
+        for (int bpc : bpcs) {
+            if (!b2_shape_ok(bt, bpc)) continue;
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2; rep++) {   // the first run warms caches and code
+                HIPCHK(copy7(copy, state7, ncells, s));
+                HIPCHK(hipEventRecord(e0, s));
+                HIPCHK(launch_batch2d(b, bt, cu_count() * bpc, s));
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            }
+            if (ms < best) { best = ms; bbt = bt; bbpc = bpc; }
+        }
+    }
+    HIPCHK(hipEventDestroy(e0));
+    HIPCHK(hipEventDestroy(e1));
+    {
+        std::lock_guard<std::mutex> g(tune_mu);
+        tune2_cache()[key] = {bbt, bbpc};
+    }
+    *block_threads = bbt;
+    *blocks_per_cu = bbpc;
+    if (best_ms) *best_ms = best;
+    return 0;
+}
+
+int cans2d_batch_graph(int nrun, const long long *off, const int *ix, const int *jx, const long long *offx,
+                       const long long *offy, const int *order, double *const *state7, const double *dx,
+                       const double *dxm, const double *dy, const double *dym, long long ncells, long long nx1d,
+                       long long ny1d, double gm, double qav, double safety, double dtmin, double tend, int nstop,
+                       int margin, double *t_out, int *ns_out, int *merr_out, int block_threads, int blocks_per_cu,
+                       void *ws, const double *const *init7, void **graph) {
+    if (!graph) return abi_fail(CANS_ERR_ARG, "cans2d_batch_graph: NULL graph");
+    if (nrun == 0) return abi_fail(CANS_ERR_ARG, "cans2d_batch_graph: empty batch");
+    if (!b2_shape_ok(block_threads, blocks_per_cu))
+        return abi_fail(CANS_ERR_ARG, "cans2d_batch_graph: block_threads=%d blocks_per_cu=%d", block_threads,
+                        blocks_per_cu);
+    if (init7)
+        for (int q = 0; q < 7; q++)
+            if (!init7[q]) return abi_fail(CANS_ERR_ARG, "cans2d_batch_graph: NULL init field %d", q);
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    Batch2dArgs b;
+    // the table is uploaded before the capture; replays read it from the workspace
+    if (int e = b2_prepare("cans2d_batch_graph", nrun, off, ix, jx, offx, offy, order, state7, dx, dxm, dy, dym,
+                           ncells, nx1d, ny1d, gm, qav, safety, dtmin, tend, nstop, margin, t_out, ns_out, merr_out,
+                           ws, s, b)) {
+        (void)hipStreamDestroy(s);
+        return e;
+    }
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        if (init7) e = copy7(state7, init7, ncells, s);   // each replay starts from init7
+        if (e == hipSuccess) e = launch_batch2d(b, block_threads, cu_count() * blocks_per_cu, s);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e == hipSuccess) e = e2;
+        hipGraphExec_t x = nullptr;
+        if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e == hipSuccess) *graph = x;
+    }
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return abi_fail(CANS_ERR_HIP, "cans2d_batch_graph: %s", hipGetErrorString(e));
     return 0;
 }
 
