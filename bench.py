@@ -799,6 +799,69 @@ def _adjoint_run(n: int, nsteps: int, ckpt: int):
             "adj_over_recording_fwd": b / f, "loss": float(L.detach()), "dL_dbeta": float(beta.grad)}
 
 
+def adjoint_roe_figures(n: int = 512, nsteps: int = 50):
+    """The reverse mode of the periodic 2D Roe-MUSCL loop (margin 3) on the
+    n^2 Orszag-Tang vortex: gradient of L = sum (rho_final - rho_target)^2
+    dx dy over nsteps steps with respect to the initial velocities and a
+    pressure scale (every state kept); backward wall time against the plain
+    forward run of the resident Roe engine for the same steps."""
+    import numpy as np
+    import torch
+
+    from utrip_amd import adjoint, cans, problems
+    dev = torch.device("cuda:0")
+    m = 3
+    pb = problems.orszag_tang_2d(n=n, margin=m)
+    names = ("ro", "pr", "vx", "vy", "bx", "by", "az")
+    base = [torch.from_numpy(np.ascontiguousarray(pb.fields[k].T)).to(dev) for k in names]
+    gen = torch.Generator(device="cpu").manual_seed(7)
+    xi = lambda shape: torch.randn(shape, generator=gen, dtype=torch.float64).to(dev)
+    for q in (0, 1, 4, 5):
+        base[q] = adjoint.periodic_fill(base[q] * (1 + 1e-3 * xi(base[q].shape)), m)
+    dx = torch.from_numpy(pb.grid["dx"]).to(dev)
+    dy = torch.from_numpy(pb.grid["dy"]).to(dev)
+    pp = pb.params
+    solver = adjoint.PeriodicRoeAdjoint(dx, dy, pp["gm"], pp["safety"], m, nsteps)
+    vx0 = (base[2][m:-m, m:-m] * (1 + 1e-3 * xi((n, n)))).requires_grad_(True)
+    vy0 = (base[3][m:-m, m:-m] * (1 + 1e-3 * xi((n, n)))).requires_grad_(True)
+    beta = torch.ones((), dtype=torch.float64, device=dev, requires_grad=True)
+    with torch.no_grad():
+        target = adjoint.FinalDensity.apply(vx0 * (1 + 0.01 * xi((n, n))), vy0 * (1 + 0.01 * xi((n, n))), beta,
+                                            solver, base)
+    w = float(pb.grid["dx"][m] * pb.grid["dy"][m])
+    grid = {"dx": dx, "dxm": dx, "dy": dy, "dym": dy}
+    run = cans.Run(2, cans.ENGINE_ROE, [t.clone() for t in base], grid, pp["gm"], pp["safety"], margin=m,
+                   bc=cans.BD_PER)
+    plains = []
+    for _ in range(3):
+        run.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.steps(nsteps)
+        torch.cuda.synchronize()
+        plains.append(time.perf_counter() - t0)
+    plain = statistics.median(plains)
+    fwd, bwd = [], []
+    for _ in range(3):
+        for t in (vx0, vy0, beta):
+            t.grad = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = ((adjoint.FinalDensity.apply(vx0, vy0, beta, solver, base) - target) ** 2).sum() * w
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd.append(t1 - t0)
+        bwd.append(t2 - t1)
+    f, b = statistics.median(fwd), statistics.median(bwd)
+    return {"cells": n * n, "steps": nsteps, "forward_plain_ms": plain * 1e3, "forward_recording_ms": f * 1e3,
+            "backward_ms": b * 1e3, "adj_over_fwd": b / plain, "loss": float(L.detach()),
+            "dL_dbeta": float(beta.grad),
+            "note": "every state kept; the recording forward steps the resident engine one step at a time"}
+
+
 def progress(what: str) -> None:
     """A progress line on stderr (stdout carries only the JSON result)."""
     print(f"bench: {what} ({time.strftime('%H:%M:%S')})", file=sys.stderr, flush=True)
@@ -859,10 +922,12 @@ def main():
     r = ot_run(n, K, args.warmup)
     progress(f"OT {n}^2: {K} steps")
     pb, pp, total_ms, avg_kern_s, info = r["pb"], r["pb"].params, r["total_ms"], r["avg_kern_s"], r["info"]
-    adj = None
+    adj = adj_roe = None
     if args.adjoint:
         adj = adjoint_figures()
         progress("adjoint")
+        adj_roe = adjoint_roe_figures()
+        progress("Roe adjoint")
     m3 = roe2 = None
     if not args.no_extra:
         m3 = m3_figures()
@@ -943,6 +1008,7 @@ def main():
         "tubes": tb,
         "runs2d": b2,
         "adjoint_ot512": adj,
+        "adjoint_roe_ot512": adj_roe,
         "spicule": spc,
         "ot1024_mlw_m3": m3,
         "ot1024_roe": roe2,

@@ -367,6 +367,20 @@ int cans2d_mlw_adjoint_run(const double *const *states, const double *dt_log, in
                            const double *dxm, int ix, const double *dy, const double *dym, int jx, int margin,
                            const double *metric_ws, void *ws, void *stream);
 /* adjoint of bdperx then bdpery on 6 fields: ghost lam added to its source, cleared */
+/* Reverse mode of the periodic 2D Roe-MUSCL loop (cans2d/hdroe/roe_m.f:2-208,
+ * roeflux_m.f, tvdminmod.f, bdperx/bdpery, cfl_m; ghost width margin >= 3),
+ * no counterpart in the reference: states holds 7*(nsteps+1) device
+ * pointers (ro,pr,vx,vy,bx,by,az of every state of the trajectory, as the
+ * resident Roe run wrote them), dt_log the nsteps dt (device); lam6 = dL/d
+ * (ro,pr,vx,vy,bx,by) of the final state on entry, of the initial state on
+ * return (not yet folded onto the interior: call cans2d_periodic_fold);
+ * tmp6 a second set of the same size.  The face-flux derivative is the
+ * generated reverse sweep of the kernels' roeflux12.  ws:
+ * cans2d_roe_adj_ws_bytes. */
+size_t cans2d_roe_adj_ws_bytes(int ix, int jx);
+int cans2d_roe_adjoint_run(const double *const *states, const double *dt_log, int nsteps, double *const *lam6,
+                           double *const *tmp6, double gm, double safety, const double *dx, const double *dy, int ix,
+                           int jx, int margin, void *ws, void *stream);
 int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *stream);
 
 /* Engines of the resident run */

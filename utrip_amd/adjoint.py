@@ -2,7 +2,9 @@
 bdperx/bdpery and cfl_m, the loop of cans2d/md_mhdshktb/hdmlw/main.f:150-206)
 by a hand-written reverse mode on the GPU (cans2d_mlw_adjoint_step).
 
-``PeriodicMlwAdjoint`` runs the forward loop on the resident fused kernel.
+``PeriodicRoeAdjoint`` is the same for the 2D Roe-MUSCL loop (every state
+kept).  ``PeriodicMlwAdjoint`` runs the forward loop on the resident fused
+kernel.
 With ``ckpt = 0`` it keeps every state of the trajectory in device memory
 (7 fields x (nsteps+1) states: 2.9 GB for 200 steps at 512^2) and its
 ``backward`` is one reverse sweep; with ``ckpt > 0`` it keeps a checkpoint
@@ -170,6 +172,65 @@ class PeriodicMlwAdjoint:
             used[b] = True
         _check(lib().cans2d_periodic_fold(lam_p, C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin), _stream()))
         rec.wait_stream(main)   # nothing on the record stream outlives the call
+        return lam
+
+
+class PeriodicRoeAdjoint:
+    """Forward/backward driver for ``nsteps`` steps of the periodic 2D
+    Roe-MUSCL loop (cans2d/hdroe/roe_m.f with cfl_m, bdperx, bdpery; ghost
+    width ``margin`` >= 3; state tensors (jx, ix)).  The forward runs the
+    resident Roe engine one step at a time and keeps every state and dt on
+    the device; the backward is one reverse sweep (cans2d_roe_adjoint_run).
+    Same interface as PeriodicMlwAdjoint (forward(state), backward(lam))."""
+
+    def __init__(self, dx, dy, gm, safety, margin, nsteps, dtmin=1e-10):
+        if margin < 3:
+            raise ValueError("the Roe engine needs margin >= 3")
+        self.dx, self.dy = dx, dy
+        self.ix, self.jx = dx.numel(), dy.numel()
+        self.gm, self.safety, self.margin, self.nsteps, self.dtmin = gm, safety, margin, nsteps, dtmin
+        dev = dx.device
+        lib().cans2d_roe_adj_ws_bytes.restype = C.c_size_t
+        nb = lib().cans2d_roe_adj_ws_bytes(C.c_int(self.ix), C.c_int(self.jx))
+        self.ws = torch.empty((nb + 7) // 8, dtype=torch.float64, device=dev)
+        self._all = None
+        self.dt_log = torch.empty(max(nsteps, 1), dtype=torch.float64, device=dev)
+
+    def forward(self, state):
+        """Run nsteps from ``state`` (7 tensors ro,pr,vx,vy,bx,by,az, ghosts
+        filled); keep every state; return the final state (clones)."""
+        if self._all is None:
+            self._all = [[torch.empty_like(t) for t in state] for _ in range(self.nsteps + 1)]
+            self._all_p = _ptr_array([t for k in range(self.nsteps + 1) for t in self._all[k]], (self.jx, self.ix))
+        for dst, src in zip(self._all[0], state):
+            dst.copy_(src)
+        grid = {"dx": self.dx, "dxm": self.dx, "dy": self.dy, "dym": self.dy}
+        run = cans.Run(2, cans.ENGINE_ROE, [t.clone() for t in state], grid, self.gm, self.safety,
+                       margin=self.margin, bc=cans.BD_PER)
+        run.reset(dtmin=self.dtmin)
+        for n in range(self.nsteps):
+            run.steps(1)
+            cur, info = run.state()
+            if info["merr"] != 0 or info["nexec"] != n + 1:
+                raise cans.CansError(f"forward run stopped early: {info}")
+            for dst, src in zip(self._all[n + 1], cur):
+                dst.copy_(src)
+            self.dt_log[n] = info["dt"]
+        return [t.clone() for t in self._all[self.nsteps]]
+
+    def backward(self, lam):
+        """``lam``: dL/dP of the final state on every cell (6 tensors
+        ro,pr,vx,vy,bx,by).  Returns dL/dP of the initial state folded onto
+        the interior."""
+        shape = (self.jx, self.ix)
+        lam = [t.clone().contiguous() for t in lam]
+        tmp = [torch.empty_like(t) for t in lam]
+        lam_p, tmp_p = _ptr_array(lam, shape), _ptr_array(tmp, shape)
+        _check(lib().cans2d_roe_adjoint_run(self._all_p, _ptr(self.dt_log), C.c_int(self.nsteps), lam_p, tmp_p,
+                                            _d(self.gm), _d(self.safety), _ptr(self.dx), _ptr(self.dy),
+                                            C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin), _ptr(self.ws),
+                                            _stream()))
+        _check(lib().cans2d_periodic_fold(lam_p, C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin), _stream()))
         return lam
 
 

@@ -634,6 +634,44 @@ int cans2d_mlw_adjoint_run(const double *const *states, const double *dt_log, in
     return 0;
 }
 
+size_t cans2d_roe_adj_ws_bytes(int ix, int jx) { return adj2d_roe_ws_doubles(ix, jx) * sizeof(double); }
+
+int cans2d_roe_adjoint_run(const double *const *states, const double *dt_log, int nsteps, double *const *lam6,
+                           double *const *tmp6, double gm, double safety, const double *dx, const double *dy, int ix,
+                           int jx, int margin, void *ws, void *stream) {
+    if (!states || !dt_log || !lam6 || !tmp6 || !dx || !dy || !ws || nsteps < 0)
+        return fail(CANS_ERR_ARG, "cans2d_roe_adjoint_run: bad argument");
+    if (ix < 8 || jx < 8 || margin < 3 || ix - 2 * margin < margin || jx - 2 * margin < margin)
+        return fail(CANS_ERR_ARG, "cans2d_roe_adjoint_run: the Roe box needs margin >= 3 and ix-2*margin >= margin "
+                    "(ix=%d jx=%d margin=%d)", ix, jx, margin);
+    for (int k = 0; k < 7 * (nsteps + 1); k++)
+        if (!states[k]) return fail(CANS_ERR_ARG, "cans2d_roe_adjoint_run: NULL state pointer %d", k);
+    for (int f = 0; f < 6; f++)
+        if (!lam6[f] || !tmp6[f]) return fail(CANS_ERR_ARG, "cans2d_roe_adjoint_run: NULL lam/tmp field %d", f);
+    hipStream_t s = (hipStream_t)stream;
+    double *const *cur = lam6;
+    double *const *oth = tmp6;
+    const size_t n = (size_t)ix * jx;
+    double *acc = (double *)ws + adj2d_roe_ws_doubles(ix, jx) - 64 + 4;   // two accumulator slots
+    HIPCHK(launch_zero(acc, 2 * kAdjAccSlot, s));
+    // as cans2d_mlw_adjoint_run: step n counts into slot n&1, its B1 applies
+    // the pending dt gradient of step n+1, which its R2 then clears
+    for (int st = nsteps - 1; st >= 0; st--) {
+        const bool pend = st < nsteps - 1;
+        HIPCHK(launch_adj2d_roe_step(states + 7 * st, states + 7 * (st + 1), cur, oth, (double *)ws, dt_log + st,
+                                     acc + kAdjAccSlot * (st & 1), pend ? dt_log + st + 1 : nullptr,
+                                     pend ? acc + kAdjAccSlot * ((st + 1) & 1) : nullptr,
+                                     acc + kAdjAccSlot * ((st + 1) & 1), dx, dy, gm, safety, ix, jx, margin, s));
+        double *const *t = cur;
+        cur = oth;
+        oth = t;
+    }
+    if (nsteps > 0)   // the dt gradient of the first step
+        HIPCHK(launch_adj2d_dt(states, cur, (double *)ws, dt_log, acc, dx, dy, gm, safety, ix, jx, s));
+    if (cur != lam6) HIPCHK(launch_copy6(lam6, cur, n, s));
+    return 0;
+}
+
 int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *stream) {
     if (!lam6 || ix < 3 || jx < 3 || margin < 1 || ix - 2 * margin < margin || jx - 2 * margin < margin)
         return fail(CANS_ERR_ARG, "cans2d_periodic_fold: bad argument (ix=%d jx=%d margin=%d)", ix, jx, margin);

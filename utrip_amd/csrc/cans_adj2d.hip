@@ -26,6 +26,8 @@
 // carries no adjoint.  Everything is FP64.
 #include "cans_common.hpp"
 #include "cans_kernels.hpp"
+#include "cans_roe.hpp"
+#include "cans_roeflux_adj.hpp"
 
 namespace cans {
 namespace {
@@ -50,6 +52,7 @@ struct AdjArgs {
     const double *pend_dt;           // the dt of step n+1, whose dt gradient B1 applies (or NULL)
     const double *pend_acc;          // its accumulator slot (ldt stripes, nties)
     double *acc_zero;                // accumulators B2 clears for step n-1 (or NULL)
+    int ilo;                         // interior() lower bound
 };
 
 struct P6 {
@@ -164,7 +167,11 @@ CANS_DEV void flux_adj(const P6 &p, double gm, double g1i, double lrx, double le
 
 CANS_DEV double step_dt(const AdjArgs &a) { return *a.dtp; }
 
-CANS_DEV bool interior(const AdjArgs &a, int i, int j) { return i >= 2 && i <= a.ix - 1 && j >= 2 && j <= a.jx - 1; }
+// cells whose update survives the periodic fill: [ilo, n+1-ilo] (MLW: ilo = 2,
+// the engine's update range; Roe: margin+1, the interior of the box)
+CANS_DEV bool interior(const AdjArgs &a, int i, int j) {
+    return i >= a.ilo && i <= a.ix + 1 - a.ilo && j >= a.ilo && j <= a.jx + 1 - a.ilo;
+}
 
 // the dL/ddt of a slot: its stripes summed in order
 CANS_DEV double acc_ldt(const double *acc) {
@@ -479,6 +486,225 @@ __global__ __launch_bounds__(256) void adj_dt(AdjArgs a) {
     a.lout[3][k] += l.vy; a.lout[4][k] += l.bx; a.lout[5][k] += l.by;
 }
 
+// ================================================================== Roe
+// Reverse mode of one periodic 2D Roe-MUSCL step (cans2d/hdroe/roe_m.f:2-208
+// with bdperx/bdpery and cfl_m), FP64.  Forward, per step:
+//   F1 = roeflux(P_c, P_c+1) on every face (first order);
+//   Uh = U(P) + 1/2 dt sum_d (F1(c-1) - F1(c))/d  on cells [2,n-1]^2,
+//   Ph = prim(Uh) there and P elsewhere;
+//   F2 = roeflux(L, R), L/R the minmod (tvdminmod.f) states of Ph;
+//   U' = U(P) + dt sum_d (F2(c-1) - F2(c))/d  on cells [3,n-2]^2, P' = prim(U');
+//   periodic fill (x then y) of margin m >= 3.
+// Reverse, as gathers (no atomics on fields): B1 (adj_b1 with interior
+// [m+1, n-m]: periodic fold, pending dt gradient, lam U'); R2 per stage-2
+// face: lam F2 from the two cells' lam U', back through the flux (the
+// generated roeflux12_vjp) and the minmod reconstruction to its four cells;
+// R3 per cell: gather lam Ph, back through prim(Uh) and U(P); R4 per
+// stage-1 face: lam F1 from lam Uh, back through the flux; R5 per cell:
+// gather.  The face fluxes' explicit dt terms go to dL/ddt.
+struct RoeAdj {
+    const double *p[6];      // P^n: ro pr vx vy bx by
+    const double *ph[6];     // half-step primitives (recomputed from P^n)
+    double *lout[6];         // lam of P^n
+    double *w;               // scratch base (SoA, stride na)
+    size_t na;
+    const double *dx, *dy;
+    const double *dtp;
+    double *ldt;
+    double *acc_zero;
+    double gm;
+    int ix, jx;
+};
+enum { RW_LDU = 0, RW_LUH = 6, RW_C2 = 12, RW_C1 = 60, RW_PH = 84, RW_F1 = 92, RW_N = 108 };
+CANS_DEV double *rw(const RoeAdj &a, int slot) { return a.w + (size_t)slot * a.na; }
+
+// sweep frame of direction dir: (vn, vt, bn, bt) as field indices (ro 0,
+// pr 1, vx 2, vy 3, bx 4, by 5); the flux's momentum/field outputs go to
+// U slots (rn, rt, bt)
+CANS_DEV void roe_frame(int dir, int &vn, int &vt, int &bn, int &bt) {
+    vn = dir == 0 ? 2 : 3;
+    vt = dir == 0 ? 3 : 2;
+    bn = dir == 0 ? 4 : 5;
+    bt = dir == 0 ? 5 : 4;
+}
+
+// d flmt(a, b) / d(a, b) for flmt = max(0, min(b sign a, |a|)) sign a
+// (tvdminmod.f statement function), as evaluated: the branch taken
+CANS_DEV void flmt_d(double a, double b, double &ga, double &gb) {
+    const double s = signF(1.0, a);
+    const double x = b * s, y = __builtin_fabs(a);
+    const double m = x < y ? x : y;
+    if (0.0 > m) { ga = 0.0; gb = 0.0; return; }
+    if (x < y) { ga = 0.0; gb = 1.0; } else { ga = 1.0; gb = 0.0; }
+}
+
+// stage-2 faces of direction dir: x faces i in [2, ix-2], j in [3, jx-2]
+// (y: transposed), between cells c and c+1 along dir
+__global__ __launch_bounds__(256) void roe_r2(RoeAdj a, int dir) {
+    const int ix = a.ix, jx = a.jx;
+    const int na_ = dir == 0 ? ix - 3 : ix - 4, nb_ = dir == 0 ? jx - 4 : jx - 3;
+    const size_t t = xcd_first_thread() + threadIdx.x;
+    double part = 0.0;
+    if (na_ > 0 && nb_ > 0 && t < (size_t)na_ * nb_) {
+        if (t == 0 && a.acc_zero && dir == 0)
+            for (int q = 0; q < kAdjAccSlot; q++) a.acc_zero[q] = 0.0;
+        const int i = (dir == 0 ? 2 : 3) + (int)(t % na_), j = (dir == 0 ? 3 : 2) + (int)(t / na_);
+        const size_t s = dir == 0 ? 1 : (size_t)ix;
+        const size_t p = at(i, j, ix);
+        const double dt = *a.dtp;
+        const double dn0 = dir == 0 ? a.dx[i - 1] : a.dy[j - 1], dn1 = dir == 0 ? a.dx[i] : a.dy[j];
+        int vn, vt, bn, bt;
+        roe_frame(dir, vn, vt, bn, bt);
+        // lam F2 in flux order (ro, ee, rn, rt, bt): U'(c) gets -dt F/dn0, U'(c+1) +dt F/dn1
+        const int uq[5] = {0, 1, vn, vt, bt};
+        double lF[5];
+        for (int k = 0; k < 5; k++) lF[k] = dt * (rw(a, RW_LDU + uq[k])[p + s] / dn1 - rw(a, RW_LDU + uq[k])[p] / dn0);
+        // minmod states of the six fields from Ph at c-1 .. c+2
+        double L[6], R[6], gaL[6], gbL[6], gaR[6], gbR[6];
+        for (int f = 0; f < 6; f++) {
+            const double *d = a.ph[f];
+            const double d0 = d[p - s], d1 = d[p], d2 = d[p + s], d3 = d[p + 2 * s];
+            L[f] = d1 + 0.5 * flmt(d2 - d1, d1 - d0);
+            R[f] = d2 - 0.5 * flmt(d2 - d1, d3 - d2);
+            flmt_d(d2 - d1, d1 - d0, gaL[f], gbL[f]);
+            flmt_d(d2 - d1, d3 - d2, gaR[f], gbR[f]);
+        }
+        double li[12], fo[5];
+        roeflux12_vjp(a.gm, L[0], L[1], L[vn], L[vt], L[bn], L[bt], R[0], R[1], R[vn], R[vt], R[bn], R[bt], lF, li, fo);
+        if (dt != 0.0) {
+            double sdt = 0.0;
+            for (int k = 0; k < 5; k++) sdt += fo[k] * lF[k];
+            part += sdt / dt;
+        }
+        const int fidx[6] = {0, 1, vn, vt, bn, bt};
+        double lL[6], lR[6];
+        for (int k = 0; k < 6; k++) {
+            lL[fidx[k]] = li[k];
+            lR[fidx[k]] = li[6 + k];
+        }
+        // contributions to cells c-1, c, c+1, c+2 (slots RW_C2 + dir*24 + off*6 + field)
+        for (int f = 0; f < 6; f++) {
+            const double c0 = lL[f] * (-0.5 * gbL[f]);
+            const double c1 = lL[f] * (1.0 + 0.5 * (gbL[f] - gaL[f])) + lR[f] * (0.5 * gaR[f]);
+            const double c2 = lL[f] * (0.5 * gaL[f]) + lR[f] * (1.0 - 0.5 * (gaR[f] - gbR[f]));
+            const double c3 = lR[f] * (-0.5 * gbR[f]);
+            rw(a, RW_C2 + dir * 24 + 0 * 6 + f)[p] = c0;
+            rw(a, RW_C2 + dir * 24 + 1 * 6 + f)[p] = c1;
+            rw(a, RW_C2 + dir * 24 + 2 * 6 + f)[p] = c2;
+            rw(a, RW_C2 + dir * 24 + 3 * 6 + f)[p] = c3;
+        }
+    }
+    block_sum_add<256>(part, a.ldt);
+}
+
+CANS_DEV bool r2_face(int dir, int i, int j, int ix, int jx) {   // stage-2 face exists
+    return dir == 0 ? (i >= 2 && i <= ix - 2 && j >= 3 && j <= jx - 2) : (i >= 3 && i <= ix - 2 && j >= 2 && j <= jx - 2);
+}
+CANS_DEV bool r4_face(int dir, int i, int j, int ix, int jx) {   // stage-1 face exists
+    return dir == 0 ? (i >= 1 && i <= ix - 1 && j >= 2 && j <= jx - 1) : (i >= 2 && i <= ix - 1 && j >= 1 && j <= jx - 1);
+}
+
+// cells: lam Ph gathered from the stage-2 faces, back through prim(Uh)
+// (cells [2,n-1]^2) or the copy Ph = P (outer cells), and U(P) of the half
+// step and of the update
+__global__ __launch_bounds__(256) void roe_r3(RoeAdj a) {
+    const int ix = a.ix, jx = a.jx;
+    const size_t n = (size_t)ix * jx;
+    const size_t k = xcd_first_thread() + threadIdx.x;
+    if (k >= n) return;
+    const int i = (int)(k % ix) + 1, j = (int)(k / ix) + 1;
+    double lph[6] = {0, 0, 0, 0, 0, 0};
+    for (int dir = 0; dir < 2; dir++) {
+        for (int off = 0; off < 4; off++) {   // face f = c + 1 - off reads this cell as its (off)th cell
+            const int fi = dir == 0 ? i + 1 - off : i, fj = dir == 0 ? j : j + 1 - off;
+            if (!r2_face(dir, fi, fj, ix, jx)) continue;
+            const size_t pf = at(fi, fj, ix);
+            for (int f = 0; f < 6; f++) lph[f] += rw(a, RW_C2 + dir * 24 + off * 6 + f)[pf];
+        }
+    }
+    const P6 pc = load6(a.p, k);
+    double lu[6];
+    for (int f = 0; f < 6; f++) lu[f] = rw(a, RW_LDU + f)[k];   // the update's U(P)
+    P6 l{0, 0, 0, 0, 0, 0};
+    const bool inner = i >= 2 && i <= ix - 1 && j >= 2 && j <= jx - 1;
+    if (inner) {
+        const P6 ph = load6(a.ph, k);
+        const P6 lp{lph[0], lph[1], lph[2], lph[3], lph[4], lph[5]};
+        double luh[6];
+        prim_adj(ph, lp, a.gm, luh);
+        for (int f = 0; f < 6; f++) {
+            rw(a, RW_LUH + f)[k] = luh[f];
+            lu[f] += luh[f];
+        }
+    } else {
+        for (int f = 0; f < 6; f++) rw(a, RW_LUH + f)[k] = 0.0;
+        l.ro = lph[0]; l.pr = lph[1]; l.vx = lph[2]; l.vy = lph[3]; l.bx = lph[4]; l.by = lph[5];
+    }
+    cons_adj(pc, lu, 1.0 / (a.gm - 1.0), l);
+    a.lout[0][k] = l.ro; a.lout[1][k] = l.pr; a.lout[2][k] = l.vx;
+    a.lout[3][k] = l.vy; a.lout[4][k] = l.bx; a.lout[5][k] = l.by;
+}
+
+// stage-1 faces: lam F1 from the two cells' lam Uh, back through the flux at
+// (P_c, P_c+1); lam of both states stored per face
+__global__ __launch_bounds__(256) void roe_r4(RoeAdj a, int dir) {
+    const int ix = a.ix, jx = a.jx;
+    const int na_ = dir == 0 ? ix - 1 : ix - 2, nb_ = dir == 0 ? jx - 2 : jx - 1;
+    const size_t t = xcd_first_thread() + threadIdx.x;
+    double part = 0.0;
+    if (t < (size_t)na_ * nb_) {
+        const int i = (dir == 0 ? 1 : 2) + (int)(t % na_), j = (dir == 0 ? 2 : 1) + (int)(t / na_);
+        const size_t s = dir == 0 ? 1 : (size_t)ix;
+        const size_t p = at(i, j, ix);
+        const double dt = *a.dtp;
+        const double dn0 = dir == 0 ? a.dx[i - 1] : a.dy[j - 1], dn1 = dir == 0 ? a.dx[i] : a.dy[j];
+        int vn, vt, bn, bt;
+        roe_frame(dir, vn, vt, bn, bt);
+        const int uq[5] = {0, 1, vn, vt, bt};
+        double lF[5];
+        for (int k = 0; k < 5; k++)
+            lF[k] = 0.5 * dt * (rw(a, RW_LUH + uq[k])[p + s] / dn1 - rw(a, RW_LUH + uq[k])[p] / dn0);
+        const P6 pl = load6(a.p, p), pr = load6(a.p, p + s);
+        const double L[6] = {pl.ro, pl.pr, pl.vx, pl.vy, pl.bx, pl.by};
+        const double R[6] = {pr.ro, pr.pr, pr.vx, pr.vy, pr.bx, pr.by};
+        double li[12], fo[5];
+        roeflux12_vjp(a.gm, L[0], L[1], L[vn], L[vt], L[bn], L[bt], R[0], R[1], R[vn], R[vt], R[bn], R[bt], lF, li, fo);
+        if (dt != 0.0) {
+            double sdt = 0.0;
+            for (int k = 0; k < 5; k++) sdt += fo[k] * lF[k];
+            part += sdt / dt;
+        }
+        const int fidx[6] = {0, 1, vn, vt, bn, bt};
+        for (int k = 0; k < 6; k++) {
+            rw(a, RW_C1 + dir * 12 + fidx[k])[p] = li[k];
+            rw(a, RW_C1 + dir * 12 + 6 + fidx[k])[p] = li[6 + k];
+        }
+    }
+    block_sum_add<256>(part, a.ldt);
+}
+
+// cells: add the stage-1 faces' lam of this cell's state
+__global__ __launch_bounds__(256) void roe_r5(RoeAdj a) {
+    const int ix = a.ix, jx = a.jx;
+    const size_t n = (size_t)ix * jx;
+    const size_t k = xcd_first_thread() + threadIdx.x;
+    if (k >= n) return;
+    const int i = (int)(k % ix) + 1, j = (int)(k / ix) + 1;
+    double l[6];
+    for (int f = 0; f < 6; f++) l[f] = a.lout[f][k];
+    for (int dir = 0; dir < 2; dir++) {
+        const int li_ = dir == 0 ? i - 1 : i, lj_ = dir == 0 ? j : j - 1;   // face before: this cell is its R
+        if (r4_face(dir, li_, lj_, ix, jx)) {
+            const size_t pf = at(li_, lj_, ix);
+            for (int f = 0; f < 6; f++) l[f] += rw(a, RW_C1 + dir * 12 + 6 + f)[pf];
+        }
+        if (r4_face(dir, i, j, ix, jx)) {                                    // face after: its L
+            for (int f = 0; f < 6; f++) l[f] += rw(a, RW_C1 + dir * 12 + f)[k];
+        }
+    }
+    for (int f = 0; f < 6; f++) a.lout[f][k] = l[f];
+}
+
 AdjArgs make_adj(const double *const *pn, const double *const *pn1, const double *const *lin, double *const *lout,
                  double *ws, const double *dtp, double *acc, const double *dx, const double *dxm, const double *dy,
                  const double *dym, Metric mx, Metric my, double qav, double gm, double safety, int ix, int jx,
@@ -501,6 +727,7 @@ AdjArgs make_adj(const double *const *pn, const double *const *pn1, const double
     a.mx = mx; a.my = my;
     a.qav = qav; a.gm = gm; a.safety = safety;
     a.ix = ix; a.jx = jx; a.margin = margin;
+    a.ilo = 2;
     return a;
 }
 
@@ -574,6 +801,57 @@ hipError_t launch_copy6(double *const *dst, double *const *src, size_t n, hipStr
     }
     hipLaunchKernelGGL(copy6_kernel, dim3(512), dim3(256), 0, s, a, n);
     return hipGetLastError();
+}
+
+hipError_t launch_adj2d_roe_step(const double *const *pn, const double *const *pn1, const double *const *lin,
+                                 double *const *lout, double *ws, const double *dtp, double *acc,
+                                 const double *pend_dt, const double *pend_acc, double *acc_zero, const double *dx,
+                                 const double *dy, double gm, double safety, int ix, int jx, int margin,
+                                 hipStream_t s) {
+    const size_t n = (size_t)ix * jx;
+    const size_t na = (n + 63) & ~size_t(63);
+    RoeAdj r{};
+    r.w = ws;
+    r.na = na;
+    for (int f = 0; f < 6; f++) {
+        r.p[f] = pn[f];
+        r.lout[f] = lout[f];
+    }
+    const int slot[6] = {0, 1, 2, 3, 5, 6};   // the Roe engine's slots of ro pr vx vy bx by
+    for (int f = 0; f < 6; f++) r.ph[f] = ws + (RW_PH + slot[f]) * na;
+    r.dx = dx; r.dy = dy; r.dtp = dtp; r.ldt = acc; r.acc_zero = acc_zero; r.gm = gm; r.ix = ix; r.jx = jx;
+    // the forward's stage-1 fluxes and half-step primitives at P^n
+    RoeArgs f{};
+    for (int q = 0; q < 6; q++) f.prim[slot[q]] = pn[q];
+    for (int q = 0; q < 8; q++) {
+        f.half[q] = ws + (RW_PH + q) * na;
+        f.flux[0][q] = ws + (RW_F1 + q) * na;
+        f.flux[1][q] = ws + (RW_F1 + 8 + q) * na;
+    }
+    f.dx = dx; f.dy = dy; f.gm = gm; f.dtp = dtp; f.ix = ix; f.jx = jx; f.kx = 1;
+    hipError_t e = launch_roe2d_half(f, s);
+    if (e != hipSuccess) return e;
+    Metric z{};
+    AdjArgs a = make_adj(pn, pn1, lin, lout, ws, dtp, acc, dx, nullptr, dy, nullptr, z, z, 0.0, gm, safety, ix, jx,
+                         margin);
+    a.ilo = margin + 1;
+    a.pend_dt = pend_dt;
+    a.pend_acc = pend_acc;
+    a.acc_zero = nullptr;
+    hipLaunchKernelGGL(adj_b1, dim3(xcd_grid(n)), dim3(256), 0, s, a);
+    for (int dir = 0; dir < 2; dir++)
+        hipLaunchKernelGGL(roe_r2, dim3(xcd_grid(n)), dim3(256), 0, s, r, dir);
+    hipLaunchKernelGGL(roe_r3, dim3(xcd_grid(n)), dim3(256), 0, s, r);
+    for (int dir = 0; dir < 2; dir++)
+        hipLaunchKernelGGL(roe_r4, dim3(xcd_grid(n)), dim3(256), 0, s, r, dir);
+    hipLaunchKernelGGL(roe_r5, dim3(xcd_grid(n)), dim3(256), 0, s, r);
+    return hipGetLastError();
+}
+
+size_t adj2d_roe_ws_doubles(int ix, int jx) {
+    const size_t n = (size_t)ix * jx;
+    const size_t na = (n + 63) & ~size_t(63);
+    return RW_N * na + 64;
 }
 
 size_t adj2d_ws_doubles(int ix, int jx) {

@@ -33,17 +33,11 @@ CANS_DEV size_t lin(int i, int j, int k, int ix, int jx) {
 
 template <int D> struct RoePi2 {};
 
-CANS_DEV double flmt(double a, double b) {   // tvdminmod.f statement function
-    return fmaxF(0.0, fminF(b * signF(1.0, a), __builtin_fabs(a))) * signF(1.0, a);
-}
-CANS_DEV float flmt(float a, float b) {
-    return fmaxF(0.0f, fminF(b * signF(1.0f, a), __builtin_fabsf(a))) * signF(1.0f, a);
-}
 
 // Run flag / dt of the step: from the clock (after the step's clock kernel
 // ran), or the by-value dt of a drop-in call.
 CANS_DEV bool step_dt(const RoeArgs &a, double &dt) {
-    if (!a.clk) { dt = a.dt; return true; }
+    if (!a.clk) { dt = a.dtp ? *a.dtp : a.dt; return true; }
     const int d = (a.step + 1) & 1;
     if (a.clk->stopped[d]) return false;
     dt = a.clk->dt_last[d];
@@ -428,6 +422,17 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
 }
 
 }  // namespace
+
+hipError_t launch_roe2d_half(const RoeArgs &a, hipStream_t s) {
+    for (int dir = 0; dir < 2; dir++) {
+        const int n[2] = {a.ix, a.jx};
+        const size_t total = (size_t)(n[dir] - 1) * (size_t)(n[1 - dir] - 2);
+        hipLaunchKernelGGL((roe_flux_kernel<2, double, false>), dim3(roe_grid<2>(total)), dim3(256), 0, s, a, dir, 1);
+    }
+    hipLaunchKernelGGL((roe_half_kernel<2, double, false>), dim3(roe_grid<2>((size_t)a.ix * a.jx)), dim3(256), 0, s,
+                       a);
+    return hipGetLastError();
+}
 
 hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s, bool comp3) {
     if (dim == 1 && comp3) return mixed ? launch_roe_t<1, float, true>(a, s) : launch_roe_t<1, double, true>(a, s);

@@ -22,9 +22,14 @@ struct RoeArgs {
     Clock *clk;              // non-null: dt and run flag of step `step`
     int step;
     int ix, jx, kx;
+    const double *dtp;       // non-null (and clk null): dt read from device memory
 };
 // comp3: roe_m3 (1D/2D with vz, bz in slots S_Z, S_BZ)
 hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s, bool comp3 = false);
+// The first half of a 2D FP64 Roe step: the stage-1 face fluxes of both
+// directions (a.flux) and the half-step primitives (a.half); the reverse
+// sweep recomputes them from the recorded state.
+hipError_t launch_roe2d_half(const RoeArgs &a, hipStream_t s);
 
 // 3D modified Lax-Wendroff (cans3d/hdmlw/mlw_m.f), three kernels with
 // global scratch: cell quantities, corner half-step fluxes, update.

@@ -19,6 +19,15 @@ template <class T> CANS_DEV T tabs(T x) { return x < T(0) ? -x : (x == T(0) ? T(
 template <> CANS_DEV double tabs<double>(double x) { return __builtin_fabs(x); }
 template <> CANS_DEV float tabs<float>(float x) { return __builtin_fabsf(x); }
 
+// minmod limiter, the tvdminmod.f statement function
+// flmt(a,b) = max(0, min(b*sign(1,a), |a|)) * sign(1,a)
+CANS_DEV double flmt(double a, double b) {
+    return fmaxF(0.0, fminF(b * signF(1.0, a), __builtin_fabs(a))) * signF(1.0, a);
+}
+CANS_DEV float flmt(float a, float b) {
+    return fmaxF(0.0f, fminF(b * signF(1.0f, a), __builtin_fabsf(a))) * signF(1.0f, a);
+}
+
 template <class T> struct RoePi {
     static constexpr double pi4d = 4.0 * kPi;          // pi4=4.0d0*pi
     static constexpr double pi4id = 1.0 / pi4d;         // pi4i=1.0d0/pi4
