@@ -859,7 +859,7 @@ def adjoint_roe_figures(n: int = 512, nsteps: int = 50):
     return {"cells": n * n, "steps": nsteps, "forward_plain_ms": plain * 1e3, "forward_recording_ms": f * 1e3,
             "backward_ms": b * 1e3, "adj_over_fwd": b / plain, "loss": float(L.detach()),
             "dL_dbeta": float(beta.grad),
-            "note": "every state kept; the recording forward steps the resident engine one step at a time"}
+            "note": "every state and half step kept (cans2d_roe_record); adj_over_fwd = backward / plain forward"}
 
 
 def progress(what: str) -> None:

@@ -371,16 +371,27 @@ int cans2d_mlw_adjoint_run(const double *const *states, const double *dt_log, in
  * roeflux_m.f, tvdminmod.f, bdperx/bdpery, cfl_m; ghost width margin >= 3),
  * no counterpart in the reference: states holds 7*(nsteps+1) device
  * pointers (ro,pr,vx,vy,bx,by,az of every state of the trajectory, as the
- * resident Roe run wrote them), dt_log the nsteps dt (device); lam6 = dL/d
+ * resident Roe run or cans2d_roe_record wrote them), half NULL or the 6*nsteps
+ * recorded half-step fields (cans2d_roe_record; else the step's first half is
+ * recomputed), dt_log the nsteps dt (device); lam6 = dL/d
  * (ro,pr,vx,vy,bx,by) of the final state on entry, of the initial state on
  * return (not yet folded onto the interior: call cans2d_periodic_fold);
  * tmp6 a second set of the same size.  The face-flux derivative is the
  * generated reverse sweep of the kernels' roeflux12.  ws:
  * cans2d_roe_adj_ws_bytes. */
 size_t cans2d_roe_adj_ws_bytes(int ix, int jx);
-int cans2d_roe_adjoint_run(const double *const *states, const double *dt_log, int nsteps, double *const *lam6,
-                           double *const *tmp6, double gm, double safety, const double *dx, const double *dy, int ix,
-                           int jx, int margin, void *ws, void *stream);
+int cans2d_roe_adjoint_run(const double *const *states, const double *const *half, const double *dt_log, int nsteps,
+                           double *const *lam6, double *const *tmp6, double gm, double safety, const double *dx,
+                           const double *dy, int ix, int jx, int margin, void *ws, void *stream);
+/* The periodic 2D Roe-MUSCL loop recorded for cans2d_roe_adjoint_run: nsteps
+ * of cfl_m, the clock (clk: cans_clock_create, reset by the caller), roe_m and
+ * bdperx/bdpery, state n+1 written to states[7(n+1) ..] (states[0..6] is the
+ * initial state), the step's half-step primitives (ro pr vx vy bx by) to
+ * half[6n ..] and its dt to dt_log[n] (device).  ws: cans_run_ws_bytes(2,
+ * CANS_ENGINE_ROE, ix, jx, 1).  No counterpart in the reference. */
+int cans2d_roe_record(double *const *states, double *const *half, int nsteps, void *clk, double gm, double safety,
+                      const double *dx, const double *dy, int ix, int jx, int margin, void *ws, double *dt_log,
+                      void *stream);
 int cans2d_periodic_fold(double *const *lam6, int ix, int jx, int margin, void *stream);
 
 /* Engines of the resident run */

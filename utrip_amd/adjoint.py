@@ -178,10 +178,10 @@ class PeriodicMlwAdjoint:
 class PeriodicRoeAdjoint:
     """Forward/backward driver for ``nsteps`` steps of the periodic 2D
     Roe-MUSCL loop (cans2d/hdroe/roe_m.f with cfl_m, bdperx, bdpery; ghost
-    width ``margin`` >= 3; state tensors (jx, ix)).  The forward runs the
-    resident Roe engine one step at a time and keeps every state and dt on
-    the device; the backward is one reverse sweep (cans2d_roe_adjoint_run).
-    Same interface as PeriodicMlwAdjoint (forward(state), backward(lam))."""
+    width ``margin`` >= 3; state tensors (jx, ix)).  The forward records the
+    trajectory on the device (cans2d_roe_record: every state, every step's
+    half-step primitives and dt); the backward is one reverse sweep
+    (cans2d_roe_adjoint_run).  Same interface as PeriodicMlwAdjoint."""
 
     def __init__(self, dx, dy, gm, safety, margin, nsteps, dtmin=1e-10):
         if margin < 3:
@@ -193,43 +193,50 @@ class PeriodicRoeAdjoint:
         lib().cans2d_roe_adj_ws_bytes.restype = C.c_size_t
         nb = lib().cans2d_roe_adj_ws_bytes(C.c_int(self.ix), C.c_int(self.jx))
         self.ws = torch.empty((nb + 7) // 8, dtype=torch.float64, device=dev)
+        lib().cans_run_ws_bytes.restype = C.c_size_t
+        nr = lib().cans_run_ws_bytes(C.c_int(2), C.c_int(cans.ENGINE_ROE), C.c_int(self.ix), C.c_int(self.jx),
+                                     C.c_int(1))
+        self.run_ws = torch.empty((nr + 7) // 8, dtype=torch.float64, device=dev)
+        self.clock = cans.Clock()
         self._all = None
         self.dt_log = torch.empty(max(nsteps, 1), dtype=torch.float64, device=dev)
 
     def forward(self, state):
         """Run nsteps from ``state`` (7 tensors ro,pr,vx,vy,bx,by,az, ghosts
-        filled); keep every state; return the final state (clones)."""
+        filled); keep every state and half step; return the final state
+        (clones)."""
+        shape = (self.jx, self.ix)
         if self._all is None:
             self._all = [[torch.empty_like(t) for t in state] for _ in range(self.nsteps + 1)]
-            self._all_p = _ptr_array([t for k in range(self.nsteps + 1) for t in self._all[k]], (self.jx, self.ix))
+            self._half = [[torch.empty_like(state[0]) for _ in range(6)] for _ in range(self.nsteps)]
+            self._all_p = _ptr_array([t for k in range(self.nsteps + 1) for t in self._all[k]], shape)
+            self._half_p = _ptr_array([t for k in range(self.nsteps) for t in self._half[k]], shape)
         for dst, src in zip(self._all[0], state):
             dst.copy_(src)
-        grid = {"dx": self.dx, "dxm": self.dx, "dy": self.dy, "dym": self.dy}
-        run = cans.Run(2, cans.ENGINE_ROE, [t.clone() for t in state], grid, self.gm, self.safety,
-                       margin=self.margin, bc=cans.BD_PER)
-        run.reset(dtmin=self.dtmin)
-        for n in range(self.nsteps):
-            run.steps(1)
-            cur, info = run.state()
-            if info["merr"] != 0 or info["nexec"] != n + 1:
-                raise cans.CansError(f"forward run stopped early: {info}")
-            for dst, src in zip(self._all[n + 1], cur):
-                dst.copy_(src)
-            self.dt_log[n] = info["dt"]
+        self.clock.reset(dtmin=self.dtmin)
+        _check(lib().cans2d_roe_record(self._all_p, self._half_p, C.c_int(self.nsteps), self.clock.handle,
+                                       _d(self.gm), _d(self.safety), _ptr(self.dx), _ptr(self.dy), C.c_int(self.ix),
+                                       C.c_int(self.jx), C.c_int(self.margin), _ptr(self.run_ws), _ptr(self.dt_log),
+                                       _stream()))
+        self.clock.issued = self.nsteps
+        info = self.clock.read()
+        if info["merr"] != 0 or info["nexec"] != self.nsteps:
+            raise cans.CansError(f"forward run stopped early: {info}")
         return [t.clone() for t in self._all[self.nsteps]]
 
-    def backward(self, lam):
+    def backward(self, lam, recompute_half=False):
         """``lam``: dL/dP of the final state on every cell (6 tensors
         ro,pr,vx,vy,bx,by).  Returns dL/dP of the initial state folded onto
-        the interior."""
+        the interior.  recompute_half: recompute each step's half step in
+        the sweep instead of reading the recorded one (same result)."""
         shape = (self.jx, self.ix)
         lam = [t.clone().contiguous() for t in lam]
         tmp = [torch.empty_like(t) for t in lam]
         lam_p, tmp_p = _ptr_array(lam, shape), _ptr_array(tmp, shape)
-        _check(lib().cans2d_roe_adjoint_run(self._all_p, _ptr(self.dt_log), C.c_int(self.nsteps), lam_p, tmp_p,
-                                            _d(self.gm), _d(self.safety), _ptr(self.dx), _ptr(self.dy),
-                                            C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin), _ptr(self.ws),
-                                            _stream()))
+        _check(lib().cans2d_roe_adjoint_run(self._all_p, None if recompute_half else self._half_p, _ptr(self.dt_log),
+                                            C.c_int(self.nsteps), lam_p, tmp_p, _d(self.gm), _d(self.safety),
+                                            _ptr(self.dx), _ptr(self.dy), C.c_int(self.ix), C.c_int(self.jx),
+                                            C.c_int(self.margin), _ptr(self.ws), _stream()))
         _check(lib().cans2d_periodic_fold(lam_p, C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin), _stream()))
         return lam
 

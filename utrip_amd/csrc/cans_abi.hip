@@ -636,9 +636,9 @@ int cans2d_mlw_adjoint_run(const double *const *states, const double *dt_log, in
 
 size_t cans2d_roe_adj_ws_bytes(int ix, int jx) { return adj2d_roe_ws_doubles(ix, jx) * sizeof(double); }
 
-int cans2d_roe_adjoint_run(const double *const *states, const double *dt_log, int nsteps, double *const *lam6,
-                           double *const *tmp6, double gm, double safety, const double *dx, const double *dy, int ix,
-                           int jx, int margin, void *ws, void *stream) {
+int cans2d_roe_adjoint_run(const double *const *states, const double *const *half, const double *dt_log, int nsteps,
+                           double *const *lam6, double *const *tmp6, double gm, double safety, const double *dx,
+                           const double *dy, int ix, int jx, int margin, void *ws, void *stream) {
     if (!states || !dt_log || !lam6 || !tmp6 || !dx || !dy || !ws || nsteps < 0)
         return fail(CANS_ERR_ARG, "cans2d_roe_adjoint_run: bad argument");
     if (ix < 8 || jx < 8 || margin < 3 || ix - 2 * margin < margin || jx - 2 * margin < margin)
@@ -658,7 +658,8 @@ int cans2d_roe_adjoint_run(const double *const *states, const double *dt_log, in
     // the pending dt gradient of step n+1, which its R2 then clears
     for (int st = nsteps - 1; st >= 0; st--) {
         const bool pend = st < nsteps - 1;
-        HIPCHK(launch_adj2d_roe_step(states + 7 * st, states + 7 * (st + 1), cur, oth, (double *)ws, dt_log + st,
+        HIPCHK(launch_adj2d_roe_step(states + 7 * st, states + 7 * (st + 1), half ? half + 6 * st : nullptr, cur, oth,
+                                     (double *)ws, dt_log + st,
                                      acc + kAdjAccSlot * (st & 1), pend ? dt_log + st + 1 : nullptr,
                                      pend ? acc + kAdjAccSlot * ((st + 1) & 1) : nullptr,
                                      acc + kAdjAccSlot * ((st + 1) & 1), dx, dy, gm, safety, ix, jx, margin, s));

@@ -396,6 +396,55 @@ size_t cans_run_ws_bytes(int dim, int engine, int ix, int jx, int kx) {
 // cfl_m (3D) as the md_mhdsn 3D driver calls it;
 // mlw_m_s: 2D 7 (ro pr vx vy bx by az) on (r, theta) or 3D 8 on (r, theta,
 // phi), bx1d = x, xm, y, ym back to back, with cfl_m_s.
+int cans2d_roe_record(double *const *states, double *const *half, int nsteps, void *clk, double gm, double safety,
+                      const double *dx, const double *dy, int ix, int jx, int margin, void *ws, double *dt_log,
+                      void *stream) {
+    if (!states || !half || !clk || !ws || !dt_log || !dx || !dy || nsteps < 0)
+        return abi_fail(CANS_ERR_ARG, "cans2d_roe_record: NULL argument");
+    if (ix < 8 || jx < 8 || margin < 3 || ix - 2 * margin < margin || jx - 2 * margin < margin)
+        return abi_fail(CANS_ERR_ARG, "cans2d_roe_record: the Roe box needs margin >= 3 and n-2*margin >= margin "
+                        "(ix=%d jx=%d margin=%d)", ix, jx, margin);
+    for (int k = 0; k < 7 * (nsteps + 1); k++)
+        if (!states[k]) return abi_fail(CANS_ERR_ARG, "cans2d_roe_record: NULL state pointer %d", k);
+    for (int k = 0; k < 6 * nsteps; k++)
+        if (!half[k]) return abi_fail(CANS_ERR_ARG, "cans2d_roe_record: NULL half-step pointer %d", k);
+    hipStream_t s = (hipStream_t)stream;
+    Clock *c = (Clock *)clk;
+    const size_t N = (size_t)ix * jx;
+    Carver cv{(double *)ws};
+    Metric m[3];
+    carve_metric(cv, ix, jx, 1, m);
+    const int map[6] = {0, 1, 2, 3, 5, 6};   // ro pr vx vy bx by -> engine slots
+    for (int st = 0; st < nsteps; st++) {
+        double *const *in = states + 7 * st;
+        double *const *out = states + 7 * (st + 1);
+        CflArgs ca{};
+        ca.ro = in[0]; ca.pr = in[1]; ca.vx = in[2]; ca.vy = in[3]; ca.bx = in[4]; ca.by = in[5];
+        ca.dx = dx; ca.dy = dy; ca.safety = safety; ca.gm = gm;
+        ca.ix = ix; ca.jx = jx; ca.kx = 1; ca.dim = 2; ca.clk = c; ca.step = st;
+        HIPCHK(launch_cfl(ca, s));
+        HIPCHK(launch_clock_step(c, st, s));
+        HIPCHK(launch_clock_dt_log(c, st, dt_log + st, s));
+        RoeArgs r{};
+        for (int f = 0; f < 6; f++) { r.prim[map[f]] = in[f]; r.out[map[f]] = out[f]; }
+        Carver ec{cv.p};
+        roe_scratch(r, ec, N, 2);
+        for (int f = 0; f < 6; f++) r.half[map[f]] = half[6 * st + f];   // the half step goes to the record
+        r.az = in[6];
+        r.out_az = out[6];
+        r.dx = dx; r.dy = dy; r.gm = gm; r.clk = c; r.step = st; r.ix = ix; r.jx = jx; r.kx = 1;
+        HIPCHK(launch_roe(r, 2, false, s));
+        for (int dir = 0; dir < 2; dir++) {
+            BndArgs bp{};
+            for (int f = 0; f < 7; f++) { bp.q[bp.nf] = out[f]; bp.kind[bp.nf] = BD_PER; bp.nf++; }
+            bp.dir = dir; bp.mbnd = margin; bp.margin = margin; bp.dm = dir == 0 ? dx : dy;
+            bp.ix = ix; bp.jx = jx; bp.kx = 1;
+            HIPCHK(launch_bnd(bp, s));
+        }
+    }
+    return 0;
+}
+
 int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b, const double *bx1d,
              const double *bxm1d, void *clk, int step0, int nsteps, double qav, double gm, double mu,
              double safety, const double *dx, const double *dxm, int ix, const double *dy, const double *dym,

@@ -803,11 +803,11 @@ hipError_t launch_copy6(double *const *dst, double *const *src, size_t n, hipStr
     return hipGetLastError();
 }
 
-hipError_t launch_adj2d_roe_step(const double *const *pn, const double *const *pn1, const double *const *lin,
-                                 double *const *lout, double *ws, const double *dtp, double *acc,
-                                 const double *pend_dt, const double *pend_acc, double *acc_zero, const double *dx,
-                                 const double *dy, double gm, double safety, int ix, int jx, int margin,
-                                 hipStream_t s) {
+hipError_t launch_adj2d_roe_step(const double *const *pn, const double *const *pn1, const double *const *ph,
+                                 const double *const *lin, double *const *lout, double *ws, const double *dtp,
+                                 double *acc, const double *pend_dt, const double *pend_acc, double *acc_zero,
+                                 const double *dx, const double *dy, double gm, double safety, int ix, int jx,
+                                 int margin, hipStream_t s) {
     const size_t n = (size_t)ix * jx;
     const size_t na = (n + 63) & ~size_t(63);
     RoeAdj r{};
@@ -818,9 +818,10 @@ hipError_t launch_adj2d_roe_step(const double *const *pn, const double *const *p
         r.lout[f] = lout[f];
     }
     const int slot[6] = {0, 1, 2, 3, 5, 6};   // the Roe engine's slots of ro pr vx vy bx by
-    for (int f = 0; f < 6; f++) r.ph[f] = ws + (RW_PH + slot[f]) * na;
+    for (int f = 0; f < 6; f++) r.ph[f] = ph ? ph[f] : ws + (RW_PH + slot[f]) * na;
     r.dx = dx; r.dy = dy; r.dtp = dtp; r.ldt = acc; r.acc_zero = acc_zero; r.gm = gm; r.ix = ix; r.jx = jx;
-    // the forward's stage-1 fluxes and half-step primitives at P^n
+    // the forward's half-step primitives at P^n: recorded, or recomputed
+    // (the stage-1 fluxes and the half step) into ws
     RoeArgs f{};
     for (int q = 0; q < 6; q++) f.prim[slot[q]] = pn[q];
     for (int q = 0; q < 8; q++) {
@@ -829,8 +830,10 @@ hipError_t launch_adj2d_roe_step(const double *const *pn, const double *const *p
         f.flux[1][q] = ws + (RW_F1 + 8 + q) * na;
     }
     f.dx = dx; f.dy = dy; f.gm = gm; f.dtp = dtp; f.ix = ix; f.jx = jx; f.kx = 1;
-    hipError_t e = launch_roe2d_half(f, s);
-    if (e != hipSuccess) return e;
+    if (!ph) {
+        hipError_t e = launch_roe2d_half(f, s);
+        if (e != hipSuccess) return e;
+    }
     Metric z{};
     AdjArgs a = make_adj(pn, pn1, lin, lout, ws, dtp, acc, dx, nullptr, dy, nullptr, z, z, 0.0, gm, safety, ix, jx,
                          margin);

@@ -237,6 +237,8 @@ __global__ void clock_reset_kernel(Clock *c, double t0, double tend, double dtmi
 }
 __global__ void arm_key_kernel(unsigned long long *k) { *k = dkey(kR4_1e20); }
 __global__ void clock_step_kernel(Clock *c, int n) { (void)clock_enter(c, n, true); }
+// the dt step n runs with (after its clock step), for a recorded trajectory
+__global__ void clock_dt_log_kernel(const Clock *c, int n, double *out) { *out = c->dt_last[(n + 1) & 1]; }
 
 }  // namespace
 
@@ -301,6 +303,11 @@ hipError_t launch_arm_key(unsigned long long *key, hipStream_t s) {
 
 hipError_t launch_clock_step(Clock *clk, int step, hipStream_t s) {
     hipLaunchKernelGGL(clock_step_kernel, dim3(1), dim3(1), 0, s, clk, step);
+    return hipGetLastError();
+}
+
+hipError_t launch_clock_dt_log(const Clock *clk, int step, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(clock_dt_log_kernel, dim3(1), dim3(1), 0, s, clk, step, out);
     return hipGetLastError();
 }
 

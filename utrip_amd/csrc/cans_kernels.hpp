@@ -257,13 +257,14 @@ hipError_t launch_key_to_dt(const unsigned long long *key, double *dt, hipStream
 hipError_t launch_zero(double *p, size_t n, hipStream_t s);
 hipError_t launch_periodic_fold(double *const *lam, int ix, int jx, int margin, hipStream_t s);
 size_t adj2d_ws_doubles(int ix, int jx);
-// reverse mode of one periodic 2D Roe-MUSCL step (margin >= 3): the forward's
-// stage-1 fluxes and half step are recomputed from pn into ws
-hipError_t launch_adj2d_roe_step(const double *const *pn, const double *const *pn1, const double *const *lin,
-                                 double *const *lout, double *ws, const double *dtp, double *acc,
-                                 const double *pend_dt, const double *pend_acc, double *acc_zero, const double *dx,
-                                 const double *dy, double gm, double safety, int ix, int jx, int margin,
-                                 hipStream_t s);
+// reverse mode of one periodic 2D Roe-MUSCL step (margin >= 3); ph: the
+// step's half-step primitives (ro pr vx vy bx by) as recorded, or NULL to
+// recompute them from pn into ws
+hipError_t launch_adj2d_roe_step(const double *const *pn, const double *const *pn1, const double *const *ph,
+                                 const double *const *lin, double *const *lout, double *ws, const double *dtp,
+                                 double *acc, const double *pend_dt, const double *pend_acc, double *acc_zero,
+                                 const double *dx, const double *dy, double gm, double safety, int ix, int jx,
+                                 int margin, hipStream_t s);
 size_t adj2d_roe_ws_doubles(int ix, int jx);
 // One accumulator slot of the adjoint step: dL/ddt in kAdjStripes partial
 // sums (blocks add to stripe blockIdx % kAdjStripes, so that same-address
@@ -286,5 +287,6 @@ hipError_t launch_clock_reset(Clock *clk, double t0, double tend, double dtmin, 
                               hipStream_t s);
 // a clock-only step (for paths whose update kernel does not carry the clock)
 hipError_t launch_clock_step(Clock *clk, int step, hipStream_t s);
+hipError_t launch_clock_dt_log(const Clock *clk, int step, double *out, hipStream_t s);
 
 }  // namespace cans
