@@ -45,6 +45,11 @@ int cans_gpu_sync(void *stream);
  * that one launch has at least `waves` wavefronts (default 32768; fewer
  * waves = longer segments).  Results do not depend on it. */
 int cans_gpu_set_march_waves(int waves);
+/* Tuning, no counterpart in the reference: the Cartesian 3D MLW step
+ * (mlw_m 3D) runs as one kernel marching in z with its cell and corner
+ * planes in LDS (path 1, the default) or as three kernels over HBM scratch
+ * (path 0).  Both give the same bits. */
+int cans_gpu_set_mlw3d_path(int path);
 
 /* ---------------------------------------------------------------- 1D
  * cans1d/hdmlw/mlw_m.f:2 mlw_m(ro,pr,vx,vy,by,bx,bxm,dt,qav,gm,dx,dxm,ix)

@@ -76,19 +76,26 @@ def test_roe3d_one_call(dev, oracle, shape):
 
 
 # ------------------------------------------------------------------ 3D MLW one call
-# (17, 9, 70), (31, 16, 34), (16, 8, 66): several z-segments of the fused
-# Cartesian cells+corners kernel (32 corner planes each; 34 leaves one plane
-# in the last), x and y tiles cut at their edges
-@pytest.mark.parametrize("shape", [(3, 3, 3), (13, 11, 10), (40, 7, 9), (17, 9, 70), (31, 16, 34), (16, 8, 66)])
-def test_mlw3d_one_call(dev, oracle, shape):
+# Path 1 (default) is the z-marching kernel on 16x16 column tiles (14x14
+# updated cells): (17, 9, 70) and (40, 7, 9) cut the last x tile short,
+# (31, 16, 34) has three x tiles and one y tile of exactly 14, (47, 33, 12)
+# three by three tiles; path 0 is the three kernels over HBM scratch.
+@pytest.mark.parametrize("path", [1, 0])
+@pytest.mark.parametrize("shape", [(3, 3, 3), (13, 11, 10), (40, 7, 9), (17, 9, 70), (31, 16, 34), (16, 8, 66),
+                                   (47, 33, 12)])
+def test_mlw3d_one_call(dev, oracle, shape, path):
     from utrip_amd import cans
-    case("mlw3d", shape)
+    case("mlw3d", shape, "path", path)
     s = smooth_state_3d(*shape, seed=7 + sum(shape))
     dx, dxm, dy, dym, dz, dzm = grid3(*shape)
     ref = {k: v.copy(order="F") for k, v in s.items()}
     oracle.mlw_m_3d(*[ref[k] for k in F3], 1.2e-3, 3.0, 5 / 3, dx, dxm, dy, dym, dz, dzm)
     t = {k: to_dev(v, dev) for k, v in s.items()}
-    cans.mlw_m_3d(*[t[k] for k in F3], 1.2e-3, 3.0, 5 / 3, *[to_dev(v, dev) for v in (dx, dxm, dy, dym, dz, dzm)])
+    cans.set_mlw3d_path(path)
+    try:
+        cans.mlw_m_3d(*[t[k] for k in F3], 1.2e-3, 3.0, 5 / 3, *[to_dev(v, dev) for v in (dx, dxm, dy, dym, dz, dzm)])
+    finally:
+        cans.set_mlw3d_path(1)
     for k in F3:
         assert_bitwise(to_host(t[k]), ref[k], k)
 

@@ -53,6 +53,13 @@ def set_march_waves(waves: int) -> None:
     _check(lib().cans_gpu_set_march_waves(int(waves)))
 
 
+def set_mlw3d_path(path: int) -> None:
+    """Cartesian 3D MLW step: 1 (default) one kernel marching in z with its
+    cell and corner planes in LDS, 0 three kernels over HBM scratch.  Both
+    give the same bits."""
+    _check(lib().cans_gpu_set_mlw3d_path(int(path)))
+
+
 def _stream() -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 

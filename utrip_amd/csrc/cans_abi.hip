@@ -87,8 +87,10 @@ int cans::abi_scratch(size_t bytes, void **out, hipStream_t s) { return pool_scr
 
 namespace {
 std::atomic<int> g_march_waves{32768};
+std::atomic<int> g_mlw3d_path{1};
 }
 int cans::march_waves() { return g_march_waves.load(); }
+int cans::mlw3d_path() { return g_mlw3d_path.load(); }
 
 // sin, cos, tan of the colatitudes y (and ym) on the host -- the C library's
 // functions, as the reference's compiled intrinsics -- into dev[0..3jx)
@@ -130,6 +132,12 @@ int cans_gpu_sync(void *stream) {
 int cans_gpu_set_march_waves(int waves) {
     if (waves < 1) return fail(CANS_ERR_ARG, "march waves %d < 1", waves);
     g_march_waves.store(waves);
+    return 0;
+}
+
+int cans_gpu_set_mlw3d_path(int path) {
+    if (path != 0 && path != 1) return fail(CANS_ERR_ARG, "mlw3d path %d (0 or 1)", path);
+    g_mlw3d_path.store(path);
     return 0;
 }
 

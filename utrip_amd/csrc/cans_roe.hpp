@@ -73,5 +73,10 @@ hipError_t launch_hlld(const HlldArgs &a, bool mixed, hipStream_t s);
 // column into segments so that a launch has at least this many waves
 // (default 32768; cans_gpu_set_march_waves changes it).
 int march_waves();
+// 3D MLW, Cartesian grids: 1 (default) = the z-marching kernel of
+// cans_mlw3d_march.hip, 0 = the three kernels over HBM scratch of
+// cans_mlw3d.hip (cans_gpu_set_mlw3d_path changes it).
+int mlw3d_path();
+hipError_t launch_mlw3d_march(const Mlw3dArgs &a, hipStream_t s);
 
 }  // namespace cans
