@@ -539,8 +539,12 @@ CANS_DEV void flmt_d(double a, double b, double &ga, double &gb) {
 }
 
 // stage-2 faces of direction dir: x faces i in [2, ix-2], j in [3, jx-2]
-// (y: transposed), between cells c and c+1 along dir
-__global__ __launch_bounds__(256) void roe_r2(RoeAdj a, int dir) {
+// (y: transposed), between cells c and c+1 along dir.  The direction is a
+// template argument: the flux slots are indexed by it, and a run-time index
+// would put those small arrays in scratch.
+template <int DIR>
+__global__ __launch_bounds__(256) void roe_r2(RoeAdj a) {
+    constexpr int dir = DIR;
     const int ix = a.ix, jx = a.jx;
     const int na_ = dir == 0 ? ix - 3 : ix - 4, nb_ = dir == 0 ? jx - 4 : jx - 3;
     const size_t t = xcd_first_thread() + threadIdx.x;
@@ -647,7 +651,9 @@ __global__ __launch_bounds__(256) void roe_r3(RoeAdj a) {
 
 // stage-1 faces: lam F1 from the two cells' lam Uh, back through the flux at
 // (P_c, P_c+1); lam of both states stored per face
-__global__ __launch_bounds__(256) void roe_r4(RoeAdj a, int dir) {
+template <int DIR>
+__global__ __launch_bounds__(256) void roe_r4(RoeAdj a) {
+    constexpr int dir = DIR;
     const int ix = a.ix, jx = a.jx;
     const int na_ = dir == 0 ? ix - 1 : ix - 2, nb_ = dir == 0 ? jx - 2 : jx - 1;
     const size_t t = xcd_first_thread() + threadIdx.x;
@@ -842,11 +848,11 @@ hipError_t launch_adj2d_roe_step(const double *const *pn, const double *const *p
     a.pend_acc = pend_acc;
     a.acc_zero = nullptr;
     hipLaunchKernelGGL(adj_b1, dim3(xcd_grid(n)), dim3(256), 0, s, a);
-    for (int dir = 0; dir < 2; dir++)
-        hipLaunchKernelGGL(roe_r2, dim3(xcd_grid(n)), dim3(256), 0, s, r, dir);
+    hipLaunchKernelGGL(roe_r2<0>, dim3(xcd_grid(n)), dim3(256), 0, s, r);
+    hipLaunchKernelGGL(roe_r2<1>, dim3(xcd_grid(n)), dim3(256), 0, s, r);
     hipLaunchKernelGGL(roe_r3, dim3(xcd_grid(n)), dim3(256), 0, s, r);
-    for (int dir = 0; dir < 2; dir++)
-        hipLaunchKernelGGL(roe_r4, dim3(xcd_grid(n)), dim3(256), 0, s, r, dir);
+    hipLaunchKernelGGL(roe_r4<0>, dim3(xcd_grid(n)), dim3(256), 0, s, r);
+    hipLaunchKernelGGL(roe_r4<1>, dim3(xcd_grid(n)), dim3(256), 0, s, r);
     hipLaunchKernelGGL(roe_r5, dim3(xcd_grid(n)), dim3(256), 0, s, r);
     return hipGetLastError();
 }

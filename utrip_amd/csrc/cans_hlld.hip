@@ -326,9 +326,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 // the FP64 step went from 31.3 to 26.4 ms, FP32-flux from 26.0 to 21.9 ms).
 // The window needs two waves per SIMD with FP64 fluxes (231 VGPRs; at three
 // the spills made it slower), three with FP32.
-template <class T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 1))) void hlld_sweep_yz(HlldArgs a, int dir, int stage, int H,
+// The direction is a template argument: the face states and fluxes are
+// indexed by it, and a run-time index would put those arrays in scratch.
+template <class T, int DIR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 1))) void hlld_sweep_yz(HlldArgs a, int stage, int H,
                                                                                                            int to_prim) {
+    constexpr int dir = DIR;
     double dt;
     if (!hstep(a, dt)) return;
     const double dts = stage == 1 ? 0.5 * dt : dt;
@@ -358,13 +361,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 #pragma unroll
         for (int q = 0; q < 8; q++) d[q] = P[q][pc];
     };
+    // the next cell of the window is loaded one face ahead, so that its
+    // loads are in flight while the current face is computed
+    double wn[8];
     load_cell(cb - 2, w0); load_cell(cb - 1, w1); load_cell(cb, w2); load_cell(cb + 1, w3);
+    load_cell(cb + 2, wn);
     double Fl[8], Fh[8];
     hlld_face_w<T>(a, dir, stage == 2 && cb - 1 >= 2 && cb - 1 <= nd - 2, w0, w1, w2, w3, Fl);
     for (int c = cb; c <= ce; c++) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) { w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; }
-        load_cell(c + 2, w3);
+        for (int q = 0; q < 8; q++) { w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; w3[q] = wn[q]; }
+        if (c < ce) load_cell(c + 3, wn);
         hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
         const size_t p = base + s * (size_t)(c - 1);
         if (to_prim) {
@@ -434,8 +441,12 @@ hipError_t launch_hlld_t(const HlldArgs &a, hipStream_t s) {
                 const int H = (n[dir] - 2 + nseg - 1) / nseg;
                 nseg = (n[dir] - 2 + H - 1) / H;
                 const size_t nt = cols * (size_t)nseg;
-                hipLaunchKernelGGL(hlld_sweep_yz<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a, dir, stage,
-                                   H, dir == 2 && fuse_p);
+                if (dir == 1)
+                    hipLaunchKernelGGL((hlld_sweep_yz<T, 1>), dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a,
+                                       stage, H, 0);
+                else
+                    hipLaunchKernelGGL((hlld_sweep_yz<T, 2>), dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a,
+                                       stage, H, fuse_p);
             }
         }
         if (!fuse_p) hipLaunchKernelGGL(hlld_prim, dim3(gN), dim3(256), 0, s, a, stage);
