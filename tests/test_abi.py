@@ -58,6 +58,23 @@ def test_fortran_dropin_libraries_export_reference_names():
         pytest.skip("Fortran drop-in libraries not built")
 
 
+def test_tuning_switches_validate_without_a_gpu():
+    """The two host-side tuning switches take effect without a device call
+    and reject values outside their range with the library's error."""
+    pytest.importorskip("torch")
+    from utrip_amd import LIB_GPU, cans
+    if not LIB_GPU.exists():
+        pytest.skip("libcans_gpu.so not built")
+    for bad in (-1, 2):
+        with pytest.raises(cans.CansError, match="mlw3d path"):
+            cans.set_mlw3d_path(bad)
+    for good in (0, 1):
+        cans.set_mlw3d_path(good)
+    with pytest.raises(cans.CansError, match="march waves"):
+        cans.set_march_waves(0)
+    cans.set_march_waves(32768)
+
+
 def test_python_mirror_rejects_bad_arrays():
     torch = pytest.importorskip("torch")
     from utrip_amd import cans
