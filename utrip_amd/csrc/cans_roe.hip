@@ -369,6 +369,152 @@ __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
     if (D >= 2) a.out[S_BX][p] = U[S_BX];
 }
 
+// ------------------------------------------------- 3D: z faces with the update
+// The z sweep of a stage fused with the cell update that follows it (the
+// predictor of stage 1, the corrector of stage 2): a thread marches the
+// z faces of its column as roe_flux_march3 does and, once it holds the
+// faces below and above cell k, updates the cell -- the x and y face
+// contributions read from their arrays, the z contribution from registers,
+// in the order and with the expressions of roe_half_kernel and
+// roe_update_kernel.  The z faces never go to memory.  A segment starts one
+// face early (computed twice, by the same operations) so that it holds the
+// face below its first cell.
+template <class T, int STAGE>
+__global__ __launch_bounds__(256) void roe_zmarch3(RoeArgs a, int H) {
+    double dt;
+    if (!step_dt(a, dt)) return;
+    constexpr int dir = 2;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int lo_n = STAGE == 1 ? 1 : 2, hi_n = STAGE == 1 ? kx - 1 : kx - 2;
+    const int lo_t = STAGE == 1 ? 2 : 3;
+    const int ci = ix + 2 - 2 * lo_t, co = jx + 2 - 2 * lo_t, nf = hi_n - lo_n + 1;
+    if (ci <= 0 || co <= 0 || nf <= 0) return;
+    const int nseg = (nf + H - 1) / H;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)ci * co * nseg) return;
+    const int i = lo_t + (int)(t % ci);
+    const int j = lo_t + (int)((t / ci) % co);
+    const int seg = (int)(t / ((size_t)ci * co));
+    const int fb = lo_n + seg * H, fe = min(fb + H - 1, hi_n);
+    const int fs = fb > lo_n ? fb - 1 : fb;
+    const int nd = kx;
+    const size_t sx = 1, sy = (size_t)ix, s = (size_t)ix * jx;
+    const size_t base = lin(i, j, 1, ix, jx);
+    const double *const *P = STAGE == 1 ? a.prim : a.half;
+    double w0[8], w1[8], w2[8], w3[8];
+    auto load_cell = [&](int c, double *d) {
+        const int cc = c < 1 ? 1 : (c > nd ? nd : c);
+        const size_t pc = base + s * (size_t)(cc - 1);
+#pragma unroll
+        for (int q = 0; q < 8; q++) d[q] = P[q][pc];
+    };
+    load_cell(fs - 1, w0); load_cell(fs, w1); load_cell(fs + 1, w2); load_cell(fs + 2, w3);
+    const T gm = T(a.gm);
+    constexpr int vn = S_Z, v1 = S_X, v2 = S_Y, bn = S_BZ, b1 = S_BX, b2 = S_BY;
+    const double dx = a.dx[i - 1], dy = a.dy[j - 1];
+    const double hdt = STAGE == 1 ? 0.5 * dt : dt;
+    double Fl[8];   // z flux of the face below the cell, U slots (normal field: unused)
+#pragma unroll
+    for (int q = 0; q < 8; q++) Fl[q] = 0.0;
+    const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
+    for (int f = fs;; f++) {
+        T L[8], R[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (STAGE == 1) {
+                L[q] = T(w1[q]);
+                R[q] = T(w2[q]);
+            } else {               // cans3d/hdroe/tvdminmod.f:15-55
+                L[q] = T(w1[q] + 0.5 * flmt(w1[q] - w0[q], w2[q] - w1[q]));
+                R[q] = T(w2[q] - 0.5 * flmt(w3[q] - w2[q], w2[q] - w1[q]));
+            }
+        }
+        T out[7];
+        roeflux3<T>(gm, L[S_RO], L[S_E], L[vn], L[v1], L[v2], L[bn], L[b1], L[b2], R[S_RO], R[S_E], R[vn], R[v1],
+                    R[v2], R[bn], R[b1], R[b2], out);
+        // the flux as the face arrays hold it (FP32 runs store FP32)
+        double Fh[8];
+        Fh[S_RO] = double(out[0]); Fh[S_E] = double(out[1]); Fh[vn] = double(out[2]); Fh[v1] = double(out[3]);
+        Fh[v2] = double(out[4]); Fh[b1] = double(out[5]); Fh[b2] = double(out[6]); Fh[bn] = 0.0;
+        if (f > fs) {
+            // cell k = f: faces k-1 (Fl) and k (Fh) along z
+            const int k = f;
+            const size_t p = base + s * (size_t)(k - 1);
+            double U[8];
+            prim_to_cons<3, false>(a, p, i, U);
+            const double dz = a.dz[k - 1];
+            double *const *Fx = a.flux[0];
+            double *const *Fy = a.flux[1];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (q == S_BX) continue;
+                U[q] = U[q] + hdt * ((fget<T>(Fx[q], p - sx) - fget<T>(Fx[q], p)) / dx);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (q == S_BY) continue;
+                U[q] = U[q] + hdt * ((fget<T>(Fy[q], p - sy) - fget<T>(Fy[q], p)) / dy);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (q == S_BZ) continue;
+                U[q] = U[q] + hdt * ((Fl[q] - Fh[q]) / dz);
+            }
+            const double ro = U[S_RO];
+            const double vx = U[S_X] / ro;
+            const double vy = U[S_Y] / ro;
+            const double vz = U[S_Z] / ro;
+            const double v2s = vx * vx + vy * vy + vz * vz;
+            const double b2s = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY] + U[S_BZ] * U[S_BZ];
+            double *const *O = STAGE == 1 ? a.half : a.out;
+            O[S_RO][p] = ro;
+            O[S_E][p] = (a.gm - 1.0) * (U[S_E] - 0.5 * ro * v2s - pi8i * b2s);
+            O[S_X][p] = vx;
+            O[S_Y][p] = vy;
+            O[S_Z][p] = vz;
+            O[S_BX][p] = U[S_BX];
+            O[S_BY][p] = U[S_BY];
+            O[S_BZ][p] = U[S_BZ];
+        }
+        if (f >= fe) break;
+#pragma unroll
+        for (int q = 0; q < 8; q++) { Fl[q] = Fh[q]; w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; }
+        load_cell(f + 3, w3);
+    }
+}
+
+// Half-step state of the outer cell layer of a 3D grid: the primitives
+// themselves (the predictor updates only cells [2, n-1]^3).
+__global__ __launch_bounds__(256) void roe_half_shell3(RoeArgs a) {
+    double dt;
+    if (!step_dt(a, dt)) return;
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const size_t fx = (size_t)jx * kx, fy = (size_t)(ix - 2) * kx, fz = (size_t)(ix - 2) * (jx - 2);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (t < 2 * fx) {
+        const size_t r = t % fx;
+        i = t < fx ? 1 : ix;
+        j = 1 + (int)(r % jx);
+        k = 1 + (int)(r / jx);
+    } else if (t < 2 * fx + 2 * fy) {
+        const size_t u = t - 2 * fx, r = u % fy;
+        j = u < fy ? 1 : jx;
+        i = 2 + (int)(r % (ix - 2));
+        k = 1 + (int)(r / (ix - 2));
+    } else if (t < 2 * fx + 2 * fy + 2 * fz) {
+        const size_t u = t - 2 * fx - 2 * fy, r = u % fz;
+        k = u < fz ? 1 : kx;
+        i = 2 + (int)(r % (ix - 2));
+        j = 2 + (int)(r / (ix - 2));
+    } else {
+        return;
+    }
+    const size_t p = lin(i, j, k, ix, jx);
+#pragma unroll
+    for (int q = 0; q < 8; q++) a.half[q][p] = a.prim[q][p];
+}
+
 // 1D and 2D kernels take their blocks in the XCD-contiguous order
 // (xcd_first_thread); measured at 1024^2 the 2D engine drops from 294 to
 // 266 us.  In 3D the linear order is faster (Roe 24.1 against 25.5 ms at
@@ -391,6 +537,28 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                 total *= (size_t)(c > 0 ? c : 0);
             }
             if (total == 0) continue;
+            if (D == 3 && dir == 2) {
+                // z faces fused with the stage's cell update (roe_zmarch3)
+                const int nf = stage == 1 ? n[2] - 1 : n[2] - 3;
+                const size_t cols = (size_t)(stage == 1 ? n[0] - 2 : n[0] - 4) *
+                                    (size_t)(stage == 1 ? n[1] - 2 : n[1] - 4);
+                int nseg = (int)(((unsigned long long)march_waves() * 64 + cols - 1) / cols);
+                nseg = nseg < 1 ? 1 : (nseg > nf ? nf : nseg);
+                const int H = (nf + nseg - 1) / nseg;
+                nseg = (nf + H - 1) / H;
+                const size_t threads = cols * (size_t)nseg;
+                if (stage == 1) {
+                    const size_t shell = 2 * ((size_t)n[1] * n[2] + (size_t)(n[0] - 2) * n[2] +
+                                              (size_t)(n[0] - 2) * (n[1] - 2));
+                    hipLaunchKernelGGL(roe_half_shell3, dim3((unsigned)((shell + 255) / 256)), dim3(256), 0, s, a);
+                    hipLaunchKernelGGL((roe_zmarch3<T, 1>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
+                                       a, H);
+                } else {
+                    hipLaunchKernelGGL((roe_zmarch3<T, 2>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
+                                       a, H);
+                }
+                continue;
+            }
             if (D == 3 && dir >= 1) {
                 // y and z faces marched (roe_flux_march3: 384^3 FP64 step 26.4 ->
                 // 23.7 ms, FP32-flux 19.5 -> 17.7 ms); segments along dir so that
@@ -411,12 +579,12 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
             hipLaunchKernelGGL((roe_flux_kernel<D, T, C3>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
                                a, dir, stage);
         }
-        if (stage == 1)
+        if (stage == 1 && D < 3)
             hipLaunchKernelGGL((roe_half_kernel<D, T, C3>), dim3(roe_grid<D>(N)), dim3(256), 0, s, a);
     }
     size_t cells = 1;
     for (int d = 0; d < D; d++) cells *= (size_t)(n[d] - 4 > 0 ? n[d] - 4 : 0);
-    if (cells)
+    if (cells && D < 3)
         hipLaunchKernelGGL((roe_update_kernel<D, T, C3>), dim3(roe_grid<D>(cells)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
