@@ -515,6 +515,144 @@ __global__ __launch_bounds__(256) void roe_half_shell3(RoeArgs a) {
     for (int q = 0; q < 8; q++) a.half[q][p] = a.prim[q][p];
 }
 
+// ------------------------------------------------- 2D: y faces with the update
+// The 2D form of roe_zmarch3: a thread marches the y faces of column i and
+// applies the stage's update (predictor or corrector, plus the corrector's
+// Az source) to each cell between two of them; x faces from their array.
+template <class T, int STAGE>
+__global__ __launch_bounds__(256) void roe_ymarch2(RoeArgs a, int H) {
+    double dt;
+    if (!step_dt(a, dt)) return;
+    const int ix = a.ix, jx = a.jx;
+    const int lo_n = STAGE == 1 ? 1 : 2, hi_n = STAGE == 1 ? jx - 1 : jx - 2;
+    const int lo_t = STAGE == 1 ? 2 : 3;
+    const int ci = ix + 2 - 2 * lo_t, nf = hi_n - lo_n + 1;
+    if (ci <= 0 || nf <= 0) return;
+    const int nseg = (nf + H - 1) / H;
+    const size_t t = xcd_first_thread() + threadIdx.x;
+    if (t >= (size_t)ci * nseg) return;
+    const int i = lo_t + (int)(t % ci);
+    const int seg = (int)(t / ci);
+    const int fb = lo_n + seg * H, fe = min(fb + H - 1, hi_n);
+    const int fs = fb > lo_n ? fb - 1 : fb;
+    const int nd = jx;
+    const size_t s = (size_t)ix;
+    const size_t base = lin(i, 1, 1, ix, jx);
+    const double *const *P = STAGE == 1 ? a.prim : a.half;
+    constexpr int NS = 6;
+    constexpr int slots[NS] = {S_RO, S_E, S_X, S_Y, S_BX, S_BY};
+    double w0[NS], w1[NS], w2[NS], w3[NS];
+    auto load_cell = [&](int c, double *d) {
+        const int cc = c < 1 ? 1 : (c > nd ? nd : c);
+        const size_t pc = base + s * (size_t)(cc - 1);
+#pragma unroll
+        for (int m = 0; m < NS; m++) d[m] = P[slots[m]][pc];
+    };
+    load_cell(fs - 1, w0); load_cell(fs, w1); load_cell(fs + 1, w2); load_cell(fs + 2, w3);
+    const T gm = T(a.gm);
+    const double dx = a.dx[i - 1];
+    const double hdt = STAGE == 1 ? 0.5 * dt : dt;
+    const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
+    double Fl[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) Fl[q] = 0.0;
+    for (int f = fs;; f++) {
+        T L[8], R[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) { L[q] = T(0); R[q] = T(0); }
+#pragma unroll
+        for (int m = 0; m < NS; m++) {
+            const int q = slots[m];
+            if (STAGE == 1) {
+                L[q] = T(w1[m]);
+                R[q] = T(w2[m]);
+            } else {               // cans2d/hdroe/tvdminmod.f
+                const double d0 = w0[m], d1 = w1[m], d2 = w2[m], d3 = w3[m];
+                L[q] = T(d1 + 0.5 * flmt(d2 - d1, d1 - d0));
+                R[q] = T(d2 - 0.5 * flmt(d2 - d1, d3 - d2));
+            }
+        }
+        // 2D y: (vy, vx, by, bx) -> ro ee ry rx bx
+        T o[5];
+        roeflux12<T>(gm, L[S_RO], L[S_E], L[S_Y], L[S_X], L[S_BY], L[S_BX], R[S_RO], R[S_E], R[S_Y], R[S_X], R[S_BY],
+                     R[S_BX], o);
+        double Fh[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) Fh[q] = 0.0;
+        Fh[S_RO] = double(o[0]); Fh[S_E] = double(o[1]); Fh[S_Y] = double(o[2]); Fh[S_X] = double(o[3]);
+        Fh[S_BX] = double(o[4]);
+        if (f > fs) {
+            const int j = f;
+            const size_t p = base + s * (size_t)(j - 1);
+            double U[8];
+            prim_to_cons<2, false>(a, p, i, U);
+            const double dy = a.dy[j - 1];
+            double *const *Fx = a.flux[0];
+            // x then y, as roe_half_kernel / roe_update_kernel (2D: no z
+            // components; no update of the face-normal field)
+            const int qs[5] = {S_RO, S_E, S_X, S_Y, S_BY};
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                const int q = qs[m];
+                U[q] = U[q] + hdt * ((fget<T>(Fx[q], p - 1) - fget<T>(Fx[q], p)) / dx);
+            }
+            const int qy[5] = {S_RO, S_E, S_X, S_Y, S_BX};
+#pragma unroll
+            for (int m = 0; m < 5; m++) {
+                const int q = qy[m];
+                U[q] = U[q] + hdt * ((Fl[q] - Fh[q]) / dy);
+            }
+            if (STAGE == 2) {   // roe_m.f:187-193
+                const double vxh = a.half[S_X][p], vyh = a.half[S_Y][p], bxh = a.half[S_BX][p], byh = a.half[S_BY][p];
+                const double ez = -vxh * byh + vyh * bxh;
+                const double saz = -ez;
+                a.out_az[p] = a.az[p] + dt * saz;
+            }
+            const double ro = U[S_RO];
+            const double vx = U[S_X] / ro;
+            const double vy = U[S_Y] / ro;
+            const double v2 = vx * vx + vy * vy;
+            const double b2 = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY];
+            double *const *O = STAGE == 1 ? a.half : a.out;
+            O[S_RO][p] = ro;
+            O[S_E][p] = (a.gm - 1.0) * (U[S_E] - 0.5 * ro * v2 - pi8i * b2);
+            O[S_X][p] = vx;
+            O[S_Y][p] = vy;
+            O[S_BY][p] = U[S_BY];
+            O[S_BX][p] = U[S_BX];
+        }
+        if (f >= fe) break;
+#pragma unroll
+        for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
+#pragma unroll
+        for (int m = 0; m < NS; m++) { w0[m] = w1[m]; w1[m] = w2[m]; w2[m] = w3[m]; }
+        load_cell(f + 3, w3);
+    }
+}
+
+// Half-step state of the outer cell ring of a 2D grid: its primitives.
+__global__ __launch_bounds__(256) void roe_half_ring2(RoeArgs a) {
+    double dt;
+    if (!step_dt(a, dt)) return;
+    const int ix = a.ix, jx = a.jx;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j;
+    if (t < 2 * (size_t)ix) {
+        i = 1 + (int)(t % ix);
+        j = t < (size_t)ix ? 1 : jx;
+    } else if (t < 2 * (size_t)ix + 2 * (size_t)(jx - 2)) {
+        const size_t u = t - 2 * (size_t)ix;
+        j = 2 + (int)(u % (jx - 2));
+        i = u < (size_t)(jx - 2) ? 1 : ix;
+    } else {
+        return;
+    }
+    const size_t p = lin(i, j, 1, ix, jx);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (a.prim[q]) a.half[q][p] = a.prim[q][p];
+}
+
 // 1D and 2D kernels take their blocks in the XCD-contiguous order
 // (xcd_first_thread); measured at 1024^2 the 2D engine drops from 294 to
 // 266 us.  In 3D the linear order is faster (Roe 24.1 against 25.5 ms at
@@ -527,6 +665,9 @@ template <int D, class T, bool C3>
 hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
     const int n[3] = {a.ix, a.jx, a.kx};
     const size_t N = (size_t)a.ix * a.jx * a.kx;
+    // 2D: y faces fused with the cell updates (roe_ymarch2) when the grid has
+    // interior cells; otherwise (and for roe_m3) the separate passes
+    const bool fy2 = D == 2 && !C3 && n[0] >= 3 && n[1] >= 3;
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < D; dir++) {
             size_t total = 1;
@@ -576,15 +717,35 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                                    dir, stage, H);
                 continue;
             }
+            if (fy2 && dir == 1) {
+                // y faces fused with the stage's cell update (roe_ymarch2):
+                // segments of about 9 faces (one face in 9 computed twice),
+                // about 2048 waves at 1024^2
+                const int nf = stage == 1 ? n[1] - 1 : n[1] - 3;
+                const size_t cols = (size_t)(stage == 1 ? n[0] - 2 : n[0] - 4);
+                int nseg = (int)((2048ull * 64 + cols - 1) / cols);
+                nseg = nseg < 1 ? 1 : (nseg > nf ? nf : nseg);
+                const int H = (nf + nseg - 1) / nseg;
+                nseg = (nf + H - 1) / H;
+                const size_t threads = cols * (size_t)nseg;
+                if (stage == 1) {
+                    const size_t ring = 2 * (size_t)n[0] + 2 * (size_t)(n[1] - 2);
+                    hipLaunchKernelGGL(roe_half_ring2, dim3((unsigned)((ring + 255) / 256)), dim3(256), 0, s, a);
+                    hipLaunchKernelGGL((roe_ymarch2<T, 1>), dim3(xcd_grid(threads)), dim3(256), 0, s, a, H);
+                } else {
+                    hipLaunchKernelGGL((roe_ymarch2<T, 2>), dim3(xcd_grid(threads)), dim3(256), 0, s, a, H);
+                }
+                continue;
+            }
             hipLaunchKernelGGL((roe_flux_kernel<D, T, C3>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
                                a, dir, stage);
         }
-        if (stage == 1 && D < 3)
+        if (stage == 1 && (D == 1 || (D == 2 && !fy2)))
             hipLaunchKernelGGL((roe_half_kernel<D, T, C3>), dim3(roe_grid<D>(N)), dim3(256), 0, s, a);
     }
     size_t cells = 1;
     for (int d = 0; d < D; d++) cells *= (size_t)(n[d] - 4 > 0 ? n[d] - 4 : 0);
-    if (cells && D < 3)
+    if (cells && (D == 1 || (D == 2 && !fy2)))
         hipLaunchKernelGGL((roe_update_kernel<D, T, C3>), dim3(roe_grid<D>(cells)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
