@@ -251,6 +251,28 @@ def test_run_3d_blast(dev, oracle, engine, margin):
         assert_bitwise(got[k], ref[k], k)
 
 
+# Longer runs of the marching and fused kernels against the oracle: the 3D
+# MLW march on several column tiles and z-segments, the 3D Roe and HLLD with
+# the stage updates fused into their z sweeps (several segments per column
+# at the default wave target), the 2D Roe with the fused y marches (several
+# y segments per column): every field and the dt sequence, bit for bit.
+@pytest.mark.parametrize("dim,engine,n,margin,nsteps", [(3, "mlw", 30, 2, 40), (3, "roe", 26, 3, 30),
+                                                        (3, "hlld", 26, 3, 30), (2, "roe", 130, 3, 80)])
+def test_longer_runs_of_the_fused_paths(dev, oracle, dim, engine, n, margin, nsteps):
+    case("longer run", dim, engine, n, nsteps)
+    if dim == 3:
+        pb = P.blast_3d(n=n, margin=margin, gaussian_b=(engine != "hlld"), r0=0.25)
+        names = F3
+    else:
+        pb = P.orszag_tang_2d(n=n, margin=margin)
+        names = F2
+    ref, dts = oracle_loop(oracle, dim, engine, pb.fields, pb.grid, pb.params, nsteps, 7, margin)
+    got, info = run_gpu(dev, dim, engine, pb.fields, pb.grid, pb.params, nsteps, 7, margin)
+    assert info["nexec"] == nsteps and info["dt"] == dts[-1]
+    for k in names:
+        assert_bitwise(got[k], ref[k], k)
+
+
 @pytest.mark.parametrize("waves", [1, 8])
 @pytest.mark.parametrize("engine", ["roe", "hlld"])
 def test_run_3d_blast_long_segments(dev, oracle, engine, waves):
