@@ -368,12 +368,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     load_cell(cb + 2, wn);
     double Fl[8], Fh[8];
     hlld_face_w<T>(a, dir, stage == 2 && cb - 1 >= 2 && cb - 1 <= nd - 2, w0, w1, w2, w3, Fl);
+    // With FP32 fluxes (registers to spare) the conserved state of the next
+    // cell is also loaded one face ahead (384^3 FP32-flux step 17.3 ->
+    // 15.8 ms); with FP64 fluxes the extra registers spill (18.9 -> 29.7 ms).
+    constexpr bool kPreU = sizeof(T) == 4;
+    double un[8];
+    if (kPreU) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) un[q] = u[q][base + s * (size_t)(cb - 1)];
+    }
     for (int c = cb; c <= ce; c++) {
 #pragma unroll
         for (int q = 0; q < 8; q++) { w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; w3[q] = wn[q]; }
         if (c < ce) load_cell(c + 3, wn);
-        hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
         const size_t p = base + s * (size_t)(c - 1);
+        if (kPreU) {
+            double uc[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) uc[q] = un[q];
+            if (c < ce) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) un[q] = u[q][p + s];
+            }
+            hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
+            double U[8];
+            const double dl = dd[c - 1];
+#pragma unroll
+            for (int q = 0; q < 8; q++) U[q] = uc[q] - dts * ((Fh[q] - Fl[q]) / dl);   // adddfdx.f
+            if (to_prim) {
+                hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; q++) u[q][p] = U[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
+            continue;
+        }
+        hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
         if (to_prim) {
             // the last sweep of the stage also recovers the primitives
             // (hlld_prim fused): half (stage 1) or out (stage 2)
