@@ -417,7 +417,23 @@ __global__ __launch_bounds__(256) void roe_zmarch3(RoeArgs a, int H) {
 #pragma unroll
     for (int q = 0; q < 8; q++) Fl[q] = 0.0;
     const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
+    // With FP32 fluxes (registers to spare) the cell update's operands are
+    // loaded before the face flux is computed, so that the flux covers
+    // their latency; with FP64 fluxes they would spill.
+    constexpr bool kPre = sizeof(T) == 4;
     for (int f = fs;; f++) {
+        double pw[8], fxl[8], fxh[8], fyl[8], fyh[8];
+        if (kPre && f > fs) {
+            const size_t p = base + s * (size_t)(f - 1);
+            double *const *Fx = a.flux[0];
+            double *const *Fy = a.flux[1];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                pw[q] = a.prim[q][p];
+                if (q != S_BX) { fxl[q] = fget<T>(Fx[q], p - sx); fxh[q] = fget<T>(Fx[q], p); }
+                if (q != S_BY) { fyl[q] = fget<T>(Fy[q], p - sy); fyh[q] = fget<T>(Fy[q], p); }
+            }
+        }
         T L[8], R[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) {
@@ -441,19 +457,35 @@ __global__ __launch_bounds__(256) void roe_zmarch3(RoeArgs a, int H) {
             const int k = f;
             const size_t p = base + s * (size_t)(k - 1);
             double U[8];
-            prim_to_cons<3, false>(a, p, i, U);
             const double dz = a.dz[k - 1];
             double *const *Fx = a.flux[0];
             double *const *Fy = a.flux[1];
+            if (kPre) {
+                // prim_to_cons<3> on the loaded primitives
+                const double ro = pw[S_RO], pr = pw[S_E], vx = pw[S_X], vy = pw[S_Y], vz = pw[S_Z];
+                const double bx = pw[S_BX], by = pw[S_BY], bz = pw[S_BZ];
+                U[S_RO] = ro;
+                U[S_X] = ro * vx;
+                U[S_Y] = ro * vy;
+                U[S_Z] = ro * vz;
+                const double v2 = vx * vx + vy * vy + vz * vz;
+                const double b2 = bx * bx + by * by + bz * bz;
+                U[S_E] = pr / (a.gm - 1.0) + 0.5 * ro * v2 + pi8i * b2;
+                U[S_BX] = bx; U[S_BY] = by; U[S_BZ] = bz;
+            } else {
+                prim_to_cons<3, false>(a, p, i, U);
+            }
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 if (q == S_BX) continue;
-                U[q] = U[q] + hdt * ((fget<T>(Fx[q], p - sx) - fget<T>(Fx[q], p)) / dx);
+                const double lo = kPre ? fxl[q] : fget<T>(Fx[q], p - sx), hi = kPre ? fxh[q] : fget<T>(Fx[q], p);
+                U[q] = U[q] + hdt * ((lo - hi) / dx);
             }
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 if (q == S_BY) continue;
-                U[q] = U[q] + hdt * ((fget<T>(Fy[q], p - sy) - fget<T>(Fy[q], p)) / dy);
+                const double lo = kPre ? fyl[q] : fget<T>(Fy[q], p - sy), hi = kPre ? fyh[q] : fget<T>(Fy[q], p);
+                U[q] = U[q] + hdt * ((lo - hi) / dy);
             }
 #pragma unroll
             for (int q = 0; q < 8; q++) {
