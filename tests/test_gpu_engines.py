@@ -254,10 +254,11 @@ def test_run_3d_blast(dev, oracle, engine, margin):
 # Longer runs of the marching and fused kernels against the oracle: the 3D
 # MLW march on several column tiles and z-segments, the 3D Roe and HLLD with
 # the stage updates fused into their z sweeps (several segments per column
-# at the default wave target), the 2D Roe with the fused y marches (several
-# y segments per column): every field and the dt sequence, bit for bit.
+# at the default wave target), the 2D Roe with the fused y marches (taken on
+# grids of at least 512K y faces: 724^2 is just above; 80 segments of 9
+# faces per column): every field and the dt sequence, bit for bit.
 @pytest.mark.parametrize("dim,engine,n,margin,nsteps", [(3, "mlw", 30, 2, 40), (3, "roe", 26, 3, 30),
-                                                        (3, "hlld", 26, 3, 30), (2, "roe", 130, 3, 80)])
+                                                        (3, "hlld", 26, 3, 30), (2, "roe", 724, 3, 16)])
 def test_longer_runs_of_the_fused_paths(dev, oracle, dim, engine, n, margin, nsteps):
     case("longer run", dim, engine, n, nsteps)
     if dim == 3:

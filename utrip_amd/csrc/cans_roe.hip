@@ -697,9 +697,13 @@ template <int D, class T, bool C3>
 hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
     const int n[3] = {a.ix, a.jx, a.kx};
     const size_t N = (size_t)a.ix * a.jx * a.kx;
-    // 2D: y faces fused with the cell updates (roe_ymarch2) when the grid has
-    // interior cells; otherwise (and for roe_m3) the separate passes
-    const bool fy2 = D == 2 && !C3 && n[0] >= 3 && n[1] >= 3;
+    // 2D: y faces fused with the cell updates (roe_ymarch2) on grids with
+    // at least 512K stage-1 y faces: segments of >= 8 faces (one in 8 or
+    // fewer computed twice) still give >= 1024 waves there.  Smaller grids
+    // (and roe_m3) take the separate passes, a thread per face: at 512^2 the
+    // march, cut into 3-face segments to fill the chip, was 6 % slower.
+    const bool fy2 = D == 2 && !C3 && n[0] >= 3 && n[1] >= 3 &&
+                     (size_t)(n[0] - 2) * (size_t)(n[1] - 1) >= ((size_t)1 << 19);
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < D; dir++) {
             size_t total = 1;
@@ -757,7 +761,8 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                 const size_t cols = (size_t)(stage == 1 ? n[0] - 2 : n[0] - 4);
                 int nseg = (int)((2048ull * 64 + cols - 1) / cols);
                 nseg = nseg < 1 ? 1 : (nseg > nf ? nf : nseg);
-                const int H = (nf + nseg - 1) / nseg;
+                int H = (nf + nseg - 1) / nseg;
+                H = H < 8 ? 8 : H;
                 nseg = (nf + H - 1) / H;
                 const size_t threads = cols * (size_t)nseg;
                 if (stage == 1) {
