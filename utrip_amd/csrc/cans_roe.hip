@@ -704,6 +704,19 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
     // march, cut into 3-face segments to fill the chip, was 6 % slower.
     const bool fy2 = D == 2 && !C3 && n[0] >= 3 && n[1] >= 3 &&
                      (size_t)(n[0] - 2) * (size_t)(n[1] - 1) >= ((size_t)1 << 19);
+    // 3D: z faces fused with the cell updates (roe_zmarch3) where the march's
+    // segments come out at >= 8 faces for the launch's wave target (about
+    // 256^3 and up at the default); on smaller grids the segments would be
+    // a face or two long and the fused march would compute most z faces
+    // twice, so they take the separate passes.
+    bool fz3 = false;
+    if (D == 3 && n[0] >= 3 && n[1] >= 3 && n[2] >= 3) {
+        const size_t cols = (size_t)(n[0] - 2) * (size_t)(n[1] - 2);
+        const int nf = n[2] - 1;
+        long nseg = (long)(((unsigned long long)march_waves() * 64 + cols - 1) / cols);
+        nseg = nseg < 1 ? 1 : (nseg > nf ? nf : nseg);
+        fz3 = (nf + nseg - 1) / nseg >= 8;
+    }
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < D; dir++) {
             size_t total = 1;
@@ -714,14 +727,15 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                 total *= (size_t)(c > 0 ? c : 0);
             }
             if (total == 0) continue;
-            if (D == 3 && dir == 2) {
+            if (fz3 && dir == 2) {
                 // z faces fused with the stage's cell update (roe_zmarch3)
                 const int nf = stage == 1 ? n[2] - 1 : n[2] - 3;
                 const size_t cols = (size_t)(stage == 1 ? n[0] - 2 : n[0] - 4) *
                                     (size_t)(stage == 1 ? n[1] - 2 : n[1] - 4);
                 int nseg = (int)(((unsigned long long)march_waves() * 64 + cols - 1) / cols);
                 nseg = nseg < 1 ? 1 : (nseg > nf ? nf : nseg);
-                const int H = (nf + nseg - 1) / nseg;
+                int H = (nf + nseg - 1) / nseg;
+                H = H < 8 ? 8 : H;
                 nseg = (nf + H - 1) / H;
                 const size_t threads = cols * (size_t)nseg;
                 if (stage == 1) {
@@ -736,7 +750,7 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                 }
                 continue;
             }
-            if (D == 3 && dir >= 1) {
+            if (D == 3 && dir >= 1) {   // y faces (and z faces without the fusion)
                 // y and z faces marched (roe_flux_march3: 384^3 FP64 step 26.4 ->
                 // 23.7 ms, FP32-flux 19.5 -> 17.7 ms); segments along dir so that
                 // the launch has >= march_waves() waves
@@ -777,12 +791,12 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
             hipLaunchKernelGGL((roe_flux_kernel<D, T, C3>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
                                a, dir, stage);
         }
-        if (stage == 1 && (D == 1 || (D == 2 && !fy2)))
+        if (stage == 1 && (D == 1 || (D == 2 && !fy2) || (D == 3 && !fz3)))
             hipLaunchKernelGGL((roe_half_kernel<D, T, C3>), dim3(roe_grid<D>(N)), dim3(256), 0, s, a);
     }
     size_t cells = 1;
     for (int d = 0; d < D; d++) cells *= (size_t)(n[d] - 4 > 0 ? n[d] - 4 : 0);
-    if (cells && (D == 1 || (D == 2 && !fy2)))
+    if (cells && (D == 1 || (D == 2 && !fy2) || (D == 3 && !fz3)))
         hipLaunchKernelGGL((roe_update_kernel<D, T, C3>), dim3(roe_grid<D>(cells)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
