@@ -39,7 +39,7 @@ constexpr int TB = 16;          // tile edge in cells, halo included
 constexpr int TO = TB - 2;      // updated cells per tile edge
 constexpr int NT = TB * TB;     // threads per block
 
-// LDS slots of a cell plane (one buffer)
+// LDS slots of a cell plane (two buffers, by plane parity)
 enum { L_RO, L_EE, L_RX, L_RY, L_BX, L_BY, L_BZ, L_EEX, L_EEY, L_RXX, L_RXY, L_RYX, L_RYY, L_RZX, L_RZY, L_EZ,
        L_VX, L_VY, L_NC };
 // z-part of a cell plane (two buffers): rz eez rxz ryz rzz ey ex
@@ -270,8 +270,9 @@ CANS_DEV bool march_dt(const Mlw3dArgs &a, double &dt) {
 }
 
 // One block: the column tile `tile` over the z-segment [k0, k1) of updated
-// cells; the march starts two planes early (cell plane k0-1, whose corner
-// partials and own-plane parts the first updates need).
+// cells.  The march starts at cell plane k0-1: its first two steps form the
+// corner plane k0-1 and the own-plane parts of cell k0, which the first
+// update (at the step of plane k0+1) needs.
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void mlw3d_march(Mlw3dArgs a,
                                                                                                int tiles_x,
                                                                                                int ntiles,
@@ -310,7 +311,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     }
     // carried from step to step
     double dxy[8], avx[8], avy[8], bzl[8];       // cell s: parts from its own plane
-    double fz2[8], alo[8], d0[8], d1[8];         // Fz(s-1), lower corner-plane parts
+    double fz2[8], alo[8], d0[8], d1[8];         // own z fluxes of the plane below; lower corner-plane parts
     double vzp = 0.0;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
@@ -342,8 +343,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
             put_cell(L, t, par, cur);
         }
         __syncthreads();
-        // P2: corner s-1 from the partials of plane s-1 and plane s; then the
-        // partials of corner s
+        // P2: corner s-1 from its eight cells in planes s-1 and s (each sum
+        // over the lower plane first, continued over the upper one)
         if (kval) {
             const int t1 = t + 1, t2 = t + TB, t3 = t + TB + 1;   // cells 10, 01, 11 of the corner
             if (s > s0) {
