@@ -251,6 +251,66 @@ def test_run_3d_blast(dev, oracle, engine, margin):
         assert_bitwise(got[k], ref[k], k)
 
 
+@pytest.mark.parametrize("axis", ["x", "y", "z"])
+@pytest.mark.parametrize("margin", [2, 3])
+def test_run_3d_mlw_periodic_ghost_cfl(dev, oracle, margin, axis):
+    """Periodic 3D MLW run (the marching kernel advances the clock and takes
+    the next step's cfl_m) on stretched grids whose smallest spacing on one
+    axis lies in the two ghost cells at its ends: the reference's cfl_m takes
+    dt from a ghost image, which the kernel folds into its interior cell's
+    term.  The run is issued in two calls (the second call's first step takes
+    cfl_m in a pass of its own)."""
+    from utrip_amd.problems import Problem
+    case("run 3d mlw periodic ghost cfl, margin", margin, "axis", axis)
+    base = P.blast_3d(n=14, margin=margin, gaussian_b=True, r0=0.25)
+    ix, jx, kx = base.fields["ro"].shape
+    grid = dict(base.grid)
+    for nm, n, seed in (("x", ix, 21), ("y", jx, 22), ("z", kx, 23)):
+        _, d, dm = stretched_grid(n, seed=seed)
+        if nm == axis:
+            d[0], d[n - 1] = 0.3 * d.min(), 0.35 * d.min()
+        grid["d" + nm], grid["d" + nm + "m"] = d, dm
+    pb = Problem(base.name, base.fields, grid, base.params)
+    ref, dts = oracle_loop(oracle, 3, "mlw", pb.fields, grid, pb.params, 8, 7, margin)
+    got, info = run_gpu(dev, 3, "mlw", pb.fields, grid, pb.params, 8, 7, margin)
+    assert info["nexec"] == 8 and info["merr"] == 0 and info["dt"] == dts[-1]
+    for k in F3:
+        assert_bitwise(got[k], ref[k], k)
+
+
+def test_run_3d_mlw_periodic_stops(dev, oracle):
+    """The clock in the periodic 3D MLW kernel: the t < tend loop test ends
+    the run after the step that reaches tend (later steps are no-ops), and
+    dt < dtmin sets merr = 9001 with the state unchanged."""
+    from utrip_amd import cans
+    case("run 3d mlw periodic, tend and dtmin stops")
+    margin = 2
+    pb = P.blast_3d(n=14, margin=margin, gaussian_b=True, r0=0.25)
+    ref, dts = oracle_loop(oracle, 3, "mlw", pb.fields, pb.grid, pb.params, 4, 7, margin)
+    t = 0.0
+    for d in dts[:3]:
+        t = t + d
+    st = [to_dev(pb.fields[k], dev) for k in F3]
+    gd = {k: to_dev(v, dev) for k, v in pb.grid.items() if k in ("dx", "dxm", "dy", "dym", "dz", "dzm")}
+    pp = pb.params
+    run = cans.Run(3, cans.ENGINE_MLW, st, gd, pp["gm"], pp["safety"], qav=pp["qav"], margin=margin, bc=7)
+    run.reset(tend=t * (1 + 1e-12), dtmin=pp["dtmin"])   # reached after step 4
+    run.steps(3)
+    run.steps(5)
+    cur, info = run.state()
+    assert info["nexec"] == 4 and info["ns"] == 4 and info["merr"] == 0
+    for k, tt in zip(F3, cur):
+        assert_bitwise(to_host(tt), ref[k], k)
+    st = [to_dev(pb.fields[k], dev) for k in F3]
+    run = cans.Run(3, cans.ENGINE_MLW, st, gd, pp["gm"], pp["safety"], qav=pp["qav"], margin=margin, bc=7)
+    run.reset(dtmin=1.0)
+    run.steps(3)
+    cur, info = run.state()
+    assert info["merr"] == 9001 and info["nexec"] == 0 and info["ns"] == 1
+    for k, tt in zip(F3, cur):
+        assert_bitwise(to_host(tt), pb.fields[k], k)
+
+
 # Longer runs of the marching and fused kernels against the oracle: the 3D
 # MLW march on several column tiles and z-segments, the 3D Roe and HLLD with
 # the stage updates fused into their z sweeps (several segments per column

@@ -492,6 +492,14 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         if (dim == 3) HIPCHK(launch_metric(dz, dzm, kx, m[2], s));
     }
     double *eng = cv.p;
+    // 3D Cartesian MLW on the marching kernel in a periodic box (every field
+    // periodic on every face): the kernel advances the clock and takes the
+    // next step's cfl_m, so a step is the engine and the ghost fill.  The
+    // first step of a call still takes cfl_m of its state in a pass of its
+    // own (the same minimum again when the call before fused it).
+    bool per3 = dim == 3 && engine == CANS_ENGINE_MLW && mlw3d_path() != 0 && safety > 0.0 && margin >= 1;
+    for (int k = 0; per3 && k < 6 * nf; k++) per3 = bc[k] == BD_PER;
+    for (int d = 0; per3 && d < 3; d++) per3 = n[d] - 2 * margin >= margin;
     for (int st = step0; st < step0 + nsteps; st++) {
         double *const *in = (st & 1) ? b : a;
         double *const *out = (st & 1) ? a : b;
@@ -521,8 +529,8 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         ca.dx = dx; ca.dy = dy; ca.dz = dz; ca.safety = safety; ca.gm = gm;
         ca.xr = gx; ca.trig = dim == 3 ? trig : nullptr;   // cfl_m_s for the spherical engine
         ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st; ca.comp3 = c3 ? 1 : 0;
-        HIPCHK(launch_cfl(ca, s));
-        HIPCHK(launch_clock_step(c, st, s));
+        if (!per3 || st == step0) HIPCHK(launch_cfl(ca, s));
+        if (!per3) HIPCHK(launch_clock_step(c, st, s));
         const int e = st - step0;
         if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e], s));
         if (cyl) {
@@ -592,6 +600,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
                 g.corner = ec.take((size_t)kMlw3dCornerFields * N);
                 g.dxm = dxm; g.dym = dym; g.dzm = dzm; g.mx = m[0]; g.my = m[1]; g.mz = m[2];
                 g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx; g.kx = kx;
+                if (per3) { g.dx = dx; g.dy = dy; g.dz = dz; g.safety = safety; g.per_margin = margin; }
                 HIPCHK(launch_mlw3d(g, s));
             }
         } else if (engine == CANS_ENGINE_ROE || engine == CANS_ENGINE_ROE3) {

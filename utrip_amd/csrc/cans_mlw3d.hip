@@ -585,7 +585,9 @@ hipError_t launch_mlw3d(const Mlw3dArgs &a, hipStream_t s) {
     const size_t nu = (size_t)(a.ix - 2) * (a.jx - 2) * (a.kx - 2);
     if (a.x && !a.xm) return hipErrorInvalidValue;
     if (a.trig && !a.x) return hipErrorInvalidValue;
-    if (!a.x && !a.trig && mlw3d_path() == 1) return launch_mlw3d_march(a, s);
+    // (a periodic-box call, per_margin > 0, needs the marching kernel: it
+    // carries the step's clock and cfl_m)
+    if (!a.x && !a.trig && (a.per_margin > 0 || mlw3d_path() == 1)) return launch_mlw3d_march(a, s);
     const dim3 gc(xcd_grid(N)), gk(xcd_grid(nc)), gu(xcd_grid(nu));
     if (a.trig) {
         hipLaunchKernelGGL(mlw3d_cells<G_SPH>, gc, dim3(256), 0, s, a);

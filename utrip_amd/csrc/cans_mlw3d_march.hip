@@ -26,6 +26,12 @@
 // Each segment starts two planes before its first updated cell; the host
 // picks the number of z-segments so that the launch fills the CUs in the
 // fewest plane-steps (one block per CU: its LDS is 152 KB).
+// In a periodic box (per_margin = m > 0) the kernel advances the device
+// clock itself and takes the next step's cfl_m (cans3d/common/cfl_m.f:39-58)
+// over the cells it updates, as the 2D kernel does: the reference's bdperx,
+// bdpery, bdperz overwrite the ghost cells with images of interior cells, so
+// cfl_m over every cell of the new state is the minimum over the interior
+// cells, each with the smallest dx, dy, dz among its images.
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_roe.hpp"
@@ -35,6 +41,7 @@ namespace {
 
 constexpr double pi8i = 1.0 / kPi / 8.0;
 constexpr double pi4i = 1.0 / kPi / 4.0;
+constexpr double pi4c = 0.25 / kPiF;   // cfl_m's onep4 = 0.25/pai, pai the REAL*4 acos(-1.0)
 constexpr int TB = 16;          // tile edge in cells, halo included
 constexpr int TO = TB - 2;      // updated cells per tile edge
 constexpr int NT = TB * TB;     // threads per block
@@ -261,12 +268,23 @@ CANS_DEV void corner_fluxes(const double *un, double gm, CornV &h) {
     h.ez = ezh;
 }
 
+// Step n's run decision and dt: the fixed dt without a clock; in a periodic
+// box the clock is advanced here (block 0 writes it); otherwise a clock
+// kernel of the step has advanced it.
 CANS_DEV bool march_dt(const Mlw3dArgs &a, double &dt) {
     if (!a.clk) { dt = a.dt; return true; }
-    const int d = (a.step + 1) & 1;
-    if (a.clk->stopped[d]) return false;
-    dt = a.clk->dt_last[d];
-    return true;
+    const StepCtl c = clock_use(a.clk, a.step, a.per_margin == 0, blockIdx.x == 0 && threadIdx.x == 0);
+    dt = c.dt;
+    return c.run;
+}
+
+// smallest of dx(i) and dx of i's periodic images (1-based i, margin m,
+// n cells; each ghost cell holds the image i +- (n - 2m) of an interior cell)
+CANS_DEV double image_min(const double *d, int i, int n, int m) {
+    double g = d[i - 1];
+    if (i >= n - 2 * m + 1) g = fminF(g, d[i - (n - 2 * m) - 1]);
+    if (i <= 2 * m) g = fminF(g, d[i + (n - 2 * m) - 1]);
+    return g;
 }
 
 // One block: the column tile `tile` over the z-segment [k0, k1) of updated
@@ -282,8 +300,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     const int item = (int)((blockIdx.x % kXcds) * per + blockIdx.x / kXcds);
     if (item >= ntiles * nseg) return;
     double dt;
-    if (!march_dt(a, dt)) return;
+    if (!march_dt(a, dt)) return;   // uniform across the launch
     __shared__ Lds L;
+    __shared__ double red[NT / 64];
     const int ix = a.ix, jx = a.jx, kx = a.kx;
     const int tile = item % ntiles;
     const int k0 = 1 + (item / ntiles) * seglen;
@@ -309,6 +328,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         ux0 = a.mx.u0[gx]; ux1 = a.mx.u1[gx]; uy0 = a.my.u0[gy]; uy1 = a.my.u1[gy];
         qx0 = a.dxm[gx]; qx1 = a.dxm[gx - 1]; qy0 = a.dym[gy]; qy1 = a.dym[gy - 1];
     }
+    // the next step's cfl_m: interior cells [m+1, n-m] (1-based) only
+    const int mg = a.per_margin;
+    const bool do_cfl = a.clk && mg > 0 && a.safety > 0.0;
+    const bool cxy = do_cfl && uval && gx >= mg && gx < ix - mg && gy >= mg && gy < jx - mg;
+    double dxyg = 0.0, dtmin_local = kR4_1e20;
+    if (cxy) dxyg = fminF(image_min(a.dx, gx + 1, ix, mg), image_min(a.dy, gy + 1, jx, mg));
     // carried from step to step
     double dxy[8], avx[8], avy[8], bzl[8];       // cell s: parts from its own plane
     double fz2[8], alo[8], d0[8], d1[8];         // own z fluxes of the plane below; lower corner-plane parts
@@ -417,8 +442,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
                     const double vz = U[4] / ro;
                     const double vv = vx * vx + vy * vy + vz * vz;
                     const double bb = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+                    const double pr = (gm - 1.0) * (U[1] - 0.5 * ro * vv - pi8i * bb);
+                    // cfl_m.f:45-54 on the stored values (b2 and v2 are bb and
+                    // vv, the same sums); the ghost images' terms through the
+                    // image minima: m -> safety*(m/sq) is monotone in IEEE
+                    // arithmetic, so the smallest term is this one bit for bit
+                    if (cxy && s >= mg + 1 && s <= kx - mg) {
+                        const double dg = fminF(dxyg, image_min(a.dz, s, kx, mg));
+                        const double ca2 = bb * pi4c / ro;
+                        const double cs2 = gm * pr / ro;
+                        const double dtq = a.safety * (dg / __builtin_sqrt(vv + cs2 + ca2));
+                        if (dtq < dtmin_local) dtmin_local = dtq;   // NaN never wins
+                    }
                     a.out[0][pk] = ro;
-                    a.out[1][pk] = (gm - 1.0) * (U[1] - 0.5 * ro * vv - pi8i * bb);
+                    a.out[1][pk] = pr;
                     a.out[2][pk] = vx;
                     a.out[3][pk] = vy;
                     a.out[4][pk] = vz;
@@ -453,6 +490,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
             vzp = vzs;
         }
     }
+    if (do_cfl) block_min_to_global(dtmin_local, red, &a.clk->dtkey[(a.step + 1) % 3]);
 }
 
 }  // namespace

@@ -46,6 +46,13 @@ struct Mlw3dArgs {
     Clock *clk;
     int step;
     int ix, jx, kx;
+    // Cartesian marching kernel in a periodic box (per_margin = m > 0): it
+    // advances the clock itself (clock_enter; no separate clock kernel for
+    // the step) and takes the next step's cfl_m over the new state, the
+    // ghost images included through the image-minimum dx, dy, dz
+    const double *dx, *dy, *dz;
+    double safety;
+    int per_margin;
 };
 hipError_t launch_mlw3d(const Mlw3dArgs &a, hipStream_t s);
 constexpr int kMlw3dCellFields = 23;
