@@ -168,15 +168,24 @@ def test_adjoint_every_state_kept_equals_checkpoints(dev):
     receives at most two of them (17 blocks), and two additions to zero
     commute exactly, so the result does not depend on their order.  At larger
     sizes the order varies from run to run in the last bits.)"""
-    case("OT 64^2, 30 steps, every state kept vs checkpoints every 7")
-    grads = []
-    for ck in (0, 7):
+    case("OT 64^2, 30 steps, every state kept vs checkpoints every 7, 10 and 40")
+    grads = {}
+    for ck in (0, 7, 10, 40):
         solver, base, vx0, vy0, beta, target, w = setup(64, 30, ck, dev)
         x = [vx0.clone().requires_grad_(True), vy0.clone().requires_grad_(True), beta.clone().requires_grad_(True)]
-        loss(*x, solver, base, target, w).backward()
-        grads.append([t.grad.detach().cpu() for t in x])
-    for name, a, b in zip(("vx0", "vy0", "beta"), *grads):
-        assert torch.equal(a, b), name
+        L = loss(*x, solver, base, target, w)
+        L.backward(retain_graph=True)
+        grads[ck] = [t.grad.detach().cpu().clone() for t in x]
+        if ck == 7:
+            # a second backward of the same forward recomputes the last
+            # segment (the first one used the states the forward recorded)
+            for t in x:
+                t.grad = None
+            L.backward()
+            grads["7 again"] = [t.grad.detach().cpu() for t in x]
+    for ck in (7, 10, 40, "7 again"):
+        for name, a, b in zip(("vx0", "vy0", "beta"), grads[0], grads[ck]):
+            assert torch.equal(a, b), (ck, name)
 
 
 @pytest.mark.parametrize("ckpt", [5, 0])

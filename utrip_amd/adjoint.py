@@ -9,7 +9,9 @@ With ``ckpt = 0`` it keeps every state of the trajectory in device memory
 (7 fields x (nsteps+1) states: 2.9 GB for 200 steps at 512^2) and its
 ``backward`` is one reverse sweep; with ``ckpt > 0`` it keeps a checkpoint
 every ``ckpt`` steps and the backward recomputes each segment from its
-checkpoint (on a second stream) before sweeping it.
+checkpoint (on a second stream) before sweeping it, except the last segment,
+whose states the forward records as it runs it (the backward sweeps that
+segment first, so its recomputation would have nothing to overlap with).
 ``FinalDensity`` wraps this as a ``torch.autograd.Function`` of the initial
 velocities (interior) and a uniform pressure scale ``beta``.
 """
@@ -80,24 +82,70 @@ class PeriodicMlwAdjoint:
             raise cans.CansError(f"forward run stopped early: {info}")
         return [t.clone() for t in self._all[self.nsteps]]
 
+    def _segment_buffers(self, like):
+        """Two segment buffers of ckpt+1 states each, their clocks, dt logs
+        and metric workspaces, and the second stream (allocated once)."""
+        if getattr(self, "_seg", None) is None:
+            dev = self.ws.device
+            self._seg = [[[torch.empty_like(t) for t in like] for _ in range(self.ckpt + 1)] for _ in range(2)]
+            self._clocks = [cans.Clock(), cans.Clock()]
+            self._dts = [torch.empty(self.ckpt, dtype=torch.float64, device=dev) for _ in range(2)]
+            self._mws = [torch.empty_like(self.metric) for _ in range(2)]
+            self._rec_stream = torch.cuda.Stream(device=dev)
+            self._seg_p = [None, None]
+            self._last_recorded = False
+
+    def _record_segment(self, c, b):
+        """States of segment c (from its checkpoint, min(ckpt, steps left)
+        steps) and their dt into segment buffer b, on the current stream
+        (cans2d_mlw_record).  Returns the step count and the state pointers."""
+        dx, dxm, dy, dym = self.grid
+        n = min(self.ckpt, self.nsteps - c * self.ckpt)
+        for dst, src in zip(self._seg[b][0], self.ckpts[c]):
+            dst.copy_(src)
+        states = _ptr_array([t for k in range(n + 1) for t in self._seg[b][k]], (self.jx, self.ix))
+        self._seg_p[b] = states
+        self._clocks[b].reset(dtmin=self.dtmin)
+        _check(lib().cans2d_mlw_record(states, C.c_int(n), self._clocks[b].handle, _d(self.qav), _d(self.gm),
+                                       _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy),
+                                       _ptr(dym), C.c_int(self.jx), C.c_int(self.margin),
+                                       _ptr(self._mws[b]), _ptr(self._dts[b]), _stream()))
+        return n, states
+
     def forward(self, state):
         """Run nsteps from ``state`` (7 tensors ro,pr,vx,vy,bx,by,az, ghosts
         filled); keep every state (ckpt = 0) or the checkpoints; return the
-        final state (clones)."""
+        final state (clones).  With checkpoints, the segments up to the last
+        run on the resident fused kernel and the last one is recorded into
+        segment buffer 0 (the same kernels as the backward's recomputation,
+        so the same states bit for bit)."""
         if self.ckpt == 0:
             return self._forward_all(state)
         self.ckpts = []
         run = self._run(state)
+        self._segment_buffers(state)
         done = 0
-        while done < self.nsteps:
+        while True:
             self.ckpts.append([t.clone() for t in self._cur(run, done)])
             k = min(self.ckpt, self.nsteps - done)
+            if done + k >= self.nsteps:
+                break
             run.steps(k)
             done += k
-        info = run.clock.read()
-        if info["merr"] != 0 or info["nexec"] != self.nsteps:
-            raise cans.CansError(f"forward run stopped early: {info}")
-        return [t.clone() for t in self._cur(run, done)]
+        if done > 0:
+            info = run.clock.read()
+            if info["merr"] != 0 or info["nexec"] != done:
+                raise cans.CansError(f"forward run stopped early: {info}")
+        self._last_recorded = False
+        n, _ = self._record_segment(len(self.ckpts) - 1, 0)
+        self._clocks[0].issued = n
+        info = self._clocks[0].read()
+        if info["merr"] != 0 or info["nexec"] != n:
+            raise cans.CansError(f"forward run stopped early (last segment): {info}")
+        self._last_ev = torch.cuda.Event()
+        self._last_ev.record(torch.cuda.current_stream(self.ws.device))
+        self._last_recorded = True
+        return [t.clone() for t in self._seg[0][n]]
 
     def backward(self, lam):
         """``lam``: dL/dP of the final state on every cell (6 tensors
@@ -109,7 +157,10 @@ class PeriodicMlwAdjoint:
         checkpoint into one of two segment buffers (cans2d_mlw_record, on a
         second stream) while the reverse sweep of the previous segment runs
         on the caller's stream (cans2d_mlw_adjoint_run); events order a
-        buffer's record before its sweep and its sweep before its reuse."""
+        buffer's record before its sweep and its sweep before its reuse.
+        The last segment's states are those the forward recorded into
+        buffer 0, the first time the backward follows that forward (a
+        second backward finds buffer 0 reused and recomputes it)."""
         dx, dxm, dy, dym = self.grid
         shape = (self.jx, self.ix)
         lam = [t.clone().contiguous() for t in lam]
@@ -124,13 +175,7 @@ class PeriodicMlwAdjoint:
             _check(lib().cans2d_periodic_fold(lam_p, C.c_int(self.ix), C.c_int(self.jx), C.c_int(self.margin),
                                               _stream()))
             return lam
-        if getattr(self, "_seg", None) is None:
-            self._seg = [[[torch.empty_like(t) for t in self.ckpts[0]] for _ in range(self.ckpt + 1)]
-                         for _ in range(2)]
-            self._clocks = [cans.Clock(), cans.Clock()]
-            self._dts = [torch.empty(self.ckpt, dtype=torch.float64, device=dev) for _ in range(2)]
-            self._mws = [torch.empty_like(self.metric) for _ in range(2)]
-            self._rec_stream = torch.cuda.Stream(device=dev)
+        self._segment_buffers(self.ckpts[0])
         main = torch.cuda.current_stream(dev)
         rec = self._rec_stream
         ev_rec = [torch.cuda.Event(), torch.cuda.Event()]
@@ -140,24 +185,21 @@ class PeriodicMlwAdjoint:
         order = list(reversed(range(len(self.ckpts))))
 
         def record(c, b):
-            n = min(self.ckpt, self.nsteps - c * self.ckpt)
             if used[b]:
                 rec.wait_event(ev_sweep[b])
             else:
                 rec.wait_stream(main)       # checkpoints and buffers were written on the caller's stream
             with torch.cuda.stream(rec):
-                for dst, src in zip(self._seg[b][0], self.ckpts[c]):
-                    dst.copy_(src)
-                states = _ptr_array([t for k in range(n + 1) for t in self._seg[b][k]], shape)
-                self._clocks[b].reset(dtmin=self.dtmin)
-                _check(lib().cans2d_mlw_record(states, C.c_int(n), self._clocks[b].handle, _d(self.qav), _d(self.gm),
-                                               _d(self.safety), _ptr(dx), _ptr(dxm), C.c_int(self.ix), _ptr(dy),
-                                               _ptr(dym), C.c_int(self.jx), C.c_int(self.margin),
-                                               _ptr(self._mws[b]), _ptr(self._dts[b]), _stream()))
+                n, states = self._record_segment(c, b)
                 ev_rec[b].record(rec)
             return n, states
 
-        pending = record(order[0], 0)
+        if self._last_recorded:   # written by the forward; its event orders the sweep after it
+            pending = (min(self.ckpt, self.nsteps - order[0] * self.ckpt), self._seg_p[0])
+            ev_rec[0] = self._last_ev
+            self._last_recorded = False
+        else:
+            pending = record(order[0], 0)
         for idx, c in enumerate(order):
             b = idx % 2
             n, states = pending
