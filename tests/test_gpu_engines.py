@@ -252,17 +252,19 @@ def test_run_3d_blast(dev, oracle, engine, margin):
 
 
 @pytest.mark.parametrize("axis", ["x", "y", "z"])
-@pytest.mark.parametrize("margin", [2, 3])
-def test_run_3d_mlw_periodic_ghost_cfl(dev, oracle, margin, axis):
-    """Periodic 3D MLW run (the marching kernel advances the clock and takes
-    the next step's cfl_m) on stretched grids whose smallest spacing on one
-    axis lies in the two ghost cells at its ends: the reference's cfl_m takes
-    dt from a ghost image, which the kernel folds into its interior cell's
-    term.  The run is issued in two calls (the second call's first step takes
-    cfl_m in a pass of its own)."""
+@pytest.mark.parametrize("engine,margin", [("mlw", 2), ("mlw", 3), ("roe", 3), ("hlld", 3)])
+def test_run_3d_periodic_ghost_cfl(dev, oracle, engine, margin, axis):
+    """Periodic 3D runs whose engine advances the clock and takes the next
+    step's cfl_m (the MLW marching kernel; the Roe step with its fused z
+    marches, forced here by a low wave target; HLLD) on stretched grids whose
+    smallest spacing on one axis lies in the two ghost cells at its ends: the
+    reference's cfl_m takes dt from a ghost image, which the kernels fold into
+    its interior cell's term.  The run is issued in two calls (the second
+    call's first step takes cfl_m in a pass of its own)."""
+    from utrip_amd import cans
     from utrip_amd.problems import Problem
-    case("run 3d mlw periodic ghost cfl, margin", margin, "axis", axis)
-    base = P.blast_3d(n=14, margin=margin, gaussian_b=True, r0=0.25)
+    case("run 3d periodic ghost cfl", engine, "margin", margin, "axis", axis)
+    base = P.blast_3d(n=14, margin=margin, gaussian_b=(engine != "hlld"), r0=0.25)
     ix, jx, kx = base.fields["ro"].shape
     grid = dict(base.grid)
     for nm, n, seed in (("x", ix, 21), ("y", jx, 22), ("z", kx, 23)):
@@ -271,41 +273,71 @@ def test_run_3d_mlw_periodic_ghost_cfl(dev, oracle, margin, axis):
             d[0], d[n - 1] = 0.3 * d.min(), 0.35 * d.min()
         grid["d" + nm], grid["d" + nm + "m"] = d, dm
     pb = Problem(base.name, base.fields, grid, base.params)
-    ref, dts = oracle_loop(oracle, 3, "mlw", pb.fields, grid, pb.params, 8, 7, margin)
-    got, info = run_gpu(dev, 3, "mlw", pb.fields, grid, pb.params, 8, 7, margin)
+    ref, dts = oracle_loop(oracle, 3, engine, pb.fields, grid, pb.params, 8, 7, margin)
+    cans.set_march_waves(8 if engine == "roe" else 32768)
+    try:
+        got, info = run_gpu(dev, 3, engine, pb.fields, grid, pb.params, 8, 7, margin)
+    finally:
+        cans.set_march_waves(32768)
     assert info["nexec"] == 8 and info["merr"] == 0 and info["dt"] == dts[-1]
     for k in F3:
         assert_bitwise(got[k], ref[k], k)
+    if engine == "roe":
+        # the Roe step takes the fused cfl_m with FP32 fluxes: the fused z
+        # marches (low wave target) against the separate passes and cfl pass
+        cans.set_march_waves(8)
+        try:
+            got32, info32 = run_gpu(dev, 3, engine, pb.fields, grid, pb.params, 8, 7, margin, mixed=True)
+        finally:
+            cans.set_march_waves(32768)
+        ref32, info_ref32 = run_gpu(dev, 3, engine, pb.fields, grid, pb.params, 8, 7, margin, mixed=True)
+        assert info32["nexec"] == 8 and info32["dt"] == info_ref32["dt"]
+        for k in F3:
+            assert_bitwise(got32[k], ref32[k], "fp32-flux " + k)
 
 
-def test_run_3d_mlw_periodic_stops(dev, oracle):
-    """The clock in the periodic 3D MLW kernel: the t < tend loop test ends
-    the run after the step that reaches tend (later steps are no-ops), and
-    dt < dtmin sets merr = 9001 with the state unchanged."""
+@pytest.mark.parametrize("engine,margin", [("mlw", 2), ("roe", 3), ("hlld", 3)])
+def test_run_3d_periodic_stops(dev, oracle, engine, margin):
+    """The clock carried by the periodic 3D engines' kernels: the t < tend
+    loop test ends the run after the step that reaches tend (later steps are
+    no-ops), and dt < dtmin sets merr = 9001 with the state unchanged."""
     from utrip_amd import cans
-    case("run 3d mlw periodic, tend and dtmin stops")
-    margin = 2
-    pb = P.blast_3d(n=14, margin=margin, gaussian_b=True, r0=0.25)
-    ref, dts = oracle_loop(oracle, 3, "mlw", pb.fields, pb.grid, pb.params, 4, 7, margin)
-    t = 0.0
-    for d in dts[:3]:
-        t = t + d
+    case("run 3d periodic, tend and dtmin stops", engine)
+    pb = P.blast_3d(n=14, margin=margin, gaussian_b=(engine != "hlld"), r0=0.25)
+    pp = pb.params
+    mixed = engine == "roe"   # Roe carries the clock with FP32 fluxes (on its fused z marches)
+    if mixed:
+        # the reference: the same FP32-flux run on the separate passes and cfl pass
+        _, info3 = run_gpu(dev, 3, engine, pb.fields, pb.grid, pp, 3, 7, margin, mixed=True)
+        t = info3["t"]
+        ref, _ = run_gpu(dev, 3, engine, pb.fields, pb.grid, pp, 4, 7, margin, mixed=True)
+    else:
+        ref, dts = oracle_loop(oracle, 3, engine, pb.fields, pb.grid, pp, 4, 7, margin)
+        t = 0.0
+        for d in dts[:3]:
+            t = t + d
     st = [to_dev(pb.fields[k], dev) for k in F3]
     gd = {k: to_dev(v, dev) for k, v in pb.grid.items() if k in ("dx", "dxm", "dy", "dym", "dz", "dzm")}
-    pp = pb.params
-    run = cans.Run(3, cans.ENGINE_MLW, st, gd, pp["gm"], pp["safety"], qav=pp["qav"], margin=margin, bc=7)
-    run.reset(tend=t * (1 + 1e-12), dtmin=pp["dtmin"])   # reached after step 4
-    run.steps(3)
-    run.steps(5)
-    cur, info = run.state()
-    assert info["nexec"] == 4 and info["ns"] == 4 and info["merr"] == 0
-    for k, tt in zip(F3, cur):
-        assert_bitwise(to_host(tt), ref[k], k)
-    st = [to_dev(pb.fields[k], dev) for k in F3]
-    run = cans.Run(3, cans.ENGINE_MLW, st, gd, pp["gm"], pp["safety"], qav=pp["qav"], margin=margin, bc=7)
-    run.reset(dtmin=1.0)
-    run.steps(3)
-    cur, info = run.state()
+    eng = {"mlw": cans.ENGINE_MLW, "roe": cans.ENGINE_ROE, "hlld": cans.ENGINE_HLLD}[engine]
+    cans.set_march_waves(8 if engine == "roe" else 32768)
+    try:
+        run = cans.Run(3, eng, st, gd, pp["gm"], pp["safety"], qav=pp.get("qav", 3.0), margin=margin, bc=7,
+                       mixed=mixed)
+        run.reset(tend=t * (1 + 1e-12), dtmin=pp["dtmin"])   # reached after step 4
+        run.steps(3)
+        run.steps(5)
+        cur, info = run.state()
+        assert info["nexec"] == 4 and info["ns"] == 4 and info["merr"] == 0
+        for k, tt in zip(F3, cur):
+            assert_bitwise(to_host(tt), ref[k], k)
+        st = [to_dev(pb.fields[k], dev) for k in F3]
+        run = cans.Run(3, eng, st, gd, pp["gm"], pp["safety"], qav=pp.get("qav", 3.0), margin=margin, bc=7,
+                       mixed=mixed)
+        run.reset(dtmin=1.0)
+        run.steps(3)
+        cur, info = run.state()
+    finally:
+        cans.set_march_waves(32768)
     assert info["merr"] == 9001 and info["nexec"] == 0 and info["ns"] == 1
     for k, tt in zip(F3, cur):
         assert_bitwise(to_host(tt), pb.fields[k], k)

@@ -492,12 +492,15 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         if (dim == 3) HIPCHK(launch_metric(dz, dzm, kx, m[2], s));
     }
     double *eng = cv.p;
-    // 3D Cartesian MLW on the marching kernel in a periodic box (every field
-    // periodic on every face): the kernel advances the clock and takes the
-    // next step's cfl_m, so a step is the engine and the ghost fill.  The
-    // first step of a call still takes cfl_m of its state in a pass of its
-    // own (the same minimum again when the call before fused it).
-    bool per3 = dim == 3 && engine == CANS_ENGINE_MLW && mlw3d_path() != 0 && safety > 0.0 && margin >= 1;
+    // 3D in a periodic box (every field periodic on every face) on the
+    // Cartesian MLW marching kernel, the FP32-flux Roe step with its fused z
+    // marches or HLLD: the engine's kernels advance the clock and take the next step's
+    // cfl_m, so a step is the engine and the ghost fill.  The first step of a
+    // call still takes cfl_m of its state in a pass of its own (the same
+    // minimum again when the call before fused it).
+    bool per3 = dim == 3 && safety > 0.0 && margin >= 1 &&
+                ((engine == CANS_ENGINE_MLW && mlw3d_path() != 0) ||
+                 (engine == CANS_ENGINE_ROE && mixed && roe3d_fused_z(ix, jx, kx)) || engine == CANS_ENGINE_HLLD);
     for (int k = 0; per3 && k < 6 * nf; k++) per3 = bc[k] == BD_PER;
     for (int d = 0; per3 && d < 3; d++) per3 = n[d] - 2 * margin >= margin;
     for (int st = step0; st < step0 + nsteps; st++) {
@@ -613,6 +616,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
             r.out_az = dim == 2 ? out[faz] : nullptr;
             r.bx = bx1d; r.bxm = bxm1d; r.dx = dx; r.dy = dy; r.dz = dz;
             r.gm = gm; r.clk = c; r.step = st; r.ix = ix; r.jx = jx; r.kx = kx;
+            if (per3) { r.safety = safety; r.per_margin = margin; }
             HIPCHK(launch_roe(r, dim, mixed != 0, s, c3));
         } else {
             HlldArgs h{};
@@ -622,6 +626,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
             h.dx = dx; h.dy = dy; h.dz = dz; h.gm = gm; h.mu = mu;
             h.mfdim[0] = h.mfdim[1] = h.mfdim[2] = 1;
             h.clk = c; h.step = st; h.ix = ix; h.jx = jx; h.kx = kx;
+            if (per3) { h.safety = safety; h.per_margin = margin; }
             HIPCHK(launch_hlld(h, mixed != 0, s));
         }
         if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e + 1], s));

@@ -49,4 +49,35 @@ CANS_DEV bool clock_live(const Clock *c, int n) {
     return !c->stopped[s] && c->ns[s] < c->nstop && c->t[s] < c->tend;
 }
 
+// ---- the next step's cfl_m fused into the kernel that writes a new state
+// In a periodic box the ghost fill that follows the engine (bdperx, bdpery,
+// bdperz, in that order) makes every ghost cell a copy of an interior cell,
+// its image (i +- (n - 2m) along each axis with a ghost index).  cfl_m over
+// every cell of the new state is then the minimum over the interior cells
+// [m+1, n-m] of the terms safety*(d/sq), sq from the cell's state and d the
+// smallest spacing among the cell's images: m -> safety*(m/sq) is monotone
+// in IEEE arithmetic for sq >= 0 (a NaN sq gives NaN terms, which never win
+// the reference's `if (dtq < dt)` either way), so that term is bit for bit
+// the smallest of the images' terms.
+
+// smallest of d(i) and d at the periodic images of cell i (1-based; n cells,
+// margin m, n - 2m >= m so that a cell has at most one image on each side)
+CANS_DEV double image_min(const double *d, int i, int n, int m) {
+    double g = d[i - 1];
+    if (i >= n - 2 * m + 1) g = fminF(g, d[i - (n - 2 * m) - 1]);
+    if (i <= 2 * m) g = fminF(g, d[i + (n - 2 * m) - 1]);
+    return g;
+}
+
+// cans3d/common/cfl_m.f:45-54 for one cell of the new state: ro, pr the
+// stored values, v2 = vx*vx + vy*vy + vz*vz and b2 = bx*bx + by*by + bz*bz
+// summed in that order, dg = min(dx, dy, dz) over the cell's images;
+// onep4 = 0.25/pai with pai the REAL*4 acos(-1.0)
+CANS_DEV double cfl3_term(double safety, double gm, double ro, double pr, double v2, double b2, double dg) {
+    constexpr double onep4 = 0.25 / kPiF;
+    const double ca2 = b2 * onep4 / ro;
+    const double cs2 = gm * pr / ro;
+    return safety * (dg / __builtin_sqrt(v2 + cs2 + ca2));
+}
+
 }  // namespace cans

@@ -27,12 +27,13 @@ namespace {
 enum { P_RO, P_PR, P_VX, P_VY, P_VZ, P_BX, P_BY, P_BZ };
 enum { U_RO, U_EE, U_RX, U_RY, U_RZ, U_BX, U_BY, U_BZ };
 
-CANS_DEV bool hstep(const HlldArgs &a, double &dt) {
+// Run flag / dt of the step: the clock as the step's clock kernel (or, with
+// `enter`, the step's first kernel here, block 0 writing it) advanced it.
+CANS_DEV bool hstep(const HlldArgs &a, double &dt, bool enter = false) {
     if (!a.clk) { dt = a.dt; return true; }
-    const int d = (a.step + 1) & 1;
-    if (a.clk->stopped[d]) return false;
-    dt = a.clk->dt_last[d];
-    return true;
+    const StepCtl c = clock_use(a.clk, a.step, !enter, blockIdx.x == 0 && threadIdx.x == 0);
+    dt = c.dt;
+    return c.run;
 }
 
 // hdhlld/hlld_flux.f90:51-280 for one face; o = fro, fee, frx, fry, frz, fby, fbz
@@ -243,24 +244,32 @@ CANS_DEV void hlld_face_flux(const HlldArgs &a, const double *const *P, int dir,
     hlld_face_w<T>(a, dir, muscl, d0, d1, d2, d3, F);
 }
 
-// primitives of one cell from its conserved state (rvtovv.f, ee2eh.f, eh2pr.f)
-CANS_DEV void hlld_prim_of(const HlldArgs &a, const double *U, double *const *P, size_t p) {
+// primitives of one cell from its conserved state (rvtovv.f, ee2eh.f, eh2pr.f);
+// with `cfl`, returns the next step's cfl_m term of the cell (dg: the
+// smallest spacing among its periodic images; the sums v2 and b2 are those
+// of cfl_m), else 1.e20
+CANS_DEV double hlld_prim_of(const HlldArgs &a, const double *U, double *const *P, size_t p, bool cfl = false,
+                             double dg = 0.0) {
     const double ro = U[U_RO];
     const double vx = U[U_RX] / ro;
     const double vy = U[U_RY] / ro;
     const double vz = U[U_RZ] / ro;
     const double bx = U[U_BX], by = U[U_BY], bz = U[U_BZ];
-    const double ek = (vx * vx + vy * vy + vz * vz) * ro / 2.0;
-    const double em = (bx * bx + by * by + bz * bz) / (2.0 * a.mu);
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    const double b2 = bx * bx + by * by + bz * bz;
+    const double ek = v2 * ro / 2.0;
+    const double em = b2 / (2.0 * a.mu);
     const double eh = U[U_EE] - ek - em;
+    const double pr = (a.gm - 1.0) * eh;
     P[P_RO][p] = ro;
-    P[P_PR][p] = (a.gm - 1.0) * eh;
+    P[P_PR][p] = pr;
     P[P_VX][p] = vx;
     P[P_VY][p] = vy;
     P[P_VZ][p] = vz;
     P[P_BX][p] = bx;
     P[P_BY][p] = by;
     P[P_BZ][p] = bz;
+    return cfl ? cfl3_term(a.safety, a.gm, ro, pr, v2, b2, dg) : kR4_1e20;
 }
 
 // adddfdx.f for one cell: u <- u - dts*((F_hi - F_lo)/dd)
@@ -278,7 +287,8 @@ template <class T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 4 : 1))) void hlld_sweep_x(HlldArgs a, int stage,
                                                                                                           int cons) {
     double dt;
-    if (!hstep(a, dt)) return;
+    // in a periodic run the stage-1 x sweep is the step's first kernel
+    if (!hstep(a, dt, a.per_margin > 0 && stage == 1)) return;
     const double dts = stage == 1 ? 0.5 * dt : dt;   // engine.f: dth = 0.5d0*dt
     const int ix = a.ix, jx = a.jx, kx = a.kx;
     const int lane = threadIdx.x & 63;
@@ -332,6 +342,7 @@ template <class T, int DIR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 1))) void hlld_sweep_yz(HlldArgs a, int stage, int H,
                                                                                                            int to_prim) {
     constexpr int dir = DIR;
+    __shared__ double red[4];
     double dt;
     if (!hstep(a, dt)) return;
     const double dts = stage == 1 ? 0.5 * dt : dt;
@@ -339,97 +350,114 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     const int nd = dir == 1 ? jx : kx, no = dir == 1 ? kx : jx;
     const int nseg = (nd - 2 + H - 1) / H;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)ix * no * nseg) return;
-    const int i = 1 + (int)(t % ix);
-    const int o = 1 + (int)((t / ix) % no);
-    const int seg = (int)(t / ((size_t)ix * no));
-    const int cb = 2 + seg * H, ce = min(cb + H - 1, nd - 1);
-    const size_t s = dir == 1 ? (size_t)ix : (size_t)ix * jx;
-    const size_t base = (size_t)(i - 1) + (dir == 1 ? (size_t)ix * jx * (size_t)(o - 1) : (size_t)ix * (size_t)(o - 1));
-    const double *P[8];
-    double *u[8];
+    // the next step's cfl_m in a periodic run (the stage-2 z sweep, which
+    // recovers the new primitives): every thread of the block reaches the
+    // block minimum at the end
+    const int mg = a.per_margin;
+    const bool do_cfl = DIR == 2 && stage == 2 && to_prim && a.clk && mg > 0 && a.safety > 0.0;
+    double dtmin_local = kR4_1e20;
+    if (t < (size_t)ix * no * nseg) {
+        const int i = 1 + (int)(t % ix);
+        const int o = 1 + (int)((t / ix) % no);
+        const int seg = (int)(t / ((size_t)ix * no));
+        const int cb = 2 + seg * H, ce = min(cb + H - 1, nd - 1);
+        const size_t s = dir == 1 ? (size_t)ix : (size_t)ix * jx;
+        const size_t base = (size_t)(i - 1) + (dir == 1 ? (size_t)ix * jx * (size_t)(o - 1) : (size_t)ix * (size_t)(o - 1));
+        const double *P[8];
+        double *u[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) { P[q] = stage == 1 ? a.prim[q] : a.half[q]; u[q] = stage == 1 ? a.u1[q] : a.u2[q]; }
-    const double *dd = dir == 1 ? a.dy : a.dz;
-    // cells c-1..c+2 of face c in a sliding window: one new cell per face
-    // (cell indices clamped to [1, nd]; a clamped cell only feeds a face
-    // whose states are piecewise constant, which does not read it)
-    double w0[8], w1[8], w2[8], w3[8];
-    auto load_cell = [&](int cell, double *d) {
-        const int cc = cell < 1 ? 1 : (cell > nd ? nd : cell);
-        const size_t pc = base + s * (size_t)(cc - 1);
+        for (int q = 0; q < 8; q++) { P[q] = stage == 1 ? a.prim[q] : a.half[q]; u[q] = stage == 1 ? a.u1[q] : a.u2[q]; }
+        const double *dd = dir == 1 ? a.dy : a.dz;
+        // interior column [m+1, n-m]^2 of the cfl_m (periodic images: image_min)
+        const bool cij = do_cfl && i >= mg + 1 && i <= ix - mg && o >= mg + 1 && o <= jx - mg;
+        const double dxyg = cij ? fminF(image_min(a.dx, i, ix, mg), image_min(a.dy, o, jx, mg)) : 0.0;
+        // cells c-1..c+2 of face c in a sliding window: one new cell per face
+        // (cell indices clamped to [1, nd]; a clamped cell only feeds a face
+        // whose states are piecewise constant, which does not read it)
+        double w0[8], w1[8], w2[8], w3[8];
+        auto load_cell = [&](int cell, double *d) {
+            const int cc = cell < 1 ? 1 : (cell > nd ? nd : cell);
+            const size_t pc = base + s * (size_t)(cc - 1);
 #pragma unroll
-        for (int q = 0; q < 8; q++) d[q] = P[q][pc];
-    };
-    // the next cell of the window is loaded one face ahead, so that its
-    // loads are in flight while the current face is computed
-    double wn[8];
-    load_cell(cb - 2, w0); load_cell(cb - 1, w1); load_cell(cb, w2); load_cell(cb + 1, w3);
-    load_cell(cb + 2, wn);
-    double Fl[8], Fh[8];
-    hlld_face_w<T>(a, dir, stage == 2 && cb - 1 >= 2 && cb - 1 <= nd - 2, w0, w1, w2, w3, Fl);
-    // With FP32 fluxes (registers to spare) the conserved state of the next
-    // cell is also loaded one face ahead (384^3 FP32-flux step 17.3 ->
-    // 15.8 ms); with FP64 fluxes the extra registers spill (18.9 -> 29.7 ms).
-    constexpr bool kPreU = sizeof(T) == 4;
-    double un[8];
-    if (kPreU) {
-#pragma unroll
-        for (int q = 0; q < 8; q++) un[q] = u[q][base + s * (size_t)(cb - 1)];
-    }
-    for (int c = cb; c <= ce; c++) {
-#pragma unroll
-        for (int q = 0; q < 8; q++) { w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; w3[q] = wn[q]; }
-        if (c < ce) load_cell(c + 3, wn);
-        const size_t p = base + s * (size_t)(c - 1);
+            for (int q = 0; q < 8; q++) d[q] = P[q][pc];
+        };
+        // the next cell of the window is loaded one face ahead, so that its
+        // loads are in flight while the current face is computed
+        double wn[8];
+        load_cell(cb - 2, w0); load_cell(cb - 1, w1); load_cell(cb, w2); load_cell(cb + 1, w3);
+        load_cell(cb + 2, wn);
+        double Fl[8], Fh[8];
+        hlld_face_w<T>(a, dir, stage == 2 && cb - 1 >= 2 && cb - 1 <= nd - 2, w0, w1, w2, w3, Fl);
+        // With FP32 fluxes (registers to spare) the conserved state of the next
+        // cell is also loaded one face ahead (384^3 FP32-flux step 17.3 ->
+        // 15.8 ms); with FP64 fluxes the extra registers spill (18.9 -> 29.7 ms).
+        constexpr bool kPreU = sizeof(T) == 4;
+        double un[8];
         if (kPreU) {
-            double uc[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) uc[q] = un[q];
-            if (c < ce) {
+            for (int q = 0; q < 8; q++) un[q] = u[q][base + s * (size_t)(cb - 1)];
+        }
+        for (int c = cb; c <= ce; c++) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) un[q] = u[q][p + s];
+            for (int q = 0; q < 8; q++) { w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; w3[q] = wn[q]; }
+            if (c < ce) load_cell(c + 3, wn);
+            const size_t p = base + s * (size_t)(c - 1);
+            if (kPreU) {
+                double uc[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) uc[q] = un[q];
+                if (c < ce) {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) un[q] = u[q][p + s];
+                }
+                hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
+                double U[8];
+                const double dl = dd[c - 1];
+#pragma unroll
+                for (int q = 0; q < 8; q++) U[q] = uc[q] - dts * ((Fh[q] - Fl[q]) / dl);   // adddfdx.f
+                if (to_prim) {
+                    const bool cc = cij && c >= mg + 1 && c <= nd - mg;
+                    const double dq = hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p, cc,
+                                                   cc ? fminF(dxyg, image_min(dd, c, nd, mg)) : 0.0);
+                    if (dq < dtmin_local) dtmin_local = dq;   // NaN never wins
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) u[q][p] = U[q];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
+                continue;
             }
             hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
-            double U[8];
-            const double dl = dd[c - 1];
-#pragma unroll
-            for (int q = 0; q < 8; q++) U[q] = uc[q] - dts * ((Fh[q] - Fl[q]) / dl);   // adddfdx.f
             if (to_prim) {
-                hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p);
-            } else {
+                // the last sweep of the stage also recovers the primitives
+                // (hlld_prim fused): half (stage 1) or out (stage 2)
+                double U[8];
+                const double dl = dd[c - 1];
 #pragma unroll
-                for (int q = 0; q < 8; q++) u[q][p] = U[q];
+                for (int q = 0; q < 8; q++) U[q] = u[q][p] - dts * ((Fh[q] - Fl[q]) / dl);
+                const bool cc = cij && c >= mg + 1 && c <= nd - mg;
+                const double dq = hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p, cc,
+                                               cc ? fminF(dxyg, image_min(dd, c, nd, mg)) : 0.0);
+                if (dq < dtmin_local) dtmin_local = dq;   // NaN never wins
+            } else {
+                hlld_apply(u, p, dts, dd[c - 1], Fh, Fl);
             }
 #pragma unroll
             for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
-            continue;
         }
-        hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
-        if (to_prim) {
-            // the last sweep of the stage also recovers the primitives
-            // (hlld_prim fused): half (stage 1) or out (stage 2)
-            double U[8];
-            const double dl = dd[c - 1];
+        if (to_prim) {   // the end cells 1 and nd of the column, not swept
+            for (int c = 1; c <= nd; c += nd - 1) {
+                if ((c == 1 && cb != 2) || (c == nd && ce != nd - 1)) continue;
+                const size_t p = base + s * (size_t)(c - 1);
+                double U[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) U[q] = u[q][p] - dts * ((Fh[q] - Fl[q]) / dl);
-            hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p);
-        } else {
-            hlld_apply(u, p, dts, dd[c - 1], Fh, Fl);
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
-    }
-    if (to_prim) {   // the end cells 1 and nd of the column, not swept
-        for (int c = 1; c <= nd; c += nd - 1) {
-            if ((c == 1 && cb != 2) || (c == nd && ce != nd - 1)) continue;
-            const size_t p = base + s * (size_t)(c - 1);
-            double U[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) U[q] = u[q][p];
-            hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p);
+                for (int q = 0; q < 8; q++) U[q] = u[q][p];
+                hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p);
+            }
         }
     }
+    if (do_cfl) block_min_to_global(dtmin_local, red, &a.clk->dtkey[(a.step + 1) % 3]);
 }
 
 // rvtovv.f, ee2eh.f, eh2pr.f: primitives of every cell from u (stage 1 -> half,
@@ -456,6 +484,8 @@ hipError_t launch_hlld_t(const HlldArgs &a, hipStream_t s) {
     // the primitives of each stage: fused into the z sweeps when z is the last
     const int fuse = a.mfdim[0] == 1 && n[0] >= 3;
     const int fuse_p = a.mfdim[2] == 1 && n[2] >= 3;
+    // a periodic run's clock and cfl_m ride on the x sweep and the z sweep
+    if (a.per_margin > 0 && !(fuse && fuse_p)) return hipErrorInvalidValue;
     if (!fuse) hipLaunchKernelGGL(hlld_cons, dim3(gN), dim3(256), 0, s, a);
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < 3; dir++) {

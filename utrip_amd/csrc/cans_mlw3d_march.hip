@@ -28,10 +28,7 @@
 // fewest plane-steps (one block per CU: its LDS is 152 KB).
 // In a periodic box (per_margin = m > 0) the kernel advances the device
 // clock itself and takes the next step's cfl_m (cans3d/common/cfl_m.f:39-58)
-// over the cells it updates, as the 2D kernel does: the reference's bdperx,
-// bdpery, bdperz overwrite the ghost cells with images of interior cells, so
-// cfl_m over every cell of the new state is the minimum over the interior
-// cells, each with the smallest dx, dy, dz among its images.
+// over the cells it updates, as the 2D kernel does (image_min, cfl3_term).
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_roe.hpp"
@@ -41,7 +38,6 @@ namespace {
 
 constexpr double pi8i = 1.0 / kPi / 8.0;
 constexpr double pi4i = 1.0 / kPi / 4.0;
-constexpr double pi4c = 0.25 / kPiF;   // cfl_m's onep4 = 0.25/pai, pai the REAL*4 acos(-1.0)
 constexpr int TB = 16;          // tile edge in cells, halo included
 constexpr int TO = TB - 2;      // updated cells per tile edge
 constexpr int NT = TB * TB;     // threads per block
@@ -278,14 +274,6 @@ CANS_DEV bool march_dt(const Mlw3dArgs &a, double &dt) {
     return c.run;
 }
 
-// smallest of dx(i) and dx of i's periodic images (1-based i, margin m,
-// n cells; each ghost cell holds the image i +- (n - 2m) of an interior cell)
-CANS_DEV double image_min(const double *d, int i, int n, int m) {
-    double g = d[i - 1];
-    if (i >= n - 2 * m + 1) g = fminF(g, d[i - (n - 2 * m) - 1]);
-    if (i <= 2 * m) g = fminF(g, d[i + (n - 2 * m) - 1]);
-    return g;
-}
 
 // One block: the column tile `tile` over the z-segment [k0, k1) of updated
 // cells.  The march starts at cell plane k0-1: its first two steps form the
@@ -443,15 +431,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
                     const double vv = vx * vx + vy * vy + vz * vz;
                     const double bb = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
                     const double pr = (gm - 1.0) * (U[1] - 0.5 * ro * vv - pi8i * bb);
-                    // cfl_m.f:45-54 on the stored values (b2 and v2 are bb and
-                    // vv, the same sums); the ghost images' terms through the
-                    // image minima: m -> safety*(m/sq) is monotone in IEEE
-                    // arithmetic, so the smallest term is this one bit for bit
+                    // the next step's cfl_m term (b2 and v2 are bb and vv,
+                    // the same sums)
                     if (cxy && s >= mg + 1 && s <= kx - mg) {
                         const double dg = fminF(dxyg, image_min(a.dz, s, kx, mg));
-                        const double ca2 = bb * pi4c / ro;
-                        const double cs2 = gm * pr / ro;
-                        const double dtq = a.safety * (dg / __builtin_sqrt(vv + cs2 + ca2));
+                        const double dtq = cfl3_term(a.safety, gm, ro, pr, vv, bb, dg);
                         if (dtq < dtmin_local) dtmin_local = dtq;   // NaN never wins
                     }
                     a.out[0][pk] = ro;

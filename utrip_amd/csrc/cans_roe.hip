@@ -35,13 +35,13 @@ template <int D> struct RoePi2 {};
 
 
 // Run flag / dt of the step: from the clock (after the step's clock kernel
-// ran), or the by-value dt of a drop-in call.
-CANS_DEV bool step_dt(const RoeArgs &a, double &dt) {
+// ran, or, with `enter`, advanced here by the step's first kernel, block 0
+// writing it), or the by-value dt of a drop-in call.
+CANS_DEV bool step_dt(const RoeArgs &a, double &dt, bool enter = false) {
     if (!a.clk) { dt = a.dtp ? *a.dtp : a.dt; return true; }
-    const int d = (a.step + 1) & 1;
-    if (a.clk->stopped[d]) return false;
-    dt = a.clk->dt_last[d];
-    return true;
+    const StepCtl c = clock_use(a.clk, a.step, !enter, blockIdx.x == 0 && threadIdx.x == 0);
+    dt = c.dt;
+    return c.run;
 }
 
 // Face fluxes are stored in the type they are computed in: FP32-flux runs
@@ -60,7 +60,8 @@ template <int D, class T, bool C3>
 __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int stage) {
     constexpr bool FULL = D == 3 || C3;      // all three components present
     double dtv;
-    if (!step_dt(a, dtv)) return;
+    // in a periodic 3D run the stage-1 x faces are the step's first kernel
+    if (!step_dt(a, dtv, D == 3 && a.per_margin > 0 && stage == 1 && dir == 0)) return;
     const int ix = a.ix, jx = a.jx, kx = a.kx;
     const int n[3] = {ix, jx, kx};
     const int lo_n = stage == 1 ? 1 : 2, hi_n = stage == 1 ? n[dir] - 1 : n[dir] - 2;
@@ -380,7 +381,11 @@ __global__ __launch_bounds__(256) void roe_update_kernel(RoeArgs a) {
 // face early (computed twice, by the same operations) so that it holds the
 // face below its first cell.
 template <class T, int STAGE>
-__global__ __launch_bounds__(256) void roe_zmarch3(RoeArgs a, int H) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && STAGE == 2 ? 2 : 1))) void roe_zmarch3(RoeArgs a, int H) {
+    __shared__ double red[4];
+    // per thread: the image-minimum dx, dy of its column and its running cfl
+    // minimum (in LDS: the march holds about 250 VGPRs already)
+    __shared__ double cfs[2][256];
     double dt;
     if (!step_dt(a, dt)) return;
     constexpr int dir = 2;
@@ -391,128 +396,149 @@ __global__ __launch_bounds__(256) void roe_zmarch3(RoeArgs a, int H) {
     if (ci <= 0 || co <= 0 || nf <= 0) return;
     const int nseg = (nf + H - 1) / H;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)ci * co * nseg) return;
-    const int i = lo_t + (int)(t % ci);
-    const int j = lo_t + (int)((t / ci) % co);
-    const int seg = (int)(t / ((size_t)ci * co));
-    const int fb = lo_n + seg * H, fe = min(fb + H - 1, hi_n);
-    const int fs = fb > lo_n ? fb - 1 : fb;
-    const int nd = kx;
-    const size_t sx = 1, sy = (size_t)ix, s = (size_t)ix * jx;
-    const size_t base = lin(i, j, 1, ix, jx);
-    const double *const *P = STAGE == 1 ? a.prim : a.half;
-    double w0[8], w1[8], w2[8], w3[8];
-    auto load_cell = [&](int c, double *d) {
-        const int cc = c < 1 ? 1 : (c > nd ? nd : c);
-        const size_t pc = base + s * (size_t)(cc - 1);
+    // the next step's cfl_m in a periodic run (stage 2 with FP32 fluxes):
+    // every thread of the block reaches the block minimum at the end.  The
+    // FP64 march keeps the separate cfl pass: with the term compiled in, its
+    // loop waits on its loads twice as often (20.0 -> 23.2 ms per 384^3 step).
+    constexpr bool kCfl = STAGE == 2 && sizeof(T) == 4;
+    const int mg = a.per_margin;
+    const bool do_cfl = kCfl && a.clk && mg > 0 && a.safety > 0.0;
+    const int tl = threadIdx.x;
+    if (kCfl && do_cfl) cfs[1][tl] = kR4_1e20;
+    if (t < (size_t)ci * co * nseg) {
+        const int i = lo_t + (int)(t % ci);
+        const int j = lo_t + (int)((t / ci) % co);
+        const int seg = (int)(t / ((size_t)ci * co));
+        const int fb = lo_n + seg * H, fe = min(fb + H - 1, hi_n);
+        const int fs = fb > lo_n ? fb - 1 : fb;
+        const int nd = kx;
+        const size_t sx = 1, sy = (size_t)ix, s = (size_t)ix * jx;
+        const size_t base = lin(i, j, 1, ix, jx);
+        const double *const *P = STAGE == 1 ? a.prim : a.half;
+        double w0[8], w1[8], w2[8], w3[8];
+        auto load_cell = [&](int c, double *d) {
+            const int cc = c < 1 ? 1 : (c > nd ? nd : c);
+            const size_t pc = base + s * (size_t)(cc - 1);
 #pragma unroll
-        for (int q = 0; q < 8; q++) d[q] = P[q][pc];
-    };
-    load_cell(fs - 1, w0); load_cell(fs, w1); load_cell(fs + 1, w2); load_cell(fs + 2, w3);
-    const T gm = T(a.gm);
-    constexpr int vn = S_Z, v1 = S_X, v2 = S_Y, bn = S_BZ, b1 = S_BX, b2 = S_BY;
-    const double dx = a.dx[i - 1], dy = a.dy[j - 1];
-    const double hdt = STAGE == 1 ? 0.5 * dt : dt;
-    double Fl[8];   // z flux of the face below the cell, U slots (normal field: unused)
+            for (int q = 0; q < 8; q++) d[q] = P[q][pc];
+        };
+        load_cell(fs - 1, w0); load_cell(fs, w1); load_cell(fs + 1, w2); load_cell(fs + 2, w3);
+        const T gm = T(a.gm);
+        constexpr int vn = S_Z, v1 = S_X, v2 = S_Y, bn = S_BZ, b1 = S_BX, b2 = S_BY;
+        const double dx = a.dx[i - 1], dy = a.dy[j - 1];
+        const double hdt = STAGE == 1 ? 0.5 * dt : dt;
+        // interior column [m+1, n-m]^2 of the cfl_m (periodic images: image_min)
+        const bool cij = kCfl && do_cfl && i >= mg + 1 && i <= ix - mg && j >= mg + 1 && j <= jx - mg;
+        if (kCfl && cij) cfs[0][tl] = fminF(image_min(a.dx, i, ix, mg), image_min(a.dy, j, jx, mg));
+        double Fl[8];   // z flux of the face below the cell, U slots (normal field: unused)
 #pragma unroll
-    for (int q = 0; q < 8; q++) Fl[q] = 0.0;
-    const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
-    // With FP32 fluxes (registers to spare) the cell update's operands are
-    // loaded before the face flux is computed, so that the flux covers
-    // their latency; with FP64 fluxes they would spill.
-    constexpr bool kPre = sizeof(T) == 4;
-    for (int f = fs;; f++) {
-        double pw[8], fxl[8], fxh[8], fyl[8], fyh[8];
-        if (kPre && f > fs) {
-            const size_t p = base + s * (size_t)(f - 1);
-            double *const *Fx = a.flux[0];
-            double *const *Fy = a.flux[1];
+        for (int q = 0; q < 8; q++) Fl[q] = 0.0;
+        const double pi8i = 5.0e-1 * (1.0 / (4.0 * kPi));
+        // With FP32 fluxes (registers to spare) the cell update's operands are
+        // loaded before the face flux is computed, so that the flux covers
+        // their latency; with FP64 fluxes they would spill.
+        constexpr bool kPre = sizeof(T) == 4;
+        for (int f = fs;; f++) {
+            double pw[8], fxl[8], fxh[8], fyl[8], fyh[8];
+            if (kPre && f > fs) {
+                const size_t p = base + s * (size_t)(f - 1);
+                double *const *Fx = a.flux[0];
+                double *const *Fy = a.flux[1];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    pw[q] = a.prim[q][p];
+                    if (q != S_BX) { fxl[q] = fget<T>(Fx[q], p - sx); fxh[q] = fget<T>(Fx[q], p); }
+                    if (q != S_BY) { fyl[q] = fget<T>(Fy[q], p - sy); fyh[q] = fget<T>(Fy[q], p); }
+                }
+            }
+            T L[8], R[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) {
-                pw[q] = a.prim[q][p];
-                if (q != S_BX) { fxl[q] = fget<T>(Fx[q], p - sx); fxh[q] = fget<T>(Fx[q], p); }
-                if (q != S_BY) { fyl[q] = fget<T>(Fy[q], p - sy); fyh[q] = fget<T>(Fy[q], p); }
+                if (STAGE == 1) {
+                    L[q] = T(w1[q]);
+                    R[q] = T(w2[q]);
+                } else {               // cans3d/hdroe/tvdminmod.f:15-55
+                    L[q] = T(w1[q] + 0.5 * flmt(w1[q] - w0[q], w2[q] - w1[q]));
+                    R[q] = T(w2[q] - 0.5 * flmt(w3[q] - w2[q], w2[q] - w1[q]));
+                }
             }
+            T out[7];
+            roeflux3<T>(gm, L[S_RO], L[S_E], L[vn], L[v1], L[v2], L[bn], L[b1], L[b2], R[S_RO], R[S_E], R[vn], R[v1],
+                        R[v2], R[bn], R[b1], R[b2], out);
+            // the flux as the face arrays hold it (FP32 runs store FP32)
+            double Fh[8];
+            Fh[S_RO] = double(out[0]); Fh[S_E] = double(out[1]); Fh[vn] = double(out[2]); Fh[v1] = double(out[3]);
+            Fh[v2] = double(out[4]); Fh[b1] = double(out[5]); Fh[b2] = double(out[6]); Fh[bn] = 0.0;
+            if (f > fs) {
+                // cell k = f: faces k-1 (Fl) and k (Fh) along z
+                const int k = f;
+                const size_t p = base + s * (size_t)(k - 1);
+                double U[8];
+                const double dz = a.dz[k - 1];
+                double *const *Fx = a.flux[0];
+                double *const *Fy = a.flux[1];
+                if (kPre) {
+                    // prim_to_cons<3> on the loaded primitives
+                    const double ro = pw[S_RO], pr = pw[S_E], vx = pw[S_X], vy = pw[S_Y], vz = pw[S_Z];
+                    const double bx = pw[S_BX], by = pw[S_BY], bz = pw[S_BZ];
+                    U[S_RO] = ro;
+                    U[S_X] = ro * vx;
+                    U[S_Y] = ro * vy;
+                    U[S_Z] = ro * vz;
+                    const double v2 = vx * vx + vy * vy + vz * vz;
+                    const double b2 = bx * bx + by * by + bz * bz;
+                    U[S_E] = pr / (a.gm - 1.0) + 0.5 * ro * v2 + pi8i * b2;
+                    U[S_BX] = bx; U[S_BY] = by; U[S_BZ] = bz;
+                } else {
+                    prim_to_cons<3, false>(a, p, i, U);
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    if (q == S_BX) continue;
+                    const double lo = kPre ? fxl[q] : fget<T>(Fx[q], p - sx), hi = kPre ? fxh[q] : fget<T>(Fx[q], p);
+                    U[q] = U[q] + hdt * ((lo - hi) / dx);
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    if (q == S_BY) continue;
+                    const double lo = kPre ? fyl[q] : fget<T>(Fy[q], p - sy), hi = kPre ? fyh[q] : fget<T>(Fy[q], p);
+                    U[q] = U[q] + hdt * ((lo - hi) / dy);
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    if (q == S_BZ) continue;
+                    U[q] = U[q] + hdt * ((Fl[q] - Fh[q]) / dz);
+                }
+                const double ro = U[S_RO];
+                const double vx = U[S_X] / ro;
+                const double vy = U[S_Y] / ro;
+                const double vz = U[S_Z] / ro;
+                const double v2s = vx * vx + vy * vy + vz * vz;
+                const double b2s = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY] + U[S_BZ] * U[S_BZ];
+                double *const *O = STAGE == 1 ? a.half : a.out;
+                const double pr = (a.gm - 1.0) * (U[S_E] - 0.5 * ro * v2s - pi8i * b2s);
+                O[S_RO][p] = ro;
+                O[S_E][p] = pr;
+                O[S_X][p] = vx;
+                O[S_Y][p] = vy;
+                O[S_Z][p] = vz;
+                O[S_BX][p] = U[S_BX];
+                O[S_BY][p] = U[S_BY];
+                O[S_BZ][p] = U[S_BZ];
+                if (kCfl && cij && k >= mg + 1 && k <= kx - mg) {
+                    // cfl_m of the stored values (v2s, b2s are its sums)
+                    const double dtq = cfl3_term(a.safety, a.gm, ro, pr, v2s, b2s,
+                                                 fminF(cfs[0][tl], image_min(a.dz, k, kx, mg)));
+                    if (dtq < cfs[1][tl]) cfs[1][tl] = dtq;   // NaN never wins
+                }
+            }
+            if (f >= fe) break;
+#pragma unroll
+            for (int q = 0; q < 8; q++) { Fl[q] = Fh[q]; w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; }
+            load_cell(f + 3, w3);
         }
-        T L[8], R[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            if (STAGE == 1) {
-                L[q] = T(w1[q]);
-                R[q] = T(w2[q]);
-            } else {               // cans3d/hdroe/tvdminmod.f:15-55
-                L[q] = T(w1[q] + 0.5 * flmt(w1[q] - w0[q], w2[q] - w1[q]));
-                R[q] = T(w2[q] - 0.5 * flmt(w3[q] - w2[q], w2[q] - w1[q]));
-            }
-        }
-        T out[7];
-        roeflux3<T>(gm, L[S_RO], L[S_E], L[vn], L[v1], L[v2], L[bn], L[b1], L[b2], R[S_RO], R[S_E], R[vn], R[v1],
-                    R[v2], R[bn], R[b1], R[b2], out);
-        // the flux as the face arrays hold it (FP32 runs store FP32)
-        double Fh[8];
-        Fh[S_RO] = double(out[0]); Fh[S_E] = double(out[1]); Fh[vn] = double(out[2]); Fh[v1] = double(out[3]);
-        Fh[v2] = double(out[4]); Fh[b1] = double(out[5]); Fh[b2] = double(out[6]); Fh[bn] = 0.0;
-        if (f > fs) {
-            // cell k = f: faces k-1 (Fl) and k (Fh) along z
-            const int k = f;
-            const size_t p = base + s * (size_t)(k - 1);
-            double U[8];
-            const double dz = a.dz[k - 1];
-            double *const *Fx = a.flux[0];
-            double *const *Fy = a.flux[1];
-            if (kPre) {
-                // prim_to_cons<3> on the loaded primitives
-                const double ro = pw[S_RO], pr = pw[S_E], vx = pw[S_X], vy = pw[S_Y], vz = pw[S_Z];
-                const double bx = pw[S_BX], by = pw[S_BY], bz = pw[S_BZ];
-                U[S_RO] = ro;
-                U[S_X] = ro * vx;
-                U[S_Y] = ro * vy;
-                U[S_Z] = ro * vz;
-                const double v2 = vx * vx + vy * vy + vz * vz;
-                const double b2 = bx * bx + by * by + bz * bz;
-                U[S_E] = pr / (a.gm - 1.0) + 0.5 * ro * v2 + pi8i * b2;
-                U[S_BX] = bx; U[S_BY] = by; U[S_BZ] = bz;
-            } else {
-                prim_to_cons<3, false>(a, p, i, U);
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                if (q == S_BX) continue;
-                const double lo = kPre ? fxl[q] : fget<T>(Fx[q], p - sx), hi = kPre ? fxh[q] : fget<T>(Fx[q], p);
-                U[q] = U[q] + hdt * ((lo - hi) / dx);
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                if (q == S_BY) continue;
-                const double lo = kPre ? fyl[q] : fget<T>(Fy[q], p - sy), hi = kPre ? fyh[q] : fget<T>(Fy[q], p);
-                U[q] = U[q] + hdt * ((lo - hi) / dy);
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                if (q == S_BZ) continue;
-                U[q] = U[q] + hdt * ((Fl[q] - Fh[q]) / dz);
-            }
-            const double ro = U[S_RO];
-            const double vx = U[S_X] / ro;
-            const double vy = U[S_Y] / ro;
-            const double vz = U[S_Z] / ro;
-            const double v2s = vx * vx + vy * vy + vz * vz;
-            const double b2s = U[S_BX] * U[S_BX] + U[S_BY] * U[S_BY] + U[S_BZ] * U[S_BZ];
-            double *const *O = STAGE == 1 ? a.half : a.out;
-            O[S_RO][p] = ro;
-            O[S_E][p] = (a.gm - 1.0) * (U[S_E] - 0.5 * ro * v2s - pi8i * b2s);
-            O[S_X][p] = vx;
-            O[S_Y][p] = vy;
-            O[S_Z][p] = vz;
-            O[S_BX][p] = U[S_BX];
-            O[S_BY][p] = U[S_BY];
-            O[S_BZ][p] = U[S_BZ];
-        }
-        if (f >= fe) break;
-#pragma unroll
-        for (int q = 0; q < 8; q++) { Fl[q] = Fh[q]; w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; }
-        load_cell(f + 3, w3);
     }
+    if (kCfl && do_cfl) block_min_to_global(cfs[1][tl], red, &a.clk->dtkey[(a.step + 1) % 3]);
 }
 
 // Half-step state of the outer cell layer of a 3D grid: the primitives
@@ -704,19 +730,10 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
     // march, cut into 3-face segments to fill the chip, was 6 % slower.
     const bool fy2 = D == 2 && !C3 && n[0] >= 3 && n[1] >= 3 &&
                      (size_t)(n[0] - 2) * (size_t)(n[1] - 1) >= ((size_t)1 << 19);
-    // 3D: z faces fused with the cell updates (roe_zmarch3) where the march's
-    // segments come out at >= 8 faces for the launch's wave target (about
-    // 256^3 and up at the default); on smaller grids the segments would be
-    // a face or two long and the fused march would compute most z faces
-    // twice, so they take the separate passes.
-    bool fz3 = false;
-    if (D == 3 && n[0] >= 3 && n[1] >= 3 && n[2] >= 3) {
-        const size_t cols = (size_t)(n[0] - 2) * (size_t)(n[1] - 2);
-        const int nf = n[2] - 1;
-        long nseg = (long)(((unsigned long long)march_waves() * 64 + cols - 1) / cols);
-        nseg = nseg < 1 ? 1 : (nseg > nf ? nf : nseg);
-        fz3 = (nf + nseg - 1) / nseg >= 8;
-    }
+    // 3D: z faces fused with the cell updates (roe_zmarch3) on large grids
+    const bool fz3 = D == 3 && roe3d_fused_z(n[0], n[1], n[2]);
+    // the fused cfl_m needs the fused z marches and FP32 fluxes
+    if (a.per_margin > 0 && !(D == 3 && fz3 && !C3 && sizeof(T) == 4)) return hipErrorInvalidValue;
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < D; dir++) {
             size_t total = 1;
@@ -802,6 +819,20 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
 }
 
 }  // namespace
+
+// 3D: z faces fused with the cell updates (roe_zmarch3) where the march's
+// segments come out at >= 8 faces for the launch's wave target (about 256^3
+// and up at the default); on smaller grids the segments would be a face or
+// two long and the fused march would compute most z faces twice, so they
+// take the separate passes.
+bool roe3d_fused_z(int ix, int jx, int kx) {
+    if (ix < 3 || jx < 3 || kx < 3) return false;
+    const size_t cols = (size_t)(ix - 2) * (size_t)(jx - 2);
+    const int nf = kx - 1;
+    long nseg = (long)(((unsigned long long)march_waves() * 64 + cols - 1) / cols);
+    nseg = nseg < 1 ? 1 : (nseg > nf ? nf : nseg);
+    return (nf + nseg - 1) / nseg >= 8;
+}
 
 hipError_t launch_roe2d_half(const RoeArgs &a, hipStream_t s) {
     for (int dir = 0; dir < 2; dir++) {

@@ -23,9 +23,16 @@ struct RoeArgs {
     int step;
     int ix, jx, kx;
     const double *dtp;       // non-null (and clk null): dt read from device memory
+    // 3D FP32-flux with the fused z march (roe3d_fused_z) in a periodic box
+    // (per_margin = m > 0): the step's first kernel advances the clock and the
+    // stage-2 z march takes the next step's cfl_m (image_min, cfl3_term)
+    double safety;
+    int per_margin;
 };
 // comp3: roe_m3 (1D/2D with vz, bz in slots S_Z, S_BZ)
 hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s, bool comp3 = false);
+// whether a 3D Roe step on this grid takes the fused z marches (roe_zmarch3)
+bool roe3d_fused_z(int ix, int jx, int kx);
 // The first half of a 2D FP64 Roe step: the stage-1 face fluxes of both
 // directions (a.flux) and the half-step primitives (a.half); the reverse
 // sweep recomputes them from the recorded state.
@@ -73,6 +80,11 @@ struct HlldArgs {
     Clock *clk;
     int step;
     int ix, jx, kx;
+    // periodic box (per_margin = m > 0; needs the x and z sweeps, mfdim[0]
+    // and mfdim[2] = 1): the stage-1 x sweep advances the clock and the
+    // stage-2 z sweep takes the next step's cfl_m (image_min, cfl3_term)
+    double safety;
+    int per_margin;
 };
 hipError_t launch_hlld(const HlldArgs &a, bool mixed, hipStream_t s);
 
