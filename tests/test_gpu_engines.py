@@ -296,16 +296,18 @@ def test_run_3d_periodic_ghost_cfl(dev, oracle, engine, margin, axis):
             assert_bitwise(got32[k], ref32[k], "fp32-flux " + k)
 
 
-@pytest.mark.parametrize("engine,margin", [("mlw", 2), ("roe", 3), ("hlld", 3)])
-def test_run_3d_periodic_stops(dev, oracle, engine, margin):
-    """The clock carried by the periodic 3D engines' kernels: the t < tend
+@pytest.mark.parametrize("engine,margin,mixed", [("mlw", 2, False), ("roe", 3, False), ("roe", 3, True),
+                                                 ("hlld", 3, False)])
+def test_run_3d_periodic_stops(dev, oracle, engine, margin, mixed):
+    """The clock carried by the periodic 3D engines' kernels (FP64 Roe: the
+    x faces take the step's cfl_m, the y faces advance the clock; FP32-flux
+    Roe on its fused z marches, forced by a low wave target): the t < tend
     loop test ends the run after the step that reaches tend (later steps are
     no-ops), and dt < dtmin sets merr = 9001 with the state unchanged."""
     from utrip_amd import cans
-    case("run 3d periodic, tend and dtmin stops", engine)
+    case("run 3d periodic, tend and dtmin stops", engine, "fp32-flux" if mixed else "fp64")
     pb = P.blast_3d(n=14, margin=margin, gaussian_b=(engine != "hlld"), r0=0.25)
     pp = pb.params
-    mixed = engine == "roe"   # Roe carries the clock with FP32 fluxes (on its fused z marches)
     if mixed:
         # the reference: the same FP32-flux run on the separate passes and cfl pass
         _, info3 = run_gpu(dev, 3, engine, pb.fields, pb.grid, pp, 3, 7, margin, mixed=True)

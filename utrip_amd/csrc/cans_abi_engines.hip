@@ -498,11 +498,15 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     // cfl_m, so a step is the engine and the ghost fill.  The first step of a
     // call still takes cfl_m of its state in a pass of its own (the same
     // minimum again when the call before fused it).
+    // The FP64 Roe step instead takes the step's own cfl_m in its stage-1 x
+    // faces (after the first step of a call, whose ghost cells need not be
+    // images yet) and its stage-1 y faces advance the clock (roe64).
     bool per3 = dim == 3 && safety > 0.0 && margin >= 1 &&
                 ((engine == CANS_ENGINE_MLW && mlw3d_path() != 0) ||
-                 (engine == CANS_ENGINE_ROE && mixed && roe3d_fused_z(ix, jx, kx)) || engine == CANS_ENGINE_HLLD);
+                 (engine == CANS_ENGINE_ROE && (!mixed || roe3d_fused_z(ix, jx, kx))) || engine == CANS_ENGINE_HLLD);
     for (int k = 0; per3 && k < 6 * nf; k++) per3 = bc[k] == BD_PER;
     for (int d = 0; per3 && d < 3; d++) per3 = n[d] - 2 * margin >= margin;
+    const bool roe64 = per3 && engine == CANS_ENGINE_ROE && !mixed;
     for (int st = step0; st < step0 + nsteps; st++) {
         double *const *in = (st & 1) ? b : a;
         double *const *out = (st & 1) ? a : b;
@@ -533,7 +537,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         ca.xr = gx; ca.trig = dim == 3 ? trig : nullptr;   // cfl_m_s for the spherical engine
         ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st; ca.comp3 = c3 ? 1 : 0;
         if (!per3 || st == step0) HIPCHK(launch_cfl(ca, s));
-        if (!per3) HIPCHK(launch_clock_step(c, st, s));
+        if (!per3 || (roe64 && st == step0)) HIPCHK(launch_clock_step(c, st, s));
         const int e = st - step0;
         if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e], s));
         if (cyl) {
@@ -616,7 +620,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
             r.out_az = dim == 2 ? out[faz] : nullptr;
             r.bx = bx1d; r.bxm = bxm1d; r.dx = dx; r.dy = dy; r.dz = dz;
             r.gm = gm; r.clk = c; r.step = st; r.ix = ix; r.jx = jx; r.kx = kx;
-            if (per3) { r.safety = safety; r.per_margin = margin; }
+            if (per3) { r.safety = safety; r.per_margin = margin; r.cfl_now = roe64 && st > step0; }
             HIPCHK(launch_roe(r, dim, mixed != 0, s, c3));
         } else {
             HlldArgs h{};

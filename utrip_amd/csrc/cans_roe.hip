@@ -51,31 +51,12 @@ template <class T> CANS_DEV void fput(double *f, size_t p, T v) { reinterpret_ca
 template <class T> CANS_DEV double fget(const double *f, size_t p) { return double(reinterpret_cast<const T *>(f)[p]); }
 
 // ------------------------------------------------------------------ fluxes
-// Face (i,j,k) in direction `dir` lies between cell (i,j,k) and the next
-// cell along `dir`.  stage 1: first-order states from P; stage 2: MUSCL-
-// minmod states from the half-step primitives Ph.  The face range is what
-// the following cell kernel reads: along dir [lo_n, n-1 or n-2], across
-// [lo_t, n+1-lo_t].
-template <int D, class T, bool C3>
-__global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int stage) {
+// The face of thread t of roe_flux_kernel; with CFL, cq = the cfl_m term
+// of the face's left cell when that cell is interior (else left as it is).
+template <int D, class T, bool C3, bool CFL>
+CANS_DEV void roe_face(const RoeArgs &a, int dir, int stage, const int *lo, const int *cnt, size_t t, double &cq) {
     constexpr bool FULL = D == 3 || C3;      // all three components present
-    double dtv;
-    // in a periodic 3D run the stage-1 x faces are the step's first kernel
-    if (!step_dt(a, dtv, D == 3 && a.per_margin > 0 && stage == 1 && dir == 0)) return;
     const int ix = a.ix, jx = a.jx, kx = a.kx;
-    const int n[3] = {ix, jx, kx};
-    const int lo_n = stage == 1 ? 1 : 2, hi_n = stage == 1 ? n[dir] - 1 : n[dir] - 2;
-    const int lo_t = stage == 1 ? 2 : 3;
-    int lo[3], cnt[3];
-    for (int d = 0; d < 3; d++) {
-        if (d >= D) { lo[d] = 1; cnt[d] = 1; continue; }
-        if (d == dir) { lo[d] = lo_n; cnt[d] = hi_n - lo_n + 1; }
-        else { lo[d] = lo_t; cnt[d] = n[d] + 1 - 2 * lo_t + 1; }
-        if (cnt[d] < 0) cnt[d] = 0;
-    }
-    const size_t total = (size_t)cnt[0] * cnt[1] * cnt[2];
-    const size_t t = (D < 3 ? xcd_first_thread() : (size_t)blockIdx.x * blockDim.x) + threadIdx.x;
-    if (t >= total) return;
     const int i = lo[0] + (int)(t % cnt[0]);
     const int j = lo[1] + (int)((t / cnt[0]) % cnt[1]);
     const int k = lo[2] + (int)(t / ((size_t)cnt[0] * cnt[1]));
@@ -113,6 +94,16 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
         L[S_BX] = T(a.bxm[i - 1]);
         R[S_BX] = T(a.bxm[i - 1]);
     }
+    if (CFL) {   // cfl_m.f:45-54 of cell i (the stage-1 left state is the cell's own)
+        const int mg = a.per_margin;
+        if (i >= mg + 1 && i <= ix - mg && j >= mg + 1 && j <= jx - mg && k >= mg + 1 && k <= kx - mg) {
+            const double v2 = L[S_X] * L[S_X] + L[S_Y] * L[S_Y] + L[S_Z] * L[S_Z];
+            const double b2 = L[S_BX] * L[S_BX] + L[S_BY] * L[S_BY] + L[S_BZ] * L[S_BZ];
+            const double dg = fminF(fminF(image_min(a.dx, i, ix, mg), image_min(a.dy, j, jx, mg)),
+                                    image_min(a.dz, k, kx, mg));
+            cq = cfl3_term(a.safety, a.gm, L[S_RO], L[S_E], v2, b2, dg);
+        }
+    }
     const T gm = T(a.gm);
     double *const *F = a.flux[dir];
     if (D == 3) {
@@ -146,6 +137,49 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
     }
 }
 
+// Face (i,j,k) in direction `dir` lies between cell (i,j,k) and the next
+// cell along `dir`.  stage 1: first-order states from P; stage 2: MUSCL-
+// minmod states from the half-step primitives Ph.  The face range is what
+// the following cell kernel reads: along dir [lo_n, n-1 or n-2], across
+// [lo_t, n+1-lo_t].
+// CFL (3D FP64, stage-1 x faces of a periodic run, not the first step of a
+// call): the kernel also takes this step's cfl_m over the interior cells it
+// reads as the left cell of a face (the state's ghost cells are images from
+// the previous step's bnd; image_min, cfl3_term) -- the stage-1 fluxes do
+// not need dt -- and the stage-1 y march that follows advances the clock.
+template <int D, class T, bool C3, bool CFL = false>
+__global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int stage) {
+    __shared__ double red[4];
+    double dtv;
+    if (CFL) {
+        if (!clock_live(a.clk, a.step)) return;   // uniform across the launch
+    } else if (!step_dt(a, dtv, D == 3 && sizeof(T) == 4 && a.per_margin > 0 && stage == 1 && dir == 0)) {
+        return;   // (a periodic FP32-flux 3D run: the stage-1 x faces advance the clock)
+    }
+    const int ix = a.ix, jx = a.jx, kx = a.kx;
+    const int n[3] = {ix, jx, kx};
+    const int lo_n = stage == 1 ? 1 : 2, hi_n = stage == 1 ? n[dir] - 1 : n[dir] - 2;
+    const int lo_t = stage == 1 ? 2 : 3;
+    int lo[3], cnt[3];
+    for (int d = 0; d < 3; d++) {
+        if (d >= D) { lo[d] = 1; cnt[d] = 1; continue; }
+        if (d == dir) { lo[d] = lo_n; cnt[d] = hi_n - lo_n + 1; }
+        else { lo[d] = lo_t; cnt[d] = n[d] + 1 - 2 * lo_t + 1; }
+        if (cnt[d] < 0) cnt[d] = 0;
+    }
+    const size_t total = (size_t)cnt[0] * cnt[1] * cnt[2];
+    const size_t t = (D < 3 ? xcd_first_thread() : (size_t)blockIdx.x * blockDim.x) + threadIdx.x;
+    if (CFL) {   // every thread of the block reaches the block minimum
+        double cq = kR4_1e20;
+        if (t < total) roe_face<D, T, C3, true>(a, dir, stage, lo, cnt, t, cq);
+        block_min_to_global(cq, red, &a.clk->dtkey[a.step % 3]);
+        return;
+    }
+    if (t >= total) return;
+    double cq;
+    roe_face<D, T, C3, false>(a, dir, stage, lo, cnt, t, cq);
+}
+
 // 3D y and z faces, marched: thread (i, transverse index, segment) walks the
 // faces f = fb..fe of its segment along dir with the four cells f-1..f+2 of
 // each face in a sliding register window, one new cell (8 loads) per face
@@ -155,7 +189,9 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
 template <class T>
 __global__ __launch_bounds__(256) void roe_flux_march3(RoeArgs a, int dir, int stage, int H) {
     double dtv;
-    if (!step_dt(a, dtv)) return;
+    // (FP64 periodic run: the stage-1 y faces advance the clock, after the
+    // x faces took the step's cfl_m)
+    if (!step_dt(a, dtv, sizeof(T) == 8 && a.cfl_now && stage == 1 && dir == 1)) return;
     const int ix = a.ix, jx = a.jx, kx = a.kx;
     const int n[3] = {ix, jx, kx};
     const int lo_n = stage == 1 ? 1 : 2, hi_n = stage == 1 ? n[dir] - 1 : n[dir] - 2;
@@ -732,8 +768,10 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                      (size_t)(n[0] - 2) * (size_t)(n[1] - 1) >= ((size_t)1 << 19);
     // 3D: z faces fused with the cell updates (roe_zmarch3) on large grids
     const bool fz3 = D == 3 && roe3d_fused_z(n[0], n[1], n[2]);
-    // the fused cfl_m needs the fused z marches and FP32 fluxes
-    if (a.per_margin > 0 && !(D == 3 && fz3 && !C3 && sizeof(T) == 4)) return hipErrorInvalidValue;
+    // a periodic run's cfl_m: FP32 fluxes in the fused z marches, FP64 in the
+    // stage-1 x faces (cfl_now)
+    if (a.per_margin > 0 && !(D == 3 && !C3 && (sizeof(T) == 8 || fz3))) return hipErrorInvalidValue;
+    const bool xcfl = D == 3 && !C3 && sizeof(T) == 8 && a.cfl_now && a.per_margin > 0;
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < D; dir++) {
             size_t total = 1;
@@ -805,8 +843,12 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                 }
                 continue;
             }
-            hipLaunchKernelGGL((roe_flux_kernel<D, T, C3>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
-                               a, dir, stage);
+            if (xcfl && stage == 1 && dir == 0)
+                hipLaunchKernelGGL((roe_flux_kernel<D, T, C3, true>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
+                                   a, dir, stage);
+            else
+                hipLaunchKernelGGL((roe_flux_kernel<D, T, C3>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
+                                   a, dir, stage);
         }
         if (stage == 1 && (D == 1 || (D == 2 && !fy2) || (D == 3 && !fz3)))
             hipLaunchKernelGGL((roe_half_kernel<D, T, C3>), dim3(roe_grid<D>(N)), dim3(256), 0, s, a);

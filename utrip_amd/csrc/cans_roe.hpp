@@ -23,11 +23,14 @@ struct RoeArgs {
     int step;
     int ix, jx, kx;
     const double *dtp;       // non-null (and clk null): dt read from device memory
-    // 3D FP32-flux with the fused z march (roe3d_fused_z) in a periodic box
-    // (per_margin = m > 0): the step's first kernel advances the clock and the
-    // stage-2 z march takes the next step's cfl_m (image_min, cfl3_term)
+    // 3D in a periodic box (per_margin = m > 0).  FP32 fluxes with the fused
+    // z march (roe3d_fused_z): the step's first kernel advances the clock and
+    // the stage-2 z march takes the next step's cfl_m.  FP64 with cfl_now
+    // (not the first step of a call): the stage-1 x faces take this step's
+    // cfl_m and the stage-1 y march advances the clock (image_min, cfl3_term)
     double safety;
     int per_margin;
+    int cfl_now;
 };
 // comp3: roe_m3 (1D/2D with vz, bz in slots S_Z, S_BZ)
 hipError_t launch_roe(const RoeArgs &a, int dim, bool mixed, hipStream_t s, bool comp3 = false);
