@@ -240,6 +240,33 @@ def test_run_2d_ot(dev, oracle, engine, margin):
         assert_bitwise(got[k], ref[k], k)
 
 
+@pytest.mark.parametrize("n,nsteps", [(40, 12), (724, 4)])
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_run_2d_roe_periodic_ghost_cfl(dev, oracle, axis, n, nsteps):
+    """Periodic 2D Roe run (after the first step of a call the stage-1 x faces
+    take the step's cfl_m and the next kernel advances the clock) on
+    stretched grids whose smallest spacing on one axis lies in the two ghost
+    cells at its ends: the reference's cfl_m takes dt from a ghost image.
+    724^2 takes the fused y marches, 40^2 the separate passes; two calls."""
+    from utrip_amd.problems import Problem
+    case("run 2d roe periodic ghost cfl", n, "axis", axis)
+    margin = 3
+    base = P.orszag_tang_2d(n=n, margin=margin)
+    ix, jx = base.fields["ro"].shape
+    grid = dict(base.grid)
+    for nm, m, seed in (("x", ix, 31), ("y", jx, 32)):
+        _, d, dm = stretched_grid(m, seed=seed)
+        if nm == axis:
+            d[0], d[m - 1] = 0.3 * d.min(), 0.35 * d.min()
+        grid["d" + nm], grid["d" + nm + "m"] = d, dm
+    pb = Problem(base.name, base.fields, grid, base.params)
+    ref, dts = oracle_loop(oracle, 2, "roe", pb.fields, grid, pb.params, nsteps, 7, margin)
+    got, info = run_gpu(dev, 2, "roe", pb.fields, grid, pb.params, nsteps, 7, margin)
+    assert info["nexec"] == nsteps and info["merr"] == 0 and info["dt"] == dts[-1]
+    for k in F2:
+        assert_bitwise(got[k], ref[k], k)
+
+
 @pytest.mark.parametrize("engine,margin", [("mlw", 2), ("roe", 3), ("hlld", 3)])
 def test_run_3d_blast(dev, oracle, engine, margin):
     case("run 3d blast", engine)

@@ -492,21 +492,26 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         if (dim == 3) HIPCHK(launch_metric(dz, dzm, kx, m[2], s));
     }
     double *eng = cv.p;
-    // 3D in a periodic box (every field periodic on every face) on the
-    // Cartesian MLW marching kernel, the FP32-flux Roe step with its fused z
-    // marches or HLLD: the engine's kernels advance the clock and take the next step's
-    // cfl_m, so a step is the engine and the ghost fill.  The first step of a
-    // call still takes cfl_m of its state in a pass of its own (the same
-    // minimum again when the call before fused it).
-    // The FP64 Roe step instead takes the step's own cfl_m in its stage-1 x
-    // faces (after the first step of a call, whose ghost cells need not be
-    // images yet) and its stage-1 y faces advance the clock (roe64).
-    bool per3 = dim == 3 && safety > 0.0 && margin >= 1 &&
-                ((engine == CANS_ENGINE_MLW && mlw3d_path() != 0) ||
-                 (engine == CANS_ENGINE_ROE && (!mixed || roe3d_fused_z(ix, jx, kx))) || engine == CANS_ENGINE_HLLD);
-    for (int k = 0; per3 && k < 6 * nf; k++) per3 = bc[k] == BD_PER;
-    for (int d = 0; per3 && d < 3; d++) per3 = n[d] - 2 * margin >= margin;
-    const bool roe64 = per3 && engine == CANS_ENGINE_ROE && !mixed;
+    // In a periodic box (every field periodic on every face) the engine's
+    // kernels carry cfl_m and the clock, so a step is the engine and the
+    // ghost fill:
+    //  * 3D Cartesian MLW (marching kernel), HLLD and FP32-flux Roe (fused z
+    //    marches): the kernels advance the clock and take the next step's
+    //    cfl_m; the first step of a call still takes cfl_m of its state in a
+    //    pass of its own (the same minimum again when the call before fused it);
+    //  * FP64 3D Roe and 2D Roe (roecfl): the stage-1 x faces take the step's
+    //    own cfl_m and the next kernel advances the clock, after the first
+    //    step of a call (whose ghost cells need not be images yet), which
+    //    keeps the cfl pass and the clock kernel.
+    bool perio = safety > 0.0 && margin >= 1 && bc != nullptr;
+    for (int k = 0; perio && k < 2 * dim * nf; k++) perio = bc[k] == BD_PER;
+    for (int d = 0; perio && d < dim; d++) perio = n[d] - 2 * margin >= margin;
+    const bool per3 = perio && dim == 3 &&
+                      ((engine == CANS_ENGINE_MLW && mlw3d_path() != 0) ||
+                       (engine == CANS_ENGINE_ROE && (!mixed || roe3d_fused_z(ix, jx, kx))) ||
+                       engine == CANS_ENGINE_HLLD);
+    const bool roecfl = engine == CANS_ENGINE_ROE && ((per3 && !mixed) || (perio && dim == 2));
+    const bool fused = per3 || roecfl;
     for (int st = step0; st < step0 + nsteps; st++) {
         double *const *in = (st & 1) ? b : a;
         double *const *out = (st & 1) ? a : b;
@@ -536,8 +541,8 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         ca.dx = dx; ca.dy = dy; ca.dz = dz; ca.safety = safety; ca.gm = gm;
         ca.xr = gx; ca.trig = dim == 3 ? trig : nullptr;   // cfl_m_s for the spherical engine
         ca.ix = ix; ca.jx = jx; ca.kx = kx; ca.dim = dim; ca.clk = c; ca.step = st; ca.comp3 = c3 ? 1 : 0;
-        if (!per3 || st == step0) HIPCHK(launch_cfl(ca, s));
-        if (!per3 || (roe64 && st == step0)) HIPCHK(launch_clock_step(c, st, s));
+        if (!fused || st == step0) HIPCHK(launch_cfl(ca, s));
+        if (!fused || (roecfl && st == step0)) HIPCHK(launch_clock_step(c, st, s));
         const int e = st - step0;
         if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e], s));
         if (cyl) {
@@ -620,7 +625,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
             r.out_az = dim == 2 ? out[faz] : nullptr;
             r.bx = bx1d; r.bxm = bxm1d; r.dx = dx; r.dy = dy; r.dz = dz;
             r.gm = gm; r.clk = c; r.step = st; r.ix = ix; r.jx = jx; r.kx = kx;
-            if (per3) { r.safety = safety; r.per_margin = margin; r.cfl_now = roe64 && st > step0; }
+            if (fused) { r.safety = safety; r.per_margin = margin; r.cfl_now = roecfl && st > step0; }
             HIPCHK(launch_roe(r, dim, mixed != 0, s, c3));
         } else {
             HlldArgs h{};

@@ -80,4 +80,13 @@ CANS_DEV double cfl3_term(double safety, double gm, double ro, double pr, double
     return safety * (dg / __builtin_sqrt(v2 + cs2 + ca2));
 }
 
+// cans2d/common/cfl_m.f:46-55, the same with pi4i = 1/pai/4, pai the REAL*8
+// acos(-1.0d0), v2 = vx*vx + vy*vy, b2 = bx*bx + by*by, dg = min(dx, dy)
+CANS_DEV double cfl2_term(double safety, double gm, double ro, double pr, double v2, double b2, double dg) {
+    constexpr double pi4i = 1.0 / kPi / 4.0;
+    const double ca2 = b2 * pi4i / ro;
+    const double cs2 = gm * pr / ro;
+    return safety * (dg / __builtin_sqrt(v2 + cs2 + ca2));
+}
+
 }  // namespace cans

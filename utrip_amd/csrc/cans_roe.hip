@@ -94,7 +94,7 @@ CANS_DEV void roe_face(const RoeArgs &a, int dir, int stage, const int *lo, cons
         L[S_BX] = T(a.bxm[i - 1]);
         R[S_BX] = T(a.bxm[i - 1]);
     }
-    if (CFL) {   // cfl_m.f:45-54 of cell i (the stage-1 left state is the cell's own)
+    if (CFL && D == 3) {   // cans3d cfl_m.f:45-54 of cell i (the stage-1 left state is the cell's own, FP64)
         const int mg = a.per_margin;
         if (i >= mg + 1 && i <= ix - mg && j >= mg + 1 && j <= jx - mg && k >= mg + 1 && k <= kx - mg) {
             const double v2 = L[S_X] * L[S_X] + L[S_Y] * L[S_Y] + L[S_Z] * L[S_Z];
@@ -102,6 +102,17 @@ CANS_DEV void roe_face(const RoeArgs &a, int dir, int stage, const int *lo, cons
             const double dg = fminF(fminF(image_min(a.dx, i, ix, mg), image_min(a.dy, j, jx, mg)),
                                     image_min(a.dz, k, kx, mg));
             cq = cfl3_term(a.safety, a.gm, L[S_RO], L[S_E], v2, b2, dg);
+        }
+    }
+    if (CFL && D == 2) {   // cans2d cfl_m.f:46-55 of cell i from its FP64 primitives
+        const int mg = a.per_margin;
+        if (i >= mg + 1 && i <= ix - mg && j >= mg + 1 && j <= jx - mg) {
+            const double *const *Q = a.prim;
+            const double vx = Q[S_X][p], vy = Q[S_Y][p], bx = Q[S_BX][p], by = Q[S_BY][p];
+            const double v2 = vx * vx + vy * vy;
+            const double b2 = bx * bx + by * by;
+            const double dg = fminF(image_min(a.dx, i, ix, mg), image_min(a.dy, j, jx, mg));
+            cq = cfl2_term(a.safety, a.gm, Q[S_RO][p], Q[S_E][p], v2, b2, dg);
         }
     }
     const T gm = T(a.gm);
@@ -153,8 +164,12 @@ __global__ __launch_bounds__(256) void roe_flux_kernel(RoeArgs a, int dir, int s
     double dtv;
     if (CFL) {
         if (!clock_live(a.clk, a.step)) return;   // uniform across the launch
-    } else if (!step_dt(a, dtv, D == 3 && sizeof(T) == 4 && a.per_margin > 0 && stage == 1 && dir == 0)) {
-        return;   // (a periodic FP32-flux 3D run: the stage-1 x faces advance the clock)
+    } else if (!step_dt(a, dtv, (D == 3 && sizeof(T) == 4 && a.per_margin > 0 && stage == 1 && dir == 0) ||
+                                    (D == 2 && a.cfl_now && stage == 1 && dir == 1))) {
+        // (a periodic run: FP32-flux 3D, the stage-1 x faces advance the clock;
+        // 2D after the x faces took cfl_m, the stage-1 y faces without the
+        // fused y march)
+        return;
     }
     const int ix = a.ix, jx = a.jx, kx = a.kx;
     const int n[3] = {ix, jx, kx};
@@ -727,7 +742,8 @@ __global__ __launch_bounds__(256) void roe_ymarch2(RoeArgs a, int H) {
 // Half-step state of the outer cell ring of a 2D grid: its primitives.
 __global__ __launch_bounds__(256) void roe_half_ring2(RoeArgs a) {
     double dt;
-    if (!step_dt(a, dt)) return;
+    // (a periodic run after the x faces took cfl_m: this kernel advances the clock)
+    if (!step_dt(a, dt, a.cfl_now != 0)) return;
     const int ix = a.ix, jx = a.jx;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j;
@@ -768,10 +784,11 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                      (size_t)(n[0] - 2) * (size_t)(n[1] - 1) >= ((size_t)1 << 19);
     // 3D: z faces fused with the cell updates (roe_zmarch3) on large grids
     const bool fz3 = D == 3 && roe3d_fused_z(n[0], n[1], n[2]);
-    // a periodic run's cfl_m: FP32 fluxes in the fused z marches, FP64 in the
-    // stage-1 x faces (cfl_now)
-    if (a.per_margin > 0 && !(D == 3 && !C3 && (sizeof(T) == 8 || fz3))) return hipErrorInvalidValue;
-    const bool xcfl = D == 3 && !C3 && sizeof(T) == 8 && a.cfl_now && a.per_margin > 0;
+    // a periodic run's cfl_m: 3D FP32 fluxes in the fused z marches; 3D FP64
+    // and 2D in the stage-1 x faces (cfl_now)
+    if (a.per_margin > 0 && !((D == 3 && !C3 && (sizeof(T) == 8 || fz3)) || (D == 2 && !C3)))
+        return hipErrorInvalidValue;
+    const bool xcfl = !C3 && a.cfl_now && a.per_margin > 0 && ((D == 3 && sizeof(T) == 8) || D == 2);
     for (int stage = 1; stage <= 2; stage++) {
         for (int dir = 0; dir < D; dir++) {
             size_t total = 1;
@@ -843,12 +860,14 @@ hipError_t launch_roe_t(const RoeArgs &a, hipStream_t s) {
                 }
                 continue;
             }
-            if (xcfl && stage == 1 && dir == 0)
-                hipLaunchKernelGGL((roe_flux_kernel<D, T, C3, true>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
-                                   a, dir, stage);
-            else
-                hipLaunchKernelGGL((roe_flux_kernel<D, T, C3>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
-                                   a, dir, stage);
+            if constexpr (D >= 2 && !C3) {
+                if (xcfl && stage == 1 && dir == 0) {
+                    hipLaunchKernelGGL((roe_flux_kernel<D, T, C3, true>), dim3(roe_grid<D>(total)), dim3(256), 0, s,
+                                       a, dir, stage);
+                    continue;
+                }
+            }
+            hipLaunchKernelGGL((roe_flux_kernel<D, T, C3>), dim3(roe_grid<D>(total)), dim3(256), 0, s, a, dir, stage);
         }
         if (stage == 1 && (D == 1 || (D == 2 && !fy2) || (D == 3 && !fz3)))
             hipLaunchKernelGGL((roe_half_kernel<D, T, C3>), dim3(roe_grid<D>(N)), dim3(256), 0, s, a);

@@ -27,7 +27,10 @@ struct RoeArgs {
     // z march (roe3d_fused_z): the step's first kernel advances the clock and
     // the stage-2 z march takes the next step's cfl_m.  FP64 with cfl_now
     // (not the first step of a call): the stage-1 x faces take this step's
-    // cfl_m and the stage-1 y march advances the clock (image_min, cfl3_term)
+    // cfl_m and the stage-1 y march advances the clock (image_min,
+    // cfl3_term).  2D (either precision) with cfl_now: the stage-1 x faces
+    // take the step's cfl_m (cfl2_term) and the next kernel (the half-step
+    // ring or the stage-1 y faces) advances the clock.
     double safety;
     int per_margin;
     int cfl_now;
