@@ -240,6 +240,40 @@ def test_run_2d_ot(dev, oracle, engine, margin):
         assert_bitwise(got[k], ref[k], k)
 
 
+@pytest.mark.parametrize("shape,margin", [((13, 17, 11), 1), ((20, 9, 12), 2), ((10, 14, 19), 3), ((21, 13), 2),
+                                          ((23, 9), 3), ((40,), 2)])
+def test_run_periodic_fill_shapes(dev, oracle, shape, margin):
+    """Periodic MLW runs whose ghost fill is the one-pass periodic fill (every
+    field periodic on every face) on uneven 1D/2D/3D boxes and margins 1-3,
+    against the reference's bdperx, bdpery, bdperz one after the other (the
+    oracle loop), bit for bit with the dt sequence; the initial ghosts are not
+    images, so the first step's cfl pass reads them as the reference does."""
+    case("run periodic fill", shape, "margin", margin)
+    dim = len(shape)
+    if dim == 3:
+        f = smooth_state_3d(*shape, seed=sum(shape))
+        dx, dxm, dy, dym, dz, dzm = grid3(*shape)
+        grid = dict(dx=dx, dxm=dxm, dy=dy, dym=dym, dz=dz, dzm=dzm)
+        fields, bx, bxm = {k: f[k] for k in F3}, None, None
+    elif dim == 2:
+        f = smooth_state_2d(*shape, seed=sum(shape))
+        _, dx, dxm = stretched_grid(shape[0], 5)
+        _, dy, dym = stretched_grid(shape[1], 6)
+        grid = dict(dx=dx, dxm=dxm, dy=dy, dym=dym)
+        fields, bx, bxm = {k: f[k] for k in F2}, None, None
+    else:
+        f = smooth_state_1d(shape[0], seed=3)
+        _, dx, dxm = stretched_grid(shape[0], 7)
+        grid = dict(dx=dx, dxm=dxm)
+        fields, bx, bxm = {k: f[k] for k in F1}, f["bx"], f["bxm"]
+    pp = dict(gm=5.0 / 3.0, safety=0.4, qav=3.0, dtmin=1e-10)
+    ref, dts = oracle_loop(oracle, dim, "mlw", fields, grid, pp, 6, 7, margin, bx, bxm)
+    got, info = run_gpu(dev, dim, "mlw", fields, grid, pp, 6, 7, margin, bx, bxm)
+    assert info["nexec"] == 6 and info["merr"] == 0 and info["dt"] == dts[-1]
+    for k in {1: F1, 2: F2, 3: F3}[dim]:
+        assert_bitwise(got[k], ref[k], k)
+
+
 @pytest.mark.parametrize("n,nsteps", [(40, 12), (724, 4)])
 @pytest.mark.parametrize("axis", ["x", "y"])
 def test_run_2d_roe_periodic_ghost_cfl(dev, oracle, axis, n, nsteps):

@@ -503,9 +503,12 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     //    own cfl_m and the next kernel advances the clock, after the first
     //    step of a call (whose ghost cells need not be images yet), which
     //    keeps the cfl pass and the clock kernel.
-    bool perio = safety > 0.0 && margin >= 1 && bc != nullptr;
-    for (int k = 0; perio && k < 2 * dim * nf; k++) perio = bc[k] == BD_PER;
-    for (int d = 0; perio && d < dim; d++) perio = n[d] - 2 * margin >= margin;
+    // (allper: every field periodic on every face; the ghost fill is then one
+    // pass, launch_bnd_periodic)
+    bool allper = margin >= 1 && bc != nullptr;
+    for (int k = 0; allper && k < 2 * dim * nf; k++) allper = bc[k] == BD_PER;
+    for (int d = 0; allper && d < dim; d++) allper = n[d] - 2 * margin >= margin;
+    const bool perio = allper && safety > 0.0;
     const bool per3 = perio && dim == 3 &&
                       ((engine == CANS_ENGINE_MLW && mlw3d_path() != 0) ||
                        (engine == CANS_ENGINE_ROE && (!mixed || roe3d_fused_z(ix, jx, kx))) ||
@@ -640,6 +643,10 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
         }
         if (events) HIPCHK(hipEventRecord((hipEvent_t)events[2 * e + 1], s));
         // bnd: x faces, then y, then z; each face: periodic fields both sides
+        if (allper) {
+            HIPCHK(launch_bnd_periodic(out, nf, ix, jx, kx, dim, margin, s));
+            continue;
+        }
         for (int dir = 0; dir < dim; dir++) {
             for (int side = 0; side < 2; side++) {
                 BndArgs bp{}, bo{};

@@ -94,6 +94,65 @@ __global__ void bnd_kernel(BndArgs a) {
 #undef Q
 }
 
+// Every field periodic on every face of a 1D/2D/3D box, all directions in
+// one pass.  bdperx, then bdpery, then bdperz (each over every transverse
+// index, the ghosts included) leave each ghost cell equal to the interior
+// cell at its images along every axis, i -> i +- (n - 2m) for a ghost index
+// (n - 2m >= m: one shift reaches the interior): each ghost cell is copied
+// from that cell directly, the same values.  Sources are interior cells,
+// which the pass never writes.  Wave units: (row with a ghost j or k, 64
+// columns) copy the whole row with lanes along x; then 64 (row, ghost
+// column) items of the rows whose j and k are interior.
+struct PerFill {
+    double *q[kMaxBndFields];
+    int nf, ix, jx, kx, mx, my, mz;
+    long long na, nb;   // units of the two kinds
+};
+CANS_DEV int per_img(int i, int n, int m) { return i <= m ? i + (n - 2 * m) : (i > n - m ? i - (n - 2 * m) : i); }
+
+__global__ __launch_bounds__(256) void per_fill_kernel(PerFill a) {
+    const int ix = a.ix, jx = a.jx, kx = a.kx, mx = a.mx, my = a.my, mz = a.mz;
+    const long long nchunk = (ix + 63) / 64;
+    const int lane = threadIdx.x & 63;
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long u = uni((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); u < a.na + a.nb; u += nw) {
+        int i, j, k;
+        bool on;
+        if (u < a.na) {
+            const long long g = u / nchunk;
+            const int c = (int)(u - g * nchunk);
+            const long long gz = 2LL * mz * jx;   // rows of the ghost z planes
+            if (g < gz) {
+                const int pk = (int)(g / jx);
+                j = (int)(g - (long long)pk * jx) + 1;
+                k = pk < mz ? pk + 1 : kx - 2 * mz + pk + 1;
+            } else {                               // interior planes: their ghost y rows
+                const long long h = g - gz;
+                const int pj = (int)(h % (2 * my));
+                k = mz + 1 + (int)(h / (2 * my));
+                j = pj < my ? pj + 1 : jx - 2 * my + pj + 1;
+            }
+            i = c * 64 + lane + 1;
+            on = i <= ix;
+        } else {
+            const long long v = (u - a.na) * 64 + lane;
+            const long long nrow = (long long)(jx - 2 * my) * (kx - 2 * mz);
+            on = v < nrow * 2 * mx;
+            const int col = (int)(v % (2 * mx));
+            const long long r = v / (2 * mx);
+            j = my + 1 + (int)(r % (jx - 2 * my));
+            k = mz + 1 + (int)(r / (jx - 2 * my));
+            i = col < mx ? col + 1 : ix - 2 * mx + col + 1;
+        }
+        if (on) {
+            const size_t dst = (size_t)(i - 1) + (size_t)ix * ((size_t)(j - 1) + (size_t)jx * (size_t)(k - 1));
+            const size_t src = (size_t)(per_img(i, ix, mx) - 1) +
+                               (size_t)ix * ((size_t)(per_img(j, jx, my) - 1) + (size_t)jx * (size_t)(per_img(k, kx, mz) - 1));
+            for (int f = 0; f < a.nf; f++) a.q[f][dst] = a.q[f][src];
+        }
+    }
+}
+
 // ---------------------------------------------------------------- cfl
 // 1D: cans1d/common/cfl_m.f:45-60, interior i=2..ix-1,
 //     |vx| + sqrt(cs2+ca2), pi4i = 0.25/pi, onero = 1.0/ro.
@@ -253,6 +312,25 @@ hipError_t launch_bnd(const BndArgs &a, hipStream_t s) {
     const long total = nlines * a.nf;
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(bnd_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bnd_periodic(double *const *q, int nf, int ix, int jx, int kx, int dim, int m, hipStream_t s) {
+    if (nf < 1 || nf > kMaxBndFields || m < 1) return hipErrorInvalidValue;
+    PerFill a{};
+    for (int f = 0; f < nf; f++) a.q[f] = q[f];
+    a.nf = nf; a.ix = ix; a.jx = jx; a.kx = kx;
+    a.mx = m; a.my = dim >= 2 ? m : 0; a.mz = dim >= 3 ? m : 0;
+    const int n[3] = {ix, jx, kx}, mm[3] = {a.mx, a.my, a.mz};
+    for (int d = 0; d < 3; d++)
+        if (mm[d] && n[d] - 2 * mm[d] < mm[d]) return hipErrorInvalidValue;   // one shift must reach the interior
+    const long long nchunk = (ix + 63) / 64;
+    a.na = (2LL * a.mz * jx + (long long)(kx - 2 * a.mz) * 2 * a.my) * nchunk;
+    a.nb = ((long long)(jx - 2 * a.my) * (kx - 2 * a.mz) * 2 * a.mx + 63) / 64;
+    long long blocks = (a.na + a.nb + 3) / 4;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) return hipSuccess;
+    hipLaunchKernelGGL(per_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

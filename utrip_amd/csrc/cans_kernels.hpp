@@ -277,6 +277,9 @@ hipError_t launch_metric(const double *d, const double *dm, int n, Metric m, hip
 hipError_t launch_mlw2d(const Mlw2dArgs &a, hipStream_t s);
 hipError_t launch_mlw1d(const Mlw1dArgs &a, hipStream_t s);
 hipError_t launch_bnd(const BndArgs &a, hipStream_t s);
+// every field periodic on every face (margin m along each of the dim axes):
+// bdperx, bdpery, bdperz in one pass
+hipError_t launch_bnd_periodic(double *const *q, int nf, int ix, int jx, int kx, int dim, int m, hipStream_t s);
 hipError_t launch_copy_interior(double *const *dst, const double *const *src, int nf,
                                 int ix, int jx, int kx, int lo, hipStream_t s);
 hipError_t launch_cfl(const CflArgs &a, hipStream_t s);
