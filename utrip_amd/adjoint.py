@@ -9,9 +9,11 @@ With ``ckpt = 0`` it keeps every state of the trajectory in device memory
 (7 fields x (nsteps+1) states: 2.9 GB for 200 steps at 512^2) and its
 ``backward`` is one reverse sweep; with ``ckpt > 0`` it keeps a checkpoint
 every ``ckpt`` steps and the backward recomputes each segment from its
-checkpoint (on a second stream) before sweeping it, except the last segment,
-whose states the forward records as it runs it (the backward sweeps that
-segment first, so its recomputation would have nothing to overlap with).
+checkpoint (on a second stream) before sweeping it, except the last two
+segments, whose states the forward records as it runs them into the two
+segment buffers (the backward sweeps them first: the last one's
+recomputation would have nothing to overlap with, and the other's would
+compete with that sweep for the CUs).
 ``FinalDensity`` wraps this as a ``torch.autograd.Function`` of the initial
 velocities (interior) and a uniform pressure scale ``beta``.
 """
@@ -116,32 +118,40 @@ class PeriodicMlwAdjoint:
         """Run nsteps from ``state`` (7 tensors ro,pr,vx,vy,bx,by,az, ghosts
         filled); keep every state (ckpt = 0) or the checkpoints; return the
         final state (clones).  With checkpoints, the segments up to the last
-        run on the resident fused kernel and the last one is recorded into
-        segment buffer 0 (the same kernels as the backward's recomputation,
-        so the same states bit for bit)."""
+        two run on the resident fused kernel and the last two are recorded
+        into the segment buffers (the last into buffer 0), by the same
+        kernels as the backward's recomputation, so the same states bit for
+        bit."""
         if self.ckpt == 0:
             return self._forward_all(state)
         self.ckpts = []
         run = self._run(state)
         self._segment_buffers(state)
+        nseg = max(1, (self.nsteps + self.ckpt - 1) // self.ckpt)
+        npre = min(2, nseg)
         done = 0
-        while True:
+        for _ in range(nseg - npre):
             self.ckpts.append([t.clone() for t in self._cur(run, done)])
             k = min(self.ckpt, self.nsteps - done)
-            if done + k >= self.nsteps:
-                break
             run.steps(k)
             done += k
         if done > 0:
             info = run.clock.read()
             if info["merr"] != 0 or info["nexec"] != done:
                 raise cans.CansError(f"forward run stopped early: {info}")
+        self.ckpts.append([t.clone() for t in self._cur(run, done)])
         self._last_recorded = False
-        n, _ = self._record_segment(len(self.ckpts) - 1, 0)
-        self._clocks[0].issued = n
-        info = self._clocks[0].read()
-        if info["merr"] != 0 or info["nexec"] != n:
-            raise cans.CansError(f"forward run stopped early (last segment): {info}")
+        self._pre = {}
+        for c in range(nseg - npre, nseg):
+            b = 0 if c == nseg - 1 else 1
+            n, _ = self._record_segment(c, b)
+            self._clocks[b].issued = n
+            info = self._clocks[b].read()
+            if info["merr"] != 0 or info["nexec"] != n:
+                raise cans.CansError(f"forward run stopped early (segment {c}): {info}")
+            self._pre[c] = b
+            if c + 1 < nseg:
+                self.ckpts.append([t.clone() for t in self._seg[b][n]])
         self._last_ev = torch.cuda.Event()
         self._last_ev.record(torch.cuda.current_stream(self.ws.device))
         self._last_recorded = True
@@ -158,9 +168,9 @@ class PeriodicMlwAdjoint:
         second stream) while the reverse sweep of the previous segment runs
         on the caller's stream (cans2d_mlw_adjoint_run); events order a
         buffer's record before its sweep and its sweep before its reuse.
-        The last segment's states are those the forward recorded into
-        buffer 0, the first time the backward follows that forward (a
-        second backward finds buffer 0 reused and recomputes it)."""
+        The last two segments' states are those the forward recorded into
+        the buffers, the first time the backward follows that forward (a
+        second backward finds the buffers reused and recomputes them)."""
         dx, dxm, dy, dym = self.grid
         shape = (self.jx, self.ix)
         lam = [t.clone().contiguous() for t in lam]
@@ -184,7 +194,17 @@ class PeriodicMlwAdjoint:
         lam_p, tmp_p = _ptr_array(lam, shape), _ptr_array(tmp, shape)
         order = list(reversed(range(len(self.ckpts))))
 
+        pre = self._pre if self._last_recorded else {}
+        self._last_recorded = False
+
         def record(c, b):
+            if pre.get(c) == b:
+                # recorded by the forward: the sweep waits on the forward's
+                # event (each buffer keeps its own event object, so that a
+                # later record into the other buffer cannot move this wait)
+                main.wait_event(self._last_ev)
+                ev_rec[b].record(main)
+                return min(self.ckpt, self.nsteps - c * self.ckpt), self._seg_p[b]
             if used[b]:
                 rec.wait_event(ev_sweep[b])
             else:
@@ -194,12 +214,7 @@ class PeriodicMlwAdjoint:
                 ev_rec[b].record(rec)
             return n, states
 
-        if self._last_recorded:   # written by the forward; its event orders the sweep after it
-            pending = (min(self.ckpt, self.nsteps - order[0] * self.ckpt), self._seg_p[0])
-            ev_rec[0] = self._last_ev
-            self._last_recorded = False
-        else:
-            pending = record(order[0], 0)
+        pending = record(order[0], 0)
         for idx, c in enumerate(order):
             b = idx % 2
             n, states = pending
