@@ -76,13 +76,21 @@ def periodic2(oracle, f, margin):
             oracle.bnd_2d(dr, 7, margin, margin, f[k])
 
 
-def test_mlw_m3_2d_resident_loop(oracle, dev):
+@pytest.mark.parametrize("margin,ghost_min", [(1, False), (2, True), (1, True)])
+def test_mlw_m3_2d_resident_loop(oracle, dev, margin, ghost_min):
     """40 steps of {cfl_m3; t=t+dt; mlw_m3; bdper x, y} in the resident loop
-    against the restatement's loop (periodic 2.5D MHD with all components)."""
-    ix, jx, margin, nsteps = 66, 40, 1, 40
+    against the restatement's loop (periodic 2.5D MHD with all components).
+    In the periodic box the kernel advances the clock and takes the next
+    step's cfl_m3 (the first step of each call a cfl pass); with ghost_min
+    the smallest dx and dy lie in the end ghost cells, so the reference's
+    cfl_m3 takes dt from a ghost image.  Issued in two calls."""
+    ix, jx, nsteps = 66, 40, 40
     s = state2(ix, jx, 77)
     _, dx, dxm = stretched_grid(ix, seed=5)
     _, dy, dym = stretched_grid(jx, seed=6)
+    if ghost_min:
+        dx[0], dx[ix - 1] = 0.3 * dx.min(), 0.35 * dx.min()
+        dy[0], dy[jx - 1] = 0.35 * dy.min(), 0.3 * dy.min()
     periodic2(oracle, s, margin)
     a = {k: v.copy(order="F") for k, v in s.items()}
     t = 0.0
@@ -97,7 +105,8 @@ def test_mlw_m3_2d_resident_loop(oracle, dev):
     grid = {k: T(v, dev) for k, v in dict(dx=dx, dxm=dxm, dy=dy, dym=dym).items()}
     run = cans.Run(2, cans.ENGINE_MLW3, st, grid, 5 / 3, 0.4, qav=3.0, margin=margin, bc=7)
     run.reset()
-    run.steps(nsteps)
+    run.steps(nsteps // 2)
+    run.steps(nsteps - nsteps // 2)
     cur, info = run.state()
     torch.cuda.synchronize()
     assert info["nexec"] == nsteps and info["merr"] == 0 and info["t"] == t

@@ -496,9 +496,10 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
     // kernels carry cfl_m and the clock, so a step is the engine and the
     // ghost fill:
     //  * 3D Cartesian MLW (marching kernel), HLLD and FP32-flux Roe (fused z
-    //    marches): the kernels advance the clock and take the next step's
-    //    cfl_m; the first step of a call still takes cfl_m of its state in a
-    //    pass of its own (the same minimum again when the call before fused it);
+    //    marches), 2D mlw_m3: the kernels advance the clock and take the next
+    //    step's cfl_m; the first step of a call still takes cfl_m of its state
+    //    in a pass of its own (the same minimum again when the call before
+    //    fused it);
     //  * FP64 3D Roe and 2D Roe (roecfl): the stage-1 x faces take the step's
     //    own cfl_m and the next kernel advances the clock, after the first
     //    step of a call (whose ghost cells need not be images yet), which
@@ -514,7 +515,8 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
                        (engine == CANS_ENGINE_ROE && (!mixed || roe3d_fused_z(ix, jx, kx))) ||
                        engine == CANS_ENGINE_HLLD);
     const bool roecfl = engine == CANS_ENGINE_ROE && ((per3 && !mixed) || (perio && dim == 2));
-    const bool fused = per3 || roecfl;
+    const bool m3cfl = perio && dim == 2 && m3;
+    const bool fused = per3 || roecfl || m3cfl;
     for (int st = step0; st < step0 + nsteps; st++) {
         double *const *in = (st & 1) ? b : a;
         double *const *out = (st & 1) ? a : b;
@@ -559,6 +561,7 @@ int cans_run(int dim, int engine, int mixed, double *const *a, double *const *b,
             for (int f = 0; f < nf; f++) { g.in[f] = in[f]; g.out[f] = out[f]; }
             g.bx = bx1d; g.bxm = bxm1d; g.dxm = dxm; g.dym = dym; g.mx = m[0]; g.my = m[1];
             g.qav = qav; g.gm = gm; g.clk = c; g.step = st; g.ix = ix; g.jx = jx;
+            if (m3cfl) { g.dx = dx; g.dy = dy; g.safety = safety; g.per_margin = margin; }
             HIPCHK(launch_mlw_m3(g, dim, s));
         } else if (sph && dim == 2) {
             Mlw2sArgs g{};

@@ -165,6 +165,12 @@ struct Mlw3Args {
     Clock *clk;                      // non-null: dt and run flag of step `step` (after its clock kernel)
     int step;
     int ix, jx;
+    // 2D mlw_m3 in a periodic box (per_margin = m > 0): the kernel advances
+    // the clock itself and takes the next step's cfl_m3 over the interior
+    // cells it updates (image_min, cfl2_term; dx, dy the cell widths)
+    const double *dx, *dy;
+    double safety;
+    int per_margin;
 };
 hipError_t launch_mlw_m3(const Mlw3Args &a, int dim, hipStream_t s);
 // mlw_m_c (cans2d/hdmlw/mlw_m_c.f), fields ro pr vx vz bx bz ay on the (x, z)

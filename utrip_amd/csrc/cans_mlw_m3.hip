@@ -27,9 +27,11 @@ namespace {
 constexpr double pi8i = 1.0 / kPi / 8.0;   // mlw_m3.f: pi8i=1./pi/8.
 constexpr double pi4i = 1.0 / kPi / 4.0;
 
+// Step n's run decision and dt: after the step's clock kernel, or (2D
+// periodic box, per_margin > 0) advanced here, block 0 writing the clock.
 CANS_DEV bool m3_step(const Mlw3Args &a, double &dt) {
     if (!a.clk) { dt = a.dt; return true; }
-    const StepCtl c = clock_peek(a.clk, a.step);
+    const StepCtl c = clock_use(a.clk, a.step, a.per_margin == 0, blockIdx.x == 0 && threadIdx.x == 0);
     dt = c.dt;
     return c.run;
 }
@@ -179,9 +181,13 @@ CANS_DEV void cell_values(double ro, double pr, double vx, double vy, double vz,
     ezo = ez;
 }
 
+CANS_DEV double m3_update(const Mlw3Args &a, double dt, double *du, double daz, int lc, int li, int lj, int ci,
+                          int cj, double (*sU)[NC], double (*sF)[NC], double (*sV)[NC], bool do_cfl);
+
 __global__ __launch_bounds__(256) void mlw2d_m3_kernel(Mlw3Args a) {
     double dt;
-    if (!m3_step(a, dt)) return;
+    if (!m3_step(a, dt)) return;   // uniform across the launch
+    __shared__ double red[4];
     __shared__ double sU[8][NC];
     __shared__ double sF[NF][NC];
     __shared__ double sV[2][NC];
@@ -307,8 +313,22 @@ __global__ __launch_bounds__(256) void mlw2d_m3_kernel(Mlw3Args a) {
 #pragma unroll
     for (int r = 0; r < NF; r++) sF[r][t] = cr[r];
     __syncthreads();
-    // (3) full step, Az source, viscosity, update (mlw_m3.f:233-374)
-    if (!own || ci > ix - 1 || cj > jx - 1) return;
+    // (3) full step, Az source, viscosity, update (mlw_m3.f:233-374); in a
+    // periodic box the next step's cfl_m3 (cans2d/common/cfl_m3.f:44-62) of
+    // the interior cells, every thread reaching the block minimum
+    const int mg = a.per_margin;
+    const bool do_cfl = a.clk && mg > 0 && a.safety > 0.0;
+    double cq = kR4_1e20;
+    if (own && ci <= ix - 1 && cj <= jx - 1) cq = m3_update(a, dt, du, daz, lc, li, lj, ci, cj, sU, sF, sV, do_cfl);
+    if (do_cfl) block_min_to_global(cq, red, &a.clk->dtkey[(a.step + 1) % 3]);
+}
+
+// step (3) of mlw2d_m3_kernel for cell (ci, cj); with do_cfl, returns its
+// cfl_m3 term when the cell is interior (else 1.e20)
+CANS_DEV double m3_update(const Mlw3Args &a, double dt, double *du, double daz, int lc, int li, int lj, int ci,
+                          int cj, double (*sU)[NC], double (*sF)[NC], double (*sV)[NC], bool do_cfl) {
+    const int ix = a.ix, jx = a.jx;
+    const double gm = a.gm;
     const int kSW = (lj - 1) * KX + (li - 1), kSE = kSW + 1, kNW = kSW + KX, kNE = kNW + 1;
     const double dxi = a.mx.di[ci - 1], dyi = a.my.di[cj - 1];
     const double ux0 = a.mx.u0[ci - 1], ux1 = a.mx.u1[ci - 1], uy0 = a.my.u0[cj - 1], uy1 = a.my.u1[cj - 1];
@@ -361,8 +381,9 @@ __global__ __launch_bounds__(256) void mlw2d_m3_kernel(Mlw3Args a) {
     const double nvz = nu[4] / nro;
     const double vv = nvx * nvx + nvy * nvy + nvz * nvz;
     const double bb = nu[5] * nu[5] + nu[6] * nu[6] + nu[7] * nu[7];
+    const double npr = (gm - 1.0) * (nu[1] - 0.5 * nro * vv - pi8i * bb);
     a.out[0][p] = nro;
-    a.out[1][p] = (gm - 1.0) * (nu[1] - 0.5 * nro * vv - pi8i * bb);
+    a.out[1][p] = npr;
     a.out[2][p] = nvx;
     a.out[3][p] = nvy;
     a.out[4][p] = nvz;
@@ -370,6 +391,13 @@ __global__ __launch_bounds__(256) void mlw2d_m3_kernel(Mlw3Args a) {
     a.out[6][p] = nu[6];
     a.out[7][p] = nu[7];
     a.out[8][p] = a.in[8][p] + daz;
+    const int mg = a.per_margin;
+    if (do_cfl && ci >= mg + 1 && ci <= ix - mg && cj >= mg + 1 && cj <= jx - mg) {
+        // v2 and b2 of cfl_m3 are vv and bb (the same sums, z components last)
+        const double dg = fminF(image_min(a.dx, ci, ix, mg), image_min(a.dy, cj, jx, mg));
+        return cfl2_term(a.safety, gm, nro, npr, vv, bb, dg);
+    }
+    return kR4_1e20;
 }
 
 }  // namespace
