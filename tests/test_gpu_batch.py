@@ -181,7 +181,8 @@ def test_full_tube_set_fp32_flux_bound(dev):
     """The whole batched workload (4096 seeded tubes, nx 256..4096, beta
     1e-2..1e2) to t = 0.1 in FP64, then with FP32 fluxes for exactly the
     steps each FP64 tube took: the density of every tube within 1e-5
-    relative L1 of FP64.  Run to t = 0.1 on its own
+    relative L1 of FP64 (the bound the mixed-precision forms are held to).
+    Run to t = 0.1 on its own
     clock instead, a tube whose FP64 time lands within rounding of 0.1 can
     take one step more or less in FP32; the tubes that keep the FP64 step
     count meet the same bound."""
