@@ -13,7 +13,9 @@
 // of a stage also sets up the conserved state from the primitives and the
 // last (z) sweep recovers the primitives (the separate hlld_cons/hlld_prim
 // passes remain for runs without an x or z sweep).  The state accumulates
-// in FP64.
+// in FP64.  In a periodic run the stage-1 x sweep also advances the device
+// clock and the stage-2 z sweep takes the next step's cfl_m (image_min,
+// cfl3_term), so a step needs no cfl pass or clock kernel of its own.
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_roe.hpp"

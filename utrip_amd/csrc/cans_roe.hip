@@ -15,6 +15,10 @@
 // C3 (1D/2D): roe_m3 (cans{1,2}d/hdroe/roe_m3.f), the same steps with the
 // z components and the roeflux_m3 flux; its 2D y sweep uses the frame
 // (y, x, z) (roe_m3.f:124-125), not the cyclic (y, z, x) of 3D.
+// In a periodic run the step carries cfl_m and the clock in its own
+// kernels (RoeArgs::per_margin): the FP32-flux 3D step takes the next step's
+// term in its stage-2 z march, the FP64 3D and the 2D steps take the
+// step's own term in their stage-1 x faces.
 #include "cans_common.hpp"
 #include "cans_clock.hpp"
 #include "cans_kernels.hpp"
