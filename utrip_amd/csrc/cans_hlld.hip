@@ -345,6 +345,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
                                                                                                            int to_prim) {
     constexpr int dir = DIR;
     __shared__ double red[4];
+    // FP64 fluxes: the conserved state of the next cell goes one face ahead
+    // into LDS by global->LDS loads (no registers held while the face is
+    // computed); per wave 8 fields x (low, high word) x 64 lanes
+    constexpr bool kLdsU = sizeof(T) == 8;
+    __shared__ unsigned ubuf[kLdsU ? 4 : 1][16][64];
     double dt;
     if (!hstep(a, dt)) return;
     const double dts = stage == 1 ? 0.5 * dt : dt;
@@ -399,6 +404,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 #pragma unroll
             for (int q = 0; q < 8; q++) un[q] = u[q][base + s * (size_t)(cb - 1)];
         }
+        // each lane writes its own words of the wave's slots and reads only
+        // them back; a read waits for every load of the wave in flight (the
+        // compiler does not order LDS reads after global->LDS loads), by then
+        // the window's next cell and the previous cell's stores have had the
+        // whole face to complete
+        const int wv = kLdsU ? (int)(threadIdx.x >> 6) : 0, lane = (int)(threadIdx.x & 63);
+        auto pre_u = [&](size_t pn) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const char *g = (const char *)(u[q] + pn);
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)g,
+                                                 (__attribute__((address_space(3))) void *)&ubuf[wv][2 * q][0], 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(g + 4),
+                                                 (__attribute__((address_space(3))) void *)&ubuf[wv][2 * q + 1][0], 4, 0, 0);
+            }
+        };
+        auto get_u = [&](double *U0) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                U0[q] = __builtin_bit_cast(double, ((unsigned long long)ubuf[wv][2 * q + 1][lane] << 32) |
+                                                       (unsigned long long)ubuf[wv][2 * q][lane]);
+        };
+        if (kLdsU) pre_u(base + s * (size_t)(cb - 1));
         for (int c = cb; c <= ce; c++) {
 #pragma unroll
             for (int q = 0; q < 8; q++) { w0[q] = w1[q]; w1[q] = w2[q]; w2[q] = w3[q]; w3[q] = wn[q]; }
@@ -431,19 +460,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
                 continue;
             }
             hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
+            double U0[8];
+            get_u(U0);
+            const double dl = dd[c - 1];
             if (to_prim) {
                 // the last sweep of the stage also recovers the primitives
                 // (hlld_prim fused): half (stage 1) or out (stage 2)
                 double U[8];
-                const double dl = dd[c - 1];
 #pragma unroll
-                for (int q = 0; q < 8; q++) U[q] = u[q][p] - dts * ((Fh[q] - Fl[q]) / dl);
+                for (int q = 0; q < 8; q++) U[q] = U0[q] - dts * ((Fh[q] - Fl[q]) / dl);   // adddfdx.f
+                if (c < ce) pre_u(p + s);
                 const bool cc = cij && c >= mg + 1 && c <= nd - mg;
                 const double dq = hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p, cc,
                                                cc ? fminF(dxyg, image_min(dd, c, nd, mg)) : 0.0);
                 if (dq < dtmin_local) dtmin_local = dq;   // NaN never wins
             } else {
-                hlld_apply(u, p, dts, dd[c - 1], Fh, Fl);
+#pragma unroll
+                for (int q = 0; q < 8; q++) u[q][p] = U0[q] - dts * ((Fh[q] - Fl[q]) / dl);   // adddfdx.f
+                if (c < ce) pre_u(p + s);
             }
 #pragma unroll
             for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
