@@ -459,10 +459,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
                 for (int q = 0; q < 8; q++) Fl[q] = Fh[q];
                 continue;
             }
+            // the spacings are loaded before the face is computed: the wait
+            // of get_u would otherwise expose their latency
+            const double dl = dd[c - 1];
+            const bool cc = to_prim && cij && c >= mg + 1 && c <= nd - mg;
+            const double dg = cc ? fminF(dxyg, image_min(dd, c, nd, mg)) : 0.0;
             hlld_face_w<T>(a, dir, stage == 2 && c >= 2 && c <= nd - 2, w0, w1, w2, w3, Fh);
             double U0[8];
             get_u(U0);
-            const double dl = dd[c - 1];
             if (to_prim) {
                 // the last sweep of the stage also recovers the primitives
                 // (hlld_prim fused): half (stage 1) or out (stage 2)
@@ -470,9 +474,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 #pragma unroll
                 for (int q = 0; q < 8; q++) U[q] = U0[q] - dts * ((Fh[q] - Fl[q]) / dl);   // adddfdx.f
                 if (c < ce) pre_u(p + s);
-                const bool cc = cij && c >= mg + 1 && c <= nd - mg;
-                const double dq = hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p, cc,
-                                               cc ? fminF(dxyg, image_min(dd, c, nd, mg)) : 0.0);
+                const double dq = hlld_prim_of(a, U, stage == 1 ? a.half : a.out, p, cc, dg);
                 if (dq < dtmin_local) dtmin_local = dq;   // NaN never wins
             } else {
 #pragma unroll
