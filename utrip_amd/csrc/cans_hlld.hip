@@ -9,7 +9,10 @@
 // cell from the neighbouring lane (DPP); along y and z each thread marches
 // over a segment of cells with the four stencil cells of its face in a
 // sliding register window and carries the previous face (one face per
-// segment is computed twice, by the same operations).  The first (x) sweep
+// segment is computed twice, by the same operations); the next cell's
+// window values and conserved state are loaded one face ahead (with FP64
+// fluxes the conserved state goes through LDS by global->LDS loads, as
+// the registers are full).  The first (x) sweep
 // of a stage also sets up the conserved state from the primitives and the
 // last (z) sweep recovers the primitives (the separate hlld_cons/hlld_prim
 // passes remain for runs without an x or z sweep).  The state accumulates
